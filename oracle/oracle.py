@@ -1,0 +1,320 @@
+"""CPU oracle drivers (TEST INFRASTRUCTURE ONLY).
+
+ctypes bindings to oracle/liboracle.so (the C restatement of the integration
+kernels, om_oracle.c) plus the drivers that restate, in the reference's
+order, HeadMat (assembleHeadMat.cpp:74-135), DipSourceMat
+(assembleSourceMat.cpp:66-104), Head2EEGMat (assembleSensors.cpp:25-39,
+danielsson.cpp:107-129) and the gains (gain.h:52-89).  Dense linear algebra
+(the reference's LAPACK DSPTRF/DSPTRI/DSPTRS, symmatrix.h:259-297,
+symmatrix.cpp:133-152) is restated with numpy/scipy LAPACK (Bunch-Kaufman
+dsytrf/dsytrs, the unpacked storage form of the same factorisation).
+
+Only tests/, the smoke() check and bench.py's cpu_baseline leg may import
+this module; the product path never does.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+from .geometry import Geometry, K, read_tri
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+i64 = C.c_int64
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int64)
+
+
+class _Geom(C.Structure):
+    _fields_ = [("nv", i64), ("verts", dp), ("nt", i64), ("tris", ip),
+                ("tri_index", ip), ("vert_index", ip), ("nm", i64),
+                ("tri_off", ip), ("mv_off", ip), ("mv_ids", ip),
+                ("adj_off", ip), ("adj_tri", ip), ("barrier", ip)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_mesh_pair.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, dp]
+        L.orc_mesh_pair.restype = C.c_int
+        L.orc_deflate_mesh.argtypes = [C.POINTER(_Geom), i64, C.c_double, dp]
+        L.orc_packed_get.argtypes = [dp, i64, i64]
+        L.orc_packed_get.restype = C.c_double
+        L.orc_dsm_mesh.argtypes = [C.POINTER(_Geom), i64, dp, C.c_int, C.c_uint, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, dp]
+        L.orc_mesh_solid_angle.argtypes = [C.POINTER(_Geom), i64, dp]
+        L.orc_mesh_solid_angle.restype = C.c_double
+        L.orc_dist_point_mesh.argtypes = [C.POINTER(_Geom), i64, dp, ip, dp]
+        L.orc_dist_point_mesh.restype = C.c_double
+        L.orc_operator_s.argtypes = [dp, dp]
+        L.orc_operator_s.restype = C.c_double
+        L.orc_p1slp.argtypes = [dp, dp, dp]
+        L.orc_solid_angle.argtypes = [dp, dp]
+        L.orc_solid_angle.restype = C.c_double
+        L.orc_integrate_s_pair.argtypes = [dp, dp, C.c_int]
+        L.orc_integrate_s_pair.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+class _Tables:
+    """Flattened geometry handed to the C kernels."""
+
+    def __init__(self, g):
+        self.g = g
+        self.verts = np.ascontiguousarray(np.array(g.vertices, dtype=np.float64).reshape(-1, 3))
+        self.tris = np.ascontiguousarray(np.concatenate([m.tris for m in g.meshes]).astype(np.int64))
+        self.tri_off = np.cumsum([0] + [len(m.tris) for m in g.meshes]).astype(np.int64)
+        self.tri_index = np.concatenate([m.tri_index for m in g.meshes]).astype(np.int64)
+        self.vert_index = np.ascontiguousarray(g.vert_index, dtype=np.int64)
+        self.mv_off = np.cumsum([0] + [len(m.verts) for m in g.meshes]).astype(np.int64)
+        self.mv_ids = np.concatenate([np.asarray(m.verts, dtype=np.int64) for m in g.meshes])
+        adj_off = [0]
+        adj = []
+        for mi, m in enumerate(g.meshes):
+            t0 = self.tri_off[mi]
+            lists = {}
+            for k, t in enumerate(m.tris):          # mesh.h:252-257
+                for v in t:
+                    lists.setdefault(int(v), []).append(int(t0 + k))
+            for v in m.verts:
+                l = lists.get(int(v), [])
+                adj.extend(l)
+                adj_off.append(adj_off[-1] + len(l))
+        self.adj_off = np.asarray(adj_off, dtype=np.int64)
+        self.adj_tri = np.asarray(adj, dtype=np.int64)
+        self.barrier = np.asarray([int(m.current_barrier) for m in g.meshes], dtype=np.int64)
+        self.s = _Geom(len(self.verts), _p(self.verts), len(self.tris), _p(self.tris, ip),
+                       _p(self.tri_index, ip), _p(self.vert_index, ip), len(g.meshes),
+                       _p(self.tri_off, ip), _p(self.mv_off, ip), _p(self.mv_ids, ip),
+                       _p(self.adj_off, ip), _p(self.adj_tri, ip), _p(self.barrier, ip))
+
+
+class OracleLib:
+    """Adapter giving oracle/geometry.py access to the C solid angle."""
+
+    def mesh_solid_angle(self, g, mid, p):
+        t = tables(g)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        return lib().orc_mesh_solid_angle(C.byref(t.s), mid, _p(p))
+
+
+def tables(g):
+    t = getattr(g, "_tables", None)
+    if t is None:
+        t = _Tables(g)
+        g._tables = t
+    return t
+
+
+def _reset_tables(g):
+    g._tables = None
+
+
+def load_geometry(geom_file, cond_file):
+    """Geometry(geom, cond) (geometry.h:67-69)."""
+    g = Geometry.from_files(geom_file, cond_file, _LazyLib())
+    _reset_tables(g)
+    return g
+
+
+class _LazyLib(OracleLib):
+    def mesh_solid_angle(self, g, mid, p):
+        # the geometry may still be under construction: build tables for the
+        # mesh triangles only (indices are not needed for solid angles)
+        m = g.meshes[mid]
+        verts = np.array(g.vertices, dtype=np.float64)
+        P = verts[m.tris]                       # (T,3,3)
+        return _solid_angle_sum(P, np.asarray(p, dtype=np.float64))
+
+
+def _solid_angle_sum(P, p):
+    L = lib()
+    s = 0.0
+    for tri in P:
+        t9 = np.ascontiguousarray(tri.reshape(9))
+        s += L.orc_solid_angle(_p(t9), _p(np.ascontiguousarray(p)))
+    return s
+
+
+def nested_geometry(meshes, conductivity):
+    """make_nested_geometry (_make_geometry.py:131-211)."""
+    g = Geometry.nested(meshes, conductivity, _LazyLib())
+    _reset_tables(g)
+    return g
+
+
+# ---------------------------------------------------------------------------
+# HeadMat
+# ---------------------------------------------------------------------------
+def head_mat(g, order=3):
+    """Details::HeadMatrix + deflate (assembleHeadMat.cpp:28-60,74-115).
+    Returns the packed upper SymMatrix data (length n(n+1)/2)."""
+    t = tables(g)
+    n = g.size
+    H = np.zeros(n * (n + 1) // 2, dtype=np.float64)
+    L = lib()
+    for (m1, m2, o, cS, cN, cD) in g.pairs:
+        rc = L.orc_mesh_pair(C.byref(t.s), m1, m2, order, cS, cN, cD, _p(H))
+        if rc != 0:
+            raise MemoryError("oracle mesh pair")
+    for part in g.parts:
+        nbv = 0
+        i_first = 0
+        for mid in part:
+            m = g.meshes[mid]
+            if m.outermost:
+                nbv += len(m.verts)
+                if i_first == 0:
+                    i_first = int(g.vert_index[m.verts[0]])
+        coef = L.orc_packed_get(_p(H), i_first, i_first) / nbv
+        for mid in part:
+            if g.meshes[mid].outermost:
+                L.orc_deflate_mesh(C.byref(t.s), mid, coef, _p(H))
+    return H
+
+
+def unpack(Hp, n):
+    """Packed upper (symmatrix.h:52-62) -> dense symmetric."""
+    iu = np.triu_indices(n)
+    # packed index i + j(j+1)/2 for i<=j: order columns j, rows i<=j
+    j = np.repeat(np.arange(n), np.arange(1, n + 1))
+    i = np.concatenate([np.arange(k + 1) for k in range(n)])
+    M = np.zeros((n, n))
+    M[i, j] = Hp
+    M[j, i] = Hp
+    return M
+
+
+def pack(M):
+    n = M.shape[0]
+    j = np.repeat(np.arange(n), np.arange(1, n + 1))
+    i = np.concatenate([np.arange(k + 1) for k in range(n)])
+    return M[i, j].copy()
+
+
+# ---------------------------------------------------------------------------
+# DipSourceMat
+# ---------------------------------------------------------------------------
+def dip_source_mat(g, dipoles, domain_name="", order=3, levels=10, tol=0.001):
+    """SourceMatrix<Dipole> (assembleSourceMat.cpp:66-104) with the default
+    adaptive Integrator(3,10,0.001) of assembleSourceMat.h:27-29."""
+    t = tables(g)
+    L = lib()
+    dipoles = np.asarray(dipoles, dtype=np.float64)
+    if dipoles.ndim != 2 or dipoles.shape[1] != 6:
+        raise ValueError("dipoles must have 6 columns")
+    n = g.size
+    out = np.zeros((n, len(dipoles)), dtype=np.float64, order="F")
+    ol = OracleLib()
+    for s, d6 in enumerate(dipoles):
+        dk = g.domain_index(domain_name) if domain_name else g.domain_of_point(d6[:3], ol)
+        dom = g.domains[dk]
+        if dom.cond == 0.0:
+            continue
+        col = np.zeros(n)
+        d6c = np.ascontiguousarray(d6)
+        for iface, inside in dom.boundaries:
+            factorD = -K if inside else K
+            for mid, o in iface.omeshes:
+                coeffD = factorD * o
+                m = g.meshes[mid]
+                L.orc_dsm_mesh(C.byref(t.s), mid, _p(d6c), order, levels, tol, coeffD,
+                               coeffD / dom.cond, int(not m.current_barrier), _p(col))
+        out[:, s] = col
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Head2EEGMat
+# ---------------------------------------------------------------------------
+def head2eeg(g, electrodes):
+    """Head2EEGMat (assembleSensors.cpp:25-39) with dist_point_geom
+    (danielsson.cpp:107-129).  Returns (cols (n_e,3), vals (n_e,3)): the
+    three vertex unknowns of the closest triangle and their weights."""
+    t = tables(g)
+    L = lib()
+    E = np.asarray(electrodes, dtype=np.float64)
+    cols = np.zeros((len(E), 3), dtype=np.int64)
+    vals = np.zeros((len(E), 3))
+    for e, p in enumerate(E):
+        pc = np.ascontiguousarray(p)
+        distmin = np.finfo(np.float64).max
+        best_tri = -1
+        alphas = np.zeros(3)
+        for dom in g.domains:
+            if dom.cond != 0.0:
+                continue
+            for iface, _ in dom.boundaries:
+                # dist_point_interface: first strict minimum over its meshes
+                dmin_i = np.finfo(np.float64).max
+                tri_i = -1
+                al_i = np.zeros(3)
+                for mid, _ in iface.omeshes:
+                    tri = C.c_int64(-1)
+                    al = np.zeros(3)
+                    dd = L.orc_dist_point_mesh(C.byref(t.s), mid, _p(pc), C.byref(tri), _p(al))
+                    if dd < dmin_i:
+                        dmin_i, tri_i, al_i = dd, tri.value, al
+                # the reference overwrites alphas with every interface's result
+                alphas = al_i
+                if dmin_i < distmin:
+                    distmin, best_tri = dmin_i, tri_i
+        vids = t.tris[best_tri]
+        cols[e] = g.vert_index[vids]
+        vals[e] = alphas
+    return cols, vals
+
+
+def h2em_dense(g, cols, vals):
+    E = np.zeros((len(cols), g.size))
+    for e in range(len(cols)):
+        for k in range(3):
+            E[e, cols[e, k]] = vals[e, k]
+    return E
+
+
+# ---------------------------------------------------------------------------
+# Gains
+# ---------------------------------------------------------------------------
+def gain_eeg(H, dsm, E):
+    """GainEEG (gain.h:69-76): (E * H^-1) * D, with H^-1 applied through a
+    symmetric-indefinite LAPACK solve."""
+    import scipy.linalg as sla
+    Y = sla.solve(H, E.T, assume_a="sym")
+    return Y.T @ dsm
+
+
+def read_dipoles(path):
+    return np.loadtxt(path, ndmin=2)
+
+
+def read_sensors(path):
+    """Sensors::load (sensors.cpp:36-143): optional label column then x y z."""
+    rows = []
+    with open(path) as f:
+        for ln in f:
+            tok = ln.split()
+            if not tok or tok[0].startswith("#"):
+                continue
+            rows.append(tok)
+    labeled = rows[0][0].count(".") != 1
+    data = [r[1:] if labeled else r for r in rows]
+    return np.array(data, dtype=np.float64)[:, :3]
