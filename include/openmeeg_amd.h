@@ -1,0 +1,174 @@
+/*
+ * openmeeg_amd: MI355X-native symmetric-BEM EEG forward path, C ABI.
+ *
+ * Every entry point returns 0 on success and a non-zero status otherwise;
+ * omh_last_error() then gives the message (per thread).  No C++ exception
+ * crosses this boundary.  Device pointers are HIP device memory; `stream` is
+ * a hipStream_t (NULL = the null stream).  Dense matrices are column-major.
+ *
+ * Unknown orderings of device matrices (`ordering`):
+ *   OMH_ORDER_REFERENCE       vertices first, then triangles (the reference's
+ *                             numbering, OpenMEEG/src/geometry.cpp:256-295);
+ *   OMH_ORDER_TRIANGLES_FIRST the same numbering rotated so that triangle
+ *                             unknowns come first; the HeadMat factors without
+ *                             pivoting in this order.
+ *
+ * Each declaration cites the reference interface it replaces (file:line in
+ * OpenMEEG 2.5.16).
+ */
+#ifndef OPENMEEG_AMD_H
+#define OPENMEEG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMH_ORDER_REFERENCE 0
+#define OMH_ORDER_TRIANGLES_FIRST 1
+
+typedef struct omh_geometry omh_geometry;
+
+/* Message of the last failed call on this thread (never NULL). */
+const char* omh_last_error(void);
+
+/* ---------------------------------------------------------------- geometry */
+
+/* Geometry(geomFileName, condFileName): OpenMEEG/include/geometry.h:67-69,
+ * GeometryIOs/GeomFile.h:158-300 (format 1.1, .tri meshes). */
+int omh_geometry_load(const char* geom_file, const char* cond_file, omh_geometry** out);
+
+/* make_nested_geometry(meshes, conductivity):
+ * wrapping/python/openmeeg/_make_geometry.py:131-211.  Meshes inner to
+ * outer; vertices[l] is n_vertices[l] x 3 row-major, triangles[l] is
+ * n_triangles[l] x 3 (0-based, mesh-local).  1 or 3 layers name the domains
+ * as the reference (Brain/Skull/Scalp/Air); other counts use Brain,
+ * Layer1..Layer{L-1}, Air. */
+int omh_geometry_nested(int n_layers, const int64_t* n_vertices, const double* const* vertices,
+                        const int64_t* n_triangles, const int64_t* const* triangles,
+                        const double* conductivities, omh_geometry** out);
+
+/* make_geometry(meshes, interfaces, domains):
+ * wrapping/python/openmeeg/_make_geometry.py:15-128.  Interface i is the
+ * list interface_meshes[interface_offsets[i] .. interface_offsets[i+1]) with
+ * orientations +1/-1; domain d is bounded by domain_interfaces[...] with
+ * domain_inside[...] = 1 for SimpleDomain::Inside, 0 for Outside. */
+int omh_geometry_create(int n_meshes, const char* const* mesh_names, const int64_t* n_vertices,
+                        const double* const* vertices, const int64_t* n_triangles,
+                        const int64_t* const* triangles, int n_interfaces, const char* const* interface_names,
+                        const int64_t* interface_offsets, const int32_t* interface_meshes,
+                        const int32_t* interface_orientations, int n_domains, const char* const* domain_names,
+                        const double* domain_conductivities, const int64_t* domain_offsets,
+                        const int32_t* domain_interfaces, const int32_t* domain_inside, omh_geometry** out);
+
+void omh_geometry_free(omh_geometry* g);
+
+/* out[0] size = nb_parameters - nb_current_barrier_triangles (HeadMat order,
+ * assembleHeadMat.cpp:77), out[1] nb_parameters (geometry.h:160),
+ * out[2] nb_current_barrier_triangles (geometry.h:253), out[3] vertex
+ * unknowns, out[4] non-barrier triangle unknowns, out[5] meshes,
+ * out[6] domains, out[7] is_nested (geometry.h:91). */
+int omh_geometry_sizes(const omh_geometry* g, int64_t out[8]);
+
+/* Geometry::domain(name) index (geometry.cpp:156-165); domain name by index. */
+int omh_geometry_domain_index(const omh_geometry* g, const char* name, int32_t* index);
+int omh_geometry_domain_name(const omh_geometry* g, int32_t index, char* buf, int64_t buflen);
+
+/* ---------------------------------------------------------------- assembly */
+
+/* HeadMat(geo, Integrator(order,0,.)): OpenMEEG/include/assemble.h:32,
+ * src/assembleHeadMat.cpp:28-135.  Writes the full symmetric n x n FP64
+ * matrix (both halves) into d_H (leading dimension ld >= n) in `ordering`.
+ * Only non-adaptive integration (levels = 0, the reference default) runs
+ * here; order 0..3 selects the 3/6/7/16-point rule. */
+int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int ordering, void* stream);
+
+/* Geometry::domain(point) for n device points (x y z every `stride`
+ * doubles): geometry.cpp:146-154.  Writes domain indices to h_domains. */
+int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,
+                       void* stream);
+
+/* SourceMatrix<Dipole>(geo, dipoles, Integrator(order,levels,tol), domain):
+ * assembleSourceMat.h:24-29, assembleSourceMat.cpp:66-104; the Python
+ * DipSourceMat of wrapping/python/openmeeg/_openmeeg.i:399-413.  d_dipoles
+ * is n_dip x 6 row-major (x y z qx qy qz).  h_domains gives the domain of
+ * every dipole, or NULL to locate each dipole (domain name ""). */
+int omh_dip_source_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains,
+                       int order, int levels, double tol, double* d_out, int64_t ld, int ordering, void* stream);
+
+/* Head2EEGMat(geo, electrodes): assemble.h:35, assembleSensors.cpp:25-39,
+ * danielsson.cpp:26-129.  d_electrodes n_el x 3 row-major.  Row e of the
+ * sparse matrix has the three entries (h_cols[3e+k], h_vals[3e+k]); columns
+ * are reference-order vertex unknowns. */
+int omh_head2eeg_mat(omh_geometry* g, const double* d_electrodes, int64_t n_el, int64_t* h_cols, double* h_vals,
+                     void* stream);
+
+/* ------------------------------------------------------- dense linear algebra */
+
+/* Workspace bytes for omh_sym_factor_* / omh_sym_solve_* (element size 8 or 4). */
+int64_t omh_factor_workspace_bytes(int64_t n, int32_t elem_size);
+
+/* Factorisation behind SymMatrix::invert/inverse/solveLin
+ * (OpenMEEGMaths/include/symmatrix.h:259-297, src/symmatrix.cpp:133-152;
+ * DSPTRF there): pivot-free signed Cholesky H = L S L^T of the lower triangle
+ * of d_A in place (matrix in OMH_ORDER_TRIANGLES_FIRST).  *info = 0, or
+ * 1 + index of the first vanishing pivot (then the factor is unusable). */
+int omh_sym_factor_f64(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
+int omh_sym_factor_f32(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
+
+/* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
+ * symmatrix.cpp:133-152; linsolve gain.h:52-57). */
+int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
+                      int64_t nrhs, void* stream);
+int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
+                      int64_t nrhs, void* stream);
+
+/* Mixed-precision solve X = H^-1 B: FP32 factor (d_L32, d_work32) with FP64
+ * residuals on the FP64 matrix d_H and iterative refinement until
+ * ||dX||/||X|| <= tol or max_iter corrections.  No reference counterpart:
+ * it stands beside DSPTRS for the mixed-precision path. */
+int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                    const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs,
+                    int32_t max_iter, double tol, int32_t* iterations, double* rel_update, void* stream);
+
+/* C = alpha op(A) op(B) + beta C: the DGEMM products of
+ * OpenMEEGMaths/include/matrix.h:333-350 and gain.h:69-89. */
+int omh_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, double alpha, const double* d_A,
+                 int64_t lda, const double* d_B, int64_t ldb, double beta, double* d_C, int64_t ldc, void* stream);
+int omh_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, float alpha, const float* d_A,
+                 int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C, int64_t ldc, void* stream);
+
+/* FP64 -> FP32 copy (lower_only: only entries with row >= col). */
+int omh_convert_f64_f32(const double* d_src, int64_t lds, float* d_dst, int64_t ldd, int64_t m, int64_t n,
+                        int32_t lower_only, void* stream);
+
+/* SparseMatrix * SymMatrix of sparse_matrix.cpp:36-51 for a Head2EEGMat
+ * (3 non-zeros per row): out(e,:) = sum_k vals(e,k) Hm(cols(e,k),:).
+ * d_cols/d_vals are device copies of omh_head2eeg_mat's output with the
+ * columns already mapped to the row order of d_Hm. */
+int omh_sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* d_Hm, int64_t ldh,
+                    int64_t n, double* d_out, int64_t ldo, void* stream);
+
+/* Unknown-order conversions between the reference's storage and device
+ * matrices.  omh_sym_unpack: packed upper SymMatrix data (symmatrix.h:52-62,
+ * reference order, device pointer) -> full symmetric matrix in `ordering`.
+ * omh_sym_pack: the inverse.  omh_permute_rows: rows of an n x ncol matrix
+ * from reference order to `ordering` (to_reference = 0) or back (1). */
+int omh_sym_unpack(const double* d_packed, int64_t n, int64_t nv, double* d_H, int64_t ld, int ordering, void* stream);
+int omh_sym_pack(const double* d_H, int64_t n, int64_t nv, int64_t ld, int ordering, double* d_packed, void* stream);
+int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ldd, int64_t n, int64_t nv,
+                     int64_t ncol, int ordering, int32_t to_reference, void* stream);
+
+/* Frees the library's cached device scratch buffers. */
+int omh_release_scratch(void);
+
+/* Writes the 16x16 lane layout of the FP64 and FP32 MFMA blocks used by the
+ * GEMMs into d_out (2 x 256 doubles) for the layout self-test. */
+int omh_probe_mfma_layout(double* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENMEEG_AMD_H */

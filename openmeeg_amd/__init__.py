@@ -1,0 +1,31 @@
+"""openmeeg_amd: MI355X-native symmetric-BEM EEG forward path.
+
+Drop-in names of the reference's Python module ``openmeeg``
+(wrapping/python/openmeeg/__init__.py:15-39) for the EEG forward path:
+HeadMat, DipSourceMat, Head2EEGMat, GainEEG, GainEEGadjoint, Integrator,
+Matrix, SymMatrix, Sensors, make_geometry, make_nested_geometry,
+read_geometry.  All computation runs in the HIP library
+(libopenmeeg_amd.so) through its C ABI (include/openmeeg_amd.h).
+"""
+from .api import (  # noqa: F401
+    DipSourceMat,
+    Factor,
+    GainEEG,
+    GainEEGadjoint,
+    Geometry,
+    Head2EEGMat,
+    HeadMat,
+    Integrator,
+    Matrix,
+    Sensors,
+    SparseMatrix,
+    SymMatrix,
+    factor,
+    gain_eeg_pipeline,
+    gain_to_numpy,
+    make_geometry,
+    make_nested_geometry,
+    read_geometry,
+)
+
+__version__ = "0.1.0"

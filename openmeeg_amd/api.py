@@ -1,0 +1,676 @@
+"""Python surface of openmeeg_amd, mirroring the reference's ``openmeeg``
+module for the symmetric-BEM EEG forward path
+(wrapping/python/openmeeg/__init__.py:15-39, _openmeeg.i, _make_geometry.py).
+
+Host code only: every computation runs in the HIP library through the C ABI
+(_lib.py); PyTorch provides device memory and the current stream.  There is
+no CPU fallback -- without the library or a GPU the calls raise.
+"""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ORDER_REFERENCE, ORDER_TRIANGLES_FIRST
+
+_torch = None
+
+
+def _t():
+    global _torch
+    if _torch is None:
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("openmeeg_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+        _torch = torch
+    return _torch
+
+
+def _stream():
+    return C.c_void_p(_t().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _dev_f64(n):
+    return _t().empty(int(n), dtype=_t().float64, device="cuda")
+
+
+def _to_dev(a, dtype=None):
+    torch = _t()
+    a = np.ascontiguousarray(a, dtype=np.float64 if dtype is None else dtype)
+    return torch.from_numpy(a).to("cuda")
+
+
+# ---------------------------------------------------------------------------
+# Integrator (integrator.h:19-37)
+# ---------------------------------------------------------------------------
+class Integrator:
+    """Integrator(order), Integrator(order, tol) (10 levels) or
+    Integrator(order, levels, tol=0.0001); order is clamped to 1..3 as
+    safe_order (integrator.h:24-29)."""
+
+    def __init__(self, order, levels=None, tol=None):
+        if levels is not None and tol is None and isinstance(levels, float):
+            levels, tol = 10, levels
+        if levels is None:
+            levels, tol = 0, 0.0
+        if tol is None:
+            tol = 0.0001
+        order = int(order)
+        self.order = order if 0 < order < 4 else (1 if order < 1 else 3)
+        self.levels = int(levels)
+        self.tolerance = float(tol)
+
+    def __repr__(self):
+        return "Integrator(%d, %d, %g)" % (self.order, self.levels, self.tolerance)
+
+
+# ---------------------------------------------------------------------------
+# Geometry
+# ---------------------------------------------------------------------------
+class Geometry:
+    """Geometry(geom_file, cond_file) (geometry.h:67-69), or the result of
+    make_nested_geometry / make_geometry."""
+
+    def __init__(self, geom_file=None, cond_file=None):
+        self._h = C.c_void_p()
+        if geom_file is not None:
+            if cond_file is None:
+                raise ValueError("a conductivity file is needed")
+            check(lib().omh_geometry_load(os.fsencode(geom_file), os.fsencode(cond_file), C.byref(self._h)))
+            self._load_sizes()
+
+    def _load_sizes(self):
+        s = (C.c_int64 * 8)()
+        check(lib().omh_geometry_sizes(self._h, s))
+        (self._size, self._nbp, self._nbct, self._nv, self._nt, self._nm, self._nd, nested) = list(s)
+        self._nested = bool(nested)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib._lib is not None:
+            _lib._lib.omh_geometry_free(h)
+            self._h = None
+
+    # reference accessors (geometry.h:91,160,253)
+    def nb_parameters(self):
+        return self._nbp
+
+    def nb_current_barrier_triangles(self):
+        return self._nbct
+
+    def is_nested(self):
+        return self._nested
+
+    @property
+    def size(self):
+        """HeadMat dimension nb_parameters - nb_current_barrier_triangles."""
+        return self._size
+
+    @property
+    def n_vertex_unknowns(self):
+        return self._nv
+
+    def domain_index(self, name):
+        idx = C.c_int32()
+        check(lib().omh_geometry_domain_index(self._h, name.encode(), C.byref(idx)))
+        return idx.value
+
+    def domain_names(self):
+        out = []
+        buf = C.create_string_buffer(256)
+        for i in range(self._nd):
+            check(lib().omh_geometry_domain_name(self._h, i, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def tri_first_index(self, ref_rows):
+        """Map reference-order unknown indices to triangles-first order."""
+        r = np.asarray(ref_rows, dtype=np.int64)
+        return np.where(r >= self._nv, r - self._nv, r + self._nt)
+
+
+def _mesh_arrays(meshes):
+    pts, tris = [], []
+    for m in meshes:
+        if isinstance(m, dict):
+            v, t = m["vertices"], m["triangles"]
+        else:
+            v, t = m
+        pts.append(np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3))
+        tris.append(np.ascontiguousarray(t, dtype=np.int64).reshape(-1, 3))
+    return pts, tris
+
+
+def make_nested_geometry(meshes, conductivity):
+    """_make_geometry.py:131-211: meshes (vertices, triangles) inner to outer."""
+    if not isinstance(meshes, (list, tuple)) or len(meshes) < 1:
+        raise ValueError("Wrong argument (should be a non-empty list of meshes)")
+    if len(conductivity) != len(meshes):
+        raise ValueError("one conductivity per layer is needed")
+    pts, tris = _mesh_arrays(meshes)
+    L = len(pts)
+    nv = (C.c_int64 * L)(*[len(p) for p in pts])
+    nt = (C.c_int64 * L)(*[len(t) for t in tris])
+    pp = (_lib.p_f64 * L)(*[p.ctypes.data_as(_lib.p_f64) for p in pts])
+    tp = (_lib.p_i64 * L)(*[t.ctypes.data_as(_lib.p_i64) for t in tris])
+    cond = (C.c_double * L)(*[float(c) for c in conductivity])
+    g = Geometry()
+    check(lib().omh_geometry_nested(L, nv, pp, nt, tp, cond, C.byref(g._h)))
+    g._load_sizes()
+    return g
+
+
+def make_geometry(meshes, interfaces, domains):
+    """_make_geometry.py:15-128.  meshes: {name: (vertices, triangles)};
+    interfaces: {name: [(mesh name, +1/-1)]}; domains: {name:
+    ([(interface name, 'Inside'|'Outside')], conductivity)}."""
+    if not isinstance(meshes, dict) or not meshes:
+        raise ValueError("Wrong argument (should be a non empty dictionary of named meshes)")
+    if not isinstance(interfaces, dict) or not interfaces:
+        raise ValueError("Wrong argument (should be a non empty dictionary of named interfaces)")
+    if not isinstance(domains, dict) or not domains:
+        raise ValueError("Wrong argument (should be a non empty dictionary of named domains)")
+    mnames = list(meshes)
+    pts, tris = _mesh_arrays([meshes[k] for k in mnames])
+    inames = list(interfaces)
+    ioff, imesh, iori = [0], [], []
+    for k in inames:
+        for mname, o in interfaces[k]:
+            if mname not in mnames:
+                raise ValueError("interface %s uses unknown mesh %s" % (k, mname))
+            imesh.append(mnames.index(mname))
+            iori.append(int(o))
+        ioff.append(len(imesh))
+    dnames = list(domains)
+    doff, dint, dins, dcond = [0], [], [], []
+    for k in dnames:
+        bnds, c = domains[k]
+        for iname, side in bnds:
+            if iname not in inames:
+                raise ValueError("Domain %s contains an unknown interface %s." % (k, iname))
+            if side not in ("Inside", "Outside", 0, 1):
+                raise ValueError("Domain %s: interface %s has a wrong side direction (In/Out)" % (k, iname))
+            dint.append(inames.index(iname))
+            dins.append(1 if side in ("Inside", 0) else 0)
+        doff.append(len(dint))
+        dcond.append(float(c))
+    M, I, D = len(mnames), len(inames), len(dnames)
+    enc = lambda names: (C.c_char_p * len(names))(*[n.encode() for n in names])
+    arr = lambda ctype, v: (ctype * max(1, len(v)))(*v)
+    g = Geometry()
+    check(lib().omh_geometry_create(
+        M, enc(mnames), arr(C.c_int64, [len(p) for p in pts]),
+        (_lib.p_f64 * M)(*[p.ctypes.data_as(_lib.p_f64) for p in pts]),
+        arr(C.c_int64, [len(t) for t in tris]),
+        (_lib.p_i64 * M)(*[t.ctypes.data_as(_lib.p_i64) for t in tris]),
+        I, enc(inames), arr(C.c_int64, ioff), arr(C.c_int32, imesh), arr(C.c_int32, iori),
+        D, enc(dnames), arr(C.c_double, dcond), arr(C.c_int64, doff), arr(C.c_int32, dint),
+        arr(C.c_int32, dins), C.byref(g._h)))
+    g._load_sizes()
+    return g
+
+
+def read_geometry(geom_file, cond_file):
+    return Geometry(geom_file, cond_file)
+
+
+# ---------------------------------------------------------------------------
+# Matrices
+# ---------------------------------------------------------------------------
+class Matrix:
+    """Dense column-major FP64 matrix (matrix.h:81-92), host numpy storage."""
+
+    def __init__(self, a=None, ncol=None):
+        if a is None:
+            self._a = np.zeros((0, 0), order="F")
+        elif isinstance(a, (int, np.integer)):
+            self._a = np.zeros((int(a), int(ncol)), order="F")
+        elif isinstance(a, Matrix):
+            self._a = a._a
+        else:
+            arr = np.asarray(a)
+            if arr.dtype != np.float64 or arr.ndim != 2:
+                raise TypeError("Matrix needs a 2-D float64 array")
+            self._a = np.asfortranarray(arr)
+
+    def nlin(self):
+        return self._a.shape[0]
+
+    def ncol(self):
+        return self._a.shape[1]
+
+    def array(self):
+        return self._a
+
+    def load(self, fname):
+        self._a = load_matrix(fname)
+
+    def save(self, fname):
+        save_matrix(fname, self._a)
+
+
+class SparseMatrix:
+    """Head2EEGMat storage: three (column, value) pairs per row
+    (sparse_matrix.h:29 holds a map; the EEG interpolation has 3 per row)."""
+
+    def __init__(self, nlin, ncol, cols, vals):
+        self._n = (int(nlin), int(ncol))
+        self.cols = np.asarray(cols, dtype=np.int64).reshape(-1, 3)
+        self.vals = np.asarray(vals, dtype=np.float64).reshape(-1, 3)
+
+    def nlin(self):
+        return self._n[0]
+
+    def ncol(self):
+        return self._n[1]
+
+    def array(self):
+        E = np.zeros(self._n)
+        for k in range(3):
+            np.add.at(E, (np.arange(self._n[0]), self.cols[:, k]), self.vals[:, k])
+        return E
+
+
+class SymMatrix:
+    """Symmetric FP64 matrix.  The reference stores it packed upper
+    (symmatrix.h:52-62); here it lives on the device as a full matrix in the
+    triangles-first unknown order (nv = number of vertex unknowns, known
+    from the geometry), and is packed on the host on demand."""
+
+    def __init__(self, n=0, nv=None):
+        self._n = int(n)
+        self.nv = nv
+        self._dev = None       # torch float64 (n*n), column-major, triangles-first
+        self._packed = None    # numpy packed upper, reference order
+
+    @classmethod
+    def _from_device(cls, dev, n, nv):
+        m = cls(n, nv)
+        m._dev = dev
+        return m
+
+    @classmethod
+    def from_packed(cls, packed, nv=None):
+        packed = np.ascontiguousarray(packed, dtype=np.float64)
+        n = int((np.sqrt(8 * packed.size + 1) - 1) / 2)
+        if n * (n + 1) // 2 != packed.size:
+            raise ValueError("not a packed symmetric matrix")
+        m = cls(n, nv)
+        m._packed = packed
+        return m
+
+    def nlin(self):
+        return self._n
+
+    def ncol(self):
+        return self._n
+
+    def _ordering(self):
+        return ORDER_TRIANGLES_FIRST if self.nv is not None else ORDER_REFERENCE
+
+    def device(self):
+        """The full device matrix (float64, n*n column-major)."""
+        if self._dev is None:
+            if self._packed is None:
+                raise ValueError("empty SymMatrix")
+            P = _to_dev(self._packed)
+            D = _dev_f64(self._n * self._n)
+            nv = self.nv if self.nv is not None else self._n
+            check(lib().omh_sym_unpack(_ptr(P), self._n, nv, _ptr(D), self._n, self._ordering(), _stream()))
+            self._dev = D
+        return self._dev
+
+    def data(self):
+        """Packed upper storage in reference order (numpy)."""
+        if self._packed is None:
+            n = self._n
+            P = _dev_f64(n * (n + 1) // 2)
+            nv = self.nv if self.nv is not None else n
+            check(lib().omh_sym_pack(_ptr(self._dev), n, nv, n, self._ordering(), _ptr(P), _stream()))
+            self._packed = P.cpu().numpy()
+        return self._packed
+
+    def array(self):
+        n = self._n
+        P = self.data()
+        j = np.repeat(np.arange(n), np.arange(1, n + 1))
+        i = np.concatenate([np.arange(k + 1) for k in range(n)]) if n else np.zeros(0, np.int64)
+        M = np.zeros((n, n), order="F")
+        M[i, j] = P
+        M[j, i] = P
+        return M
+
+    def _factor(self, precision="fp64"):
+        if self.nv is None:
+            raise ValueError("SymMatrix.invert/solveLin need the unknown layout of a HeadMat (nv)")
+        return factor(self.device().clone(), self._n, precision)
+
+    def inverse(self):
+        """SymMatrix::inverse (symmatrix.h:259-278)."""
+        n = self._n
+        F = self._factor()
+        X = _t().eye(n, dtype=_t().float64, device="cuda").reshape(-1).clone()
+        F.solve(X, n)
+        return SymMatrix._from_device(X, n, self.nv)
+
+    def invert(self):
+        """SymMatrix::invert (symmatrix.h:280-297), in place."""
+        inv = self.inverse()
+        self._dev = inv._dev
+        self._packed = None
+
+    def solveLin(self, rhs):
+        """SymMatrix::solveLin(Matrix&) (symmatrix.cpp:133-152)."""
+        B = rhs.array() if isinstance(rhs, Matrix) else np.asarray(rhs, dtype=np.float64)
+        B = B.reshape(self._n, -1)
+        F = self._factor()
+        Bd = to_device_rows(B, self._n, self.nv)
+        F.solve(Bd, B.shape[1])
+        return Matrix(from_device_rows(Bd, self._n, self.nv, B.shape[1]))
+
+    def save(self, fname):
+        save_symmatrix(fname, self.data())
+
+    def load(self, fname):
+        P = load_symmatrix(fname)
+        m = SymMatrix.from_packed(P, self.nv)
+        self.__dict__.update(m.__dict__)
+
+
+def to_device_rows(B, n, nv):
+    """Host matrix (n x c, reference rows) -> device column-major buffer in
+    triangles-first row order."""
+    B = np.asarray(B, dtype=np.float64).reshape(n, -1)
+    c = B.shape[1]
+    src = _to_dev(np.asfortranarray(B).ravel(order="F"))
+    dst = _dev_f64(n * c)
+    check(lib().omh_permute_rows(_ptr(src), n, _ptr(dst), n, n, nv, c, ORDER_TRIANGLES_FIRST, 0, _stream()))
+    return dst
+
+
+def from_device_rows(D, n, nv, c):
+    dst = _dev_f64(n * c)
+    check(lib().omh_permute_rows(_ptr(D), n, _ptr(dst), n, n, nv, c, ORDER_TRIANGLES_FIRST, 1, _stream()))
+    return dst.cpu().numpy().reshape(c, n).T.copy(order="F")
+
+
+class Factor:
+    """A stored pivot-free signed Cholesky factor on the device."""
+
+    def __init__(self, L, n, work, precision, H64=None):
+        self.L, self.n, self.work, self.precision, self.H64 = L, n, work, precision, H64
+        self.iterations = 0
+        self.rel_update = 0.0
+
+    def solve(self, B, nrhs, max_iter=10, tol=1e-14):
+        """B <- H^-1 B (float64 device buffer, n x nrhs column-major)."""
+        if self.precision == "fp64":
+            check(lib().omh_sym_solve_f64(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n, nrhs,
+                                          _stream()))
+            return B
+        X = _dev_f64(self.n * nrhs)
+        it, rel = C.c_int32(), C.c_double()
+        check(lib().omh_mixed_solve(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work), _ptr(B),
+                                    self.n, _ptr(X), self.n, nrhs, max_iter, tol, C.byref(it), C.byref(rel),
+                                    _stream()))
+        self.iterations, self.rel_update = it.value, rel.value
+        B.copy_(X)
+        return B
+
+
+def factor(Hdev, n, precision="fp64"):
+    """Factor a device HeadMat (triangles-first order).  precision 'fp64'
+    factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
+    factors an FP32 copy."""
+    torch = _t()
+    info = C.c_int32()
+    if precision == "fp64":
+        work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
+        check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
+        F = Factor(Hdev, n, work, "fp64")
+    elif precision == "mixed":
+        L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
+        check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
+        work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
+        check(lib().omh_sym_factor_f32(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
+        F = Factor(L32, n, work, "mixed", H64=Hdev)
+    else:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    if info.value != 0:
+        raise _lib.OpenMEEGError("pivot-free factorisation broke down at pivot %d" % (info.value - 1))
+    return F
+
+
+# ---------------------------------------------------------------------------
+# Assembly (assemble.h:32-35, assembleSourceMat.h:24-29)
+# ---------------------------------------------------------------------------
+def _head_mat_device(geom, integrator=None):
+    integ = integrator or Integrator(3, 0, 0.005)
+    if integ.levels != 0:
+        raise ValueError("adaptive HeadMat integration is not supported on this path (levels must be 0)")
+    n = geom.size
+    H = _dev_f64(n * n)
+    check(lib().omh_head_mat(geom._h, integ.order, _ptr(H), n, ORDER_TRIANGLES_FIRST, _stream()))
+    return H
+
+
+def HeadMat(geom, integrator=None):
+    """HeadMat(geo, Integrator(3,0,0.005)) -- assemble.h:32."""
+    return SymMatrix._from_device(_head_mat_device(geom, integrator), geom.size, geom.n_vertex_unknowns)
+
+
+def _dipoles_array(dipoles):
+    a = dipoles.array() if isinstance(dipoles, Matrix) else np.asarray(dipoles, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != 6:
+        raise ValueError("Dipoles must have 6 columns (x y z qx qy qz)")
+    return np.ascontiguousarray(a)
+
+
+def _dsm_device(geom, dipoles, domain_name="", integrator=None):
+    integ = integrator or Integrator(3, 10, 0.001)
+    dip = _dipoles_array(dipoles)
+    n, nd = geom.size, len(dip)
+    D = _dev_f64(n * max(nd, 1))
+    dd = _to_dev(dip)
+    doms = None
+    if domain_name:
+        k = geom.domain_index(domain_name)
+        doms = (C.c_int32 * nd)(*([k] * nd))
+    check(lib().omh_dip_source_mat(geom._h, _ptr(dd), nd, doms, integ.order, integ.levels, integ.tolerance, _ptr(D),
+                                   n, ORDER_TRIANGLES_FIRST, _stream()))
+    return D
+
+
+def DipSourceMat(geom, dipoles, domain_name="", integrator=None):
+    """DipSourceMat(geom, dipoles, domain) -- _openmeeg.i:399-413,
+    SourceMatrix<Dipole> with the adaptive Integrator(3,10,0.001)."""
+    D = _dsm_device(geom, dipoles, domain_name, integrator)
+    nd = len(_dipoles_array(dipoles))
+    return Matrix(from_device_rows(D, geom.size, geom.n_vertex_unknowns, nd))
+
+
+def _electrodes_array(sensors):
+    if isinstance(sensors, Sensors):
+        return sensors.positions
+    a = sensors.array() if isinstance(sensors, Matrix) else np.asarray(sensors, dtype=np.float64)
+    return np.ascontiguousarray(a[:, :3])
+
+
+def Head2EEGMat(geom, sensors):
+    """Head2EEGMat(geo, electrodes) -- assemble.h:35."""
+    el = _electrodes_array(sensors)
+    ne = len(el)
+    cols = np.zeros((ne, 3), dtype=np.int64)
+    vals = np.zeros((ne, 3))
+    if ne:
+        de = _to_dev(el)
+        check(lib().omh_head2eeg_mat(geom._h, _ptr(de), ne, cols.ctypes.data_as(_lib.p_i64),
+                                     vals.ctypes.data_as(_lib.p_f64), _stream()))
+    return SparseMatrix(ne, geom.size, cols, vals)
+
+
+def _eeg_rhs(geom, h2em):
+    """E^T (n x n_e, triangles-first rows) on the device."""
+    n, ne = geom.size, h2em.nlin()
+    B = np.zeros((n, ne), order="F")
+    rows = geom.tri_first_index(h2em.cols)
+    for k in range(3):
+        np.add.at(B, (rows[:, k], np.arange(ne)), h2em.vals[:, k])
+    return _to_dev(B.ravel(order="F"))
+
+
+def _gemm_tn(Y, D, n, ne, nd):
+    G = _dev_f64(ne * nd)
+    check(lib().omh_gemm_f64(1, 0, ne, nd, n, 1.0, _ptr(Y), n, _ptr(D), n, 0.0, _ptr(G), ne, _stream()))
+    return G
+
+
+def GainEEG(hminv, dsm, h2em):
+    """GainEEG(HeadMatInv, SourceMat, Head2EEGMat) = (E H^-1) D -- gain.h:69-76."""
+    n = hminv.nlin()
+    nv = hminv.nv
+    if nv is None:
+        raise ValueError("GainEEG needs the HeadMat unknown layout (a SymMatrix from HeadMat)")
+    Hi = hminv.device()
+    ne = h2em.nlin()
+    rows = np.where(h2em.cols >= nv, h2em.cols - nv, h2em.cols + (n - nv))
+    dc = _to_dev(rows.ravel(), np.int64)
+    dv = _to_dev(h2em.vals.ravel())
+    EH = _dev_f64(ne * n)
+    check(lib().omh_sparse_rows(_ptr(dc), _ptr(dv), ne, _ptr(Hi), n, n, _ptr(EH), ne, _stream()))
+    D = dsm.array()
+    Dd = to_device_rows(D, n, nv)
+    nd = D.shape[1]
+    G = _dev_f64(ne * nd)
+    check(lib().omh_gemm_f64(0, 0, ne, nd, n, 1.0, _ptr(EH), ne, _ptr(Dd), n, 0.0, _ptr(G), ne, _stream()))
+    return Matrix(G.cpu().numpy().reshape(nd, ne).T.copy(order="F"))
+
+
+def GainEEGadjoint(geom, dipoles, hm, h2em, precision="fp64"):
+    """GainEEGadjoint(geo, dipoles, HeadMat, Head2EEGMat) -- gain.h:78-89:
+    Y = H^-1 E^T with the stored factor, then G = Y^T DSM."""
+    n, ne = geom.size, h2em.nlin()
+    F = factor(hm.device().clone(), n, precision)
+    Y = _eeg_rhs(geom, h2em)
+    F.solve(Y, ne)
+    D = _dsm_device(geom, dipoles, "")
+    nd = len(_dipoles_array(dipoles))
+    G = _gemm_tn(Y, D, n, ne, nd)
+    return Matrix(G.cpu().numpy().reshape(nd, ne).T.copy(order="F"))
+
+
+class Sensors:
+    """EEG electrodes: Sensors::load (sensors.cpp:36-143) for labelled or
+    unlabelled position files, or Sensors(positions)."""
+
+    def __init__(self, positions=None, geom=None):
+        self.labels = []
+        self.positions = np.zeros((0, 3))
+        if positions is not None:
+            a = positions.array() if isinstance(positions, Matrix) else np.asarray(positions, dtype=np.float64)
+            self.positions = np.ascontiguousarray(a[:, :3], dtype=np.float64)
+
+    def load(self, fname):
+        rows = []
+        with open(fname) as f:
+            for ln in f:
+                tok = ln.split()
+                if tok and not tok[0].startswith("#"):
+                    rows.append(tok)
+        if not rows:
+            raise ValueError("empty sensor file")
+        if any(len(r) != len(rows[0]) for r in rows):
+            raise ValueError("Problem while reading Sensors file: each line should have the same number of elements")
+        labeled = rows[0][0].count(".") != 1
+        if labeled:
+            self.labels = [r[0] for r in rows]
+            rows = [r[1:] for r in rows]
+        self.positions = np.ascontiguousarray(np.array(rows, dtype=np.float64)[:, :3])
+
+    def getNumberOfSensors(self):
+        return len(self.positions)
+
+    def getPositions(self):
+        return Matrix(np.asfortranarray(self.positions))
+
+
+# ---------------------------------------------------------------------------
+# Device pipeline (the path the benchmark times)
+# ---------------------------------------------------------------------------
+def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="", integrator=None,
+                      dsm_integrator=None, max_refine=10, refine_tol=1e-14, sync_stages=False):
+    """HeadMat, DipSourceMat and Head2EEGMat assembly, the HeadMat factor and
+    the adjoint gain G = (H^-1 E^T)^T DSM, all on the device.  Returns
+    (G as a device tensor n_e x n_dip in column-major storage, info dict)."""
+    torch = _t()
+    n = geom.size
+    times = {}
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    marks = [("start", ev())]
+    marks[-1][1].record()
+    H = _head_mat_device(geom, integrator)
+    marks.append(("headmat", ev())); marks[-1][1].record()
+    D = _dsm_device(geom, dipoles, domain_name, dsm_integrator)
+    marks.append(("dsm", ev())); marks[-1][1].record()
+    h2em = Head2EEGMat(geom, electrodes)
+    Y = _eeg_rhs(geom, h2em)
+    marks.append(("h2em", ev())); marks[-1][1].record()
+    F = factor(H, n, precision)
+    marks.append(("factor", ev())); marks[-1][1].record()
+    F.solve(Y, h2em.nlin(), max_iter=max_refine, tol=refine_tol)
+    marks.append(("solve", ev())); marks[-1][1].record()
+    nd = len(_dipoles_array(dipoles))
+    G = _gemm_tn(Y, D, n, h2em.nlin(), nd)
+    marks.append(("gain", ev())); marks[-1][1].record()
+    torch.cuda.synchronize()
+    for (a, ea), (b, eb) in zip(marks[:-1], marks[1:]):
+        times[b] = ea.elapsed_time(eb) / 1e3
+    info = dict(times=times, refine_iterations=F.iterations, refine_rel_update=F.rel_update, n=n,
+                n_electrodes=h2em.nlin(), n_dipoles=nd)
+    del F, H, D, Y
+    return G, info
+
+
+def gain_to_numpy(G, ne, nd):
+    return G.cpu().numpy().reshape(nd, ne).T.copy(order="F")
+
+
+# ---------------------------------------------------------------------------
+# File IO: TrivialBinIO (.bin, TrivialBinIO.H:107-137) and ascii (.txt)
+# ---------------------------------------------------------------------------
+def save_matrix(fname, a):
+    a = np.asfortranarray(a, dtype=np.float64)
+    if fname.endswith(".bin"):
+        with open(fname, "wb") as f:
+            f.write(struct.pack("<II", a.shape[0], a.shape[1]))
+            f.write(a.tobytes(order="F"))
+    else:
+        np.savetxt(fname, a)
+
+
+def load_matrix(fname):
+    if fname.endswith(".bin"):
+        with open(fname, "rb") as f:
+            nl, nc = struct.unpack("<II", f.read(8))
+            return np.frombuffer(f.read(8 * nl * nc), dtype=np.float64).reshape(nc, nl).T.copy(order="F")
+    return np.asfortranarray(np.loadtxt(fname, ndmin=2))
+
+
+def save_symmatrix(fname, packed):
+    packed = np.ascontiguousarray(packed, dtype=np.float64)
+    n = int((np.sqrt(8 * packed.size + 1) - 1) / 2)
+    with open(fname, "wb") as f:
+        f.write(struct.pack("<I", n))
+        f.write(packed.tobytes())
+
+
+def load_symmatrix(fname):
+    with open(fname, "rb") as f:
+        (n,) = struct.unpack("<I", f.read(4))
+        return np.frombuffer(f.read(8 * n * (n + 1) // 2), dtype=np.float64).copy()

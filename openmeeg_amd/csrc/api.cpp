@@ -1,0 +1,445 @@
+// extern "C" boundary of the library (declared in include/openmeeg_amd.h).
+// Every entry point catches C++ exceptions and turns them into a status code
+// plus a per-thread message.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/openmeeg_amd.h"
+#include "assembly.hpp"
+#include "common.hpp"
+#include "geometry.hpp"
+#include "layout.hpp"
+#include "linalg.hpp"
+#include "sources.hpp"
+
+struct omh_geometry {
+    std::unique_ptr<omh::Geometry> g;
+};
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_err = std::string("invalid argument: ") + e.what();
+        return 2;
+    } catch (const omh::HipError& e) {
+        g_err = std::string("HIP error: ") + e.what();
+        return 3;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return 4;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    } catch (...) {
+        g_err = "unknown error";
+        return 1;
+    }
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+omh::Layout layout_of(const omh::Geometry& g, int64_t ld, int ordering) {
+    if (ordering != OMH_ORDER_REFERENCE && ordering != OMH_ORDER_TRIANGLES_FIRST)
+        throw std::invalid_argument("unknown ordering");
+    if (ld < g.size()) throw std::invalid_argument("leading dimension smaller than the number of unknowns");
+    omh::Layout L;
+    L.ld = ld;
+    L.nv = g.n_vert_unknowns;
+    L.nt = g.n_tri_unknowns;
+    L.tri_first = ordering == OMH_ORDER_TRIANGLES_FIRST ? 1 : 0;
+    return L;
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string(what) + " is NULL");
+}
+
+std::vector<std::vector<std::array<double, 3>>> gather_pts(int n, const int64_t* nv, const double* const* v) {
+    std::vector<std::vector<std::array<double, 3>>> pts(n);
+    for (int m = 0; m < n; ++m) {
+        if (nv[m] <= 0) throw std::invalid_argument("mesh without vertices");
+        need(v[m], "vertices");
+        pts[m].resize(nv[m]);
+        for (int64_t i = 0; i < nv[m]; ++i) pts[m][i] = {v[m][3 * i], v[m][3 * i + 1], v[m][3 * i + 2]};
+    }
+    return pts;
+}
+
+std::vector<std::vector<std::array<int64_t, 3>>> gather_tris(int n, const int64_t* nt, const int64_t* const* t) {
+    std::vector<std::vector<std::array<int64_t, 3>>> tris(n);
+    for (int m = 0; m < n; ++m) {
+        if (nt[m] <= 0) throw std::invalid_argument("mesh without triangles");
+        need(t[m], "triangles");
+        tris[m].resize(nt[m]);
+        for (int64_t i = 0; i < nt[m]; ++i) tris[m][i] = {t[m][3 * i], t[m][3 * i + 1], t[m][3 * i + 2]};
+    }
+    return tris;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* omh_last_error(void) { return g_err.c_str(); }
+
+int omh_geometry_load(const char* geom_file, const char* cond_file, omh_geometry** out) {
+    return guard([&] {
+        need(geom_file, "geometry file");
+        need(cond_file, "conductivity file");
+        need(out, "out");
+        auto h = std::make_unique<omh_geometry>();
+        h->g = omh::load_geometry(geom_file, cond_file);
+        *out = h.release();
+    });
+}
+
+int omh_geometry_nested(int n_layers, const int64_t* n_vertices, const double* const* vertices,
+                        const int64_t* n_triangles, const int64_t* const* triangles, const double* conductivities,
+                        omh_geometry** out) {
+    return guard([&] {
+        if (n_layers < 1) throw std::invalid_argument("at least one layer is needed");
+        need(n_vertices, "n_vertices");
+        need(vertices, "vertices");
+        need(n_triangles, "n_triangles");
+        need(triangles, "triangles");
+        need(conductivities, "conductivities");
+        need(out, "out");
+        const int L = n_layers;
+        std::vector<std::string> names;
+        if (L == 3) names = {"Cortex", "Skull", "Head"};
+        else if (L == 1) names = {"Cortex"};
+        else
+            for (int k = 0; k < L; ++k) names.push_back("Layer" + std::to_string(k));
+        auto g = std::make_unique<omh::Geometry>();
+        g->import_meshes(names, gather_pts(L, n_vertices, vertices), gather_tris(L, n_triangles, triangles));
+        std::vector<omh::Interface> itf;
+        for (int k = 0; k < L; ++k) itf.push_back(omh::Interface{names[k], {{k, 1}}});
+        auto dom = [&](const std::string& name, double c, std::vector<std::pair<int, bool>> b) {
+            omh::Domain d;
+            d.name = name;
+            d.cond = c;
+            for (auto& x : b) d.boundaries.push_back({itf[x.first], x.second});
+            g->domains.push_back(d);
+        };
+        // _make_geometry.py:167-204
+        if (L == 3) {
+            dom("Scalp", conductivities[2], {{1, false}, {2, true}});
+            dom("Brain", conductivities[0], {{0, true}});
+            dom("Air", 0.0, {{2, false}});
+            dom("Skull", conductivities[1], {{0, false}, {1, true}});
+        } else if (L == 1) {
+            dom("Brain", conductivities[0], {{0, true}});
+            dom("Air", 0.0, {{0, false}});
+        } else {
+            dom("Brain", conductivities[0], {{0, true}});
+            for (int k = 1; k < L; ++k) dom("Layer" + std::to_string(k), conductivities[k], {{k - 1, false}, {k, true}});
+            dom("Air", 0.0, {{L - 1, false}});
+        }
+        g->finalize();
+        auto h = std::make_unique<omh_geometry>();
+        h->g = std::move(g);
+        *out = h.release();
+    });
+}
+
+int omh_geometry_create(int n_meshes, const char* const* mesh_names, const int64_t* n_vertices,
+                        const double* const* vertices, const int64_t* n_triangles, const int64_t* const* triangles,
+                        int n_interfaces, const char* const* interface_names, const int64_t* interface_offsets,
+                        const int32_t* interface_meshes, const int32_t* interface_orientations, int n_domains,
+                        const char* const* domain_names, const double* domain_conductivities,
+                        const int64_t* domain_offsets, const int32_t* domain_interfaces, const int32_t* domain_inside,
+                        omh_geometry** out) {
+    return guard([&] {
+        if (n_meshes < 1 || n_interfaces < 1 || n_domains < 1)
+            throw std::invalid_argument("meshes, interfaces and domains must be non-empty");
+        need(out, "out");
+        std::vector<std::string> names;
+        for (int m = 0; m < n_meshes; ++m) names.push_back(mesh_names[m] ? mesh_names[m] : std::to_string(m + 1));
+        auto g = std::make_unique<omh::Geometry>();
+        g->import_meshes(names, gather_pts(n_meshes, n_vertices, vertices),
+                         gather_tris(n_meshes, n_triangles, triangles));
+        std::vector<omh::Interface> itf(n_interfaces);
+        for (int i = 0; i < n_interfaces; ++i) {
+            itf[i].name = interface_names && interface_names[i] ? interface_names[i] : std::to_string(i + 1);
+            for (int64_t p = interface_offsets[i]; p < interface_offsets[i + 1]; ++p) {
+                const int m = interface_meshes[p], o = interface_orientations[p];
+                if (m < 0 || m >= n_meshes) throw std::invalid_argument("interface mesh index out of range");
+                if (o != 1 && o != -1) throw std::invalid_argument("interface orientation must be +1 or -1");
+                itf[i].omeshes.push_back({m, o});
+            }
+            if (itf[i].omeshes.empty()) throw std::invalid_argument("empty interface " + itf[i].name);
+        }
+        for (int d = 0; d < n_domains; ++d) {
+            omh::Domain dm;
+            dm.name = domain_names && domain_names[d] ? domain_names[d] : std::to_string(d + 1);
+            dm.cond = domain_conductivities[d];
+            for (int64_t p = domain_offsets[d]; p < domain_offsets[d + 1]; ++p) {
+                const int i = domain_interfaces[p];
+                if (i < 0 || i >= n_interfaces) throw std::invalid_argument("domain interface index out of range");
+                dm.boundaries.push_back({itf[i], domain_inside[p] != 0});
+            }
+            if (dm.boundaries.empty()) throw std::invalid_argument("domain " + dm.name + " has no boundary");
+            g->domains.push_back(dm);
+        }
+        g->finalize();
+        auto h = std::make_unique<omh_geometry>();
+        h->g = std::move(g);
+        *out = h.release();
+    });
+}
+
+void omh_geometry_free(omh_geometry* g) { delete g; }
+
+int omh_geometry_sizes(const omh_geometry* g, int64_t out[8]) {
+    return guard([&] {
+        need(g, "geometry");
+        need(out, "out");
+        const omh::Geometry& G = *g->g;
+        out[0] = G.size();
+        out[1] = G.nb_parameters;
+        out[2] = G.nb_current_barrier_triangles;
+        out[3] = G.n_vert_unknowns;
+        out[4] = G.n_tri_unknowns;
+        out[5] = (int64_t)G.meshes.size();
+        out[6] = (int64_t)G.domains.size();
+        out[7] = G.nested ? 1 : 0;
+    });
+}
+
+int omh_geometry_domain_index(const omh_geometry* g, const char* name, int32_t* index) {
+    return guard([&] {
+        need(g, "geometry");
+        need(name, "name");
+        need(index, "index");
+        *index = g->g->domain_index(name);
+    });
+}
+
+int omh_geometry_domain_name(const omh_geometry* g, int32_t index, char* buf, int64_t buflen) {
+    return guard([&] {
+        need(g, "geometry");
+        need(buf, "buf");
+        const auto& d = g->g->domains.at(index);
+        if ((int64_t)d.name.size() + 1 > buflen) throw std::invalid_argument("buffer too small");
+        std::memcpy(buf, d.name.c_str(), d.name.size() + 1);
+    });
+}
+
+int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_H, "d_H");
+        omh::assemble_headmat(*g->g, order, d_H, layout_of(*g->g, ld, ordering), S(stream));
+    });
+}
+
+int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,
+                       void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(h_domains, "h_domains");
+        if (n > 0) need(d_points, "d_points");
+        const auto d = omh::locate_domains(*g->g, d_points, stride, n, S(stream));
+        for (int64_t i = 0; i < n; ++i) h_domains[i] = d[i];
+    });
+}
+
+int omh_dip_source_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains, int order,
+                       int levels, double tol, double* d_out, int64_t ld, int ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_out, "d_out");
+        if (n_dip < 0) throw std::invalid_argument("negative number of dipoles");
+        if (n_dip > 0) need(d_dipoles, "d_dipoles");
+        std::vector<int> dom;
+        if (h_domains) {
+            dom.assign(h_domains, h_domains + n_dip);
+            for (int d : dom)
+                if (d < 0 || d >= (int)g->g->domains.size()) throw std::invalid_argument("domain index out of range");
+        } else {
+            dom = omh::locate_domains(*g->g, d_dipoles, 6, n_dip, S(stream));
+        }
+        omh::dip_source_mat(*g->g, d_dipoles, n_dip, dom, order, levels, tol, d_out, ld,
+                            layout_of(*g->g, ld, ordering), S(stream));
+    });
+}
+
+int omh_head2eeg_mat(omh_geometry* g, const double* d_electrodes, int64_t n_el, int64_t* h_cols, double* h_vals,
+                     void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        if (n_el > 0) {
+            need(d_electrodes, "d_electrodes");
+            need(h_cols, "h_cols");
+            need(h_vals, "h_vals");
+        }
+        omh::head2eeg(*g->g, d_electrodes, n_el, h_cols, h_vals, S(stream));
+    });
+}
+
+int64_t omh_factor_workspace_bytes(int64_t n, int32_t elem_size) {
+    if (n <= 0 || (elem_size != 4 && elem_size != 8)) return -1;
+    return (int64_t)omh::factor_workspace_bytes(n, (size_t)elem_size);
+}
+
+int omh_sym_factor_f64(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream) {
+    return guard([&] {
+        need(d_A, "d_A");
+        need(d_work, "d_work");
+        if (n <= 0 || ld < n) throw std::invalid_argument("bad matrix size");
+        const auto fi = omh::sym_factor<double>(d_A, n, ld, d_work, S(stream));
+        if (info) *info = fi.info;
+    });
+}
+
+int omh_sym_factor_f32(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream) {
+    return guard([&] {
+        need(d_A, "d_A");
+        need(d_work, "d_work");
+        if (n <= 0 || ld < n) throw std::invalid_argument("bad matrix size");
+        const auto fi = omh::sym_factor<float>(d_A, n, ld, d_work, S(stream));
+        if (info) *info = fi.info;
+    });
+}
+
+int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
+                      int64_t nrhs, void* stream) {
+    return guard([&] {
+        need(d_L, "d_L");
+        need(d_work, "d_work");
+        need(d_B, "d_B");
+        if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
+        omh::sym_solve<double>(d_L, n, ld, d_work, d_B, ldb, nrhs, S(stream));
+    });
+}
+
+int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
+                      int64_t nrhs, void* stream) {
+    return guard([&] {
+        need(d_L, "d_L");
+        need(d_work, "d_work");
+        need(d_B, "d_B");
+        if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
+        omh::sym_solve<float>(d_L, n, ld, d_work, d_B, ldb, nrhs, S(stream));
+    });
+}
+
+int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32, const void* d_work32,
+                    const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs, int32_t max_iter,
+                    double tol, int32_t* iterations, double* rel_update, void* stream) {
+    return guard([&] {
+        need(d_H, "d_H");
+        need(d_L32, "d_L32");
+        need(d_work32, "d_work32");
+        need(d_B, "d_B");
+        need(d_X, "d_X");
+        if (n <= 0 || ldh < n || ld32 < n || ldb < n || ldx < n || nrhs < 0 || max_iter < 1)
+            throw std::invalid_argument("bad sizes");
+        double rel = 0.0;
+        const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
+                                         &rel, S(stream));
+        if (iterations) *iterations = it;
+        if (rel_update) *rel_update = rel;
+    });
+}
+
+int omh_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, double alpha, const double* d_A,
+                 int64_t lda, const double* d_B, int64_t ldb, double beta, double* d_C, int64_t ldc, void* stream) {
+    return guard([&] {
+        if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("negative size");
+        omh::gemm<double>(trans_a != 0, trans_b != 0, false, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
+                          S(stream));
+    });
+}
+
+int omh_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, float alpha, const float* d_A,
+                 int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C, int64_t ldc, void* stream) {
+    return guard([&] {
+        if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("negative size");
+        omh::gemm<float>(trans_a != 0, trans_b != 0, false, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
+                         S(stream));
+    });
+}
+
+int omh_convert_f64_f32(const double* d_src, int64_t lds, float* d_dst, int64_t ldd, int64_t m, int64_t n,
+                        int32_t lower_only, void* stream) {
+    return guard([&] {
+        need(d_src, "d_src");
+        need(d_dst, "d_dst");
+        omh::convert_f64_to_f32(d_src, lds, d_dst, ldd, m, n, lower_only != 0, S(stream));
+    });
+}
+
+int omh_sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* d_Hm, int64_t ldh, int64_t n,
+                    double* d_out, int64_t ldo, void* stream) {
+    return guard([&] {
+        need(d_cols, "d_cols");
+        need(d_vals, "d_vals");
+        need(d_Hm, "d_Hm");
+        need(d_out, "d_out");
+        omh::sparse_rows(d_cols, d_vals, ne, d_Hm, ldh, n, d_out, ldo, S(stream));
+    });
+}
+
+static omh::Layout raw_layout(int64_t n, int64_t nv, int64_t ld, int ordering) {
+    if (nv < 0 || nv > n) throw std::invalid_argument("bad vertex-unknown count");
+    if (ld < n) throw std::invalid_argument("leading dimension too small");
+    omh::Layout L;
+    L.ld = ld;
+    L.nv = nv;
+    L.nt = n - nv;
+    L.tri_first = ordering == OMH_ORDER_TRIANGLES_FIRST ? 1 : 0;
+    return L;
+}
+
+int omh_sym_unpack(const double* d_packed, int64_t n, int64_t nv, double* d_H, int64_t ld, int ordering, void* stream) {
+    return guard([&] {
+        need(d_packed, "d_packed");
+        need(d_H, "d_H");
+        omh::sym_unpack(d_packed, n, d_H, raw_layout(n, nv, ld, ordering), S(stream));
+    });
+}
+
+int omh_sym_pack(const double* d_H, int64_t n, int64_t nv, int64_t ld, int ordering, double* d_packed, void* stream) {
+    return guard([&] {
+        need(d_packed, "d_packed");
+        need(d_H, "d_H");
+        omh::sym_pack(d_H, n, raw_layout(n, nv, ld, ordering), d_packed, S(stream));
+    });
+}
+
+int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ldd, int64_t n, int64_t nv,
+                     int64_t ncol, int ordering, int32_t to_reference, void* stream) {
+    return guard([&] {
+        need(d_src, "d_src");
+        need(d_dst, "d_dst");
+        if (lds < n || ldd < n) throw std::invalid_argument("leading dimension too small");
+        omh::permute_rows(d_src, lds, d_dst, ldd, n, ncol, raw_layout(n, nv, n, ordering), to_reference != 0,
+                          S(stream));
+    });
+}
+
+int omh_release_scratch(void) {
+    return guard([&] { omh::release_scratch(); });
+}
+
+int omh_probe_mfma_layout(double* d_out, void* stream) {
+    return guard([&] {
+        need(d_out, "d_out");
+        omh::probe_mfma_layout(d_out, S(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,511 @@
+// HeadMat assembly on the GPU: S, N, D/D* blocks and deflation.
+//
+// Follows Details::HeadMatrix (assembleHeadMat.cpp:74-115) pair by pair:
+// HeadMatrixBlocks::set_blocks (operators.h:517-525) with the block rules of
+// DiagonalBlock (operators.h:180-197) / NonDiagonalBlock (:363-384), then
+// deflate (assembleHeadMat.cpp:28-60).
+//
+// Work decomposition (FP64-VALU bound, see the roofline notes in DESIGN.md):
+//   S  : one thread per triangle pair, 16 analytic OperatorS evaluations
+//        (the analytic triangle is the row mesh, the Gauss triangle the
+//        column mesh, as the reference).  Diagonal blocks run over the upper
+//        triangle of tiles only.
+//   D  : two passes.  Pass 1: one thread per (row triangle, column triangle)
+//        pair computes the three P1 integrals into a scratch tile.  Pass 2:
+//        one thread per (row triangle, column vertex) gathers them over the
+//        vertex's triangles in mesh order -- the accumulation order of the
+//        reference, without atomics.
+//   N  : one thread per vertex pair gathering the ~6x6 incident triangle
+//        pairs' S values (operators.h:134-148).
+// Every entry is written to both halves of the symmetric matrix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "assembly.hpp"
+#include "bem_device.hpp"
+#include "common.hpp"
+#include "device_geometry.hpp"
+
+namespace omh {
+
+// ---------------------------------------------------------------------------
+// scratch arena and fill
+// ---------------------------------------------------------------------------
+namespace {
+struct Arena {
+    void* p[SCR_NSLOTS] = {};
+    size_t sz[SCR_NSLOTS] = {};
+};
+std::mutex g_arena_mu;
+std::map<int, Arena> g_arenas;
+}  // namespace
+
+void* scratch(ScratchSlot slot, size_t bytes) {
+    int dev = 0;
+    OMH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    Arena& a = g_arenas[dev];
+    if (a.sz[slot] < bytes) {
+        OMH_HIP(hipDeviceSynchronize());
+        if (a.p[slot]) OMH_HIP(hipFree(a.p[slot]));
+        a.p[slot] = nullptr;
+        a.sz[slot] = 0;
+        const size_t want = (bytes + 4095) & ~size_t(4095);
+        OMH_HIP(hipMalloc(&a.p[slot], want));
+        a.sz[slot] = want;
+    }
+    return a.p[slot];
+}
+
+void release_scratch() {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& kv : g_arenas) {
+        (void)hipSetDevice(kv.first);
+        (void)hipDeviceSynchronize();
+        for (int s = 0; s < SCR_NSLOTS; ++s)
+            if (kv.second.p[s]) (void)hipFree(kv.second.p[s]);
+    }
+    g_arenas.clear();
+    (void)hipSetDevice(cur);
+}
+
+__global__ void k_fill(double* p, int64_t rows, int64_t cols, int64_t ld, double v) {
+    const int64_t total = rows * cols;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = t / rows, r = t - c * rows;
+        p[r + c * ld] = v;
+    }
+}
+
+void fill_f64(double* p, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return;
+    const int64_t total = rows * cols;
+    const int64_t blocks = std::min<int64_t>(cdiv(total, 256), 65536);
+    k_fill<<<(unsigned)blocks, 256, 0, s>>>(p, rows, cols, ld, v);
+    OMH_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// device geometry tables
+// ---------------------------------------------------------------------------
+DeviceGeometry::~DeviceGeometry() {
+    if (device < 0) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(tri);
+    (void)hipFree(tri_vid);
+    (void)hipFree(vert_unk);
+    (void)hipFree(slot_vid);
+    (void)hipFree(adj_off);
+    (void)hipFree(adj);
+    (void)hipSetDevice(cur);
+}
+
+template <typename T>
+static T* upload(const std::vector<T>& v) {
+    T* d = nullptr;
+    const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    OMH_HIP(hipMalloc(&d, bytes));
+    if (!v.empty()) OMH_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+DeviceGeometry& device_geometry(Geometry& g) {
+    int dev = 0;
+    OMH_HIP(hipGetDevice(&dev));
+    if (g.dev && g.dev->device == dev) return *g.dev;
+    auto dg = std::make_shared<DeviceGeometry>();
+    std::vector<TriRec> recs;
+    std::vector<int32_t> tvid;
+    std::vector<int32_t> slots;
+    std::vector<int64_t> adj_off{0};
+    std::vector<AdjRec> adj;
+    for (const auto& m : g.meshes) {
+        MeshDev md;
+        md.tri_begin = (int64_t)recs.size();
+        md.slot_begin = (int64_t)slots.size();
+        md.tri_unk0 = m.tri_first_index;
+        md.barrier = m.current_barrier ? 1 : 0;
+        for (const auto& t : m.tris) {
+            TriRec r{};
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) r.p[k][c] = g.vertices[t[k]][c];
+            // Triangle ctor, triangle.h:57-62
+            double e1[3], e2[3], on[3];
+            for (int c = 0; c < 3; ++c) { e1[c] = r.p[1][c] - r.p[0][c]; e2[c] = r.p[2][c] - r.p[0][c]; }
+            on[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            on[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            on[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            const double nr = std::sqrt(on[0] * on[0] + on[1] * on[1] + on[2] * on[2]);
+            r.area = 0.5 * nr;
+            r.inv_area = 1.0 / r.area;
+            for (int c = 0; c < 3; ++c) r.n[c] = on[c] / nr;
+            // OperatorS ctor, analytics.h:22-30
+            for (int i = 0; i < 3; ++i) {
+                const int nx = (i + 1) % 3;
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = r.p[i][c] - r.p[nx][c];
+                const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                for (int c = 0; c < 3; ++c) v[c] /= len;
+                r.len[i] = len;
+                r.enrm[i][0] = r.n[1] * v[2] - r.n[2] * v[1];
+                r.enrm[i][1] = r.n[2] * v[0] - r.n[0] * v[2];
+                r.enrm[i][2] = r.n[0] * v[1] - r.n[1] * v[0];
+                // P1SingleLayerPotential ctor, analytics.h:81-85: D_i = p_{i+1}-p_i
+                double d[3];
+                for (int c = 0; c < 3; ++c) d[c] = r.p[nx][c] - r.p[i][c];
+                const double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                for (int c = 0; c < 3; ++c) r.U[i][c] = d[c] / dl;
+            }
+            recs.push_back(r);
+            for (int k = 0; k < 3; ++k) tvid.push_back((int32_t)t[k]);
+        }
+        md.tri_end = (int64_t)recs.size();
+        // vertex -> incident triangles in mesh order (mesh.h:252-257)
+        std::map<int64_t, std::vector<std::pair<int64_t, int>>> inc;
+        for (size_t ti = 0; ti < m.tris.size(); ++ti)
+            for (int k = 0; k < 3; ++k) inc[m.tris[ti][k]].push_back({md.tri_begin + (int64_t)ti, k});
+        static const int opp[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+        for (int64_t v : m.verts) {
+            slots.push_back((int32_t)v);
+            auto it = inc.find(v);
+            if (it != inc.end())
+                for (const auto& tk : it->second) {
+                    AdjRec a{};
+                    a.tri = (int32_t)tk.first;
+                    a.k = tk.second;
+                    const TriRec& r = recs[tk.first];
+                    for (int c = 0; c < 3; ++c) a.cb[c] = r.p[opp[a.k][0]][c] - r.p[opp[a.k][1]][c];
+                    adj.push_back(a);
+                }
+            adj_off.push_back((int64_t)adj.size());
+        }
+        md.slot_end = (int64_t)slots.size();
+        dg->meshes.push_back(md);
+    }
+    dg->n_tris = (int64_t)recs.size();
+    dg->n_verts = (int64_t)g.vertices.size();
+    dg->n_slots = (int64_t)slots.size();
+    dg->tri = upload(recs);
+    dg->tri_vid = upload(tvid);
+    dg->vert_unk = upload(g.vert_index);
+    dg->slot_vid = upload(slots);
+    dg->adj_off = upload(adj_off);
+    dg->adj = upload(adj);
+    dg->device = dev;
+    g.dev = dg;
+    return *g.dev;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int TILE = 16;
+
+__device__ __forceinline__ int64_t cdiv_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Linear tile id -> (lo, hi) with lo <= hi over the upper triangle of tiles.
+__device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) {
+    int64_t h = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (h * (h + 1) / 2 > t) --h;
+    while ((h + 1) * (h + 2) / 2 <= t) ++h;
+    hi = h;
+    lo = t - h * (h + 1) / 2;
+}
+
+// S block: value(t1,t2) = coeff * integral over t2 (Gauss) of OperatorS(t1).
+// TEMP: writes the dense temporary block tmp[i + j*n1] (and its mirror for
+// diagonal blocks) -- the SymBloc/Bloc of operators.h:159-173,337-352.
+template <bool TEMP>
+__global__ void __launch_bounds__(256) k_s_block(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
+                                                 int64_t b0, int64_t n2, int diag, int64_t ntx, int order,
+                                                 double coeff, double* __restrict__ H, Layout L,
+                                                 int64_t ua0, int64_t ub0, double* __restrict__ tmp) {
+    int64_t ti, tj;
+    if (diag) {
+        upper_tile(blockIdx.x, ti, tj);
+    } else {
+        ti = blockIdx.x / ntx;
+        tj = blockIdx.x - ti * ntx;
+    }
+    const int64_t i = ti * TILE + threadIdx.x / TILE;
+    const int64_t j = tj * TILE + threadIdx.x % TILE;
+    if (i >= n1 || j >= n2) return;
+    if (diag && j < i) return;
+    const OpS o = load_ops(tri[a0 + i]);
+    const TriRec& t2 = tri[b0 + j];
+    const d3 P[3] = {ld3(t2.p[0]), ld3(t2.p[1]), ld3(t2.p[2])};
+    const int np = c_nb_pts[order];
+    double res = 0.0;
+    for (int k = 0; k < np; ++k) res += c_rules[order][k][3] * ops_eval(o, gauss_point(order, k, P));
+    const double v = (t2.area * res) * coeff;
+    if (TEMP) {
+        tmp[i + j * n1] = v;
+        if (diag) tmp[j + i * n1] = v;
+    } else {
+        const int64_t r = dev_index(L, ua0 + i), c = dev_index(L, ub0 + j);
+        H[r + c * L.ld] = v;
+        H[c + r * L.ld] = v;
+    }
+}
+
+// D pass 1: W[(k*n2 + j)*n1c + i] = integral over row triangle (a0+i)
+// (Gauss) of P1SLP(column triangle b0+j)_k  (operators.h:103-106).
+__global__ void __launch_bounds__(256) k_d_pairs(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
+                                                 int64_t b0, int64_t n2, int order, double* __restrict__ W) {
+    const int64_t ni = cdiv_dev(n1c, 64);
+    const int64_t tj = blockIdx.x / ni, tb = blockIdx.x - tj * ni;
+    const int64_t i = tb * 64 + threadIdx.x % 64;
+    const int64_t j = tj * 4 + threadIdx.x / 64;
+    if (i >= n1c || j >= n2) return;
+    const P1S o = load_p1s(tri[b0 + j]);
+    const TriRec& t1 = tri[a0 + i];
+    const d3 P[3] = {ld3(t1.p[0]), ld3(t1.p[1]), ld3(t1.p[2])};
+    const int np = c_nb_pts[order];
+    double rx = 0.0, ry = 0.0, rz = 0.0;
+    for (int k = 0; k < np; ++k) {
+        const double w = c_rules[order][k][3];
+        const d3 v = p1s_eval(o, gauss_point(order, k, P));
+        rx += w * v.x;
+        ry += w * v.y;
+        rz += w * v.z;
+    }
+    const double A = t1.area;
+    W[(0 * n2 + j) * n1c + i] = A * rx;
+    W[(1 * n2 + j) * n1c + i] = A * ry;
+    W[(2 * n2 + j) * n1c + i] = A * rz;
+}
+
+// D pass 2: H(row t1, vertex v) += sum over the triangles t2 of v (mesh
+// order) of W_k(t1,t2) * coeff.
+__global__ void __launch_bounds__(256) k_d_gather(const double* __restrict__ W, int64_t n1c, int64_t n2,
+                                                  int64_t b0, const int64_t* __restrict__ adj_off,
+                                                  const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                  const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
+                                                  double coeff, double* __restrict__ H, Layout L, int64_t urow0) {
+    const int64_t ni = cdiv_dev(n1c, 64);
+    const int64_t ts = blockIdx.x / ni, tb = blockIdx.x - ts * ni;
+    const int64_t i = tb * 64 + threadIdx.x % 64;
+    const int64_t s = ts * 4 + threadIdx.x / 64;
+    if (i >= n1c || s >= ns) return;
+    const int64_t r = dev_index(L, urow0 + i);
+    const int64_t c = dev_index(L, vert_unk[slot_vid[s0 + s]]);
+    double acc = H[r + c * L.ld];
+    for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
+        const AdjRec e = adj[p];
+        acc += W[((int64_t)e.k * n2 + (e.tri - b0)) * n1c + i] * coeff;
+    }
+    H[r + c * L.ld] = acc;
+    H[c + r * L.ld] = acc;
+}
+
+// N block (operators.h:134-148 and the loops of :272-296 / :455-480).
+// S values come from the HeadMat itself (src_temp == 0, S block present) or
+// from the dense temporary block tmp (ld = tmp_ld, triangle offsets ta0/tb0).
+__global__ void __launch_bounds__(256) k_n_block(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
+                                                 const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                 const int64_t* __restrict__ vert_unk, int64_t sa0, int64_t na,
+                                                 int64_t sb0, int64_t nb, int same, int64_t ntx, double coeff,
+                                                 const double* __restrict__ tmp, int64_t tmp_ld, int64_t ta0,
+                                                 int64_t tb0, int64_t ua0, int64_t ub0, double* __restrict__ H,
+                                                 Layout L, int use_atomic) {
+    int64_t ta, tb;
+    if (same) {
+        upper_tile(blockIdx.x, tb, ta);   // tb <= ta
+    } else {
+        ta = blockIdx.x / ntx;
+        tb = blockIdx.x - ta * ntx;
+    }
+    const int64_t a = ta * TILE + threadIdx.x / TILE;
+    const int64_t b = tb * TILE + threadIdx.x % TILE;
+    if (a >= na || b >= nb) return;
+    if (same && b > a) return;
+    const int32_t v1 = slot_vid[sa0 + a], v2 = slot_vid[sb0 + b];
+    const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
+    double res = 0.0;
+    for (int64_t p = adj_off[sa0 + a]; p < adj_off[sa0 + a + 1]; ++p) {
+        const AdjRec e1 = adj[p];
+        const d3 cb1 = ld3(e1.cb);
+        const double A1 = tri[e1.tri].area;
+        for (int64_t q = adj_off[sb0 + b]; q < adj_off[sb0 + b + 1]; ++q) {
+            const AdjRec e2 = adj[q];
+            const double A2 = tri[e2.tri].area;
+            double s;
+            if (tmp) {
+                s = tmp[(e1.tri - ta0) + (e2.tri - tb0) * tmp_ld];
+            } else {
+                const int64_t r = dev_index(L, ua0 + (e1.tri - ta0));
+                const int64_t c = dev_index(L, ub0 + (e2.tri - tb0));
+                s = H[r + c * L.ld];
+            }
+            res -= factor * dot(cb1, ld3(e2.cb)) * s / (A1 * A2);
+        }
+    }
+    const double val = res * coeff;
+    const int64_t r = dev_index(L, vert_unk[v1]), c = dev_index(L, vert_unk[v2]);
+    if (use_atomic) {
+        atomicAdd(&H[r + c * L.ld], val);
+        if (r != c) atomicAdd(&H[c + r * L.ld], val);
+    } else {
+        H[r + c * L.ld] += val;
+        if (r != c) H[c + r * L.ld] += val;
+    }
+}
+
+__global__ void k_deflate_coef(const double* __restrict__ H, Layout L, int64_t u, double nbv, double* coef) {
+    const int64_t d = dev_index(L, u);
+    coef[0] = H[d + d * L.ld] / nbv;
+}
+
+// deflate (assembleHeadMat.cpp:40-57): every (v1, v2) vertex pair of the
+// mesh with v2 at or after v1 in mesh order gets += coef.
+__global__ void __launch_bounds__(256) k_deflate(const int32_t* __restrict__ slot_vid, const int64_t* __restrict__ vert_unk,
+                                                 int64_t s0, int64_t ns, const double* __restrict__ coef,
+                                                 double* __restrict__ H, Layout L) {
+    int64_t ta, tb;
+    upper_tile(blockIdx.x, ta, tb);   // ta <= tb
+    const int64_t a = ta * TILE + threadIdx.x / TILE;
+    const int64_t b = tb * TILE + threadIdx.x % TILE;
+    if (a >= ns || b >= ns || b < a) return;
+    const double c0 = coef[0];
+    const int64_t r = dev_index(L, vert_unk[slot_vid[s0 + a]]), c = dev_index(L, vert_unk[slot_vid[s0 + b]]);
+    H[r + c * L.ld] += c0;
+    if (r != c) H[c + r * L.ld] += c0;
+}
+
+int64_t n_upper_tiles(int64_t n) {
+    const int64_t t = cdiv(n, TILE);
+    return t * (t + 1) / 2;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff, double* H, const Layout& L,
+                     double* tmp, hipStream_t st) {
+    const MeshDev& A = dg.meshes[m1];
+    const MeshDev& B = dg.meshes[m2];
+    const int64_t n1 = A.tri_end - A.tri_begin, n2 = B.tri_end - B.tri_begin;
+    const int diag = (m1 == m2);
+    const int64_t ntx = cdiv(n2, TILE);
+    const int64_t nblk = diag ? n_upper_tiles(n1) : cdiv(n1, TILE) * ntx;
+    if (tmp)
+        k_s_block<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
+                                                        coeff, H, L, A.tri_unk0, B.tri_unk0, tmp);
+    else
+        k_s_block<false><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
+                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp);
+    OMH_LAUNCH_CHECK();
+}
+
+static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, double coeff, const double* tmp,
+                     double* H, const Layout& L, hipStream_t st) {
+    const MeshDev& A = dg.meshes[m1];
+    const MeshDev& B = dg.meshes[m2];
+    const int64_t na = A.slot_end - A.slot_begin, nb = B.slot_end - B.slot_begin;
+    const int same = (m1 == m2);
+    int use_atomic = 0;
+    if (!same) {   // vertices shared by the two meshes can alias entries
+        const auto& va = g.meshes[m1].verts;
+        std::vector<int64_t> sa(va.begin(), va.end());
+        std::sort(sa.begin(), sa.end());
+        for (int64_t v : g.meshes[m2].verts)
+            if (std::binary_search(sa.begin(), sa.end(), v)) { use_atomic = 1; break; }
+    }
+    const int64_t ntx = cdiv(nb, TILE);
+    const int64_t nblk = same ? n_upper_tiles(na) : cdiv(na, TILE) * ntx;
+    k_n_block<<<(unsigned)nblk, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
+                                              B.slot_begin, nb, same, ntx, coeff, tmp, A.tri_end - A.tri_begin,
+                                              A.tri_begin, B.tri_begin, A.tri_unk0, B.tri_unk0, H, L, use_atomic);
+    OMH_LAUNCH_CHECK();
+}
+
+// D(rows = triangles of mesh mr, columns = vertices of mesh mc), operators.h:84-115.
+static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff, double* H, const Layout& L,
+                     hipStream_t st) {
+    const MeshDev& R = dg.meshes[mr];
+    const MeshDev& Cm = dg.meshes[mc];
+    const int64_t n1 = R.tri_end - R.tri_begin, n2 = Cm.tri_end - Cm.tri_begin;
+    const int64_t ns = Cm.slot_end - Cm.slot_begin;
+    const size_t budget = size_t(1) << 30;   // 1 GiB of pass-1 scratch
+    int64_t chunk = std::max<int64_t>(64, (int64_t)(budget / (3 * sizeof(double) * (size_t)n2)));
+    chunk = std::min(chunk, n1);
+    double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)chunk);
+    for (int64_t c0 = 0; c0 < n1; c0 += chunk) {
+        const int64_t nc = std::min(chunk, n1 - c0);
+        const int64_t blk1 = cdiv(nc, 64) * cdiv(n2, 4);
+        k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W);
+        OMH_LAUNCH_CHECK();
+        const int64_t blk2 = cdiv(nc, 64) * cdiv(ns, 4);
+        k_d_gather<<<(unsigned)blk2, 256, 0, st>>>(W, nc, n2, Cm.tri_begin, dg.adj_off, dg.adj, dg.slot_vid,
+                                                   dg.vert_unk, Cm.slot_begin, ns, coeff, H, L, R.tri_unk0 + c0);
+        OMH_LAUNCH_CHECK();
+    }
+}
+
+void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    DeviceGeometry& dg = device_geometry(g);
+    const int64_t n = g.size();
+    fill_f64(H, n, n, L.ld, 0.0, st);
+    for (const MeshPair& mp : g.pairs) {
+        const int m1 = mp.m1, m2 = mp.m2;
+        const bool b1 = g.meshes[m1].current_barrier, b2 = g.meshes[m2].current_barrier;
+        const bool same = (m1 == m2);
+        double Scoeff = 0.0;
+        if (same ? !b1 : (!b1 && !b2)) {   // set_S_block
+            launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, st);
+            Scoeff = mp.cS;
+        }
+        if (Scoeff != 0.0) {               // set_N_block
+            launch_n(dg, g, m1, m2, mp.cN / Scoeff, nullptr, H, L, st);
+        } else {
+            const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
+            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+            launch_s(dg, m1, m2, order, 1.0, H, L, tmp, st);
+            launch_n(dg, g, m1, m2, mp.cN, tmp, H, L, st);
+        }
+        if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, st);              // set_D_block
+        if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, st);     // set_Dstar_block
+    }
+    // deflate all current barriers, one constant per isolated part
+    double* coef = (double*)scratch(SCR_SMALL, 64);
+    for (const auto& part : g.parts) {
+        int64_t nbv = 0, i_first = 0;
+        for (int mid : part) {
+            const Mesh& m = g.meshes[mid];
+            if (m.outermost) {
+                nbv += (int64_t)m.verts.size();
+                if (i_first == 0) i_first = g.vert_index[m.verts.front()];
+            }
+        }
+        if (nbv == 0) continue;
+        k_deflate_coef<<<1, 1, 0, st>>>(H, L, i_first, (double)nbv, coef);
+        OMH_LAUNCH_CHECK();
+        for (int mid : part) {
+            if (!g.meshes[mid].outermost) continue;
+            const MeshDev& md = dg.meshes[mid];
+            const int64_t ns = md.slot_end - md.slot_begin;
+            k_deflate<<<(unsigned)n_upper_tiles(ns), 256, 0, st>>>(dg.slot_vid, dg.vert_unk, md.slot_begin, ns, coef,
+                                                                   H, L);
+            OMH_LAUNCH_CHECK();
+        }
+    }
+}
+
+}  // namespace omh

@@ -1,0 +1,39 @@
+// Shared host helpers: error checking and the per-device scratch arena.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace omh {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define OMH_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::omh::HipError(std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define OMH_LAUNCH_CHECK() OMH_HIP(hipGetLastError())
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grow-only device scratch buffer per slot and per device.  Kernels that use
+// it run on the caller's stream; growing synchronises the device first so
+// that no in-flight kernel still uses the old buffer.
+enum ScratchSlot { SCR_ASSEMBLY = 0, SCR_DTMP = 1, SCR_SOURCES = 2, SCR_LINALG = 3, SCR_LINALG2 = 4, SCR_SMALL = 5, SCR_NSLOTS = 6 };
+void* scratch(ScratchSlot slot, size_t bytes);
+void release_scratch();
+
+// Fill kernel launcher (a kernel, not hipMemsetAsync, so that it is safe in
+// captured graphs).
+void fill_f64(double* p, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s);
+
+}  // namespace omh

@@ -1,0 +1,526 @@
+// Dense linear algebra on MFMA for the HeadMat solve and the gain.
+//
+// The reference factors the packed HeadMat with Bunch-Kaufman DSPTRF and
+// inverts/solves with DSPTRI/DSPTRS (symmatrix.h:259-297,
+// symmatrix.cpp:133-152), then forms the gain with sparse x dense and DGEMM
+// products (gain.h:52-89, sparse_matrix.cpp:36-51, matrix.h:333-350).
+//
+// Here the matrix is held in the triangles-first unknown order, where
+// H = L S L^T exists without pivoting (S = diag(+-1): triangle unknowns +1,
+// vertex unknowns -1).  The factorisation is right-looking and blocked by
+// NB = 256:
+//   1. k_diag_factor: one workgroup factors the NB x NB diagonal block
+//      (inner blocks of 32 in LDS), writes the signs and the explicit
+//      inverse of the diagonal L block;
+//   2. panel X = A21 * Linv^T (MFMA GEMM), L21 = X * S1;
+//   3. trailing update A22 -= X * L21^T on the lower triangle (MFMA GEMM).
+// Triangular solves with many right-hand sides use the stored diagonal
+// inverses and MFMA GEMM updates.
+//
+// MFMA: v_mfma_f64_16x16x4f64 (FP64) and v_mfma_f32_16x16x4f32 (FP32).
+// Operand lane layout (one 16x16x4 block): A(i,k) in lane i + 16k,
+// B(k,j) in lane j + 16k, D(i,j) in lane j + 16*(i/4), element i%4.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "linalg.hpp"
+
+namespace omh {
+
+namespace {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+template <> struct Acc<double> {
+    typedef double4_t type;
+    static __device__ __forceinline__ type mfma(double a, double b, type c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Acc<float> {
+    typedef float4_t type;
+    static __device__ __forceinline__ type mfma(float a, float b, type c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) {
+    int64_t h = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (h * (h + 1) / 2 > t) --h;
+    while ((h + 1) * (h + 2) / 2 <= t) ++h;
+    hi = h;
+    lo = t - h * (h + 1) / 2;
+}
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+// C = alpha * op(A) * op(B) + beta * C on a BM x BN tile per workgroup.
+// TA: op(A)(i,kk) = A[kk + i*lda] (A stored k x m); else A[i + kk*lda].
+// TB: op(B)(kk,j) = B[j + kk*ldb] (B stored n x k); else B[kk + j*ldb].
+// LOWER: only tiles meeting the lower triangle (row >= col) run, and only
+// entries with row >= col are written (m == n).
+template <typename T, bool TA, bool TB, bool LOWER>
+__global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
+                                              int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
+                                              T* __restrict__ C, int64_t ldc, int64_t ntm) {
+    __shared__ T As[BK][BM];
+    __shared__ T Bs[BK][BN];
+    int64_t tm, tn;
+    if (LOWER) {
+        upper_tile(blockIdx.x, tn, tm);   // tn <= tm
+    } else {
+        tn = blockIdx.x / ntm;
+        tm = blockIdx.x - tn * ntm;
+    }
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+    typename Acc<T>::type acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = typename Acc<T>::type{0, 0, 0, 0};
+
+    T ra[4], rb[4];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int i, kk;
+            if (TA) { kk = tid & 15; i = (tid >> 4) + 16 * p; }
+            else { i = tid & 63; kk = (tid >> 6) + 4 * p; }
+            const int64_t gi = row0 + i, gk = k0 + kk;
+            T v = 0;
+            if (gi < m && gk < k) v = TA ? A[gk + gi * lda] : A[gi + gk * lda];
+            ra[p] = v;
+            int j, kb;
+            if (TB) { j = tid & 63; kb = (tid >> 6) + 4 * p; }
+            else { kb = tid & 15; j = (tid >> 4) + 16 * p; }
+            const int64_t gj = col0 + j, gkb = k0 + kb;
+            T w = 0;
+            if (gj < n && gkb < k) w = TB ? B[gj + gkb * ldb] : B[gkb + gj * ldb];
+            rb[p] = w;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int i, kk;
+            if (TA) { kk = tid & 15; i = (tid >> 4) + 16 * p; }
+            else { i = tid & 63; kk = (tid >> 6) + 4 * p; }
+            As[kk][i] = ra[p];
+            int j, kb;
+            if (TB) { j = tid & 63; kb = (tid >> 6) + 4 * p; }
+            else { kb = tid & 15; j = (tid >> 4) + 16 * p; }
+            Bs[kb][j] = rb[p];
+        }
+    };
+
+    const int64_t nk = (k + BK - 1) / BK;
+    load(0);
+    store();
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int k4 = 0; k4 < BK / 4; ++k4) {
+            const int kr = k4 * 4 + (lane >> 4);
+            T fa[2], fb[2];
+            fa[0] = As[kr][wm + (lane & 15)];
+            fa[1] = As[kr][wm + 16 + (lane & 15)];
+            fb[0] = Bs[kr][wn + (lane & 15)];
+            fb[1] = Bs[kr][wn + 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = Acc<T>::mfma(fa[a], fb[b], acc[a][b]);
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+            store();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gi = row0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                const int64_t gj = col0 + wn + b * 16 + (lane & 15);
+                if (gi < m && gj < n && (!LOWER || gi >= gj)) {
+                    T v = alpha * acc[a][b][r];
+                    if (beta != T(0)) v += beta * C[gi + gj * ldc];
+                    C[gi + gj * ldc] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// diagonal block: signed Cholesky A11 = L11 S1 L11^T and Linv = L11^-1.
+// One workgroup of 256 threads; the block lives in global memory (L2
+// resident) and is processed in inner panels of IB = 32 columns staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int IB = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag_factor(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv,
+                                                     int64_t ldl, T* __restrict__ sgn, int* __restrict__ info,
+                                                     int64_t k0, T tiny) {
+    __shared__ T D[IB][IB + 1];
+    __shared__ T ds[IB];
+    const int tid = threadIdx.x;
+    for (int jb = 0; jb < nb; jb += IB) {
+        const int w = min(IB, nb - jb);
+        // 1. load the inner diagonal block (lower part)
+        for (int t = tid; t < IB * IB; t += 256) {
+            const int i = t % IB, j = t / IB;
+            D[i][j] = (i < w && j < w && i >= j) ? A[(jb + i) + (int64_t)(jb + j) * lda] : T(0);
+        }
+        __syncthreads();
+        // unblocked signed Cholesky in LDS
+        for (int j = 0; j < w; ++j) {
+            if (tid == 0) {
+                const T d = D[j][j];
+                if (!(fabs((double)d) > (double)tiny)) {   // also catches NaN
+                    if (info[0] == 0) info[0] = (int)(k0 + jb + j + 1);
+                }
+                const T s = d >= T(0) ? T(1) : T(-1);
+                ds[j] = s;
+                D[j][j] = sqrt(fabs(d));
+            }
+            __syncthreads();
+            const T ljj = D[j][j], sj = ds[j];
+            if (tid > j && tid < w) D[tid][j] = D[tid][j] / (ljj * sj);
+            __syncthreads();
+            for (int t = tid; t < IB * IB; t += 256) {
+                const int i = t % IB, l = t / IB;
+                if (l > j && i >= l && i < w) D[i][l] -= D[i][j] * sj * D[l][j];
+            }
+            __syncthreads();
+        }
+        for (int t = tid; t < IB * IB; t += 256) {
+            const int i = t % IB, j = t / IB;
+            if (i < w && j < w && i >= j) A[(jb + i) + (int64_t)(jb + j) * lda] = D[i][j];
+        }
+        if (tid < w) sgn[k0 + jb + tid] = ds[tid];
+        __syncthreads();
+        // 2. panel rows below the inner block: L_p = A_p L_D^-T S_D (row-wise substitution)
+        for (int i = jb + w + tid; i < nb; i += 256) {
+            T row[IB];
+            for (int j = 0; j < w; ++j) row[j] = A[i + (int64_t)(jb + j) * lda];
+            for (int j = 0; j < w; ++j) {
+                T v = row[j];
+                for (int l = 0; l < j; ++l) v -= row[l] * ds[l] * D[j][l];
+                row[j] = v / (D[j][j] * ds[j]);
+            }
+            for (int j = 0; j < w; ++j) A[i + (int64_t)(jb + j) * lda] = row[j];
+        }
+        __syncthreads();
+        // 3. trailing update inside the diagonal block (lower part)
+        const int r0 = jb + w;
+        const int m = nb - r0;
+        for (int64_t t = tid; t < (int64_t)m * m; t += 256) {
+            const int i = r0 + (int)(t % m), l = r0 + (int)(t / m);
+            if (i < l) continue;
+            T v = A[i + (int64_t)l * lda];
+            for (int j = 0; j < w; ++j) v -= A[i + (int64_t)(jb + j) * lda] * ds[j] * A[l + (int64_t)(jb + j) * lda];
+            A[i + (int64_t)l * lda] = v;
+        }
+        __syncthreads();
+    }
+    // Linv = L11^-1 (lower triangular), column j by thread j
+    for (int j = tid; j < nb; j += 256) {
+        for (int i = 0; i < j; ++i) Linv[i + (int64_t)j * ldl] = T(0);
+        Linv[j + (int64_t)j * ldl] = T(1) / A[j + (int64_t)j * lda];
+        for (int i = j + 1; i < nb; ++i) {
+            T v = 0;
+            for (int l = j; l < i; ++l) v += A[i + (int64_t)l * lda] * Linv[l + (int64_t)j * ldl];
+            Linv[i + (int64_t)j * ldl] = -v / A[i + (int64_t)i * lda];
+        }
+    }
+}
+
+// L21 = X * S1 (column scaling by the block's signs) written into A21.
+template <typename T>
+__global__ void k_scale_cols(const T* __restrict__ X, int64_t ldx, T* __restrict__ Aout, int64_t lda, int64_t m,
+                             int nb, const T* __restrict__ s) {
+    const int64_t total = m * nb;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        Aout[i + j * lda] = X[i + j * ldx] * s[j];
+    }
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ src, int64_t lds, T* __restrict__ dst, int64_t ldd, int64_t m,
+                       int64_t n) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        dst[i + j * ldd] = src[i + j * lds];
+    }
+}
+
+template <typename T>
+__global__ void k_scale_rows(T* __restrict__ W, int64_t ldw, int64_t m, int64_t n, const T* __restrict__ s) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        W[i + j * ldw] *= s[i];
+    }
+}
+
+template <typename TS, typename TD>
+__global__ void k_convert(const TS* __restrict__ src, int64_t lds, TD* __restrict__ dst, int64_t ldd, int64_t m,
+                          int64_t n, int lower_only) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        if (lower_only && i < j) continue;
+        dst[i + j * ldd] = (TD)src[i + j * lds];
+    }
+}
+
+__global__ void k_axpy_f32_to_f64(const float* __restrict__ d, int64_t ldd, double* __restrict__ x, int64_t ldx,
+                                  int64_t m, int64_t n) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        x[i + j * ldx] += (double)d[i + j * ldd];
+    }
+}
+
+// partial sums of squares, one per block
+template <typename T>
+__global__ void __launch_bounds__(256) k_sumsq(const T* __restrict__ a, int64_t lda, int64_t m, int64_t n,
+                                               double* __restrict__ part) {
+    const int64_t total = m * n;
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m;
+        const double v = (double)a[i + j * lda];
+        acc += v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// EH(e, :) = sum_k vals(e,k) * Hinv(cols(e,k), :) -- the sparse x dense
+// product of sparse_matrix.cpp:36-51 for a 3-nnz-per-row Head2EEGMat.
+__global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __restrict__ vals, int64_t ne,
+                              const double* __restrict__ Hm, int64_t ldh, int64_t n, double* __restrict__ out,
+                              int64_t ldo) {
+    const int64_t total = ne * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t % ne, j = t / ne;
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k) v += vals[3 * e + k] * Hm[cols[3 * e + k] + j * ldh];
+        out[e + j * ldo] = v;
+    }
+}
+
+__global__ void k_set_int(int* p, int v) { p[0] = v; }
+
+inline unsigned grid_for(int64_t total) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(total, 256), 16384)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host API
+// ---------------------------------------------------------------------------
+template <typename T>
+void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    const int64_t ntm = cdiv(m, BM), ntn = cdiv(n, BN);
+    if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
+    const int64_t nblk = lower ? ntm * (ntm + 1) / 2 : ntm * ntn;
+    if (nblk > 0xffffffffLL) throw std::invalid_argument("gemm grid too large");
+#define OMH_GEMM_CASE(TA_, TB_, LO_)                                                                          \
+    if (ta == TA_ && tb == TB_ && lower == LO_) {                                                             \
+        k_gemm<T, TA_, TB_, LO_><<<(unsigned)nblk, 256, 0, st>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, \
+                                                                 ntm);                                       \
+        OMH_LAUNCH_CHECK();                                                                                   \
+        return;                                                                                               \
+    }
+    OMH_GEMM_CASE(false, false, false)
+    OMH_GEMM_CASE(true, false, false)
+    OMH_GEMM_CASE(false, true, false)
+    OMH_GEMM_CASE(true, true, false)
+    OMH_GEMM_CASE(false, true, true)
+#undef OMH_GEMM_CASE
+    throw std::invalid_argument("unsupported gemm variant");
+}
+
+template void gemm<double>(bool, bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
+                           int64_t, double, double*, int64_t, hipStream_t);
+template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*,
+                          int64_t, float, float*, int64_t, hipStream_t);
+
+size_t factor_workspace_bytes(int64_t n, size_t elem) {
+    const int64_t nblk = cdiv(n, FACT_NB);
+    return elem * ((size_t)n * FACT_NB            // panel X
+                   + (size_t)nblk * FACT_NB * FACT_NB   // diagonal inverses
+                   + (size_t)n)                   // signs
+           + 64;
+}
+
+template <typename T>
+FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
+    T* X = (T*)work;
+    T* Dinv = X + (size_t)n * FACT_NB;
+    const int64_t nblk = cdiv(n, FACT_NB);
+    T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
+    int* info = (int*)scratch(SCR_SMALL, 64);
+    k_set_int<<<1, 1, 0, st>>>(info, 0);
+    OMH_LAUNCH_CHECK();
+    const T tiny = sizeof(T) == 8 ? T(1e-290) : T(1e-30);
+    for (int64_t k = 0; k < n; k += FACT_NB) {
+        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+        T* A11 = A + k + k * lda;
+        T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+        k_diag_factor<T><<<1, 256, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
+        OMH_LAUNCH_CHECK();
+        const int64_t e = k + nb, m = n - e;
+        if (m <= 0) break;
+        T* A21 = A + e + k * lda;
+        // X = A21 * Linv^T
+        gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, m, st);
+        k_scale_cols<T><<<grid_for(m * nb), 256, 0, st>>>(X, m, A21, lda, m, nb, sgn + k);
+        OMH_LAUNCH_CHECK();
+        // A22 -= X * L21^T (lower)
+        gemm<T>(false, true, true, m, m, nb, T(-1), X, m, A21, lda, T(1), A + e + e * lda, lda, st);
+    }
+    FactorInfo fi;
+    int h_info = 0;
+    OMH_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    fi.info = h_info;
+    fi.dinv = Dinv;
+    fi.signs = sgn;
+    return fi;
+}
+
+template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t);
+template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t);
+
+template <typename T>
+void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st) {
+    if (r <= 0) return;
+    const T* Dinv = (const T*)work + (size_t)n * FACT_NB;
+    const int64_t nblk = cdiv(n, FACT_NB);
+    const T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
+    T* tmp = (T*)scratch(SCR_LINALG2, sizeof(T) * FACT_NB * (size_t)r);
+    // forward: W <- L^-1 W
+    for (int64_t k = 0; k < n; k += FACT_NB) {
+        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+        const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+        gemm<T>(false, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
+        k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
+        OMH_LAUNCH_CHECK();
+        const int64_t e = k + nb;
+        if (e < n) gemm<T>(false, false, false, n - e, r, nb, T(-1), L + e + k * lda, lda, W + k, ldw, T(1), W + e, ldw, st);
+    }
+    // W <- S W
+    k_scale_rows<T><<<grid_for(n * r), 256, 0, st>>>(W, ldw, n, r, sgn);
+    OMH_LAUNCH_CHECK();
+    // backward: W <- L^-T W
+    const int64_t last = ((n - 1) / FACT_NB) * FACT_NB;
+    for (int64_t k = last; k >= 0; k -= FACT_NB) {
+        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+        const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+        gemm<T>(true, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
+        k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
+        OMH_LAUNCH_CHECK();
+        if (k > 0) gemm<T>(true, false, false, k, r, nb, T(-1), L + k, lda, W + k, ldw, T(1), W, ldw, st);
+    }
+}
+
+template void sym_solve<double>(const double*, int64_t, int64_t, const void*, double*, int64_t, int64_t, hipStream_t);
+template void sym_solve<float>(const float*, int64_t, int64_t, const void*, float*, int64_t, int64_t, hipStream_t);
+
+void convert_f64_to_f32(const double* src, int64_t lds, float* dst, int64_t ldd, int64_t m, int64_t n, bool lower,
+                        hipStream_t st) {
+    k_convert<double, float><<<grid_for(m * n), 256, 0, st>>>(src, lds, dst, ldd, m, n, lower ? 1 : 0);
+    OMH_LAUNCH_CHECK();
+}
+
+void copy_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t n, hipStream_t st) {
+    k_copy<double><<<grid_for(m * n), 256, 0, st>>>(src, lds, dst, ldd, m, n);
+    OMH_LAUNCH_CHECK();
+}
+
+double frob_norm_f64(const double* a, int64_t lda, int64_t m, int64_t n, hipStream_t st) {
+    const unsigned g = std::min<unsigned>(grid_for(m * n), 1024);
+    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * 1024);
+    k_sumsq<double><<<g, 256, 0, st>>>(a, lda, m, n, part);
+    OMH_LAUNCH_CHECK();
+    std::vector<double> h(g);
+    OMH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : h) s += v;
+    return std::sqrt(s);
+}
+
+double frob_norm_f32(const float* a, int64_t lda, int64_t m, int64_t n, hipStream_t st) {
+    const unsigned g = std::min<unsigned>(grid_for(m * n), 1024);
+    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * 1024);
+    k_sumsq<float><<<g, 256, 0, st>>>(a, lda, m, n, part);
+    OMH_LAUNCH_CHECK();
+    std::vector<double> h(g);
+    OMH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : h) s += v;
+    return std::sqrt(s);
+}
+
+int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
+                 const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
+                 double* final_rel_update, hipStream_t st) {
+    float* R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r);
+    double* R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r);
+    fill_f64(X, n, r, ldx, 0.0, st);
+    copy_f64(B, ldb, R, n, n, r, st);
+    int it = 0;
+    double rel = 1.0;
+    for (; it < max_iter; ++it) {
+        convert_f64_to_f32(R, n, R32, n, n, r, false, st);
+        sym_solve<float>(L32, n, ld32, work32, R32, n, r, st);
+        k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
+        OMH_LAUNCH_CHECK();
+        const double dn = frob_norm_f32(R32, n, n, r, st);
+        const double xn = frob_norm_f64(X, ldx, n, r, st);
+        rel = xn > 0 ? dn / xn : 0.0;
+        if (rel <= tol) { ++it; break; }
+        // R = B - H X
+        copy_f64(B, ldb, R, n, n, r, st);
+        gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, n, st);
+    }
+    if (final_rel_update) *final_rel_update = rel;
+    return it;
+}
+
+void sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* Hm, int64_t ldh, int64_t n,
+                 double* out, int64_t ldo, hipStream_t st) {
+    k_sparse_rows<<<grid_for(ne * n), 256, 0, st>>>(d_cols, d_vals, ne, Hm, ldh, n, out, ldo);
+    OMH_LAUNCH_CHECK();
+}
+
+}  // namespace omh

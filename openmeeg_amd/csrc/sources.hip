@@ -1,0 +1,510 @@
+// Right-hand sides and sensors on the GPU: DipSourceMat (adaptive
+// quadrature), point location by solid angles and the Danielsson
+// closest-point projection of Head2EEGMat.
+//
+// Reference behaviour (OpenMEEG 2.5.16):
+//   SourceMatrix<Dipole>          assembleSourceMat.cpp:66-104
+//   operatorPotential             operators.cpp:41-60
+//   operatorPotentialDerivative   operators.cpp:63-88
+//   PotentialDerivative           analytics.h:132-158
+//   adaptive integration          integrator.h:44-93
+//   Geometry::domain(point)       geometry.cpp:146-154, domain.cpp:5-10,
+//                                 interface.cpp:18-45, mesh.cpp:239-244
+//   Head2EEGMat                   assembleSensors.cpp:25-39
+//   dpc / dist_point_*            danielsson.cpp:26-129
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "bem_device.hpp"
+#include "common.hpp"
+#include "device_geometry.hpp"
+#include "sources.hpp"
+
+namespace omh {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kK = 1.0 / (4.0 * kPi);
+constexpr int MAX_FRAMES = 12;
+
+__device__ __forceinline__ int64_t cdiv_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// The two integrands of the dipole right-hand side.
+struct PotentialFn {            // Dipole::potential (dipole.h:29-35)
+    d3 r0, q;
+    __device__ __forceinline__ void operator()(d3 r, double* out) const { out[0] = dip_potential(r0, q, r); }
+};
+
+struct DerivativeFn {           // PotentialDerivative (analytics.h:140-150)
+    d3 r0, q, n, p0, e10, e20;
+    double d11, d12, d22;
+    __device__ void init(const TriRec& t, d3 pos, d3 mom) {
+        r0 = pos; q = mom;
+        n = ld3(t.n);
+        p0 = ld3(t.p[0]);
+        e10 = sub(ld3(t.p[1]), p0);
+        e20 = sub(ld3(t.p[2]), p0);
+        const double c11 = dot(e10, e10), c12 = dot(e10, e20), c22 = dot(e20, e20);
+        const double inv = 1.0 / (c11 * c22 - c12 * c12);   // BarycentricCoordinates.h:33-51
+        d11 = c22 * inv; d12 = -c12 * inv; d22 = c11 * inv;
+    }
+    __device__ __forceinline__ void operator()(d3 r, double* out) const {
+        const d3 v = sub(r, p0);                            // BarycentricCoordinates.h:53-69
+        const double a = dot(v, e10), b = dot(v, e20);
+        const double l1 = d11 * a + d12 * b;
+        const double l2 = d12 * a + d22 * b;
+        const double l0 = 1.0 - l1 - l2;
+        const double g = dip_gradient_n(r0, q, r, n);
+        out[0] = g * l0; out[1] = g * l1; out[2] = g * l2;
+    }
+};
+
+template <int NC, class F>
+__device__ __forceinline__ void tri_integration(const F& f, int order, d3 A, d3 B, d3 C, double area, double* res) {
+    const int np = c_nb_pts[order];
+    double acc[NC];
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+    for (int i = 0; i < np; ++i) {
+        const double b0 = c_rules[order][i][0], b1 = c_rules[order][i][1], b2 = c_rules[order][i][2];
+        const d3 x = mk(b0 * A.x + b1 * B.x + b2 * C.x, b0 * A.y + b1 * B.y + b2 * C.y, b0 * A.z + b1 * B.z + b2 * C.z);
+        double v[NC];
+        f(x, v);
+        const double w = c_rules[order][i][3];
+        for (int c = 0; c < NC; ++c) acc[c] += w * v[c];
+    }
+    for (int c = 0; c < NC; ++c) res[c] = area * acc[c];
+}
+
+// 4-way split of integrator.h:71-75.
+__device__ __forceinline__ void child_tri(const d3 P[3], int c, d3 out[3]) {
+    const d3 m0 = scl(0.5, add(P[1], P[2])), m1 = scl(0.5, add(P[2], P[0])), m2 = scl(0.5, add(P[0], P[1]));
+    switch (c) {
+        case 0: out[0] = P[0]; out[1] = m1; out[2] = m2; break;
+        case 1: out[0] = m0; out[1] = P[1]; out[2] = m2; break;
+        case 2: out[0] = m0; out[1] = m1; out[2] = P[2]; break;
+        default: out[0] = m0; out[1] = m1; out[2] = m2; break;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ double vnorm(const double* a) {
+    if (NC == 1) return fabs(a[0]);
+    return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+
+template <int NC>
+__device__ __forceinline__ bool converged(const double* coarse, const double* refined, double tol) {
+    double d[NC];
+    for (int c = 0; c < NC; ++c) d[c] = coarse[c] - refined[c];
+    return vnorm<NC>(d) <= tol * vnorm<NC>(coarse);
+}
+
+template <int NC>
+struct Frame {
+    d3 P[3];
+    double area;
+    double integ[4][NC];
+    double sum[NC];
+    int child;
+    int level;
+};
+
+// Integrator::integrate (integrator.h:44-48) with the adaptive recursion of
+// integrator.h:69-93 unrolled onto an explicit stack; the child results are
+// summed in the reference's order.
+template <int NC, class F>
+__device__ void integrate(const F& f, int order, int levels, double tol, const d3 P[3], double area, double* res) {
+    double coarse[NC];
+    tri_integration<NC>(f, order, P[0], P[1], P[2], area, coarse);
+    if (levels <= 0) {
+        for (int c = 0; c < NC; ++c) res[c] = coarse[c];
+        return;
+    }
+    Frame<NC> st[MAX_FRAMES];
+    int top = 0;
+    st[0].P[0] = P[0]; st[0].P[1] = P[1]; st[0].P[2] = P[2];
+    st[0].area = area;
+    double refined[NC];
+    for (int c = 0; c < NC; ++c) refined[c] = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        d3 C[3];
+        child_tri(P, i, C);
+        tri_integration<NC>(f, order, C[0], C[1], C[2], 0.25 * area, st[0].integ[i]);
+        for (int c = 0; c < NC; ++c) refined[c] += st[0].integ[i][c];
+    }
+    if (converged<NC>(coarse, refined, tol)) {
+        for (int c = 0; c < NC; ++c) res[c] = refined[c];
+        return;
+    }
+    st[0].level = levels;
+    st[0].child = 0;
+    for (int c = 0; c < NC; ++c) st[0].sum[c] = 0.0;
+    while (true) {
+        Frame<NC>& fr = st[top];
+        if (fr.child == 4) {
+            if (top == 0) {
+                for (int c = 0; c < NC; ++c) res[c] = fr.sum[c];
+                return;
+            }
+            --top;
+            for (int c = 0; c < NC; ++c) st[top].sum[c] += fr.sum[c];
+            st[top].child++;
+            continue;
+        }
+        // call adaptive(child, area/4, integ[child], level-1)
+        d3 C[3];
+        child_tri(fr.P, fr.child, C);
+        const double ca = 0.25 * fr.area;
+        double integ[4][NC], ref[NC];
+        for (int c = 0; c < NC; ++c) ref[c] = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            d3 G[3];
+            child_tri(C, i, G);
+            tri_integration<NC>(f, order, G[0], G[1], G[2], 0.25 * ca, integ[i]);
+            for (int c = 0; c < NC; ++c) ref[c] += integ[i][c];
+        }
+        const int lvl = fr.level - 1;
+        if (lvl == 0 || converged<NC>(fr.integ[fr.child], ref, tol) || top + 1 >= MAX_FRAMES) {
+            for (int c = 0; c < NC; ++c) fr.sum[c] += ref[c];
+            fr.child++;
+        } else {
+            Frame<NC>& nf = st[top + 1];
+            nf.P[0] = C[0]; nf.P[1] = C[1]; nf.P[2] = C[2];
+            nf.area = ca;
+            for (int i = 0; i < 4; ++i)
+                for (int c = 0; c < NC; ++c) nf.integ[i][c] = integ[i][c];
+            for (int c = 0; c < NC; ++c) nf.sum[c] = 0.0;
+            nf.child = 0;
+            nf.level = lvl;
+            ++top;
+        }
+    }
+}
+
+// One (triangle, dipole) pair of one mesh: the P1 derivative integral into
+// W[(k*nd + s)*nt + t] and, for non-barrier meshes, the P0 potential
+// integral added to out(row(t), s).
+__global__ void __launch_bounds__(256) k_dsm_pairs(const TriRec* __restrict__ tri, int64_t t0, int64_t nt,
+                                                   const double* __restrict__ dip, int64_t s0, int64_t nd,
+                                                   const double* __restrict__ cD, const double* __restrict__ cP,
+                                                   int64_t nm, int mesh, int with_pot, int order, int levels,
+                                                   double tol, double* __restrict__ W, double* __restrict__ out,
+                                                   int64_t ld_out, Layout L, int64_t tri_unk0) {
+    const int64_t nti = cdiv_dev(nt, 64);
+    const int64_t ts = blockIdx.x / nti, tb = blockIdx.x - ts * nti;
+    const int64_t t = tb * 64 + threadIdx.x % 64;
+    const int64_t s = ts * 4 + threadIdx.x / 64;
+    if (t >= nt || s >= nd) return;
+    const int64_t sg = s0 + s;
+    const double coefD = cD[sg * nm + mesh];
+    double v3v[3] = {0.0, 0.0, 0.0};
+    if (coefD != 0.0) {
+        const TriRec& T = tri[t0 + t];
+        const d3 P[3] = {ld3(T.p[0]), ld3(T.p[1]), ld3(T.p[2])};
+        const d3 pos = ld3(dip + 6 * sg), mom = ld3(dip + 6 * sg + 3);
+        DerivativeFn fd;
+        fd.init(T, pos, mom);
+        integrate<3>(fd, order, levels, tol, P, T.area, v3v);
+        if (with_pot) {
+            PotentialFn fp{pos, mom};
+            double pv;
+            integrate<1>(fp, order, levels, tol, P, T.area, &pv);
+            const int64_t r = dev_index(L, tri_unk0 + t);
+            out[r + sg * ld_out] += pv * cP[sg * nm + mesh];
+        }
+    }
+    W[(0 * nd + s) * nt + t] = v3v[0];
+    W[(1 * nd + s) * nt + t] = v3v[1];
+    W[(2 * nd + s) * nt + t] = v3v[2];
+}
+
+// out(row(v), s) += sum over the triangles of v (mesh order) of W * coeffD.
+__global__ void __launch_bounds__(256) k_dsm_gather(const double* __restrict__ W, int64_t t0, int64_t nt,
+                                                    int64_t s0, int64_t nd, const double* __restrict__ cD,
+                                                    int64_t nm, int mesh, const int64_t* __restrict__ adj_off,
+                                                    const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                    const int64_t* __restrict__ vert_unk, int64_t sl0, int64_t nsl,
+                                                    double* __restrict__ out, int64_t ld_out, Layout L) {
+    const int64_t nvi = cdiv_dev(nsl, 64);
+    const int64_t ts = blockIdx.x / nvi, tb = blockIdx.x - ts * nvi;
+    const int64_t v = tb * 64 + threadIdx.x % 64;
+    const int64_t s = ts * 4 + threadIdx.x / 64;
+    if (v >= nsl || s >= nd) return;
+    const int64_t sg = s0 + s;
+    const double coefD = cD[sg * nm + mesh];
+    if (coefD == 0.0) return;
+    const int64_t r = dev_index(L, vert_unk[slot_vid[sl0 + v]]);
+    double acc = out[r + sg * ld_out];
+    for (int64_t p = adj_off[sl0 + v]; p < adj_off[sl0 + v + 1]; ++p) {
+        const AdjRec e = adj[p];
+        acc += W[((int64_t)e.k * nd + s) * nt + (e.tri - t0)] * coefD;
+    }
+    out[r + sg * ld_out] = acc;
+}
+
+// Solid angle of mesh m seen from point s: block reduction over triangles.
+__global__ void __launch_bounds__(256) k_mesh_solid_angle(const TriRec* __restrict__ tri, const int64_t* __restrict__ mt,
+                                                          int nm, const double* __restrict__ pts, int64_t stride,
+                                                          double* __restrict__ out) {
+    const int64_t s = blockIdx.x / nm;
+    const int m = (int)(blockIdx.x - s * nm);
+    const d3 x = ld3(pts + stride * s);
+    double acc = 0.0;
+    for (int64_t t = mt[m] + threadIdx.x; t < mt[m + 1]; t += blockDim.x) {
+        const TriRec& T = tri[t];
+        const d3 r0 = sub(ld3(T.p[0]), x), r1 = sub(ld3(T.p[1]), x), r2 = sub(ld3(T.p[2]), x);
+        const double d0 = norm(r0), d1 = norm(r1), d2 = norm(r2);
+        const double det = 2 * T.area * dot(r0, ld3(T.n));
+        if (fabs(det) >= 1e-10)
+            acc += 2 * atan2(det, d0 * d1 * d2 + (d0 * dot(r1, r2) + d1 * dot(r2, r0) + d2 * dot(r0, r1)));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s * nm + m] = red[0];
+}
+
+// dpc of danielsson.cpp:26-74 (recursive form unrolled: at most 3 levels).
+__device__ double dpc(d3 p, const d3 Q[3], double al[3]) {
+    int idx[3] = {0, 1, 2};
+    int nb = 3;
+    while (true) {
+        if (nb == 1) {
+            al[idx[0]] = 1.0;
+            return norm(sub(p, Q[idx[0]]));
+        }
+        d3 A0Ai[3];
+        for (int i = 1; i < nb; ++i) A0Ai[i] = sub(Q[idx[i]], Q[idx[0]]);
+        const d3 A0M = sub(p, Q[idx[0]]);
+        if (nb == 2) {
+            al[idx[1]] = dot(A0M, A0Ai[1]) / dot(A0Ai[1], A0Ai[1]);
+            al[idx[0]] = 1.0 - al[idx[1]];
+        } else {
+            const double a00 = dot(A0Ai[1], A0Ai[1]), a10 = dot(A0Ai[1], A0Ai[2]), a11 = dot(A0Ai[2], A0Ai[2]);
+            const double b0 = dot(A0M, A0Ai[1]), b1 = dot(A0M, A0Ai[2]);
+            const double d = a00 * a11 - a10 * a10;
+            al[idx[1]] = (b0 * a11 - b1 * a10) / d;
+            al[idx[2]] = (a00 * b1 - a10 * b0) / d;
+            al[idx[0]] = 1.0 - al[idx[1]] - al[idx[2]];
+        }
+        int neg = -1;
+        for (int i = 0; i < nb; ++i)
+            if (al[idx[i]] < 0) { neg = i; break; }
+        if (neg >= 0) {
+            al[idx[neg]] = 0;
+            const int tmp = idx[neg]; idx[neg] = idx[nb - 1]; idx[nb - 1] = tmp;
+            --nb;
+            continue;
+        }
+        d3 MH = scl(-1.0, A0M);
+        for (int i = 1; i < nb; ++i) MH = add(MH, scl(al[idx[i]], A0Ai[i]));
+        return norm(MH);
+    }
+}
+
+// Closest triangle of mesh m to electrode e: the first strict minimum in
+// mesh order (dist_point_interface, danielsson.cpp:85-102).
+__global__ void __launch_bounds__(256) k_closest(const TriRec* __restrict__ tri, const int64_t* __restrict__ mt,
+                                                 int nm, const double* __restrict__ el, double* __restrict__ dist,
+                                                 int64_t* __restrict__ tri_out, double* __restrict__ alpha_out) {
+    const int64_t e = blockIdx.x / nm;
+    const int m = (int)(blockIdx.x - e * nm);
+    const d3 p = ld3(el + 3 * e);
+    double best = DBL_MAX;
+    int64_t bt = INT64_MAX;
+    double ba[3] = {0, 0, 0};
+    for (int64_t t = mt[m] + threadIdx.x; t < mt[m + 1]; t += blockDim.x) {
+        const TriRec& T = tri[t];
+        const d3 Q[3] = {ld3(T.p[0]), ld3(T.p[1]), ld3(T.p[2])};
+        double al[3] = {0, 0, 0};
+        const double d = dpc(p, Q, al);
+        if (d < best) { best = d; bt = t; ba[0] = al[0]; ba[1] = al[1]; ba[2] = al[2]; }
+    }
+    __shared__ double sd[256];
+    __shared__ int64_t stt[256];
+    __shared__ int sid[256];
+    sd[threadIdx.x] = best;
+    stt[threadIdx.x] = bt;
+    sid[threadIdx.x] = threadIdx.x;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const int o = threadIdx.x + w;
+            if (sd[o] < sd[threadIdx.x] || (sd[o] == sd[threadIdx.x] && stt[o] < stt[threadIdx.x])) {
+                sd[threadIdx.x] = sd[o];
+                stt[threadIdx.x] = stt[o];
+                sid[threadIdx.x] = sid[o];
+            }
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x == sid[0]) {
+        dist[e * nm + m] = best;
+        tri_out[e * nm + m] = bt;
+        alpha_out[3 * (e * nm + m) + 0] = ba[0];
+        alpha_out[3 * (e * nm + m) + 1] = ba[1];
+        alpha_out[3 * (e * nm + m) + 2] = ba[2];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static std::vector<double> mesh_solid_angles(Geometry& g, const double* d_pts, int64_t stride, int64_t npts,
+                                             hipStream_t st) {
+    DeviceGeometry& dg = device_geometry(g);
+    const int nm = (int)g.meshes.size();
+    std::vector<int64_t> mt(nm + 1);
+    for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
+    mt[nm] = dg.n_tris;
+    int64_t* d_mt = (int64_t*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + sizeof(double) * npts * nm);
+    double* d_sa = (double*)(d_mt + nm + 1);
+    OMH_HIP(hipMemcpyAsync(d_mt, mt.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    k_mesh_solid_angle<<<(unsigned)(npts * nm), 256, 0, st>>>(dg.tri, d_mt, nm, d_pts, stride, d_sa);
+    OMH_LAUNCH_CHECK();
+    std::vector<double> sa((size_t)npts * nm);
+    OMH_HIP(hipMemcpyAsync(sa.data(), d_sa, sizeof(double) * sa.size(), hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    return sa;
+}
+
+std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride, int64_t npts, hipStream_t st) {
+    const int nm = (int)g.meshes.size();
+    const std::vector<double> sa = mesh_solid_angles(g, d_pts, stride, npts, st);
+    std::vector<int> dom(npts, -1);
+    for (int64_t s = 0; s < npts; ++s) {
+        for (size_t k = 0; k < g.domains.size() && dom[s] < 0; ++k) {
+            bool inside = true;
+            for (const auto& b : g.domains[k].boundaries) {
+                double a = 0.0;
+                for (const auto& om : b.first.omeshes) a += om.second * sa[s * nm + om.first];
+                inside = inside && (g.interface_contains_sa(a) == b.second);
+            }
+            if (inside) dom[s] = (int)k;
+        }
+        if (dom[s] < 0) throw std::runtime_error("a dipole lies in no domain");
+    }
+    return dom;
+}
+
+void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    if (levels < 0 || levels >= MAX_FRAMES) throw std::invalid_argument("adaptive levels must be 0..11");
+    DeviceGeometry& dg = device_geometry(g);
+    const int64_t n = g.size();
+    const int nm = (int)g.meshes.size();
+    fill_f64(d_out, n, n_dip, ld_out, 0.0, st);
+    if (n_dip == 0) return;
+    // per (dipole, mesh) coefficients of assembleSourceMat.cpp:84-99
+    std::vector<double> cD((size_t)n_dip * nm, 0.0), cP((size_t)n_dip * nm, 0.0);
+    std::vector<char> used(nm, 0);
+    for (int64_t s = 0; s < n_dip; ++s) {
+        const Domain& d = g.domains.at(domain[s]);
+        if (d.cond == 0.0) continue;
+        for (const auto& b : d.boundaries) {
+            const double fD = b.second ? -kK : kK;
+            for (const auto& om : b.first.omeshes) {
+                const double c = fD * om.second;
+                cD[s * nm + om.first] += c;
+                if (!g.meshes[om.first].current_barrier) cP[s * nm + om.first] += c / d.cond;
+                used[om.first] = 1;
+            }
+        }
+    }
+    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
+    OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));   // the host vectors die with this frame
+    for (int m = 0; m < nm; ++m) {
+        if (!used[m]) continue;
+        const MeshDev& md = dg.meshes[m];
+        const int64_t nt = md.tri_end - md.tri_begin, nsl = md.slot_end - md.slot_begin;
+        const size_t budget = size_t(1) << 29;
+        int64_t chunk = std::max<int64_t>(4, (int64_t)(budget / (3 * sizeof(double) * (size_t)nt)));
+        chunk = std::min(chunk, n_dip);
+        double* W = (double*)scratch(SCR_SOURCES, 3 * sizeof(double) * (size_t)nt * (size_t)chunk);
+        for (int64_t s0 = 0; s0 < n_dip; s0 += chunk) {
+            const int64_t nd = std::min(chunk, n_dip - s0);
+            const int64_t b1 = cdiv(nt, 64) * cdiv(nd, 4);
+            k_dsm_pairs<<<(unsigned)b1, 256, 0, st>>>(dg.tri, md.tri_begin, nt, d_dip, s0, nd, d_c, d_c + cD.size(), nm,
+                                                      m, md.barrier ? 0 : 1, order, levels, tol, W, d_out, ld_out, L,
+                                                      md.tri_unk0);
+            OMH_LAUNCH_CHECK();
+            const int64_t b2 = cdiv(nsl, 64) * cdiv(nd, 4);
+            k_dsm_gather<<<(unsigned)b2, 256, 0, st>>>(W, md.tri_begin, nt, s0, nd, d_c, nm, m, dg.adj_off, dg.adj,
+                                                       dg.slot_vid, dg.vert_unk, md.slot_begin, nsl, d_out, ld_out, L);
+            OMH_LAUNCH_CHECK();
+        }
+    }
+}
+
+void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, double* vals, hipStream_t st) {
+    DeviceGeometry& dg = device_geometry(g);
+    const int nm = (int)g.meshes.size();
+    std::vector<int64_t> mt(nm + 1);
+    for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
+    mt[nm] = dg.n_tris;
+    const size_t cnt = (size_t)n_el * nm;
+    char* buf = (char*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + cnt * (sizeof(double) * 4 + sizeof(int64_t)));
+    int64_t* d_mt = (int64_t*)buf;
+    double* d_dist = (double*)(d_mt + nm + 1);
+    int64_t* d_tri = (int64_t*)(d_dist + cnt);
+    double* d_al = (double*)(d_tri + cnt);
+    OMH_HIP(hipMemcpyAsync(d_mt, mt.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    if (n_el > 0) {
+        k_closest<<<(unsigned)(n_el * nm), 256, 0, st>>>(dg.tri, d_mt, nm, d_el, d_dist, d_tri, d_al);
+        OMH_LAUNCH_CHECK();
+    }
+    std::vector<double> dist(cnt), al(3 * cnt);
+    std::vector<int64_t> tri(cnt);
+    OMH_HIP(hipMemcpyAsync(dist.data(), d_dist, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipMemcpyAsync(tri.data(), d_tri, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipMemcpyAsync(al.data(), d_al, sizeof(double) * 3 * cnt, hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    // global triangle id -> vertex ids
+    std::vector<std::array<i64, 3>> tv;
+    for (const auto& m : g.meshes) tv.insert(tv.end(), m.tris.begin(), m.tris.end());
+    // dist_point_geom (danielsson.cpp:107-129): interfaces of zero-conductivity
+    // domains; `alphas` holds the result of the last interface scanned.
+    for (int64_t e = 0; e < n_el; ++e) {
+        double distmin = DBL_MAX;
+        int64_t best_tri = -1;
+        double alphas[3] = {0, 0, 0};
+        for (const auto& d : g.domains) {
+            if (d.cond != 0.0) continue;
+            for (const auto& b : d.boundaries) {
+                double dm = DBL_MAX;
+                int64_t bt = -1;
+                const double* ba = nullptr;
+                for (const auto& om : b.first.omeshes) {
+                    const size_t k = (size_t)e * nm + om.first;
+                    if (dist[k] < dm) { dm = dist[k]; bt = tri[k]; ba = &al[3 * k]; }
+                }
+                if (ba) { alphas[0] = ba[0]; alphas[1] = ba[1]; alphas[2] = ba[2]; }
+                if (dm < distmin) { distmin = dm; best_tri = bt; }
+            }
+        }
+        if (best_tri < 0) throw std::runtime_error("Head2EEGMat: the geometry has no zero-conductivity domain");
+        for (int k = 0; k < 3; ++k) {
+            cols[3 * e + k] = g.vert_index[tv[best_tri][k]];
+            vals[3 * e + k] = alphas[k];
+        }
+    }
+}
+
+}  // namespace omh

@@ -1,0 +1,30 @@
+// Dipole right-hand side, point location and EEG sensor projection (device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "device_geometry.hpp"
+#include "geometry.hpp"
+
+namespace omh {
+
+// Geometry::domain(point) for npts device points (stride doubles apart):
+// domain index per point (geometry.cpp:146-154).
+std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride, int64_t npts, hipStream_t st);
+
+// SourceMatrix<Dipole> (assembleSourceMat.cpp:66-104): d_dip is n_dip x 6
+// row-major on the device (x y z qx qy qz); domain[s] the domain of dipole s.
+// Writes the n x n_dip matrix d_out (column-major, leading dimension ld_out)
+// with rows in the unknown order of L.
+void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st);
+
+// Head2EEGMat (assembleSensors.cpp:25-39): for each electrode (d_el, n_el x 3
+// row-major on the device) the three vertex unknowns (reference order) of the
+// closest triangle and their barycentric weights (host arrays n_el x 3).
+void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, double* vals, hipStream_t st);
+
+}  // namespace omh

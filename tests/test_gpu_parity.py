@@ -1,0 +1,192 @@
+"""Parity of the HIP path (through the C ABI) with the reference's goldens
+and the CPU oracle.  Needs a HIP GPU.
+
+Tolerances (relative Frobenius unless noted):
+  * HeadMat / DipSourceMat entries vs goldens or oracle: 1e-13 (FP64, device
+    math library vs glibc: a few ulp per entry);
+  * Head2EEGMat: exact indices, weights to 1e-14;
+  * inverse and gains (FP64 path): 1e-10, the bar the reference's gain must
+    meet;
+  * mixed-precision gain vs the FP64 GPU gain: 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def om():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openmeeg_amd
+    return openmeeg_amd
+
+
+@pytest.fixture(scope="module")
+def head1(om):
+    g = om.read_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    from oracle import oracle as O
+    dip = O.read_dipoles(os.path.join(GOLDEN, "head1", "Head1.dip"))
+    sens = om.Sensors()
+    sens.load(os.path.join(GOLDEN, "head1", "Head1.patches"))
+    return g, dip, sens
+
+
+def test_mfma_lane_layout(om):
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    out = torch.zeros(512, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().omh_probe_mfma_layout(C.c_void_p(out.data_ptr()), C.c_void_p(0)))
+    torch.cuda.synchronize()
+    i = np.arange(16)
+    A = 4.0 * i[:, None] + np.arange(4)[None, :] + 1.0
+    B = (np.arange(4)[:, None] + 1.0) * (i[None, :] % 5 + 1.0)
+    ref = A @ B
+    o = out.cpu().numpy()
+    np.testing.assert_array_equal(o[:256].reshape(16, 16).T, ref)
+    np.testing.assert_array_equal(o[256:].reshape(16, 16).T, ref)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_gemm_matches_numpy(om, dtype, ta, tb):
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    rng = np.random.default_rng(10 + 2 * ta + tb)
+    m, n, k = 131, 77, 203
+    npdt = np.float64 if dtype == "f64" else np.float32
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    A = rng.standard_normal((k, m) if ta else (m, k)).astype(npdt)
+    B = rng.standard_normal((n, k) if tb else (k, n)).astype(npdt)
+    Cm = rng.standard_normal((m, n)).astype(npdt)
+    dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    dB = torch.from_numpy(np.asfortranarray(B).ravel(order="F").copy()).cuda()
+    dC = torch.from_numpy(np.asfortranarray(Cm).ravel(order="F").copy()).cuda()
+    fn = _lib.lib().omh_gemm_f64 if dtype == "f64" else _lib.lib().omh_gemm_f32
+    _lib.check(fn(ta, tb, m, n, k, 1.5, C.c_void_p(dA.data_ptr()), A.shape[0], C.c_void_p(dB.data_ptr()), B.shape[0],
+                  -0.5, C.c_void_p(dC.data_ptr()), m, C.c_void_p(0)))
+    torch.cuda.synchronize()
+    ref = 1.5 * (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64) - 0.5 * Cm
+    got = dC.cpu().numpy().reshape(n, m).T
+    assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
+
+
+def test_headmat_head1_matches_golden(om, head1):
+    g, _, _ = head1
+    hm = om.HeadMat(g)
+    assert hm.nlin() == hm.ncol() == g.size == 286
+    assert rel_fro(hm.data(), np.load(os.path.join(GOLDEN, "head1", "hm.npy"))) < 1e-13
+
+
+def test_dsm_head1_matches_golden(om, head1):
+    g, dip, _ = head1
+    ref = np.load(os.path.join(GOLDEN, "head1", "dsm.npy"))
+    for name in ("Brain", ""):
+        dsm = om.DipSourceMat(g, dip, name)
+        assert (dsm.nlin(), dsm.ncol()) == (286, 6)
+        assert rel_fro(dsm.array(), ref) < 1e-13
+
+
+def test_h2em_head1_matches_golden(om, head1):
+    import scipy.sparse as sp
+    g, _, sens = head1
+    h2em = om.Head2EEGMat(g, sens)
+    d, ir, jc = (np.load(os.path.join(GOLDEN, "head1", "h2em_%s.npy" % k)) for k in ("data", "ir", "jc"))
+    ref = sp.csc_matrix((d, ir, jc), shape=(32, 286)).toarray()
+    got = h2em.array()
+    assert np.array_equal(got != 0, ref != 0)
+    assert np.abs(got - ref).max() < 1e-14
+
+
+def test_inverse_and_gain_head1_match_golden(om, head1):
+    g, dip, sens = head1
+    hm = om.HeadMat(g)
+    hminv = hm.inverse()
+    ref_inv = np.load(os.path.join(GOLDEN, "head1", "hm_inv.npy"))
+    assert rel_fro(hminv.data(), ref_inv) < 1e-12
+    dsm = om.DipSourceMat(g, dip, "Brain")
+    h2em = om.Head2EEGMat(g, sens)
+    ref = np.load(os.path.join(GOLDEN, "head1", "dgem.npy"))
+    assert rel_fro(om.GainEEG(hminv, dsm, h2em).array(), ref) < 1e-11
+    assert rel_fro(om.GainEEGadjoint(g, dip, hm, h2em).array(), ref) < 1e-11
+    # SymMatrix.invert in place equals inverse (test_python.py:126-130)
+    hm2 = om.HeadMat(g)
+    hm2.invert()
+    assert rel_fro(hm2.data(), hminv.data()) < 1e-14
+
+
+def test_solvelin_head1(om, head1):
+    g, dip, _ = head1
+    hm = om.HeadMat(g)
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((g.size, 7))
+    X = hm.solveLin(om.Matrix(np.asfortranarray(B))).array()
+    H = hm.array()
+    assert rel_fro(H @ X, B) < 1e-12
+
+
+def test_headmat_c0_matches_oracle(om, c0_oracle):
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    hm = om.HeadMat(g)
+    from oracle import oracle as O
+    assert rel_fro(hm.data(), O.pack(c["H"])) < 1e-13
+
+
+def test_dsm_h2em_c0_match_oracle(om, c0_oracle):
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    assert rel_fro(om.DipSourceMat(g, c["dip"], "Brain").array(), c["D"]) < 1e-13
+    h2em = om.Head2EEGMat(g, c["el"])
+    assert np.array_equal(h2em.cols, c["cols"])
+    assert np.abs(h2em.vals - c["vals"]).max() < 1e-14
+
+
+def test_gain_c0_fp64_and_mixed(om, c0_oracle):
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    G64, info64 = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="fp64", domain_name="Brain")
+    G64 = om.gain_to_numpy(G64, len(c["el"]), len(c["dip"]))
+    assert rel_fro(G64, c["G"]) < 1e-10
+    Gm, infom = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="mixed", domain_name="Brain")
+    Gm = om.gain_to_numpy(Gm, len(c["el"]), len(c["dip"]))
+    assert rel_fro(Gm, G64) < 1e-6
+    assert 1 <= infom["refine_iterations"] <= 10
+
+
+def test_gain_c0_vs_analytic_sphere(om, c0_oracle):
+    """Independent of the reference: the BEM gain on the 642-vertex spheres
+    against Zhang's series (RDM/MAG, tests/analytic tolerances for Head3:
+    0.03)."""
+    from oracle import analytic as A
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    G, _ = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="fp64", domain_name="Brain")
+    G = om.gain_to_numpy(G, len(c["el"]), len(c["dip"]))
+    ref = A.sphere_gain(c["el"], c["dip"], c["cond"], (0.87, 0.92, 1.0))
+    r = np.linalg.norm(c["dip"][:, :3], axis=1)
+    shallow = np.where(r < 0.7)[0]
+    assert np.median([A.rdm(G[:, j], ref[:, j]) for j in shallow]) < 0.03
+    assert np.median([A.mag(G[:, j], ref[:, j]) for j in shallow]) < 0.03
+
+
+def test_factor_breakdown_is_reported(om):
+    """A vanishing pivot is reported (forced from the host: a zero matrix)."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    n = 300
+    A = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    work = torch.empty(int(_lib.lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
+    info = C.c_int32()
+    _lib.check(_lib.lib().omh_sym_factor_f64(C.c_void_p(A.data_ptr()), n, n, C.c_void_p(work.data_ptr()),
+                                             C.byref(info), C.c_void_p(0)))
+    assert info.value == 1
