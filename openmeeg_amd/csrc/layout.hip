@@ -57,8 +57,9 @@ __global__ void k_permute_rows(const double* __restrict__ src, int64_t lds, doub
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-// One wave: D = A B with A(i,k) = 4i + k + 1, B(k,j) = (k+1)(j%5+1), written
-// back assuming D(i,j) in lane j + 16*(i/4), element i%4.
+// One wave: D = A B with A(i,k) = 4i + k + 1, B(k,j) = (k+1)(j%5+1); the raw
+// accumulator element r of lane l goes to out[4l + r] (FP64) and
+// out[256 + 4l + r] (FP32), so that the host can read off the layout.
 __global__ void k_probe(double* out) {
     const int l = threadIdx.x;
     const int i = l % 16, k = l / 16, j = l % 16;
@@ -69,9 +70,8 @@ __global__ void k_probe(double* out) {
     float4_t c32 = {0, 0, 0, 0};
     c32 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, c32, 0, 0, 0);
     for (int r = 0; r < 4; ++r) {
-        const int row = 4 * (l / 16) + r, col = l % 16;
-        out[row + 16 * col] = c64[r];
-        out[256 + row + 16 * col] = (double)c32[r];
+        out[4 * l + r] = c64[r];
+        out[256 + 4 * l + r] = (double)c32[r];
     }
 }
 

@@ -19,7 +19,8 @@
 //
 // MFMA: v_mfma_f64_16x16x4f64 (FP64) and v_mfma_f32_16x16x4f32 (FP32).
 // Operand lane layout (one 16x16x4 block): A(i,k) in lane i + 16k,
-// B(k,j) in lane j + 16k, D(i,j) in lane j + 16*(i/4), element i%4.
+// B(k,j) in lane j + 16k; D(i,j) in lane j + 16*(i%4), element i/4 for FP64
+// and in lane j + 16*(i/4), element i%4 for FP32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,17 +40,21 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Acc;
+// row(lane, r): row inside the 16x16 block of accumulator element r of a lane
+// (the column is lane % 16).  Measured on gfx950 by omh_probe_mfma_layout.
 template <> struct Acc<double> {
     typedef double4_t type;
     static __device__ __forceinline__ type mfma(double a, double b, type c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct Acc<float> {
     typedef float4_t type;
     static __device__ __forceinline__ type mfma(float a, float b, type c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 __device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) {
@@ -154,7 +159,7 @@ __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gi = row0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                const int64_t gi = row0 + wm + a * 16 + Acc<T>::row(lane, r);
                 const int64_t gj = col0 + wn + b * 16 + (lane & 15);
                 if (gi < m && gj < n && (!LOWER || gi >= gj)) {
                     T v = alpha * acc[a][b][r];

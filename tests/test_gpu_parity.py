@@ -39,6 +39,9 @@ def head1(om):
 
 
 def test_mfma_lane_layout(om):
+    """D(i,j) of one 16x16x4 block sits in lane j + 16*(i % 4), element
+    i // 4, for the FP64 instruction and in lane j + 16*(i // 4), element
+    i % 4, for the FP32 one (the layouts the GEMM epilogue uses)."""
     import ctypes as C
     import torch
     from openmeeg_amd import _lib
@@ -50,8 +53,14 @@ def test_mfma_lane_layout(om):
     B = (np.arange(4)[:, None] + 1.0) * (i[None, :] % 5 + 1.0)
     ref = A @ B
     o = out.cpu().numpy()
-    np.testing.assert_array_equal(o[:256].reshape(16, 16).T, ref)
-    np.testing.assert_array_equal(o[256:].reshape(16, 16).T, ref)
+    rows = {"f64": lambda lane, r: (lane >> 4) + 4 * r, "f32": lambda lane, r: 4 * (lane >> 4) + r}
+    for part, kind in ((o[:256], "f64"), (o[256:], "f32")):
+        raw = part.reshape(64, 4)
+        got = np.zeros((16, 16))
+        for lane in range(64):
+            for r in range(4):
+                got[rows[kind](lane, r), lane & 15] = raw[lane, r]
+        np.testing.assert_array_equal(got, ref, err_msg=kind)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
