@@ -408,7 +408,7 @@ class Factor:
         self.iterations = 0
         self.rel_update = 0.0
 
-    def solve(self, B, nrhs, max_iter=10, tol=1e-14):
+    def solve(self, B, nrhs, max_iter=10, tol=1e-12):
         """B <- H^-1 B (float64 device buffer, n x nrhs column-major)."""
         if self.precision == "fp64":
             check(lib().omh_sym_solve_f64(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n, nrhs,
@@ -604,7 +604,7 @@ class Sensors:
 # Device pipeline (the path the benchmark times)
 # ---------------------------------------------------------------------------
 def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="", integrator=None,
-                      dsm_integrator=None, max_refine=10, refine_tol=1e-14, sync_stages=False):
+                      dsm_integrator=None, max_refine=10, refine_tol=1e-12, sync_stages=False):
     """HeadMat, DipSourceMat and Head2EEGMat assembly, the HeadMat factor and
     the adjoint gain G = (H^-1 E^T)^T DSM, all on the device.  Returns
     (G as a device tensor n_e x n_dip in column-major storage, info dict)."""

@@ -171,86 +171,134 @@ __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T
 
 // ---------------------------------------------------------------------------
 // diagonal block: signed Cholesky A11 = L11 S1 L11^T and Linv = L11^-1.
-// One workgroup of 256 threads; the block lives in global memory (L2
-// resident) and is processed in inner panels of IB = 32 columns staged in LDS.
+// One workgroup of 512 threads; the NB x NB block lives in global memory
+// (L2 resident).  Inner panels of IB = 32 columns: the 32 x 32 diagonal
+// piece is factored in LDS, the rows below it are solved row-wise and kept
+// in LDS (transposed, conflict-free), and the rest of the block is updated
+// from LDS.  Linv is then built by 32-blocks: diagonal inverses first, then
+// Linv(r,c) = -Linv(r,r) * sum_{c<=q<r} L(r,q) Linv(q,c).
 // ---------------------------------------------------------------------------
 constexpr int IB = 32;
+constexpr int DIAG_THREADS = 512;
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_diag_factor(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv,
-                                                     int64_t ldl, T* __restrict__ sgn, int* __restrict__ info,
-                                                     int64_t k0, T tiny) {
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
+                                                              T* __restrict__ Linv, int64_t ldl, T* __restrict__ sgn,
+                                                              int* __restrict__ info, int64_t k0, T tiny) {
     __shared__ T D[IB][IB + 1];
     __shared__ T ds[IB];
+    __shared__ T Pt[IB][FACT_NB - IB + 1];   // panel rows below the inner block, transposed
     const int tid = threadIdx.x;
     for (int jb = 0; jb < nb; jb += IB) {
         const int w = min(IB, nb - jb);
-        // 1. load the inner diagonal block (lower part)
-        for (int t = tid; t < IB * IB; t += 256) {
+        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
             const int i = t % IB, j = t / IB;
             D[i][j] = (i < w && j < w && i >= j) ? A[(jb + i) + (int64_t)(jb + j) * lda] : T(0);
         }
         __syncthreads();
-        // unblocked signed Cholesky in LDS
         for (int j = 0; j < w; ++j) {
             if (tid == 0) {
                 const T d = D[j][j];
-                if (!(fabs((double)d) > (double)tiny)) {   // also catches NaN
-                    if (info[0] == 0) info[0] = (int)(k0 + jb + j + 1);
-                }
-                const T s = d >= T(0) ? T(1) : T(-1);
-                ds[j] = s;
+                if (!(fabs((double)d) > (double)tiny) && info[0] == 0) info[0] = (int)(k0 + jb + j + 1);
+                ds[j] = d >= T(0) ? T(1) : T(-1);
                 D[j][j] = sqrt(fabs(d));
             }
             __syncthreads();
             const T ljj = D[j][j], sj = ds[j];
             if (tid > j && tid < w) D[tid][j] = D[tid][j] / (ljj * sj);
             __syncthreads();
-            for (int t = tid; t < IB * IB; t += 256) {
+            for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
                 const int i = t % IB, l = t / IB;
                 if (l > j && i >= l && i < w) D[i][l] -= D[i][j] * sj * D[l][j];
             }
             __syncthreads();
         }
-        for (int t = tid; t < IB * IB; t += 256) {
+        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
             const int i = t % IB, j = t / IB;
             if (i < w && j < w && i >= j) A[(jb + i) + (int64_t)(jb + j) * lda] = D[i][j];
         }
         if (tid < w) sgn[k0 + jb + tid] = ds[tid];
-        __syncthreads();
-        // 2. panel rows below the inner block: L_p = A_p L_D^-T S_D (row-wise substitution)
-        for (int i = jb + w + tid; i < nb; i += 256) {
-            T row[IB];
-            for (int j = 0; j < w; ++j) row[j] = A[i + (int64_t)(jb + j) * lda];
-            for (int j = 0; j < w; ++j) {
-                T v = row[j];
-                for (int l = 0; l < j; ++l) v -= row[l] * ds[l] * D[j][l];
-                row[j] = v / (D[j][j] * ds[j]);
-            }
-            for (int j = 0; j < w; ++j) A[i + (int64_t)(jb + j) * lda] = row[j];
-        }
-        __syncthreads();
-        // 3. trailing update inside the diagonal block (lower part)
         const int r0 = jb + w;
         const int m = nb - r0;
-        for (int64_t t = tid; t < (int64_t)m * m; t += 256) {
-            const int i = r0 + (int)(t % m), l = r0 + (int)(t / m);
+        // panel rows: L_p = A_p L_D^-T S_D by row-wise substitution
+        for (int t = tid; t < m; t += DIAG_THREADS) {
+            const int i = r0 + t;
+            T row[IB];
+#pragma unroll
+            for (int j = 0; j < IB; ++j) row[j] = j < w ? A[i + (int64_t)(jb + j) * lda] : T(0);
+#pragma unroll
+            for (int j = 0; j < IB; ++j) {
+                if (j < w) {
+                    T v = row[j];
+                    for (int l = 0; l < j; ++l) v -= row[l] * ds[l] * D[j][l];
+                    row[j] = v / (D[j][j] * ds[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < IB; ++j)
+                if (j < w) {
+                    A[i + (int64_t)(jb + j) * lda] = row[j];
+                    Pt[j][t] = row[j] * ds[j];
+                }
+        }
+        __syncthreads();
+        // trailing update of the rest of the block (lower part) from LDS:
+        // A(i,l) -= sum_j L(i,j) s_j L(l,j); Pt holds L*s, so use
+        // L(l,j) = Pt[j][l] * s_j.
+        for (int64_t t = tid; t < (int64_t)m * m; t += DIAG_THREADS) {
+            const int i = (int)(t % m), l = (int)(t / m);
             if (i < l) continue;
-            T v = A[i + (int64_t)l * lda];
-            for (int j = 0; j < w; ++j) v -= A[i + (int64_t)(jb + j) * lda] * ds[j] * A[l + (int64_t)(jb + j) * lda];
-            A[i + (int64_t)l * lda] = v;
+            T v = A[(r0 + i) + (int64_t)(r0 + l) * lda];
+            for (int j = 0; j < w; ++j) v -= Pt[j][i] * (Pt[j][l] * ds[j]);
+            A[(r0 + i) + (int64_t)(r0 + l) * lda] = v;
         }
         __syncthreads();
     }
-    // Linv = L11^-1 (lower triangular), column j by thread j
-    for (int j = tid; j < nb; j += 256) {
-        for (int i = 0; i < j; ++i) Linv[i + (int64_t)j * ldl] = T(0);
+    // Linv: inverses of the diagonal 32-blocks (one thread per column)
+    const int nblk = (nb + IB - 1) / IB;
+    for (int t = tid; t < nblk * IB; t += DIAG_THREADS) {
+        const int b = t / IB, jj = t % IB;
+        const int o = b * IB;
+        const int w = min(IB, nb - o);
+        if (jj >= w) continue;
+        const int j = o + jj;
+        for (int i = 0; i < nb; ++i) Linv[i + (int64_t)j * ldl] = T(0);
         Linv[j + (int64_t)j * ldl] = T(1) / A[j + (int64_t)j * lda];
-        for (int i = j + 1; i < nb; ++i) {
+        for (int i = j + 1; i < o + w; ++i) {
             T v = 0;
             for (int l = j; l < i; ++l) v += A[i + (int64_t)l * lda] * Linv[l + (int64_t)j * ldl];
             Linv[i + (int64_t)j * ldl] = -v / A[i + (int64_t)i * lda];
         }
+    }
+    __syncthreads();
+    // off-diagonal 32-blocks, row block by row block
+    for (int r = 1; r < nblk; ++r) {
+        const int ro = r * IB, rw = min(IB, nb - ro);
+        // T(rr, cc) = sum_{q=c}^{r-1} L(r,q) Linv(q,c) for all c < r, written
+        // into Linv(r, c) as a temporary
+        for (int t = tid; t < r * IB * IB; t += DIAG_THREADS) {
+            const int c = t / (IB * IB), e = t % (IB * IB);
+            const int ii = e % IB, cc = e / IB;
+            if (ii >= rw) continue;
+            const int row = ro + ii, col = c * IB + cc;
+            T v = 0;
+            for (int l = col; l < ro; ++l) v += A[row + (int64_t)l * lda] * Linv[l + (int64_t)col * ldl];
+            Linv[row + (int64_t)col * ldl] = v;
+        }
+        __syncthreads();
+        // Linv(r, c) = -Linv(r,r) * T: each thread owns one column of one
+        // block and walks it bottom-up so the column is overwritten in place
+        for (int t = tid; t < r * IB; t += DIAG_THREADS) {
+            const int c = t / IB, cc = t % IB;
+            const int col = c * IB + cc;
+            for (int ii = rw - 1; ii >= 0; --ii) {
+                T v = 0;
+                for (int kk = 0; kk <= ii; ++kk)
+                    v += Linv[(ro + ii) + (int64_t)(ro + kk) * ldl] * Linv[(ro + kk) + (int64_t)col * ldl];
+                Linv[(ro + ii) + (int64_t)col * ldl] = -v;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -399,7 +447,7 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
         const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
         T* A11 = A + k + k * lda;
         T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        k_diag_factor<T><<<1, 256, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
+        k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
         OMH_LAUNCH_CHECK();
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) break;
@@ -504,7 +552,7 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
     fill_f64(X, n, r, ldx, 0.0, st);
     copy_f64(B, ldb, R, n, n, r, st);
     int it = 0;
-    double rel = 1.0;
+    double rel = 1.0, prev = 2.0;
     for (; it < max_iter; ++it) {
         convert_f64_to_f32(R, n, R32, n, n, r, false, st);
         sym_solve<float>(L32, n, ld32, work32, R32, n, r, st);
@@ -513,7 +561,9 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
         const double dn = frob_norm_f32(R32, n, n, r, st);
         const double xn = frob_norm_f64(X, ldx, n, r, st);
         rel = xn > 0 ? dn / xn : 0.0;
-        if (rel <= tol) { ++it; break; }
+        // converged, or stagnating at the FP64 rounding floor
+        if (rel <= tol || (it >= 2 && rel > 0.5 * prev)) { ++it; break; }
+        prev = rel;
         // R = B - H X
         copy_f64(B, ldb, R, n, n, r, st);
         gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, n, st);
