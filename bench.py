@@ -44,12 +44,22 @@ def run(args):
         G, info = om.gain_eeg_pipeline(geom, dip, el, precision=precision, domain_name="Brain")
         return G, info
 
+    from openmeeg_amd import _lib
+    L = _lib.lib()
+
+    def prof(name):
+        import ctypes as C
+        calls, sec, work = C.c_int64(), C.c_double(), C.c_double()
+        _lib.check(L.omh_profile_read(name.encode(), C.byref(calls), C.byref(sec), C.byref(work)))
+        return calls.value, sec.value, work.value
+
     for w in range(args.warmup):
         t0 = time.time()
         G, info = step(args.precision)
         del G
         log("bench: warmup %d done in %.2f s" % (w, time.time() - t0))
     torch.cuda.synchronize()
+    _lib.check(L.omh_profile_enable(1))
     t0 = time.time()
     infos = []
     Gm = None
@@ -59,6 +69,9 @@ def run(args):
         log("bench: step %d stages %s" % (s, {k: round(v, 4) for k, v in info["times"].items()}))
     torch.cuda.synchronize()
     dt = (time.time() - t0) / max(1, args.steps)
+    kern = {k: prof(k) for k in ("factor_update_f32", "factor_update_f64", "diag_factor_f32",
+                                  "diag_factor_f64", "s_block", "d_pairs")}
+    _lib.check(L.omh_profile_enable(0))
     Gm_h = om.gain_to_numpy(Gm, len(el), len(dip))
     del Gm
     torch.cuda.empty_cache()
@@ -66,27 +79,45 @@ def run(args):
     # plain FP64 path on the same inputs (outside the timed region)
     err64 = None
     t64 = None
+    stages64 = None
     if args.precision != "fp64":
         torch.cuda.synchronize()
         t1 = time.time()
         G64, info64 = step("fp64")
         torch.cuda.synchronize()
         t64 = time.time() - t1
+        stages64 = info64["times"]
         G64h = om.gain_to_numpy(G64, len(el), len(dip))
         del G64
         torch.cuda.empty_cache()
         err64 = float(np.linalg.norm(Gm_h - G64h) / np.linalg.norm(G64h))
         log("bench: fp64 path %.3f s, stages %s, mixed vs fp64 rel err %.3e"
-            % (t64, {k: round(v, 4) for k, v in info64["times"].items()}, err64))
+            % (t64, {k: round(v, 4) for k, v in stages64.items()}, err64))
 
     stages = infos[-1]["times"]
-    fact_s = stages["factor"]
-    flops = n ** 3 / 3.0
     dtype_f = "f32" if args.precision == "mixed" else "f64"
-    achieved = flops / fact_s / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[dtype_f],
+    calls, sec, work = kern["factor_update_" + dtype_f]
+    achieved = work / sec / 1e12 if sec > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[dtype_f],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[dtype_f], 4), "traffic": None,
-                "kernel": "signed-Cholesky factorisation (%s MFMA), n^3/3 flop" % dtype_f}
+                "kernel": "k_gemm<%s,false,true,true> (trailing update of the signed Cholesky)"
+                          % ("float" if dtype_f == "f32" else "double"),
+                "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
+                "flop_per_launch": work / max(calls, 1)}
+    others = {}
+    for k, (c, sc, wk) in kern.items():
+        if c:
+            others[k] = {"launches": c, "seconds": round(sc / args.steps, 4), "work_per_s": wk / sc}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        est, wall, desc, parts = O.time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=args.cpu_rows,
+                                                     n_factor=args.cpu_factor_n)
+        log("bench: cpu baseline %.1f s extrapolated from a %.1f s sample (%s)" % (est, wall, parts))
+        cpu = {"value": round(est, 1), "unit": "s", "cores": cores, "kind": "port",
+               "sample": desc + "; sample wall time %.1f s" % wall,
+               "stages_s": {k: round(v, 1) for k, v in parts.items()}}
     out = {
         "metric": "gain-matrix seconds, mixed vs FP64 on 1 GPU at <=1e-6 rel err vs FP64",
         "value": round(dt, 4),
@@ -105,9 +136,11 @@ def run(args):
         "fp64_seconds": None if t64 is None else round(t64, 4),
         "rel_err_vs_fp64": err64,
         "stages_s": {k: round(v, 4) for k, v in stages.items()},
+        "fp64_stages_s": None if stages64 is None else {k: round(v, 4) for k, v in stages64.items()},
         "refine_iterations": infos[-1]["refine_iterations"],
         "roofline": roofline,
-        "cpu_baseline": None,
+        "kernels": others,
+        "cpu_baseline": cpu,
     }
     return out
 
@@ -120,6 +153,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=1500)
+    ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)

@@ -160,6 +160,16 @@ int omh_sym_pack(const double* d_H, int64_t n, int64_t nv, int64_t ld, int order
 int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ldd, int64_t n, int64_t nv,
                      int64_t ncol, int ordering, int32_t to_reference, void* stream);
 
+/* Kernel timing for benchmarks (no reference counterpart): when enabled,
+ * HIP event pairs are recorded on the launch stream around the named
+ * kernels ("factor_update_f64", "factor_update_f32", "diag_factor_f64",
+ * "diag_factor_f32", "s_block", "d_pairs").  omh_profile_enable clears the
+ * records; omh_profile_read returns the launches, the summed seconds and the
+ * summed algorithmic work (flop for the factor kernels, Gauss-point
+ * evaluations for the assembly kernels) of one name. */
+int omh_profile_enable(int32_t on);
+int omh_profile_read(const char* name, int64_t* calls, double* seconds, double* work);
+
 /* Frees the library's cached device scratch buffers. */
 int omh_release_scratch(void);
 

@@ -45,6 +45,8 @@ _SIGS = {
     "omh_sym_pack": ([vp, i64, i64, i64, C.c_int, vp, vp], C.c_int),
     "omh_permute_rows": ([vp, i64, vp, i64, i64, i64, i64, C.c_int, i32, vp], C.c_int),
     "omh_release_scratch": ([], C.c_int),
+    "omh_profile_enable": ([i32], C.c_int),
+    "omh_profile_read": ([cp, p_i64, p_f64, p_f64], C.c_int),
     "omh_probe_mfma_layout": ([vp, vp], C.c_int),
 }
 

@@ -435,6 +435,20 @@ int omh_release_scratch(void) {
     return guard([&] { omh::release_scratch(); });
 }
 
+int omh_profile_enable(int32_t on) {
+    return guard([&] { omh::prof_enable(on != 0); });
+}
+
+int omh_profile_read(const char* name, int64_t* calls, double* seconds, double* work) {
+    return guard([&] {
+        need(name, "name");
+        need(calls, "calls");
+        need(seconds, "seconds");
+        need(work, "work");
+        omh::prof_read(name, *calls, *seconds, *work);
+    });
+}
+
 int omh_probe_mfma_layout(double* d_out, void* stream) {
     return guard([&] {
         need(d_out, "d_out");

@@ -404,6 +404,10 @@ static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff
     const int diag = (m1 == m2);
     const int64_t ntx = cdiv(n2, TILE);
     const int64_t nblk = diag ? n_upper_tiles(n1) : cdiv(n1, TILE) * ntx;
+    // algorithmic work: OperatorS evaluations (pairs x Gauss points)
+    static const int npts[4] = {3, 6, 7, 16};
+    const double pairs = diag ? 0.5 * (double)n1 * (double)(n1 + 1) : (double)n1 * (double)n2;
+    ProfScope ps("s_block", pairs * npts[order], st);
     if (tmp)
         k_s_block<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp);
@@ -449,6 +453,8 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
     for (int64_t c0 = 0; c0 < n1; c0 += chunk) {
         const int64_t nc = std::min(chunk, n1 - c0);
         const int64_t blk1 = cdiv(nc, 64) * cdiv(n2, 4);
+        static const int npts[4] = {3, 6, 7, 16};
+        ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
         k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W);
         OMH_LAUNCH_CHECK();
         const int64_t blk2 = cdiv(nc, 64) * cdiv(ns, 4);

@@ -36,4 +36,24 @@ void release_scratch();
 // captured graphs).
 void fill_f64(double* p, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s);
 
+// Optional kernel timing: a ProfScope records a HIP event pair around the
+// launches it encloses on stream st, with `work` units of algorithmic work
+// (flops or evaluations), aggregated per name.
+bool prof_enabled();
+void prof_enable(bool on);
+bool prof_read(const std::string& name, int64_t& calls, double& seconds, double& work);
+class ProfScope {
+public:
+    ProfScope(const char* name, double work, hipStream_t st);
+    ~ProfScope();
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+private:
+    hipStream_t st_;
+    bool active_ = false;
+    std::string name_;
+    double work_ = 0.0;
+    hipEvent_t a_{}, b_{};
+};
+
 }  // namespace omh

@@ -65,98 +65,193 @@ __device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) 
     lo = t - h * (h + 1) / 2;
 }
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BK = 16;
+constexpr int LDS_PAD = 16;   // row stride BM+16: the four 16-lane row groups of a fragment read hit disjoint banks
 
-// C = alpha * op(A) * op(B) + beta * C on a BM x BN tile per workgroup.
+template <typename T> struct Vec;
+template <> struct Vec<float> { typedef float4 type; static constexpr int n = 4; };
+template <> struct Vec<double> { typedef double2 type; static constexpr int n = 2; };
+
+// Loads an R x BK operand tile into registers.  MK: element (r, kk) at
+// base[r + kk*ld] (contiguous along r); else base[kk + r*ld] (contiguous
+// along kk).  load_full: the tile is in range and 16-byte aligned (vector
+// loads, no guards); load_guarded: element-wise guarded loads.
+template <typename T, bool MK, int R>
+struct TileLoader {
+    static constexpr int V = Vec<T>::n;
+    static constexpr int NV = R * BK / V / 256;   // vectors per thread
+    T reg[NV][V];
+
+    __device__ __forceinline__ static void coords(int v, int& r, int& kk) {
+        if (MK) { kk = v / (R / V); r = (v % (R / V)) * V; }
+        else { r = v / (BK / V); kk = (v % (BK / V)) * V; }
+    }
+
+    __device__ __forceinline__ void load_full(const T* __restrict__ base, int64_t ld, int64_t r0, int64_t k0, int tid) {
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            int r, kk;
+            coords(tid + 256 * p, r, kk);
+            const T* ptr = MK ? base + (r0 + r) + (k0 + kk) * ld : base + (k0 + kk) + (r0 + r) * ld;
+            const typename Vec<T>::type x = *reinterpret_cast<const typename Vec<T>::type*>(ptr);
+            const T* xs = reinterpret_cast<const T*>(&x);
+#pragma unroll
+            for (int q = 0; q < V; ++q) reg[p][q] = xs[q];
+        }
+    }
+
+    __device__ __forceinline__ void load_guarded(const T* __restrict__ base, int64_t ld, int64_t r0, int64_t k0,
+                                                 int64_t rmax, int64_t kmax, int tid) {
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            int r, kk;
+            coords(tid + 256 * p, r, kk);
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const int64_t rr = r0 + r + (MK ? q : 0), kq = k0 + kk + (MK ? 0 : q);
+                reg[p][q] = (rr < rmax && kq < kmax) ? (MK ? base[rr + kq * ld] : base[kq + rr * ld]) : T(0);
+            }
+        }
+    }
+
+    // LDS tile S[kk][r] (row stride R + LDS_PAD)
+    __device__ __forceinline__ void store(T* __restrict__ S, int tid) const {
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            int r, kk;
+            coords(tid + 256 * p, r, kk);
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                if (MK) S[kk * (R + LDS_PAD) + r + q] = reg[p][q];
+                else S[(kk + q) * (R + LDS_PAD) + r] = reg[p][q];
+            }
+        }
+    }
+};
+
+// Tile enumeration of a launch: 0 full grid (ntm row tiles), 1 lower
+// triangle of tiles, 2 one row strip (tile row tm0), 3 one column strip
+// (tile column tn0, rows below tm_lim excluded).
+struct TileMap {
+    int kind;
+    int64_t ntm, tm0, tn0;
+};
+
+// C = alpha * op(A) * op(B) + beta * C on a (32 WB) x (32 WB) tile per
+// workgroup of 256 threads (2 x 2 waves, each 16 WB x 16 WB = WB x WB MFMA
+// blocks), BK = 16 k-slices double-buffered in LDS.
 // TA: op(A)(i,kk) = A[kk + i*lda] (A stored k x m); else A[i + kk*lda].
 // TB: op(B)(kk,j) = B[j + kk*ldb] (B stored n x k); else B[kk + j*ldb].
-// LOWER: only tiles meeting the lower triangle (row >= col) run, and only
-// entries with row >= col are written (m == n).
-template <typename T, bool TA, bool TB, bool LOWER>
+// LOWER: only entries with row >= col are written (m == n).
+// EDGE = false: every tile of the launch lies inside m x n and the operands
+// are 16-byte aligned, so the k loop uses unguarded vector loads (a partial
+// last k-slice is done after the loop).  EDGE = true: guarded loads.
+template <typename T, bool TA, bool TB, bool LOWER, int WB, bool EDGE>
 __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
                                               int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
-                                              T* __restrict__ C, int64_t ldc, int64_t ntm) {
-    __shared__ T As[BK][BM];
-    __shared__ T Bs[BK][BN];
+                                              T* __restrict__ C, int64_t ldc, TileMap map) {
+    constexpr int BM = 32 * WB, BN = 32 * WB;
+    __shared__ T As[2][BK * (BM + LDS_PAD)];
+    __shared__ T Bs[2][BK * (BN + LDS_PAD)];
     int64_t tm, tn;
-    if (LOWER) {
-        upper_tile(blockIdx.x, tn, tm);   // tn <= tm
-    } else {
-        tn = blockIdx.x / ntm;
-        tm = blockIdx.x - tn * ntm;
+    switch (map.kind) {
+        case 1: upper_tile(blockIdx.x, tn, tm); break;   // tn <= tm
+        case 2: tm = map.tm0; tn = blockIdx.x; break;
+        case 3: tn = map.tn0; tm = blockIdx.x; break;
+        default: tn = blockIdx.x / map.ntm; tm = blockIdx.x - tn * map.ntm; break;
     }
     const int64_t row0 = tm * BM, col0 = tn * BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * (16 * WB), wn = (wave & 1) * (16 * WB);
 
-    typename Acc<T>::type acc[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) acc[a][b] = typename Acc<T>::type{0, 0, 0, 0};
-
-    T ra[4], rb[4];
-    auto load = [&](int64_t k0) {
+    typename Acc<T>::type acc[WB][WB];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            int i, kk;
-            if (TA) { kk = tid & 15; i = (tid >> 4) + 16 * p; }
-            else { i = tid & 63; kk = (tid >> 6) + 4 * p; }
-            const int64_t gi = row0 + i, gk = k0 + kk;
-            T v = 0;
-            if (gi < m && gk < k) v = TA ? A[gk + gi * lda] : A[gi + gk * lda];
-            ra[p] = v;
-            int j, kb;
-            if (TB) { j = tid & 63; kb = (tid >> 6) + 4 * p; }
-            else { kb = tid & 15; j = (tid >> 4) + 16 * p; }
-            const int64_t gj = col0 + j, gkb = k0 + kb;
-            T w = 0;
-            if (gj < n && gkb < k) w = TB ? B[gj + gkb * ldb] : B[gkb + gj * ldb];
-            rb[p] = w;
-        }
-    };
-    auto store = [&]() {
+    for (int a = 0; a < WB; ++a)
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            int i, kk;
-            if (TA) { kk = tid & 15; i = (tid >> 4) + 16 * p; }
-            else { i = tid & 63; kk = (tid >> 6) + 4 * p; }
-            As[kk][i] = ra[p];
-            int j, kb;
-            if (TB) { j = tid & 63; kb = (tid >> 6) + 4 * p; }
-            else { kb = tid & 15; j = (tid >> 4) + 16 * p; }
-            Bs[kb][j] = rb[p];
-        }
-    };
+        for (int b = 0; b < WB; ++b) acc[a][b] = typename Acc<T>::type{0, 0, 0, 0};
 
-    const int64_t nk = (k + BK - 1) / BK;
-    load(0);
-    store();
-    __syncthreads();
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * BK);
+    TileLoader<T, !TA, BM> la;
+    TileLoader<T, TB, BN> lb;
+
+    auto compute = [&](const T* Ab, const T* Bb) {
 #pragma unroll
         for (int k4 = 0; k4 < BK / 4; ++k4) {
             const int kr = k4 * 4 + (lane >> 4);
-            T fa[2], fb[2];
-            fa[0] = As[kr][wm + (lane & 15)];
-            fa[1] = As[kr][wm + 16 + (lane & 15)];
-            fb[0] = Bs[kr][wn + (lane & 15)];
-            fb[1] = Bs[kr][wn + 16 + (lane & 15)];
+            T fa[WB], fb[WB];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < WB; ++a) fa[a] = Ab[kr * (BM + LDS_PAD) + wm + a * 16 + (lane & 15)];
 #pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = Acc<T>::mfma(fa[a], fb[b], acc[a][b]);
+            for (int b = 0; b < WB; ++b) fb[b] = Bb[kr * (BN + LDS_PAD) + wn + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < WB; ++a)
+#pragma unroll
+                for (int b = 0; b < WB; ++b) acc[a][b] = Acc<T>::mfma(fa[a], fb[b], acc[a][b]);
         }
-        __syncthreads();
-        if (kt + 1 < nk) {
-            store();
+    };
+
+    if (!EDGE) {
+        const int64_t nk = k / BK;   // full k-slices
+        if (nk > 0) {
+            la.load_full(A, lda, row0, 0, tid);
+            lb.load_full(B, ldb, col0, 0, tid);
+            la.store(As[0], tid);
+            lb.store(Bs[0], tid);
+            __syncthreads();
+            for (int64_t kt = 0; kt < nk; ++kt) {
+                const int buf = (int)(kt & 1);
+                if (kt + 1 < nk) {
+                    la.load_full(A, lda, row0, (kt + 1) * BK, tid);
+                    lb.load_full(B, ldb, col0, (kt + 1) * BK, tid);
+                }
+                compute(As[buf], Bs[buf]);
+                if (kt + 1 < nk) {
+                    la.store(As[buf ^ 1], tid);
+                    lb.store(Bs[buf ^ 1], tid);
+                }
+                __syncthreads();
+            }
+        }
+        if (k % BK) {   // partial last k-slice
+            la.load_guarded(A, lda, row0, nk * BK, m, k, tid);
+            lb.load_guarded(B, ldb, col0, nk * BK, n, k, tid);
+            la.store(As[0], tid);
+            lb.store(Bs[0], tid);
+            __syncthreads();
+            compute(As[0], Bs[0]);
+        }
+    } else {
+        const int64_t nk = (k + BK - 1) / BK;
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            la.load_guarded(A, lda, row0, kt * BK, m, k, tid);
+            lb.load_guarded(B, ldb, col0, kt * BK, n, k, tid);
+            la.store(As[0], tid);
+            lb.store(Bs[0], tid);
+            __syncthreads();
+            compute(As[0], Bs[0]);
             __syncthreads();
         }
     }
+    // Epilogue.  FP32: a lane's 4 elements are 4 consecutive rows of one
+    // column, written with one 16-byte access on interior tiles.
+    const bool vec_c = !EDGE && (!LOWER || tm != tn) && (reinterpret_cast<uintptr_t>(C) % 16 == 0) && (ldc % 4 == 0);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WB; ++b) {
+            if (sizeof(T) == 4 && vec_c) {
+                const int64_t gi = row0 + wm + a * 16 + 4 * (lane >> 4);
+                const int64_t gj = col0 + wn + b * 16 + (lane & 15);
+                float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
+                float4 v = make_float4(alpha * acc[a][b][0], alpha * acc[a][b][1], alpha * acc[a][b][2],
+                                       alpha * acc[a][b][3]);
+                if (beta != T(0)) {
+                    const float4 c = *cp;
+                    v.x += beta * c.x; v.y += beta * c.y; v.z += beta * c.z; v.w += beta * c.w;
+                }
+                *cp = v;
+                continue;
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t gi = row0 + wm + a * 16 + Acc<T>::row(lane, r);
@@ -167,6 +262,7 @@ __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T
                     C[gi + gj * ldc] = v;
                 }
             }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -396,18 +492,16 @@ inline unsigned grid_for(int64_t total) { return (unsigned)std::max<int64_t>(1, 
 // ---------------------------------------------------------------------------
 // host API
 // ---------------------------------------------------------------------------
-template <typename T>
-void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
-          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
-    if (m <= 0 || n <= 0) return;
-    const int64_t ntm = cdiv(m, BM), ntn = cdiv(n, BN);
-    if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
-    const int64_t nblk = lower ? ntm * (ntm + 1) / 2 : ntm * ntn;
+template <typename T, int WB, bool EDGE>
+static void launch_tiles(bool ta, bool tb, bool lower, int64_t nblk, TileMap map, int64_t m, int64_t n, int64_t k,
+                         T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
+                         hipStream_t st) {
+    if (nblk <= 0) return;
     if (nblk > 0xffffffffLL) throw std::invalid_argument("gemm grid too large");
 #define OMH_GEMM_CASE(TA_, TB_, LO_)                                                                          \
     if (ta == TA_ && tb == TB_ && lower == LO_) {                                                             \
-        k_gemm<T, TA_, TB_, LO_><<<(unsigned)nblk, 256, 0, st>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, \
-                                                                 ntm);                                       \
+        k_gemm<T, TA_, TB_, LO_, WB, EDGE><<<(unsigned)nblk, 256, 0, st>>>(m, n, k, alpha, A, lda, B, ldb, beta, \
+                                                                           C, ldc, map);                      \
         OMH_LAUNCH_CHECK();                                                                                   \
         return;                                                                                               \
     }
@@ -418,6 +512,60 @@ void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha
     OMH_GEMM_CASE(false, true, true)
 #undef OMH_GEMM_CASE
     throw std::invalid_argument("unsupported gemm variant");
+}
+
+// Interior tiles run the unguarded kernel; the partial last tile row and
+// column (if any) run as edge strips.
+template <typename T, int WB>
+static void gemm_launch(bool ta, bool tb, bool lower, bool vec, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
+                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+    constexpr int TILE = 32 * WB;
+    const int64_t ntm = cdiv(m, TILE), ntn = cdiv(n, TILE);
+    if (!vec) {
+        const int64_t nblk = lower ? ntm * (ntm + 1) / 2 : ntm * ntn;
+        launch_tiles<T, WB, true>(ta, tb, lower, nblk, TileMap{lower ? 1 : 0, ntm, 0, 0}, m, n, k, alpha, A, lda, B,
+                                  ldb, beta, C, ldc, st);
+        return;
+    }
+    const int64_t fm = m / TILE, fn = n / TILE;   // full tile rows / columns
+    if (lower) {
+        launch_tiles<T, WB, false>(ta, tb, true, fm * (fm + 1) / 2, TileMap{1, fm, 0, 0}, m, n, k, alpha, A, lda, B,
+                                   ldb, beta, C, ldc, st);
+        if (fm < ntm)   // last (partial) tile row, all its lower tiles
+            launch_tiles<T, WB, true>(ta, tb, true, ntm, TileMap{2, ntm, fm, 0}, m, n, k, alpha, A, lda, B, ldb,
+                                      beta, C, ldc, st);
+        return;
+    }
+    launch_tiles<T, WB, false>(ta, tb, false, fm * fn, TileMap{0, fm, 0, 0}, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                               ldc, st);
+    if (fm < ntm)   // partial last tile row (all tile columns)
+        launch_tiles<T, WB, true>(ta, tb, false, ntn, TileMap{2, ntm, fm, 0}, m, n, k, alpha, A, lda, B, ldb, beta,
+                                  C, ldc, st);
+    if (fn < ntn)   // partial last tile column (full tile rows only)
+        launch_tiles<T, WB, true>(ta, tb, false, fm, TileMap{3, ntm, 0, fn}, m, n, k, alpha, A, lda, B, ldb, beta,
+                                  C, ldc, st);
+}
+
+template <typename T>
+static bool aligned_for_vec(const T* p, int64_t ld) {
+    constexpr int V = 16 / sizeof(T);
+    return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && (ld % V == 0);
+}
+
+template <typename T>
+void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
+    const bool vec = aligned_for_vec(A, lda) && aligned_for_vec(B, ldb);
+    // FP64: 64 x 64 tiles (128 x 128 accumulators would limit FP64 to one wave
+    // per SIMD).  FP32: 128 x 128 tiles when they give at least ~2 waves of
+    // workgroups over the 256 CUs.
+    const int64_t big_tiles = lower ? cdiv(m, 128) * (cdiv(m, 128) + 1) / 2 : cdiv(m, 128) * cdiv(n, 128);
+    if (sizeof(T) == 4 && big_tiles >= 512)
+        gemm_launch<T, 4>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    else
+        gemm_launch<T, 2>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
 }
 
 template void gemm<double>(bool, bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
@@ -447,8 +595,11 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
         const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
         T* A11 = A + k + k * lda;
         T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
-        OMH_LAUNCH_CHECK();
+        {
+            ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
+            k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
+            OMH_LAUNCH_CHECK();
+        }
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) break;
         T* A21 = A + e + k * lda;
@@ -456,8 +607,11 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
         gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, m, st);
         k_scale_cols<T><<<grid_for(m * nb), 256, 0, st>>>(X, m, A21, lda, m, nb, sgn + k);
         OMH_LAUNCH_CHECK();
-        // A22 -= X * L21^T (lower)
-        gemm<T>(false, true, true, m, m, nb, T(-1), X, m, A21, lda, T(1), A + e + e * lda, lda, st);
+        // A22 -= X * L21^T (lower); algorithmic work m(m+1) nb flop
+        {
+            ProfScope ps(sizeof(T) == 8 ? "factor_update_f64" : "factor_update_f32", (double)m * (m + 1) * nb, st);
+            gemm<T>(false, true, true, m, m, nb, T(-1), X, m, A21, lda, T(1), A + e + e * lda, lda, st);
+        }
     }
     FactorInfo fi;
     int h_info = 0;

@@ -569,3 +569,30 @@ double orc_integrate_s_pair(const double* tri_a, const double* tri_b, int order)
     OpS o; ops_init(&o, &a);
     return integrate_S(&o, &b, order);
 }
+
+/* Sample of the HeadMat work for the CPU-baseline timing: rows [r0, r1) of
+ * mesh m1 against all triangles of m2, S pairs (mode 1, analytic side m1 as
+ * s_block) or D pairs (mode 2, as d_block), accumulated into a scratch sum
+ * so that nothing is optimised away.  Returns the number of pairs done. */
+double orc_sample_rows(const Geom* g, i64 m1, i64 m2, int order, i64 r0, i64 r1, int mode, double* sink) {
+    const i64 a0 = g->tri_off[m1];
+    const i64 b0 = g->tri_off[m2], b1 = g->tri_off[m2 + 1];
+    double acc = 0.0;
+    #pragma omp parallel for reduction(+:acc) schedule(dynamic, 1)
+    for (i64 i = a0 + r0; i < a0 + r1; ++i) {
+        Tri t1 = geom_tri(g, i);
+        OpS o; ops_init(&o, &t1);
+        for (i64 j = b0; j < b1; ++j) {
+            Tri t2 = geom_tri(g, j);
+            if (mode == 1) {
+                acc += integrate_S(&o, &t2, order);
+            } else {
+                P1S p; p1s_init(&p, &t2);
+                V3 v = integrate_D(&p, &t1, order);
+                acc += v.x + v.y + v.z;
+            }
+        }
+    }
+    *sink = acc;
+    return (double)(r1 - r0) * (double)(b1 - b0);
+}

@@ -66,6 +66,8 @@ def lib():
         L.orc_solid_angle.restype = C.c_double
         L.orc_integrate_s_pair.argtypes = [dp, dp, C.c_int]
         L.orc_integrate_s_pair.restype = C.c_double
+        L.orc_sample_rows.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, i64, i64, C.c_int, dp]
+        L.orc_sample_rows.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -318,3 +320,61 @@ def read_sensors(path):
     labeled = rows[0][0].count(".") != 1
     data = [r[1:] if labeled else r for r in rows]
     return np.array(data, dtype=np.float64)[:, :3]
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline (bench.py's cpu_baseline leg)
+# ---------------------------------------------------------------------------
+def time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=6, n_dip_sample=16, n_factor=4000):
+    """Times the oracle on a bounded sample of a gain workload and scales
+    each stage to the full problem:
+      * HeadMat: S+D evaluations of `rows_per_pair` rows of every
+        communicating mesh pair (per-evaluation time x the pair count of
+        every block, S on the upper triangle of diagonal blocks);
+      * DipSourceMat: `n_dip_sample` dipoles (x n_dip / sample);
+      * factor + solve: LAPACK dsytrf/dsytrs on an n_factor symmetric
+        matrix (x (n / n_factor)^3).
+    Returns (extrapolated seconds, sample wall seconds, description)."""
+    import time
+    import scipy.linalg.lapack as la
+    g = nested_geometry(meshes, cond)
+    t = tables(g)
+    L = lib()
+    sink = C.c_double()
+    wall0 = time.time()
+    # per-pair cost of S and D evaluations, sampled on the innermost mesh pair
+    costs = {}
+    for mode in (1, 2):
+        t0 = time.time()
+        pairs = L.orc_sample_rows(C.byref(t.s), 0, 0, 3, 0, rows_per_pair, mode, C.byref(sink))
+        costs[mode] = (time.time() - t0) / pairs
+    # pair counts of the whole HeadMat (assembleHeadMat.cpp:83-108 with the
+    # block rules of operators.h:180-197,363-384; N and deflation are O(V^2)
+    # gathers and not counted)
+    n_s = n_d = 0.0
+    for (m1, m2, o, cS, cN, cD) in g.pairs:
+        n1, n2 = len(g.meshes[m1].tris), len(g.meshes[m2].tris)
+        b1, b2 = g.meshes[m1].current_barrier, g.meshes[m2].current_barrier
+        n_s += n1 * (n1 + 1) / 2.0 if m1 == m2 else n1 * n2
+        if not b1:
+            n_d += n1 * n2
+        if m1 != m2 and not b2:
+            n_d += n1 * n2
+    t_asm = n_s * costs[1] + n_d * costs[2]
+    ns = min(n_dip_sample, len(dip))
+    t0 = time.time()
+    dip_source_mat(g, dip[:ns], "Brain")
+    t_dsm = (time.time() - t0) * len(dip) / ns
+    rng = np.random.default_rng(0)
+    M = rng.standard_normal((n_factor, n_factor))
+    M = M + M.T + n_factor * np.eye(n_factor)
+    B = rng.standard_normal((n_factor, len(el)))
+    t0 = time.time()
+    lu, piv, info = la.dsytrf(M)
+    la.dsytrs(lu, piv, B)
+    t_fac = (time.time() - t0) * (g.size / n_factor) ** 3
+    wall = time.time() - wall0
+    desc = ("oracle on a sample of the same workload: S and D pairs of %d rows of the innermost mesh, %d of %d dipoles, "
+            "LAPACK dsytrf+dsytrs at n=%d; each stage scaled to n=%d (assembly by pair count, DSM by dipole "
+            "count, factor by n^3)" % (rows_per_pair, ns, len(dip), n_factor, g.size))
+    return t_asm + t_dsm + t_fac, wall, desc, dict(assembly=t_asm, dsm=t_dsm, factor_solve=t_fac)
