@@ -29,6 +29,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(config, dtype_f):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "factor_update_%s_%s_traffic.md" % (dtype_f, config))
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_launch:"):
+            return float(ln.split(":")[1])
+    return None
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -69,8 +81,8 @@ def run(args):
         log("bench: step %d stages %s" % (s, {k: round(v, 4) for k, v in info["times"].items()}))
     torch.cuda.synchronize()
     dt = (time.time() - t0) / max(1, args.steps)
-    kern = {k: prof(k) for k in ("factor_update_f32", "factor_update_f64", "diag_factor_f32",
-                                  "diag_factor_f64", "s_block", "d_pairs")}
+    kern = {k: prof(k) for k in ("factor_update_f32_t128", "factor_update_f32_t64", "factor_update_f64_t64",
+                                  "diag_factor_f32", "diag_factor_f64", "s_block", "d_pairs")}
     _lib.check(L.omh_profile_enable(0))
     Gm_h = om.gain_to_numpy(Gm, len(el), len(dip))
     del Gm
@@ -80,7 +92,7 @@ def run(args):
     err64 = None
     t64 = None
     stages64 = None
-    if args.precision != "fp64":
+    if args.precision != "fp64" and not args.skip_fp64:
         torch.cuda.synchronize()
         t1 = time.time()
         G64, info64 = step("fp64")
@@ -96,12 +108,14 @@ def run(args):
 
     stages = infos[-1]["times"]
     dtype_f = "f32" if args.precision == "mixed" else "f64"
-    calls, sec, work = kern["factor_update_" + dtype_f]
+    dom = "factor_update_f32_t128" if dtype_f == "f32" else "factor_update_f64_t64"
+    calls, sec, work = kern[dom]
     achieved = work / sec / 1e12 if sec > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[dtype_f],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[dtype_f], 4), "traffic": None,
-                "kernel": "k_gemm<%s,false,true,true> (trailing update of the signed Cholesky)"
-                          % ("float" if dtype_f == "f32" else "double"),
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[dtype_f], 4),
+                "traffic": measured_traffic(args.config, dtype_f),
+                "kernel": "k_gemm<%s,false,true,true,%d,false> (interior tiles of the trailing update of the "
+                          "signed Cholesky)" % ("float" if dtype_f == "f32" else "double", 4 if dtype_f == "f32" else 2),
                 "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
                 "flop_per_launch": work / max(calls, 1)}
     others = {}
@@ -153,12 +167,25 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--cpu-rows", type=int, default=1500)
     ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}), flush=True)
+        return
     out = run(args)
     print(json.dumps(out), flush=True)
 

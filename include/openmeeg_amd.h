@@ -162,8 +162,10 @@ int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ld
 
 /* Kernel timing for benchmarks (no reference counterpart): when enabled,
  * HIP event pairs are recorded on the launch stream around the named
- * kernels ("factor_update_f64", "factor_update_f32", "diag_factor_f64",
- * "diag_factor_f32", "s_block", "d_pairs").  omh_profile_enable clears the
+ * kernels: "factor_update_f32_t128", "factor_update_f32_t64",
+ * "factor_update_f64_t64" (interior launches of the trailing-update GEMM
+ * with 128- or 64-wide tiles), "diag_factor_f64", "diag_factor_f32",
+ * "s_block", "d_pairs".  omh_profile_enable clears the
  * records; omh_profile_read returns the launches, the summed seconds and the
  * summed algorithmic work (flop for the factor kernels, Gauss-point
  * evaluations for the assembly kernels) of one name. */

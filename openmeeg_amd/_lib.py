@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libopenmeeg_amd.so")
+LIB_PATH = os.environ.get("OMH_LIBRARY", os.path.join(_HERE, "libopenmeeg_amd.so"))
 
 ORDER_REFERENCE = 0
 ORDER_TRIANGLES_FIRST = 1

@@ -93,8 +93,9 @@ class Geometry:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and h.value and _lib._lib is not None:
-            _lib._lib.omh_geometry_free(h)
+        L = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h and h.value and L is not None:
+            L.omh_geometry_free(h)
             self._h = None
 
     # reference accessors (geometry.h:91,160,253)

@@ -213,6 +213,11 @@ DeviceGeometry& device_geometry(Geometry& g) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Minimum waves per SIMD requested for the FP64 pair kernels (register cap).
+#ifndef OMH_PAIR_MIN_WAVES
+#define OMH_PAIR_MIN_WAVES 2
+#endif
+
 constexpr int TILE = 16;
 
 __device__ __forceinline__ int64_t cdiv_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -230,7 +235,7 @@ __device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) 
 // TEMP: writes the dense temporary block tmp[i + j*n1] (and its mirror for
 // diagonal blocks) -- the SymBloc/Bloc of operators.h:159-173,337-352.
 template <bool TEMP>
-__global__ void __launch_bounds__(256) k_s_block(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
+__global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
                                                  int64_t b0, int64_t n2, int diag, int64_t ntx, int order,
                                                  double coeff, double* __restrict__ H, Layout L,
                                                  int64_t ua0, int64_t ub0, double* __restrict__ tmp) {
@@ -264,7 +269,7 @@ __global__ void __launch_bounds__(256) k_s_block(const TriRec* __restrict__ tri,
 
 // D pass 1: W[(k*n2 + j)*n1c + i] = integral over row triangle (a0+i)
 // (Gauss) of P1SLP(column triangle b0+j)_k  (operators.h:103-106).
-__global__ void __launch_bounds__(256) k_d_pairs(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
+__global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_d_pairs(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
                                                  int64_t b0, int64_t n2, int order, double* __restrict__ W) {
     const int64_t ni = cdiv_dev(n1c, 64);
     const int64_t tj = blockIdx.x / ni, tb = blockIdx.x - tj * ni;

@@ -114,9 +114,10 @@ __device__ __forceinline__ OpS load_ops(const TriRec& t) {
 __device__ __forceinline__ double ops_eval(const OpS& o, d3 x) {
     const d3 r0 = sub(o.p0, x), r1 = sub(o.p1, x), r2 = sub(o.p2, x);
     const double d0 = norm(r0), d1 = norm(r1), d2 = norm(r2);
-    double sum = dot(r0, o.e0) * atanh(o.l0 / (d0 + d1));
-    sum += dot(r1, o.e1) * atanh(o.l1 / (d1 + d2));
-    sum += dot(r2, o.e2) * atanh(o.l2 / (d2 + d0));
+    // atanh(l/R) written as log1p(2l/(R-l))/2: one division instead of two
+    double sum = dot(r0, o.e0) * (0.5 * log1p(2 * o.l0 / (d0 + d1 - o.l0)));
+    sum += dot(r1, o.e1) * (0.5 * log1p(2 * o.l1 / (d1 + d2 - o.l1)));
+    sum += dot(r2, o.e2) * (0.5 * log1p(2 * o.l2 / (d2 + d0 - o.l2)));
     const double hn = dot(r0, o.n);
     const double det = 2 * o.area * hn;
     double omega = 0.0;

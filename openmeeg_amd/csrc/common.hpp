@@ -46,6 +46,7 @@ class ProfScope {
 public:
     ProfScope(const char* name, double work, hipStream_t st);
     ~ProfScope();
+    void cancel();   // drop the record (timing not wanted after all)
     ProfScope(const ProfScope&) = delete;
     ProfScope& operator=(const ProfScope&) = delete;
 private:

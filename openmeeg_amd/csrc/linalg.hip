@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -518,7 +520,8 @@ static void launch_tiles(bool ta, bool tb, bool lower, int64_t nblk, TileMap map
 // column (if any) run as edge strips.
 template <typename T, int WB>
 static void gemm_launch(bool ta, bool tb, bool lower, bool vec, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
-                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st,
+                        const char* prof) {
     constexpr int TILE = 32 * WB;
     const int64_t ntm = cdiv(m, TILE), ntn = cdiv(n, TILE);
     if (!vec) {
@@ -529,8 +532,18 @@ static void gemm_launch(bool ta, bool tb, bool lower, bool vec, int64_t m, int64
     }
     const int64_t fm = m / TILE, fn = n / TILE;   // full tile rows / columns
     if (lower) {
-        launch_tiles<T, WB, false>(ta, tb, true, fm * (fm + 1) / 2, TileMap{1, fm, 0, 0}, m, n, k, alpha, A, lda, B,
-                                   ldb, beta, C, ldc, st);
+        {
+            // optional timing of the interior launch; algorithmic work: the
+            // lower triangle of the interior square, (fm T)(fm T + 1) k flop
+            std::string name;
+            if (prof) name = std::string(prof) + "_t" + std::to_string(TILE);
+            const double side = (double)(fm * TILE);
+            ProfScope ps(prof ? name.c_str() : "", prof ? side * (side + 1) * (double)k : 0.0,
+                         prof ? st : nullptr);
+            if (!prof) ps.cancel();
+            launch_tiles<T, WB, false>(ta, tb, true, fm * (fm + 1) / 2, TileMap{1, fm, 0, 0}, m, n, k, alpha, A, lda,
+                                       B, ldb, beta, C, ldc, st);
+        }
         if (fm < ntm)   // last (partial) tile row, all its lower tiles
             launch_tiles<T, WB, true>(ta, tb, true, ntm, TileMap{2, ntm, fm, 0}, m, n, k, alpha, A, lda, B, ldb,
                                       beta, C, ldc, st);
@@ -554,7 +567,7 @@ static bool aligned_for_vec(const T* p, int64_t ld) {
 
 template <typename T>
 void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
-          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st, const char* prof) {
     if (m <= 0 || n <= 0) return;
     if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
     const bool vec = aligned_for_vec(A, lda) && aligned_for_vec(B, ldb);
@@ -563,15 +576,15 @@ void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha
     // workgroups over the 256 CUs.
     const int64_t big_tiles = lower ? cdiv(m, 128) * (cdiv(m, 128) + 1) / 2 : cdiv(m, 128) * cdiv(n, 128);
     if (sizeof(T) == 4 && big_tiles >= 512)
-        gemm_launch<T, 4>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+        gemm_launch<T, 4>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st, prof);
     else
-        gemm_launch<T, 2>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+        gemm_launch<T, 2>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st, prof);
 }
 
 template void gemm<double>(bool, bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
-                           int64_t, double, double*, int64_t, hipStream_t);
+                           int64_t, double, double*, int64_t, hipStream_t, const char*);
 template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*,
-                          int64_t, float, float*, int64_t, hipStream_t);
+                          int64_t, float, float*, int64_t, hipStream_t, const char*);
 
 size_t factor_workspace_bytes(int64_t n, size_t elem) {
     const int64_t nblk = cdiv(n, FACT_NB);
@@ -581,6 +594,41 @@ size_t factor_workspace_bytes(int64_t n, size_t elem) {
            + 64;
 }
 
+// Side stream and events of the look-ahead (one set per device, default
+// priority).
+struct LookAhead {
+    hipStream_t side = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+};
+
+static LookAhead& lookahead() {
+    static std::mutex mu;
+    static std::map<int, LookAhead> per_dev;
+    int dev = 0;
+    OMH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    LookAhead& la = per_dev[dev];
+    if (!la.side) {
+        OMH_HIP(hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking));
+        OMH_HIP(hipEventCreateWithFlags(&la.ready, hipEventDisableTiming));
+        OMH_HIP(hipEventCreateWithFlags(&la.done, hipEventDisableTiming));
+    }
+    return la;
+}
+
+template <typename T>
+static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, int* info, T tiny, hipStream_t st) {
+    T* A11 = A + k + k * lda;
+    T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+    ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
+    k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
+    OMH_LAUNCH_CHECK();
+}
+
+// Right-looking blocked factorisation with a one-block look-ahead: after
+// the panel of block k, the columns of block k+1 are updated first, then
+// block k+1 is factored on the side stream while the rest of the trailing
+// matrix is updated on the caller's stream.
 template <typename T>
 FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
     T* X = (T*)work;
@@ -591,27 +639,36 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
     const T tiny = sizeof(T) == 8 ? T(1e-290) : T(1e-30);
+    LookAhead& la = lookahead();
+    const char* upd = sizeof(T) == 8 ? "factor_update_f64" : "factor_update_f32";
+    diag_factor<T>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
     for (int64_t k = 0; k < n; k += FACT_NB) {
         const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
-        T* A11 = A + k + k * lda;
-        T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        {
-            ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
-            k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
-            OMH_LAUNCH_CHECK();
-        }
+        const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) break;
         T* A21 = A + e + k * lda;
-        // X = A21 * Linv^T
+        // panel: X = A21 * Linv^T, L21 = X * S1
         gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, m, st);
         k_scale_cols<T><<<grid_for(m * nb), 256, 0, st>>>(X, m, A21, lda, m, nb, sgn + k);
         OMH_LAUNCH_CHECK();
-        // A22 -= X * L21^T (lower); algorithmic work m(m+1) nb flop
-        {
-            ProfScope ps(sizeof(T) == 8 ? "factor_update_f64" : "factor_update_f32", (double)m * (m + 1) * nb, st);
-            gemm<T>(false, true, true, m, m, nb, T(-1), X, m, A21, lda, T(1), A + e + e * lda, lda, st);
-        }
+        // trailing update A22 -= X L21^T (lower)
+        const int nb2 = (int)std::min<int64_t>(FACT_NB, m);   // next block
+        T* A22 = A + e + e * lda;
+        // (a) columns of the next block
+        gemm<T>(false, true, true, nb2, nb2, nb, T(-1), X, m, A21, lda, T(1), A22, lda, st);
+        if (m > nb2)
+            gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, m, A21, lda, T(1), A22 + nb2, lda, st);
+        // factor the next block on the side stream
+        OMH_HIP(hipEventRecord(la.ready, st));
+        OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
+        diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
+        OMH_HIP(hipEventRecord(la.done, la.side));
+        // (b) the rest of the trailing matrix
+        if (m > nb2)
+            gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, m, A21 + nb2, lda, T(1),
+                    A22 + nb2 + nb2 * lda, lda, st, upd);
+        OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
     }
     FactorInfo fi;
     int h_info = 0;

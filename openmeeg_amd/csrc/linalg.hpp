@@ -12,9 +12,11 @@ constexpr int FACT_NB = 256;   // outer block of the factorisation
 
 // C = alpha op(A) op(B) + beta C (column-major).  lower: square C, only
 // entries with row >= col are computed (only ta=false, tb=true).
+// prof: optional name under which the interior launch of a lower (SYRK)
+// product is timed (ProfScope name + "_t<tile>").
 template <typename T>
 void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
-          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st);
+          const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st, const char* prof = nullptr);
 
 struct FactorInfo {
     int info = 0;              // 0, or 1 + index of the first (near-)zero pivot

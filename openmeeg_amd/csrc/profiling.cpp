@@ -54,6 +54,13 @@ ProfScope::ProfScope(const char* name, double work, hipStream_t st) : st_(st) {
     OMH_HIP(hipEventRecord(a_, st_));
 }
 
+void ProfScope::cancel() {
+    if (!active_) return;
+    active_ = false;
+    (void)hipEventDestroy(a_);
+    (void)hipEventDestroy(b_);
+}
+
 ProfScope::~ProfScope() {
     if (!active_) return;
     (void)hipEventRecord(b_, st_);
