@@ -97,6 +97,33 @@ int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, 
 int omh_dip_source_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains,
                        int order, int levels, double tol, double* d_out, int64_t ld, int ordering, void* stream);
 
+/* Adjoint of omh_dip_source_mat (no reference counterpart; the gradient
+ * path of the forward model): with the non-adaptive rule of `order`,
+ * d_grad[6s + 0..2] = d/d(x y z) and d_grad[6s + 3..5] = d/d(qx qy qz) of
+ * W(:,s) . rhs_s, where rhs_s is the DipSourceMat column of dipole s and W
+ * an n x n_dip device matrix (leading dimension ldw) in `ordering`. */
+int omh_dip_source_adjoint(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains,
+                           int order, const double* d_W, int64_t ldw, int ordering, double* d_grad, void* stream);
+
+/* Conductivity of a domain (Domain::conductivity, domain.h:77-79) and its
+ * change with the pair coefficients recomputed (a non-conductive domain
+ * cannot change).  Pair i of the communicating mesh pairs
+ * (geometry.cpp:405-420): meshes[2], relative orientation, coeffs[3] =
+ * (S, N, D) coefficients of assembleHeadMat.cpp:92-97, domain_mask = bit d
+ * set for each common domain d (at most 31 domains).  Mesh m (info[8]):
+ * info[0] first triangle unknown, info[1] triangles, info[2..3] vertex
+ * unknown range (-1 if not contiguous), info[4] vertices, info[5] flags
+ * (1 current barrier, 2 isolated, 4 outermost), info[6] its part
+ * (connected component), info[7] the unknown whose diagonal entry sets the
+ * part's deflation constant (assembleHeadMat.cpp:28-60) if this mesh holds
+ * it, else -1. */
+int omh_geometry_domain_conductivity(const omh_geometry* g, int32_t domain, double* value);
+int omh_geometry_set_conductivity(omh_geometry* g, int32_t domain, double value);
+int omh_geometry_pair_count(const omh_geometry* g, int32_t* count);
+int omh_geometry_pair_info(const omh_geometry* g, int32_t i, int32_t* meshes, int32_t* orientation, double* coeffs,
+                           int32_t* domain_mask);
+int omh_geometry_mesh_info(const omh_geometry* g, int32_t m, int64_t* info);
+
 /* Head2EEGMat(geo, electrodes): assemble.h:35, assembleSensors.cpp:25-39,
  * danielsson.cpp:26-129.  d_electrodes n_el x 3 row-major.  Row e of the
  * sparse matrix has the three entries (h_cols[3e+k], h_vals[3e+k]); columns

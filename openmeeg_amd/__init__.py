@@ -4,7 +4,8 @@ Drop-in names of the reference's Python module ``openmeeg``
 (wrapping/python/openmeeg/__init__.py:15-39) for the EEG forward path:
 HeadMat, DipSourceMat, Head2EEGMat, GainEEG, GainEEGadjoint, Integrator,
 Matrix, SymMatrix, Sensors, make_geometry, make_nested_geometry,
-read_geometry.  All computation runs in the HIP library
+read_geometry; plus EEGForward, the EEG gain differentiable in the dipoles
+and the conductivities (no reference counterpart).  All computation runs in the HIP library
 (libopenmeeg_amd.so) through its C ABI (include/openmeeg_amd.h).
 """
 from .api import (  # noqa: F401
@@ -29,3 +30,4 @@ from .api import (  # noqa: F401
 )
 
 __version__ = "0.1.0"
+from .differentiable import EEGForward  # noqa: F401,E402

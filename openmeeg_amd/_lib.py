@@ -31,6 +31,12 @@ _SIGS = {
     "omh_locate_domains": ([vp, vp, i64, i64, p_i32, vp], C.c_int),
     "omh_dip_source_mat": ([vp, vp, i64, p_i32, C.c_int, C.c_int, f64, vp, i64, C.c_int, vp], C.c_int),
     "omh_head2eeg_mat": ([vp, vp, i64, p_i64, p_f64, vp], C.c_int),
+    "omh_dip_source_adjoint": ([vp, vp, i64, p_i32, C.c_int, vp, i64, C.c_int, vp, vp], C.c_int),
+    "omh_geometry_domain_conductivity": ([vp, i32, p_f64], C.c_int),
+    "omh_geometry_set_conductivity": ([vp, i32, f64], C.c_int),
+    "omh_geometry_pair_count": ([vp, p_i32], C.c_int),
+    "omh_geometry_pair_info": ([vp, i32, p_i32, p_i32, p_f64, p_i32], C.c_int),
+    "omh_geometry_mesh_info": ([vp, i32, p_i64], C.c_int),
     "omh_factor_workspace_bytes": ([i64, i32], i64),
     "omh_sym_factor_f64": ([vp, i64, i64, vp, p_i32, vp], C.c_int),
     "omh_sym_factor_f32": ([vp, i64, i64, vp, p_i32, vp], C.c_int),

@@ -130,6 +130,52 @@ class Geometry:
             out.append(buf.value.decode())
         return out
 
+    def conductivities(self):
+        """Conductivity of every domain, in domain order (domain.h:77-79)."""
+        v = C.c_double()
+        out = []
+        for d in range(self._nd):
+            check(lib().omh_geometry_domain_conductivity(self._h, d, C.byref(v)))
+            out.append(v.value)
+        return np.array(out)
+
+    def set_conductivity(self, domain, value):
+        """Change the conductivity of a conductive domain (index or name);
+        the HeadMat / source coefficients follow."""
+        d = self.domain_index(domain) if isinstance(domain, str) else int(domain)
+        check(lib().omh_geometry_set_conductivity(self._h, d, float(value)))
+
+    def mesh_pairs(self):
+        """Communicating mesh pairs (geometry.cpp:405-420) as dicts with the
+        meshes, orientation, (S, N, D) coefficients and common domains."""
+        cnt = C.c_int32()
+        check(lib().omh_geometry_pair_count(self._h, C.byref(cnt)))
+        out = []
+        for i in range(cnt.value):
+            m = (C.c_int32 * 2)()
+            o, mask = C.c_int32(), C.c_int32()
+            c = (C.c_double * 3)()
+            check(lib().omh_geometry_pair_info(self._h, i, m, C.byref(o), c, C.byref(mask)))
+            out.append(dict(meshes=(m[0], m[1]), orientation=o.value, coeffs=tuple(c),
+                            domains=[d for d in range(self._nd) if mask.value >> d & 1]))
+        return out
+
+    def mesh_info(self, m):
+        """Unknown ranges and flags of mesh m (reference numbering)."""
+        a = (C.c_int64 * 8)()
+        check(lib().omh_geometry_mesh_info(self._h, m, a))
+        return dict(tri_first=a[0], ntris=a[1], vert_first=a[2], vert_end=a[3], nverts=a[4],
+                    current_barrier=bool(a[5] & 1), isolated=bool(a[5] & 2), outermost=bool(a[5] & 4),
+                    part=a[6], deflation_pivot=a[7])
+
+    @property
+    def n_meshes(self):
+        return self._nm
+
+    @property
+    def n_domains(self):
+        return self._nd
+
     def tri_first_index(self, ref_rows):
         """Map reference-order unknown indices to triangles-first order."""
         r = np.asarray(ref_rows, dtype=np.int64)

@@ -237,6 +237,92 @@ int omh_geometry_domain_name(const omh_geometry* g, int32_t index, char* buf, in
     });
 }
 
+int omh_geometry_set_conductivity(omh_geometry* g, int32_t domain, double value) {
+    return guard([&] {
+        need(g, "geometry");
+        g->g->set_conductivity(domain, value);
+    });
+}
+
+int omh_geometry_pair_count(const omh_geometry* g, int32_t* count) {
+    return guard([&] {
+        need(g, "geometry");
+        need(count, "count");
+        *count = (int32_t)g->g->pairs.size();
+    });
+}
+
+int omh_geometry_pair_info(const omh_geometry* g, int32_t i, int32_t* meshes, int32_t* orientation, double* coeffs,
+                           int32_t* domain_mask) {
+    return guard([&] {
+        need(g, "geometry");
+        const auto& G = *g->g;
+        if (i < 0 || i >= (int)G.pairs.size()) throw std::invalid_argument("pair index out of range");
+        const auto& p = G.pairs[i];
+        if (meshes) { meshes[0] = p.m1; meshes[1] = p.m2; }
+        if (orientation) *orientation = p.orientation;
+        if (coeffs) { coeffs[0] = p.cS; coeffs[1] = p.cN; coeffs[2] = p.cD; }
+        if (domain_mask) {
+            int32_t mask = 0;
+            for (int d : G.common_domains(p.m1, p.m2)) {
+                if (d >= 31) throw std::invalid_argument("domain masks hold at most 31 domains");
+                mask |= (1 << d);
+            }
+            *domain_mask = mask;
+        }
+    });
+}
+
+int omh_geometry_mesh_info(const omh_geometry* g, int32_t m, int64_t* info) {
+    return guard([&] {
+        need(g, "geometry");
+        need(info, "info");
+        const auto& G = *g->g;
+        if (m < 0 || m >= (int)G.meshes.size()) throw std::invalid_argument("mesh index out of range");
+        const auto& M = G.meshes[m];
+        int64_t v0 = -1, v1 = -1;
+        bool contiguous = true;
+        for (size_t k = 0; k < M.verts.size(); ++k) {
+            const int64_t u = G.vert_index[M.verts[k]];
+            if (k == 0) v0 = u;
+            else if (u != v0 + (int64_t)k) contiguous = false;
+            v1 = u + 1;
+        }
+        info[0] = M.tri_first_index;
+        info[1] = (int64_t)M.tris.size();
+        info[2] = contiguous ? v0 : -1;
+        info[3] = contiguous ? v1 : -1;
+        info[4] = (int64_t)M.verts.size();
+        info[5] = (M.current_barrier ? 1 : 0) | (M.isolated ? 2 : 0) | (M.outermost ? 4 : 0);
+        // part and deflation pivot (assembleHeadMat.cpp:28-60: the first
+        // vertex of the first outermost mesh of the part)
+        info[6] = -1;
+        info[7] = -1;
+        for (size_t p = 0; p < G.parts.size(); ++p) {
+            int64_t i_first = 0;
+            int pivot = -1;
+            for (int mid : G.parts[p]) {
+                if (mid == m) info[6] = (int64_t)p;
+                const auto& Q = G.meshes[mid];
+                if (Q.outermost && i_first == 0 && !Q.verts.empty()) {
+                    i_first = G.vert_index[Q.verts.front()];
+                    pivot = mid;
+                }
+            }
+            if (pivot == m) info[7] = i_first;
+        }
+    });
+}
+
+int omh_geometry_domain_conductivity(const omh_geometry* g, int32_t domain, double* value) {
+    return guard([&] {
+        need(g, "geometry");
+        need(value, "value");
+        if (domain < 0 || domain >= (int)g->g->domains.size()) throw std::invalid_argument("domain index out of range");
+        *value = g->g->domains[domain].cond;
+    });
+}
+
 int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int ordering, void* stream) {
     return guard([&] {
         need(g, "geometry");
@@ -273,6 +359,27 @@ int omh_dip_source_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, 
         }
         omh::dip_source_mat(*g->g, d_dipoles, n_dip, dom, order, levels, tol, d_out, ld,
                             layout_of(*g->g, ld, ordering), S(stream));
+    });
+}
+
+int omh_dip_source_adjoint(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains, int order,
+                           const double* d_W, int64_t ldw, int ordering, double* d_grad, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_W, "d_W");
+        need(d_grad, "d_grad");
+        if (n_dip < 0) throw std::invalid_argument("negative number of dipoles");
+        if (n_dip > 0) need(d_dipoles, "d_dipoles");
+        std::vector<int> dom;
+        if (h_domains) {
+            dom.assign(h_domains, h_domains + n_dip);
+            for (int d : dom)
+                if (d < 0 || d >= (int)g->g->domains.size()) throw std::invalid_argument("domain index out of range");
+        } else {
+            dom = omh::locate_domains(*g->g, d_dipoles, 6, n_dip, S(stream));
+        }
+        omh::dip_source_adjoint(*g->g, d_dipoles, n_dip, dom, order, d_W, ldw, layout_of(*g->g, ldw, ordering), d_grad,
+                                S(stream));
     });
 }
 

@@ -325,6 +325,14 @@ void Geometry::make_pairs() {
     }
 }
 
+void Geometry::set_conductivity(int d, double value) {
+    if (d < 0 || d >= (int)domains.size()) throw std::invalid_argument("domain index out of range");
+    if (!(value > 0.0)) throw std::invalid_argument("conductivity must stay positive");
+    if (almost_equal(domains[d].cond, 0.0)) throw std::invalid_argument("cannot change a non-conductive domain");
+    domains[d].cond = value;
+    make_pairs();
+}
+
 void Geometry::finalize() {   // geometry.h:228-249
     bool has_cond = true;
     for (const auto& d : domains) has_cond = has_cond && d.cond != -1.0;

@@ -83,6 +83,11 @@ struct Geometry {
     int domain_index(const std::string& name) const;
     void finalize();
 
+    // Changes the conductivity of a domain and recomputes the pair
+    // coefficients.  A zero conductivity changes the structure (current
+    // barriers, unknowns) and is refused here.
+    void set_conductivity(int domain, double value);
+
     // queries
     std::vector<int> domains_of(int m) const;
     std::vector<int> common_domains(int m1, int m2) const;
@@ -99,6 +104,8 @@ private:
     void mark_current_barriers();
     void check_nested();
     void generate_indices();
+
+public:
     void make_pairs();
 };
 

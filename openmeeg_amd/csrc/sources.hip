@@ -357,6 +357,87 @@ __global__ void __launch_bounds__(256) k_closest(const TriRec* __restrict__ tri,
     }
 }
 
+// Adjoint of the dipole right-hand side with respect to the dipole
+// parameters.  For one (triangle t of a mesh, dipole s) pair, the
+// non-adaptive rule integrates the derivatives of the two integrands
+// (assembleSourceMat.cpp:84-99 with levels = 0),
+//   V(r)   = q.x/|x|^3,                    x = r - r0   (dipole.h:29-35)
+//   g(r)   = (q.n)/|x|^3 - 3 (q.x)(x.n)/|x|^5           (dipole.h:38-46)
+// with respect to q and r0, weights them with the adjoint field W (the
+// potential row of t and the three vertex rows, times the right-hand-side
+// coefficients), and reduces the six sums over the triangles of the wave
+// into grad[s*6 + 0..2] (d/dr0) and grad[s*6 + 3..5] (d/dq).
+__global__ void __launch_bounds__(256) k_dsm_adjoint(const TriRec* __restrict__ tri, const int32_t* __restrict__ tri_vid,
+                                                     const int64_t* __restrict__ vert_unk, int64_t t0, int64_t nt,
+                                                     const double* __restrict__ dip, int64_t nd,
+                                                     const double* __restrict__ cD, const double* __restrict__ cP,
+                                                     int64_t nm, int mesh, int with_pot, int order,
+                                                     const double* __restrict__ W, int64_t ldw, Layout L,
+                                                     int64_t tri_unk0, double* __restrict__ grad) {
+    const int64_t nti = cdiv_dev(nt, 64);
+    const int64_t ts = blockIdx.x / nti, tb = blockIdx.x - ts * nti;
+    const int64_t t = tb * 64 + threadIdx.x % 64;
+    const int64_t s = ts * 4 + threadIdx.x / 64;
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    if (t < nt && s < nd) {
+        const double coefD = cD[s * nm + mesh], coefP = with_pot ? cP[s * nm + mesh] : 0.0;
+        if (coefD != 0.0 || coefP != 0.0) {
+            const TriRec& T = tri[t0 + t];
+            const d3 P[3] = {ld3(T.p[0]), ld3(T.p[1]), ld3(T.p[2])};
+            const d3 n = ld3(T.n);
+            const d3 r0 = ld3(dip + 6 * s), q = ld3(dip + 6 * s + 3);
+            const double wp = with_pot ? W[dev_index(L, tri_unk0 + t) + s * ldw] * coefP : 0.0;
+            double wv[3];
+            for (int k = 0; k < 3; ++k) {
+                const int64_t u = vert_unk[tri_vid[3 * (t0 + t) + k]];
+                wv[k] = u < 0 ? 0.0 : W[dev_index(L, u) + s * ldw] * coefD;
+            }
+            // barycentric coordinates (BarycentricCoordinates.h:33-69)
+            const d3 e10 = sub(P[1], P[0]), e20 = sub(P[2], P[0]);
+            const double c11 = dot(e10, e10), c12 = dot(e10, e20), c22 = dot(e20, e20);
+            const double inv = 1.0 / (c11 * c22 - c12 * c12);
+            const double d11 = c22 * inv, d12 = -c12 * inv, d22 = c11 * inv;
+            const int np = c_nb_pts[order];
+            double acc[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < np; ++i) {
+                const d3 r = gauss_point(order, i, P);
+                const d3 x = sub(r, r0);
+                const double i2 = 1.0 / dot(x, x);
+                const double i3 = i2 * sqrt(i2), i5 = i3 * i2, i7 = i5 * i2;
+                const double qx = dot(q, x), xn = dot(x, n), qn = dot(q, n);
+                const d3 v = sub(r, P[0]);
+                const double a = dot(v, e10), b = dot(v, e20);
+                const double l1 = d11 * a + d12 * b, l2 = d12 * a + d22 * b, l0 = 1.0 - l1 - l2;
+                const double wsum = wv[0] * l0 + wv[1] * l1 + wv[2] * l2;   // P1-weighted adjoint
+                // dV/dq = x/|x|^3 ; dV/dr0 = -(q/|x|^3 - 3 (q.x) x/|x|^5)
+                // dg/dq = n/|x|^3 - 3 (x.n) x/|x|^5
+                // dg/dr0 = 3 (q.n) x/|x|^5 + 3 ((x.n) q + (q.x) n)/|x|^5 - 15 (q.x)(x.n) x/|x|^7
+                const double cx = 3 * qx * i5;
+                const double cg = 3 * qn * i5 - 15 * qx * xn * i7;
+                const double w = c_rules[order][i][3];
+                const double xs[3] = {x.x, x.y, x.z}, qs[3] = {q.x, q.y, q.z}, ns[3] = {n.x, n.y, n.z};
+                for (int c = 0; c < 3; ++c) {
+                    const double dV_dr0 = -(qs[c] * i3 - cx * xs[c]);
+                    const double dg_dr0 = cg * xs[c] + 3 * (xn * qs[c] + qx * ns[c]) * i5;
+                    const double dV_dq = xs[c] * i3;
+                    const double dg_dq = ns[c] * i3 - 3 * xn * xs[c] * i5;
+                    acc[c] += w * (wp * dV_dr0 + wsum * dg_dr0);
+                    acc[3 + c] += w * (wp * dV_dq + wsum * dg_dq);
+                }
+            }
+            for (int c = 0; c < 6; ++c) g[c] = T.area * acc[c];
+        }
+    }
+    // wave reduction (one dipole per wave), then one atomic per component
+    for (int c = 0; c < 6; ++c) {
+        double v = g[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        g[c] = v;
+    }
+    if ((threadIdx.x & 63) == 0 && s < nd)
+        for (int c = 0; c < 6; ++c) atomicAdd(&grad[s * 6 + c], g[c]);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -400,18 +481,14 @@ std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride
     return dom;
 }
 
-void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
-                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st) {
-    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
-    if (levels < 0 || levels >= MAX_FRAMES) throw std::invalid_argument("adaptive levels must be 0..11");
-    DeviceGeometry& dg = device_geometry(g);
-    const int64_t n = g.size();
+// Per (dipole, mesh) coefficients of assembleSourceMat.cpp:84-99: cD of the
+// P1 derivative rows, cP of the P0 potential rows (zero on current barriers).
+static void rhs_coefficients(const Geometry& g, int64_t n_dip, const std::vector<int>& domain, std::vector<double>& cD,
+                             std::vector<double>& cP, std::vector<char>& used) {
     const int nm = (int)g.meshes.size();
-    fill_f64(d_out, n, n_dip, ld_out, 0.0, st);
-    if (n_dip == 0) return;
-    // per (dipole, mesh) coefficients of assembleSourceMat.cpp:84-99
-    std::vector<double> cD((size_t)n_dip * nm, 0.0), cP((size_t)n_dip * nm, 0.0);
-    std::vector<char> used(nm, 0);
+    cD.assign((size_t)n_dip * nm, 0.0);
+    cP.assign((size_t)n_dip * nm, 0.0);
+    used.assign(nm, 0);
     for (int64_t s = 0; s < n_dip; ++s) {
         const Domain& d = g.domains.at(domain[s]);
         if (d.cond == 0.0) continue;
@@ -425,6 +502,20 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
             }
         }
     }
+}
+
+void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    if (levels < 0 || levels >= MAX_FRAMES) throw std::invalid_argument("adaptive levels must be 0..11");
+    DeviceGeometry& dg = device_geometry(g);
+    const int64_t n = g.size();
+    const int nm = (int)g.meshes.size();
+    fill_f64(d_out, n, n_dip, ld_out, 0.0, st);
+    if (n_dip == 0) return;
+    std::vector<double> cD, cP;
+    std::vector<char> used;
+    rhs_coefficients(g, n_dip, domain, cD, cP, used);
     double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
     OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
@@ -449,6 +540,32 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
                                                        dg.slot_vid, dg.vert_unk, md.slot_begin, nsl, d_out, ld_out, L);
             OMH_LAUNCH_CHECK();
         }
+    }
+}
+
+void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                        const double* d_W, int64_t ldw, const Layout& L, double* d_grad, hipStream_t st) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    DeviceGeometry& dg = device_geometry(g);
+    const int nm = (int)g.meshes.size();
+    fill_f64(d_grad, 6, n_dip, 6, 0.0, st);
+    if (n_dip == 0) return;
+    std::vector<double> cD, cP;
+    std::vector<char> used;
+    rhs_coefficients(g, n_dip, domain, cD, cP, used);
+    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
+    OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    for (int m = 0; m < nm; ++m) {
+        if (!used[m]) continue;
+        const MeshDev& md = dg.meshes[m];
+        const int64_t nt = md.tri_end - md.tri_begin;
+        const int64_t blocks = cdiv(nt, 64) * cdiv(n_dip, 4);
+        k_dsm_adjoint<<<(unsigned)blocks, 256, 0, st>>>(dg.tri, dg.tri_vid, dg.vert_unk, md.tri_begin, nt, d_dip, n_dip,
+                                                        d_c, d_c + cD.size(), nm, m, md.barrier ? 0 : 1, order, d_W,
+                                                        ldw, L, md.tri_unk0, d_grad);
+        OMH_LAUNCH_CHECK();
     }
 }
 

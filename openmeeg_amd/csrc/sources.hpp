@@ -22,6 +22,13 @@ std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride
 void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
                     int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st);
 
+// Adjoint of SourceMatrix<Dipole> with the non-adaptive rule of `order`:
+// d_grad (n_dip x 6, row-major) = for each dipole s the gradient of
+// W(:,s) . rhs_s with respect to (x y z) and (qx qy qz), W being n x n_dip
+// (column-major, leading dimension ldw, rows in the unknown order of L).
+void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                        const double* d_W, int64_t ldw, const Layout& L, double* d_grad, hipStream_t st);
+
 // Head2EEGMat (assembleSensors.cpp:25-39): for each electrode (d_el, n_el x 3
 // row-major on the device) the three vertex unknowns (reference order) of the
 // closest triangle and their barycentric weights (host arrays n_el x 3).

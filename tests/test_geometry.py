@@ -79,3 +79,58 @@ def test_integrator_safe_order():
     assert (i.levels, i.tolerance) == (10, 0.001)
     i = om.Integrator(3, 0, 0.005)
     assert (i.order, i.levels, i.tolerance) == (3, 0, 0.005)
+
+
+def test_conductivity_change_matches_fresh_geometry():
+    """set_conductivity recomputes the pair coefficients exactly as a
+    geometry built with the new conductivities (oracle restatement of
+    geometry.cpp:405-420 and assembleHeadMat.cpp:92-97)."""
+    meshes = S.nested_sphere_meshes(4, (0.87, 0.92, 1.0))
+    g = om.make_nested_geometry(meshes, (1.0, 0.0125, 1.0))
+    names = g.domain_names()
+    layer_cond = dict(Brain=1.0, Skull=0.0125, Scalp=1.0, Air=0.0)
+    np.testing.assert_array_equal(g.conductivities(), [layer_cond[k] for k in names])
+    new = (0.7, 0.02, 1.3)
+    for name, v in zip(("Brain", "Skull", "Scalp"), new):
+        g.set_conductivity(name, v)
+    og = O.nested_geometry(meshes, new)
+    got = [(p["meshes"][0], p["meshes"][1], p["orientation"]) + p["coeffs"] for p in g.mesh_pairs()]
+    assert len(got) == len(og.pairs)
+    for a, b in zip(got, og.pairs):
+        assert a[:3] == tuple(b[:3])
+        np.testing.assert_allclose(a[3:], b[3:6], rtol=1e-15, atol=0)
+    with pytest.raises(ValueError):
+        g.set_conductivity(0, -1.0)
+    with pytest.raises(ValueError):
+        g.set_conductivity("Air", 1.0)   # the non-conductive outer domain
+    with pytest.raises(ValueError):
+        g.set_conductivity(7, 1.0)
+
+
+def test_mesh_info_layout():
+    """Per-mesh unknown ranges tile the unknowns in the reference order
+    (geometry.cpp:256-295): vertices of every mesh first, then the triangles
+    of non-barrier meshes; the outer mesh of a nested head is the outermost
+    current barrier and holds the deflation pivot (its first vertex)."""
+    meshes = S.nested_sphere_meshes(4, (0.87, 0.92, 1.0))
+    g = om.make_nested_geometry(meshes, (1.0, 0.0125, 1.0))
+    info = [g.mesh_info(m) for m in range(g.n_meshes)]
+    v = 0
+    for M in info:
+        assert (M["vert_first"], M["vert_end"]) == (v, v + M["nverts"])
+        v = M["vert_end"]
+    assert v == g.n_vertex_unknowns
+    t = v
+    for M in info:
+        if not M["current_barrier"]:
+            assert M["tri_first"] == t
+            t += M["ntris"]
+    assert t == g.size
+    assert [M["current_barrier"] for M in info] == [False, False, True]
+    assert [M["outermost"] for M in info] == [False, False, True]
+    assert [M["deflation_pivot"] for M in info] == [-1, -1, info[2]["vert_first"]]
+    assert len({M["part"] for M in info}) == 1
+    pairs = g.mesh_pairs()
+    assert sorted(p["meshes"] for p in pairs) == [(0, 0), (1, 0), (1, 1), (2, 1), (2, 2)]
+    for p in pairs:
+        assert p["domains"] == sorted(set(p["domains"]))

@@ -1,0 +1,131 @@
+"""Gradients of the EEG gain (EEGForward) on the GPU.
+
+The forward gain is checked against the oracle (same non-adaptive source
+rule, 1e-10).  The adjoint gradients of a fixed linear functional
+L(G) = sum(R * G) with respect to dipole positions, moments and domain
+conductivities are checked against central finite differences of the GPU
+forward: moments exactly (G is linear in them, 1e-9), positions and
+conductivities with steps of 1e-5 (relative), where the truncation error is
+O(h^2) and the tolerance 1e-6 relative to the gradient norm.  Mixed and FP64
+gradients agree to 1e-6.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAYERS = ((0.87, 0.92, 1.0), (1.0, 0.0125, 1.0))
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openmeeg_amd as om
+    from openmeeg_amd import spheres as S
+    meshes = S.nested_sphere_meshes(4, LAYERS[0])
+    rng = np.random.default_rng(11)
+    dip = S.random_dipoles(rng, 6, 0.78)
+    el = S.sphere_electrodes(rng, 16)
+    g = om.make_nested_geometry(meshes, LAYERS[1])
+    R = rng.standard_normal((len(el), len(dip)))
+    return om, meshes, dip, el, g, R
+
+
+def _loss_and_grads(model, dip, R, sigma=None):
+    import torch
+    pos = torch.tensor(dip[:, :3], requires_grad=True)
+    mom = torch.tensor(dip[:, 3:], requires_grad=True)
+    sig = torch.tensor(model.conductivities() if sigma is None else sigma, requires_grad=True)
+    G = model.gain(pos, mom, sig)
+    L = (G.cpu() * torch.tensor(R)).sum()
+    L.backward()
+    return float(L.detach()), pos.grad.numpy().copy(), mom.grad.numpy().copy(), sig.grad.numpy().copy(), G
+
+
+def _loss(model, dip, R, sigma):
+    import torch
+    with torch.no_grad():
+        G = model.gain(torch.tensor(dip[:, :3]), torch.tensor(dip[:, 3:]), torch.tensor(sigma))
+    return float((G.cpu().numpy() * R).sum())
+
+
+def test_forward_matches_oracle(setup):
+    om, meshes, dip, el, g, R = setup
+    from oracle import oracle as O
+    model = om.EEGForward(g, el, precision="fp64")
+    _, _, _, _, G = _loss_and_grads(model, dip, R)
+    og = O.nested_geometry(meshes, LAYERS[1])
+    H = O.unpack(O.head_mat(og), og.size)
+    cols, vals = O.head2eeg(og, el)
+    Gref = O.gain_eeg(H, O.dip_source_mat(og, dip, "", levels=0), O.h2em_dense(og, cols, vals))
+    G = G.detach().cpu().numpy()
+    err = np.linalg.norm(G - Gref) / np.linalg.norm(Gref)
+    print("forward gain vs oracle %.3e" % err, flush=True)
+    assert err < 1e-10
+
+
+def test_gradients_match_finite_differences(setup):
+    om, meshes, dip, el, g, R = setup
+    model = om.EEGForward(g, el, precision="fp64")
+    sig0 = model.conductivities()
+    L0, gp, gm, gs, _ = _loss_and_grads(model, dip, R)
+    h = 1e-5
+    fp = np.zeros_like(gp)
+    for s in range(len(dip)):
+        for c in range(3):
+            d1, d2 = dip.copy(), dip.copy()
+            d1[s, c] += h
+            d2[s, c] -= h
+            fp[s, c] = (_loss(model, d1, R, sig0) - _loss(model, d2, R, sig0)) / (2 * h)
+    fm = np.zeros_like(gm)
+    for s in range(len(dip)):
+        for c in range(3):
+            d1, d2 = dip.copy(), dip.copy()
+            d1[s, 3 + c] += 1.0
+            d2[s, 3 + c] -= 1.0
+            fm[s, c] = (_loss(model, d1, R, sig0) - _loss(model, d2, R, sig0)) / 2.0
+    fs = np.zeros_like(gs)
+    for d in range(len(sig0)):
+        if sig0[d] == 0:
+            continue
+        hs = h * sig0[d]
+        s1, s2 = sig0.copy(), sig0.copy()
+        s1[d] += hs
+        s2[d] -= hs
+        fs[d] = (_loss(model, dip, R, s1) - _loss(model, dip, R, s2)) / (2 * hs)
+    ep = np.linalg.norm(gp - fp) / np.linalg.norm(fp)
+    em = np.linalg.norm(gm - fm) / np.linalg.norm(fm)
+    es = np.abs(gs - fs) / np.abs(fs).max()
+    print("position %.3e moment %.3e conductivity %s" % (ep, em, es), flush=True)
+    assert em < 1e-9
+    assert ep < 1e-6
+    assert es.max() < 1e-6
+    assert gs[list(sig0).index(0.0)] == 0.0
+
+
+def test_mixed_gradients_match_fp64(setup):
+    om, meshes, dip, el, g, R = setup
+    a = _loss_and_grads(om.EEGForward(g, el, precision="fp64"), dip, R)
+    b = _loss_and_grads(om.EEGForward(g, el, precision="mixed"), dip, R)
+    for x, y in zip(a[1:4], b[1:4]):
+        assert np.linalg.norm(x - y) / np.linalg.norm(x) < 1e-6
+
+
+def test_moved_conductivity_gradient(setup):
+    """Gradients at conductivities away from the geometry's initial ones
+    (the model changes the geometry's conductivities and its coefficients)."""
+    om, meshes, dip, el, g, R = setup
+    model = om.EEGForward(g, el, precision="fp64")
+    sig = model.conductivities()
+    live = sig > 0
+    sig[live] *= np.array([1.3, 0.7, 1.1])[: live.sum()]
+    _, _, _, gs, _ = _loss_and_grads(model, dip, R, sig)
+    h = 1e-5
+    for d in np.flatnonzero(live):
+        s1, s2 = sig.copy(), sig.copy()
+        s1[d] += h * sig[d]
+        s2[d] -= h * sig[d]
+        fd = (_loss(model, dip, R, s1) - _loss(model, dip, R, s2)) / (2 * h * sig[d])
+        assert abs(gs[d] - fd) < 1e-6 * np.abs(gs).max()
