@@ -95,7 +95,8 @@ class EEGForward:
         sigma = np.asarray(sigma, dtype=np.float64)
         if sigma.shape != self._sigma.shape:
             raise ValueError("expected %d conductivities" % len(self._sigma))
-        for d, (old, new) in enumerate(zip(self._sigma, sigma)):
+        # compare with the geometry itself: another model may share it
+        for d, (old, new) in enumerate(zip(self.geom.conductivities(), sigma)):
             if new != old:
                 if old == 0.0:
                     raise ValueError("the conductivity of non-conductive domain %d cannot change" % d)

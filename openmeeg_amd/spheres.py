@@ -187,6 +187,12 @@ def make_config(name, **over):
     return meshes, tuple(c["cond"]), dip, el
 
 
+def noisy_targets(G, rng, snr=10.0):
+    """Measurement targets G + noise with sigma_noise = rms(G) / snr."""
+    G = np.asarray(G, dtype=np.float64)
+    return G + rng.standard_normal(G.shape) * (np.sqrt(np.mean(G * G)) / snr)
+
+
 def make_batched_heads(n_heads=32, seed=4, freqs=(8, 10, 12, 14, 16)):
     """Uneven perturbed 3-layer heads: list of (meshes, cond, dipoles, electrodes)."""
     rng = np.random.default_rng(seed)
