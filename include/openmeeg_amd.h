@@ -199,6 +199,42 @@ int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ld
 int omh_profile_enable(int32_t on);
 int omh_profile_read(const char* name, int64_t* calls, double* seconds, double* work);
 
+/* Batched EEG gains (no reference counterpart; the reference computes one
+ * head per GainEEGadjoint call, gain.h:78-89).  A batch holds n_heads heads,
+ * head h being geometry geoms[h] (kept alive by the caller) with n_dipoles[h]
+ * dipoles (h_dipoles[h]: n x 6 host array, x y z qx qy qz, each located in
+ * its domain) and n_electrodes[h] electrodes (h_electrodes[h]: n x 3).
+ * precision: 0 FP64 factor and solve, 1 FP32 factor with FP64 refinement.
+ * A pass assembles HeadMat and DipSourceMat (Integrator(3,10,0.001)) of
+ * every head and computes G = (H^-1 E^T)^T DSM with the dense algebra of
+ * all heads grouped in lock step.
+ *
+ * omh_batch_run: one pass after the work on `stream`.  The first pass (and
+ * every pass with use_graph = 0) runs eagerly and fixes the number of
+ * refinement steps; with use_graph = 1 later passes replay one captured HIP
+ * graph of the whole pass (*used_graph tells which ran).
+ * omh_batch_tune: times the candidate tile sizes (64, 128) and grid caps of
+ * the grouped trailing update on the batch's own shapes and keeps the
+ * fastest (n_results x {tile, grid_cap, seconds} written to h_results if not
+ * NULL, at most max_results); the next pass runs eagerly again.
+ * omh_batch_gain: device pointer to head h's gain (n_el x n_dip,
+ * column-major), valid until omh_batch_free.
+ * omh_batch_stats: factor status (0 or 1 + first vanishing pivot) and the
+ * last relative refinement update of head h.
+ * omh_batch_config: refinement steps, trailing tile and grid cap, grouped
+ * launches per pass and nodes of the captured graph (0 before capture). */
+typedef struct omh_batch omh_batch;
+int omh_batch_create(int32_t n_heads, omh_geometry* const* geoms, const double* const* h_dipoles,
+                     const int64_t* n_dipoles, const double* const* h_electrodes, const int64_t* n_electrodes,
+                     int32_t precision, omh_batch** out);
+int omh_batch_run(omh_batch* b, int32_t use_graph, int32_t* used_graph, void* stream);
+int omh_batch_tune(omh_batch* b, double* h_results, int32_t max_results, int32_t* n_results, void* stream);
+int omh_batch_gain(const omh_batch* b, int32_t head, const double** d_gain, int64_t* n_el, int64_t* n_dip);
+int omh_batch_stats(omh_batch* b, int32_t head, int32_t* factor_info, double* rel_update, void* stream);
+int omh_batch_config(const omh_batch* b, int32_t* refine_steps, int32_t* tile, int64_t* grid_cap,
+                     int64_t* launches, int64_t* graph_nodes);
+int omh_batch_free(omh_batch* b);
+
 /* Frees the library's cached device scratch buffers. */
 int omh_release_scratch(void);
 

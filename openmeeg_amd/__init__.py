@@ -5,7 +5,8 @@ Drop-in names of the reference's Python module ``openmeeg``
 HeadMat, DipSourceMat, Head2EEGMat, GainEEG, GainEEGadjoint, Integrator,
 Matrix, SymMatrix, Sensors, make_geometry, make_nested_geometry,
 read_geometry; plus EEGForward, the EEG gain differentiable in the dipoles
-and the conductivities (no reference counterpart).  All computation runs in the HIP library
+and the conductivities, and BatchedGain, the gains of many heads in one
+graph-captured pass (no reference counterparts).  All computation runs in the HIP library
 (libopenmeeg_amd.so) through its C ABI (include/openmeeg_amd.h).
 """
 from .api import (  # noqa: F401
@@ -31,3 +32,4 @@ from .api import (  # noqa: F401
 
 __version__ = "0.1.0"
 from .differentiable import EEGForward  # noqa: F401,E402
+from .batch import BatchedGain  # noqa: F401,E402

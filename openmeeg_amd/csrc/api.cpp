@@ -11,6 +11,7 @@
 
 #include "../../include/openmeeg_amd.h"
 #include "assembly.hpp"
+#include "batch.hpp"
 #include "common.hpp"
 #include "geometry.hpp"
 #include "layout.hpp"
@@ -19,6 +20,10 @@
 
 struct omh_geometry {
     std::unique_ptr<omh::Geometry> g;
+};
+
+struct omh_batch {
+    std::unique_ptr<omh::BatchPlan> plan;
 };
 
 namespace {
@@ -536,6 +541,97 @@ int omh_permute_rows(const double* d_src, int64_t lds, double* d_dst, int64_t ld
         omh::permute_rows(d_src, lds, d_dst, ldd, n, ncol, raw_layout(n, nv, n, ordering), to_reference != 0,
                           S(stream));
     });
+}
+
+int omh_batch_create(int32_t n_heads, omh_geometry* const* geoms, const double* const* h_dipoles,
+                     const int64_t* n_dipoles, const double* const* h_electrodes, const int64_t* n_electrodes,
+                     int32_t precision, omh_batch** out) {
+    return guard([&] {
+        need(out, "out");
+        *out = nullptr;
+        if (n_heads <= 0) throw std::invalid_argument("a batch needs at least one head");
+        need(geoms, "geoms");
+        need(h_dipoles, "h_dipoles");
+        need(n_dipoles, "n_dipoles");
+        need(h_electrodes, "h_electrodes");
+        need(n_electrodes, "n_electrodes");
+        std::vector<omh::BatchHeadInput> in(n_heads);
+        for (int h = 0; h < n_heads; ++h) {
+            need(geoms[h], "geometry");
+            if (n_dipoles[h] <= 0 || n_electrodes[h] <= 0)
+                throw std::invalid_argument("each head needs dipoles and electrodes");
+            need(h_dipoles[h], "dipoles");
+            need(h_electrodes[h], "electrodes");
+            in[h].g = geoms[h]->g.get();
+            in[h].dipoles.assign(h_dipoles[h], h_dipoles[h] + 6 * n_dipoles[h]);
+            in[h].electrodes.assign(h_electrodes[h], h_electrodes[h] + 3 * n_electrodes[h]);
+        }
+        auto b = std::make_unique<omh_batch>();
+        b->plan = std::make_unique<omh::BatchPlan>(in, precision);
+        *out = b.release();
+    });
+}
+
+int omh_batch_run(omh_batch* b, int32_t use_graph, int32_t* used_graph, void* stream) {
+    return guard([&] {
+        need(b, "batch");
+        const bool g = b->plan->run(S(stream), use_graph != 0);
+        if (used_graph) *used_graph = g ? 1 : 0;
+    });
+}
+
+int omh_batch_tune(omh_batch* b, double* h_results, int32_t max_results, int32_t* n_results, void* stream) {
+    return guard([&] {
+        need(b, "batch");
+        const auto res = b->plan->tune(S(stream));
+        if (n_results) *n_results = (int32_t)res.size();
+        if (h_results)
+            for (int i = 0; i < (int)res.size() && i < max_results; ++i) {
+                h_results[3 * i] = res[i].tile;
+                h_results[3 * i + 1] = (double)res[i].grid_cap;
+                h_results[3 * i + 2] = res[i].seconds;
+            }
+    });
+}
+
+int omh_batch_gain(const omh_batch* b, int32_t head, const double** d_gain, int64_t* n_el, int64_t* n_dip) {
+    return guard([&] {
+        need(b, "batch");
+        need(d_gain, "d_gain");
+        if (head < 0 || head >= b->plan->n_heads()) throw std::invalid_argument("head index out of range");
+        int64_t ne = 0, nd = 0;
+        *d_gain = b->plan->gain(head, ne, nd);
+        if (n_el) *n_el = ne;
+        if (n_dip) *n_dip = nd;
+    });
+}
+
+int omh_batch_stats(omh_batch* b, int32_t head, int32_t* factor_info, double* rel_update, void* stream) {
+    return guard([&] {
+        need(b, "batch");
+        if (head < 0 || head >= b->plan->n_heads()) throw std::invalid_argument("head index out of range");
+        int fi = 0;
+        double ru = 0.0;
+        b->plan->stats(head, fi, ru, S(stream));
+        if (factor_info) *factor_info = fi;
+        if (rel_update) *rel_update = ru;
+    });
+}
+
+int omh_batch_config(const omh_batch* b, int32_t* refine_steps, int32_t* tile, int64_t* grid_cap, int64_t* launches,
+                     int64_t* graph_nodes) {
+    return guard([&] {
+        need(b, "batch");
+        if (refine_steps) *refine_steps = b->plan->refine_steps();
+        if (tile) *tile = b->plan->trailing_tile();
+        if (grid_cap) *grid_cap = b->plan->trailing_grid_cap();
+        if (launches) *launches = b->plan->launches();
+        if (graph_nodes) *graph_nodes = b->plan->graph_nodes();
+    });
+}
+
+int omh_batch_free(omh_batch* b) {
+    return guard([&] { delete b; });
 }
 
 int omh_release_scratch(void) {

@@ -46,20 +46,46 @@ std::mutex g_arena_mu;
 std::map<int, Arena> g_arenas;
 }  // namespace
 
+thread_local ScratchArena* tl_arena = nullptr;   // ArenaScope override of this thread
+
+static void* grow(void*& p, size_t& sz, size_t bytes) {
+    OMH_HIP(hipDeviceSynchronize());
+    if (p) OMH_HIP(hipFree(p));
+    p = nullptr;
+    sz = 0;
+    const size_t want = (bytes + 4095) & ~size_t(4095);
+    OMH_HIP(hipMalloc(&p, want));
+    sz = want;
+    return p;
+}
+
+void ScratchArena::release() {
+    for (int s = 0; s < SCR_NSLOTS; ++s)
+        if (p[s]) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p[s]);
+            p[s] = nullptr;
+            sz[s] = 0;
+        }
+}
+
+ArenaScope::ArenaScope(ScratchArena* a) : prev_(tl_arena) { tl_arena = a; }
+ArenaScope::~ArenaScope() { tl_arena = prev_; }
+
 void* scratch(ScratchSlot slot, size_t bytes) {
+    if (tl_arena) {
+        ScratchArena& a = *tl_arena;
+        if (a.sz[slot] < bytes) {
+            if (a.frozen) throw std::runtime_error("frozen scratch arena is too small (plan sizes changed)");
+            grow(a.p[slot], a.sz[slot], bytes);
+        }
+        return a.p[slot];
+    }
     int dev = 0;
     OMH_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena& a = g_arenas[dev];
-    if (a.sz[slot] < bytes) {
-        OMH_HIP(hipDeviceSynchronize());
-        if (a.p[slot]) OMH_HIP(hipFree(a.p[slot]));
-        a.p[slot] = nullptr;
-        a.sz[slot] = 0;
-        const size_t want = (bytes + 4095) & ~size_t(4095);
-        OMH_HIP(hipMalloc(&a.p[slot], want));
-        a.sz[slot] = want;
-    }
+    if (a.sz[slot] < bytes) grow(a.p[slot], a.sz[slot], bytes);
     return a.p[slot];
 }
 

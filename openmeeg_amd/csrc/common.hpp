@@ -32,6 +32,27 @@ enum ScratchSlot { SCR_ASSEMBLY = 0, SCR_DTMP = 1, SCR_SOURCES = 2, SCR_LINALG =
 void* scratch(ScratchSlot slot, size_t bytes);
 void release_scratch();
 
+// A private set of scratch slots (e.g. owned by a batch plan whose launches
+// are captured into a graph: the buffers must not move once captured).  While
+// an ArenaScope is alive, scratch() on this thread serves from it; a frozen
+// arena refuses to grow.
+struct ScratchArena {
+    void* p[SCR_NSLOTS] = {};
+    size_t sz[SCR_NSLOTS] = {};
+    bool frozen = false;
+    void release();
+    ~ScratchArena() { release(); }
+};
+class ArenaScope {
+public:
+    explicit ArenaScope(ScratchArena* a);
+    ~ArenaScope();
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+private:
+    ScratchArena* prev_;
+};
+
 // Fill kernel launcher (a kernel, not hipMemsetAsync, so that it is safe in
 // captured graphs).
 void fill_f64(double* p, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s);

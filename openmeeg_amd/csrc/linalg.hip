@@ -29,6 +29,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -131,13 +132,14 @@ struct TileLoader {
     }
 };
 
-// Tile enumeration of a launch: 0 full grid (ntm row tiles), 1 lower
-// triangle of tiles, 2 one row strip (tile row tm0), 3 one column strip
-// (tile column tn0, rows below tm_lim excluded).
-struct TileMap {
-    int kind;
-    int64_t ntm, tm0, tn0;
-};
+__device__ __forceinline__ void map_tile(const TileMap& map, int64_t t, int64_t& tm, int64_t& tn) {
+    switch (map.kind) {
+        case 1: upper_tile(t, tn, tm); break;   // tn <= tm
+        case 2: tm = map.tm0; tn = t; break;
+        case 3: tn = map.tn0; tm = t; break;
+        default: tn = t / map.ntm; tm = t - tn * map.ntm; break;
+    }
+}
 
 // C = alpha * op(A) * op(B) + beta * C on a (32 WB) x (32 WB) tile per
 // workgroup of 256 threads (2 x 2 waves, each 16 WB x 16 WB = WB x WB MFMA
@@ -149,19 +151,12 @@ struct TileMap {
 // are 16-byte aligned, so the k loop uses unguarded vector loads (a partial
 // last k-slice is done after the loop).  EDGE = true: guarded loads.
 template <typename T, bool TA, bool TB, bool LOWER, int WB, bool EDGE>
-__global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
-                                              int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
-                                              T* __restrict__ C, int64_t ldc, TileMap map) {
+__device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
+                                          int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
+                                          T* __restrict__ C, int64_t ldc, int64_t tm, int64_t tn) {
     constexpr int BM = 32 * WB, BN = 32 * WB;
     __shared__ T As[2][BK * (BM + LDS_PAD)];
     __shared__ T Bs[2][BK * (BN + LDS_PAD)];
-    int64_t tm, tn;
-    switch (map.kind) {
-        case 1: upper_tile(blockIdx.x, tn, tm); break;   // tn <= tm
-        case 2: tm = map.tm0; tn = blockIdx.x; break;
-        case 3: tn = map.tn0; tm = blockIdx.x; break;
-        default: tn = blockIdx.x / map.ntm; tm = blockIdx.x - tn * map.ntm; break;
-    }
     const int64_t row0 = tm * BM, col0 = tn * BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -267,6 +262,36 @@ __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T
         }
 }
 
+template <typename T, bool TA, bool TB, bool LOWER, int WB, bool EDGE>
+__global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
+                                              int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
+                                              T* __restrict__ C, int64_t ldc, TileMap map) {
+    int64_t tm, tn;
+    map_tile(map, blockIdx.x, tm, tn);
+    gemm_tile<T, TA, TB, LOWER, WB, EDGE>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+}
+
+// Grouped GEMM: the tiles of ng independent products (descriptor d[g], its
+// tiles numbered from d[g].tile0) in one launch; each workgroup walks the
+// tile list with stride gridDim.x (persistent when the grid is capped).
+template <typename T, bool TA, bool TB, bool LOWER, int WB, bool EDGE>
+__global__ void __launch_bounds__(256) k_gemm_grouped(const GemmDesc* __restrict__ d, int ng, int64_t total) {
+    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = ng - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (d[mid].tile0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GemmDesc& g = d[lo];
+        int64_t tm, tn;
+        map_tile(g.map, t - g.tile0, tm, tn);
+        __syncthreads();   // the previous tile's LDS reads are done
+        gemm_tile<T, TA, TB, LOWER, WB, EDGE>(g.m, g.n, g.k, (T)g.alpha, (const T*)g.A, g.lda, (const T*)g.B, g.ldb,
+                                              (T)g.beta, (T*)g.C, g.ldc, tm, tn);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // diagonal block: signed Cholesky A11 = L11 S1 L11^T and Linv = L11^-1.
 // One workgroup of 512 threads; the NB x NB block lives in global memory
@@ -280,9 +305,8 @@ constexpr int IB = 32;
 constexpr int DIAG_THREADS = 512;
 
 template <typename T>
-__global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
-                                                              T* __restrict__ Linv, int64_t ldl, T* __restrict__ sgn,
-                                                              int* __restrict__ info, int64_t k0, T tiny) {
+__device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv, int64_t ldl,
+                                          T* __restrict__ sgn, int* __restrict__ info, int64_t k0, T tiny) {
     __shared__ T D[IB][IB + 1];
     __shared__ T ds[IB];
     __shared__ T Pt[IB][FACT_NB - IB + 1];   // panel rows below the inner block, transposed
@@ -397,6 +421,50 @@ __global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A,
             }
         }
         __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
+                                                              T* __restrict__ Linv, int64_t ldl, T* __restrict__ sgn,
+                                                              int* __restrict__ info, int64_t k0, T tiny) {
+    diag_body<T>(A, lda, nb, Linv, ldl, sgn, info, k0, tiny);
+}
+
+// One workgroup per diagonal block of several factorisations.
+template <typename T>
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_grouped(const DiagDesc* __restrict__ d, T tiny) {
+    const DiagDesc& g = d[blockIdx.x];
+    diag_body<T>((T*)g.A11, g.lda, g.nb, (T*)g.Linv, FACT_NB, (T*)g.sgn, g.info, g.k0, tiny);
+}
+
+// Element-wise operations of several matrices (group = blockIdx.y).
+__global__ void __launch_bounds__(256) k_elem_grouped(const ElemDesc* __restrict__ d) {
+    const ElemDesc& g = d[blockIdx.y];
+    const int64_t total = g.m * g.n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / g.m, i = t - j * g.m;
+        const int64_t si = i + j * g.lds, di = i + j * g.ldd;
+        switch (g.op) {
+            case ELEM_COPY_F64: ((double*)g.dst)[di] = ((const double*)g.src)[si]; break;
+            case ELEM_COPY_F32: ((float*)g.dst)[di] = ((const float*)g.src)[si]; break;
+            case ELEM_F64_TO_F32: ((float*)g.dst)[di] = (float)((const double*)g.src)[si]; break;
+            case ELEM_F64_TO_F32_LOWER:
+                if (i >= j) ((float*)g.dst)[di] = (float)((const double*)g.src)[si];
+                break;
+            case ELEM_ADD_F32_TO_F64: ((double*)g.dst)[di] += (double)((const float*)g.src)[si]; break;
+            case ELEM_SCALE_ROWS_F32: ((float*)g.dst)[di] *= ((const float*)g.aux)[i]; break;
+            case ELEM_SCALE_ROWS_F64: ((double*)g.dst)[di] *= ((const double*)g.aux)[i]; break;
+            case ELEM_SCALE_COLS_F32:
+                ((float*)g.dst)[di] = ((const float*)g.src)[si] * ((const float*)g.aux)[j];
+                break;
+            case ELEM_SCALE_COLS_F64:
+                ((double*)g.dst)[di] = ((const double*)g.src)[si] * ((const double*)g.aux)[j];
+                break;
+            case ELEM_ZERO_F64: ((double*)g.dst)[di] = 0.0; break;
+            case ELEM_ZERO_INT: ((int*)g.dst)[di] = 0; break;
+            default: break;
+        }
     }
 }
 
@@ -563,6 +631,98 @@ template <typename T>
 static bool aligned_for_vec(const T* p, int64_t ld) {
     constexpr int V = 16 / sizeof(T);
     return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && (ld % V == 0);
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches
+// ---------------------------------------------------------------------------
+static bool aligned_bytes(const void* p, int64_t ld, size_t elem) {
+    return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((ld * (int64_t)elem) % 16 == 0);
+}
+
+void build_grouped_gemm(const std::vector<GemmJob>& jobs, bool f32, bool ta, bool tb, bool lower, int wb,
+                        std::vector<GemmDesc>& descs, std::vector<GemmLaunch>& out) {
+    if (wb != 2 && wb != 4) throw std::invalid_argument("tile factor must be 2 or 4");
+    if (!f32 && wb != 2) throw std::invalid_argument("FP64 grouped GEMM uses 64 x 64 tiles");
+    if (lower && (ta || !tb)) throw std::invalid_argument("lower products are X * Y^T only");
+    const int64_t TILE = 32 * wb;
+    const size_t elem = f32 ? 4 : 8;
+    std::vector<GemmDesc> in, ed;
+    int64_t ti = 0, te = 0;
+    auto add = [](std::vector<GemmDesc>& v, int64_t& tot, const GemmJob& j, TileMap map, int64_t ntiles) {
+        if (ntiles <= 0) return;
+        v.push_back(GemmDesc{j.A, j.B, j.C, j.lda, j.ldb, j.ldc, j.m, j.n, j.k, j.alpha, j.beta, map, tot});
+        tot += ntiles;
+    };
+    for (const GemmJob& j : jobs) {
+        if (j.m <= 0 || j.n <= 0) continue;
+        if (lower && j.m != j.n) throw std::invalid_argument("lower gemm needs a square output");
+        const int64_t ntm = cdiv(j.m, TILE), ntn = cdiv(j.n, TILE);
+        const bool vec = aligned_bytes(j.A, j.lda, elem) && aligned_bytes(j.B, j.ldb, elem);
+        if (!vec) {
+            add(ed, te, j, TileMap{lower ? 1 : 0, ntm, 0, 0}, lower ? ntm * (ntm + 1) / 2 : ntm * ntn);
+            continue;
+        }
+        const int64_t fm = j.m / TILE, fn = j.n / TILE;
+        if (lower) {
+            add(in, ti, j, TileMap{1, fm, 0, 0}, fm * (fm + 1) / 2);
+            if (fm < ntm) add(ed, te, j, TileMap{2, ntm, fm, 0}, ntm);
+        } else {
+            add(in, ti, j, TileMap{0, fm, 0, 0}, fm * fn);
+            if (fm < ntm) add(ed, te, j, TileMap{2, ntm, fm, 0}, ntn);
+            if (fn < ntn) add(ed, te, j, TileMap{3, ntm, 0, fn}, fm);
+        }
+    }
+    const int base = (f32 ? 1 : 0) | (ta ? 2 : 0) | (tb ? 4 : 0) | (lower ? 8 : 0) | (wb == 4 ? 16 : 0);
+    if (!in.empty()) {
+        out.push_back(GemmLaunch{base, descs.size(), (int)in.size(), ti});
+        descs.insert(descs.end(), in.begin(), in.end());
+    }
+    if (!ed.empty()) {
+        out.push_back(GemmLaunch{base | 32, descs.size(), (int)ed.size(), te});
+        descs.insert(descs.end(), ed.begin(), ed.end());
+    }
+}
+
+void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t grid_cap, hipStream_t st) {
+    if (l.total <= 0) return;
+    const int64_t g = grid_cap > 0 ? std::min(l.total, grid_cap) : l.total;
+    if (g > 0x7fffffffLL) throw std::invalid_argument("grouped gemm grid too large");
+    const unsigned grid = (unsigned)g;
+    const GemmDesc* d = d_descs + l.off;
+#define OMH_GG(F32, TA, TB, LO, W4, ED)                                                                    \
+    case (F32) | ((TA) << 1) | ((TB) << 2) | ((LO) << 3) | ((W4) << 4) | ((ED) << 5):                    \
+        k_gemm_grouped<typename std::conditional<(F32) != 0, float, double>::type, (TA) != 0, (TB) != 0,   \
+                       (LO) != 0, (W4) ? 4 : 2, (ED) != 0><<<grid, 256, 0, st>>>(d, l.ng, l.total);        \
+        break;
+#define OMH_GG_OPS(F32, W4, ED) \
+    OMH_GG(F32, 0, 0, 0, W4, ED) OMH_GG(F32, 1, 0, 0, W4, ED) OMH_GG(F32, 0, 1, 0, W4, ED) OMH_GG(F32, 0, 1, 1, W4, ED)
+    switch (l.variant) {
+        OMH_GG_OPS(1, 0, 0)
+        OMH_GG_OPS(1, 0, 1)
+        OMH_GG_OPS(1, 1, 0)
+        OMH_GG_OPS(1, 1, 1)
+        OMH_GG_OPS(0, 0, 0)
+        OMH_GG_OPS(0, 0, 1)
+        default: throw std::invalid_argument("unsupported grouped gemm variant");
+    }
+#undef OMH_GG_OPS
+#undef OMH_GG
+    OMH_LAUNCH_CHECK();
+}
+
+void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st) {
+    if (ng <= 0) return;
+    if (f32) k_diag_grouped<float><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-30f);
+    else k_diag_grouped<double><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-290);
+    OMH_LAUNCH_CHECK();
+}
+
+void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hipStream_t st) {
+    if (ng <= 0 || max_elems <= 0) return;
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(max_elems, 256), 2048));
+    k_elem_grouped<<<dim3(gx, (unsigned)ng), 256, 0, st>>>(d_descs);
+    OMH_LAUNCH_CHECK();
 }
 
 template <typename T>

@@ -5,10 +5,92 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace omh {
 
 constexpr int FACT_NB = 256;   // outer block of the factorisation
+
+// Tile enumeration of a GEMM launch: 0 full grid (ntm row tiles), 1 lower
+// triangle of tiles, 2 one row strip (tile row tm0), 3 one column strip
+// (tile column tn0).
+struct TileMap {
+    int kind;
+    int64_t ntm, tm0, tn0;
+};
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one kernel launch runs the same operation on several
+// independent matrices (e.g. the heads of a batch), each described by a
+// descriptor in device memory.
+// ---------------------------------------------------------------------------
+struct GemmDesc {               // C = alpha op(A) op(B) + beta C on the tiles of map
+    const void* A;
+    const void* B;
+    void* C;
+    int64_t lda, ldb, ldc, m, n, k;
+    double alpha, beta;
+    TileMap map;
+    int64_t tile0;              // index of the first tile of this group in the launch
+};
+
+struct DiagDesc {               // diagonal block of one factorisation (see sym_factor)
+    void* A11;
+    int64_t lda;
+    int nb;
+    void* Linv;                 // FACT_NB x FACT_NB, leading dimension FACT_NB
+    void* sgn;                  // signs of the whole factorisation
+    int* info;
+    int64_t k0;                 // global index of the block's first column
+};
+
+enum ElemOp {
+    ELEM_COPY_F64, ELEM_COPY_F32, ELEM_F64_TO_F32, ELEM_F64_TO_F32_LOWER, ELEM_ADD_F32_TO_F64,
+    ELEM_SCALE_ROWS_F32, ELEM_SCALE_ROWS_F64, ELEM_SCALE_COLS_F32, ELEM_SCALE_COLS_F64, ELEM_ZERO_F64, ELEM_ZERO_INT
+};
+
+struct ElemDesc {               // dst (m x n, ldd) <- op(src (lds), aux)
+    int op;
+    const void* src;
+    int64_t lds;
+    void* dst;
+    int64_t ldd;
+    int64_t m, n;
+    const void* aux;            // row or column scale factors
+};
+
+// One product of a grouped GEMM, before tiling.
+struct GemmJob {
+    int64_t m, n, k;
+    double alpha;
+    const void* A;
+    int64_t lda;
+    const void* B;
+    int64_t ldb;
+    double beta;
+    void* C;
+    int64_t ldc;
+};
+
+// A grouped launch: descriptors [off, off + ng) of a descriptor table.
+struct GemmLaunch {
+    int variant;                // encodes T, op(A), op(B), lower, tile size, edge handling
+    size_t off;
+    int ng;
+    int64_t total;              // tiles
+};
+
+// Tiles the jobs (all with the same T = float if f32 else double, ta, tb,
+// lower) into descriptors appended to `descs`: an interior launch with
+// unguarded vector loads for the full tiles of aligned products and an edge
+// launch with guarded loads for the rest (the rules of gemm()).
+void build_grouped_gemm(const std::vector<GemmJob>& jobs, bool f32, bool ta, bool tb, bool lower, int wb,
+                        std::vector<GemmDesc>& descs, std::vector<GemmLaunch>& out);
+// Launches l with descriptors at d_descs (device); grid_cap > 0 caps the grid
+// (workgroups then loop over tiles).
+void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t grid_cap, hipStream_t st);
+void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st);
+void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hipStream_t st);
 
 // C = alpha op(A) op(B) + beta C (column-major).  lower: square C, only
 // entries with row >= col are computed (only ta=false, tb=true).

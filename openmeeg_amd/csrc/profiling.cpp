@@ -46,6 +46,8 @@ bool prof_enabled() { return g_on; }
 
 ProfScope::ProfScope(const char* name, double work, hipStream_t st) : st_(st) {
     if (!g_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;   // not in graphs
     active_ = true;
     name_ = name;
     work_ = work;

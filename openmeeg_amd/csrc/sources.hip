@@ -483,7 +483,7 @@ std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride
 
 // Per (dipole, mesh) coefficients of assembleSourceMat.cpp:84-99: cD of the
 // P1 derivative rows, cP of the P0 potential rows (zero on current barriers).
-static void rhs_coefficients(const Geometry& g, int64_t n_dip, const std::vector<int>& domain, std::vector<double>& cD,
+void dsm_coefficients(const Geometry& g, int64_t n_dip, const std::vector<int>& domain, std::vector<double>& cD,
                              std::vector<double>& cP, std::vector<char>& used) {
     const int nm = (int)g.meshes.size();
     cD.assign((size_t)n_dip * nm, 0.0);
@@ -504,8 +504,9 @@ static void rhs_coefficients(const Geometry& g, int64_t n_dip, const std::vector
     }
 }
 
-void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
-                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st) {
+void dsm_launch(Geometry& g, const double* d_dip, int64_t n_dip, const double* d_cD, const double* d_cP,
+                const std::vector<char>& used, int order, int levels, double tol, double* d_out, int64_t ld_out,
+                const Layout& L, hipStream_t st) {
     if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
     if (levels < 0 || levels >= MAX_FRAMES) throw std::invalid_argument("adaptive levels must be 0..11");
     DeviceGeometry& dg = device_geometry(g);
@@ -513,13 +514,6 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
     const int nm = (int)g.meshes.size();
     fill_f64(d_out, n, n_dip, ld_out, 0.0, st);
     if (n_dip == 0) return;
-    std::vector<double> cD, cP;
-    std::vector<char> used;
-    rhs_coefficients(g, n_dip, domain, cD, cP, used);
-    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
-    OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
-    OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
-    OMH_HIP(hipStreamSynchronize(st));   // the host vectors die with this frame
     for (int m = 0; m < nm; ++m) {
         if (!used[m]) continue;
         const MeshDev& md = dg.meshes[m];
@@ -531,16 +525,30 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
         for (int64_t s0 = 0; s0 < n_dip; s0 += chunk) {
             const int64_t nd = std::min(chunk, n_dip - s0);
             const int64_t b1 = cdiv(nt, 64) * cdiv(nd, 4);
-            k_dsm_pairs<<<(unsigned)b1, 256, 0, st>>>(dg.tri, md.tri_begin, nt, d_dip, s0, nd, d_c, d_c + cD.size(), nm,
-                                                      m, md.barrier ? 0 : 1, order, levels, tol, W, d_out, ld_out, L,
+            k_dsm_pairs<<<(unsigned)b1, 256, 0, st>>>(dg.tri, md.tri_begin, nt, d_dip, s0, nd, d_cD, d_cP, nm, m,
+                                                      md.barrier ? 0 : 1, order, levels, tol, W, d_out, ld_out, L,
                                                       md.tri_unk0);
             OMH_LAUNCH_CHECK();
             const int64_t b2 = cdiv(nsl, 64) * cdiv(nd, 4);
-            k_dsm_gather<<<(unsigned)b2, 256, 0, st>>>(W, md.tri_begin, nt, s0, nd, d_c, nm, m, dg.adj_off, dg.adj,
+            k_dsm_gather<<<(unsigned)b2, 256, 0, st>>>(W, md.tri_begin, nt, s0, nd, d_cD, nm, m, dg.adj_off, dg.adj,
                                                        dg.slot_vid, dg.vert_unk, md.slot_begin, nsl, d_out, ld_out, L);
             OMH_LAUNCH_CHECK();
         }
     }
+}
+
+void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
+                    int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st) {
+    std::vector<double> cD, cP;
+    std::vector<char> used;
+    dsm_coefficients(g, n_dip, domain, cD, cP, used);
+    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * std::max<size_t>(1, 2 * cD.size()));
+    if (!cD.empty()) {
+        OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
+        OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
+        OMH_HIP(hipStreamSynchronize(st));   // the host vectors die with this frame
+    }
+    dsm_launch(g, d_dip, n_dip, d_c, d_c + cD.size(), used, order, levels, tol, d_out, ld_out, L, st);
 }
 
 void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
@@ -552,7 +560,7 @@ void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const s
     if (n_dip == 0) return;
     std::vector<double> cD, cP;
     std::vector<char> used;
-    rhs_coefficients(g, n_dip, domain, cD, cP, used);
+    dsm_coefficients(g, n_dip, domain, cD, cP, used);
     double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
     OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));

@@ -22,6 +22,16 @@ std::vector<int> locate_domains(Geometry& g, const double* d_pts, int64_t stride
 void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain, int order,
                     int levels, double tol, double* d_out, int64_t ld_out, const Layout& L, hipStream_t st);
 
+// The two halves of dip_source_mat: per (dipole, mesh) coefficients on the
+// host (cD, cP: n_dip x n_meshes row-major; used[m]: mesh m has a non-zero
+// coefficient), then the device launches with those coefficients already
+// in device memory (no host synchronisation: capture-safe).
+void dsm_coefficients(const Geometry& g, int64_t n_dip, const std::vector<int>& domain, std::vector<double>& cD,
+                      std::vector<double>& cP, std::vector<char>& used);
+void dsm_launch(Geometry& g, const double* d_dip, int64_t n_dip, const double* d_cD, const double* d_cP,
+                const std::vector<char>& used, int order, int levels, double tol, double* d_out, int64_t ld_out,
+                const Layout& L, hipStream_t st);
+
 // Adjoint of SourceMatrix<Dipole> with the non-adaptive rule of `order`:
 // d_grad (n_dip x 6, row-major) = for each dipole s the gradient of
 // W(:,s) . rhs_s with respect to (x y z) and (qx qy qz), W being n x n_dip
