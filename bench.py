@@ -41,6 +41,83 @@ def measured_traffic(config, dtype_f):
     return None
 
 
+def batched_extra(om, S, torch, reps=3):
+    """32 uneven perturbed heads (642..2562 vertices per surface, 200..2000
+    dipoles, 64..256 electrodes): the single-head mixed pipeline head after
+    head vs one batch (tuned, eager, then graph replays)."""
+    hs = S.make_batched_heads(n_heads=32, seed=4)
+    heads = [(om.make_nested_geometry(m, c), d, e) for m, c, d, e in hs]
+    sizes = [g.size for g, _, _ in heads]
+    ref = []
+    for rep in range(2):   # the second round is timed (device tables built in the first)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for g, d, e in heads:
+            G, _ = om.gain_eeg_pipeline(g, d, e, precision="mixed")
+            if rep == 1:
+                ref.append(om.gain_to_numpy(G, len(e), len(d)))
+            del G
+        torch.cuda.synchronize()
+        t_seq = time.time() - t0
+    log("bench: batched heads, sequential pipeline %.3f s" % t_seq)
+    b = om.BatchedGain(heads, precision="mixed")
+    tune = b.tune()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    t_eager = time.time() - t0
+    t0 = time.time()
+    b.run(graph=True)      # captures, then replays
+    torch.cuda.synchronize()
+    t_capture = time.time() - t0
+    t0 = time.time()
+    for _ in range(reps):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    t_graph = (time.time() - t0) / reps
+    err = max(float(np.linalg.norm(b.gain(i) - ref[i]) / np.linalg.norm(ref[i])) for i in range(len(heads)))
+    cfg = b.config()
+    log("bench: batched eager %.3f s, first graph pass %.3f s, graph replay %.3f s, max rel diff vs pipeline %.2e, %s"
+        % (t_eager, t_capture, t_graph, err, cfg))
+    del b
+    return {"heads": len(heads), "unknowns_min_max": [min(sizes), max(sizes)], "precision": "mixed",
+            "sequential_pipeline_s": round(t_seq, 4), "batch_eager_s": round(t_eager, 4),
+            "batch_graph_replay_s": round(t_graph, 4), "speedup_vs_sequential": round(t_seq / t_graph, 2),
+            "max_rel_diff_vs_pipeline": err, "config": cfg,
+            "tuning": [{"tile": t, "grid_cap": c, "s": round(x, 6)} for t, c, x in tune]}
+
+
+def gradient_extra(om, S, torch):
+    """Least-squares dipole fit on the 2562-vertex 3-layer head, 4096 dipoles,
+    128 electrodes, SNR-10 targets: one forward and one backward pass."""
+    meshes, cond, dip, el = S.make_config("c3")
+    g = om.make_nested_geometry(meshes, cond)
+    model = om.EEGForward(g, el, precision="mixed")
+    rng = np.random.default_rng(3)
+    with torch.no_grad():
+        Gt = model.gain(torch.tensor(dip[:, :3]), torch.tensor(dip[:, 3:]))
+    T = torch.tensor(S.noisy_targets(Gt.cpu().numpy(), rng, 10.0), device="cuda")
+    out = {}
+    for rep in range(2):
+        p = torch.tensor(dip[:, :3] * 0.99, requires_grad=True)
+        q = torch.tensor(dip[:, 3:], requires_grad=True)
+        s = torch.tensor(model.conductivities(), requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        G = model.gain(p, q, s)
+        L = 0.5 * ((G - T) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.time()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.time()
+        out = {"forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4), "n": g.size,
+               "dipoles": len(dip), "electrodes": len(el), "precision": "mixed"}
+    log("bench: c3 gradient %s" % out)
+    return out
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -122,6 +199,15 @@ def run(args):
     for k, (c, sc, wk) in kern.items():
         if c:
             others[k] = {"launches": c, "seconds": round(sc / args.steps, 4), "work_per_s": wk / sc}
+    extras = {}
+    if not args.no_extras:
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
+        extras["batched_32_heads"] = batched_extra(om, S, torch)
+        torch.cuda.empty_cache()
+        extras["gradient_c3"] = gradient_extra(om, S, torch)
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -155,6 +241,7 @@ def run(args):
         "roofline": roofline,
         "kernels": others,
         "cpu_baseline": cpu,
+        "extras": extras or None,
     }
     return out
 
@@ -170,6 +257,10 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration) and exit")
+    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient"],
+                    help="gain: the headline measurement (default); batched / gradient: only that extra")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-heads and gradient measurements")
     ap.add_argument("--cpu-rows", type=int, default=1500)
     ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
@@ -185,6 +276,15 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}), flush=True)
+        return
+    if args.workload != "gain":
+        # one of the extra workloads alone (profiling); not the headline line
+        import torch
+        import openmeeg_amd as om
+        from openmeeg_amd import spheres as S
+        torch.cuda.set_device(0)
+        fn = batched_extra if args.workload == "batched" else gradient_extra
+        print(json.dumps({"workload": args.workload, "result": fn(om, S, torch)}), flush=True)
         return
     out = run(args)
     print(json.dumps(out), flush=True)

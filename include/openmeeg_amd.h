@@ -152,9 +152,11 @@ int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_wor
                       int64_t nrhs, void* stream);
 
 /* Mixed-precision solve X = H^-1 B: FP32 factor (d_L32, d_work32) with FP64
- * residuals on the FP64 matrix d_H and iterative refinement until
- * ||dX||/||X|| <= tol or max_iter corrections.  No reference counterpart:
- * it stands beside DSPTRS for the mixed-precision path. */
+ * residuals on the FP64 matrix d_H and iterative refinement until the
+ * relative correction r = ||dX||/||X|| <= tol, or the estimated error left
+ * after it, r times its contraction ratio r / r_previous, <= tol, or
+ * max_iter corrections.  No reference counterpart: it stands beside DSPTRS
+ * for the mixed-precision path. */
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
                     const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs,
                     int32_t max_iter, double tol, int32_t* iterations, double* rel_update, void* stream);

@@ -390,7 +390,7 @@ void BatchPlan::run_eager(hipStream_t st) {
                 H.rel_update = xn > 0 ? dn / xn : 0.0;
                 worst = std::max(worst, H.rel_update);
             }
-            if (worst <= refine_tol_ || (it >= 2 && worst > 0.5 * prev)) { ++it; break; }
+            if (refine_converged(it, worst, prev, refine_tol_)) { ++it; break; }
             prev = worst;
             launch(ir_resid_, st);
         }

@@ -119,7 +119,7 @@ private:
     bool calibrated_ = false;
     int refine_steps_ = 0;
     int max_refine_ = 10;
-    double refine_tol_ = 1e-10;
+    double refine_tol_ = 1e-10;   // estimated error left (refine_converged)
     int trail_wb_ = 4;
     int64_t trail_cap_ = 0;
 

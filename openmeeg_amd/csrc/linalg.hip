@@ -68,7 +68,12 @@ __device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) 
     lo = t - h * (h + 1) / 2;
 }
 
-constexpr int BK = 16;
+// k-slice depth of the GEMM tiles (FP32 may use deeper slices: fewer
+// barriers per MFMA, at twice the LDS per workgroup).
+#ifndef OMH_GEMM_BK_F32
+#define OMH_GEMM_BK_F32 16
+#endif
+template <typename T> struct KSlice { static constexpr int v = sizeof(T) == 4 ? OMH_GEMM_BK_F32 : 16; };
 constexpr int LDS_PAD = 16;   // row stride BM+16: the four 16-lane row groups of a fragment read hit disjoint banks
 
 template <typename T> struct Vec;
@@ -81,6 +86,7 @@ template <> struct Vec<double> { typedef double2 type; static constexpr int n = 
 // loads, no guards); load_guarded: element-wise guarded loads.
 template <typename T, bool MK, int R>
 struct TileLoader {
+    static constexpr int BK = KSlice<T>::v;
     static constexpr int V = Vec<T>::n;
     static constexpr int NV = R * BK / V / 256;   // vectors per thread
     T reg[NV][V];
@@ -132,18 +138,65 @@ struct TileLoader {
     }
 };
 
+// L2-aware orders.  Tile rows are grouped in bands of BAND rows; inside a band
+// the tiles run column by column, so a run of consecutive tiles covers a
+// compact block of rows and columns and shares its operand panels.  (The
+// lower-triangle band B holds 8B*h tiles left of its diagonal block and the
+// h(h+1)/2 of that block, h = min(8, ntm - 8B); full bands before B hold
+// 32B^2 + 4B tiles in all.)
+constexpr int BAND = 8;
+
+__device__ __forceinline__ void lower_band_tile(int64_t t, int64_t ntm, int64_t& tm, int64_t& tn) {
+    const int64_t nfull = ntm / BAND;
+    int64_t B = (int64_t)((sqrt(16.0 + 128.0 * (double)t) - 4.0) / 64.0);
+    while (B > 0 && 32 * B * B + 4 * B > t) --B;
+    while (32 * (B + 1) * (B + 1) + 4 * (B + 1) <= t) ++B;
+    if (B > nfull) B = nfull;
+    const int64_t h = min((int64_t)BAND, ntm - BAND * B);
+    int64_t o = t - (32 * B * B + 4 * B);
+    const int64_t left = BAND * B * h;
+    if (o < left) {
+        tn = o / h;
+        tm = BAND * B + o % h;
+        return;
+    }
+    o -= left;
+    int64_t c = 0, start = 0;
+    while (c < h - 1 && start + (h - c) <= o) {
+        start += h - c;
+        ++c;
+    }
+    tn = BAND * B + c;
+    tm = BAND * B + c + (o - start);
+}
+
+__device__ __forceinline__ void full_band_tile(int64_t t, int64_t ntm, int64_t ntn, int64_t& tm, int64_t& tn) {
+    const int64_t B = t / (BAND * ntn);
+    const int64_t h = min((int64_t)BAND, ntm - BAND * B);
+    const int64_t o = t - B * BAND * ntn;
+    tn = o / h;
+    tm = BAND * B + o % h;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin; workgroup b takes tile
+// (b % 8) * per + b / 8 so that each XCD (and its L2) works through one
+// contiguous part of the tile order.  Tiles past ntiles return at once.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t per) { return (b % 8) * per + b / 8; }
+
 __device__ __forceinline__ void map_tile(const TileMap& map, int64_t t, int64_t& tm, int64_t& tn) {
     switch (map.kind) {
         case 1: upper_tile(t, tn, tm); break;   // tn <= tm
         case 2: tm = map.tm0; tn = t; break;
         case 3: tn = map.tn0; tm = t; break;
+        case 4: lower_band_tile(t, map.ntm, tm, tn); break;
+        case 5: full_band_tile(t, map.ntm, map.tn0, tm, tn); break;   // tn0 holds the tile columns
         default: tn = t / map.ntm; tm = t - tn * map.ntm; break;
     }
 }
 
 // C = alpha * op(A) * op(B) + beta * C on a (32 WB) x (32 WB) tile per
 // workgroup of 256 threads (2 x 2 waves, each 16 WB x 16 WB = WB x WB MFMA
-// blocks), BK = 16 k-slices double-buffered in LDS.
+// blocks), k-slices of BK (16; FP32 configurable) double-buffered in LDS.
 // TA: op(A)(i,kk) = A[kk + i*lda] (A stored k x m); else A[i + kk*lda].
 // TB: op(B)(kk,j) = B[j + kk*ldb] (B stored n x k); else B[kk + j*ldb].
 // LOWER: only entries with row >= col are written (m == n).
@@ -155,6 +208,7 @@ __device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alp
                                           int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
                                           T* __restrict__ C, int64_t ldc, int64_t tm, int64_t tn) {
     constexpr int BM = 32 * WB, BN = 32 * WB;
+    constexpr int BK = KSlice<T>::v;
     __shared__ T As[2][BK * (BM + LDS_PAD)];
     __shared__ T Bs[2][BK * (BN + LDS_PAD)];
     const int64_t row0 = tm * BM, col0 = tn * BN;
@@ -266,8 +320,13 @@ template <typename T, bool TA, bool TB, bool LOWER, int WB, bool EDGE>
 __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
                                               int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
                                               T* __restrict__ C, int64_t ldc, TileMap map) {
+    int64_t t = blockIdx.x;
+    if (map.kind >= 4) {
+        t = xcd_tile(t, map.per);
+        if (t >= map.ntiles) return;
+    }
     int64_t tm, tn;
-    map_tile(map, blockIdx.x, tm, tn);
+    map_tile(map, t, tm, tn);
     gemm_tile<T, TA, TB, LOWER, WB, EDGE>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 
@@ -298,15 +357,15 @@ __global__ void __launch_bounds__(256) k_gemm_grouped(const GemmDesc* __restrict
 // (L2 resident).  Inner panels of IB = 32 columns: the 32 x 32 diagonal
 // piece is factored in LDS, the rows below it are solved row-wise and kept
 // in LDS (transposed, conflict-free), and the rest of the block is updated
-// from LDS.  Linv is then built by 32-blocks: diagonal inverses first, then
-// Linv(r,c) = -Linv(r,r) * sum_{c<=q<r} L(r,q) Linv(q,c).
+// from LDS.  Linv is built by a second kernel, one workgroup per 32-column
+// block (diag_inverse_body).
 // ---------------------------------------------------------------------------
 constexpr int IB = 32;
 constexpr int DIAG_THREADS = 512;
 
 template <typename T>
-__device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv, int64_t ldl,
-                                          T* __restrict__ sgn, int* __restrict__ info, int64_t k0, T tiny) {
+__device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ sgn,
+                                          int* __restrict__ info, int64_t k0, T tiny) {
     __shared__ T D[IB][IB + 1];
     __shared__ T ds[IB];
     __shared__ T Pt[IB][FACT_NB - IB + 1];   // panel rows below the inner block, transposed
@@ -376,66 +435,111 @@ __device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb
         }
         __syncthreads();
     }
-    // Linv: inverses of the diagonal 32-blocks (one thread per column)
-    const int nblk = (nb + IB - 1) / IB;
-    for (int t = tid; t < nblk * IB; t += DIAG_THREADS) {
-        const int b = t / IB, jj = t % IB;
-        const int o = b * IB;
-        const int w = min(IB, nb - o);
-        if (jj >= w) continue;
-        const int j = o + jj;
-        for (int i = 0; i < nb; ++i) Linv[i + (int64_t)j * ldl] = T(0);
-        Linv[j + (int64_t)j * ldl] = T(1) / A[j + (int64_t)j * lda];
-        for (int i = j + 1; i < o + w; ++i) {
-            T v = 0;
-            for (int l = j; l < i; ++l) v += A[i + (int64_t)l * lda] * Linv[l + (int64_t)j * ldl];
-            Linv[i + (int64_t)j * ldl] = -v / A[i + (int64_t)i * lda];
-        }
+}
+
+// Inverse of the lower-triangular w x w block Ls into Out (zero above the
+// diagonal), by forward substitution: thread j < IB owns column j.
+template <typename T>
+__device__ __forceinline__ void tri_inv_block(const T (*Ls)[IB + 1], T (*Out)[IB + 1], int w, int tid) {
+    if (tid >= IB) return;
+    const int j = tid;
+    for (int i = 0; i < IB; ++i) Out[i][j] = T(0);
+    if (j >= w) return;
+    Out[j][j] = T(1) / Ls[j][j];
+    for (int i = j + 1; i < w; ++i) {
+        T v = 0;
+        for (int l = j; l < i; ++l) v += Ls[i][l] * Out[l][j];
+        Out[i][j] = -v / Ls[i][i];
+    }
+}
+
+// Column block cb (IB columns) of Linv = L11^-1 for the factored nb x nb
+// diagonal block L11 (lower triangle of A).  Block forward substitution with
+// the block column X kept in LDS: X_c = L_cc^-1, then for each row block r > c
+// X_r = -L_rr^-1 sum_{c <= q < r} L_rq X_q.  Rows above the block are zero.
+template <typename T>
+__device__ __forceinline__ void diag_inverse_body(const T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv,
+                                                  int64_t ldl, int cb) {
+    __shared__ T Xs[FACT_NB][IB + 1];
+    __shared__ T Ls[IB][IB + 1];
+    __shared__ T Di[IB][IB + 1];
+    __shared__ T Ts[IB][IB + 1];
+    const int tid = threadIdx.x;
+    const int c0 = cb * IB;
+    if (c0 >= nb) return;
+    const int wc = min(IB, nb - c0);
+    for (int t = tid; t < c0 * wc; t += DIAG_THREADS) {
+        const int i = t % c0, j = t / c0;
+        Linv[i + (int64_t)(c0 + j) * ldl] = T(0);
+    }
+    for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+        const int i = t % IB, j = t / IB;
+        Ls[i][j] = (i < wc && j < wc && i >= j) ? A[(c0 + i) + (int64_t)(c0 + j) * lda] : T(0);
     }
     __syncthreads();
-    // off-diagonal 32-blocks, row block by row block
-    for (int r = 1; r < nblk; ++r) {
-        const int ro = r * IB, rw = min(IB, nb - ro);
-        // T(rr, cc) = sum_{q=c}^{r-1} L(r,q) Linv(q,c) for all c < r, written
-        // into Linv(r, c) as a temporary
-        for (int t = tid; t < r * IB * IB; t += DIAG_THREADS) {
-            const int c = t / (IB * IB), e = t % (IB * IB);
-            const int ii = e % IB, cc = e / IB;
-            if (ii >= rw) continue;
-            const int row = ro + ii, col = c * IB + cc;
+    tri_inv_block<T>(Ls, Di, wc, tid);
+    __syncthreads();
+    for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+        const int i = t % IB, j = t / IB;
+        if (i < wc) Xs[c0 + i][j] = Di[i][j];
+    }
+    __syncthreads();
+    for (int r0 = c0 + IB; r0 < nb; r0 += IB) {
+        const int rw = min(IB, nb - r0);
+        // T = L(r0.., c0..r0) X(c0..r0, :); L(r0+i, q) is read coalesced over i
+        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+            const int i = t % IB, j = t / IB;
             T v = 0;
-            for (int l = col; l < ro; ++l) v += A[row + (int64_t)l * lda] * Linv[l + (int64_t)col * ldl];
-            Linv[row + (int64_t)col * ldl] = v;
+            if (i < rw && j < wc)
+                for (int q = c0; q < r0; ++q) v += A[(r0 + i) + (int64_t)q * lda] * Xs[q][j];
+            Ts[i][j] = v;
+            Ls[i][j] = (i < rw && j < rw && i >= j) ? A[(r0 + i) + (int64_t)(r0 + j) * lda] : T(0);
         }
         __syncthreads();
-        // Linv(r, c) = -Linv(r,r) * T: each thread owns one column of one
-        // block and walks it bottom-up so the column is overwritten in place
-        for (int t = tid; t < r * IB; t += DIAG_THREADS) {
-            const int c = t / IB, cc = t % IB;
-            const int col = c * IB + cc;
-            for (int ii = rw - 1; ii >= 0; --ii) {
+        tri_inv_block<T>(Ls, Di, rw, tid);
+        __syncthreads();
+        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+            const int i = t % IB, j = t / IB;
+            if (i < rw && j < wc) {
                 T v = 0;
-                for (int kk = 0; kk <= ii; ++kk)
-                    v += Linv[(ro + ii) + (int64_t)(ro + kk) * ldl] * Linv[(ro + kk) + (int64_t)col * ldl];
-                Linv[(ro + ii) + (int64_t)col * ldl] = -v;
+                for (int l = 0; l <= i; ++l) v += Di[i][l] * Ts[l][j];
+                Xs[r0 + i][j] = -v;
             }
         }
         __syncthreads();
+    }
+    const int rows = nb - c0;
+    for (int t = tid; t < rows * wc; t += DIAG_THREADS) {
+        const int i = t % rows, j = t / rows;
+        Linv[(c0 + i) + (int64_t)(c0 + j) * ldl] = Xs[c0 + i][j];
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
-                                                              T* __restrict__ Linv, int64_t ldl, T* __restrict__ sgn,
-                                                              int* __restrict__ info, int64_t k0, T tiny) {
-    diag_body<T>(A, lda, nb, Linv, ldl, sgn, info, k0, tiny);
+                                                              T* __restrict__ sgn, int* __restrict__ info, int64_t k0,
+                                                              T tiny) {
+    diag_body<T>(A, lda, nb, sgn, info, k0, tiny);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse(const T* __restrict__ A, int64_t lda, int nb,
+                                                               T* __restrict__ Linv, int64_t ldl) {
+    diag_inverse_body<T>(A, lda, nb, Linv, ldl, blockIdx.x);
 }
 
 // One workgroup per diagonal block of several factorisations.
 template <typename T>
 __global__ void __launch_bounds__(DIAG_THREADS) k_diag_grouped(const DiagDesc* __restrict__ d, T tiny) {
     const DiagDesc& g = d[blockIdx.x];
-    diag_body<T>((T*)g.A11, g.lda, g.nb, (T*)g.Linv, FACT_NB, (T*)g.sgn, g.info, g.k0, tiny);
+    diag_body<T>((T*)g.A11, g.lda, g.nb, (T*)g.sgn, g.info, g.k0, tiny);
+}
+
+// Column block blockIdx.x of the inverse of diagonal block blockIdx.y.
+template <typename T>
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse_grouped(const DiagDesc* __restrict__ d) {
+    const DiagDesc& g = d[blockIdx.y];
+    diag_inverse_body<T>((const T*)g.A11, g.lda, g.nb, (T*)g.Linv, FACT_NB, blockIdx.x);
 }
 
 // Element-wise operations of several matrices (group = blockIdx.y).
@@ -566,6 +670,7 @@ template <typename T, int WB, bool EDGE>
 static void launch_tiles(bool ta, bool tb, bool lower, int64_t nblk, TileMap map, int64_t m, int64_t n, int64_t k,
                          T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
                          hipStream_t st) {
+    if (nblk == 0) nblk = map.kind >= 4 ? 8 * map.per : map.ntiles;   // grid from the map
     if (nblk <= 0) return;
     if (nblk > 0xffffffffLL) throw std::invalid_argument("gemm grid too large");
 #define OMH_GEMM_CASE(TA_, TB_, LO_)                                                                          \
@@ -582,6 +687,15 @@ static void launch_tiles(bool ta, bool tb, bool lower, int64_t nblk, TileMap map
     OMH_GEMM_CASE(false, true, true)
 #undef OMH_GEMM_CASE
     throw std::invalid_argument("unsupported gemm variant");
+}
+
+// Tile map of an interior launch: the L2-aware order (kinds 4/5) when the
+// launch has enough tiles to spread over the XCDs, else the plain order.
+static TileMap l2_map(bool lower, int64_t fm, int64_t fn) {
+    const int64_t tiles = lower ? fm * (fm + 1) / 2 : fm * fn;
+    if (tiles < 1024) return lower ? TileMap{1, fm, 0, 0, tiles, 0} : TileMap{0, fm, 0, 0, tiles, 0};
+    const int64_t per = cdiv(tiles, 8);
+    return lower ? TileMap{4, fm, 0, 0, tiles, per} : TileMap{5, fm, 0, fn, tiles, per};
 }
 
 // Interior tiles run the unguarded kernel; the partial last tile row and
@@ -609,16 +723,16 @@ static void gemm_launch(bool ta, bool tb, bool lower, bool vec, int64_t m, int64
             ProfScope ps(prof ? name.c_str() : "", prof ? side * (side + 1) * (double)k : 0.0,
                          prof ? st : nullptr);
             if (!prof) ps.cancel();
-            launch_tiles<T, WB, false>(ta, tb, true, fm * (fm + 1) / 2, TileMap{1, fm, 0, 0}, m, n, k, alpha, A, lda,
-                                       B, ldb, beta, C, ldc, st);
+            launch_tiles<T, WB, false>(ta, tb, true, 0, l2_map(true, fm, fm), m, n, k, alpha, A, lda, B, ldb, beta,
+                                       C, ldc, st);
         }
         if (fm < ntm)   // last (partial) tile row, all its lower tiles
             launch_tiles<T, WB, true>(ta, tb, true, ntm, TileMap{2, ntm, fm, 0}, m, n, k, alpha, A, lda, B, ldb,
                                       beta, C, ldc, st);
         return;
     }
-    launch_tiles<T, WB, false>(ta, tb, false, fm * fn, TileMap{0, fm, 0, 0}, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                               ldc, st);
+    launch_tiles<T, WB, false>(ta, tb, false, 0, l2_map(false, fm, fn), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                               st);
     if (fm < ntm)   // partial last tile row (all tile columns)
         launch_tiles<T, WB, true>(ta, tb, false, ntn, TileMap{2, ntm, fm, 0}, m, n, k, alpha, A, lda, B, ldb, beta,
                                   C, ldc, st);
@@ -713,8 +827,16 @@ void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t g
 
 void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st) {
     if (ng <= 0) return;
-    if (f32) k_diag_grouped<float><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-30f);
-    else k_diag_grouped<double><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-290);
+    const dim3 inv(FACT_NB / IB, (unsigned)ng);
+    if (f32) {
+        k_diag_grouped<float><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-30f);
+        OMH_LAUNCH_CHECK();
+        k_diag_inverse_grouped<float><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
+    } else {
+        k_diag_grouped<double><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-290);
+        OMH_LAUNCH_CHECK();
+        k_diag_inverse_grouped<double><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
+    }
     OMH_LAUNCH_CHECK();
 }
 
@@ -746,11 +868,16 @@ template void gemm<double>(bool, bool, bool, int64_t, int64_t, int64_t, double, 
 template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*,
                           int64_t, float, float*, int64_t, hipStream_t, const char*);
 
+// Workspace: panel X | diagonal inverses | signs (padded) | second panel
+// (the look-ahead computes the next panel while the current one is in use).
+static size_t sign_span(int64_t n) { return (size_t)((n + 63) / 64 * 64); }
+
 size_t factor_workspace_bytes(int64_t n, size_t elem) {
     const int64_t nblk = cdiv(n, FACT_NB);
     return elem * ((size_t)n * FACT_NB            // panel X
                    + (size_t)nblk * FACT_NB * FACT_NB   // diagonal inverses
-                   + (size_t)n)                   // signs
+                   + sign_span(n)                 // signs
+                   + (size_t)n * FACT_NB)         // second panel
            + 64;
 }
 
@@ -781,7 +908,9 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
     T* A11 = A + k + k * lda;
     T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
     ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
-    k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB, sgn, info, k, tiny);
+    k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, sgn, info, k, tiny);
+    OMH_LAUNCH_CHECK();
+    k_diag_inverse<T><<<cdiv(nb, IB), DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB);
     OMH_LAUNCH_CHECK();
 }
 
@@ -791,42 +920,54 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 // matrix is updated on the caller's stream.
 template <typename T>
 FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
-    T* X = (T*)work;
-    T* Dinv = X + (size_t)n * FACT_NB;
+    T* Dinv = (T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
+    T* Xbuf[2] = {(T*)work, sgn + sign_span(n)};
     int* info = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
     const T tiny = sizeof(T) == 8 ? T(1e-290) : T(1e-30);
     LookAhead& la = lookahead();
     const char* upd = sizeof(T) == 8 ? "factor_update_f64" : "factor_update_f32";
-    diag_factor<T>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
-    for (int64_t k = 0; k < n; k += FACT_NB) {
-        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+    // panel of block k: X = A21 Linv^T (leading dimension m rounded up to 4
+    // for aligned vector loads), L21 = X S1 written over A21
+    auto panel = [&](int64_t k, int nb, T* X, hipStream_t s) {
+        const int64_t e = k + nb, m = n - e;
+        if (m <= 0) return;
         const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+        const int64_t ldx = (m + 3) / 4 * 4;
+        T* A21 = A + e + k * lda;
+        gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, ldx, s);
+        k_scale_cols<T><<<grid_for(m * nb), 256, 0, s>>>(X, ldx, A21, lda, m, nb, sgn + k);
+        OMH_LAUNCH_CHECK();
+    };
+    diag_factor<T>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
+    panel(0, (int)std::min<int64_t>(FACT_NB, n), Xbuf[0], st);
+    for (int64_t k = 0, it = 0; k < n; k += FACT_NB, ++it) {
+        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) break;
+        T* X = Xbuf[it & 1];
+        const int64_t ldx = (m + 3) / 4 * 4;
         T* A21 = A + e + k * lda;
-        // panel: X = A21 * Linv^T, L21 = X * S1
-        gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, m, st);
-        k_scale_cols<T><<<grid_for(m * nb), 256, 0, st>>>(X, m, A21, lda, m, nb, sgn + k);
-        OMH_LAUNCH_CHECK();
         // trailing update A22 -= X L21^T (lower)
         const int nb2 = (int)std::min<int64_t>(FACT_NB, m);   // next block
         T* A22 = A + e + e * lda;
         // (a) columns of the next block
-        gemm<T>(false, true, true, nb2, nb2, nb, T(-1), X, m, A21, lda, T(1), A22, lda, st);
+        gemm<T>(false, true, true, nb2, nb2, nb, T(-1), X, ldx, A21, lda, T(1), A22, lda, st);
         if (m > nb2)
-            gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, m, A21, lda, T(1), A22 + nb2, lda, st);
-        // factor the next block on the side stream
+            gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, ldx, A21, lda, T(1), A22 + nb2, lda, st);
+        // factor the next block and form its panel on the side stream (it
+        // touches only the next block's columns, which (b) does not)
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
         diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
+        panel(e, nb2, Xbuf[(it + 1) & 1], la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
         // (b) the rest of the trailing matrix
         if (m > nb2)
-            gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, m, A21 + nb2, lda, T(1),
+            gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
                     A22 + nb2 + nb2 * lda, lda, st, upd);
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
     }
@@ -915,6 +1056,12 @@ double frob_norm_f32(const float* a, int64_t lda, int64_t m, int64_t n, hipStrea
     return std::sqrt(s);
 }
 
+bool refine_converged(int it, double rel, double prev, double tol) {
+    if (rel <= tol) return true;                          // the update itself
+    if (it >= 1 && rel * (rel / prev) <= tol) return true;   // update x its contraction ratio
+    return it >= 2 && rel > 0.5 * prev;                   // stagnation at the FP64 rounding floor
+}
+
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
                  double* final_rel_update, hipStream_t st) {
@@ -932,8 +1079,7 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
         const double dn = frob_norm_f32(R32, n, n, r, st);
         const double xn = frob_norm_f64(X, ldx, n, r, st);
         rel = xn > 0 ? dn / xn : 0.0;
-        // converged, or stagnating at the FP64 rounding floor
-        if (rel <= tol || (it >= 2 && rel > 0.5 * prev)) { ++it; break; }
+        if (refine_converged(it, rel, prev, tol)) { ++it; break; }
         prev = rel;
         // R = B - H X
         copy_f64(B, ldb, R, n, n, r, st);

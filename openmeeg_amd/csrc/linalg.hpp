@@ -13,10 +13,12 @@ constexpr int FACT_NB = 256;   // outer block of the factorisation
 
 // Tile enumeration of a GEMM launch: 0 full grid (ntm row tiles), 1 lower
 // triangle of tiles, 2 one row strip (tile row tm0), 3 one column strip
-// (tile column tn0).
+// (tile column tn0); 4 and 5: the lower triangle / full grid in an L2-aware
+// order (see map_tile), over a grid of 8 * per workgroups for ntiles tiles.
 struct TileMap {
     int kind;
     int64_t ntm, tm0, tn0;
+    int64_t ntiles = 0, per = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -125,9 +127,16 @@ void copy_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t 
 double frob_norm_f64(const double* a, int64_t lda, int64_t m, int64_t n, hipStream_t st);
 double frob_norm_f32(const float* a, int64_t lda, int64_t m, int64_t n, hipStream_t st);
 
+// Stopping rule of the refinement after correction `it` (0-based) with
+// relative size rel = ||dX||_F / ||X||_F (prev: that of the correction
+// before): rel <= tol; or the estimated error left, rel times the
+// contraction ratio rel / prev, <= tol; or stagnation (rel > prev / 2 from
+// the third correction on).
+bool refine_converged(int it, double rel, double prev, double tol);
+
 // Iterative refinement: X = H^-1 B with the FP32 factor (L32, work32) and
-// FP64 residuals R = B - H X.  Stops when ||dX||_F / ||X||_F <= tol or after
-// max_iter corrections; returns the number of corrections applied.
+// FP64 residuals R = B - H X.  Stops by refine_converged or after max_iter
+// corrections; returns the number of corrections applied.
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
                  double* final_rel_update, hipStream_t st);

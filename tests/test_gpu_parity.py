@@ -88,6 +88,54 @@ def test_gemm_matches_numpy(om, dtype, ta, tb):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("ta", [0, 1])
+def test_gemm_large_tile_order(om, dtype, ta):
+    """Products with more than 1024 interior tiles run in the XCD-aware,
+    band-swizzled tile order (plus partial edge strips)."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    rng = np.random.default_rng(31 + ta)
+    m, n, k = 3001, 2003, 40
+    npdt = np.float64 if dtype == "f64" else np.float32
+    A = rng.standard_normal((k, m) if ta else (m, k)).astype(npdt)
+    B = rng.standard_normal((k, n)).astype(npdt)
+    dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    dB = torch.from_numpy(np.asfortranarray(B).ravel(order="F").copy()).cuda()
+    dC = torch.zeros(m * n, dtype=dA.dtype, device="cuda")
+    fn = _lib.lib().omh_gemm_f64 if dtype == "f64" else _lib.lib().omh_gemm_f32
+    _lib.check(fn(ta, 0, m, n, k, 1.0, C.c_void_p(dA.data_ptr()), A.shape[0], C.c_void_p(dB.data_ptr()), k,
+                  0.0, C.c_void_p(dC.data_ptr()), m, C.c_void_p(0)))
+    torch.cuda.synchronize()
+    ref = (A.T if ta else A).astype(np.float64) @ B.astype(np.float64)
+    got = dC.cpu().numpy().reshape(n, m).T
+    assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_factor_solve_residual_large(om, precision):
+    """HeadMat of the 2562-vertex 3-layer head (n = 17926): the trailing
+    updates use the L2-aware lower-triangle order.  Residual of H X = B for
+    random right-hand sides."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    n = g.size
+    H = _head_mat_device(g)
+    Href = H.clone().view(n, n)
+    rng = np.random.default_rng(4)
+    B = torch.from_numpy(rng.standard_normal(n * 8)).cuda()
+    F = factor(H, n, precision)
+    X = B.clone()
+    F.solve(X, 8)
+    R = B.view(8, n) - (X.view(8, n) @ Href)          # H symmetric: rows of X^T H
+    res = float(torch.linalg.norm(R) / torch.linalg.norm(B))
+    print("n=%d %s residual %.3e" % (n, precision, res), flush=True)
+    assert res < 1e-12
+
+
 def test_headmat_head1_matches_golden(om, head1):
     g, _, _ = head1
     hm = om.HeadMat(g)
