@@ -5,8 +5,10 @@ Workload (default): the 4-layer nested-sphere head with 10242 vertices per
 surface (radii 0.80/0.84/0.90/1.0, conductivities 0.33/1.79/0.0042/0.33,
 102408 unknowns), 5000 seeded dipoles in r <= 0.72 and 256 electrodes.
 A step = the whole gain computation on the device: HeadMat, DipSourceMat
-and Head2EEGMat assembly, the HeadMat factorisation (FP32 MFMA for the mixed
-path), the adjoint solve with FP64 iterative refinement, and the gain GEMM.
+and Head2EEGMat assembly, the HeadMat factorisation (split-FP16 MFMA trailing
+updates for the default mixed path, FP32 MFMA with --precision mixed, FP64
+with --precision fp64), the adjoint solve with FP64 iterative refinement,
+and the gain GEMM.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
 """
@@ -22,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}
 
 
 def log(*a):
@@ -158,7 +160,8 @@ def run(args):
         log("bench: step %d stages %s" % (s, {k: round(v, 4) for k, v in info["times"].items()}))
     torch.cuda.synchronize()
     dt = (time.time() - t0) / max(1, args.steps)
-    kern = {k: prof(k) for k in ("factor_update_f32_t128", "factor_update_f32_t64", "factor_update_f64_t64",
+    kern = {k: prof(k) for k in ("factor_update_f32s", "factor_update_f32_t128", "factor_update_f32_t64",
+                                  "factor_update_f64_t64",
                                   "diag_factor_f32", "diag_factor_f64", "s_block", "d_pairs")}
     _lib.check(L.omh_profile_enable(0))
     Gm_h = om.gain_to_numpy(Gm, len(el), len(dip))
@@ -184,17 +187,25 @@ def run(args):
             % (t64, {k: round(v, 4) for k, v in stages64.items()}, err64))
 
     stages = infos[-1]["times"]
-    dtype_f = "f32" if args.precision == "mixed" else "f64"
-    dom = "factor_update_f32_t128" if dtype_f == "f32" else "factor_update_f64_t64"
+    if args.precision == "mixed16":
+        # split-FP16 trailing update: 3 FP16 MFMA products per FP32-equivalent
+        # product; the roofline counts the executed FP16 MFMA flops
+        dom, peak, mult, tkey = "factor_update_f32s", PEAK_TFLOPS["f16"], 3.0, "f32s"
+        kname = "k_syrk_split (split-FP16 trailing update of the signed Cholesky, 3 FP16 MFMA products)"
+    elif args.precision == "mixed":
+        dom, peak, mult, tkey = "factor_update_f32_t128", PEAK_TFLOPS["f32"], 1.0, "f32"
+        kname = "k_gemm<float,false,true,true,4,false> (interior tiles of the FP32 trailing update)"
+    else:
+        dom, peak, mult, tkey = "factor_update_f64_t64", PEAK_TFLOPS["f64"], 1.0, "f64"
+        kname = "k_gemm<double,false,true,true,2,false> (interior tiles of the FP64 trailing update)"
     calls, sec, work = kern[dom]
-    achieved = work / sec / 1e12 if sec > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[dtype_f],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[dtype_f], 4),
-                "traffic": measured_traffic(args.config, dtype_f),
-                "kernel": "k_gemm<%s,false,true,true,%d,false> (interior tiles of the trailing update of the "
-                          "signed Cholesky)" % ("float" if dtype_f == "f32" else "double", 4 if dtype_f == "f32" else 2),
-                "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
-                "flop_per_launch": work / max(calls, 1)}
+    achieved = mult * work / sec / 1e12 if sec > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": measured_traffic(args.config, tkey),
+                "kernel": kname, "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
+                "flop_per_launch": mult * work / max(calls, 1),
+                "fp32_equivalent_tflops": round(work / sec / 1e12, 2) if sec > 0 else 0.0}
     others = {}
     for k, (c, sc, wk) in kern.items():
         if c:
@@ -229,7 +240,8 @@ def run(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma-factor+f64" if args.precision == "mixed" else "f64",
+        "dtype": {"mixed16": "f16x3-split-mfma-factor+f64-refinement", "mixed": "f32-mfma-factor+f64-refinement",
+                  "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "%s nested-sphere head, n=%d unknowns, %d dipoles, %d electrodes"
                    % (args.config, n, len(dip), len(el)), "precision": args.precision},
@@ -252,7 +264,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed16", choices=["mixed16", "mixed", "fp64"],
+                    help="mixed16: split-FP16 MFMA factor + FP64 refinement (default); mixed: FP32 MFMA factor")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
     ap.add_argument("--copy-probe", type=int, default=0,

@@ -143,6 +143,13 @@ int64_t omh_factor_workspace_bytes(int64_t n, int32_t elem_size);
  * 1 + index of the first vanishing pivot (then the factor is unusable). */
 int omh_sym_factor_f64(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
 int omh_sym_factor_f32(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
+/* The FP32 factorisation with its trailing updates as split-FP16 MFMA
+ * products: each panel row is scaled by a power of two and split into two
+ * FP16 parts, three FP16 products accumulate in FP32 (about 22 significant
+ * bits per operand).  Same factor layout and workspace as
+ * omh_sym_factor_f32; for the refined (mixed-precision) solve.  No
+ * reference counterpart. */
+int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
 
 /* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
  * symmatrix.cpp:133-152; linsolve gain.h:52-57). */
@@ -240,8 +247,9 @@ int omh_batch_free(omh_batch* b);
 /* Frees the library's cached device scratch buffers. */
 int omh_release_scratch(void);
 
-/* Writes the 16x16 lane layout of the FP64 and FP32 MFMA blocks used by the
- * GEMMs into d_out (2 x 256 doubles) for the layout self-test. */
+/* Writes the raw outputs of one FP64 16x16x4, one FP32 16x16x4 and one
+ * FP16 16x16x32 MFMA block on known operands into d_out (3 x 256 doubles:
+ * lane l, element r at 4 l + r) for the lane-layout self-test. */
 int omh_probe_mfma_layout(double* d_out, void* stream);
 
 #ifdef __cplusplus

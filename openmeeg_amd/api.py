@@ -474,21 +474,23 @@ class Factor:
 def factor(Hdev, n, precision="fp64"):
     """Factor a device HeadMat (triangles-first order).  precision 'fp64'
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
-    factors an FP32 copy."""
+    factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
+    trailing updates."""
     torch = _t()
     info = C.c_int32()
     if precision == "fp64":
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
         check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
         F = Factor(Hdev, n, work, "fp64")
-    elif precision == "mixed":
+    elif precision in ("mixed", "mixed16"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
         check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
-        check(lib().omh_sym_factor_f32(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
+        fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
+        check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
         F = Factor(L32, n, work, "mixed", H64=Hdev)
     else:
-        raise ValueError("precision must be 'fp64' or 'mixed'")
+        raise ValueError("precision must be 'fp64', 'mixed' or 'mixed16'")
     if info.value != 0:
         raise _lib.OpenMEEGError("pivot-free factorisation broke down at pivot %d" % (info.value - 1))
     return F

@@ -426,6 +426,16 @@ int omh_sym_factor_f32(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t*
     });
 }
 
+int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream) {
+    return guard([&] {
+        need(d_A, "d_A");
+        need(d_work, "d_work");
+        if (n <= 0 || ld < n) throw std::invalid_argument("bad matrix size");
+        const auto fi = omh::sym_factor<float>(d_A, n, ld, d_work, S(stream), true);
+        if (info) *info = fi.info;
+    });
+}
+
 int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
                       int64_t nrhs, void* stream) {
     return guard([&] {

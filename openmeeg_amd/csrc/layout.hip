@@ -60,6 +60,8 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 // One wave: D = A B with A(i,k) = 4i + k + 1, B(k,j) = (k+1)(j%5+1); the raw
 // accumulator element r of lane l goes to out[4l + r] (FP64) and
 // out[256 + 4l + r] (FP32), so that the host can read off the layout.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
 __global__ void k_probe(double* out) {
     const int l = threadIdx.x;
     const int i = l % 16, k = l / 16, j = l % 16;
@@ -69,9 +71,20 @@ __global__ void k_probe(double* out) {
     c64 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c64, 0, 0, 0);
     float4_t c32 = {0, 0, 0, 0};
     c32 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, c32, 0, 0, 0);
+    // fp16 16x16x32: lane l holds A(l % 16, 8 (l / 16) + e) and
+    // B(8 (l / 16) + e, l % 16), e = 0..7 (the operand layout under test)
+    half8_t a16, b16;
+    for (int e = 0; e < 8; ++e) {
+        const int kk = 8 * (l / 16) + e;
+        a16[e] = (_Float16)(float)((i + 2 * kk) % 7 - 3);
+        b16[e] = (_Float16)(float)((3 * kk + j) % 5 - 2);
+    }
+    float4_t c16 = {0, 0, 0, 0};
+    c16 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a16, b16, c16, 0, 0, 0);
     for (int r = 0; r < 4; ++r) {
         out[4 * l + r] = c64[r];
         out[256 + 4 * l + r] = (double)c32[r];
+        out[512 + 4 * l + r] = (double)c16[r];
     }
 }
 

@@ -657,6 +657,183 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-FP16 trailing update.  The SYRK C -= X S X^T of a factor step runs on
+// the FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32 accumulation): panel row i is
+// scaled by a power of two s_i >= max_k |X_ik| and split into FP16 parts,
+// X_ik / s_i = h + l to within 2^-22 of the row maximum, and
+//   C_ij -= s_i s_j (Ah_i . Bh_j + Ah_i . Bl_j + Al_i . Bh_j),
+// with B = X S (the signs folded into the B parts).  Operands are stored
+// slice-major: element (row, k) at ((k / SK) mp + row) SK + k % SK, so a
+// 32-wide k-slice of 128 rows is 8 KB of contiguous memory.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+constexpr int SK = 32;        // k-slice of the FP16 MFMA
+constexpr int S_TILE = 128;   // output tile of the split SYRK
+constexpr int S_KP = 256;     // k extent of the operands (FACT_NB)
+
+// 256 threads per 64 panel rows; thread (row t % 64, k quarter t / 64).
+// Writes the four operand arrays, the row scales and L21 = X S into A21.
+__global__ void __launch_bounds__(256) k_split_panel(const float* __restrict__ X, int64_t ldx, int64_t m, int nb,
+                                                     const float* __restrict__ sgn, float* __restrict__ A21,
+                                                     int64_t lda, int64_t mp, _Float16* __restrict__ Ah,
+                                                     _Float16* __restrict__ Al, _Float16* __restrict__ Bh,
+                                                     _Float16* __restrict__ Bl, float* __restrict__ scale) {
+    __shared__ float red[4][64];
+    const int ri = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int64_t row = (int64_t)blockIdx.x * 64 + ri;
+    float v[64];
+    float mx = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < 64; ++kk) {
+        const int k = kq * 64 + kk;
+        v[kk] = (row < m && k < nb) ? X[row + (int64_t)k * ldx] : 0.f;
+        mx = fmaxf(mx, fabsf(v[kk]));
+    }
+    red[kq][ri] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0][ri], red[1][ri]), fmaxf(red[2][ri], red[3][ri]));
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
+    const float inv = ldexpf(1.f, -e);
+    if (kq == 0) scale[row] = ldexpf(1.f, e);
+#pragma unroll
+    for (int kk = 0; kk < 64; kk += 8) {
+        half8_t h, l, bh, bl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = kq * 64 + kk + q;
+            const float x = v[kk + q] * inv;
+            const _Float16 hx = (_Float16)x;
+            const _Float16 lx = (_Float16)(x - (float)hx);
+            const bool neg = k < nb && sgn[k] < 0.f;
+            h[q] = hx;
+            l[q] = lx;
+            bh[q] = neg ? -hx : hx;
+            bl[q] = neg ? -lx : lx;
+            if (row < m && k < nb) A21[row + (int64_t)k * lda] = neg ? -v[kk + q] : v[kk + q];
+        }
+        const int k0 = kq * 64 + kk;
+        const int64_t off = ((int64_t)(k0 / SK) * mp + row) * SK + (k0 % SK);
+        *reinterpret_cast<half8_t*>(Ah + off) = h;
+        *reinterpret_cast<half8_t*>(Al + off) = l;
+        *reinterpret_cast<half8_t*>(Bh + off) = bh;
+        *reinterpret_cast<half8_t*>(Bl + off) = bl;
+    }
+}
+
+// LDS position of 16-byte chunk c (of 4 per 32-wide row) of tile row r: the
+// chunk index is XOR-ed with (r / 4) % 4 so that the 16 rows of a fragment
+// read hit disjoint banks.
+__device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
+
+// C (m x m, lower triangle, leading dimension ldc) -= split product of
+// operand rows r_off + [0, m).  256 threads, a 128 x 128 tile, 2 x 2 waves of
+// 64 x 64 (4 x 4 MFMA blocks, 3 products each), k-slices double-buffered
+// in LDS.  Operand rows up to r_off + 128 * ceil(m / 128) exist (zero
+// padding), so loads need no guards.
+__global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
+                                                    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
+                                                    const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                                    const float* __restrict__ scale, float* __restrict__ C,
+                                                    int64_t ldc, TileMap map) {
+    __shared__ half8_t S[2][4][S_TILE * 4];
+    int64_t t = blockIdx.x;
+    if (map.kind >= 4) {
+        t = xcd_tile(t, map.per);
+        if (t >= map.ntiles) return;
+    }
+    int64_t tm, tn;
+    map_tile(map, t, tm, tn);
+    const int64_t row0 = r_off + tm * S_TILE, col0 = r_off + tn * S_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const _Float16* src[4] = {Ah, Al, Bh, Bl};
+    half8_t stg[4][2];
+    auto gload = [&](int ks) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int q = tid + 256 * p, r = q >> 2, c = q & 3;
+                const int64_t base = a < 2 ? row0 : col0;
+                stg[a][p] = *reinterpret_cast<const half8_t*>(src[a] + (((int64_t)ks * mp + base + r) * SK + 8 * c));
+            }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int q = tid + 256 * p, r = q >> 2, c = q & 3;
+                S[buf][a][swz(r, c)] = stg[a][p];
+            }
+    };
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    constexpr int NKS = S_KP / SK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int fr = lane & 15, fc = lane >> 4;
+    for (int ks = 0; ks < NKS; ++ks) {
+        const int buf = ks & 1;
+        if (ks + 1 < NKS) gload(ks + 1);
+        half8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int r = wm + 16 * a + fr;
+            ah[a] = S[buf][0][swz(r, fc)];
+            al[a] = S[buf][1][swz(r, fc)];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int r = wn + 16 * b + fr;
+            bh[b] = S[buf][2][swz(r, fc)];
+            bl[b] = S[buf][3][swz(r, fc)];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+            }
+        if (ks + 1 < NKS) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    // epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of a block
+    const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
+    const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t gi = i0 + wm + 16 * a + 4 * fc, gj = j0 + wn + 16 * b + fr;
+            const float sj = scale[r_off + gj];
+            if (vec) {
+                const float4 si = *reinterpret_cast<const float4*>(scale + r_off + gi);
+                float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
+                float4 c = *cp;
+                c.x -= si.x * sj * acc[a][b][0];
+                c.y -= si.y * sj * acc[a][b][1];
+                c.z -= si.z * sj * acc[a][b][2];
+                c.w -= si.w * sj * acc[a][b][3];
+                *cp = c;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = gi + r;
+                    if (i < m && gj < m && i >= gj) C[i + gj * ldc] -= scale[r_off + i] * sj * acc[a][b][r];
+                }
+            }
+        }
+}
+
 __global__ void k_set_int(int* p, int v) { p[0] = v; }
 
 inline unsigned grid_for(int64_t total) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(total, 256), 16384)); }
@@ -872,13 +1049,25 @@ template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, co
 // (the look-ahead computes the next panel while the current one is in use).
 static size_t sign_span(int64_t n) { return (size_t)((n + 63) / 64 * 64); }
 
-size_t factor_workspace_bytes(int64_t n, size_t elem) {
+// Split-FP16 operands of one panel (FP32 factorisation): four arrays of
+// mp x S_KP halves and mp row scales, mp = n rounded up to S_TILE.
+static size_t split_set_bytes(int64_t n) {
+    const size_t mp = (size_t)cdiv(n, S_TILE) * S_TILE;
+    return 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float);
+}
+
+static size_t dense_workspace_bytes(int64_t n, size_t elem) {
     const int64_t nblk = cdiv(n, FACT_NB);
     return elem * ((size_t)n * FACT_NB            // panel X
                    + (size_t)nblk * FACT_NB * FACT_NB   // diagonal inverses
                    + sign_span(n)                 // signs
-                   + (size_t)n * FACT_NB)         // second panel
-           + 64;
+                   + (size_t)n * FACT_NB);        // second panel
+}
+
+size_t factor_workspace_bytes(int64_t n, size_t elem) {
+    size_t b = dense_workspace_bytes(n, elem) + 64;
+    if (elem == 4) b = (b + 255) / 256 * 256 + 2 * split_set_bytes(n) + 256;   // two split panels
+    return b;
 }
 
 // Side stream and events of the look-ahead (one set per device, default
@@ -919,11 +1108,29 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 // block k+1 is factored on the side stream while the rest of the trailing
 // matrix is updated on the caller's stream.
 template <typename T>
-FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
+FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16) {
     T* Dinv = (T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
     T* Xbuf[2] = {(T*)work, sgn + sign_span(n)};
+    const bool split = split16 && sizeof(T) == 4;
+    struct SplitSet {
+        _Float16 *Ah, *Al, *Bh, *Bl;
+        float* sc;
+    } sets[2] = {};
+    const int64_t mp_max = cdiv(n, S_TILE) * S_TILE;
+    if (split) {
+        char* p = (char*)work + (dense_workspace_bytes(n, 4) + 64 + 255) / 256 * 256;
+        for (int b = 0; b < 2; ++b) {
+            const size_t arr = (size_t)mp_max * S_KP * sizeof(_Float16);
+            sets[b].Ah = (_Float16*)p;
+            sets[b].Al = (_Float16*)(p + arr);
+            sets[b].Bh = (_Float16*)(p + 2 * arr);
+            sets[b].Bl = (_Float16*)(p + 3 * arr);
+            sets[b].sc = (float*)(p + 4 * arr);
+            p += split_set_bytes(n);
+        }
+    }
     int* info = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
@@ -932,18 +1139,26 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
     const char* upd = sizeof(T) == 8 ? "factor_update_f64" : "factor_update_f32";
     // panel of block k: X = A21 Linv^T (leading dimension m rounded up to 4
     // for aligned vector loads), L21 = X S1 written over A21
-    auto panel = [&](int64_t k, int nb, T* X, hipStream_t s) {
+    auto panel = [&](int64_t k, int nb, int b, hipStream_t s) {
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) return;
         const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
         const int64_t ldx = (m + 3) / 4 * 4;
         T* A21 = A + e + k * lda;
+        T* X = Xbuf[b];
         gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, ldx, s);
-        k_scale_cols<T><<<grid_for(m * nb), 256, 0, s>>>(X, ldx, A21, lda, m, nb, sgn + k);
+        if (split && nb == S_KP) {   // L21 = X S and the FP16 parts of the trailing update
+            const int64_t mp = cdiv(m, S_TILE) * S_TILE;
+            k_split_panel<<<(unsigned)(mp / 64), 256, 0, s>>>((const float*)X, ldx, m, nb, (const float*)sgn + k,
+                                                               (float*)A21, lda, mp, sets[b].Ah, sets[b].Al,
+                                                               sets[b].Bh, sets[b].Bl, sets[b].sc);
+        } else {
+            k_scale_cols<T><<<grid_for(m * nb), 256, 0, s>>>(X, ldx, A21, lda, m, nb, sgn + k);
+        }
         OMH_LAUNCH_CHECK();
     };
     diag_factor<T>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
-    panel(0, (int)std::min<int64_t>(FACT_NB, n), Xbuf[0], st);
+    panel(0, (int)std::min<int64_t>(FACT_NB, n), 0, st);
     for (int64_t k = 0, it = 0; k < n; k += FACT_NB, ++it) {
         const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
         const int64_t e = k + nb, m = n - e;
@@ -963,12 +1178,23 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
         diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
-        panel(e, nb2, Xbuf[(it + 1) & 1], la.side);
+        panel(e, nb2, (int)((it + 1) & 1), la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
         // (b) the rest of the trailing matrix
-        if (m > nb2)
+        if (m > nb2 && split && nb == S_KP) {
+            const int64_t mb = m - nb2, mp = cdiv(m, S_TILE) * S_TILE, ntm = cdiv(mb, S_TILE);
+            const int64_t tiles = ntm * (ntm + 1) / 2;
+            const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
+                                              : TileMap{1, ntm, 0, 0, tiles, 0};
+            const SplitSet& ss = sets[it & 1];
+            ProfScope ps("factor_update_f32s", (double)mb * (double)(mb + 1) * (double)nb, st);
+            k_syrk_split<<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
+                mb, mp, nb2, ss.Ah, ss.Al, ss.Bh, ss.Bl, ss.sc, (float*)(A22 + nb2 + nb2 * lda), lda, map);
+            OMH_LAUNCH_CHECK();
+        } else if (m > nb2) {
             gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
                     A22 + nb2 + nb2 * lda, lda, st, upd);
+        }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
     }
     FactorInfo fi;
@@ -981,8 +1207,8 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st) 
     return fi;
 }
 
-template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t);
-template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t);
+template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool);
+template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t, bool);
 
 template <typename T>
 void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st) {

@@ -114,8 +114,11 @@ size_t factor_workspace_bytes(int64_t n, size_t elem);
 
 // In-place pivot-free signed Cholesky H = L S L^T of the lower triangle of A
 // (n x n, leading dimension lda).  The strict upper triangle is not touched.
+// split16 (FP32 only): the trailing updates run as split-FP16 MFMA products
+// (each panel row scaled and split into two FP16 parts, three products,
+// FP32 accumulation; about 22 significant bits per operand).
 template <typename T>
-FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st);
+FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16 = false);
 
 // W <- H^-1 W for r right-hand sides with the factor of sym_factor.
 template <typename T>

@@ -44,7 +44,8 @@ def _gemm(ta, tb, m, n, k, A, lda, B, ldb, Cout, ldc, a_off=0, b_off=0, alpha=1.
 class EEGForward:
     """EEG forward model on one geometry and electrode set, differentiable in
     the dipoles and the conductivities.  precision 'mixed' (FP32 factor with
-    FP64 refinement) or 'fp64'."""
+    FP64 refinement), 'mixed16' (split-FP16 factor with FP64 refinement) or
+    'fp64'."""
 
     def __init__(self, geom, electrodes, precision="mixed", order=3, max_refine=10, refine_tol=1e-12):
         self.geom = geom
@@ -122,7 +123,7 @@ class EEGForward:
         doms = (C.c_int32 * max(nd, 1))()
         check(lib().omh_locate_domains(g._h, _ptr(dip), 6, nd, doms, _stream()))
         H = _head_mat_device(g, Integrator(self.order))
-        Hkeep = H if self.precision == "mixed" else H.clone()
+        Hkeep = H.clone() if self.precision == "fp64" else H   # the FP64 factor overwrites H
         D = _dev_f64(n * max(nd, 1))
         check(lib().omh_dip_source_mat(g._h, _ptr(dip), nd, doms, self.order, 0, 0.0, _ptr(D), n,
                                        ORDER_TRIANGLES_FIRST, _stream()))

@@ -84,9 +84,11 @@ def test_batch_mixed_graph_replays(om, heads, fp64_batch):
     print("mixed batch config before capture", cfg, flush=True)
     assert 1 <= cfg["refine_steps"] <= 10
     for i in range(len(heads)):
-        assert rel_fro(eager[i], G64[i]) < 1e-6
+        # refinement stops at an estimated error of 1e-10 (last update times
+        # its contraction ratio): the gain is far inside the 1e-6 bar
+        assert rel_fro(eager[i], G64[i]) < 1e-8
         st = b.stats(i)
-        assert st["factor_info"] == 0 and st["rel_update"] <= 1e-10
+        assert st["factor_info"] == 0 and st["rel_update"] <= 1e-6
     for rep in range(3):
         x = torch.randn(512, 512, device="cuda", dtype=torch.float64)
         y = (x @ x).sum().item()                 # unrelated work between replays

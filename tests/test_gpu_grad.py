@@ -105,10 +105,11 @@ def test_gradients_match_finite_differences(setup):
     assert gs[list(sig0).index(0.0)] == 0.0
 
 
-def test_mixed_gradients_match_fp64(setup):
+@pytest.mark.parametrize("precision", ["mixed", "mixed16"])
+def test_mixed_gradients_match_fp64(setup, precision):
     om, meshes, dip, el, g, R = setup
     a = _loss_and_grads(om.EEGForward(g, el, precision="fp64"), dip, R)
-    b = _loss_and_grads(om.EEGForward(g, el, precision="mixed"), dip, R)
+    b = _loss_and_grads(om.EEGForward(g, el, precision=precision), dip, R)
     for x, y in zip(a[1:4], b[1:4]):
         assert np.linalg.norm(x - y) / np.linalg.norm(x) < 1e-6
 

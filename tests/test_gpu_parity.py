@@ -45,7 +45,7 @@ def test_mfma_lane_layout(om):
     import ctypes as C
     import torch
     from openmeeg_amd import _lib
-    out = torch.zeros(512, dtype=torch.float64, device="cuda")
+    out = torch.zeros(768, dtype=torch.float64, device="cuda")
     _lib.check(_lib.lib().omh_probe_mfma_layout(C.c_void_p(out.data_ptr()), C.c_void_p(0)))
     torch.cuda.synchronize()
     i = np.arange(16)
@@ -54,13 +54,24 @@ def test_mfma_lane_layout(om):
     ref = A @ B
     o = out.cpu().numpy()
     rows = {"f64": lambda lane, r: (lane >> 4) + 4 * r, "f32": lambda lane, r: 4 * (lane >> 4) + r}
-    for part, kind in ((o[:256], "f64"), (o[256:], "f32")):
+    for part, kind in ((o[:256], "f64"), (o[256:512], "f32")):
         raw = part.reshape(64, 4)
         got = np.zeros((16, 16))
         for lane in range(64):
             for r in range(4):
                 got[rows[kind](lane, r), lane & 15] = raw[lane, r]
         np.testing.assert_array_equal(got, ref, err_msg=kind)
+    # FP16 16x16x32: operands A(i, k) in lane i + 16 (k // 8), element k % 8
+    # and B(k, j) in lane j + 16 (k // 8), element k % 8; output as FP32
+    k32 = np.arange(32)
+    A16 = (i[:, None] + 2 * k32[None, :]) % 7 - 3.0
+    B16 = (3 * k32[:, None] + i[None, :]) % 5 - 2.0
+    raw = o[512:].reshape(64, 4)
+    got = np.zeros((16, 16))
+    for lane in range(64):
+        for r in range(4):
+            got[rows["f32"](lane, r), lane & 15] = raw[lane, r]
+    np.testing.assert_array_equal(got, A16 @ B16, err_msg="f16")
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
@@ -113,7 +124,7 @@ def test_gemm_large_tile_order(om, dtype, ta):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed16"])
 def test_factor_solve_residual_large(om, precision):
     """HeadMat of the 2562-vertex 3-layer head (n = 17926): the trailing
     updates use the L2-aware lower-triangle order.  Residual of H X = B for
@@ -213,10 +224,13 @@ def test_gain_c0_fp64_and_mixed(om, c0_oracle):
     G64, info64 = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="fp64", domain_name="Brain")
     G64 = om.gain_to_numpy(G64, len(c["el"]), len(c["dip"]))
     assert rel_fro(G64, c["G"]) < 1e-10
-    Gm, infom = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="mixed", domain_name="Brain")
-    Gm = om.gain_to_numpy(Gm, len(c["el"]), len(c["dip"]))
-    assert rel_fro(Gm, G64) < 1e-6
-    assert 1 <= infom["refine_iterations"] <= 10
+    for prec in ("mixed", "mixed16"):
+        Gm, infom = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision=prec, domain_name="Brain")
+        Gm = om.gain_to_numpy(Gm, len(c["el"]), len(c["dip"]))
+        err = rel_fro(Gm, G64)
+        print("%s vs fp64 %.3e after %d refinements" % (prec, err, infom["refine_iterations"]), flush=True)
+        assert err < 1e-6
+        assert 1 <= infom["refine_iterations"] <= 10
 
 
 def test_gain_c0_vs_analytic_sphere(om, c0_oracle):
