@@ -402,25 +402,17 @@ __device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb
         const int r0 = jb + w;
         const int m = nb - r0;
         // panel rows: L_p = A_p L_D^-T S_D by row-wise substitution
+        // (the row is kept in this thread's column of Pt, never in a
+        // dynamically indexed register array)
         for (int t = tid; t < m; t += DIAG_THREADS) {
             const int i = r0 + t;
-            T row[IB];
-#pragma unroll
-            for (int j = 0; j < IB; ++j) row[j] = j < w ? A[i + (int64_t)(jb + j) * lda] : T(0);
-#pragma unroll
-            for (int j = 0; j < IB; ++j) {
-                if (j < w) {
-                    T v = row[j];
-                    for (int l = 0; l < j; ++l) v -= row[l] * ds[l] * D[j][l];
-                    row[j] = v / (D[j][j] * ds[j]);
-                }
+            for (int j = 0; j < w; ++j) {
+                T v = A[i + (int64_t)(jb + j) * lda];
+                for (int l = 0; l < j; ++l) v -= Pt[l][t] * D[j][l];   // Pt[l][t] = L(i, jb + l) s_l
+                const T lij = v / (D[j][j] * ds[j]);
+                A[i + (int64_t)(jb + j) * lda] = lij;
+                Pt[j][t] = lij * ds[j];
             }
-#pragma unroll
-            for (int j = 0; j < IB; ++j)
-                if (j < w) {
-                    A[i + (int64_t)(jb + j) * lda] = row[j];
-                    Pt[j][t] = row[j] * ds[j];
-                }
         }
         __syncthreads();
         // trailing update of the rest of the block (lower part) from LDS:
@@ -658,9 +650,10 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 }
 
 // ---------------------------------------------------------------------------
-// Split-FP16 trailing update.  The SYRK C -= X S X^T of a factor step runs on
-// the FP16 MFMA (v_mfma_f32_16x16x32_f16, FP32 accumulation): panel row i is
-// scaled by a power of two s_i >= max_k |X_ik| and split into FP16 parts,
+// Split-FP16 trailing update.  The SYRK C -= X S X^T of a factor step (X the
+// two panels of a block pair, 512 columns) runs on the FP16 MFMA
+// (v_mfma_f32_16x16x32_f16, FP32 accumulation): row i of X is scaled by a
+// power of two s_i >= max_k |X_ik| and split into FP16 parts,
 // X_ik / s_i = h + l to within 2^-22 of the row maximum, and
 //   C_ij -= s_i s_j (Ah_i . Bh_j + Ah_i . Bl_j + Al_i . Bh_j),
 // with B = X S (the signs folded into the B parts).  Operands are stored
@@ -670,29 +663,40 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 constexpr int SK = 32;        // k-slice of the FP16 MFMA
 constexpr int S_TILE = 128;   // output tile of the split SYRK
-constexpr int S_KP = 256;     // k extent of the operands (FACT_NB)
+constexpr int S_KP = 2 * FACT_NB;   // k extent of the operands: two panels
 
-// 256 threads per 64 panel rows; thread (row t % 64, k quarter t / 64).
-// Writes the four operand arrays, the row scales and L21 = X S into A21.
-__global__ void __launch_bounds__(256) k_split_panel(const float* __restrict__ X, int64_t ldx, int64_t m, int nb,
-                                                     const float* __restrict__ sgn, float* __restrict__ A21,
-                                                     int64_t lda, int64_t mp, _Float16* __restrict__ Ah,
-                                                     _Float16* __restrict__ Al, _Float16* __restrict__ Bh,
-                                                     _Float16* __restrict__ Bl, float* __restrict__ scale) {
-    __shared__ float red[4][64];
+// Operands of a rank-512 update from two consecutive panels X_p, X_q (FP32,
+// m rows from row offsets offp, offq): row i gets one power-of-two scale for
+// both panels; k < 256 come from X_p (signs sgnp), k >= 256 from X_q.  512
+// threads per 64 rows: thread (row t % 64, k eighth t / 64).  Padding rows
+// (m <= i < mp) are zero.
+__global__ void __launch_bounds__(512) k_split_pair(const float* __restrict__ Xp, int64_t ldxp, int64_t offp,
+                                                    const float* __restrict__ Xq, int64_t ldxq, int64_t offq,
+                                                    int64_t m, const float* __restrict__ sgnp,
+                                                    const float* __restrict__ sgnq, int64_t mp,
+                                                    _Float16* __restrict__ Ah, _Float16* __restrict__ Al,
+                                                    _Float16* __restrict__ Bh, _Float16* __restrict__ Bl,
+                                                    float* __restrict__ scale) {
+    __shared__ float red[8][64];
     const int ri = threadIdx.x & 63, kq = threadIdx.x >> 6;
     const int64_t row = (int64_t)blockIdx.x * 64 + ri;
+    const bool second = kq >= 4;
+    const float* X = second ? Xq + offq : Xp + offp;
+    const int64_t ldx = second ? ldxq : ldxp;
+    const float* sg = second ? sgnq : sgnp;
+    const int kb = (kq & 3) * 64;   // first column inside its panel
     float v[64];
     float mx = 0.f;
 #pragma unroll
     for (int kk = 0; kk < 64; ++kk) {
-        const int k = kq * 64 + kk;
-        v[kk] = (row < m && k < nb) ? X[row + (int64_t)k * ldx] : 0.f;
+        v[kk] = row < m ? X[row + (int64_t)(kb + kk) * ldx] : 0.f;
         mx = fmaxf(mx, fabsf(v[kk]));
     }
     red[kq][ri] = mx;
     __syncthreads();
-    mx = fmaxf(fmaxf(red[0][ri], red[1][ri]), fmaxf(red[2][ri], red[3][ri]));
+    mx = 0.f;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) mx = fmaxf(mx, red[g][ri]);
     int e = 0;
     if (mx > 0.f) (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
     const float inv = ldexpf(1.f, -e);
@@ -702,16 +706,14 @@ __global__ void __launch_bounds__(256) k_split_panel(const float* __restrict__ X
         half8_t h, l, bh, bl;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int k = kq * 64 + kk + q;
             const float x = v[kk + q] * inv;
             const _Float16 hx = (_Float16)x;
             const _Float16 lx = (_Float16)(x - (float)hx);
-            const bool neg = k < nb && sgn[k] < 0.f;
+            const bool neg = sg[kb + kk + q] < 0.f;
             h[q] = hx;
             l[q] = lx;
             bh[q] = neg ? -hx : hx;
             bl[q] = neg ? -lx : lx;
-            if (row < m && k < nb) A21[row + (int64_t)k * lda] = neg ? -v[kk + q] : v[kk + q];
         }
         const int k0 = kq * 64 + kk;
         const int64_t off = ((int64_t)(k0 / SK) * mp + row) * SK + (k0 % SK);
@@ -728,10 +730,11 @@ __global__ void __launch_bounds__(256) k_split_panel(const float* __restrict__ X
 __device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
 
 // C (m x m, lower triangle, leading dimension ldc) -= split product of
-// operand rows r_off + [0, m).  256 threads, a 128 x 128 tile, 2 x 2 waves of
+// operand rows r_off + [0, m), over k extent KP.  256 threads, a 128 x 128 tile, 2 x 2 waves of
 // 64 x 64 (4 x 4 MFMA blocks, 3 products each), k-slices double-buffered
 // in LDS.  Operand rows up to r_off + 128 * ceil(m / 128) exist (zero
 // padding), so loads need no guards.
+template <int KP>
 __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
                                                     const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
@@ -749,24 +752,27 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const _Float16* src[4] = {Ah, Al, Bh, Bl};
-    half8_t stg[4][2];
-    auto gload = [&](int ks) {
+    // operand slices travel global -> registers (two slots, loaded two
+    // slices ahead) -> LDS (two buffers)
+    half8_t stg[2][4][2];
+    auto gload = [&](int ks, int slot) {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const int q = tid + 256 * p, r = q >> 2, c = q & 3;
                 const int64_t base = a < 2 ? row0 : col0;
-                stg[a][p] = *reinterpret_cast<const half8_t*>(src[a] + (((int64_t)ks * mp + base + r) * SK + 8 * c));
+                stg[slot][a][p] =
+                    *reinterpret_cast<const half8_t*>(src[a] + (((int64_t)ks * mp + base + r) * SK + 8 * c));
             }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int slot, int buf) {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const int q = tid + 256 * p, r = q >> 2, c = q & 3;
-                S[buf][a][swz(r, c)] = stg[a][p];
+                S[buf][a][swz(r, c)] = stg[slot][a][p];
             }
     };
     float4_t acc[4][4];
@@ -774,14 +780,19 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    constexpr int NKS = S_KP / SK;
-    gload(0);
-    lstore(0);
+    constexpr int NKS = KP / SK;
+    const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
+    const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+    float4 c_pre[4][4], si_pre[4];
+    gload(0, 0);
+    lstore(0, 0);
+    gload(1, 1);
+    gload(2, 0);
     __syncthreads();
     const int fr = lane & 15, fc = lane >> 4;
-    for (int ks = 0; ks < NKS; ++ks) {
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {   // unrolled: register slots stay static
         const int buf = ks & 1;
-        if (ks + 1 < NKS) gload(ks + 1);
         half8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -795,20 +806,41 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
             bh[b] = S[buf][2][swz(r, fc)];
             bl[b] = S[buf][3][swz(r, fc)];
         }
+        // the three products, each over all 16 accumulators in turn
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < 4; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-            }
-        if (ks + 1 < NKS) lstore(buf ^ 1);
+        // slice ks + 1 (in register slot (ks + 1) % 2) into the other buffer,
+        // then refill that slot with slice ks + 3
+        if (ks + 1 < NKS) lstore((ks + 1) & 1, buf ^ 1);
         __syncthreads();
+        if (ks + 3 < NKS) gload(ks + 3, (ks + 1) & 1);
+        if (ks == NKS - 2 && vec) {
+            // the operand registers are free now: fetch this lane's part of
+            // the C tile and the row scales, consumed after the last slice
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                si_pre[a] = *reinterpret_cast<const float4*>(scale + r_off + i0 + wm + 16 * a + 4 * fc);
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    c_pre[a][b] = *reinterpret_cast<const float4*>(
+                        C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc);
+            }
+        }
     }
     // epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of a block
-    const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
-    const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -816,9 +848,9 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
             const int64_t gi = i0 + wm + 16 * a + 4 * fc, gj = j0 + wn + 16 * b + fr;
             const float sj = scale[r_off + gj];
             if (vec) {
-                const float4 si = *reinterpret_cast<const float4*>(scale + r_off + gi);
+                const float4 si = si_pre[a];
                 float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
-                float4 c = *cp;
+                float4 c = c_pre[a][b];
                 c.x -= si.x * sj * acc[a][b][0];
                 c.y -= si.y * sj * acc[a][b][1];
                 c.z -= si.z * sj * acc[a][b][2];
@@ -1049,7 +1081,7 @@ template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, co
 // (the look-ahead computes the next panel while the current one is in use).
 static size_t sign_span(int64_t n) { return (size_t)((n + 63) / 64 * 64); }
 
-// Split-FP16 operands of one panel (FP32 factorisation): four arrays of
+// Split-FP16 operands of one block pair (FP32 factorisation): four arrays of
 // mp x S_KP halves and mp row scales, mp = n rounded up to S_TILE.
 static size_t split_set_bytes(int64_t n) {
     const size_t mp = (size_t)cdiv(n, S_TILE) * S_TILE;
@@ -1066,7 +1098,7 @@ static size_t dense_workspace_bytes(int64_t n, size_t elem) {
 
 size_t factor_workspace_bytes(int64_t n, size_t elem) {
     size_t b = dense_workspace_bytes(n, elem) + 64;
-    if (elem == 4) b = (b + 255) / 256 * 256 + 2 * split_set_bytes(n) + 256;   // two split panels
+    if (elem == 4) b = (b + 255) / 256 * 256 + split_set_bytes(n) + 256;   // split operands
     return b;
 }
 
@@ -1107,30 +1139,119 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 // the panel of block k, the columns of block k+1 are updated first, then
 // block k+1 is factored on the side stream while the rest of the trailing
 // matrix is updated on the caller's stream.
+// FP32 factorisation with split-FP16 trailing updates, in super-steps of
+// two blocks (p, q = p + 1) so that the large update has rank 512 and reads
+// and writes the trailing matrix half as often as rank-256 steps would:
+//   (a1) block q's columns -= panel p; diagonal block q; panel q;
+//   (a2) the next pair's 512 columns -= panels p and q (FP32 GEMMs);
+//   the next pair's first block and panel on the side stream, while
+//   (b) the rest -= [X_p X_q] S [X_p X_q]^T as one split-FP16 SYRK.
+static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
+    float* Dinv = (float*)work + (size_t)n * FACT_NB;
+    const int64_t nblk = cdiv(n, FACT_NB);
+    float* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
+    float* Xbuf[2] = {(float*)work, sgn + sign_span(n)};
+    const int64_t mp_max = cdiv(n, S_TILE) * S_TILE;
+    char* sp = (char*)work + (dense_workspace_bytes(n, 4) + 64 + 255) / 256 * 256;
+    const size_t arr = (size_t)mp_max * S_KP * sizeof(_Float16);
+    _Float16* Ah = (_Float16*)sp;
+    _Float16* Al = (_Float16*)(sp + arr);
+    _Float16* Bh = (_Float16*)(sp + 2 * arr);
+    _Float16* Bl = (_Float16*)(sp + 3 * arr);
+    float* sc = (float*)(sp + 4 * arr);
+    int* info = (int*)scratch(SCR_SMALL, 64);
+    k_set_int<<<1, 1, 0, st>>>(info, 0);
+    OMH_LAUNCH_CHECK();
+    const float tiny = 1e-30f;
+    LookAhead& la = lookahead();
+    auto ldx_of = [&](int64_t e) { return (n - e + 3) / 4 * 4; };   // panel whose rows start at e
+    // panel of block b into Xbuf[slot]: X = A21 Linv^T, L21 = X S1
+    auto panel = [&](int64_t b, int slot, hipStream_t s) {
+        const int64_t k = b * FACT_NB;
+        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+        const int64_t e = k + nb, m = n - e;
+        if (m <= 0) return;
+        const float* Li = Dinv + (size_t)b * FACT_NB * FACT_NB;
+        float* A21 = A + e + k * lda;
+        gemm<float>(false, true, false, m, nb, nb, 1.f, A21, lda, Li, FACT_NB, 0.f, Xbuf[slot], ldx_of(e), s);
+        k_scale_cols<float><<<grid_for(m * nb), 256, 0, s>>>(Xbuf[slot], ldx_of(e), A21, lda, m, nb, sgn + k);
+        OMH_LAUNCH_CHECK();
+    };
+    // C(rows >= r, cols [r, r + w)) -= X (rows from r) L21^T for panel b whose
+    // rows start at e (X in slot): lower w x w block plus the rows below
+    auto update_cols = [&](int64_t b, int slot, int64_t r, int64_t w) {
+        const int64_t k = b * FACT_NB, e = k + FACT_NB;
+        const int64_t ldx = ldx_of(e);
+        const float* X = Xbuf[slot] + (r - e);
+        const float* L21 = A + r + k * lda;
+        float* C = A + r + r * lda;
+        gemm<float>(false, true, true, w, w, FACT_NB, -1.f, X, ldx, L21, lda, 1.f, C, lda, st);
+        if (n - r > w) gemm<float>(false, true, false, n - r - w, w, FACT_NB, -1.f, X + w, ldx, L21, lda, 1.f, C + w,
+                                   lda, st);
+    };
+    diag_factor<float>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
+    panel(0, 0, st);
+    for (int64_t p = 0; p + 1 < nblk; p += 2) {
+        const int64_t q = p + 1, kq = q * FACT_NB;
+        const int nbq = (int)std::min<int64_t>(FACT_NB, n - kq);
+        // (a1) block q's columns with panel p, then block q and its panel
+        update_cols(p, 0, kq, nbq);
+        diag_factor<float>(A, lda, kq, nbq, Dinv, sgn, info, tiny, st);
+        panel(q, 1, st);
+        const int64_t r0 = kq + nbq;   // first column of the next pair
+        if (r0 >= n) break;
+        const int64_t w2 = std::min<int64_t>(2 * FACT_NB, n - r0);
+        // (a2) the next pair's columns with both panels
+        update_cols(p, 0, r0, w2);
+        update_cols(q, 1, r0, w2);
+        // split operands of the rank-512 update of the rows below the next pair
+        const int64_t r2 = r0 + w2, mb = n - r2;
+        if (mb > 0) {
+            const int64_t mp = cdiv(mb, S_TILE) * S_TILE;
+            k_split_pair<<<(unsigned)(mp / 64), 512, 0, st>>>(Xbuf[0], ldx_of(kq), r2 - kq, Xbuf[1], ldx_of(r0),
+                                                               r2 - r0, mb, sgn + p * FACT_NB, sgn + kq, mp, Ah, Al,
+                                                               Bh, Bl, sc);
+            OMH_LAUNCH_CHECK();
+        }
+        // the next pair's first block and panel on the side stream (the panel
+        // buffers are free once the split operands exist)
+        OMH_HIP(hipEventRecord(la.ready, st));
+        OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
+        const int64_t pn = p + 2;
+        diag_factor<float>(A, lda, pn * FACT_NB, (int)std::min<int64_t>(FACT_NB, n - pn * FACT_NB), Dinv, sgn, info,
+                           tiny, la.side);
+        panel(pn, 0, la.side);
+        OMH_HIP(hipEventRecord(la.done, la.side));
+        // (b) the rest of the trailing matrix
+        if (mb > 0) {
+            const int64_t mp = cdiv(mb, S_TILE) * S_TILE, ntm = cdiv(mb, S_TILE);
+            const int64_t tiles = ntm * (ntm + 1) / 2;
+            const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
+                                              : TileMap{1, ntm, 0, 0, tiles, 0};
+            ProfScope ps("factor_update_f32s", (double)mb * (double)(mb + 1) * (double)S_KP, st);
+            k_syrk_split<S_KP><<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
+                mb, mp, 0, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map);
+            OMH_LAUNCH_CHECK();
+        }
+        OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
+    }
+    FactorInfo fi;
+    int h_info = 0;
+    OMH_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    fi.info = h_info;
+    fi.dinv = Dinv;
+    fi.signs = sgn;
+    return fi;
+}
+
 template <typename T>
 FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16) {
     T* Dinv = (T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
     T* Xbuf[2] = {(T*)work, sgn + sign_span(n)};
-    const bool split = split16 && sizeof(T) == 4;
-    struct SplitSet {
-        _Float16 *Ah, *Al, *Bh, *Bl;
-        float* sc;
-    } sets[2] = {};
-    const int64_t mp_max = cdiv(n, S_TILE) * S_TILE;
-    if (split) {
-        char* p = (char*)work + (dense_workspace_bytes(n, 4) + 64 + 255) / 256 * 256;
-        for (int b = 0; b < 2; ++b) {
-            const size_t arr = (size_t)mp_max * S_KP * sizeof(_Float16);
-            sets[b].Ah = (_Float16*)p;
-            sets[b].Al = (_Float16*)(p + arr);
-            sets[b].Bh = (_Float16*)(p + 2 * arr);
-            sets[b].Bl = (_Float16*)(p + 3 * arr);
-            sets[b].sc = (float*)(p + 4 * arr);
-            p += split_set_bytes(n);
-        }
-    }
+    if (split16 && sizeof(T) == 4) return sym_factor_split((float*)A, n, lda, work, st);
     int* info = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
@@ -1147,14 +1268,7 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         T* A21 = A + e + k * lda;
         T* X = Xbuf[b];
         gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, ldx, s);
-        if (split && nb == S_KP) {   // L21 = X S and the FP16 parts of the trailing update
-            const int64_t mp = cdiv(m, S_TILE) * S_TILE;
-            k_split_panel<<<(unsigned)(mp / 64), 256, 0, s>>>((const float*)X, ldx, m, nb, (const float*)sgn + k,
-                                                               (float*)A21, lda, mp, sets[b].Ah, sets[b].Al,
-                                                               sets[b].Bh, sets[b].Bl, sets[b].sc);
-        } else {
-            k_scale_cols<T><<<grid_for(m * nb), 256, 0, s>>>(X, ldx, A21, lda, m, nb, sgn + k);
-        }
+        k_scale_cols<T><<<grid_for(m * nb), 256, 0, s>>>(X, ldx, A21, lda, m, nb, sgn + k);
         OMH_LAUNCH_CHECK();
     };
     diag_factor<T>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
@@ -1181,20 +1295,9 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         panel(e, nb2, (int)((it + 1) & 1), la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
         // (b) the rest of the trailing matrix
-        if (m > nb2 && split && nb == S_KP) {
-            const int64_t mb = m - nb2, mp = cdiv(m, S_TILE) * S_TILE, ntm = cdiv(mb, S_TILE);
-            const int64_t tiles = ntm * (ntm + 1) / 2;
-            const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
-                                              : TileMap{1, ntm, 0, 0, tiles, 0};
-            const SplitSet& ss = sets[it & 1];
-            ProfScope ps("factor_update_f32s", (double)mb * (double)(mb + 1) * (double)nb, st);
-            k_syrk_split<<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
-                mb, mp, nb2, ss.Ah, ss.Al, ss.Bh, ss.Bl, ss.sc, (float*)(A22 + nb2 + nb2 * lda), lda, map);
-            OMH_LAUNCH_CHECK();
-        } else if (m > nb2) {
+        if (m > nb2)
             gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
                     A22 + nb2 + nb2 * lda, lda, st, upd);
-        }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
     }
     FactorInfo fi;
