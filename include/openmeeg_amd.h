@@ -84,6 +84,16 @@ int omh_geometry_domain_name(const omh_geometry* g, int32_t index, char* buf, in
  * here; order 0..3 selects the 3/6/7/16-point rule. */
 int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int ordering, void* stream);
 
+/* omh_head_mat with flags.  OMH_HEADMAT_FAR_FIELD (no reference
+ * counterpart; for the mixed-precision path): triangle pairs whose centroid
+ * distance is at least 4 times the longer of their longest edges use Gauss
+ * quadrature on both triangles (16 inner points, 7 from 10 diameters on)
+ * instead of the analytic inner integral.  Relative change of such entries:
+ * below 1e-10 (S) and at the analytic formula's own cancellation level
+ * (D).  Order 3 only; other orders ignore the flag. */
+#define OMH_HEADMAT_FAR_FIELD 1
+int omh_head_mat_ex(omh_geometry* g, int order, int32_t flags, double* d_H, int64_t ld, int ordering, void* stream);
+
 /* Geometry::domain(point) for n device points (x y z every `stride`
  * doubles): geometry.cpp:146-154.  Writes domain indices to h_domains. */
 int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,

@@ -499,13 +499,19 @@ def factor(Hdev, n, precision="fp64"):
 # ---------------------------------------------------------------------------
 # Assembly (assemble.h:32-35, assembleSourceMat.h:24-29)
 # ---------------------------------------------------------------------------
-def _head_mat_device(geom, integrator=None):
+HEADMAT_FAR_FIELD = 1
+
+
+def _head_mat_device(geom, integrator=None, far_field=False):
+    """Device HeadMat (triangles-first order).  far_field: Gauss quadrature on
+    both triangles of well-separated pairs (the mixed-precision path)."""
     integ = integrator or Integrator(3, 0, 0.005)
     if integ.levels != 0:
         raise ValueError("adaptive HeadMat integration is not supported on this path (levels must be 0)")
     n = geom.size
     H = _dev_f64(n * n)
-    check(lib().omh_head_mat(geom._h, integ.order, _ptr(H), n, ORDER_TRIANGLES_FIRST, _stream()))
+    check(lib().omh_head_mat_ex(geom._h, integ.order, HEADMAT_FAR_FIELD if far_field else 0, _ptr(H), n,
+                                ORDER_TRIANGLES_FIRST, _stream()))
     return H
 
 
@@ -653,20 +659,24 @@ class Sensors:
 # Device pipeline (the path the benchmark times)
 # ---------------------------------------------------------------------------
 def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="", integrator=None,
-                      dsm_integrator=None, max_refine=10, refine_tol=1e-8):
+                      dsm_integrator=None, max_refine=10, refine_tol=1e-8, far_field=None):
     """HeadMat, DipSourceMat and Head2EEGMat assembly, the HeadMat factor and
     the adjoint gain G = (H^-1 E^T)^T DSM, all on the device.  Returns
     (G as a device tensor n_e x n_dip in column-major storage, info dict).
     Mixed precision refines until the estimated relative error of H^-1 E^T
     is below refine_tol (default 1e-8, a hundredth of the 1e-6 the mixed
-    path must meet against FP64)."""
+    path must meet against FP64) and, unless far_field is False, assembles
+    well-separated pairs by Gauss quadrature (far_field defaults to True for
+    the mixed paths, False for 'fp64')."""
+    if far_field is None:
+        far_field = precision != "fp64"
     torch = _t()
     n = geom.size
     times = {}
     ev = lambda: torch.cuda.Event(enable_timing=True)
     marks = [("start", ev())]
     marks[-1][1].record()
-    H = _head_mat_device(geom, integrator)
+    H = _head_mat_device(geom, integrator, far_field)
     marks.append(("headmat", ev())); marks[-1][1].record()
     D = _dsm_device(geom, dipoles, domain_name, dsm_integrator)
     marks.append(("dsm", ev())); marks[-1][1].record()

@@ -336,6 +336,16 @@ int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int orderi
     });
 }
 
+int omh_head_mat_ex(omh_geometry* g, int order, int32_t flags, double* d_H, int64_t ld, int ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_H, "d_H");
+        if (flags & ~OMH_HEADMAT_FAR_FIELD) throw std::invalid_argument("unknown HeadMat flags");
+        omh::assemble_headmat(*g->g, order, d_H, layout_of(*g->g, ld, ordering), S(stream),
+                              (flags & OMH_HEADMAT_FAR_FIELD) != 0);
+    });
+}
+
 int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,
                        void* stream) {
     return guard([&] {

@@ -135,7 +135,25 @@ DeviceGeometry::~DeviceGeometry() {
     (void)hipFree(slot_vid);
     (void)hipFree(adj_off);
     (void)hipFree(adj);
+    (void)hipFree(gp);
     (void)hipSetDevice(cur);
+}
+
+// Far-field Gauss points of every triangle (GP_STRIDE doubles each: the
+// order-3 then the order-2 rule, by the multadd sequence of gauss_point).
+__global__ void k_gauss_table(const TriRec* __restrict__ tri, int64_t nt, double* __restrict__ gp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const d3 P[3] = {ld3(tri[t].p[0]), ld3(tri[t].p[1]), ld3(tri[t].p[2])};
+    double* o = gp + t * GP_STRIDE;
+    for (int k = 0; k < 16; ++k) {
+        const d3 x = gauss_point(3, k, P);
+        o[GP16 + 3 * k] = x.x; o[GP16 + 3 * k + 1] = x.y; o[GP16 + 3 * k + 2] = x.z;
+    }
+    for (int k = 0; k < 7; ++k) {
+        const d3 x = gauss_point(2, k, P);
+        o[GP7 + 3 * k] = x.x; o[GP7 + 3 * k + 1] = x.y; o[GP7 + 3 * k + 2] = x.z;
+    }
 }
 
 template <typename T>
@@ -194,6 +212,7 @@ DeviceGeometry& device_geometry(Geometry& g) {
                 const double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                 for (int c = 0; c < 3; ++c) r.U[i][c] = d[c] / dl;
             }
+            r.diam = std::max(r.len[0], std::max(r.len[1], r.len[2]));
             recs.push_back(r);
             for (int k = 0; k < 3; ++k) tvid.push_back((int32_t)t[k]);
         }
@@ -229,6 +248,12 @@ DeviceGeometry& device_geometry(Geometry& g) {
     dg->slot_vid = upload(slots);
     dg->adj_off = upload(adj_off);
     dg->adj = upload(adj);
+    OMH_HIP(hipMalloc(&dg->gp, sizeof(double) * GP_STRIDE * (size_t)std::max<int64_t>(1, dg->n_tris)));
+    if (dg->n_tris > 0) {
+        k_gauss_table<<<(unsigned)cdiv(dg->n_tris, 256), 256>>>(dg->tri, dg->n_tris, dg->gp);
+        OMH_LAUNCH_CHECK();
+        OMH_HIP(hipDeviceSynchronize());
+    }
     dg->device = dev;
     g.dev = dg;
     return *g.dev;
@@ -264,7 +289,8 @@ template <bool TEMP>
 __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
                                                  int64_t b0, int64_t n2, int diag, int64_t ntx, int order,
                                                  double coeff, double* __restrict__ H, Layout L,
-                                                 int64_t ua0, int64_t ub0, double* __restrict__ tmp) {
+                                                 int64_t ua0, int64_t ub0, double* __restrict__ tmp,
+                                                 const double* __restrict__ gp, int far) {
     int64_t ti, tj;
     if (diag) {
         upper_tile(blockIdx.x, ti, tj);
@@ -276,12 +302,21 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRe
     const int64_t j = tj * TILE + threadIdx.x % TILE;
     if (i >= n1 || j >= n2) return;
     if (diag && j < i) return;
-    const OpS o = load_ops(tri[a0 + i]);
+    const TriRec& t1 = tri[a0 + i];
     const TriRec& t2 = tri[b0 + j];
-    const d3 P[3] = {ld3(t2.p[0]), ld3(t2.p[1]), ld3(t2.p[2])};
-    const int np = c_nb_pts[order];
+    int fc = 0;
+    if (far && order == 3) {
+        fc = far_class(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
+    }
     double res = 0.0;
-    for (int k = 0; k < np; ++k) res += c_rules[order][k][3] * ops_eval(o, gauss_point(order, k, P));
+    if (fc) {   // Gauss on both triangles (outer: column triangle, inner: row triangle)
+        res = t1.area * s_far(gp + (b0 + j) * GP_STRIDE, gp + (a0 + i) * GP_STRIDE, fc);
+    } else {
+        const OpS o = load_ops(t1);
+        const d3 P[3] = {ld3(t2.p[0]), ld3(t2.p[1]), ld3(t2.p[2])};
+        const int np = c_nb_pts[order];
+        for (int k = 0; k < np; ++k) res += c_rules[order][k][3] * ops_eval(o, gauss_point(order, k, P));
+    }
     const double v = (t2.area * res) * coeff;
     if (TEMP) {
         tmp[i + j * n1] = v;
@@ -296,23 +331,36 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRe
 // D pass 1: W[(k*n2 + j)*n1c + i] = integral over row triangle (a0+i)
 // (Gauss) of P1SLP(column triangle b0+j)_k  (operators.h:103-106).
 __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_d_pairs(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
-                                                 int64_t b0, int64_t n2, int order, double* __restrict__ W) {
+                                                 int64_t b0, int64_t n2, int order, double* __restrict__ W,
+                                                 const double* __restrict__ gp, int far) {
     const int64_t ni = cdiv_dev(n1c, 64);
     const int64_t tj = blockIdx.x / ni, tb = blockIdx.x - tj * ni;
     const int64_t i = tb * 64 + threadIdx.x % 64;
     const int64_t j = tj * 4 + threadIdx.x / 64;
     if (i >= n1c || j >= n2) return;
-    const P1S o = load_p1s(tri[b0 + j]);
     const TriRec& t1 = tri[a0 + i];
-    const d3 P[3] = {ld3(t1.p[0]), ld3(t1.p[1]), ld3(t1.p[2])};
-    const int np = c_nb_pts[order];
+    const TriRec& t2 = tri[b0 + j];
+    int fc = 0;
+    if (far && order == 3) {
+        fc = far_class(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
+    }
     double rx = 0.0, ry = 0.0, rz = 0.0;
-    for (int k = 0; k < np; ++k) {
-        const double w = c_rules[order][k][3];
-        const d3 v = p1s_eval(o, gauss_point(order, k, P));
-        rx += w * v.x;
-        ry += w * v.y;
-        rz += w * v.z;
+    if (fc) {   // Gauss on both triangles (outer: row triangle, inner: column triangle)
+        const d3 v = d_far(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, ld3(t2.n), fc);
+        rx = t2.area * v.x;
+        ry = t2.area * v.y;
+        rz = t2.area * v.z;
+    } else {
+        const P1S o = load_p1s(t2);
+        const d3 P[3] = {ld3(t1.p[0]), ld3(t1.p[1]), ld3(t1.p[2])};
+        const int np = c_nb_pts[order];
+        for (int k = 0; k < np; ++k) {
+            const double w = c_rules[order][k][3];
+            const d3 v = p1s_eval(o, gauss_point(order, k, P));
+            rx += w * v.x;
+            ry += w * v.y;
+            rz += w * v.z;
+        }
     }
     const double A = t1.area;
     W[(0 * n2 + j) * n1c + i] = A * rx;
@@ -428,7 +476,7 @@ int64_t n_upper_tiles(int64_t n) {
 // host drivers
 // ---------------------------------------------------------------------------
 static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff, double* H, const Layout& L,
-                     double* tmp, hipStream_t st) {
+                     double* tmp, int far, hipStream_t st) {
     const MeshDev& A = dg.meshes[m1];
     const MeshDev& B = dg.meshes[m2];
     const int64_t n1 = A.tri_end - A.tri_begin, n2 = B.tri_end - B.tri_begin;
@@ -441,10 +489,10 @@ static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff
     ProfScope ps("s_block", pairs * npts[order], st);
     if (tmp)
         k_s_block<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
-                                                        coeff, H, L, A.tri_unk0, B.tri_unk0, tmp);
+                                                        coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far);
     else
         k_s_block<false><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
-                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp);
+                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far);
     OMH_LAUNCH_CHECK();
 }
 
@@ -472,7 +520,7 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
 
 // D(rows = triangles of mesh mr, columns = vertices of mesh mc), operators.h:84-115.
 static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff, double* H, const Layout& L,
-                     hipStream_t st) {
+                     int far, hipStream_t st) {
     const MeshDev& R = dg.meshes[mr];
     const MeshDev& Cm = dg.meshes[mc];
     const int64_t n1 = R.tri_end - R.tri_begin, n2 = Cm.tri_end - Cm.tri_begin;
@@ -486,7 +534,8 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
         const int64_t blk1 = cdiv(nc, 64) * cdiv(n2, 4);
         static const int npts[4] = {3, 6, 7, 16};
         ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
-        k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W);
+        k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W, dg.gp,
+                                                  far);
         OMH_LAUNCH_CHECK();
         const int64_t blk2 = cdiv(nc, 64) * cdiv(ns, 4);
         k_d_gather<<<(unsigned)blk2, 256, 0, st>>>(W, nc, n2, Cm.tri_begin, dg.adj_off, dg.adj, dg.slot_vid,
@@ -495,7 +544,8 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
     }
 }
 
-void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st) {
+void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field) {
+    const int far = far_field ? 1 : 0;
     if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
     DeviceGeometry& dg = device_geometry(g);
     const int64_t n = g.size();
@@ -506,7 +556,7 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         const bool same = (m1 == m2);
         double Scoeff = 0.0;
         if (same ? !b1 : (!b1 && !b2)) {   // set_S_block
-            launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, st);
+            launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, far, st);
             Scoeff = mp.cS;
         }
         if (Scoeff != 0.0) {               // set_N_block
@@ -514,11 +564,11 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         } else {
             const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
             double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
-            launch_s(dg, m1, m2, order, 1.0, H, L, tmp, st);
+            launch_s(dg, m1, m2, order, 1.0, H, L, tmp, far, st);
             launch_n(dg, g, m1, m2, mp.cN, tmp, H, L, st);
         }
-        if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, st);              // set_D_block
-        if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, st);     // set_Dstar_block
+        if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, far, st);              // set_D_block
+        if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, far, st);     // set_Dstar_block
     }
     // deflate all current barriers, one constant per isolated part
     double* coef = (double*)scratch(SCR_SMALL, 64);

@@ -13,6 +13,9 @@ namespace omh {
 // column-major, both halves written) in the unknown order of L, with the
 // non-adaptive Gauss rule of the given order (Integrator(order,0,.),
 // integrator.h:33-48).
-void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st);
+// far_field: pairs at least FAR_NEAR diameters apart use Gauss quadrature
+// on both triangles instead of the analytic inner integral (the mixed
+// path; the plain FP64 path keeps the reference's integrals everywhere).
+void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field = false);
 
 }  // namespace omh

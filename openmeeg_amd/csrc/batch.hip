@@ -346,7 +346,7 @@ void BatchPlan::enqueue_assembly(hipStream_t st) {
     ArenaScope scope(&arena_);
     for (Head& H : heads_) {
         const Layout L{H.ld, H.nv, H.nt, 1};
-        assemble_headmat(*H.g, 3, H.H, L, st);
+        assemble_headmat(*H.g, 3, H.H, L, st, precision_ == 1);   // far-field Gauss for the mixed path
         dsm_launch(*H.g, H.dip, H.nd, H.cD, H.cP, H.used, 3, 10, 0.001, H.D, H.ld, L, st);
     }
 }

@@ -43,6 +43,7 @@ struct DeviceGeometry {
     int32_t* slot_vid = nullptr;     // vertex slot -> global vertex id
     int64_t* adj_off = nullptr;      // slot -> [adj_off[s], adj_off[s+1]) in adj
     AdjRec* adj = nullptr;
+    double* gp = nullptr;            // far-field Gauss points, GP_STRIDE per triangle
     std::vector<MeshDev> meshes;
     int64_t n_tris = 0, n_verts = 0, n_slots = 0;
     ~DeviceGeometry();

@@ -122,7 +122,7 @@ class EEGForward:
         self._apply_sigma(sigma)
         doms = (C.c_int32 * max(nd, 1))()
         check(lib().omh_locate_domains(g._h, _ptr(dip), 6, nd, doms, _stream()))
-        H = _head_mat_device(g, Integrator(self.order))
+        H = _head_mat_device(g, Integrator(self.order), far_field=self.precision != "fp64")
         Hkeep = H.clone() if self.precision == "fp64" else H   # the FP64 factor overwrites H
         D = _dev_f64(n * max(nd, 1))
         check(lib().omh_dip_source_mat(g._h, _ptr(dip), nd, doms, self.order, 0, 0.0, _ptr(D), n,

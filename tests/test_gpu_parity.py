@@ -233,6 +233,26 @@ def test_gain_c0_fp64_and_mixed(om, c0_oracle):
         assert 1 <= infom["refine_iterations"] <= 10
 
 
+def test_headmat_far_field_close_to_analytic(om, c0_oracle):
+    """The mixed path's far-field assembly (Gauss quadrature on both
+    triangles of pairs at least 4 diameters apart) against the reference's
+    analytic integrals on the 642-vertex head: entries agree to 1e-10 of the
+    matrix scale."""
+    import torch
+    from openmeeg_amd.api import _head_mat_device
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    n = g.size
+    Ha = _head_mat_device(g).view(n, n)
+    Hf = _head_mat_device(g, far_field=True).view(n, n)
+    d = float(torch.linalg.norm(Hf - Ha) / torch.linalg.norm(Ha))
+    m = float((Hf - Ha).abs().max() / Ha.abs().max())
+    changed = float(((Hf - Ha) != 0).double().mean())
+    print("far field vs analytic: rel fro %.3e, max %.3e, entries changed %.3f" % (d, m, changed), flush=True)
+    assert d < 1e-10 and m < 1e-10
+    assert changed > 0.5            # most pairs are far
+
+
 def test_gain_c0_vs_analytic_sphere(om, c0_oracle):
     """Independent of the reference: the BEM gain on the 642-vertex spheres
     against Zhang's series (RDM/MAG, tests/analytic tolerances for Head3:
