@@ -89,10 +89,11 @@ __device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2])
 // Far field.  For a triangle pair whose centroid distance d is at least
 // FAR_NEAR times the longer of the two longest edges, the inner analytic
 // integral of a pair integral is replaced by Gauss quadrature on the inner
-// triangle: the 16-point rule, or the 7-point rule from FAR_COARSE on.
-// Against the analytic OperatorS the relative error is below 1.2e-11 at
-// d = 4 diameters (16 points) and 1e-10 at 10 (7 points); the analytic P1
-// double layer itself loses digits to cancellation at such distances.
+// triangle: the 16-point rule on both triangles, or from FAR_COARSE on the
+// 7-point rule on both.  Against the analytic OperatorS the relative error is
+// below 1.2e-11 at d = 4 diameters (16 x 16) and 8e-11 at 10 (7 x 7); the
+// analytic P1 double layer itself loses digits to cancellation at such
+// distances.
 // Per-triangle Gauss points are tabulated: GP_STRIDE doubles per triangle,
 // the 16 points of the order-3 rule then the 7 of the order-2 rule.
 // ---------------------------------------------------------------------------
@@ -204,45 +205,46 @@ __device__ __forceinline__ int far_class(const double* __restrict__ g1, const do
     return d >= FAR_COARSE * diam ? 2 : (d >= FAR_NEAR * diam ? 1 : 0);
 }
 
-// sum over the 16 outer points y_k (table go) of w_k sum_a w_a / |y_k - x_a|
-// over the NI inner points x_a (table gi at offset OFF, rule IO), the inner
-// points held in registers.  Times the inner area it is the outer Gauss sum
-// of OperatorS (analytics.h:19-67) of the inner triangle.
-template <int NI, int IO, int OFF>
+// sum over the NO outer points y_k (table go at OOFF, rule OO) of
+// w_k sum_a w_a / |y_k - x_a| over the NI inner points x_a (table gi at IOFF,
+// rule IO), the inner points held in registers.  Times the inner area it is
+// the outer Gauss sum of OperatorS (analytics.h:19-67) of the inner triangle.
+template <int NO, int OO, int OOFF, int NI, int IO, int IOFF>
 __device__ __forceinline__ double s_far_n(const double* __restrict__ go, const double* __restrict__ gi) {
     d3 x[NI];
 #pragma unroll
-    for (int a = 0; a < NI; ++a) x[a] = ld3(gi + OFF + 3 * a);
+    for (int a = 0; a < NI; ++a) x[a] = ld3(gi + IOFF + 3 * a);
     double res = 0.0;
-    for (int k = 0; k < 16; ++k) {
-        const d3 y = ld3(go + 3 * k);
+    for (int k = 0; k < NO; ++k) {
+        const d3 y = ld3(go + OOFF + 3 * k);
         double acc = 0.0;
 #pragma unroll
         for (int a = 0; a < NI; ++a) {
             const d3 r = sub(y, x[a]);
             acc += c_rules[IO][a][3] * rsqrt(dot(r, r));
         }
-        res += c_rules[3][k][3] * acc;
+        res += c_rules[OO][k][3] * acc;
     }
     return res;
 }
 
+// Far class 1: 16 x 16 points; 2: 7 x 7 points (16- and 7-point rules).
 __device__ __forceinline__ double s_far(const double* __restrict__ go, const double* __restrict__ gi, int fc) {
-    return fc == 2 ? s_far_n<7, 2, GP7>(go, gi) : s_far_n<16, 3, GP16>(go, gi);
+    return fc == 2 ? s_far_n<7, 2, GP7, 7, 2, GP7>(go, gi) : s_far_n<16, 3, GP16, 16, 3, GP16>(go, gi);
 }
 
-// sum over the 16 outer points x_k (table go) of w_k sum_b w_b lambda_b
+// sum over the NO outer points x_k (table go) of w_k sum_b w_b lambda_b
 // n.(y_b - x_k) / |y_b - x_k|^3 over the NI inner points y_b (table gi,
 // registers) with their barycentric coordinates lambda_b: times the two
 // areas, the outer Gauss sum of the P1 double layer (analytics.h:75-128).
-template <int NI, int IO, int OFF>
+template <int NO, int OO, int OOFF, int NI, int IO, int IOFF>
 __device__ __forceinline__ d3 d_far_n(const double* __restrict__ go, const double* __restrict__ gi, d3 n) {
     d3 y[NI];
 #pragma unroll
-    for (int a = 0; a < NI; ++a) y[a] = ld3(gi + OFF + 3 * a);
+    for (int a = 0; a < NI; ++a) y[a] = ld3(gi + IOFF + 3 * a);
     double rx = 0.0, ry = 0.0, rz = 0.0;
-    for (int k = 0; k < 16; ++k) {
-        const d3 x = ld3(go + 3 * k);
+    for (int k = 0; k < NO; ++k) {
+        const d3 x = ld3(go + OOFF + 3 * k);
         double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
         for (int a = 0; a < NI; ++a) {
@@ -253,7 +255,7 @@ __device__ __forceinline__ d3 d_far_n(const double* __restrict__ go, const doubl
             ay += kw * c_rules[IO][a][1];
             az += kw * c_rules[IO][a][2];
         }
-        const double w = c_rules[3][k][3];
+        const double w = c_rules[OO][k][3];
         rx += w * ax;
         ry += w * ay;
         rz += w * az;
@@ -262,7 +264,7 @@ __device__ __forceinline__ d3 d_far_n(const double* __restrict__ go, const doubl
 }
 
 __device__ __forceinline__ d3 d_far(const double* __restrict__ go, const double* __restrict__ gi, d3 n, int fc) {
-    return fc == 2 ? d_far_n<7, 2, GP7>(go, gi, n) : d_far_n<16, 3, GP16>(go, gi, n);
+    return fc == 2 ? d_far_n<7, 2, GP7, 7, 2, GP7>(go, gi, n) : d_far_n<16, 3, GP16, 16, 3, GP16>(go, gi, n);
 }
 
 }  // namespace omh

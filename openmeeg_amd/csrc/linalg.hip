@@ -725,9 +725,10 @@ __global__ void __launch_bounds__(512) k_split_pair(const float* __restrict__ Xp
 }
 
 // LDS position of 16-byte chunk c (of 4 per 32-wide row) of tile row r: the
-// chunk index is XOR-ed with (r / 4) % 4 so that the 16 rows of a fragment
-// read hit disjoint banks.
-__device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
+// chunk index is XOR-ed with ((r / 2) ^ (r / 4)) % 4 so that the rows of a
+// fragment read hit disjoint banks whether the 16-byte reads are served 16
+// lanes over 64 banks or 8 lanes over 32 banks.
+__device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ (((r >> 1) ^ (r >> 2)) & 3)); }
 
 // C (m x m, lower triangle, leading dimension ldc) -= split product of
 // operand rows r_off + [0, m), over k extent KP.  256 threads, a 128 x 128 tile, 2 x 2 waves of
@@ -1179,31 +1180,39 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     };
     // C(rows >= r, cols [r, r + w)) -= X (rows from r) L21^T for panel b whose
     // rows start at e (X in slot): lower w x w block plus the rows below
-    auto update_cols = [&](int64_t b, int slot, int64_t r, int64_t w) {
+    auto update_cols = [&](int64_t b, int slot, int64_t r, int64_t w, hipStream_t s) {
         const int64_t k = b * FACT_NB, e = k + FACT_NB;
         const int64_t ldx = ldx_of(e);
         const float* X = Xbuf[slot] + (r - e);
         const float* L21 = A + r + k * lda;
         float* C = A + r + r * lda;
-        gemm<float>(false, true, true, w, w, FACT_NB, -1.f, X, ldx, L21, lda, 1.f, C, lda, st);
+        gemm<float>(false, true, true, w, w, FACT_NB, -1.f, X, ldx, L21, lda, 1.f, C, lda, s);
         if (n - r > w) gemm<float>(false, true, false, n - r - w, w, FACT_NB, -1.f, X + w, ldx, L21, lda, 1.f, C + w,
-                                   lda, st);
+                                   lda, s);
     };
-    diag_factor<float>(A, lda, 0, (int)std::min<int64_t>(FACT_NB, n), Dinv, sgn, info, tiny, st);
-    panel(0, 0, st);
+    // everything of pair (p, p + 1) that touches only its own 512 columns:
+    // block p and its panel, block p + 1's columns with that panel, block
+    // p + 1 and its panel
+    auto pair_prologue = [&](int64_t p, hipStream_t s) {
+        const int64_t kp = p * FACT_NB;
+        diag_factor<float>(A, lda, kp, (int)std::min<int64_t>(FACT_NB, n - kp), Dinv, sgn, info, tiny, s);
+        panel(p, 0, s);
+        const int64_t q = p + 1, kq = q * FACT_NB;
+        if (q >= nblk) return;
+        const int nbq = (int)std::min<int64_t>(FACT_NB, n - kq);
+        update_cols(p, 0, kq, nbq, s);
+        diag_factor<float>(A, lda, kq, nbq, Dinv, sgn, info, tiny, s);
+        panel(q, 1, s);
+    };
+    pair_prologue(0, st);
     for (int64_t p = 0; p + 1 < nblk; p += 2) {
         const int64_t q = p + 1, kq = q * FACT_NB;
-        const int nbq = (int)std::min<int64_t>(FACT_NB, n - kq);
-        // (a1) block q's columns with panel p, then block q and its panel
-        update_cols(p, 0, kq, nbq);
-        diag_factor<float>(A, lda, kq, nbq, Dinv, sgn, info, tiny, st);
-        panel(q, 1, st);
-        const int64_t r0 = kq + nbq;   // first column of the next pair
+        const int64_t r0 = kq + std::min<int64_t>(FACT_NB, n - kq);   // first column of the next pair
         if (r0 >= n) break;
         const int64_t w2 = std::min<int64_t>(2 * FACT_NB, n - r0);
         // (a2) the next pair's columns with both panels
-        update_cols(p, 0, r0, w2);
-        update_cols(q, 1, r0, w2);
+        update_cols(p, 0, r0, w2, st);
+        update_cols(q, 1, r0, w2, st);
         // split operands of the rank-512 update of the rows below the next pair
         const int64_t r2 = r0 + w2, mb = n - r2;
         if (mb > 0) {
@@ -1213,14 +1222,12 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
                                                                Bh, Bl, sc);
             OMH_LAUNCH_CHECK();
         }
-        // the next pair's first block and panel on the side stream (the panel
-        // buffers are free once the split operands exist)
+        // the next pair's prologue on the side stream (its 512 columns are
+        // outside the SYRK's region; the panel buffers are free once the
+        // split operands exist)
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
-        const int64_t pn = p + 2;
-        diag_factor<float>(A, lda, pn * FACT_NB, (int)std::min<int64_t>(FACT_NB, n - pn * FACT_NB), Dinv, sgn, info,
-                           tiny, la.side);
-        panel(pn, 0, la.side);
+        pair_prologue(p + 2, la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
         // (b) the rest of the trailing matrix
         if (mb > 0) {
