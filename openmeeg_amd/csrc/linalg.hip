@@ -353,109 +353,254 @@ __global__ void __launch_bounds__(256) k_gemm_grouped(const GemmDesc* __restrict
 
 // ---------------------------------------------------------------------------
 // diagonal block: signed Cholesky A11 = L11 S1 L11^T and Linv = L11^-1.
-// One workgroup of 512 threads; the NB x NB block lives in global memory
-// (L2 resident).  Inner panels of IB = 32 columns: the 32 x 32 diagonal
-// piece is factored in LDS, the rows below it are solved row-wise and kept
-// in LDS (transposed, conflict-free), and the rest of the block is updated
-// from LDS.  Linv is built by a second kernel, one workgroup per 32-column
-// block (diag_inverse_body).
+// Both kernels are latency bound (one NB x NB block, about 11 MFLOP), so
+// they are built to keep the serial chain short.
+// k_diag_factor: one workgroup; inner panels of IB = 32 columns:
+//   1. the panel (rows c0.., IB columns) is read into LDS, transposed;
+//   2. one wave factors the IB x IB pivot block in registers (lane i holds
+//      the symmetric row i, row j is broadcast with readlane, so the 32
+//      pivot steps need no workgroup barrier) and inverts it by forward
+//      substitution (lane j holds column j of the inverse Mi);
+//   3. the rows below: L_r = A_r Mi^T S1 (one thread per row);
+//   4. the rest of the block -= L S L^T (4 x 4 register tiles over the lower
+//      triangle; A is read and written in global memory, L2 resident).
+// The IB x IB inverses are stored as the diagonal IB-blocks of Linv.
+// k_diag_inverse: one workgroup per IB-column block c of Linv; the column
+// block is built in LDS by block forward substitution from those inverses:
+// X_rc = -Mi_r sum_{c <= q < r} L_rq X_qc.
 // ---------------------------------------------------------------------------
 constexpr int IB = 32;
-constexpr int DIAG_THREADS = 512;
+constexpr int DF_THREADS = 256;     // factor: one wave per SIMD, so that the
+                                    // workgroup fits beside a running GEMM
+                                    // workgroup on one CU
+constexpr int DIAG_THREADS = 512;   // inverse
+constexpr int PT_LD = FACT_NB + 4;   // row stride of the transposed panel (16-byte rows)
+constexpr int DT_TILES = 4;          // trailing 4 x 4 tiles per thread and pass
+
+__device__ __forceinline__ float lane_read(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_read(double v, int src) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// four consecutive LDS values (16-byte aligned for float, 32 for double)
+__device__ __forceinline__ void lds4(const float* p, float (&v)[4]) {
+    const float4 q = *(const float4*)p;
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void lds4(const double* p, double (&v)[4]) {
+    const double2 q0 = *(const double2*)p, q1 = *(const double2*)(p + 2);
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+}
+
+// Step 2 (wave 0).  Pt[l][i] (l, i < IB) holds the symmetric pivot block,
+// padded with the identity beyond w.  Lanes 0..31 hold rows; lanes 32..63
+// repeat them and store nothing.  L replaces the pivot block in Pt (row i
+// in Pt[.][i]; entries above the diagonal are not meaningful) and is also
+// kept row-major in Ls; Mi = L^-1 is stored row-major.
+template <typename T>
+__device__ __forceinline__ void pivot_block(T (*Pt)[PT_LD], int w, T (*Mi)[IB], T (*Ls)[IB + 4], T* __restrict__ sg,
+                                            T* __restrict__ ig, int* __restrict__ info, int64_t kglob, T tiny) {
+    const int lane = threadIdx.x & 63;
+    int i = lane & (IB - 1);
+    asm volatile("" : "+v"(i));   // keeps per-lane selects inside the panel loop
+    T a[IB];
+#pragma unroll
+    for (int l = 0; l < IB; ++l) a[l] = Pt[l][i];
+    // right-looking, one pivot per step: with a = column j after the
+    // previous steps and d = a_j, A(i,l) -= a_i a_l / d and
+    // L(i,j) = a_i / (sqrt|d| s_j), s_j = sign(d)
+    // (one reciprocal and one square root per step: 1/(sqrt|d| s) = sqrt|d| / d)
+    int bad = -1;
+#pragma unroll
+    for (int j = 0; j < IB; ++j) {
+        const T d = lane_read(a[j], j);
+        const T s = d >= T(0) ? T(1) : T(-1);
+        const T id = T(1) / d;
+        const T r = sqrt(fabs(d));
+        const T g = r * id;
+        const T f = a[j] * id;
+#pragma unroll
+        for (int l = j + 1; l < IB; ++l) a[l] -= f * lane_read(a[l], j);
+        a[j] = (i == j) ? r : a[j] * g;
+        if (lane == 0) ig[j] = s * g;   // 1 / L(j, j)
+        if (j < w) {
+            if (lane == 0) sg[j] = s;
+            if (bad < 0 && !(fabs(d) > tiny)) bad = j;
+        }
+    }
+    // L to LDS (row i of L in Ls[i][.] for the substitution, in Pt[.][i]
+    // for the stores)
+    if (lane < IB) {
+#pragma unroll
+        for (int l = 0; l < IB; ++l) {
+            Ls[i][l] = a[l];
+            Pt[l][i] = a[l];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Mi = L^-1 by forward substitution, lane i holds column i
+    T x[IB];
+#pragma unroll
+    for (int r = 0; r < IB; ++r) {
+        T acc0 = (r == i) ? T(1) : T(0), acc1 = T(0);
+#pragma unroll
+        for (int l = 0; l < r; ++l) {
+            if (l & 1) acc1 -= Ls[r][l] * x[l];
+            else acc0 -= Ls[r][l] * x[l];
+        }
+        x[r] = (acc0 + acc1) * ig[r];
+    }
+    if (lane < IB) {
+#pragma unroll
+        for (int r = 0; r < IB; ++r) Mi[r][i] = x[r];
+    }
+    if (lane == 0 && bad >= 0 && info[0] == 0) info[0] = (int)(kglob + bad + 1);
+}
 
 template <typename T>
-__device__ __forceinline__ void diag_body(T* __restrict__ A, int64_t lda, int nb, T* __restrict__ sgn,
-                                          int* __restrict__ info, int64_t k0, T tiny) {
-    __shared__ T D[IB][IB + 1];
-    __shared__ T ds[IB];
-    __shared__ T Pt[IB][FACT_NB - IB + 1];   // panel rows below the inner block, transposed
+__device__ __forceinline__ void diag_body(T* __restrict__ A11, int64_t lda, int nb, T* __restrict__ sgn,
+                                          int* __restrict__ info, int64_t k0, T tiny, T* __restrict__ Linv,
+                                          int64_t ldl) {
+    __shared__ __align__(16) T Pt[IB][PT_LD];
+    __shared__ T Mi[IB][IB];
+    __shared__ __align__(16) T Ls[IB][IB + 4];
+    __shared__ T sg[IB];
+    __shared__ T ig[IB];
     const int tid = threadIdx.x;
-    for (int jb = 0; jb < nb; jb += IB) {
-        const int w = min(IB, nb - jb);
-        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
-            const int i = t % IB, j = t / IB;
-            D[i][j] = (i < w && j < w && i >= j) ? A[(jb + i) + (int64_t)(jb + j) * lda] : T(0);
+    for (int c0 = 0; c0 < nb; c0 += IB) {
+        const int w = min(IB, nb - c0);
+        const int m = nb - c0;
+        // 1. panel into LDS; the pivot block is mirrored (its part above the
+        // diagonal in A is stale) and padded with the identity
+        for (int e = tid; e < m * w; e += DF_THREADS) {
+            const int r = e % m, l = e / m;
+            if (r < l) continue;
+            const T v = A11[(c0 + r) + (int64_t)(c0 + l) * lda];
+            Pt[l][r] = v;
+            if (r < w) Pt[r][l] = v;
         }
-        __syncthreads();
-        for (int j = 0; j < w; ++j) {
-            if (tid == 0) {
-                const T d = D[j][j];
-                if (!(fabs((double)d) > (double)tiny) && info[0] == 0) info[0] = (int)(k0 + jb + j + 1);
-                ds[j] = d >= T(0) ? T(1) : T(-1);
-                D[j][j] = sqrt(fabs(d));
-            }
-            __syncthreads();
-            const T ljj = D[j][j], sj = ds[j];
-            if (tid > j && tid < w) D[tid][j] = D[tid][j] / (ljj * sj);
-            __syncthreads();
-            for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
-                const int i = t % IB, l = t / IB;
-                if (l > j && i >= l && i < w) D[i][l] -= D[i][j] * sj * D[l][j];
-            }
-            __syncthreads();
-        }
-        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
-            const int i = t % IB, j = t / IB;
-            if (i < w && j < w && i >= j) A[(jb + i) + (int64_t)(jb + j) * lda] = D[i][j];
-        }
-        if (tid < w) sgn[k0 + jb + tid] = ds[tid];
-        const int r0 = jb + w;
-        const int m = nb - r0;
-        // panel rows: L_p = A_p L_D^-T S_D by row-wise substitution
-        // (the row is kept in this thread's column of Pt, never in a
-        // dynamically indexed register array)
-        for (int t = tid; t < m; t += DIAG_THREADS) {
-            const int i = r0 + t;
-            for (int j = 0; j < w; ++j) {
-                T v = A[i + (int64_t)(jb + j) * lda];
-                for (int l = 0; l < j; ++l) v -= Pt[l][t] * D[j][l];   // Pt[l][t] = L(i, jb + l) s_l
-                const T lij = v / (D[j][j] * ds[j]);
-                A[i + (int64_t)(jb + j) * lda] = lij;
-                Pt[j][t] = lij * ds[j];
+        if (w < IB) {
+            for (int e = tid; e < IB * IB; e += DF_THREADS) {
+                const int r = e % IB, l = e / IB;
+                if (r >= w || l >= w) Pt[l][r] = (r == l) ? T(1) : T(0);
             }
         }
         __syncthreads();
-        // trailing update of the rest of the block (lower part) from LDS:
-        // A(i,l) -= sum_j L(i,j) s_j L(l,j); Pt holds L*s, so use
-        // L(l,j) = Pt[j][l] * s_j.
-        for (int64_t t = tid; t < (int64_t)m * m; t += DIAG_THREADS) {
-            const int i = (int)(t % m), l = (int)(t / m);
-            if (i < l) continue;
-            T v = A[(r0 + i) + (int64_t)(r0 + l) * lda];
-            for (int j = 0; j < w; ++j) v -= Pt[j][i] * (Pt[j][l] * ds[j]);
-            A[(r0 + i) + (int64_t)(r0 + l) * lda] = v;
+        // 2. pivot block
+        if (tid < 64) pivot_block<T>(Pt, w, Mi, Ls, sg, ig, info, k0 + c0, tiny);
+        __syncthreads();
+        for (int e = tid; e < IB * IB; e += DF_THREADS) {
+            const int r = e % IB, j = e / IB;
+            if (r < w && j < w) {
+                Linv[(c0 + r) + (int64_t)(c0 + j) * ldl] = Mi[r][j];
+                if (r >= j) A11[(c0 + r) + (int64_t)(c0 + j) * lda] = Pt[j][r];
+            }
+        }
+        if (tid < w) sgn[k0 + c0 + tid] = sg[tid];
+        if (m <= w) break;   // last inner panel (w == IB otherwise)
+        // 3. rows below, one thread per row: L(r, j) = s_j sum_{l <= j}
+        // A(r, l) Mi(j, l).  Pt keeps L s.
+        const int mt = m - IB;
+        for (int rl = tid; rl < mt; rl += DF_THREADS) {
+            T row[IB];
+#pragma unroll
+            for (int l = 0; l < IB; ++l) row[l] = Pt[l][IB + rl];
+#pragma unroll 2
+            for (int j = 0; j < IB; ++j) {
+                T acc0 = 0, acc1 = 0;   // Mi(j, l) = 0 for l > j
+#pragma unroll
+                for (int l = 0; l < IB; l += 2) {
+                    acc0 += row[l] * Mi[j][l];
+                    acc1 += row[l + 1] * Mi[j][l + 1];
+                }
+                const T acc = acc0 + acc1;
+                A11[(c0 + IB + rl) + (int64_t)(c0 + j) * lda] = acc * sg[j];
+                Pt[j][IB + rl] = acc;
+            }
+        }
+        __syncthreads();
+        // 4. trailing lower triangle, rows and columns IB.. of the panel
+        const int nt = (mt + 3) >> 2;
+        const int ntri = nt * (nt + 1) / 2;
+        for (int t0 = 0; t0 < ntri; t0 += DF_THREADS * DT_TILES) {
+            T acc[DT_TILES][4][4];
+            int R[DT_TILES], C[DT_TILES];
+#pragma unroll
+            for (int u = 0; u < DT_TILES; ++u) {
+                const int t = t0 + u * DF_THREADS + tid;
+                int rr = -1, cc = 0;
+                if (t < ntri) {
+                    // t = q (q + 1) / 2 + p, p <= q: tile (nt-1-p, nt-1-q), so
+                    // that consecutive lanes take consecutive row tiles
+                    int q = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+                    while (q * (q + 1) / 2 > t) --q;
+                    while ((q + 1) * (q + 2) / 2 <= t) ++q;
+                    rr = nt - 1 - (t - q * (q + 1) / 2);
+                    cc = nt - 1 - q;
+                }
+                R[u] = rr;
+                C[u] = cc;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int r = 4 * rr + a, c = 4 * cc + b;
+                        acc[u][a][b] = (rr >= 0 && r < mt && r >= c)
+                                           ? A11[(c0 + IB + r) + (int64_t)(c0 + IB + c) * lda]
+                                           : T(0);
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < DT_TILES; ++u) {
+                if (R[u] < 0) continue;
+#pragma unroll 8
+                for (int j = 0; j < IB; ++j) {
+                    T x[4], y[4];
+                    lds4(&Pt[j][IB + 4 * R[u]], x);
+                    lds4(&Pt[j][IB + 4 * C[u]], y);
+                    const T s = sg[j];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) y[b] *= s;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) acc[u][a][b] -= x[a] * y[b];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < DT_TILES; ++u) {
+                const int rr = R[u], cc = C[u];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int r = 4 * rr + a, c = 4 * cc + b;
+                        if (rr >= 0 && r < mt && r >= c)
+                            A11[(c0 + IB + r) + (int64_t)(c0 + IB + c) * lda] = acc[u][a][b];
+                    }
+            }
         }
         __syncthreads();
     }
 }
 
-// Inverse of the lower-triangular w x w block Ls into Out (zero above the
-// diagonal), by forward substitution: thread j < IB owns column j.
-template <typename T>
-__device__ __forceinline__ void tri_inv_block(const T (*Ls)[IB + 1], T (*Out)[IB + 1], int w, int tid) {
-    if (tid >= IB) return;
-    const int j = tid;
-    for (int i = 0; i < IB; ++i) Out[i][j] = T(0);
-    if (j >= w) return;
-    Out[j][j] = T(1) / Ls[j][j];
-    for (int i = j + 1; i < w; ++i) {
-        T v = 0;
-        for (int l = j; l < i; ++l) v += Ls[i][l] * Out[l][j];
-        Out[i][j] = -v / Ls[i][i];
-    }
-}
-
-// Column block cb (IB columns) of Linv = L11^-1 for the factored nb x nb
-// diagonal block L11 (lower triangle of A).  Block forward substitution with
-// the block column X kept in LDS: X_c = L_cc^-1, then for each row block r > c
-// X_r = -L_rr^-1 sum_{c <= q < r} L_rq X_q.  Rows above the block are zero.
+// Column block cb (IB columns) of Linv for the factored nb x nb diagonal
+// block (lower triangle of A); the diagonal IB-blocks of Linv are already
+// stored.  Rows above the column block are zero.
 template <typename T>
 __device__ __forceinline__ void diag_inverse_body(const T* __restrict__ A, int64_t lda, int nb, T* __restrict__ Linv,
                                                   int64_t ldl, int cb) {
-    __shared__ T Xs[FACT_NB][IB + 1];
-    __shared__ T Ls[IB][IB + 1];
-    __shared__ T Di[IB][IB + 1];
-    __shared__ T Ts[IB][IB + 1];
+    __shared__ T Xs[FACT_NB][IB + 1];           // rows c0.. of the column block
+    __shared__ T Lr[IB][FACT_NB - IB + 1];      // L(r0 + i, c0 + q), q < r0 - c0
+    __shared__ T Mr[IB][IB + 1];                // Mi of block row r0
+    __shared__ T Tt[IB][IB + 1];
     const int tid = threadIdx.x;
     const int c0 = cb * IB;
     if (c0 >= nb) return;
@@ -466,52 +611,56 @@ __device__ __forceinline__ void diag_inverse_body(const T* __restrict__ A, int64
     }
     for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
         const int i = t % IB, j = t / IB;
-        Ls[i][j] = (i < wc && j < wc && i >= j) ? A[(c0 + i) + (int64_t)(c0 + j) * lda] : T(0);
+        Xs[i][j] = (i < wc && j < wc) ? Linv[(c0 + i) + (int64_t)(c0 + j) * ldl] : T(0);
     }
-    __syncthreads();
-    tri_inv_block<T>(Ls, Di, wc, tid);
-    __syncthreads();
-    for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
-        const int i = t % IB, j = t / IB;
-        if (i < wc) Xs[c0 + i][j] = Di[i][j];
-    }
-    __syncthreads();
+    const int ti = tid / (IB / 2), tj = 2 * (tid % (IB / 2));   // 32 rows x 16 column pairs
     for (int r0 = c0 + IB; r0 < nb; r0 += IB) {
         const int rw = min(IB, nb - r0);
-        // T = L(r0.., c0..r0) X(c0..r0, :); L(r0+i, q) is read coalesced over i
+        const int K = r0 - c0;
+        for (int t = tid; t < IB * K; t += DIAG_THREADS) {
+            const int i = t % IB, q = t / IB;
+            Lr[i][q] = (i < rw) ? A[(r0 + i) + (int64_t)(c0 + q) * lda] : T(0);
+        }
         for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
             const int i = t % IB, j = t / IB;
-            T v = 0;
-            if (i < rw && j < wc)
-                for (int q = c0; q < r0; ++q) v += A[(r0 + i) + (int64_t)q * lda] * Xs[q][j];
-            Ts[i][j] = v;
-            Ls[i][j] = (i < rw && j < rw && i >= j) ? A[(r0 + i) + (int64_t)(r0 + j) * lda] : T(0);
+            Mr[i][j] = (i < rw && j < rw) ? Linv[(r0 + i) + (int64_t)(r0 + j) * ldl] : T(0);
         }
         __syncthreads();
-        tri_inv_block<T>(Ls, Di, rw, tid);
-        __syncthreads();
-        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
-            const int i = t % IB, j = t / IB;
-            if (i < rw && j < wc) {
-                T v = 0;
-                for (int l = 0; l <= i; ++l) v += Di[i][l] * Ts[l][j];
-                Xs[r0 + i][j] = -v;
+        {
+            T a0 = 0, a1 = 0;
+            for (int q = 0; q < K; ++q) {
+                const T l = Lr[ti][q];
+                a0 += l * Xs[q][tj];
+                a1 += l * Xs[q][tj + 1];
             }
+            Tt[ti][tj] = a0;
+            Tt[ti][tj + 1] = a1;
+        }
+        __syncthreads();
+        {
+            T a0 = 0, a1 = 0;
+            for (int l = 0; l <= ti; ++l) {
+                const T mm = Mr[ti][l];
+                a0 += mm * Tt[l][tj];
+                a1 += mm * Tt[l][tj + 1];
+            }
+            Xs[K + ti][tj] = -a0;
+            Xs[K + ti][tj + 1] = -a1;
         }
         __syncthreads();
     }
-    const int rows = nb - c0;
+    const int rows = nb - c0 - IB;
     for (int t = tid; t < rows * wc; t += DIAG_THREADS) {
         const int i = t % rows, j = t / rows;
-        Linv[(c0 + i) + (int64_t)(c0 + j) * ldl] = Xs[c0 + i][j];
+        Linv[(c0 + IB + i) + (int64_t)(c0 + j) * ldl] = Xs[IB + i][j];
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(DIAG_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
+__global__ void __launch_bounds__(DF_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
                                                               T* __restrict__ sgn, int* __restrict__ info, int64_t k0,
-                                                              T tiny) {
-    diag_body<T>(A, lda, nb, sgn, info, k0, tiny);
+                                                              T tiny, T* __restrict__ Linv, int64_t ldl) {
+    diag_body<T>(A, lda, nb, sgn, info, k0, tiny, Linv, ldl);
 }
 
 template <typename T>
@@ -522,9 +671,9 @@ __global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse(const T* __restri
 
 // One workgroup per diagonal block of several factorisations.
 template <typename T>
-__global__ void __launch_bounds__(DIAG_THREADS) k_diag_grouped(const DiagDesc* __restrict__ d, T tiny) {
+__global__ void __launch_bounds__(DF_THREADS) k_diag_grouped(const DiagDesc* __restrict__ d, T tiny) {
     const DiagDesc& g = d[blockIdx.x];
-    diag_body<T>((T*)g.A11, g.lda, g.nb, (T*)g.sgn, g.info, g.k0, tiny);
+    diag_body<T>((T*)g.A11, g.lda, g.nb, (T*)g.sgn, g.info, g.k0, tiny, (T*)g.Linv, FACT_NB);
 }
 
 // Column block blockIdx.x of the inverse of diagonal block blockIdx.y.
@@ -651,7 +800,7 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 
 // ---------------------------------------------------------------------------
 // Split-FP16 trailing update.  The SYRK C -= X S X^T of a factor step (X the
-// two panels of a block pair, 512 columns) runs on the FP16 MFMA
+// S_PANELS panels of a block group, S_KP columns) runs on the FP16 MFMA
 // (v_mfma_f32_16x16x32_f16, FP32 accumulation): row i of X is scaled by a
 // power of two s_i >= max_k |X_ik| and split into FP16 parts,
 // X_ik / s_i = h + l to within 2^-22 of the row maximum, and
@@ -663,28 +812,43 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 constexpr int SK = 32;        // k-slice of the FP16 MFMA
 constexpr int S_TILE = 128;   // output tile of the split SYRK
-constexpr int S_KP = 2 * FACT_NB;   // k extent of the operands: two panels
+// Panels per split update (its rank / FACT_NB).  n = 102400: 2 panels
+// 1.48 s, 3 panels 1.51 s, 4 panels 1.34 s for the whole factorisation
+// (larger groups halve the trailing-matrix traffic again but lengthen the
+// group prologue).
+#ifndef OMH_SPLIT_PANELS
+#define OMH_SPLIT_PANELS 4
+#endif
+constexpr int S_PANELS = OMH_SPLIT_PANELS;
+constexpr int S_KP = S_PANELS * FACT_NB;      // k extent of the operands
 
-// Operands of a rank-512 update from two consecutive panels X_p, X_q (FP32,
-// m rows from row offsets offp, offq): row i gets one power-of-two scale for
-// both panels; k < 256 come from X_p (signs sgnp), k >= 256 from X_q.  512
-// threads per 64 rows: thread (row t % 64, k eighth t / 64).  Padding rows
+// The S_PANELS consecutive panels of a split update (FP32, their rows
+// starting at the update's first row at offsets off[]).
+struct SplitSrc {
+    const float* X[S_PANELS];
+    int64_t ldx[S_PANELS];
+    int64_t off[S_PANELS];
+    const float* sgn[S_PANELS];
+};
+
+// Operands of the split update: row i gets one power-of-two scale over all
+// S_PANELS panels; k in [256 p, 256 p + 256) comes from panel p.  64 rows per
+// block, thread (row t % 64, 64-wide k group t / 64).  Padding rows
 // (m <= i < mp) are zero.
-__global__ void __launch_bounds__(512) k_split_pair(const float* __restrict__ Xp, int64_t ldxp, int64_t offp,
-                                                    const float* __restrict__ Xq, int64_t ldxq, int64_t offq,
-                                                    int64_t m, const float* __restrict__ sgnp,
-                                                    const float* __restrict__ sgnq, int64_t mp,
-                                                    _Float16* __restrict__ Ah, _Float16* __restrict__ Al,
-                                                    _Float16* __restrict__ Bh, _Float16* __restrict__ Bl,
-                                                    float* __restrict__ scale) {
-    __shared__ float red[8][64];
+__global__ void __launch_bounds__(64 * 4 * S_PANELS) k_split_group(SplitSrc src, int64_t m, int64_t mp,
+                                                                  _Float16* __restrict__ Ah,
+                                                                  _Float16* __restrict__ Al,
+                                                                  _Float16* __restrict__ Bh,
+                                                                  _Float16* __restrict__ Bl,
+                                                                  float* __restrict__ scale) {
+    __shared__ float red[4 * S_PANELS][64];
     const int ri = threadIdx.x & 63, kq = threadIdx.x >> 6;
     const int64_t row = (int64_t)blockIdx.x * 64 + ri;
-    const bool second = kq >= 4;
-    const float* X = second ? Xq + offq : Xp + offp;
-    const int64_t ldx = second ? ldxq : ldxp;
-    const float* sg = second ? sgnq : sgnp;
-    const int kb = (kq & 3) * 64;   // first column inside its panel
+    const int pn = kq >> 2;                 // panel
+    const float* X = src.X[pn] + src.off[pn];
+    const int64_t ldx = src.ldx[pn];
+    const float* sg = src.sgn[pn];
+    const int kb = (kq & 3) * 64;           // first column inside its panel
     float v[64];
     float mx = 0.f;
 #pragma unroll
@@ -696,7 +860,7 @@ __global__ void __launch_bounds__(512) k_split_pair(const float* __restrict__ Xp
     __syncthreads();
     mx = 0.f;
 #pragma unroll
-    for (int g = 0; g < 8; ++g) mx = fmaxf(mx, red[g][ri]);
+    for (int g = 0; g < 4 * S_PANELS; ++g) mx = fmaxf(mx, red[g][ri]);
     int e = 0;
     if (mx > 0.f) (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
     const float inv = ldexpf(1.f, -e);
@@ -1039,11 +1203,11 @@ void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t 
     if (ng <= 0) return;
     const dim3 inv(FACT_NB / IB, (unsigned)ng);
     if (f32) {
-        k_diag_grouped<float><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-30f);
+        k_diag_grouped<float><<<ng, DF_THREADS, 0, st>>>(d_descs, 1e-30f);
         OMH_LAUNCH_CHECK();
         k_diag_inverse_grouped<float><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
     } else {
-        k_diag_grouped<double><<<ng, DIAG_THREADS, 0, st>>>(d_descs, 1e-290);
+        k_diag_grouped<double><<<ng, DF_THREADS, 0, st>>>(d_descs, 1e-290);
         OMH_LAUNCH_CHECK();
         k_diag_inverse_grouped<double><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
     }
@@ -1082,11 +1246,13 @@ template void gemm<float>(bool, bool, bool, int64_t, int64_t, int64_t, float, co
 // (the look-ahead computes the next panel while the current one is in use).
 static size_t sign_span(int64_t n) { return (size_t)((n + 63) / 64 * 64); }
 
-// Split-FP16 operands of one block pair (FP32 factorisation): four arrays of
-// mp x S_KP halves and mp row scales, mp = n rounded up to S_TILE.
+// Split-FP16 operands of one block group (FP32 factorisation): four arrays
+// of mp x S_KP halves and mp row scales, mp = n rounded up to S_TILE; then
+// the panels beyond the two of the dense workspace.
 static size_t split_set_bytes(int64_t n) {
     const size_t mp = (size_t)cdiv(n, S_TILE) * S_TILE;
-    return 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float);
+    return 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float) +
+           (size_t)(S_PANELS > 2 ? S_PANELS - 2 : 0) * ((size_t)n * FACT_NB * sizeof(float) + 256);
 }
 
 static size_t dense_workspace_bytes(int64_t n, size_t elem) {
@@ -1130,7 +1296,7 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
     T* A11 = A + k + k * lda;
     T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
     ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
-    k_diag_factor<T><<<1, DIAG_THREADS, 0, st>>>(A11, lda, nb, sgn, info, k, tiny);
+    k_diag_factor<T><<<1, DF_THREADS, 0, st>>>(A11, lda, nb, sgn, info, k, tiny, Li, FACT_NB);
     OMH_LAUNCH_CHECK();
     k_diag_inverse<T><<<cdiv(nb, IB), DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB);
     OMH_LAUNCH_CHECK();
@@ -1141,17 +1307,17 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 // block k+1 is factored on the side stream while the rest of the trailing
 // matrix is updated on the caller's stream.
 // FP32 factorisation with split-FP16 trailing updates, in super-steps of
-// two blocks (p, q = p + 1) so that the large update has rank 512 and reads
-// and writes the trailing matrix half as often as rank-256 steps would:
-//   (a1) block q's columns -= panel p; diagonal block q; panel q;
-//   (a2) the next pair's 512 columns -= panels p and q (FP32 GEMMs);
-//   the next pair's first block and panel on the side stream, while
-//   (b) the rest -= [X_p X_q] S [X_p X_q]^T as one split-FP16 SYRK.
+// S_PANELS blocks (a group) so that the large update has rank S_KP and reads
+// and writes the trailing matrix once per S_PANELS blocks:
+//   prologue of a group: each block in turn is factored, forms its panel and
+//     updates the group's later columns (FP32 GEMMs);
+//   the next group's columns -= all panels of the group (FP32 GEMMs);
+//   the next group's prologue on the side stream, while the rest of the
+//   trailing matrix -= [X_1 .. X_P] S [X_1 .. X_P]^T as one split-FP16 SYRK.
 static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
     float* Dinv = (float*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     float* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
-    float* Xbuf[2] = {(float*)work, sgn + sign_span(n)};
     const int64_t mp_max = cdiv(n, S_TILE) * S_TILE;
     char* sp = (char*)work + (dense_workspace_bytes(n, 4) + 64 + 255) / 256 * 256;
     const size_t arr = (size_t)mp_max * S_KP * sizeof(_Float16);
@@ -1160,76 +1326,93 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     _Float16* Bh = (_Float16*)(sp + 2 * arr);
     _Float16* Bl = (_Float16*)(sp + 3 * arr);
     float* sc = (float*)(sp + 4 * arr);
+    float* Xbuf[S_PANELS];
+    Xbuf[0] = (float*)work;
+    Xbuf[1] = sgn + sign_span(n);
+    {
+        char* xp = (char*)(sc + mp_max);
+        for (int b = 2; b < S_PANELS; ++b) {
+            xp = (char*)(((uintptr_t)xp + 255) / 256 * 256);
+            Xbuf[b] = (float*)xp;
+            xp += (size_t)n * FACT_NB * sizeof(float);
+        }
+    }
     int* info = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
     const float tiny = 1e-30f;
     LookAhead& la = lookahead();
+    auto start_of = [&](int64_t b) { return b * FACT_NB; };
+    auto width_of = [&](int64_t b) { return (int)std::min<int64_t>(FACT_NB, n - b * FACT_NB); };
     auto ldx_of = [&](int64_t e) { return (n - e + 3) / 4 * 4; };   // panel whose rows start at e
-    // panel of block b into Xbuf[slot]: X = A21 Linv^T, L21 = X S1
-    auto panel = [&](int64_t b, int slot, hipStream_t s) {
-        const int64_t k = b * FACT_NB;
-        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+    // panel of block b into Xbuf[b % S_PANELS]: X = A21 Linv^T, L21 = X S1
+    auto panel = [&](int64_t b, hipStream_t s) {
+        const int64_t k = start_of(b);
+        const int nb = width_of(b);
         const int64_t e = k + nb, m = n - e;
         if (m <= 0) return;
+        float* X = Xbuf[b % S_PANELS];
         const float* Li = Dinv + (size_t)b * FACT_NB * FACT_NB;
         float* A21 = A + e + k * lda;
-        gemm<float>(false, true, false, m, nb, nb, 1.f, A21, lda, Li, FACT_NB, 0.f, Xbuf[slot], ldx_of(e), s);
-        k_scale_cols<float><<<grid_for(m * nb), 256, 0, s>>>(Xbuf[slot], ldx_of(e), A21, lda, m, nb, sgn + k);
+        gemm<float>(false, true, false, m, nb, nb, 1.f, A21, lda, Li, FACT_NB, 0.f, X, ldx_of(e), s);
+        k_scale_cols<float><<<grid_for(m * nb), 256, 0, s>>>(X, ldx_of(e), A21, lda, m, nb, sgn + k);
         OMH_LAUNCH_CHECK();
     };
-    // C(rows >= r, cols [r, r + w)) -= X (rows from r) L21^T for panel b whose
-    // rows start at e (X in slot): lower w x w block plus the rows below
-    auto update_cols = [&](int64_t b, int slot, int64_t r, int64_t w, hipStream_t s) {
-        const int64_t k = b * FACT_NB, e = k + FACT_NB;
+    // C(rows >= r, cols [r, r + w)) -= X (rows from r) L21^T with the panel
+    // of (full) block b: lower w x w block plus the rows below
+    auto update_cols = [&](int64_t b, int64_t r, int64_t w, hipStream_t s) {
+        const int64_t k = start_of(b), e = k + FACT_NB;
         const int64_t ldx = ldx_of(e);
-        const float* X = Xbuf[slot] + (r - e);
+        const float* X = Xbuf[b % S_PANELS] + (r - e);
         const float* L21 = A + r + k * lda;
         float* C = A + r + r * lda;
         gemm<float>(false, true, true, w, w, FACT_NB, -1.f, X, ldx, L21, lda, 1.f, C, lda, s);
         if (n - r > w) gemm<float>(false, true, false, n - r - w, w, FACT_NB, -1.f, X + w, ldx, L21, lda, 1.f, C + w,
                                    lda, s);
     };
-    // everything of pair (p, p + 1) that touches only its own 512 columns:
-    // block p and its panel, block p + 1's columns with that panel, block
-    // p + 1 and its panel
-    auto pair_prologue = [&](int64_t p, hipStream_t s) {
-        const int64_t kp = p * FACT_NB;
-        diag_factor<float>(A, lda, kp, (int)std::min<int64_t>(FACT_NB, n - kp), Dinv, sgn, info, tiny, s);
-        panel(p, 0, s);
-        const int64_t q = p + 1, kq = q * FACT_NB;
-        if (q >= nblk) return;
-        const int nbq = (int)std::min<int64_t>(FACT_NB, n - kq);
-        update_cols(p, 0, kq, nbq, s);
-        diag_factor<float>(A, lda, kq, nbq, Dinv, sgn, info, tiny, s);
-        panel(q, 1, s);
+    // everything of group g that touches only its own columns
+    auto group_prologue = [&](int64_t g, hipStream_t s) {
+        const int64_t b0 = g * S_PANELS, b1 = std::min<int64_t>(b0 + S_PANELS, nblk);
+        const int64_t gend = std::min<int64_t>(start_of(b1), n);
+        for (int64_t b = b0; b < b1; ++b) {
+            diag_factor<float>(A, lda, start_of(b), width_of(b), Dinv, sgn, info, tiny, s);
+            panel(b, s);
+            const int64_t r = start_of(b) + width_of(b);
+            if (b + 1 < b1 && r < gend) update_cols(b, r, gend - r, s);
+        }
     };
-    pair_prologue(0, st);
-    for (int64_t p = 0; p + 1 < nblk; p += 2) {
-        const int64_t q = p + 1, kq = q * FACT_NB;
-        const int64_t r0 = kq + std::min<int64_t>(FACT_NB, n - kq);   // first column of the next pair
+    const int64_t ngroups = cdiv(nblk, S_PANELS);
+    group_prologue(0, st);
+    for (int64_t g = 0; g + 1 < ngroups; ++g) {
+        const int64_t b0 = g * S_PANELS;
+        const int64_t r0 = start_of(b0 + S_PANELS);   // first column of the next group
         if (r0 >= n) break;
-        const int64_t w2 = std::min<int64_t>(2 * FACT_NB, n - r0);
-        // (a2) the next pair's columns with both panels
-        update_cols(p, 0, r0, w2, st);
-        update_cols(q, 1, r0, w2, st);
-        // split operands of the rank-512 update of the rows below the next pair
+        const int64_t w2 = std::min<int64_t>(S_KP, n - r0);
+        // the next group's columns with all panels of this group
+        for (int64_t b = b0; b < b0 + S_PANELS; ++b) update_cols(b, r0, w2, st);
+        // split operands of the rank-S_KP update of the rows below the next group
         const int64_t r2 = r0 + w2, mb = n - r2;
         if (mb > 0) {
             const int64_t mp = cdiv(mb, S_TILE) * S_TILE;
-            k_split_pair<<<(unsigned)(mp / 64), 512, 0, st>>>(Xbuf[0], ldx_of(kq), r2 - kq, Xbuf[1], ldx_of(r0),
-                                                               r2 - r0, mb, sgn + p * FACT_NB, sgn + kq, mp, Ah, Al,
-                                                               Bh, Bl, sc);
+            SplitSrc src;
+            for (int i = 0; i < S_PANELS; ++i) {
+                const int64_t b = b0 + i, e = start_of(b) + FACT_NB;
+                src.X[i] = Xbuf[b % S_PANELS];
+                src.ldx[i] = ldx_of(e);
+                src.off[i] = r2 - e;
+                src.sgn[i] = sgn + start_of(b);
+            }
+            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, mb, mp, Ah, Al, Bh, Bl, sc);
             OMH_LAUNCH_CHECK();
         }
-        // the next pair's prologue on the side stream (its 512 columns are
+        // the next group's prologue on the side stream (its columns are
         // outside the SYRK's region; the panel buffers are free once the
         // split operands exist)
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
-        pair_prologue(p + 2, la.side);
+        group_prologue(g + 1, la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
-        // (b) the rest of the trailing matrix
+        // the rest of the trailing matrix
         if (mb > 0) {
             const int64_t mp = cdiv(mb, S_TILE) * S_TILE, ntm = cdiv(mb, S_TILE);
             const int64_t tiles = ntm * (ntm + 1) / 2;
