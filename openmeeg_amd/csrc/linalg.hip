@@ -190,6 +190,16 @@ __device__ __forceinline__ void map_tile(const TileMap& map, int64_t t, int64_t&
         case 3: tn = map.tn0; tm = t; break;
         case 4: lower_band_tile(t, map.ntm, tm, tn); break;
         case 5: full_band_tile(t, map.ntm, map.tn0, tm, tn); break;   // tn0 holds the tile columns
+        case 6: {   // lower part of the first tn0 tile columns, row by row
+            const int64_t nc = map.tn0, top = nc * (nc + 1) / 2;
+            if (t < top) {
+                upper_tile(t, tn, tm);
+            } else {
+                tm = nc + (t - top) / nc;
+                tn = (t - top) % nc;
+            }
+            break;
+        }
         default: tn = t / map.ntm; tm = t - tn * map.ntm; break;
     }
 }
@@ -894,17 +904,18 @@ __global__ void __launch_bounds__(64 * 4 * S_PANELS) k_split_group(SplitSrc src,
 // lanes over 64 banks or 8 lanes over 32 banks.
 __device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ (((r >> 1) ^ (r >> 2)) & 3)); }
 
-// C (m x m, lower triangle, leading dimension ldc) -= split product of
-// operand rows r_off + [0, m), over k extent KP.  256 threads, a 128 x 128 tile, 2 x 2 waves of
-// 64 x 64 (4 x 4 MFMA blocks, 3 products each), k-slices double-buffered
-// in LDS.  Operand rows up to r_off + 128 * ceil(m / 128) exist (zero
-// padding), so loads need no guards.
+// C (m x m, leading dimension ldc) -= split product of operand rows
+// r_off + [0, m), over k extent KP, on the tiles of map: the lower triangle,
+// or (kind 6) the lower part of a strip of whole tile columns; entries with
+// row < col or an index >= m are not written.  256 threads, a 128 x 128
+// tile, 2 x 2 waves of 64 x 64 (4 x 4 MFMA blocks, 3 products each),
+// k-slices double-buffered in LDS.  Operand rows up to
+// r_off + 128 * ceil(m / 128) exist (zero padding), so loads need no guards.
 template <int KP>
-__global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
-                                                    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
-                                                    const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
-                                                    const float* __restrict__ scale, float* __restrict__ C,
-                                                    int64_t ldc, TileMap map) {
+__device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
+                                                const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
+                                                const _Float16* __restrict__ Bl, const float* __restrict__ scale,
+                                                float* __restrict__ C, int64_t ldc, TileMap map) {
     __shared__ half8_t S[2][4][S_TILE * 4];
     int64_t t = blockIdx.x;
     if (map.kind >= 4) {
@@ -1029,6 +1040,26 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
                 }
             }
         }
+}
+
+// The trailing matrix below the next block group.
+template <int KP>
+__global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
+                                                    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
+                                                    const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                                    const float* __restrict__ scale, float* __restrict__ C,
+                                                    int64_t ldc, TileMap map) {
+    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map);
+}
+
+// The next block group's columns (a strip of whole tile columns, map kind 6).
+template <int KP>
+__global__ void __launch_bounds__(256) k_syrk_strip(int64_t m, int64_t mp, int64_t r_off,
+                                                    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
+                                                    const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                                    const float* __restrict__ scale, float* __restrict__ C,
+                                                    int64_t ldc, TileMap map) {
+    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map);
 }
 
 __global__ void k_set_int(int* p, int v) { p[0] = v; }
@@ -1388,21 +1419,30 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         const int64_t r0 = start_of(b0 + S_PANELS);   // first column of the next group
         if (r0 >= n) break;
         const int64_t w2 = std::min<int64_t>(S_KP, n - r0);
-        // the next group's columns with all panels of this group
-        for (int64_t b = b0; b < b0 + S_PANELS; ++b) update_cols(b, r0, w2, st);
-        // split operands of the rank-S_KP update of the rows below the next group
-        const int64_t r2 = r0 + w2, mb = n - r2;
-        if (mb > 0) {
-            const int64_t mp = cdiv(mb, S_TILE) * S_TILE;
+        // split operands of the rank-S_KP update of all rows from r0
+        const int64_t ms = n - r0, r2 = r0 + w2, mb = n - r2;
+        const int64_t mp = cdiv(ms, S_TILE) * S_TILE;   // operand rows (zero padded)
+        {
             SplitSrc src;
             for (int i = 0; i < S_PANELS; ++i) {
                 const int64_t b = b0 + i, e = start_of(b) + FACT_NB;
                 src.X[i] = Xbuf[b % S_PANELS];
                 src.ldx[i] = ldx_of(e);
-                src.off[i] = r2 - e;
+                src.off[i] = r0 - e;
                 src.sgn[i] = sgn + start_of(b);
             }
-            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, mb, mp, Ah, Al, Bh, Bl, sc);
+            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, ms, mp, Ah, Al, Bh, Bl, sc);
+            OMH_LAUNCH_CHECK();
+        }
+        // the next group's columns: a strip of whole tile columns (w2 = S_KP
+        // whenever rows remain below it, else the strip is the remainder)
+        {
+            const int64_t ntm = cdiv(ms, S_TILE), nc = cdiv(w2, S_TILE);
+            const int64_t tiles = nc * (nc + 1) / 2 + (ntm - nc) * nc;
+            const TileMap map{6, ntm, 0, nc, tiles, cdiv(tiles, 8)};
+            ProfScope ps("factor_strip_f32s", (double)(2 * ms - w2 + 1) * (double)w2 * (double)S_KP, st);
+            k_syrk_strip<S_KP><<<(unsigned)(8 * map.per), 256, 0, st>>>(ms, mp, 0, Ah, Al, Bh, Bl, sc,
+                                                                     A + r0 + r0 * lda, lda, map);
             OMH_LAUNCH_CHECK();
         }
         // the next group's prologue on the side stream (its columns are
@@ -1414,13 +1454,13 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         OMH_HIP(hipEventRecord(la.done, la.side));
         // the rest of the trailing matrix
         if (mb > 0) {
-            const int64_t mp = cdiv(mb, S_TILE) * S_TILE, ntm = cdiv(mb, S_TILE);
+            const int64_t ntm = cdiv(mb, S_TILE);
             const int64_t tiles = ntm * (ntm + 1) / 2;
             const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
                                               : TileMap{1, ntm, 0, 0, tiles, 0};
             ProfScope ps("factor_update_f32s", (double)mb * (double)(mb + 1) * (double)S_KP, st);
             k_syrk_split<S_KP><<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
-                mb, mp, 0, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map);
+                mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map);
             OMH_LAUNCH_CHECK();
         }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
@@ -1503,6 +1543,12 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
 template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool);
 template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t, bool);
 
+// Diagonal blocks per group of the triangular solves.
+#ifndef OMH_SOLVE_GROUP
+#define OMH_SOLVE_GROUP 4
+#endif
+constexpr int SOLVE_GROUP = OMH_SOLVE_GROUP;
+
 template <typename T>
 void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st) {
     if (r <= 0) return;
@@ -1510,28 +1556,48 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     const int64_t nblk = cdiv(n, FACT_NB);
     const T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
     T* tmp = (T*)scratch(SCR_LINALG2, sizeof(T) * FACT_NB * (size_t)r);
+    // Blocks are taken in groups of SOLVE_GROUP: inside a group block by
+    // block (the diagonal inverse, then the group's later rows), then the
+    // rest of W with the whole group in one product of depth
+    // SOLVE_GROUP x NB.
+    constexpr int64_t GW = (int64_t)SOLVE_GROUP * FACT_NB;
     // forward: W <- L^-1 W
-    for (int64_t k = 0; k < n; k += FACT_NB) {
-        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
-        const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        gemm<T>(false, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
-        k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
-        OMH_LAUNCH_CHECK();
-        const int64_t e = k + nb;
-        if (e < n) gemm<T>(false, false, false, n - e, r, nb, T(-1), L + e + k * lda, lda, W + k, ldw, T(1), W + e, ldw, st);
+    for (int64_t k0 = 0; k0 < n; k0 += GW) {
+        const int64_t gend = std::min<int64_t>(k0 + GW, n);
+        for (int64_t k = k0; k < gend; k += FACT_NB) {
+            const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+            const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+            gemm<T>(false, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
+            k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
+            OMH_LAUNCH_CHECK();
+            const int64_t e = k + nb;
+            if (e < gend)
+                gemm<T>(false, false, false, gend - e, r, nb, T(-1), L + e + k * lda, lda, W + k, ldw, T(1), W + e,
+                        ldw, st);
+        }
+        if (gend < n)
+            gemm<T>(false, false, false, n - gend, r, gend - k0, T(-1), L + gend + k0 * lda, lda, W + k0, ldw, T(1),
+                    W + gend, ldw, st);
     }
     // W <- S W
     k_scale_rows<T><<<grid_for(n * r), 256, 0, st>>>(W, ldw, n, r, sgn);
     OMH_LAUNCH_CHECK();
-    // backward: W <- L^-T W
-    const int64_t last = ((n - 1) / FACT_NB) * FACT_NB;
-    for (int64_t k = last; k >= 0; k -= FACT_NB) {
-        const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
-        const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        gemm<T>(true, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
-        k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
-        OMH_LAUNCH_CHECK();
-        if (k > 0) gemm<T>(true, false, false, k, r, nb, T(-1), L + k, lda, W + k, ldw, T(1), W, ldw, st);
+    // backward: W <- L^-T W (rows of a group already carry the later
+    // groups' contributions)
+    for (int64_t k0 = ((n - 1) / GW) * GW; k0 >= 0; k0 -= GW) {
+        const int64_t gend = std::min<int64_t>(k0 + GW, n);
+        for (int64_t k = ((gend - 1) / FACT_NB) * FACT_NB; k >= k0; k -= FACT_NB) {
+            const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
+            const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
+            gemm<T>(true, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
+            k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
+            OMH_LAUNCH_CHECK();
+            if (k > k0)
+                gemm<T>(true, false, false, k - k0, r, nb, T(-1), L + k + k0 * lda, lda, W + k, ldw, T(1), W + k0,
+                        ldw, st);
+        }
+        if (k0 > 0)
+            gemm<T>(true, false, false, k0, r, gend - k0, T(-1), L + k0, lda, W + k0, ldw, T(1), W, ldw, st);
     }
 }
 
