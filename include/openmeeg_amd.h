@@ -196,6 +196,15 @@ int omh_convert_f64_f32(const double* d_src, int64_t lds, float* d_dst, int64_t 
 int omh_sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* d_Hm, int64_t ldh,
                     int64_t n, double* d_out, int64_t ldo, void* stream);
 
+/* The right-hand sides of the adjoint solve, linsolve(HeadMat,
+ * Head2EEGMat) (gain.h:28-49, used at gain.h:84): d_out (n x ne, leading
+ * dimension ldo) = the dense transpose of the sparse ne x n matrix whose
+ * row e holds d_vals(e,k) at column d_rows(e,k), k < 3 (device arrays of
+ * 3 ne entries, rows already in the row order of d_out; repeated rows add,
+ * rows outside [0, n) are ignored). */
+int omh_sparse_transpose_dense(const int64_t* d_rows, const double* d_vals, int64_t ne, int64_t n, double* d_out,
+                               int64_t ldo, void* stream);
+
 /* Unknown-order conversions between the reference's storage and device
  * matrices.  omh_sym_unpack: packed upper SymMatrix data (symmatrix.h:52-62,
  * reference order, device pointer) -> full symmetric matrix in `ordering`.

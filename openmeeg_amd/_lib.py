@@ -56,6 +56,7 @@ _SIGS = {
     "omh_gemm_f32": ([i32, i32, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, vp], C.c_int),
     "omh_convert_f64_f32": ([vp, i64, vp, i64, i64, i64, i32, vp], C.c_int),
     "omh_sparse_rows": ([vp, vp, i64, vp, i64, i64, vp, i64, vp], C.c_int),
+    "omh_sparse_transpose_dense": ([vp, vp, i64, i64, vp, i64, vp], C.c_int),
     "omh_sym_unpack": ([vp, i64, i64, vp, i64, C.c_int, vp], C.c_int),
     "omh_sym_pack": ([vp, i64, i64, i64, C.c_int, vp, vp], C.c_int),
     "omh_permute_rows": ([vp, i64, vp, i64, i64, i64, i64, C.c_int, i32, vp], C.c_int),

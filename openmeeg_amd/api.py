@@ -571,13 +571,14 @@ def Head2EEGMat(geom, sensors):
 
 
 def _eeg_rhs(geom, h2em):
-    """E^T (n x n_e, triangles-first rows) on the device."""
+    """E^T (n x n_e, triangles-first rows) on the device, built there from the
+    3 n_e non-zeros of the Head2EEGMat (gain.h:28-49)."""
     n, ne = geom.size, h2em.nlin()
-    B = np.zeros((n, ne), order="F")
-    rows = geom.tri_first_index(h2em.cols)
-    for k in range(3):
-        np.add.at(B, (rows[:, k], np.arange(ne)), h2em.vals[:, k])
-    return _to_dev(B.ravel(order="F"))
+    rows = _to_dev(np.ascontiguousarray(geom.tri_first_index(h2em.cols), dtype=np.int64).ravel(), np.int64)
+    vals = _to_dev(np.ascontiguousarray(h2em.vals, dtype=np.float64).ravel())
+    B = _dev_f64(n * ne)
+    check(lib().omh_sparse_transpose_dense(_ptr(rows), _ptr(vals), ne, n, _ptr(B), n, _stream()))
+    return B
 
 
 def _gemm_tn(Y, D, n, ne, nd):

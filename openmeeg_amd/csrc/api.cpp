@@ -525,6 +525,17 @@ int omh_sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, con
     });
 }
 
+int omh_sparse_transpose_dense(const int64_t* d_rows, const double* d_vals, int64_t ne, int64_t n, double* d_out,
+                               int64_t ldo, void* stream) {
+    return guard([&] {
+        need(d_rows, "d_rows");
+        need(d_vals, "d_vals");
+        need(d_out, "d_out");
+        if (ne < 0 || n < 0 || ldo < std::max<int64_t>(1, n)) throw std::invalid_argument("bad sizes");
+        omh::sparse_transpose_dense(d_rows, d_vals, ne, n, d_out, ldo, S(stream));
+    });
+}
+
 static omh::Layout raw_layout(int64_t n, int64_t nv, int64_t ld, int ordering) {
     if (nv < 0 || nv > n) throw std::invalid_argument("bad vertex-unknown count");
     if (ld < n) throw std::invalid_argument("leading dimension too small");

@@ -391,56 +391,84 @@ __global__ void __launch_bounds__(256) k_d_gather(const double* __restrict__ W, 
     H[c + r * L.ld] = acc;
 }
 
-// N block (operators.h:134-148 and the loops of :272-296 / :455-480).
-// S values come from the HeadMat itself (src_temp == 0, S block present) or
-// from the dense temporary block tmp (ld = tmp_ld, triangle offsets ta0/tb0).
-__global__ void __launch_bounds__(256) k_n_block(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
-                                                 const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
-                                                 const int64_t* __restrict__ vert_unk, int64_t sa0, int64_t na,
-                                                 int64_t sb0, int64_t nb, int same, int64_t ntx, double coeff,
-                                                 const double* __restrict__ tmp, int64_t tmp_ld, int64_t ta0,
-                                                 int64_t tb0, int64_t ua0, int64_t ub0, double* __restrict__ H,
-                                                 Layout L, int use_atomic) {
-    int64_t ta, tb;
-    if (same) {
-        upper_tile(blockIdx.x, tb, ta);   // tb <= ta
-    } else {
-        ta = blockIdx.x / ntx;
-        tb = blockIdx.x - ta * ntx;
+// N block (operators.h:134-148 and the loops of :272-296 / :455-480):
+//   N(v1, v2) = -f coeff sum_{t1 ~ v1} sum_{t2 ~ v2} (cb1 . cb2) S(t1, t2) / (A1 A2)
+// (f = 1/4, or 1/2 for a vertex shared by two different meshes), evaluated
+// as two contractions so that the S block is read coalesced:
+//   stage 1: U_d(t1, v2) = sum_{t2 ~ v2} cb2[d] / A2 S(t1, t2)   (d = 0..2)
+//   stage 2: N(v1, v2) = -f coeff sum_{t1 ~ v1} (cb1 . U(t1, v2)) / A1.
+// S(i, j) = S[i + j lds] for local triangle indices of the two meshes (the
+// S block inside the HeadMat, or the dense temporary of a barrier pair).
+// Stage 1: thread = triangle i of mesh A (consecutive lanes, consecutive
+// rows of S), block row y = vertex column j of the chunk; U_d has leading
+// dimension ldu, the three arrays nj columns apart.
+__global__ void __launch_bounds__(256) k_n_stage1(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
+                                                  const AdjRec* __restrict__ adj, int64_t slot0, int64_t nt1,
+                                                  const double* __restrict__ S, int64_t lds, int64_t tb0,
+                                                  double* __restrict__ U, int64_t ldu, int64_t nj) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= nt1) return;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    for (int64_t p = adj_off[slot0 + j]; p < adj_off[slot0 + j + 1]; ++p) {
+        const AdjRec e = adj[p];
+        const double s = S[i + (e.tri - tb0) * lds] / tri[e.tri].area;
+        u0 += e.cb[0] * s;
+        u1 += e.cb[1] * s;
+        u2 += e.cb[2] * s;
     }
-    const int64_t a = ta * TILE + threadIdx.x / TILE;
-    const int64_t b = tb * TILE + threadIdx.x % TILE;
-    if (a >= na || b >= nb) return;
-    if (same && b > a) return;
-    const int32_t v1 = slot_vid[sa0 + a], v2 = slot_vid[sb0 + b];
-    const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
-    double res = 0.0;
+    double* u = U + i + j * ldu;
+    u[0] = u0;
+    u[ldu * nj] = u1;
+    u[2 * ldu * nj] = u2;
+}
+
+// Stage 2: thread = vertex slot a of mesh A and N2_COLS consecutive columns
+// of the chunk (block row y; column jb is vertex slot j0 + jb of mesh B), so
+// that the incidences of a are read once; for a block of one mesh only
+// a >= b is computed and both symmetric entries are written.
+constexpr int N2_COLS = 16;
+__global__ void __launch_bounds__(256) k_n_stage2(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
+                                                  const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                  const int64_t* __restrict__ vert_unk, int64_t sa0, int64_t na,
+                                                  int64_t sb0, int64_t j0, int same, int64_t ta0, double coeff,
+                                                  const double* __restrict__ U, int64_t ldu, int64_t nj,
+                                                  double* __restrict__ H, Layout L, int use_atomic) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t jb0 = (int64_t)blockIdx.y * N2_COLS;
+    if (a >= na) return;
+    if (same && j0 + jb0 > a) return;   // all of this thread's columns lie above the diagonal
+    const int64_t ncol = std::min<int64_t>(N2_COLS, nj - jb0);
+    const int64_t cs = ldu * nj;        // distance between the three U arrays
+    double acc[N2_COLS];
+#pragma unroll
+    for (int q = 0; q < N2_COLS; ++q) acc[q] = 0.0;
     for (int64_t p = adj_off[sa0 + a]; p < adj_off[sa0 + a + 1]; ++p) {
-        const AdjRec e1 = adj[p];
-        const d3 cb1 = ld3(e1.cb);
-        const double A1 = tri[e1.tri].area;
-        for (int64_t q = adj_off[sb0 + b]; q < adj_off[sb0 + b + 1]; ++q) {
-            const AdjRec e2 = adj[q];
-            const double A2 = tri[e2.tri].area;
-            double s;
-            if (tmp) {
-                s = tmp[(e1.tri - ta0) + (e2.tri - tb0) * tmp_ld];
-            } else {
-                const int64_t r = dev_index(L, ua0 + (e1.tri - ta0));
-                const int64_t c = dev_index(L, ub0 + (e2.tri - tb0));
-                s = H[r + c * L.ld];
-            }
-            res -= factor * dot(cb1, ld3(e2.cb)) * s / (A1 * A2);
-        }
+        const AdjRec e = adj[p];
+        const double ia = 1.0 / tri[e.tri].area;
+        const double w0 = e.cb[0] * ia, w1 = e.cb[1] * ia, w2 = e.cb[2] * ia;
+        const double* u = U + (e.tri - ta0) + jb0 * ldu;
+#pragma unroll
+        for (int q = 0; q < N2_COLS; ++q)
+            if (q < ncol) acc[q] += w0 * u[q * ldu] + w1 * u[q * ldu + cs] + w2 * u[q * ldu + 2 * cs];
     }
-    const double val = res * coeff;
-    const int64_t r = dev_index(L, vert_unk[v1]), c = dev_index(L, vert_unk[v2]);
-    if (use_atomic) {
-        atomicAdd(&H[r + c * L.ld], val);
-        if (r != c) atomicAdd(&H[c + r * L.ld], val);
-    } else {
-        H[r + c * L.ld] += val;
-        if (r != c) H[c + r * L.ld] += val;
+    const int32_t v1 = slot_vid[sa0 + a];
+    const int64_t r = dev_index(L, vert_unk[v1]);
+#pragma unroll
+    for (int q = 0; q < N2_COLS; ++q) {
+        const int64_t b = j0 + jb0 + q;
+        if (q >= ncol || (same && b > a)) break;
+        const int32_t v2 = slot_vid[sb0 + b];
+        const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
+        const double val = -factor * acc[q] * coeff;
+        const int64_t c = dev_index(L, vert_unk[v2]);
+        if (use_atomic) {
+            atomicAdd(&H[r + c * L.ld], val);
+            if (r != c) atomicAdd(&H[c + r * L.ld], val);
+        } else {
+            H[r + c * L.ld] += val;
+            if (r != c) H[c + r * L.ld] += val;
+        }
     }
 }
 
@@ -501,6 +529,8 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
     const MeshDev& A = dg.meshes[m1];
     const MeshDev& B = dg.meshes[m2];
     const int64_t na = A.slot_end - A.slot_begin, nb = B.slot_end - B.slot_begin;
+    const int64_t nt1 = A.tri_end - A.tri_begin;
+    if (na <= 0 || nb <= 0 || nt1 <= 0) return;
     const int same = (m1 == m2);
     int use_atomic = 0;
     if (!same) {   // vertices shared by the two meshes can alias entries
@@ -510,12 +540,33 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
         for (int64_t v : g.meshes[m2].verts)
             if (std::binary_search(sa.begin(), sa.end(), v)) { use_atomic = 1; break; }
     }
-    const int64_t ntx = cdiv(nb, TILE);
-    const int64_t nblk = same ? n_upper_tiles(na) : cdiv(na, TILE) * ntx;
-    k_n_block<<<(unsigned)nblk, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
-                                              B.slot_begin, nb, same, ntx, coeff, tmp, A.tri_end - A.tri_begin,
-                                              A.tri_begin, B.tri_begin, A.tri_unk0, B.tri_unk0, H, L, use_atomic);
-    OMH_LAUNCH_CHECK();
+    // the S block: inside the HeadMat (triangle unknowns of a mesh are
+    // contiguous in either ordering) or the temporary of a barrier pair
+    const double* S;
+    int64_t lds;
+    if (tmp) {
+        S = tmp;
+        lds = nt1;
+    } else {
+        S = H + dev_index(L, A.tri_unk0) + dev_index(L, B.tri_unk0) * L.ld;
+        lds = L.ld;
+    }
+    const int64_t ldu = (nt1 + 31) / 32 * 32;
+    const size_t budget = size_t(1) << 30;   // 1 GiB of U per chunk (shares the D pass-1 scratch)
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nb, 65535, (int64_t)(budget / (3 * sizeof(double) * (size_t)ldu))}));
+    double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)chunk);
+    for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
+        const int64_t nj = std::min<int64_t>(chunk, nb - j0);
+        // a block of one mesh only needs rows a >= j0
+        const dim3 g1((unsigned)cdiv(nt1, 256), (unsigned)nj);
+        k_n_stage1<<<g1, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, B.slot_begin + j0, nt1, S, lds, B.tri_begin, U, ldu,
+                                       nj);
+        OMH_LAUNCH_CHECK();
+        const dim3 g2((unsigned)cdiv(na, 256), (unsigned)cdiv(nj, N2_COLS));
+        k_n_stage2<<<g2, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
+                                       B.slot_begin, j0, same, A.tri_begin, coeff, U, ldu, nj, H, L, use_atomic);
+        OMH_LAUNCH_CHECK();
+    }
 }
 
 // D(rows = triangles of mesh mr, columns = vertices of mesh mc), operators.h:84-115.

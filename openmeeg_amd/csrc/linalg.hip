@@ -796,6 +796,18 @@ __global__ void __launch_bounds__(256) k_sumsq(const T* __restrict__ a, int64_t 
 
 // EH(e, :) = sum_k vals(e,k) * Hinv(cols(e,k), :) -- the sparse x dense
 // product of sparse_matrix.cpp:36-51 for a 3-nnz-per-row Head2EEGMat.
+// Column e of out (zeroed before) gets vals(e, k) at row rows(e, k), k < 3
+// (one thread per column, so repeated rows add without atomics).
+__global__ void k_sparse_cols(const int64_t* __restrict__ rows, const double* __restrict__ vals, int64_t ne, int64_t n,
+                              double* __restrict__ out, int64_t ldo) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    for (int k = 0; k < 3; ++k) {
+        const int64_t r = rows[3 * e + k];
+        if (r >= 0 && r < n) out[r + e * ldo] += vals[3 * e + k];
+    }
+}
+
 __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __restrict__ vals, int64_t ne,
                               const double* __restrict__ Hm, int64_t ldh, int64_t n, double* __restrict__ out,
                               int64_t ldo) {
@@ -1672,6 +1684,14 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
     }
     if (final_rel_update) *final_rel_update = rel;
     return it;
+}
+
+void sparse_transpose_dense(const int64_t* d_rows, const double* d_vals, int64_t ne, int64_t n, double* out, int64_t ldo,
+                            hipStream_t st) {
+    fill_f64(out, n, ne, ldo, 0.0, st);
+    if (ne <= 0) return;
+    k_sparse_cols<<<(unsigned)cdiv(ne, 256), 256, 0, st>>>(d_rows, d_vals, ne, n, out, ldo);
+    OMH_LAUNCH_CHECK();
 }
 
 void sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* Hm, int64_t ldh, int64_t n,

@@ -145,6 +145,10 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
                  double* final_rel_update, hipStream_t st);
 
 // out(e, j) = sum_k vals(e,k) Hm(cols(e,k), j)  (3 non-zeros per row).
+// out (n x ne, leading dimension ldo) = the dense transpose of a sparse
+// ne x n matrix with 3 entries per row (rows(e, k), vals(e, k)).
+void sparse_transpose_dense(const int64_t* d_rows, const double* d_vals, int64_t ne, int64_t n, double* out, int64_t ldo,
+                            hipStream_t st);
 void sparse_rows(const int64_t* d_cols, const double* d_vals, int64_t ne, const double* Hm, int64_t ldh, int64_t n,
                  double* out, int64_t ldo, hipStream_t st);
 
