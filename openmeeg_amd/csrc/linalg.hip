@@ -282,15 +282,27 @@ __device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alp
             compute(As[0], Bs[0]);
         }
     } else {
+        // the same double-buffered pipeline with guarded loads
         const int64_t nk = (k + BK - 1) / BK;
-        for (int64_t kt = 0; kt < nk; ++kt) {
-            la.load_guarded(A, lda, row0, kt * BK, m, k, tid);
-            lb.load_guarded(B, ldb, col0, kt * BK, n, k, tid);
+        if (nk > 0) {
+            la.load_guarded(A, lda, row0, 0, m, k, tid);
+            lb.load_guarded(B, ldb, col0, 0, n, k, tid);
             la.store(As[0], tid);
             lb.store(Bs[0], tid);
             __syncthreads();
-            compute(As[0], Bs[0]);
-            __syncthreads();
+            for (int64_t kt = 0; kt < nk; ++kt) {
+                const int buf = (int)(kt & 1);
+                if (kt + 1 < nk) {
+                    la.load_guarded(A, lda, row0, (kt + 1) * BK, m, k, tid);
+                    lb.load_guarded(B, ldb, col0, (kt + 1) * BK, n, k, tid);
+                }
+                compute(As[buf], Bs[buf]);
+                if (kt + 1 < nk) {
+                    la.store(As[buf ^ 1], tid);
+                    lb.store(Bs[buf ^ 1], tid);
+                }
+                __syncthreads();
+            }
         }
     }
     // Epilogue.  FP32: a lane's 4 elements are 4 consecutive rows of one

@@ -124,6 +124,36 @@ def test_gemm_large_tile_order(om, dtype, ta):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
+def test_sparse_transpose_dense(om):
+    """Right-hand sides of the adjoint solve (linsolve's rows, gain.h:28-49):
+    the dense transpose of a 3-entries-per-row sparse matrix, with repeated
+    rows adding and rows outside [0, n) ignored, into a padded leading
+    dimension whose padding rows must stay zero."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    rng = np.random.default_rng(5)
+    n, ne, ld = 1000, 77, 1003
+    rows = rng.integers(0, n, size=(ne, 3)).astype(np.int64)
+    rows[5, 1] = rows[5, 0]          # repeated row: the values add
+    rows[9, 2] = n + 4               # outside: ignored
+    vals = rng.standard_normal((ne, 3))
+    ref = np.zeros((n, ne))
+    for e in range(ne):
+        for k in range(3):
+            if rows[e, k] < n:
+                ref[rows[e, k], e] += vals[e, k]
+    dr = torch.from_numpy(rows.ravel().copy()).cuda()
+    dv = torch.from_numpy(vals.ravel().copy()).cuda()
+    out = torch.full((ld * ne,), 7.0, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().omh_sparse_transpose_dense(C.c_void_p(dr.data_ptr()), C.c_void_p(dv.data_ptr()), ne, n,
+                                                     C.c_void_p(out.data_ptr()), ld, C.c_void_p(0)))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(ne, ld).T
+    np.testing.assert_array_equal(got[:n], ref)
+    assert np.all(got[n:] == 7.0)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed16"])
 def test_factor_solve_residual_large(om, precision):
     """HeadMat of the 2562-vertex 3-layer head (n = 17926): the trailing
