@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -553,7 +554,13 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
     }
     const int64_t ldu = (nt1 + 31) / 32 * 32;
     const size_t budget = size_t(1) << 30;   // 1 GiB of U per chunk (shares the D pass-1 scratch)
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nb, 65535, (int64_t)(budget / (3 * sizeof(double) * (size_t)ldu))}));
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nb, 65535, (int64_t)(budget / (3 * sizeof(double) * (size_t)ldu))}));
+    // OMH_N_CHUNK_COLS caps the columns per chunk (tests run the chunked
+    // path on small meshes with it)
+    if (const char* env = std::getenv("OMH_N_CHUNK_COLS")) {
+        const long long v = std::atoll(env);
+        if (v > 0) chunk = std::min<int64_t>(chunk, (int64_t)v);
+    }
     double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)chunk);
     for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
         const int64_t nj = std::min<int64_t>(chunk, nb - j0);

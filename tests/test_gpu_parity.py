@@ -124,6 +124,20 @@ def test_gemm_large_tile_order(om, dtype, ta):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
+def test_n_blocks_chunked(om, monkeypatch):
+    """The N blocks are assembled in column chunks (1 GiB of per-triangle
+    sums each, several chunks from about 2200 vertices per surface on); with
+    the chunk width capped at 37 columns the c0 HeadMat (642 vertices per
+    surface, 18 chunks per block) is bitwise the single-chunk one."""
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device
+    g = om.make_nested_geometry(S.nested_sphere_meshes(8, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    H1 = _head_mat_device(g).cpu().numpy()
+    monkeypatch.setenv("OMH_N_CHUNK_COLS", "37")
+    H2 = _head_mat_device(g).cpu().numpy()
+    assert np.array_equal(H1, H2)
+
+
 def test_sparse_transpose_dense(om):
     """Right-hand sides of the adjoint solve (linsolve's rows, gain.h:28-49):
     the dense transpose of a 3-entries-per-row sparse matrix, with repeated
