@@ -161,6 +161,19 @@ int omh_sym_factor_f32(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t*
  * reference counterpart. */
 int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t* info, void* stream);
 
+/* The same factorisation told the block structure of the matrix, so that
+ * the trailing updates skip tiles that only multiply structural zeros (no
+ * counterpart in the reference, which factors the packed matrix densely,
+ * symmatrix.h:259-297; the result is the same).  The unknowns are cut into
+ * nseg consecutive segments starting at seg_start[0] = 0 < seg_start[1] <
+ * ...; fill[s * nseg + t] (s > t) is 0 when rows of segment s stay zero in
+ * L below the columns of segment t.  The pattern must be closed under
+ * elimination and the blocks it declares zero must be zero in d_A (both
+ * checked: invalid argument otherwise).  HeadMat's pattern comes from its
+ * mesh pairs (api.py Geometry.factor_segments). */
+int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
+                                    const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream);
+
 /* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
  * symmatrix.cpp:133-152; linsolve gain.h:52-57). */
 int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,

@@ -176,6 +176,62 @@ class Geometry:
     def n_domains(self):
         return self._nd
 
+    def factor_segments(self):
+        """Block structure of the HeadMat in triangles-first order for the
+        factorisation: segments (each mesh's triangle unknowns in mesh order,
+        then each mesh's vertex unknowns) and the lower-triangle pattern of
+        L over them, from a symbolic elimination of the mesh-pair coupling
+        (operators of assembleHeadMat.cpp:73-135: S between triangle sets,
+        N between vertex sets, D/D* between the triangles of one mesh and
+        the vertices of the other; the deflation couples the outermost
+        meshes of a part).  Returns (starts, fill) with fill an nseg x nseg
+        uint8 array, or None when a mesh's vertex unknowns are not one
+        contiguous range."""
+        nt, nv = self._nt, self._nv
+        tseg, vseg = {}, {}
+        for m in range(self._nm):
+            inf = self.mesh_info(m)
+            if inf["nverts"] > 0:
+                if inf["vert_first"] < 0:
+                    return None
+                vseg[m] = (inf["vert_first"] + nt, inf["vert_end"] + nt)
+            if inf["ntris"] > 0 and not inf["current_barrier"]:   # barrier meshes have no triangle unknowns
+                tseg[m] = (inf["tri_first"] - nv, inf["tri_first"] - nv + inf["ntris"])
+        segs = sorted([(a, b, "T", m) for m, (a, b) in tseg.items()] + [(a, b, "V", m) for m, (a, b) in vseg.items()])
+        if not segs or segs[0][0] != 0 or segs[-1][1] != self.size:
+            return None
+        for (a0, b0, _, _), (a1, _, _, _) in zip(segs, segs[1:]):
+            if b0 != a1:
+                return None
+        idx = {(k, m): i for i, (_, _, k, m) in enumerate(segs)}
+        ns = len(segs)
+        adj = np.eye(ns, dtype=bool)
+        def link(a, b):
+            if a in idx and b in idx:
+                adj[idx[a], idx[b]] = adj[idx[b], idx[a]] = True
+        for p in self.mesh_pairs():
+            m1, m2 = p["meshes"]
+            link(("T", m1), ("T", m2))
+            link(("V", m1), ("V", m2))
+            link(("T", m1), ("V", m2))
+            link(("T", m2), ("V", m1))
+        outer = {}
+        for m in range(self._nm):
+            inf = self.mesh_info(m)
+            if inf["outermost"]:
+                outer.setdefault(inf["part"], []).append(m)
+        for ms in outer.values():
+            for a in ms:
+                for b in ms:
+                    link(("V", a), ("V", b))
+        for k in range(ns):   # symbolic elimination in segment order
+            nb = [i for i in range(k + 1, ns) if adj[i, k]]
+            for i in nb:
+                for j in nb:
+                    adj[i, j] = True
+        fill = np.tril(adj, -1).astype(np.uint8)
+        return np.array([a for a, _, _, _ in segs], dtype=np.int64), fill
+
     def tri_first_index(self, ref_rows):
         """Map reference-order unknown indices to triangles-first order."""
         r = np.asarray(ref_rows, dtype=np.int64)
@@ -471,11 +527,12 @@ class Factor:
         return B
 
 
-def factor(Hdev, n, precision="fp64"):
+def factor(Hdev, n, precision="fp64", structure=None):
     """Factor a device HeadMat (triangles-first order).  precision 'fp64'
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
     factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
-    trailing updates."""
+    trailing updates and, given structure = Geometry.factor_segments(),
+    skips the trailing-update tiles that only multiply structural zeros."""
     torch = _t()
     info = C.c_int32()
     if precision == "fp64":
@@ -486,8 +543,16 @@ def factor(Hdev, n, precision="fp64"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
         check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
-        fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
-        check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
+        if precision == "mixed16" and structure is not None:
+            starts, fill = structure
+            starts = np.ascontiguousarray(starts, dtype=np.int64)
+            fill = np.ascontiguousarray(fill, dtype=np.uint8)
+            check(lib().omh_sym_factor_f32_split_blocks(
+                _ptr(L32), n, n, _ptr(work), len(starts), starts.ctypes.data_as(C.c_void_p),
+                fill.ctypes.data_as(C.c_void_p), C.byref(info), _stream()))
+        else:
+            fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
+            check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
         F = Factor(L32, n, work, "mixed", H64=Hdev)
     else:
         raise ValueError("precision must be 'fp64', 'mixed' or 'mixed16'")
@@ -684,7 +749,7 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     h2em = Head2EEGMat(geom, electrodes)
     Y = _eeg_rhs(geom, h2em)
     marks.append(("h2em", ev())); marks[-1][1].record()
-    F = factor(H, n, precision)
+    F = factor(H, n, precision, structure=geom.factor_segments() if precision == "mixed16" else None)
     marks.append(("factor", ev())); marks[-1][1].record()
     F.solve(Y, h2em.nlin(), max_iter=max_refine, tol=refine_tol)
     marks.append(("solve", ev())); marks[-1][1].record()

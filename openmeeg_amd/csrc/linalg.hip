@@ -204,6 +204,22 @@ __device__ __forceinline__ void map_tile(const TileMap& map, int64_t t, int64_t&
     }
 }
 
+// Tile rows of a launch that skips structurally zero rows: compressed tile
+// index x in [c[k], c[k + 1]) is tile row s[k] + x - c[k] (n intervals in
+// ascending order; n = 0: the identity).
+constexpr int MAX_REMAP = 8;
+struct TileRemap {
+    int n = 0;
+    int64_t c[MAX_REMAP + 1] = {};
+    int64_t s[MAX_REMAP] = {};
+};
+
+__device__ __forceinline__ int64_t remap_tile(const TileRemap& r, int64_t x) {
+    int k = 0;
+    while (k + 1 < r.n && x >= r.c[k + 1]) ++k;
+    return r.s[k] + (x - r.c[k]);
+}
+
 // C = alpha * op(A) * op(B) + beta * C on a (32 WB) x (32 WB) tile per
 // workgroup of 256 threads (2 x 2 waves, each 16 WB x 16 WB = WB x WB MFMA
 // blocks), k-slices of BK (16; FP32 configurable) double-buffered in LDS.
@@ -939,7 +955,8 @@ template <int KP>
 __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
                                                 const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
                                                 const _Float16* __restrict__ Bl, const float* __restrict__ scale,
-                                                float* __restrict__ C, int64_t ldc, TileMap map) {
+                                                float* __restrict__ C, int64_t ldc, TileMap map,
+                                                const TileRemap& rm) {
     __shared__ half8_t S[2][4][S_TILE * 4];
     int64_t t = blockIdx.x;
     if (map.kind >= 4) {
@@ -948,6 +965,10 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
     }
     int64_t tm, tn;
     map_tile(map, t, tm, tn);
+    if (rm.n) {   // compressed order over the structurally non-zero tile rows
+        tm = remap_tile(rm, tm);
+        tn = remap_tile(rm, tn);
+    }
     const int64_t row0 = r_off + tm * S_TILE, col0 = r_off + tn * S_TILE;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -1072,8 +1093,8 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
                                                     const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                     const float* __restrict__ scale, float* __restrict__ C,
-                                                    int64_t ldc, TileMap map) {
-    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map);
+                                                    int64_t ldc, TileMap map, TileRemap rm) {
+    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, rm);
 }
 
 // The next block group's columns (a strip of whole tile columns, map kind 6).
@@ -1083,7 +1104,20 @@ __global__ void __launch_bounds__(256) k_syrk_strip(int64_t m, int64_t mp, int64
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                     const float* __restrict__ scale, float* __restrict__ C,
                                                     int64_t ldc, TileMap map) {
-    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map);
+    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, TileRemap{});
+}
+
+// flag = 1 if any entry of the m x n block is non-zero.
+__global__ void __launch_bounds__(256) k_any_nonzero(const float* __restrict__ A, int64_t lda, int64_t m, int64_t n,
+                                                     int* __restrict__ flag) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t % m, j = t / m;
+        if (A[i + j * lda] != 0.f) {
+            *flag = 1;
+            return;
+        }
+    }
 }
 
 __global__ void k_set_int(int* p, int v) { p[0] = v; }
@@ -1369,7 +1403,44 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 //   the next group's columns -= all panels of the group (FP32 GEMMs);
 //   the next group's prologue on the side stream, while the rest of the
 //   trailing matrix -= [X_1 .. X_P] S [X_1 .. X_P]^T as one split-FP16 SYRK.
-static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st) {
+// Tile rows (S_TILE, relative to row r) that the rank-S_KP update of the
+// block group with columns [c0, c1) can touch: the segments whose rows may be
+// non-zero below those columns.  Returns false when that is every row.
+static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r, TileRemap& rm,
+                             int64_t& rows) {
+    const int ns = bf.nseg();
+    std::vector<std::pair<int64_t, int64_t>> iv;   // tile intervals
+    rows = 0;
+    for (int s = 0; s < ns; ++s) {
+        bool act = false;
+        for (int t = 0; t < ns && !act; ++t)
+            if (bf.start[t] < c1 && bf.start[t + 1] > c0 && bf.nz(s, t)) act = true;
+        const int64_t lo = std::max(bf.start[s], r), hi = std::min(bf.start[s + 1], n);
+        if (!act || hi <= lo) continue;
+        rows += hi - lo;
+        iv.emplace_back((lo - r) / S_TILE, cdiv(hi - r, S_TILE));
+    }
+    std::sort(iv.begin(), iv.end());
+    std::vector<std::pair<int64_t, int64_t>> mg;
+    for (const auto& v : iv) {
+        if (!mg.empty() && v.first <= mg.back().second) mg.back().second = std::max(mg.back().second, v.second);
+        else mg.push_back(v);
+    }
+    const int64_t ntm = cdiv(n - r, S_TILE);
+    if (mg.empty() || mg.size() > (size_t)MAX_REMAP || (mg.size() == 1 && mg[0].first == 0 && mg[0].second >= ntm))
+        return false;
+    rm.n = (int)mg.size();
+    int64_t c = 0;
+    for (int k = 0; k < rm.n; ++k) {
+        rm.c[k] = c;
+        rm.s[k] = mg[k].first;
+        c += mg[k].second - mg[k].first;
+    }
+    rm.c[rm.n] = c;
+    return true;
+}
+
+static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st, const BlockFill* bf) {
     float* Dinv = (float*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     float* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
@@ -1478,13 +1549,19 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         OMH_HIP(hipEventRecord(la.done, la.side));
         // the rest of the trailing matrix
         if (mb > 0) {
-            const int64_t ntm = cdiv(mb, S_TILE);
+            // only the tile rows the group's panels can reach (block structure)
+            TileRemap rm;
+            int64_t rows = mb;
+            int64_t ntm = cdiv(mb, S_TILE);
+            if (bf && active_tile_rows(*bf, n, start_of(b0), std::min(n, start_of(b0 + S_PANELS)), r2, rm, rows))
+                ntm = rm.c[rm.n];
             const int64_t tiles = ntm * (ntm + 1) / 2;
             const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
                                               : TileMap{1, ntm, 0, 0, tiles, 0};
-            ProfScope ps("factor_update_f32s", (double)mb * (double)(mb + 1) * (double)S_KP, st);
-            k_syrk_split<S_KP><<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
-                mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map);
+            ProfScope ps("factor_update_f32s", (double)rows * (double)(rows + 1) * (double)S_KP, st);
+            if (tiles > 0)
+                k_syrk_split<S_KP><<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
+                    mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map, rm);
             OMH_LAUNCH_CHECK();
         }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
@@ -1500,12 +1577,12 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
 }
 
 template <typename T>
-FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16) {
+FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16, const BlockFill* bf) {
     T* Dinv = (T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
     T* Xbuf[2] = {(T*)work, sgn + sign_span(n)};
-    if (split16 && sizeof(T) == 4) return sym_factor_split((float*)A, n, lda, work, st);
+    if (split16 && sizeof(T) == 4) return sym_factor_split((float*)A, n, lda, work, st, bf);
     int* info = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
@@ -1564,8 +1641,29 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
     return fi;
 }
 
-template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool);
-template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t, bool);
+template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool, const BlockFill*);
+template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t, bool, const BlockFill*);
+
+int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st) {
+    int* flag = (int*)scratch(SCR_SMALL, 64);
+    k_set_int<<<1, 1, 0, st>>>(flag, 0);
+    OMH_LAUNCH_CHECK();
+    const int ns = bf.nseg();
+    for (int s = 0; s < ns; ++s)
+        for (int t = 0; t < s; ++t) {
+            if (bf.nz(s, t)) continue;
+            const int64_t r0 = bf.start[s], r1 = std::min(bf.start[s + 1], n);
+            const int64_t c0 = bf.start[t], c1 = std::min(bf.start[t + 1], n);
+            if (r1 <= r0 || c1 <= c0) continue;
+            const int64_t m = r1 - r0, k = c1 - c0;
+            k_any_nonzero<<<std::min<unsigned>(grid_for(m * k), 4096), 256, 0, st>>>(A + r0 + c0 * lda, lda, m, k, flag);
+            OMH_LAUNCH_CHECK();
+        }
+    int h = 0;
+    OMH_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    return h;
+}
 
 // Diagonal blocks per group of the triangular solves.
 #ifndef OMH_SOLVE_GROUP

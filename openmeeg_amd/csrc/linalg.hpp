@@ -112,13 +112,32 @@ struct FactorInfo {
 // element size elem.
 size_t factor_workspace_bytes(int64_t n, size_t elem);
 
+// Block structure of a matrix for the factorisation: the unknowns are cut
+// into consecutive segments (start[s] .. start[s + 1], start[nseg] = n) and
+// fill[s * nseg + t] (s > t) is 0 when rows of segment s are structurally
+// zero in L below the columns of segment t (a symbolic factorisation, closed
+// under elimination).  Trailing updates skip the tiles whose rows or columns
+// lie only in such segments; those tiles would add exact zeros.
+struct BlockFill {
+    std::vector<int64_t> start;
+    std::vector<uint8_t> fill;
+    int nseg() const { return (int)start.size() - 1; }
+    bool nz(int s, int t) const { return s <= t || fill[(size_t)s * nseg() + t] != 0; }
+};
+
 // In-place pivot-free signed Cholesky H = L S L^T of the lower triangle of A
 // (n x n, leading dimension lda).  The strict upper triangle is not touched.
 // split16 (FP32 only): the trailing updates run as split-FP16 MFMA products
 // (each panel row scaled and split into two FP16 parts, three products,
-// FP32 accumulation; about 22 significant bits per operand).
+// FP32 accumulation; about 22 significant bits per operand).  bf: optional
+// block structure (split16 path) whose zero blocks the trailing updates skip.
 template <typename T>
-FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16 = false);
+FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16 = false,
+                      const BlockFill* bf = nullptr);
+
+// 1 if any entry of the blocks a block structure declares zero (below the
+// diagonal) is non-zero.
+int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st);
 
 // W <- H^-1 W for r right-hand sides with the factor of sym_factor.
 template <typename T>

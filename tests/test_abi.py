@@ -50,3 +50,20 @@ def test_gpu_entry_points_refuse_without_device():
     g = om.read_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
     with pytest.raises(RuntimeError, match="GPU"):
         om.HeadMat(g)
+
+
+def test_block_structure_must_be_closed():
+    """omh_sym_factor_f32_split_blocks refuses a segment pattern that is not
+    closed under elimination before touching the device (two segments reached
+    from the same earlier segment must be coupled below it)."""
+    import ctypes as C
+    from openmeeg_amd import _lib
+    starts = np.array([0, 10, 20], dtype=np.int64)
+    fill = np.zeros((3, 3), dtype=np.uint8)
+    fill[1, 0] = fill[2, 0] = 1          # 1 and 2 both below 0 ...
+    info = C.c_int32()
+    dummy = C.c_void_p(16)               # never dereferenced: the check fails first
+    rc = _lib.lib().omh_sym_factor_f32_split_blocks(dummy, 30, 30, dummy, 3, starts.ctypes.data_as(C.c_void_p),
+                                                     fill.ctypes.data_as(C.c_void_p), C.byref(info), C.c_void_p(0))
+    assert rc == 2                       # ... but 2 is not coupled to 1
+    assert b"closed" in _lib.lib().omh_last_error()

@@ -191,6 +191,51 @@ def test_factor_solve_residual_large(om, precision):
     assert res < 1e-12
 
 
+@pytest.mark.parametrize("layers", [3, 4])
+def test_block_structured_factor_is_bitwise_dense(om, layers):
+    """The split-FP16 factorisation told the HeadMat's block structure
+    (Geometry.factor_segments) skips trailing-update tiles that only multiply
+    structural zeros: the factor must be bitwise the dense one (the skipped
+    tiles would add exact zeros).  3 layers of 2562 vertices (n = 17926) and
+    4 layers of 1442 vertices (n = 14408); a pattern that declares a
+    non-zero block zero is refused."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import spheres as S, _lib
+    from openmeeg_amd.api import _head_mat_device, factor
+    if layers == 3:
+        g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    else:
+        g = om.make_nested_geometry(S.nested_sphere_meshes(12, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    starts, fill = g.factor_segments()
+    assert fill.sum() < fill.shape[0] * (fill.shape[0] - 1) // 2   # some blocks stay zero
+    H = _head_mat_device(g, far_field=True)
+    Fd = factor(H, n, "mixed16")
+    Fs = factor(H, n, "mixed16", structure=(starts, fill))
+    Ld = Fd.L.view(n, n).t().tril()       # lower triangle (column-major storage)
+    Ls = Fs.L.view(n, n).t().tril()
+    print("n=%d segments %s, zero blocks %d: factor max |diff| %.3e" % (
+        n, starts.tolist(), int((np.tril(np.ones_like(fill), -1) - fill).sum()), float((Ld - Ls).abs().max())),
+        flush=True)
+    assert torch.equal(Ld, Ls)
+    bad = fill.copy()
+    s, t = np.argwhere(np.tril(fill, -1))[0]
+    bad[s, t] = 0
+    # keep the pattern closed: the rows of s then no longer reach later blocks through t
+    L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
+    check = _lib.check
+    check(_lib.lib().omh_convert_f64_f32(C.c_void_p(H.data_ptr()), n, C.c_void_p(L32.data_ptr()), n, n, n, 1,
+                                         C.c_void_p(0)))
+    work = torch.empty(int(_lib.lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
+    info = C.c_int32()
+    rc = _lib.lib().omh_sym_factor_f32_split_blocks(
+        C.c_void_p(L32.data_ptr()), n, n, C.c_void_p(work.data_ptr()), len(starts),
+        np.ascontiguousarray(starts).ctypes.data_as(C.c_void_p), np.ascontiguousarray(bad).ctypes.data_as(C.c_void_p),
+        C.byref(info), C.c_void_p(0))
+    assert rc == 2   # refused: not closed, or a declared-zero block holds entries
+
+
 def test_headmat_head1_matches_golden(om, head1):
     g, _, _ = head1
     hm = om.HeadMat(g)
