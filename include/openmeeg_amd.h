@@ -173,6 +173,13 @@ int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, in
  * mesh pairs (api.py Geometry.factor_segments). */
 int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
                                     const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream);
+/* The same for omh_sym_factor_f32 / omh_sym_factor_f64 (the trailing update
+ * of each block runs as products over the pairs of row intervals its panel
+ * can reach). */
+int omh_sym_factor_f32_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
+                              const uint8_t* fill, int32_t* info, void* stream);
+int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
+                              const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream);
 
 /* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
  * symmatrix.cpp:133-152; linsolve gain.h:52-57). */

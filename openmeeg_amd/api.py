@@ -531,25 +531,30 @@ def factor(Hdev, n, precision="fp64", structure=None):
     """Factor a device HeadMat (triangles-first order).  precision 'fp64'
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
     factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
-    trailing updates and, given structure = Geometry.factor_segments(),
-    skips the trailing-update tiles that only multiply structural zeros."""
+    trailing updates.  Given structure = Geometry.factor_segments(), the
+    trailing updates skip the parts that only multiply structural zeros
+    (the factor is the same)."""
     torch = _t()
     info = C.c_int32()
+    blocks = ()
+    if structure is not None:
+        starts = np.ascontiguousarray(structure[0], dtype=np.int64)
+        fill = np.ascontiguousarray(structure[1], dtype=np.uint8)
+        blocks = (len(starts), starts.ctypes.data_as(C.c_void_p), fill.ctypes.data_as(C.c_void_p))
     if precision == "fp64":
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
-        check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
+        if blocks:
+            check(lib().omh_sym_factor_f64_blocks(_ptr(Hdev), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
+        else:
+            check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
         F = Factor(Hdev, n, work, "fp64")
     elif precision in ("mixed", "mixed16"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
         check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
-        if precision == "mixed16" and structure is not None:
-            starts, fill = structure
-            starts = np.ascontiguousarray(starts, dtype=np.int64)
-            fill = np.ascontiguousarray(fill, dtype=np.uint8)
-            check(lib().omh_sym_factor_f32_split_blocks(
-                _ptr(L32), n, n, _ptr(work), len(starts), starts.ctypes.data_as(C.c_void_p),
-                fill.ctypes.data_as(C.c_void_p), C.byref(info), _stream()))
+        if blocks:
+            fn = lib().omh_sym_factor_f32_blocks if precision == "mixed" else lib().omh_sym_factor_f32_split_blocks
+            check(fn(_ptr(L32), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
         else:
             fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
             check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
@@ -749,7 +754,7 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     h2em = Head2EEGMat(geom, electrodes)
     Y = _eeg_rhs(geom, h2em)
     marks.append(("h2em", ev())); marks[-1][1].record()
-    F = factor(H, n, precision, structure=geom.factor_segments() if precision == "mixed16" else None)
+    F = factor(H, n, precision, structure=geom.factor_segments())
     marks.append(("factor", ev())); marks[-1][1].record()
     F.solve(Y, h2em.nlin(), max_iter=max_refine, tol=refine_tol)
     marks.append(("solve", ev())); marks[-1][1].record()

@@ -446,35 +446,62 @@ int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, in
     });
 }
 
-int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
-                                    const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream) {
+extern "C++" {
+// Block structure of the *_blocks factorisations: segments from seg_start,
+// fill pattern closed under elimination, its zero blocks zero in d_A.
+template <typename T>
+static omh::BlockFill block_structure(const T* d_A, int64_t n, int64_t ld, int32_t nseg, const int64_t* seg_start,
+                                      const uint8_t* fill, hipStream_t st) {
+    need(seg_start, "seg_start");
+    need(fill, "fill");
+    if (nseg < 1 || nseg > 4096) throw std::invalid_argument("bad segment count");
+    omh::BlockFill bf;
+    bf.start.assign(seg_start, seg_start + nseg);
+    bf.start.push_back(n);
+    if (bf.start[0] != 0) throw std::invalid_argument("the first segment must start at 0");
+    for (int s = 0; s < nseg; ++s)
+        if (bf.start[s + 1] <= bf.start[s]) throw std::invalid_argument("segment starts must increase below n");
+    bf.fill.assign(fill, fill + (size_t)nseg * nseg);
+    // closed under elimination: two segments reached from the same earlier
+    // column segment are coupled below it
+    for (int t = 0; t < nseg; ++t)
+        for (int a = t + 1; a < nseg; ++a)
+            for (int b = a + 1; b < nseg; ++b)
+                if (bf.nz(a, t) && bf.nz(b, t) && !bf.nz(b, a))
+                    throw std::invalid_argument("block structure is not closed under elimination");
+    if (omh::block_fill_violated<T>(d_A, n, ld, bf, st))
+        throw std::invalid_argument("a block declared zero holds non-zero entries");
+    return bf;
+}
+
+template <typename T>
+static int factor_blocks(T* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
+                         const uint8_t* fill, int32_t* info, void* stream, bool split16) {
     return guard([&] {
         need(d_A, "d_A");
         need(d_work, "d_work");
-        need(seg_start, "seg_start");
-        need(fill, "fill");
         if (n <= 0 || ld < n) throw std::invalid_argument("bad matrix size");
-        if (nseg < 1 || nseg > 4096) throw std::invalid_argument("bad segment count");
-        omh::BlockFill bf;
-        bf.start.assign(seg_start, seg_start + nseg);
-        bf.start.push_back(n);
-        if (bf.start[0] != 0) throw std::invalid_argument("the first segment must start at 0");
-        for (int s = 0; s < nseg; ++s)
-            if (bf.start[s + 1] <= bf.start[s]) throw std::invalid_argument("segment starts must increase below n");
-        bf.fill.assign(fill, fill + (size_t)nseg * nseg);
-        // closed under elimination: two segments reached from the same
-        // earlier column segment are coupled below it
-        for (int t = 0; t < nseg; ++t)
-            for (int a = t + 1; a < nseg; ++a)
-                for (int b = a + 1; b < nseg; ++b)
-                    if (bf.nz(a, t) && bf.nz(b, t) && !bf.nz(b, a))
-                        throw std::invalid_argument("block structure is not closed under elimination");
         hipStream_t st = S(stream);
-        if (omh::block_fill_violated(d_A, n, ld, bf, st))
-            throw std::invalid_argument("a block declared zero holds non-zero entries");
-        const auto fi = omh::sym_factor<float>(d_A, n, ld, d_work, st, true, &bf);
+        const omh::BlockFill bf = block_structure<T>(d_A, n, ld, nseg, seg_start, fill, st);
+        const auto fi = omh::sym_factor<T>(d_A, n, ld, d_work, st, split16, &bf);
         if (info) *info = fi.info;
     });
+}
+}  // extern "C++"
+
+int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
+                                    const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream) {
+    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, true);
+}
+
+int omh_sym_factor_f32_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
+                              const uint8_t* fill, int32_t* info, void* stream) {
+    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, false);
+}
+
+int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
+                              const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream) {
+    return factor_blocks<double>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, false);
 }
 
 int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,

@@ -1108,12 +1108,13 @@ __global__ void __launch_bounds__(256) k_syrk_strip(int64_t m, int64_t mp, int64
 }
 
 // flag = 1 if any entry of the m x n block is non-zero.
-__global__ void __launch_bounds__(256) k_any_nonzero(const float* __restrict__ A, int64_t lda, int64_t m, int64_t n,
+template <typename T>
+__global__ void __launch_bounds__(256) k_any_nonzero(const T* __restrict__ A, int64_t lda, int64_t m, int64_t n,
                                                      int* __restrict__ flag) {
     const int64_t total = m * n;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = t % m, j = t / m;
-        if (A[i + j * lda] != 0.f) {
+        if (A[i + j * lda] != T(0)) {
             *flag = 1;
             return;
         }
@@ -1403,6 +1404,25 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 //   the next group's columns -= all panels of the group (FP32 GEMMs);
 //   the next group's prologue on the side stream, while the rest of the
 //   trailing matrix -= [X_1 .. X_P] S [X_1 .. X_P]^T as one split-FP16 SYRK.
+// Row intervals [lo, hi) at or below row r that the update with the panel
+// of columns [c0, c1) can reach (segments whose rows may be non-zero there),
+// merged; empty when that is every row from r on.
+static void active_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r,
+                        std::vector<std::pair<int64_t, int64_t>>& iv) {
+    iv.clear();
+    const int ns = bf.nseg();
+    for (int s = 0; s < ns; ++s) {
+        bool act = false;
+        for (int t = 0; t < ns && !act; ++t)
+            if (bf.start[t] < c1 && bf.start[t + 1] > c0 && bf.nz(s, t)) act = true;
+        const int64_t lo = std::max(bf.start[s], r), hi = std::min(bf.start[s + 1], n);
+        if (!act || hi <= lo) continue;
+        if (!iv.empty() && iv.back().second == lo) iv.back().second = hi;
+        else iv.emplace_back(lo, hi);
+    }
+    if (iv.size() == 1 && iv[0].first == r && iv[0].second == n) iv.clear();
+}
+
 // Tile rows (S_TILE, relative to row r) that the rank-S_KP update of the
 // block group with columns [c0, c1) can touch: the segments whose rows may be
 // non-zero below those columns.  Returns false when that is every row.
@@ -1625,10 +1645,24 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
         panel(e, nb2, (int)((it + 1) & 1), la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
-        // (b) the rest of the trailing matrix
-        if (m > nb2)
-            gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
-                    A22 + nb2 + nb2 * lda, lda, st, upd);
+        // (b) the rest of the trailing matrix: with a block structure, only
+        // the row intervals the panel can reach, pair by pair
+        if (m > nb2) {
+            std::vector<std::pair<int64_t, int64_t>> iv;
+            if (bf) active_rows(*bf, n, k, e, e + nb2, iv);
+            if (iv.empty()) {
+                gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
+                        A22 + nb2 + nb2 * lda, lda, st, upd);
+            } else {
+                for (size_t a = 0; a < iv.size(); ++a)
+                    for (size_t b = 0; b <= a; ++b) {
+                        const int64_t r0 = iv[a].first, r1 = iv[a].second, c0 = iv[b].first, c1 = iv[b].second;
+                        // C[I_a, I_b] -= X[I_a] L21[I_b]^T
+                        gemm<T>(false, true, a == b, r1 - r0, c1 - c0, nb, T(-1), X + (r0 - e), ldx, A + c0 + k * lda,
+                                lda, T(1), A + r0 + c0 * lda, lda, st, upd);
+                    }
+            }
+        }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));
     }
     FactorInfo fi;
@@ -1644,7 +1678,8 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
 template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool, const BlockFill*);
 template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream_t, bool, const BlockFill*);
 
-int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st) {
+template <typename T>
+int block_fill_violated(const T* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st) {
     int* flag = (int*)scratch(SCR_SMALL, 64);
     k_set_int<<<1, 1, 0, st>>>(flag, 0);
     OMH_LAUNCH_CHECK();
@@ -1656,7 +1691,8 @@ int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill&
             const int64_t c0 = bf.start[t], c1 = std::min(bf.start[t + 1], n);
             if (r1 <= r0 || c1 <= c0) continue;
             const int64_t m = r1 - r0, k = c1 - c0;
-            k_any_nonzero<<<std::min<unsigned>(grid_for(m * k), 4096), 256, 0, st>>>(A + r0 + c0 * lda, lda, m, k, flag);
+            k_any_nonzero<T><<<std::min<unsigned>(grid_for(m * k), 4096), 256, 0, st>>>(A + r0 + c0 * lda, lda, m, k,
+                                                                                       flag);
             OMH_LAUNCH_CHECK();
         }
     int h = 0;
@@ -1664,6 +1700,9 @@ int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill&
     OMH_HIP(hipStreamSynchronize(st));
     return h;
 }
+
+template int block_fill_violated<float>(const float*, int64_t, int64_t, const BlockFill&, hipStream_t);
+template int block_fill_violated<double>(const double*, int64_t, int64_t, const BlockFill&, hipStream_t);
 
 // Diagonal blocks per group of the triangular solves.
 #ifndef OMH_SOLVE_GROUP

@@ -130,14 +130,15 @@ struct BlockFill {
 // split16 (FP32 only): the trailing updates run as split-FP16 MFMA products
 // (each panel row scaled and split into two FP16 parts, three products,
 // FP32 accumulation; about 22 significant bits per operand).  bf: optional
-// block structure (split16 path) whose zero blocks the trailing updates skip.
+// block structure whose zero blocks the trailing updates skip.
 template <typename T>
 FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16 = false,
                       const BlockFill* bf = nullptr);
 
 // 1 if any entry of the blocks a block structure declares zero (below the
 // diagonal) is non-zero.
-int block_fill_violated(const float* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st);
+template <typename T>
+int block_fill_violated(const T* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st);
 
 // W <- H^-1 W for r right-hand sides with the factor of sym_factor.
 template <typename T>

@@ -191,10 +191,10 @@ def test_factor_solve_residual_large(om, precision):
     assert res < 1e-12
 
 
-@pytest.mark.parametrize("layers", [3, 4])
-def test_block_structured_factor_is_bitwise_dense(om, layers):
-    """The split-FP16 factorisation told the HeadMat's block structure
-    (Geometry.factor_segments) skips trailing-update tiles that only multiply
+@pytest.mark.parametrize("layers,precision", [(3, "mixed16"), (4, "mixed16"), (4, "mixed"), (4, "fp64")])
+def test_block_structured_factor_is_bitwise_dense(om, layers, precision):
+    """The factorisations told the HeadMat's block structure
+    (Geometry.factor_segments) skip trailing-update work that only multiplies
     structural zeros: the factor must be bitwise the dense one (the skipped
     tiles would add exact zeros).  3 layers of 2562 vertices (n = 17926) and
     4 layers of 1442 vertices (n = 14408); a pattern that declares a
@@ -210,14 +210,15 @@ def test_block_structured_factor_is_bitwise_dense(om, layers):
     n = g.size
     starts, fill = g.factor_segments()
     assert fill.sum() < fill.shape[0] * (fill.shape[0] - 1) // 2   # some blocks stay zero
-    H = _head_mat_device(g, far_field=True)
-    Fd = factor(H, n, "mixed16")
-    Fs = factor(H, n, "mixed16", structure=(starts, fill))
+    H = _head_mat_device(g, far_field=precision != "fp64")
+    inplace = precision == "fp64"
+    Fd = factor(H.clone() if inplace else H, n, precision)
+    Fs = factor(H.clone() if inplace else H, n, precision, structure=(starts, fill))
     Ld = Fd.L.view(n, n).t().tril()       # lower triangle (column-major storage)
     Ls = Fs.L.view(n, n).t().tril()
-    print("n=%d segments %s, zero blocks %d: factor max |diff| %.3e" % (
-        n, starts.tolist(), int((np.tril(np.ones_like(fill), -1) - fill).sum()), float((Ld - Ls).abs().max())),
-        flush=True)
+    print("n=%d %s segments %s, zero blocks %d: factor max |diff| %.3e" % (
+        n, precision, starts.tolist(), int((np.tril(np.ones_like(fill), -1) - fill).sum()),
+        float((Ld - Ls).abs().max())), flush=True)
     assert torch.equal(Ld, Ls)
     bad = fill.copy()
     s, t = np.argwhere(np.tril(fill, -1))[0]
