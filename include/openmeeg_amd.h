@@ -166,20 +166,21 @@ int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, in
  * counterpart in the reference, which factors the packed matrix densely,
  * symmatrix.h:259-297; the result is the same).  The unknowns are cut into
  * nseg consecutive segments starting at seg_start[0] = 0 < seg_start[1] <
- * ...; fill[s * nseg + t] (s > t) is 0 when rows of segment s stay zero in
- * L below the columns of segment t.  The pattern must be closed under
- * elimination and the blocks it declares zero must be zero in d_A (both
- * checked: invalid argument otherwise).  HeadMat's pattern comes from its
- * mesh pairs (api.py Geometry.factor_segments). */
+ * ... < n; hfill[s * nseg + t] (s > t; the other entries are not read) is 0
+ * when the block of d_A with the rows of segment s and the columns of
+ * segment t is zero.  L's pattern is its symbolic factorisation.  The
+ * blocks declared zero are checked on the device (invalid argument when
+ * one holds a non-zero).  HeadMat's pattern comes from its mesh pairs
+ * (api.py Geometry.factor_segments). */
 int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
-                                    const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream);
+                                    const int64_t* seg_start, const uint8_t* hfill, int32_t* info, void* stream);
 /* The same for omh_sym_factor_f32 / omh_sym_factor_f64 (the trailing update
  * of each block runs as products over the pairs of row intervals its panel
  * can reach). */
 int omh_sym_factor_f32_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
-                              const uint8_t* fill, int32_t* info, void* stream);
+                              const uint8_t* hfill, int32_t* info, void* stream);
 int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
-                              const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream);
+                              const int64_t* seg_start, const uint8_t* hfill, int32_t* info, void* stream);
 
 /* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
  * symmatrix.cpp:133-152; linsolve gain.h:52-57). */
@@ -187,6 +188,15 @@ int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_wo
                       int64_t nrhs, void* stream);
 int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
                       int64_t nrhs, void* stream);
+/* The same with the block structure the factor was computed with (the
+ * arguments of the *_blocks factorisations): the updates below and above
+ * each group of diagonal blocks run only over the rows that L's non-zero
+ * blocks reach.  Not checked against d_L: pass the structure the factor
+ * was made with.  No reference counterpart. */
+int omh_sym_solve_f64_blocks(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
+                             int64_t nrhs, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill, void* stream);
+int omh_sym_solve_f32_blocks(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
+                             int64_t nrhs, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill, void* stream);
 
 /* Mixed-precision solve X = H^-1 B: FP32 factor (d_L32, d_work32) with FP64
  * residuals on the FP64 matrix d_H and iterative refinement until the
@@ -197,6 +207,14 @@ int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_wor
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
                     const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs,
                     int32_t max_iter, double tol, int32_t* iterations, double* rel_update, void* stream);
+/* The same with the block structure of d_H (as for the *_blocks
+ * factorisations, and the one the factor d_L32 was made with): the FP64
+ * residual skips d_H's zero blocks, the FP32 solves L's.  Not checked
+ * against d_H. */
+int omh_mixed_solve_blocks(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                           const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                           int64_t nrhs, int32_t max_iter, double tol, int32_t nseg, const int64_t* seg_start,
+                           const uint8_t* hfill, int32_t* iterations, double* rel_update, void* stream);
 
 /* C = alpha op(A) op(B) + beta C: the DGEMM products of
  * OpenMEEGMaths/include/matrix.h:333-350 and gain.h:69-89. */

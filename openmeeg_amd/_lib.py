@@ -55,6 +55,10 @@ _SIGS = {
     "omh_sym_solve_f64": ([vp, i64, i64, vp, vp, i64, i64, vp], C.c_int),
     "omh_sym_solve_f32": ([vp, i64, i64, vp, vp, i64, i64, vp], C.c_int),
     "omh_mixed_solve": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, p_i32, p_f64, vp], C.c_int),
+    "omh_sym_solve_f64_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
+    "omh_sym_solve_f32_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
+    "omh_mixed_solve_blocks": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, i32, vp, vp, p_i32, p_f64,
+                                vp], C.c_int),
     "omh_gemm_f64": ([i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64, vp], C.c_int),
     "omh_gemm_f32": ([i32, i32, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64, vp], C.c_int),
     "omh_convert_f64_f32": ([vp, i64, vp, i64, i64, i64, i32, vp], C.c_int),

@@ -178,15 +178,15 @@ class Geometry:
 
     def factor_segments(self):
         """Block structure of the HeadMat in triangles-first order for the
-        factorisation: segments (each mesh's triangle unknowns in mesh order,
-        then each mesh's vertex unknowns) and the lower-triangle pattern of
-        L over them, from a symbolic elimination of the mesh-pair coupling
-        (operators of assembleHeadMat.cpp:73-135: S between triangle sets,
-        N between vertex sets, D/D* between the triangles of one mesh and
-        the vertices of the other; the deflation couples the outermost
-        meshes of a part).  Returns (starts, fill) with fill an nseg x nseg
-        uint8 array, or None when a mesh's vertex unknowns are not one
-        contiguous range."""
+        factorisation and the solves: segments (each mesh's triangle unknowns
+        in mesh order, then each mesh's vertex unknowns) and the
+        lower-triangle pattern of the matrix over them, from the mesh-pair
+        coupling (operators of assembleHeadMat.cpp:73-135: S between
+        triangle sets, N between vertex sets, D/D* between the triangles of
+        one mesh and the vertices of the other; the deflation couples the
+        outermost meshes of a part).  Returns (starts, hfill) with hfill an
+        nseg x nseg uint8 array, or None when a mesh's vertex unknowns are
+        not one contiguous range."""
         nt, nv = self._nt, self._nv
         tseg, vseg = {}, {}
         for m in range(self._nm):
@@ -224,13 +224,8 @@ class Geometry:
             for a in ms:
                 for b in ms:
                     link(("V", a), ("V", b))
-        for k in range(ns):   # symbolic elimination in segment order
-            nb = [i for i in range(k + 1, ns) if adj[i, k]]
-            for i in nb:
-                for j in nb:
-                    adj[i, j] = True
-        fill = np.tril(adj, -1).astype(np.uint8)
-        return np.array([a for a, _, _, _ in segs], dtype=np.int64), fill
+        hfill = np.tril(adj, -1).astype(np.uint8)
+        return np.array([a for a, _, _, _ in segs], dtype=np.int64), hfill
 
     def tri_first_index(self, ref_rows):
         """Map reference-order unknown indices to triangles-first order."""
@@ -506,25 +501,50 @@ def from_device_rows(D, n, nv, c):
 class Factor:
     """A stored pivot-free signed Cholesky factor on the device."""
 
-    def __init__(self, L, n, work, precision, H64=None):
+    def __init__(self, L, n, work, precision, H64=None, structure=None):
         self.L, self.n, self.work, self.precision, self.H64 = L, n, work, precision, H64
         self.iterations = 0
         self.rel_update = 0.0
+        self._structure = _block_args(structure)
 
     def solve(self, B, nrhs, max_iter=10, tol=1e-12):
-        """B <- H^-1 B (float64 device buffer, n x nrhs column-major)."""
+        """B <- H^-1 B (float64 device buffer, n x nrhs column-major).  With
+        the block structure the factor was made with, the solves and the
+        residuals skip its zero blocks."""
+        blocks = self._structure[2:] if self._structure else ()
         if self.precision == "fp64":
-            check(lib().omh_sym_solve_f64(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n, nrhs,
-                                          _stream()))
+            if blocks:
+                check(lib().omh_sym_solve_f64_blocks(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n,
+                                                     nrhs, *blocks, _stream()))
+            else:
+                check(lib().omh_sym_solve_f64(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n, nrhs,
+                                              _stream()))
             return B
         X = _dev_f64(self.n * nrhs)
         it, rel = C.c_int32(), C.c_double()
-        check(lib().omh_mixed_solve(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work), _ptr(B),
-                                    self.n, _ptr(X), self.n, nrhs, max_iter, tol, C.byref(it), C.byref(rel),
-                                    _stream()))
+        if blocks:
+            check(lib().omh_mixed_solve_blocks(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
+                                               _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, *blocks,
+                                               C.byref(it), C.byref(rel), _stream()))
+        else:
+            check(lib().omh_mixed_solve(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
+                                        _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, C.byref(it),
+                                        C.byref(rel), _stream()))
         self.iterations, self.rel_update = it.value, rel.value
         B.copy_(X)
         return B
+
+
+def _block_args(structure):
+    """(starts, hfill, nseg, starts pointer, hfill pointer) for the *_blocks
+    entry points (the arrays are kept with the pointers), or None."""
+    if structure is None:
+        return None
+    starts = np.ascontiguousarray(structure[0], dtype=np.int64)
+    hfill = np.ascontiguousarray(structure[1], dtype=np.uint8)
+    if hfill.shape != (len(starts), len(starts)):
+        raise ValueError("hfill must be nseg x nseg")
+    return (starts, hfill, len(starts), starts.ctypes.data_as(C.c_void_p), hfill.ctypes.data_as(C.c_void_p))
 
 
 def factor(Hdev, n, precision="fp64", structure=None):
@@ -532,22 +552,19 @@ def factor(Hdev, n, precision="fp64", structure=None):
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
     factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
     trailing updates.  Given structure = Geometry.factor_segments(), the
-    trailing updates skip the parts that only multiply structural zeros
-    (the factor is the same)."""
+    trailing updates, and later the solves, skip the parts that only
+    multiply structural zeros (the factor is the same)."""
     torch = _t()
     info = C.c_int32()
-    blocks = ()
-    if structure is not None:
-        starts = np.ascontiguousarray(structure[0], dtype=np.int64)
-        fill = np.ascontiguousarray(structure[1], dtype=np.uint8)
-        blocks = (len(starts), starts.ctypes.data_as(C.c_void_p), fill.ctypes.data_as(C.c_void_p))
+    sargs = _block_args(structure)
+    blocks = sargs[2:] if sargs else ()
     if precision == "fp64":
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
         if blocks:
             check(lib().omh_sym_factor_f64_blocks(_ptr(Hdev), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
         else:
             check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(Hdev, n, work, "fp64")
+        F = Factor(Hdev, n, work, "fp64", structure=structure)
     elif precision in ("mixed", "mixed16"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
         check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
@@ -558,7 +575,7 @@ def factor(Hdev, n, precision="fp64", structure=None):
         else:
             fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
             check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(L32, n, work, "mixed", H64=Hdev)
+        F = Factor(L32, n, work, "mixed", H64=Hdev, structure=structure)
     else:
         raise ValueError("precision must be 'fp64', 'mixed' or 'mixed16'")
     if info.value != 0:

@@ -447,13 +447,12 @@ int omh_sym_factor_f32_split(float* d_A, int64_t n, int64_t ld, void* d_work, in
 }
 
 extern "C++" {
-// Block structure of the *_blocks factorisations: segments from seg_start,
-// fill pattern closed under elimination, its zero blocks zero in d_A.
-template <typename T>
-static omh::BlockFill block_structure(const T* d_A, int64_t n, int64_t ld, int32_t nseg, const int64_t* seg_start,
-                                      const uint8_t* fill, hipStream_t st) {
+// Block structure of the *_blocks entry points: segments from seg_start, the
+// matrix's block pattern from hfill (lower triangle, s > t), L's from its
+// symbolic factorisation.
+static omh::BlockFill block_pattern(int64_t n, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill) {
     need(seg_start, "seg_start");
-    need(fill, "fill");
+    need(hfill, "hfill");
     if (nseg < 1 || nseg > 4096) throw std::invalid_argument("bad segment count");
     omh::BlockFill bf;
     bf.start.assign(seg_start, seg_start + nseg);
@@ -461,28 +460,22 @@ static omh::BlockFill block_structure(const T* d_A, int64_t n, int64_t ld, int32
     if (bf.start[0] != 0) throw std::invalid_argument("the first segment must start at 0");
     for (int s = 0; s < nseg; ++s)
         if (bf.start[s + 1] <= bf.start[s]) throw std::invalid_argument("segment starts must increase below n");
-    bf.fill.assign(fill, fill + (size_t)nseg * nseg);
-    // closed under elimination: two segments reached from the same earlier
-    // column segment are coupled below it
-    for (int t = 0; t < nseg; ++t)
-        for (int a = t + 1; a < nseg; ++a)
-            for (int b = a + 1; b < nseg; ++b)
-                if (bf.nz(a, t) && bf.nz(b, t) && !bf.nz(b, a))
-                    throw std::invalid_argument("block structure is not closed under elimination");
-    if (omh::block_fill_violated<T>(d_A, n, ld, bf, st))
-        throw std::invalid_argument("a block declared zero holds non-zero entries");
+    bf.hfill.assign(hfill, hfill + (size_t)nseg * nseg);
+    bf.symbolic();
     return bf;
 }
 
 template <typename T>
 static int factor_blocks(T* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
-                         const uint8_t* fill, int32_t* info, void* stream, bool split16) {
+                         const uint8_t* hfill, int32_t* info, void* stream, bool split16) {
     return guard([&] {
         need(d_A, "d_A");
         need(d_work, "d_work");
         if (n <= 0 || ld < n) throw std::invalid_argument("bad matrix size");
         hipStream_t st = S(stream);
-        const omh::BlockFill bf = block_structure<T>(d_A, n, ld, nseg, seg_start, fill, st);
+        const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
+        if (omh::block_fill_violated<T>(d_A, n, ld, bf, st))
+            throw std::invalid_argument("a block declared zero holds non-zero entries");
         const auto fi = omh::sym_factor<T>(d_A, n, ld, d_work, st, split16, &bf);
         if (info) *info = fi.info;
     });
@@ -490,18 +483,18 @@ static int factor_blocks(T* d_A, int64_t n, int64_t ld, void* d_work, int32_t ns
 }  // extern "C++"
 
 int omh_sym_factor_f32_split_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
-                                    const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream) {
-    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, true);
+                                    const int64_t* seg_start, const uint8_t* hfill, int32_t* info, void* stream) {
+    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, hfill, info, stream, true);
 }
 
 int omh_sym_factor_f32_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
-                              const uint8_t* fill, int32_t* info, void* stream) {
-    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, false);
+                              const uint8_t* hfill, int32_t* info, void* stream) {
+    return factor_blocks<float>(d_A, n, ld, d_work, nseg, seg_start, hfill, info, stream, false);
 }
 
 int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
-                              const int64_t* seg_start, const uint8_t* fill, int32_t* info, void* stream) {
-    return factor_blocks<double>(d_A, n, ld, d_work, nseg, seg_start, fill, info, stream, false);
+                              const int64_t* seg_start, const uint8_t* hfill, int32_t* info, void* stream) {
+    return factor_blocks<double>(d_A, n, ld, d_work, nseg, seg_start, hfill, info, stream, false);
 }
 
 int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
@@ -526,6 +519,30 @@ int omh_sym_solve_f32(const float* d_L, int64_t n, int64_t ld, const void* d_wor
     });
 }
 
+int omh_sym_solve_f64_blocks(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,
+                             int64_t nrhs, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill, void* stream) {
+    return guard([&] {
+        need(d_L, "d_L");
+        need(d_work, "d_work");
+        need(d_B, "d_B");
+        if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
+        const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
+        omh::sym_solve<double>(d_L, n, ld, d_work, d_B, ldb, nrhs, S(stream), &bf);
+    });
+}
+
+int omh_sym_solve_f32_blocks(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
+                             int64_t nrhs, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill, void* stream) {
+    return guard([&] {
+        need(d_L, "d_L");
+        need(d_work, "d_work");
+        need(d_B, "d_B");
+        if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
+        const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
+        omh::sym_solve<float>(d_L, n, ld, d_work, d_B, ldb, nrhs, S(stream), &bf);
+    });
+}
+
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32, const void* d_work32,
                     const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs, int32_t max_iter,
                     double tol, int32_t* iterations, double* rel_update, void* stream) {
@@ -540,6 +557,27 @@ int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L3
         double rel = 0.0;
         const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
                                          &rel, S(stream));
+        if (iterations) *iterations = it;
+        if (rel_update) *rel_update = rel;
+    });
+}
+
+int omh_mixed_solve_blocks(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                           const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                           int64_t nrhs, int32_t max_iter, double tol, int32_t nseg, const int64_t* seg_start,
+                           const uint8_t* hfill, int32_t* iterations, double* rel_update, void* stream) {
+    return guard([&] {
+        need(d_H, "d_H");
+        need(d_L32, "d_L32");
+        need(d_work32, "d_work32");
+        need(d_B, "d_B");
+        need(d_X, "d_X");
+        if (n <= 0 || ldh < n || ld32 < n || ldb < n || ldx < n || nrhs < 0 || max_iter < 1)
+            throw std::invalid_argument("bad sizes");
+        const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
+        double rel = 0.0;
+        const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
+                                         &rel, S(stream), &bf);
         if (iterations) *iterations = it;
         if (rel_update) *rel_update = rel;
     });

@@ -1405,9 +1405,10 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
 //   the next group's prologue on the side stream, while the rest of the
 //   trailing matrix -= [X_1 .. X_P] S [X_1 .. X_P]^T as one split-FP16 SYRK.
 // Row intervals [lo, hi) at or below row r that the update with the panel
-// of columns [c0, c1) can reach (segments whose rows may be non-zero there),
-// merged; empty when that is every row from r on.
-static void active_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r,
+// of columns [c0, c1) can reach (segments whose rows may be non-zero there
+// in L), merged.  Returns false when that is every row from r on (iv is then
+// empty); true with the intervals (possibly none) otherwise.
+static bool active_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r,
                         std::vector<std::pair<int64_t, int64_t>>& iv) {
     iv.clear();
     const int ns = bf.nseg();
@@ -1420,7 +1421,34 @@ static void active_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, 
         if (!iv.empty() && iv.back().second == lo) iv.back().second = hi;
         else iv.emplace_back(lo, hi);
     }
-    if (iv.size() == 1 && iv[0].first == r && iv[0].second == n) iv.clear();
+    if (iv.size() == 1 && iv[0].first == r && iv[0].second == n) {
+        iv.clear();
+        return false;
+    }
+    return true;
+}
+
+// Column intervals [lo, hi) below column c that rows [r0, r1) of L can
+// reach (the column segments of their possibly non-zero blocks), merged;
+// the same convention as active_rows.
+static bool active_cols(const BlockFill& bf, int64_t r0, int64_t r1, int64_t c,
+                        std::vector<std::pair<int64_t, int64_t>>& iv) {
+    iv.clear();
+    const int ns = bf.nseg();
+    for (int t = 0; t < ns; ++t) {
+        bool act = false;
+        for (int s = 0; s < ns && !act; ++s)
+            if (bf.start[s] < r1 && bf.start[s + 1] > r0 && s >= t && bf.nz(s, t)) act = true;
+        const int64_t lo = bf.start[t], hi = std::min(bf.start[t + 1], c);
+        if (!act || hi <= lo) continue;
+        if (!iv.empty() && iv.back().second == lo) iv.back().second = hi;
+        else iv.emplace_back(lo, hi);
+    }
+    if (iv.size() == 1 && iv[0].first == 0 && iv[0].second == c) {
+        iv.clear();
+        return false;
+    }
+    return true;
 }
 
 // Tile rows (S_TILE, relative to row r) that the rank-S_KP update of the
@@ -1649,8 +1677,7 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         // the row intervals the panel can reach, pair by pair
         if (m > nb2) {
             std::vector<std::pair<int64_t, int64_t>> iv;
-            if (bf) active_rows(*bf, n, k, e, e + nb2, iv);
-            if (iv.empty()) {
+            if (!bf || !active_rows(*bf, n, k, e, e + nb2, iv)) {
                 gemm<T>(false, true, true, m - nb2, m - nb2, nb, T(-1), X + nb2, ldx, A21 + nb2, lda, T(1),
                         A22 + nb2 + nb2 * lda, lda, st, upd);
             } else {
@@ -1686,7 +1713,7 @@ int block_fill_violated(const T* A, int64_t n, int64_t lda, const BlockFill& bf,
     const int ns = bf.nseg();
     for (int s = 0; s < ns; ++s)
         for (int t = 0; t < s; ++t) {
-            if (bf.nz(s, t)) continue;
+            if (bf.hnz(s, t)) continue;
             const int64_t r0 = bf.start[s], r1 = std::min(bf.start[s + 1], n);
             const int64_t c0 = bf.start[t], c1 = std::min(bf.start[t + 1], n);
             if (r1 <= r0 || c1 <= c0) continue;
@@ -1711,7 +1738,8 @@ template int block_fill_violated<double>(const double*, int64_t, int64_t, const 
 constexpr int SOLVE_GROUP = OMH_SOLVE_GROUP;
 
 template <typename T>
-void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st) {
+void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st,
+               const BlockFill* bf) {
     if (r <= 0) return;
     const T* Dinv = (const T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
@@ -1722,6 +1750,7 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     // rest of W with the whole group in one product of depth
     // SOLVE_GROUP x NB.
     constexpr int64_t GW = (int64_t)SOLVE_GROUP * FACT_NB;
+    std::vector<std::pair<int64_t, int64_t>> iv;
     // forward: W <- L^-1 W
     for (int64_t k0 = 0; k0 < n; k0 += GW) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
@@ -1736,9 +1765,17 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
                 gemm<T>(false, false, false, gend - e, r, nb, T(-1), L + e + k * lda, lda, W + k, ldw, T(1), W + e,
                         ldw, st);
         }
-        if (gend < n)
-            gemm<T>(false, false, false, n - gend, r, gend - k0, T(-1), L + gend + k0 * lda, lda, W + k0, ldw, T(1),
-                    W + gend, ldw, st);
+        // rows below the group: with a block structure only those its
+        // columns reach in L
+        if (gend < n) {
+            if (!bf || !active_rows(*bf, n, k0, gend, gend, iv))
+                gemm<T>(false, false, false, n - gend, r, gend - k0, T(-1), L + gend + k0 * lda, lda, W + k0, ldw,
+                        T(1), W + gend, ldw, st);
+            else
+                for (const auto& v : iv)
+                    gemm<T>(false, false, false, v.second - v.first, r, gend - k0, T(-1), L + v.first + k0 * lda,
+                            lda, W + k0, ldw, T(1), W + v.first, ldw, st);
+        }
     }
     // W <- S W
     k_scale_rows<T><<<grid_for(n * r), 256, 0, st>>>(W, ldw, n, r, sgn);
@@ -1757,13 +1794,22 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
                 gemm<T>(true, false, false, k - k0, r, nb, T(-1), L + k + k0 * lda, lda, W + k, ldw, T(1), W + k0,
                         ldw, st);
         }
-        if (k0 > 0)
-            gemm<T>(true, false, false, k0, r, gend - k0, T(-1), L + k0, lda, W + k0, ldw, T(1), W, ldw, st);
+        // rows above the group: only the columns its rows reach in L
+        if (k0 > 0) {
+            if (!bf || !active_cols(*bf, k0, gend, k0, iv))
+                gemm<T>(true, false, false, k0, r, gend - k0, T(-1), L + k0, lda, W + k0, ldw, T(1), W, ldw, st);
+            else
+                for (const auto& v : iv)
+                    gemm<T>(true, false, false, v.second - v.first, r, gend - k0, T(-1), L + k0 + v.first * lda, lda,
+                            W + k0, ldw, T(1), W + v.first, ldw, st);
+        }
     }
 }
 
-template void sym_solve<double>(const double*, int64_t, int64_t, const void*, double*, int64_t, int64_t, hipStream_t);
-template void sym_solve<float>(const float*, int64_t, int64_t, const void*, float*, int64_t, int64_t, hipStream_t);
+template void sym_solve<double>(const double*, int64_t, int64_t, const void*, double*, int64_t, int64_t, hipStream_t,
+                                const BlockFill*);
+template void sym_solve<float>(const float*, int64_t, int64_t, const void*, float*, int64_t, int64_t, hipStream_t,
+                               const BlockFill*);
 
 void convert_f64_to_f32(const double* src, int64_t lds, float* dst, int64_t ldd, int64_t m, int64_t n, bool lower,
                         hipStream_t st) {
@@ -1808,9 +1854,32 @@ bool refine_converged(int it, double rel, double prev, double tol) {
     return it >= 2 && rel > 0.5 * prev;                   // stagnation at the FP64 rounding floor
 }
 
+// R <- R - H X; with a block structure row segment by row segment, each
+// run of consecutive non-zero column blocks of H in one product.
+static void residual_update(const double* H, int64_t n, int64_t ldh, const double* X, int64_t ldx, int64_t r,
+                            double* R, int64_t ldr, hipStream_t st, const BlockFill* bf) {
+    if (!bf) {
+        gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, ldr, st);
+        return;
+    }
+    const int ns = bf->nseg();
+    for (int s = 0; s < ns; ++s) {
+        const int64_t r0 = bf->start[s], r1 = bf->start[s + 1];
+        for (int t = 0; t < ns;) {
+            if (!bf->hnz(s, t)) { ++t; continue; }
+            int u = t + 1;
+            while (u < ns && bf->hnz(s, u)) ++u;
+            const int64_t c0 = bf->start[t], c1 = bf->start[u];
+            gemm<double>(false, false, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, X + c0, ldx, 1.0,
+                         R + r0, ldr, st);
+            t = u;
+        }
+    }
+}
+
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, hipStream_t st) {
+                 double* final_rel_update, hipStream_t st, const BlockFill* bf) {
     float* R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r);
     double* R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r);
     fill_f64(X, n, r, ldx, 0.0, st);
@@ -1819,7 +1888,7 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
     double rel = 1.0, prev = 2.0;
     for (; it < max_iter; ++it) {
         convert_f64_to_f32(R, n, R32, n, n, r, false, st);
-        sym_solve<float>(L32, n, ld32, work32, R32, n, r, st);
+        sym_solve<float>(L32, n, ld32, work32, R32, n, r, st, bf);
         k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
         OMH_LAUNCH_CHECK();
         const double dn = frob_norm_f32(R32, n, n, r, st);
@@ -1829,7 +1898,7 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
         prev = rel;
         // R = B - H X
         copy_f64(B, ldb, R, n, n, r, st);
-        gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, n, st);
+        residual_update(H, n, ldh, X, ldx, r, R, n, st, bf);
     }
     if (final_rel_update) *final_rel_update = rel;
     return it;

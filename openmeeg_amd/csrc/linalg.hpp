@@ -112,17 +112,32 @@ struct FactorInfo {
 // element size elem.
 size_t factor_workspace_bytes(int64_t n, size_t elem);
 
-// Block structure of a matrix for the factorisation: the unknowns are cut
-// into consecutive segments (start[s] .. start[s + 1], start[nseg] = n) and
-// fill[s * nseg + t] (s > t) is 0 when rows of segment s are structurally
-// zero in L below the columns of segment t (a symbolic factorisation, closed
-// under elimination).  Trailing updates skip the tiles whose rows or columns
-// lie only in such segments; those tiles would add exact zeros.
+// Block structure of a symmetric matrix: the unknowns are cut into
+// consecutive segments (start[s] .. start[s + 1], start[nseg] = n);
+// hfill[s * nseg + t] (s > t) is 0 when the block (rows of s, columns of t)
+// of the matrix is zero, and fill, its symbolic factorisation, is 0 when
+// that block stays zero in L.  Factor and solve updates skip the work that
+// only multiplies such blocks (it would add exact zeros).
 struct BlockFill {
     std::vector<int64_t> start;
-    std::vector<uint8_t> fill;
+    std::vector<uint8_t> hfill, fill;
     int nseg() const { return (int)start.size() - 1; }
-    bool nz(int s, int t) const { return s <= t || fill[(size_t)s * nseg() + t] != 0; }
+    bool nz(int s, int t) const { return s <= t || fill[(size_t)s * nseg() + t] != 0; }     // L, s >= t
+    bool hnz(int s, int t) const {                                                          // the matrix
+        if (s == t) return true;
+        return s > t ? hfill[(size_t)s * nseg() + t] != 0 : hfill[(size_t)t * nseg() + s] != 0;
+    }
+    // fill from hfill by elimination in segment order
+    void symbolic() {
+        const int ns = nseg();
+        fill = hfill;
+        for (int k = 0; k < ns; ++k)
+            for (int a = k + 1; a < ns; ++a) {
+                if (!fill[(size_t)a * ns + k]) continue;
+                for (int b = a + 1; b < ns; ++b)
+                    if (fill[(size_t)b * ns + k]) fill[(size_t)b * ns + a] = 1;
+            }
+    }
 };
 
 // In-place pivot-free signed Cholesky H = L S L^T of the lower triangle of A
@@ -140,9 +155,11 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
 template <typename T>
 int block_fill_violated(const T* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st);
 
-// W <- H^-1 W for r right-hand sides with the factor of sym_factor.
+// W <- H^-1 W for r right-hand sides with the factor of sym_factor (bf:
+// optional block structure of H; the updates skip L's zero blocks).
 template <typename T>
-void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st);
+void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st,
+               const BlockFill* bf = nullptr);
 
 void convert_f64_to_f32(const double* src, int64_t lds, float* dst, int64_t ldd, int64_t m, int64_t n, bool lower,
                         hipStream_t st);
@@ -162,7 +179,7 @@ bool refine_converged(int it, double rel, double prev, double tol);
 // corrections; returns the number of corrections applied.
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, hipStream_t st);
+                 double* final_rel_update, hipStream_t st, const BlockFill* bf = nullptr);
 
 // out(e, j) = sum_k vals(e,k) Hm(cols(e,k), j)  (3 non-zeros per row).
 // out (n x ne, leading dimension ldo) = the dense transpose of a sparse

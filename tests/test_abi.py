@@ -52,18 +52,46 @@ def test_gpu_entry_points_refuse_without_device():
         om.HeadMat(g)
 
 
-def test_block_structure_must_be_closed():
-    """omh_sym_factor_f32_split_blocks refuses a segment pattern that is not
-    closed under elimination before touching the device (two segments reached
-    from the same earlier segment must be coupled below it)."""
+def test_block_structure_arguments_are_checked():
+    """The *_blocks entry points refuse segment starts that do not begin at 0
+    and increase below n before touching the device."""
     import ctypes as C
     from openmeeg_amd import _lib
-    starts = np.array([0, 10, 20], dtype=np.int64)
-    fill = np.zeros((3, 3), dtype=np.uint8)
-    fill[1, 0] = fill[2, 0] = 1          # 1 and 2 both below 0 ...
+    L = _lib.lib()
+    hfill = np.zeros((3, 3), dtype=np.uint8)
     info = C.c_int32()
     dummy = C.c_void_p(16)               # never dereferenced: the check fails first
-    rc = _lib.lib().omh_sym_factor_f32_split_blocks(dummy, 30, 30, dummy, 3, starts.ctypes.data_as(C.c_void_p),
-                                                     fill.ctypes.data_as(C.c_void_p), C.byref(info), C.c_void_p(0))
-    assert rc == 2                       # ... but 2 is not coupled to 1
-    assert b"closed" in _lib.lib().omh_last_error()
+    hp = hfill.ctypes.data_as(C.c_void_p)
+    for starts in ([0, 20, 10], [1, 10, 20], [0, 10, 30]):
+        st = np.array(starts, dtype=np.int64)
+        sp = st.ctypes.data_as(C.c_void_p)
+        assert L.omh_sym_factor_f32_split_blocks(dummy, 30, 30, dummy, 3, sp, hp, C.byref(info), C.c_void_p(0)) == 2
+        assert b"segment" in L.omh_last_error()
+        assert L.omh_sym_solve_f64_blocks(dummy, 30, 30, dummy, dummy, 30, 4, 3, sp, hp, C.c_void_p(0)) == 2
+        it, rel = C.c_int32(), C.c_double()
+        assert L.omh_mixed_solve_blocks(dummy, 30, 30, dummy, 30, dummy, dummy, 30, dummy, 30, 4, 5, 1e-8, 3, sp, hp,
+                                        C.byref(it), C.byref(rel), C.c_void_p(0)) == 2
+        assert b"segment" in L.omh_last_error()
+
+
+def test_head1_block_pattern_matches_golden_headmat():
+    """Geometry.factor_segments' block pattern of the HeadMat against the
+    reference's HeadMat of Head1 (tests/golden/head1/hm.npy): every block it
+    declares zero is exactly zero there, every block it declares non-zero
+    holds entries."""
+    import openmeeg_amd as om
+    g = om.read_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    from oracle import oracle as O
+    n = g.size
+    hm = O.unpack(np.load(os.path.join(GOLDEN, "head1", "hm.npy")), n)   # packed upper -> dense
+    p = g.tri_first_index(np.arange(n))
+    Htf = np.empty_like(hm)
+    Htf[np.ix_(p, p)] = hm
+    starts, hfill = g.factor_segments()
+    ends = list(starts[1:]) + [n]
+    ns = len(starts)
+    assert hfill.shape == (ns, ns) and ns >= 3
+    for s in range(ns):
+        for t in range(s):
+            blk = Htf[starts[s]:ends[s], starts[t]:ends[t]]
+            assert bool(np.any(blk != 0)) == bool(hfill[s, t]), (s, t)

@@ -192,13 +192,15 @@ def test_factor_solve_residual_large(om, precision):
 
 
 @pytest.mark.parametrize("layers,precision", [(3, "mixed16"), (4, "mixed16"), (4, "mixed"), (4, "fp64")])
-def test_block_structured_factor_is_bitwise_dense(om, layers, precision):
+def test_block_structured_factor_and_solve(om, layers, precision):
     """The factorisations told the HeadMat's block structure
     (Geometry.factor_segments) skip trailing-update work that only multiplies
     structural zeros: the factor must be bitwise the dense one (the skipped
-    tiles would add exact zeros).  3 layers of 2562 vertices (n = 17926) and
-    4 layers of 1442 vertices (n = 14408); a pattern that declares a
-    non-zero block zero is refused."""
+    tiles would add exact zeros).  The structured solves skip L's and H's
+    zero blocks: the same solution (FP64 triangular solves: the same sums;
+    refined: the residual's sums in another order, 1e-12).  3 layers of 2562
+    vertices (n = 17926) and 4 layers of 1442 vertices (n = 14408); a
+    pattern that declares a non-zero block zero is refused."""
     import ctypes as C
     import torch
     from openmeeg_amd import spheres as S, _lib
@@ -208,22 +210,30 @@ def test_block_structured_factor_is_bitwise_dense(om, layers, precision):
     else:
         g = om.make_nested_geometry(S.nested_sphere_meshes(12, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
     n = g.size
-    starts, fill = g.factor_segments()
-    assert fill.sum() < fill.shape[0] * (fill.shape[0] - 1) // 2   # some blocks stay zero
+    starts, hfill = g.factor_segments()
+    assert hfill.sum() < hfill.shape[0] * (hfill.shape[0] - 1) // 2   # some blocks of H are zero
     H = _head_mat_device(g, far_field=precision != "fp64")
     inplace = precision == "fp64"
     Fd = factor(H.clone() if inplace else H, n, precision)
-    Fs = factor(H.clone() if inplace else H, n, precision, structure=(starts, fill))
+    Fs = factor(H.clone() if inplace else H, n, precision, structure=(starts, hfill))
     Ld = Fd.L.view(n, n).t().tril()       # lower triangle (column-major storage)
     Ls = Fs.L.view(n, n).t().tril()
-    print("n=%d %s segments %s, zero blocks %d: factor max |diff| %.3e" % (
-        n, precision, starts.tolist(), int((np.tril(np.ones_like(fill), -1) - fill).sum()),
+    print("n=%d %s segments %s, zero blocks of H %d: factor max |diff| %.3e" % (
+        n, precision, starts.tolist(), int((np.tril(np.ones_like(hfill), -1) - hfill).sum()),
         float((Ld - Ls).abs().max())), flush=True)
     assert torch.equal(Ld, Ls)
-    bad = fill.copy()
-    s, t = np.argwhere(np.tril(fill, -1))[0]
-    bad[s, t] = 0
-    # keep the pattern closed: the rows of s then no longer reach later blocks through t
+    del Ld, Ls
+    torch.manual_seed(5)
+    B = torch.randn(n * 24, dtype=torch.float64, device="cuda")
+    Xd, Xs = Fd.solve(B.clone(), 24), Fs.solve(B.clone(), 24)
+    diff = float(torch.linalg.norm(Xd - Xs) / torch.linalg.norm(Xd))
+    print("solve: structured vs dense %.3e (bitwise %s), refinement steps %d / %d" % (
+        diff, bool(torch.equal(Xd, Xs)), Fd.iterations, Fs.iterations), flush=True)
+    assert diff < (1e-15 if precision == "fp64" else 1e-12)
+    del Fd, Fs, Xd, Xs
+    bad = hfill.copy()
+    s, t = np.argwhere(np.tril(hfill, -1))[0]
+    bad[s, t] = 0                          # a non-zero block of H declared zero
     L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
     check = _lib.check
     check(_lib.lib().omh_convert_f64_f32(C.c_void_p(H.data_ptr()), n, C.c_void_p(L32.data_ptr()), n, n, n, 1,
@@ -234,7 +244,8 @@ def test_block_structured_factor_is_bitwise_dense(om, layers, precision):
         C.c_void_p(L32.data_ptr()), n, n, C.c_void_p(work.data_ptr()), len(starts),
         np.ascontiguousarray(starts).ctypes.data_as(C.c_void_p), np.ascontiguousarray(bad).ctypes.data_as(C.c_void_p),
         C.byref(info), C.c_void_p(0))
-    assert rc == 2   # refused: not closed, or a declared-zero block holds entries
+    assert rc == 2
+    assert b"declared zero" in _lib.lib().omh_last_error()
 
 
 def test_headmat_head1_matches_golden(om, head1):
