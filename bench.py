@@ -162,7 +162,7 @@ def run(args):
     dt = (time.time() - t0) / max(1, args.steps)
     kern = {k: prof(k) for k in ("factor_update_f32s", "factor_update_f32_t128", "factor_update_f32_t64",
                                   "factor_update_f64_t64",
-                                  "diag_factor_f32", "diag_factor_f64", "s_block", "d_pairs")}
+                                  "diag_factor_f32", "diag_factor_f64", "s_block", "d_pairs", "pairs_far")}
     _lib.check(L.omh_profile_enable(0))
     Gm_h = om.gain_to_numpy(Gm, len(el), len(dip))
     del Gm

@@ -5,7 +5,8 @@
 // DiagonalBlock (operators.h:180-197) / NonDiagonalBlock (:363-384), then
 // deflate (assembleHeadMat.cpp:28-60).
 //
-// Work decomposition (FP64-VALU bound, see the roofline notes in DESIGN.md):
+// Work decomposition (FP64 VALU bound: every pair evaluation is a few
+// dozen FP64 operations on per-triangle constants that stay in cache):
 //   S  : one thread per triangle pair, 16 analytic OperatorS evaluations
 //        (the analytic triangle is the row mesh, the Gauss triangle the
 //        column mesh, as the reference).  Diagonal blocks run over the upper
@@ -17,6 +18,9 @@
 //        reference, without atomics.
 //   N  : one thread per vertex pair gathering the ~6x6 incident triangle
 //        pairs' S values (operators.h:134-148).
+//   Far field (the mixed-precision path): S, D and D* of a mesh pair in one
+//   pass over its triangle pairs, far pairs from one set of Gauss-point
+//   distances, the other pairs through a queue (k_pairs_far / k_pairs_fix).
 // Every entry is written to both halves of the symmetric matrix.
 #include <hip/hip_runtime.h>
 
@@ -392,6 +396,236 @@ __global__ void __launch_bounds__(256) k_d_gather(const double* __restrict__ W, 
     H[c + r * L.ld] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Far-field assembly of a mesh pair (A rows, B columns), one stripe of A's
+// triangles at a time: S(t, u), D(t, u) and D(u, t) of each triangle pair
+// come out of one pass over the pairs, the far pairs from one set of
+// Gauss-point distances (pair_far).  Pairs that are not in the 7-point far
+// class go to a queue (near pairs from the front, 16-point pairs from the
+// back) for a second kernel, so that the waves of the first one do not
+// diverge over the classes.  The double-layer values go to two scratch
+// arrays, W12(k, t - c0, u) = D_k(t, u) and W21(k, t - c0, u) = D_k(u, t)
+// (3 x nc x n2 each, column fastest), gathered per vertex afterwards.
+// When A == B only pairs u >= t are evaluated (the gathers take D(t, u) for
+// u >= t from W12 and for u < t from W21).
+// ---------------------------------------------------------------------------
+struct PairStripe {
+    int64_t a0, c0, nc, n1;   // A's first triangle, stripe start and size, A's triangle count
+    int64_t b0, n2;           // B's first triangle, B's triangle count
+    int diag;                 // A == B
+    int temp;                 // S into tmp (n1 x n2) instead of the HeadMat
+    double coeffS;
+    int64_t ua0, ub0;         // reference unknowns of A's and B's first triangles
+    double* tmp;
+    double* W12;
+    double* W21;
+    int64_t* q;               // queue, capacity qcap
+    unsigned* qn;             // [near pairs, 16-point pairs]
+    int64_t qcap;
+};
+
+// Writes one pair's values: sv = S(t, u) with its coefficient, a = D(t, u),
+// b = D(u, t) (each already scaled by the areas).
+template <bool DA, bool DB>
+__device__ __forceinline__ void pair_store(const PairStripe& ps, double* __restrict__ H, const Layout& L, int64_t i,
+                                           int64_t il, int64_t j, double sv, d3 a, d3 b) {
+    if (ps.temp) {
+        ps.tmp[i + j * ps.n1] = sv;
+        if (ps.diag) ps.tmp[j + i * ps.n1] = sv;
+    } else {
+        const int64_t r = dev_index(L, ps.ua0 + i), c = dev_index(L, ps.ub0 + j);
+        H[r + c * L.ld] = sv;
+        H[c + r * L.ld] = sv;
+    }
+    const int64_t cs = ps.nc * ps.n2, o = il * ps.n2 + j;
+    if (DA) {
+        ps.W12[o] = a.x;
+        ps.W12[o + cs] = a.y;
+        ps.W12[o + 2 * cs] = a.z;
+    }
+    if (DB) {
+        ps.W21[o] = b.x;
+        ps.W21[o + cs] = b.y;
+        ps.W21[o + 2 * cs] = b.z;
+    }
+}
+
+// Far pair values with the area factors of s_far / d_far (operators.h:103-106).
+template <bool DA, bool DB>
+__device__ __forceinline__ void pair_far_values(const TriRec& T, const TriRec& U, const double* gt, const double* gu,
+                                                int fc, double coeffS, double& sv, d3& a, d3& b) {
+    double s;
+    pair_far<DA, DB>(gt, gu, ld3(T.n), ld3(U.n), fc, s, a, b);
+    sv = (U.area * (T.area * s)) * coeffS;
+    a = mk(T.area * (U.area * a.x), T.area * (U.area * a.y), T.area * (U.area * a.z));
+    b = mk(U.area * (T.area * b.x), U.area * (T.area * b.y), U.area * (T.area * b.z));
+}
+
+// Pass 1: one thread per pair, 16 x 16 pairs per block (a wave covers 8 x 8).
+#ifndef OMH_PAIR_FAR_WAVES
+#define OMH_PAIR_FAR_WAVES 4
+#endif
+template <bool DA, bool DB>
+__global__ void __launch_bounds__(256, OMH_PAIR_FAR_WAVES) k_pairs_far(const TriRec* __restrict__ tri, const double* __restrict__ gp,
+                                                   PairStripe ps, double* __restrict__ H, Layout L) {
+    const int64_t ntj = cdiv_dev(ps.n2, 16);
+    const int64_t ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
+    if (ps.diag && tj * 16 + 15 < ps.c0 + ti * 16) return;   // the whole tile lies below the diagonal
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t il = ti * 16 + (lane >> 3) + 8 * (w >> 1);
+    const int64_t j = tj * 16 + (lane & 7) + 8 * (w & 1);
+    const int64_t i = ps.c0 + il;
+    const bool valid = il < ps.nc && j < ps.n2 && !(ps.diag && j < i);
+    int fc = 2;
+    if (valid)
+        fc = (ps.diag && i == j) ? 0
+                                 : far_class(gp + (ps.a0 + i) * GP_STRIDE, gp + (ps.b0 + j) * GP_STRIDE,
+                                             fmax(tri[ps.a0 + i].diam, tri[ps.b0 + j].diam));
+    // queue the other classes: one atomic per wave and class
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool mine = valid && fc == k;
+        const unsigned long long m = __ballot(mine);
+        if (m) {
+            const int leader = __builtin_ctzll(m);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&ps.qn[k], (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (mine) {
+                const int64_t pos = (int64_t)base + __popcll(m & below);
+                ps.q[k == 0 ? pos : ps.qcap - 1 - pos] = il * ps.n2 + j;
+            }
+        }
+    }
+    if (!valid || fc != 2) return;
+    const TriRec& T = tri[ps.a0 + i];
+    const TriRec& U = tri[ps.b0 + j];
+    double sv;
+    d3 a, b;
+    pair_far_values<DA, DB>(T, U, gp + (ps.a0 + i) * GP_STRIDE, gp + (ps.b0 + j) * GP_STRIDE, 2, ps.coeffS, sv, a, b);
+    pair_store<DA, DB>(ps, H, L, i, il, j, sv, a, b);
+}
+
+// Pass 2 over the queue: near pairs with the analytic integrals of k_s_block
+// and k_d_pairs (the same arithmetic), 16-point pairs with pair_far.
+template <bool DA, bool DB>
+__global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_pairs_fix(const TriRec* __restrict__ tri, const double* __restrict__ gp,
+                                                   PairStripe ps, double* __restrict__ H, Layout L) {
+    const int64_t nn = ps.qn[0], nf = ps.qn[1];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn + nf;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t code = e < nn ? ps.q[e] : ps.q[ps.qcap - 1 - (e - nn)];
+        const int64_t il = code / ps.n2, j = code - il * ps.n2, i = ps.c0 + il;
+        const TriRec& T = tri[ps.a0 + i];
+        const TriRec& U = tri[ps.b0 + j];
+        double sv;
+        d3 a = mk(0.0, 0.0, 0.0), b = mk(0.0, 0.0, 0.0);
+        if (e >= nn) {
+            pair_far_values<DA, DB>(T, U, gp + (ps.a0 + i) * GP_STRIDE, gp + (ps.b0 + j) * GP_STRIDE, 1, ps.coeffS,
+                                    sv, a, b);
+        } else {
+            constexpr int order = 3;
+            const int np = c_nb_pts[order];
+            {   // S: OperatorS of t at the Gauss points of u
+                const OpS o = load_ops(T);
+                const d3 P[3] = {ld3(U.p[0]), ld3(U.p[1]), ld3(U.p[2])};
+                double res = 0.0;
+                for (int k = 0; k < np; ++k) res += c_rules[order][k][3] * ops_eval(o, gauss_point(order, k, P));
+                sv = (U.area * res) * ps.coeffS;
+            }
+            if (DA) {   // D(t, u): P1 double layer of u at the Gauss points of t
+                const P1S o = load_p1s(U);
+                const d3 P[3] = {ld3(T.p[0]), ld3(T.p[1]), ld3(T.p[2])};
+                double rx = 0.0, ry = 0.0, rz = 0.0;
+                for (int k = 0; k < np; ++k) {
+                    const double wk = c_rules[order][k][3];
+                    const d3 v = p1s_eval(o, gauss_point(order, k, P));
+                    rx += wk * v.x;
+                    ry += wk * v.y;
+                    rz += wk * v.z;
+                }
+                a = mk(T.area * rx, T.area * ry, T.area * rz);
+            }
+            if (DB && !(ps.diag && i == j)) {   // D(u, t)
+                const P1S o = load_p1s(T);
+                const d3 P[3] = {ld3(U.p[0]), ld3(U.p[1]), ld3(U.p[2])};
+                double rx = 0.0, ry = 0.0, rz = 0.0;
+                for (int k = 0; k < np; ++k) {
+                    const double wk = c_rules[order][k][3];
+                    const d3 v = p1s_eval(o, gauss_point(order, k, P));
+                    rx += wk * v.x;
+                    ry += wk * v.y;
+                    rz += wk * v.z;
+                }
+                b = mk(U.area * rx, U.area * ry, U.area * rz);
+            }
+        }
+        pair_store<DA, DB>(ps, H, L, i, il, j, sv, a, b);
+    }
+}
+
+// Gather of D(t, u) for the stripe's rows t: H(t, vertex of slot s of B)
+// += coeff sum over the triangles u of s (mesh order; u >= t when A == B)
+// of W12(k, t - c0, u).  Thread = slot (x), row (block y).
+__global__ void __launch_bounds__(256) k_gather12(const double* __restrict__ W12, PairStripe ps,
+                                                  const int64_t* __restrict__ adj_off, const AdjRec* __restrict__ adj,
+                                                  const int32_t* __restrict__ slot_vid,
+                                                  const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
+                                                  double coeff, double* __restrict__ H, Layout L) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, il = blockIdx.y;
+    if (s >= ns) return;
+    const int64_t i = ps.c0 + il;
+    const int64_t r = dev_index(L, ps.ua0 + i), c = dev_index(L, vert_unk[slot_vid[s0 + s]]);
+    const int64_t cs = ps.nc * ps.n2;
+    double acc = H[c + r * L.ld];
+    for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
+        const AdjRec e = adj[p];
+        const int64_t u = e.tri - ps.b0;
+        if (ps.diag && u < i) continue;
+        acc += W12[e.k * cs + il * ps.n2 + u] * coeff;
+    }
+    H[c + r * L.ld] = acc;
+    H[r + c * L.ld] = acc;
+}
+
+// Gather of D(u, t) for the stripe's columns t: H(row triangle u of B,
+// vertex of slot s of A) += coeff sum over the triangles t of s inside the
+// stripe (t < u when A == B) of W21(k, t - c0, u).  Block = slot of A,
+// threads over the rows u; slots without a triangle in the stripe return.
+__global__ void __launch_bounds__(256) k_gather21(const double* __restrict__ W21, PairStripe ps,
+                                                  const int64_t* __restrict__ adj_off, const AdjRec* __restrict__ adj,
+                                                  const int32_t* __restrict__ slot_vid,
+                                                  const int64_t* __restrict__ vert_unk, int64_t s0, double coeff,
+                                                  double* __restrict__ H, Layout L) {
+    const int64_t s = blockIdx.x;
+    const int64_t p0 = adj_off[s0 + s], p1 = adj_off[s0 + s + 1];
+    bool any = false;
+    for (int64_t p = p0; p < p1; ++p) {
+        const int64_t tl = adj[p].tri - ps.a0 - ps.c0;
+        any |= tl >= 0 && tl < ps.nc;
+    }
+    if (!any) return;
+    const int64_t c = dev_index(L, vert_unk[slot_vid[s0 + s]]);
+    const int64_t cs = ps.nc * ps.n2;
+    for (int64_t u = threadIdx.x; u < ps.n2; u += blockDim.x) {
+        const int64_t r = dev_index(L, ps.ub0 + u);
+        double acc = H[r + c * L.ld];
+        for (int64_t p = p0; p < p1; ++p) {
+            const AdjRec e = adj[p];
+            const int64_t tl = e.tri - ps.a0 - ps.c0;
+            if (tl < 0 || tl >= ps.nc || (ps.diag && ps.c0 + tl >= u)) continue;
+            acc += W21[e.k * cs + tl * ps.n2 + u] * coeff;
+        }
+        H[r + c * L.ld] = acc;
+        H[c + r * L.ld] = acc;
+    }
+}
+
+__global__ void k_zero_counts(unsigned* qn) {
+    if (threadIdx.x < 2) qn[threadIdx.x] = 0u;
+}
+
 // N block (operators.h:134-148 and the loops of :272-296 / :455-480):
 //   N(v1, v2) = -f coeff sum_{t1 ~ v1} sum_{t2 ~ v2} (cb1 . cb2) S(t1, t2) / (A1 A2)
 // (f = 1/4, or 1/2 for a vertex shared by two different meshes), evaluated
@@ -602,16 +836,104 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
     }
 }
 
+// Far-field assembly of the S, D and D* blocks of one mesh pair (m1 rows,
+// m2 columns): S with coefficient coeffS into the HeadMat, or into tmp
+// (n1 x n2) for a barrier pair; D(m1, m2) (da) and D(m2, m1) (db) with
+// coefficient cD.  m1 == m2: da and db together make the diagonal D block.
+template <bool DA, bool DB>
+static void launch_pairs_far_t(DeviceGeometry& dg, int m1, int m2, double coeffS, double* tmp, double cD, double* H,
+                               const Layout& L, hipStream_t st) {
+    const MeshDev& A = dg.meshes[m1];
+    const MeshDev& B = dg.meshes[m2];
+    const int64_t n1 = A.tri_end - A.tri_begin, n2 = B.tri_end - B.tri_begin;
+    if (n1 <= 0 || n2 <= 0) return;
+    const int diag = (m1 == m2);
+    const size_t budget = size_t(1) << 30;   // bytes per W array
+    int64_t nc = (int64_t)(budget / (3 * sizeof(double) * (size_t)n2));
+    nc = std::max<int64_t>(16, std::min<int64_t>({nc, n1, 32768}));
+    if (const char* env = std::getenv("OMH_PAIR_STRIPE")) {   // tests run the striped path on small meshes
+        const long long v = std::atoll(env);
+        if (v > 0) nc = std::min<int64_t>(nc, (int64_t)v);
+    }
+    const size_t wsz = 3 * (size_t)nc * (size_t)n2;
+    double* W = (DA || DB) ? (double*)scratch(SCR_DTMP, sizeof(double) * 2 * wsz) : nullptr;
+    const int64_t qcap = nc * n2;
+    char* qb = (char*)scratch(SCR_PAIRQ, sizeof(int64_t) * (size_t)qcap + 256);
+    PairStripe ps{};
+    ps.a0 = A.tri_begin;
+    ps.n1 = n1;
+    ps.b0 = B.tri_begin;
+    ps.n2 = n2;
+    ps.diag = diag;
+    ps.temp = tmp != nullptr;
+    ps.coeffS = coeffS;
+    ps.ua0 = A.tri_unk0;
+    ps.ub0 = B.tri_unk0;
+    ps.tmp = tmp;
+    ps.W12 = W;
+    ps.W21 = W ? W + wsz : nullptr;
+    ps.qn = (unsigned*)qb;
+    ps.q = (int64_t*)(qb + 256);
+    ps.qcap = qcap;
+    const int64_t ns1 = A.slot_end - A.slot_begin, ns2 = B.slot_end - B.slot_begin;
+    for (int64_t c0 = 0; c0 < n1; c0 += nc) {
+        ps.c0 = c0;
+        ps.nc = std::min(nc, n1 - c0);
+        const double pairs = diag ? (double)ps.nc * (double)(n2 - c0) - 0.5 * (double)ps.nc * (double)(ps.nc - 1)
+                                  : (double)ps.nc * (double)n2;
+        ProfScope prof("pairs_far", pairs, st);
+        k_zero_counts<<<1, 64, 0, st>>>(ps.qn);
+        OMH_LAUNCH_CHECK();
+        k_pairs_far<DA, DB><<<(unsigned)(cdiv(ps.nc, 16) * cdiv(n2, 16)), 256, 0, st>>>(dg.tri, dg.gp, ps, H, L);
+        OMH_LAUNCH_CHECK();
+        k_pairs_fix<DA, DB><<<2048, 256, 0, st>>>(dg.tri, dg.gp, ps, H, L);
+        OMH_LAUNCH_CHECK();
+        if (DA) {
+            k_gather12<<<dim3((unsigned)cdiv(ns2, 256), (unsigned)ps.nc), 256, 0, st>>>(
+                ps.W12, ps, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, B.slot_begin, ns2, cD, H, L);
+            OMH_LAUNCH_CHECK();
+        }
+        if (DB && ns1 > 0) {
+            k_gather21<<<(unsigned)ns1, 256, 0, st>>>(ps.W21, ps, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk,
+                                                       A.slot_begin, cD, H, L);
+            OMH_LAUNCH_CHECK();
+        }
+    }
+}
+
+static void launch_pairs_far(DeviceGeometry& dg, int m1, int m2, double coeffS, double* tmp, bool da, bool db,
+                             double cD, double* H, const Layout& L, hipStream_t st) {
+    if (da && db) launch_pairs_far_t<true, true>(dg, m1, m2, coeffS, tmp, cD, H, L, st);
+    else if (da) launch_pairs_far_t<true, false>(dg, m1, m2, coeffS, tmp, cD, H, L, st);
+    else if (db) launch_pairs_far_t<false, true>(dg, m1, m2, coeffS, tmp, cD, H, L, st);
+    else launch_pairs_far_t<false, false>(dg, m1, m2, coeffS, tmp, cD, H, L, st);
+}
+
 void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field) {
     const int far = far_field ? 1 : 0;
     if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
     DeviceGeometry& dg = device_geometry(g);
     const int64_t n = g.size();
     fill_f64(H, n, n, L.ld, 0.0, st);
+    // OMH_FAR_SEPARATE=1: the far field through the per-operator kernels
+    // (tests compare the fused pass against them)
+    const char* sep = std::getenv("OMH_FAR_SEPARATE");
+    const bool fused = far && order == 3 && !(sep && std::atoi(sep) != 0);
     for (const MeshPair& mp : g.pairs) {
         const int m1 = mp.m1, m2 = mp.m2;
         const bool b1 = g.meshes[m1].current_barrier, b2 = g.meshes[m2].current_barrier;
         const bool same = (m1 == m2);
+        if (fused) {   // S, D and D* of the pair in one pass, then N from S
+            const bool inH = (same ? !b1 : (!b1 && !b2)) && mp.cS != 0.0;
+            double* tmp = nullptr;
+            if (!inH) {
+                const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
+                tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+            }
+            launch_pairs_far(dg, m1, m2, inH ? mp.cS : 1.0, tmp, !b1, same ? !b1 : !b2, mp.cD, H, L, st);
+            launch_n(dg, g, m1, m2, inH ? mp.cN / mp.cS : mp.cN, tmp, H, L, st);
+            continue;
+        }
         double Scoeff = 0.0;
         if (same ? !b1 : (!b1 && !b2)) {   // set_S_block
             launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, far, st);

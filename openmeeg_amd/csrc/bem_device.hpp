@@ -267,4 +267,68 @@ __device__ __forceinline__ d3 d_far(const double* __restrict__ go, const double*
     return fc == 2 ? d_far_n<7, 2, GP7, 7, 2, GP7>(go, gi, n) : d_far_n<16, 3, GP16, 16, 3, GP16>(go, gi, n);
 }
 
+// One far-field triangle pair (t, u) for all three pair operators from a
+// single set of Gauss-point distances.  With x_p the points of t and y_q
+// those of u (NP-point rule R, table offset OFF), d = y_q - x_p:
+//   s   = sum_p w_p sum_q w_q / |d|                          (single layer)
+//   a_c = sum_p w_p sum_q w_q l_qc (n_u . d) / |d|^3         (P1 double layer of u, t outer)
+//   b_c = -sum_p w_p l_pc sum_q w_q (n_t . d) / |d|^3        (P1 double layer of t, u outer)
+// (l: barycentric coordinates of the rule's points).  s times the two areas
+// is s_far's pair value, a and b those of d_far for (t, u) and (u, t); a
+// (b) is left zero unless DA (DB).
+template <int NP, int R, int OFF, bool DA, bool DB>
+__device__ __forceinline__ void pair_far_n(const double* __restrict__ gt, const double* __restrict__ gu, d3 nt, d3 nu,
+                                           double& s, d3& a, d3& b) {
+    d3 y[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) y[q] = ld3(gu + OFF + 3 * q);
+    s = 0.0;
+    a = mk(0.0, 0.0, 0.0);
+    b = mk(0.0, 0.0, 0.0);
+#pragma unroll 1
+    for (int p = 0; p < NP; ++p) {
+        const d3 x = ld3(gt + OFF + 3 * p);
+        double sp = 0.0, ax = 0.0, ay = 0.0, az = 0.0, tp = 0.0;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const d3 d = sub(y[q], x);
+            const double ir = rsqrt(dot(d, d));
+            const double wq = c_rules[R][q][3];
+            sp += wq * ir;
+            if (DA || DB) {
+                const double e = wq * (ir * ir * ir);
+                if (DA) {
+                    const double g = dot(nu, d) * e;
+                    ax += g * c_rules[R][q][0];
+                    ay += g * c_rules[R][q][1];
+                    az += g * c_rules[R][q][2];
+                }
+                if (DB) tp += dot(nt, d) * e;
+            }
+        }
+        const double wp = c_rules[R][p][3];
+        s += wp * sp;
+        if (DA) {
+            a.x += wp * ax;
+            a.y += wp * ay;
+            a.z += wp * az;
+        }
+        if (DB) {
+            const double wt = wp * tp;
+            b.x -= wt * c_rules[R][p][0];
+            b.y -= wt * c_rules[R][p][1];
+            b.z -= wt * c_rules[R][p][2];
+        }
+    }
+}
+
+// Far classes of far_class: 1 (16-point rules), 2 (7-point rules); DA / DB
+// select the double-layer values computed.
+template <bool DA, bool DB>
+__device__ __forceinline__ void pair_far(const double* __restrict__ gt, const double* __restrict__ gu, d3 nt, d3 nu,
+                                         int fc, double& s, d3& a, d3& b) {
+    if (fc == 2) pair_far_n<7, 2, GP7, DA, DB>(gt, gu, nt, nu, s, a, b);
+    else pair_far_n<16, 3, GP16, DA, DB>(gt, gu, nt, nu, s, a, b);
+}
+
 }  // namespace omh

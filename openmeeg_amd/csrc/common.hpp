@@ -28,7 +28,16 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Grow-only device scratch buffer per slot and per device.  Kernels that use
 // it run on the caller's stream; growing synchronises the device first so
 // that no in-flight kernel still uses the old buffer.
-enum ScratchSlot { SCR_ASSEMBLY = 0, SCR_DTMP = 1, SCR_SOURCES = 2, SCR_LINALG = 3, SCR_LINALG2 = 4, SCR_SMALL = 5, SCR_NSLOTS = 6 };
+enum ScratchSlot {
+    SCR_ASSEMBLY = 0,
+    SCR_DTMP = 1,
+    SCR_SOURCES = 2,
+    SCR_LINALG = 3,
+    SCR_LINALG2 = 4,
+    SCR_SMALL = 5,
+    SCR_PAIRQ = 6,
+    SCR_NSLOTS = 7
+};
 void* scratch(ScratchSlot slot, size_t bytes);
 void release_scratch();
 

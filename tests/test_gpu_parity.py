@@ -354,6 +354,37 @@ def test_headmat_far_field_close_to_analytic(om, c0_oracle):
     assert changed > 0.5            # most pairs are far
 
 
+@pytest.mark.parametrize("layers", [3, 4])
+def test_far_field_fused_pass(om, monkeypatch, layers):
+    """The fused far-field pass (S, D and D* of a mesh pair from one set of
+    Gauss-point distances, near and 16-point pairs through a queue) against
+    the per-operator far-field kernels: the same integrals summed in another
+    order (1e-13 of the largest entry).  In stripes of 37 row triangles (the
+    vertex gathers then add partial sums over several stripes) the same to
+    1e-13.  3 layers of 642 vertices with a barrier scalp; 4 layers of 362."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device
+    if layers == 3:
+        g = om.make_nested_geometry(S.nested_sphere_meshes(8, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    else:
+        g = om.make_nested_geometry(S.nested_sphere_meshes(6, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    Hf = _head_mat_device(g, far_field=True).view(n, n)
+    monkeypatch.setenv("OMH_FAR_SEPARATE", "1")
+    Hs = _head_mat_device(g, far_field=True).view(n, n)
+    monkeypatch.delenv("OMH_FAR_SEPARATE")
+    monkeypatch.setenv("OMH_PAIR_STRIPE", "37")
+    Hc = _head_mat_device(g, far_field=True).view(n, n)
+    monkeypatch.delenv("OMH_PAIR_STRIPE")
+    scale = float(Hs.abs().max())
+    d_sep = float((Hf - Hs).abs().max()) / scale
+    d_str = float((Hc - Hf).abs().max()) / scale
+    print("n=%d fused vs separate kernels %.3e, striped vs one stripe %.3e" % (n, d_sep, d_str), flush=True)
+    assert d_sep < 1e-13 and d_str < 1e-13
+    assert torch.equal(Hf, Hf.t()) and torch.equal(Hc, Hc.t())
+
+
 def test_gain_c0_vs_analytic_sphere(om, c0_oracle):
     """Independent of the reference: the BEM gain on the 642-vertex spheres
     against Zhang's series (RDM/MAG, tests/analytic tolerances for Head3:
