@@ -404,10 +404,13 @@ __global__ void __launch_bounds__(256) k_d_gather(const double* __restrict__ W, 
 // class go to a queue (near pairs from the front, 16-point pairs from the
 // back) for a second kernel, so that the waves of the first one do not
 // diverge over the classes.  The double-layer values go to two scratch
-// arrays, W12(k, t - c0, u) = D_k(t, u) and W21(k, t - c0, u) = D_k(u, t)
-// (3 x nc x n2 each, column fastest), gathered per vertex afterwards.
+// arrays, W12[(k n2 + u) nc + t - c0] = D_k(t, u) (row fastest) and
+// W21[(k nc + t - c0) n2 + u] = D_k(u, t) (column fastest), so that both
+// per-vertex gathers read them along the rows of the HeadMat they write.
 // When A == B only pairs u >= t are evaluated (the gathers take D(t, u) for
-// u >= t from W12 and for u < t from W21).
+// u >= t from W12 and for u < t from W21).  The gathers write the (triangle
+// row, vertex column) half of the D blocks only; k_mirror_tv copies it to
+// the other half once all pairs are done.
 // ---------------------------------------------------------------------------
 struct PairStripe {
     int64_t a0, c0, nc, n1;   // A's first triangle, stripe start and size, A's triangle count
@@ -437,13 +440,15 @@ __device__ __forceinline__ void pair_store(const PairStripe& ps, double* __restr
         H[r + c * L.ld] = sv;
         H[c + r * L.ld] = sv;
     }
-    const int64_t cs = ps.nc * ps.n2, o = il * ps.n2 + j;
+    const int64_t cs = ps.nc * ps.n2;
     if (DA) {
+        const int64_t o = j * ps.nc + il;
         ps.W12[o] = a.x;
         ps.W12[o + cs] = a.y;
         ps.W12[o + 2 * cs] = a.z;
     }
     if (DB) {
+        const int64_t o = il * ps.n2 + j;
         ps.W21[o] = b.x;
         ps.W21[o + cs] = b.y;
         ps.W21[o + 2 * cs] = b.z;
@@ -567,25 +572,27 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_pairs_fix(const Tri
 
 // Gather of D(t, u) for the stripe's rows t: H(t, vertex of slot s of B)
 // += coeff sum over the triangles u of s (mesh order; u >= t when A == B)
-// of W12(k, t - c0, u).  Thread = slot (x), row (block y).
+// of W12(k, t - c0, u).  Thread = row (64 per block), slot (4 per block).
 __global__ void __launch_bounds__(256) k_gather12(const double* __restrict__ W12, PairStripe ps,
                                                   const int64_t* __restrict__ adj_off, const AdjRec* __restrict__ adj,
                                                   const int32_t* __restrict__ slot_vid,
                                                   const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
                                                   double coeff, double* __restrict__ H, Layout L) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, il = blockIdx.y;
-    if (s >= ns) return;
+    const int64_t ni = cdiv_dev(ps.nc, 64);
+    const int64_t ts = blockIdx.x / ni, tb = blockIdx.x - ts * ni;
+    const int64_t il = tb * 64 + threadIdx.x % 64;
+    const int64_t s = ts * 4 + threadIdx.x / 64;
+    if (il >= ps.nc || s >= ns) return;
     const int64_t i = ps.c0 + il;
     const int64_t r = dev_index(L, ps.ua0 + i), c = dev_index(L, vert_unk[slot_vid[s0 + s]]);
     const int64_t cs = ps.nc * ps.n2;
-    double acc = H[c + r * L.ld];
+    double acc = H[r + c * L.ld];
     for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
         const AdjRec e = adj[p];
         const int64_t u = e.tri - ps.b0;
         if (ps.diag && u < i) continue;
-        acc += W12[e.k * cs + il * ps.n2 + u] * coeff;
+        acc += W12[e.k * cs + u * ps.nc + il] * coeff;
     }
-    H[c + r * L.ld] = acc;
     H[r + c * L.ld] = acc;
 }
 
@@ -618,7 +625,25 @@ __global__ void __launch_bounds__(256) k_gather21(const double* __restrict__ W21
             acc += W21[e.k * cs + tl * ps.n2 + u] * coeff;
         }
         H[r + c * L.ld] = acc;
-        H[c + r * L.ld] = acc;
+    }
+}
+
+// H(v, t) = H(t, v) for the device rows [t0, t0 + nt) and columns
+// [v0, v0 + nv): 64 x 64 tiles through LDS, both sides coalesced.
+__global__ void __launch_bounds__(256) k_mirror_tv(double* __restrict__ H, int64_t ld, int64_t t0, int64_t nt,
+                                                   int64_t v0, int64_t nv) {
+    __shared__ double tile[64][65];
+    const int64_t ntt = cdiv_dev(nt, 64);
+    const int64_t bv = blockIdx.x / ntt, bt = blockIdx.x - bv * ntt;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int k = ty; k < 64; k += 4) {   // rows t along tx, columns v along k
+        const int64_t t = bt * 64 + tx, v = bv * 64 + k;
+        if (t < nt && v < nv) tile[k][tx] = H[(t0 + t) + (v0 + v) * ld];
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {   // rows v along tx, columns t along k
+        const int64_t v = bv * 64 + tx, t = bt * 64 + k;
+        if (t < nt && v < nv) H[(v0 + v) + (t0 + t) * ld] = tile[tx][k];
     }
 }
 
@@ -888,8 +913,8 @@ static void launch_pairs_far_t(DeviceGeometry& dg, int m1, int m2, double coeffS
         OMH_LAUNCH_CHECK();
         k_pairs_fix<DA, DB><<<2048, 256, 0, st>>>(dg.tri, dg.gp, ps, H, L);
         OMH_LAUNCH_CHECK();
-        if (DA) {
-            k_gather12<<<dim3((unsigned)cdiv(ns2, 256), (unsigned)ps.nc), 256, 0, st>>>(
+        if (DA && ns2 > 0) {
+            k_gather12<<<(unsigned)(cdiv(ps.nc, 64) * cdiv(ns2, 4)), 256, 0, st>>>(
                 ps.W12, ps, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, B.slot_begin, ns2, cD, H, L);
             OMH_LAUNCH_CHECK();
         }
@@ -949,6 +974,14 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         }
         if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, far, st);              // set_D_block
         if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, far, st);     // set_Dstar_block
+    }
+    if (fused) {   // the D blocks' other half
+        const int64_t nt = L.nt, nv = L.nv;
+        const int64_t t0 = L.tri_first ? 0 : nv, v0 = L.tri_first ? nt : 0;
+        if (nt > 0 && nv > 0) {
+            k_mirror_tv<<<(unsigned)(cdiv(nt, 64) * cdiv(nv, 64)), 256, 0, st>>>(H, L.ld, t0, nt, v0, nv);
+            OMH_LAUNCH_CHECK();
+        }
     }
     // deflate all current barriers, one constant per isolated part
     double* coef = (double*)scratch(SCR_SMALL, 64);
