@@ -468,7 +468,7 @@ __device__ __forceinline__ void pair_far_values(const TriRec& T, const TriRec& U
 
 // Pass 1: one thread per pair, 16 x 16 pairs per block (a wave covers 8 x 8).
 #ifndef OMH_PAIR_FAR_WAVES
-#define OMH_PAIR_FAR_WAVES 4
+#define OMH_PAIR_FAR_WAVES 3
 #endif
 template <bool DA, bool DB>
 __global__ void __launch_bounds__(256, OMH_PAIR_FAR_WAVES) k_pairs_far(const TriRec* __restrict__ tri, const double* __restrict__ gp,

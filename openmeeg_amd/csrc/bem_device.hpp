@@ -267,57 +267,81 @@ __device__ __forceinline__ d3 d_far(const double* __restrict__ go, const double*
     return fc == 2 ? d_far_n<7, 2, GP7, 7, 2, GP7>(go, gi, n) : d_far_n<16, 3, GP16, 16, 3, GP16>(go, gi, n);
 }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate and one Newton step
+// with its cubic term (y = y0 + y0 e (1/2 + 3/8 e), e = 1 - x y0^2).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
 // One far-field triangle pair (t, u) for all three pair operators from a
 // single set of Gauss-point distances.  With x_p the points of t and y_q
 // those of u (NP-point rule R, table offset OFF), d = y_q - x_p:
 //   s   = sum_p w_p sum_q w_q / |d|                          (single layer)
-//   a_c = sum_p w_p sum_q w_q l_qc (n_u . d) / |d|^3         (P1 double layer of u, t outer)
+//   a_c = sum_q w_q l_qc sum_p w_p (n_u . d) / |d|^3         (P1 double layer of u, t outer)
 //   b_c = -sum_p w_p l_pc sum_q w_q (n_t . d) / |d|^3        (P1 double layer of t, u outer)
 // (l: barycentric coordinates of the rule's points).  s times the two areas
 // is s_far's pair value, a and b those of d_far for (t, u) and (u, t); a
-// (b) is left zero unless DA (DB).
+// (b) is left zero unless DA (DB).  n . d is taken as n . y_q - n . x_p
+// (far pairs: |d| is at least 4 triangle diameters, the difference loses
+// no more than a few units of the last place relative to |n . d| + |d|).
+// PRE: keep n . y_q of both normals in registers (the 7-point rule; the
+// 16-point rule recomputes them, for its register budget).
 template <int NP, int R, int OFF, bool DA, bool DB>
 __device__ __forceinline__ void pair_far_n(const double* __restrict__ gt, const double* __restrict__ gu, d3 nt, d3 nu,
                                            double& s, d3& a, d3& b) {
+    constexpr bool PRE = NP <= 8;
+    constexpr int NPRE = PRE ? NP : 1;
     d3 y[NP];
+    double uy[NPRE], ty[NPRE], G[NP];
 #pragma unroll
-    for (int q = 0; q < NP; ++q) y[q] = ld3(gu + OFF + 3 * q);
+    for (int q = 0; q < NP; ++q) {
+        y[q] = ld3(gu + OFF + 3 * q);
+        if (PRE) {
+            uy[q % NPRE] = dot(nu, y[q]);
+            ty[q % NPRE] = dot(nt, y[q]);
+        }
+        G[q] = 0.0;
+    }
     s = 0.0;
-    a = mk(0.0, 0.0, 0.0);
     b = mk(0.0, 0.0, 0.0);
 #pragma unroll 1
     for (int p = 0; p < NP; ++p) {
         const d3 x = ld3(gt + OFF + 3 * p);
-        double sp = 0.0, ax = 0.0, ay = 0.0, az = 0.0, tp = 0.0;
+        const double ux = dot(nu, x), tx = dot(nt, x);
+        const double wp = c_rules[R][p][3];
+        double sp = 0.0, tp = 0.0;
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
             const d3 d = sub(y[q], x);
-            const double ir = rsqrt(dot(d, d));
+            const double ir = rsqrt_pos(dot(d, d));
             const double wq = c_rules[R][q][3];
-            sp += wq * ir;
+            sp = fma(wq, ir, sp);
             if (DA || DB) {
-                const double e = wq * (ir * ir * ir);
-                if (DA) {
-                    const double g = dot(nu, d) * e;
-                    ax += g * c_rules[R][q][0];
-                    ay += g * c_rules[R][q][1];
-                    az += g * c_rules[R][q][2];
-                }
-                if (DB) tp += dot(nt, d) * e;
+                const double ir3 = ir * ir * ir;
+                const double nud = PRE ? uy[q % NPRE] - ux : dot(nu, d);
+                const double ntd = PRE ? ty[q % NPRE] - tx : dot(nt, d);
+                if (DA) G[q] = fma(wp, nud * ir3, G[q]);
+                if (DB) tp = fma(ntd, wq * ir3, tp);
             }
         }
-        const double wp = c_rules[R][p][3];
-        s += wp * sp;
-        if (DA) {
-            a.x += wp * ax;
-            a.y += wp * ay;
-            a.z += wp * az;
-        }
+        s = fma(wp, sp, s);
         if (DB) {
             const double wt = wp * tp;
             b.x -= wt * c_rules[R][p][0];
             b.y -= wt * c_rules[R][p][1];
             b.z -= wt * c_rules[R][p][2];
+        }
+    }
+    a = mk(0.0, 0.0, 0.0);
+    if (DA) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const double g = c_rules[R][q][3] * G[q];
+            a.x = fma(g, c_rules[R][q][0], a.x);
+            a.y = fma(g, c_rules[R][q][1], a.y);
+            a.z = fma(g, c_rules[R][q][2], a.z);
         }
     }
 }
