@@ -1737,6 +1737,107 @@ template int block_fill_violated<double>(const double*, int64_t, int64_t, const 
 #endif
 constexpr int SOLVE_GROUP = OMH_SOLVE_GROUP;
 
+namespace {
+
+// acc[c] = sum_k A(i, k) X[k][c] for the workgroup's thread i < m, k < kk,
+// A(i, k) = A[i + k lda] (or A[k + i lda] when TRANS), X in LDS (read as a
+// broadcast).  A is read straight from memory, SKU values of k in flight per
+// thread (one workgroup per CU runs this, so the loads in flight hide the
+// cache latency): coalesced across the threads, or (TRANS) 16-byte loads
+// along each thread's own column when the column is aligned for them.
+constexpr int SKU = 32;
+template <typename T, bool TRANS, int SV>
+__device__ __forceinline__ void group_product(const T* __restrict__ A, int64_t lda, int m, int kk, const T (*X)[SV],
+                                              T acc[SV]) {
+    constexpr int VW = 16 / sizeof(T);   // elements per 16-byte load
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < SV; ++c) acc[c] = T(0);
+    if (tid >= m) return;
+    const T* a0 = TRANS ? A + (int64_t)tid * lda : A + tid;
+    const int64_t ks = TRANS ? 1 : lda;
+    const bool vload = TRANS && (lda % VW) == 0 && (reinterpret_cast<uintptr_t>(A) % 16) == 0;
+    int k = 0;
+    for (; k + SKU <= kk; k += SKU) {
+        T a[SKU];
+        if (vload) {
+#pragma unroll
+            for (int u = 0; u < SKU; u += VW) {
+                const vec_t v = *reinterpret_cast<const vec_t*>(a0 + k + u);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) a[u + e] = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < SKU; ++u) a[u] = a0[(int64_t)(k + u) * ks];
+        }
+#pragma unroll
+        for (int u = 0; u < SKU; ++u)
+#pragma unroll
+            for (int c = 0; c < SV; ++c) acc[c] = fma(a[u], X[k + u][c], acc[c]);
+    }
+    for (; k < kk; ++k) {
+        const T a = a0[(int64_t)k * ks];
+#pragma unroll
+        for (int c = 0; c < SV; ++c) acc[c] = fma(a, X[k][c], acc[c]);
+    }
+}
+
+// Triangular solve inside one group of diagonal blocks (rows [k0, gend))
+// for SV right-hand sides per workgroup (a narrow strip, so that a few
+// hundred right-hand sides still spread over many CUs), the group's rows of
+// W in LDS:
+// forward, block by block, W_b <- Linv_b W_b, then W_b' -= L_b'b W_b for
+// the group's later blocks; backward (BACK) W_b <- Linv_b^T W_b, then
+// W_b' -= L_bb'^T W_b for its earlier blocks.
+template <typename T, bool BACK, int SV>
+__global__ void __launch_bounds__(256) k_solve_group(const T* __restrict__ L, int64_t lda, const T* __restrict__ Dinv,
+                                                     int64_t k0, int64_t gend, T* __restrict__ W, int64_t ldw,
+                                                     int64_t r) {
+    __shared__ T Wg[SOLVE_GROUP * FACT_NB][SV];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * SV;
+    const int nc = (int)min<int64_t>(SV, r - c0);
+    const int g = (int)(gend - k0);
+    for (int e = tid; e < g * SV; e += 256) {
+        const int i = e % g, c = e / g;
+        Wg[i][c] = c < nc ? W[k0 + i + (c0 + c) * ldw] : T(0);
+    }
+    __syncthreads();
+    const int nbk = (g + FACT_NB - 1) / FACT_NB;
+    T acc[SV];
+    for (int bi = 0; bi < nbk; ++bi) {
+        const int b = BACK ? nbk - 1 - bi : bi;
+        const int64_t kb = k0 + (int64_t)b * FACT_NB;
+        const int nb = (int)min<int64_t>(FACT_NB, gend - kb);
+        const T* Li = Dinv + (size_t)(kb / FACT_NB) * FACT_NB * FACT_NB;
+        group_product<T, BACK, SV>(Li, FACT_NB, nb, nb, &Wg[b * FACT_NB], acc);
+        __syncthreads();   // every thread has read W_b
+        if (tid < nb) {
+#pragma unroll
+            for (int c = 0; c < SV; ++c) Wg[b * FACT_NB + tid][c] = acc[c];
+        }
+        __syncthreads();
+        for (int b2 = BACK ? 0 : b + 1; BACK ? b2 < b : b2 < nbk; ++b2) {
+            const int64_t kb2 = k0 + (int64_t)b2 * FACT_NB;
+            const int nb2 = (int)min<int64_t>(FACT_NB, gend - kb2);
+            const T* A = BACK ? L + kb + kb2 * lda : L + kb2 + kb * lda;
+            group_product<T, BACK, SV>(A, lda, nb2, nb, &Wg[b * FACT_NB], acc);
+            if (tid < nb2) {
+#pragma unroll
+                for (int c = 0; c < SV; ++c) Wg[b2 * FACT_NB + tid][c] -= acc[c];
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = tid; e < g * SV; e += 256) {
+        const int i = e % g, c = e / g;
+        if (c < nc) W[k0 + i + (c0 + c) * ldw] = Wg[i][c];
+    }
+}
+}  // namespace
+
 template <typename T>
 void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st,
                const BlockFill* bf) {
@@ -1744,27 +1845,31 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     const T* Dinv = (const T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
     const T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
-    T* tmp = (T*)scratch(SCR_LINALG2, sizeof(T) * FACT_NB * (size_t)r);
     // Blocks are taken in groups of SOLVE_GROUP: inside a group block by
-    // block (the diagonal inverse, then the group's later rows), then the
-    // rest of W with the whole group in one product of depth
-    // SOLVE_GROUP x NB.
+    // block in one kernel (k_solve_group: the diagonal inverse, then the
+    // group's later rows), then the rest of W with the whole group in one
+    // product of depth SOLVE_GROUP x NB.
     constexpr int64_t GW = (int64_t)SOLVE_GROUP * FACT_NB;
     std::vector<std::pair<int64_t, int64_t>> iv;
+    // strips of 2 right-hand sides up to 512 of them (>= 128 workgroups
+    // for 256), else 8
+    const bool narrow = r <= 512;
+    auto group_solve = [&](bool back, int64_t k0, int64_t gend) {
+        if (narrow) {
+            const unsigned g = (unsigned)cdiv(r, 2);
+            if (back) k_solve_group<T, true, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+            else k_solve_group<T, false, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+        } else {
+            const unsigned g = (unsigned)cdiv(r, 8);
+            if (back) k_solve_group<T, true, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+            else k_solve_group<T, false, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+        }
+        OMH_LAUNCH_CHECK();
+    };
     // forward: W <- L^-1 W
     for (int64_t k0 = 0; k0 < n; k0 += GW) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
-        for (int64_t k = k0; k < gend; k += FACT_NB) {
-            const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
-            const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-            gemm<T>(false, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
-            k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
-            OMH_LAUNCH_CHECK();
-            const int64_t e = k + nb;
-            if (e < gend)
-                gemm<T>(false, false, false, gend - e, r, nb, T(-1), L + e + k * lda, lda, W + k, ldw, T(1), W + e,
-                        ldw, st);
-        }
+        group_solve(false, k0, gend);
         // rows below the group: with a block structure only those its
         // columns reach in L
         if (gend < n) {
@@ -1784,16 +1889,7 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     // groups' contributions)
     for (int64_t k0 = ((n - 1) / GW) * GW; k0 >= 0; k0 -= GW) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
-        for (int64_t k = ((gend - 1) / FACT_NB) * FACT_NB; k >= k0; k -= FACT_NB) {
-            const int nb = (int)std::min<int64_t>(FACT_NB, n - k);
-            const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-            gemm<T>(true, false, false, nb, r, nb, T(1), Li, FACT_NB, W + k, ldw, T(0), tmp, FACT_NB, st);
-            k_copy<T><<<grid_for(nb * r), 256, 0, st>>>(tmp, FACT_NB, W + k, ldw, nb, r);
-            OMH_LAUNCH_CHECK();
-            if (k > k0)
-                gemm<T>(true, false, false, k - k0, r, nb, T(-1), L + k + k0 * lda, lda, W + k, ldw, T(1), W + k0,
-                        ldw, st);
-        }
+        group_solve(true, k0, gend);
         // rows above the group: only the columns its rows reach in L
         if (k0 > 0) {
             if (!bf || !active_cols(*bf, k0, gend, k0, iv))
@@ -1854,26 +1950,62 @@ bool refine_converged(int it, double rel, double prev, double tol) {
     return it >= 2 && rel > 0.5 * prev;                   // stagnation at the FP64 rounding floor
 }
 
+// Side streams (default priority) for independent products that are too
+// small to fill the GPU one at a time.
+constexpr int N_SIDE = 3;
+struct SidePool {
+    hipStream_t s[N_SIDE] = {};
+    hipEvent_t fork = nullptr, join[N_SIDE] = {};
+};
+
+static SidePool& side_pool() {
+    static std::mutex mu;
+    static std::map<int, SidePool> per_dev;
+    int dev = 0;
+    OMH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    SidePool& p = per_dev[dev];
+    if (!p.fork) {
+        OMH_HIP(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming));
+        for (int i = 0; i < N_SIDE; ++i) {
+            OMH_HIP(hipStreamCreateWithFlags(&p.s[i], hipStreamNonBlocking));
+            OMH_HIP(hipEventCreateWithFlags(&p.join[i], hipEventDisableTiming));
+        }
+    }
+    return p;
+}
+
 // R <- R - H X; with a block structure row segment by row segment, each
-// run of consecutive non-zero column blocks of H in one product.
+// run of consecutive non-zero column blocks of H in one product.  The row
+// segments write disjoint rows of R and run on up to N_SIDE + 1 streams, so
+// that the last, partly filled round of tiles of one product overlaps the
+// others.
 static void residual_update(const double* H, int64_t n, int64_t ldh, const double* X, int64_t ldx, int64_t r,
                             double* R, int64_t ldr, hipStream_t st, const BlockFill* bf) {
     if (!bf) {
         gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, ldr, st);
         return;
     }
+    SidePool& sp = side_pool();
+    OMH_HIP(hipEventRecord(sp.fork, st));
+    for (int i = 0; i < N_SIDE; ++i) OMH_HIP(hipStreamWaitEvent(sp.s[i], sp.fork, 0));
     const int ns = bf->nseg();
     for (int s = 0; s < ns; ++s) {
         const int64_t r0 = bf->start[s], r1 = bf->start[s + 1];
+        hipStream_t ss = (s % (N_SIDE + 1)) == 0 ? st : sp.s[s % (N_SIDE + 1) - 1];
         for (int t = 0; t < ns;) {
             if (!bf->hnz(s, t)) { ++t; continue; }
             int u = t + 1;
             while (u < ns && bf->hnz(s, u)) ++u;
             const int64_t c0 = bf->start[t], c1 = bf->start[u];
             gemm<double>(false, false, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, X + c0, ldx, 1.0,
-                         R + r0, ldr, st);
+                         R + r0, ldr, ss);
             t = u;
         }
+    }
+    for (int i = 0; i < N_SIDE; ++i) {
+        OMH_HIP(hipEventRecord(sp.join[i], sp.s[i]));
+        OMH_HIP(hipStreamWaitEvent(st, sp.join[i], 0));
     }
 }
 
