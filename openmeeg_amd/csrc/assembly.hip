@@ -419,7 +419,7 @@ struct PairStripe {
     int temp;                 // S into tmp (n1 x n2) instead of the HeadMat
     double coeffS;
     int64_t ua0, ub0;         // reference unknowns of A's and B's first triangles
-    double* tmp;
+    double* tmp;              // S(i, j) at tmp[i + j n1]
     double* W12;
     double* W21;
     int64_t* q;               // queue, capacity qcap
@@ -654,81 +654,110 @@ __global__ void k_zero_counts(unsigned* qn) {
 // N block (operators.h:134-148 and the loops of :272-296 / :455-480):
 //   N(v1, v2) = -f coeff sum_{t1 ~ v1} sum_{t2 ~ v2} (cb1 . cb2) S(t1, t2) / (A1 A2)
 // (f = 1/4, or 1/2 for a vertex shared by two different meshes), evaluated
-// as two contractions so that the S block is read coalesced:
+// as two contractions, both reading and writing along contiguous memory:
 //   stage 1: U_d(t1, v2) = sum_{t2 ~ v2} cb2[d] / A2 S(t1, t2)   (d = 0..2)
 //   stage 2: N(v1, v2) = -f coeff sum_{t1 ~ v1} (cb1 . U(t1, v2)) / A1.
-// S(i, j) = S[i + j lds] for local triangle indices of the two meshes (the
-// S block inside the HeadMat, or the dense temporary of a barrier pair).
-// Stage 1: thread = triangle i of mesh A (consecutive lanes, consecutive
-// rows of S), block row y = vertex column j of the chunk; U_d has leading
-// dimension ldu, the three arrays nj columns apart.
+// S(t1, t2) = S[t1 + t2 lds] for local triangle indices of the two meshes
+// (the S block inside the HeadMat, or the dense temporary of a barrier
+// pair).  U_d(t1, jj) = U[jj + t1 ldu + d nt1 ldu] for the vertex columns
+// j0 + jj of a chunk (column fastest, for stage 2).
+// Stage 1: a block covers 64 triangles t1 (lanes, reading S down its
+// columns) x 16 vertex columns (4 per wave, incidences read once per wave);
+// the 3 x 16 x 64 results are transposed in LDS and written column fastest.
+constexpr int N1_ROWS = 64, N1_COLS = 16;
 __global__ void __launch_bounds__(256) k_n_stage1(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
-                                                  const AdjRec* __restrict__ adj, int64_t slot0, int64_t nt1,
+                                                  const AdjRec* __restrict__ adj, int64_t slot0, int64_t nj,
                                                   const double* __restrict__ S, int64_t lds, int64_t tb0,
-                                                  double* __restrict__ U, int64_t ldu, int64_t nj) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j = blockIdx.y;
-    if (i >= nt1) return;
-    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-    for (int64_t p = adj_off[slot0 + j]; p < adj_off[slot0 + j + 1]; ++p) {
-        const AdjRec e = adj[p];
-        const double s = S[i + (e.tri - tb0) * lds] / tri[e.tri].area;
-        u0 += e.cb[0] * s;
-        u1 += e.cb[1] * s;
-        u2 += e.cb[2] * s;
+                                                  double* __restrict__ U, int64_t ldu, int64_t nt1) {
+    __shared__ double T[3][N1_COLS][N1_ROWS + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t t1 = (int64_t)blockIdx.x * N1_ROWS + lane;
+    const int64_t jb = (int64_t)blockIdx.y * N1_COLS;
+#pragma unroll
+    for (int k = 0; k < N1_COLS / 4; ++k) {
+        const int jl = wave * (N1_COLS / 4) + k;
+        const int64_t jj = jb + jl;
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+        if (jj < nj && t1 < nt1) {
+            for (int64_t p = adj_off[slot0 + jj]; p < adj_off[slot0 + jj + 1]; ++p) {
+                const AdjRec e = adj[p];
+                const double sv = S[t1 + (e.tri - tb0) * lds] / tri[e.tri].area;
+                u0 += e.cb[0] * sv;
+                u1 += e.cb[1] * sv;
+                u2 += e.cb[2] * sv;
+            }
+        }
+        T[0][jl][lane] = u0;
+        T[1][jl][lane] = u1;
+        T[2][jl][lane] = u2;
     }
-    double* u = U + i + j * ldu;
-    u[0] = u0;
-    u[ldu * nj] = u1;
-    u[2 * ldu * nj] = u2;
+    __syncthreads();
+    const int64_t cs = nt1 * ldu;
+    const int jl = threadIdx.x % N1_COLS;
+    for (int r = threadIdx.x / N1_COLS; r < N1_ROWS; r += 256 / N1_COLS) {
+        const int64_t row = (int64_t)blockIdx.x * N1_ROWS + r, jj = jb + jl;
+        if (row < nt1 && jj < nj) {
+            double* u = U + jj + row * ldu;
+            u[0] = T[0][jl][r];
+            u[cs] = T[1][jl][r];
+            u[2 * cs] = T[2][jl][r];
+        }
+    }
 }
 
-// Stage 2: thread = vertex slot a of mesh A and N2_COLS consecutive columns
-// of the chunk (block row y; column jb is vertex slot j0 + jb of mesh B), so
-// that the incidences of a are read once; for a block of one mesh only
-// a >= b is computed and both symmetric entries are written.
-constexpr int N2_COLS = 16;
+// Stage 2: threads over the chunk's columns (vertex slots j0 + jj of mesh
+// B), block row y = vertex slot a of mesh A; the rows U(t1, .) of a's
+// triangles are read along the columns.  A block of one mesh computes the
+// columns b >= a.  Only the lower-triangle entry of each vertex pair is
+// accumulated (k_mirror_lower fills the upper one once the V x V block is
+// complete).
 __global__ void __launch_bounds__(256) k_n_stage2(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
                                                   const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
                                                   const int64_t* __restrict__ vert_unk, int64_t sa0, int64_t na,
-                                                  int64_t sb0, int64_t j0, int same, int64_t ta0, double coeff,
-                                                  const double* __restrict__ U, int64_t ldu, int64_t nj,
-                                                  double* __restrict__ H, Layout L, int use_atomic) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t jb0 = (int64_t)blockIdx.y * N2_COLS;
-    if (a >= na) return;
-    if (same && j0 + jb0 > a) return;   // all of this thread's columns lie above the diagonal
-    const int64_t ncol = std::min<int64_t>(N2_COLS, nj - jb0);
-    const int64_t cs = ldu * nj;        // distance between the three U arrays
-    double acc[N2_COLS];
-#pragma unroll
-    for (int q = 0; q < N2_COLS; ++q) acc[q] = 0.0;
-    for (int64_t p = adj_off[sa0 + a]; p < adj_off[sa0 + a + 1]; ++p) {
-        const AdjRec e = adj[p];
-        const double ia = 1.0 / tri[e.tri].area;
-        const double w0 = e.cb[0] * ia, w1 = e.cb[1] * ia, w2 = e.cb[2] * ia;
-        const double* u = U + (e.tri - ta0) + jb0 * ldu;
-#pragma unroll
-        for (int q = 0; q < N2_COLS; ++q)
-            if (q < ncol) acc[q] += w0 * u[q * ldu] + w1 * u[q * ldu + cs] + w2 * u[q * ldu + 2 * cs];
-    }
-    const int32_t v1 = slot_vid[sa0 + a];
-    const int64_t r = dev_index(L, vert_unk[v1]);
-#pragma unroll
-    for (int q = 0; q < N2_COLS; ++q) {
-        const int64_t b = j0 + jb0 + q;
-        if (q >= ncol || (same && b > a)) break;
-        const int32_t v2 = slot_vid[sb0 + b];
-        const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
-        const double val = -factor * acc[q] * coeff;
-        const int64_t c = dev_index(L, vert_unk[v2]);
-        if (use_atomic) {
-            atomicAdd(&H[r + c * L.ld], val);
-            if (r != c) atomicAdd(&H[c + r * L.ld], val);
-        } else {
-            H[r + c * L.ld] += val;
-            if (r != c) H[c + r * L.ld] += val;
+                                                  int64_t sb0, int64_t j0, int64_t nj, int same, int64_t ta0,
+                                                  double coeff, const double* __restrict__ U, int64_t ldu,
+                                                  int64_t nt1, double* __restrict__ H, Layout L, int use_atomic) {
+    const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= nj) return;
+    const int64_t b = j0 + jj;
+    const int64_t cs = nt1 * ldu;
+    const int32_t v2 = slot_vid[sb0 + b];
+    const int64_t c = dev_index(L, vert_unk[v2]);
+    for (int64_t a = blockIdx.y; a < na; a += gridDim.y) {
+        if (same && b < a) continue;
+        double acc = 0.0;
+        for (int64_t p = adj_off[sa0 + a]; p < adj_off[sa0 + a + 1]; ++p) {
+            const AdjRec e = adj[p];
+            const double ia = 1.0 / tri[e.tri].area;
+            const double w0 = e.cb[0] * ia, w1 = e.cb[1] * ia, w2 = e.cb[2] * ia;
+            const double* u = U + jj + (e.tri - ta0) * ldu;
+            acc += w0 * u[0] + w1 * u[cs] + w2 * u[2 * cs];
         }
+        const int32_t v1 = slot_vid[sa0 + a];
+        const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
+        const double val = -factor * acc * coeff;
+        const int64_t r = dev_index(L, vert_unk[v1]);
+        double* h = r <= c ? &H[c + r * L.ld] : &H[r + c * L.ld];
+        if (use_atomic) atomicAdd(h, val);
+        else *h += val;
+    }
+}
+
+// H(j, i) = H(i, j) for i > j inside the square block [o, o + nb)^2:
+// 64 x 64 tiles on or below the diagonal through LDS, both sides coalesced.
+__global__ void __launch_bounds__(256) k_mirror_lower(double* __restrict__ H, int64_t ld, int64_t o, int64_t nb) {
+    __shared__ double tile[64][65];
+    int64_t bj, bi;
+    upper_tile(blockIdx.x, bj, bi);   // bj <= bi
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int k = ty; k < 64; k += 4) {   // rows i along tx, columns j along k
+        const int64_t i = bi * 64 + tx, j = bj * 64 + k;
+        if (i < nb && j < nb && i > j) tile[k][tx] = H[(o + i) + (o + j) * ld];
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {   // rows j along tx, columns i along k
+        const int64_t j = bj * 64 + tx, i = bi * 64 + k;
+        if (i < nb && j < nb && i > j) H[(o + j) + (o + i) * ld] = tile[tx][k];
     }
 }
 
@@ -811,26 +840,26 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
         S = H + dev_index(L, A.tri_unk0) + dev_index(L, B.tri_unk0) * L.ld;
         lds = L.ld;
     }
-    const int64_t ldu = (nt1 + 31) / 32 * 32;
     const size_t budget = size_t(1) << 30;   // 1 GiB of U per chunk (shares the D pass-1 scratch)
-    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({nb, 65535, (int64_t)(budget / (3 * sizeof(double) * (size_t)ldu))}));
+    int64_t chunk = std::max<int64_t>(32, (int64_t)(budget / (3 * sizeof(double) * (size_t)nt1)) / 32 * 32);
+    chunk = std::min<int64_t>(chunk, nb);
     // OMH_N_CHUNK_COLS caps the columns per chunk (tests run the chunked
     // path on small meshes with it)
     if (const char* env = std::getenv("OMH_N_CHUNK_COLS")) {
         const long long v = std::atoll(env);
         if (v > 0) chunk = std::min<int64_t>(chunk, (int64_t)v);
     }
-    double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)chunk);
+    const int64_t ldu = (chunk + 31) / 32 * 32;
+    double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)nt1);
     for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
         const int64_t nj = std::min<int64_t>(chunk, nb - j0);
-        // a block of one mesh only needs rows a >= j0
-        const dim3 g1((unsigned)cdiv(nt1, 256), (unsigned)nj);
-        k_n_stage1<<<g1, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, B.slot_begin + j0, nt1, S, lds, B.tri_begin, U, ldu,
-                                       nj);
+        const dim3 g1((unsigned)cdiv(nt1, N1_ROWS), (unsigned)cdiv(nj, N1_COLS));
+        k_n_stage1<<<g1, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, B.slot_begin + j0, nj, S, lds, B.tri_begin, U, ldu,
+                                       nt1);
         OMH_LAUNCH_CHECK();
-        const dim3 g2((unsigned)cdiv(na, 256), (unsigned)cdiv(nj, N2_COLS));
+        const dim3 g2((unsigned)cdiv(nj, 256), (unsigned)std::min<int64_t>(na, 65535));
         k_n_stage2<<<g2, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
-                                       B.slot_begin, j0, same, A.tri_begin, coeff, U, ldu, nj, H, L, use_atomic);
+                                       B.slot_begin, j0, nj, same, A.tri_begin, coeff, U, ldu, nt1, H, L, use_atomic);
         OMH_LAUNCH_CHECK();
     }
 }
@@ -1005,6 +1034,13 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
                                                                    H, L);
             OMH_LAUNCH_CHECK();
         }
+    }
+    // the N blocks were accumulated in the lower triangle of the V x V block
+    if (L.nv > 0) {
+        const int64_t v0 = L.tri_first ? L.nt : 0;
+        const int64_t t = cdiv(L.nv, 64);
+        k_mirror_lower<<<(unsigned)(t * (t + 1) / 2), 256, 0, st>>>(H, L.ld, v0, L.nv);
+        OMH_LAUNCH_CHECK();
     }
 }
 
