@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -1087,6 +1088,142 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
         }
 }
 
+// The same update on 256 x 256 tiles: 512 threads, 8 waves of 64 x 128
+// (4 x 8 MFMA blocks, 3 products each), so that every fragment read from LDS
+// feeds more products (24 reads per 96 MFMAs against 16 per 48) while two
+// waves per SIMD remain.  Operand slices go global -> LDS directly
+// (global_load_lds, 16 bytes per lane, lane L to base + 16 L; each lane
+// fetches the chunk whose swizzled LDS slot it lands in), two stages in
+// separate LDS arrays; a slice's fragments are all read before the next
+// slice's loads are issued, and the loads run under the slice's MFMAs.
+// Operand rows up to r_off + 256 * ceil(m / 256) exist (zero padding).
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int S2_TILE = 256;
+
+template <int KP>
+__global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, int64_t r_off,
+                                                       const _Float16* __restrict__ Ah,
+                                                       const _Float16* __restrict__ Al,
+                                                       const _Float16* __restrict__ Bh,
+                                                       const _Float16* __restrict__ Bl,
+                                                       const float* __restrict__ scale, float* __restrict__ C,
+                                                       int64_t ldc, TileMap map, TileRemap rm) {
+    __shared__ half8_t S0[4][S2_TILE * 4];
+    __shared__ half8_t S1[4][S2_TILE * 4];
+    int64_t t = blockIdx.x;
+    if (map.kind >= 4) {
+        t = xcd_tile(t, map.per);
+        if (t >= map.ntiles) return;
+    }
+    int64_t tm, tn;
+    map_tile(map, t, tm, tn);
+    if (rm.n) {
+        tm = remap_tile(rm, tm);
+        tn = remap_tile(rm, tn);
+    }
+    const int64_t row0 = r_off + tm * S2_TILE, col0 = r_off + tn * S2_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
+    // this lane's DMA sources: rows 32 wave + 16 p + lane / 4 of each array,
+    // the chunk that belongs in LDS slot lane % 4 of that row (the swizzle
+    // depends on the row's low 4 bits only: one 32-bit offset serves both p)
+    const int rl = lane >> 2;
+    const uint32_t goff = (uint32_t)(((32 * wave + rl) * SK + 8 * ((lane & 3) ^ (((rl >> 1) ^ (rl >> 2)) & 3))) * 2);
+    auto dma = [&](int ks, half8_t (*S)[S2_TILE * 4]) {
+        const int64_t so = (int64_t)ks * mp * SK;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const char* base = reinterpret_cast<const char*>((a == 0 ? Ah : a == 1 ? Al : a == 2 ? Bh : Bl) + so +
+                                                             (a < 2 ? row0 : col0) * SK);
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+                __builtin_amdgcn_global_load_lds((const void*)(base + goff + p * 16 * SK * 2),
+                                                 (lds_void*)&S[a][(32 * wave + 16 * p) * 4], 16, 0, 0);
+        }
+    };
+    float4_t acc[4][8];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const int fr = lane & 15, fc = lane >> 4;
+    // fragment (row r0 + fr, chunk fc) of a 16-row block at r0: slot
+    // r0 * 4 + swz(fr, fc), r0 a multiple of 16
+    const int lo = swz(fr, fc);
+    constexpr int NKS = KP / SK;
+    auto step = [&](int ks, half8_t (*Sc)[S2_TILE * 4], half8_t (*Sn)[S2_TILE * 4]) {
+        __builtin_amdgcn_sched_barrier(0);   // the previous slice's products stay above the wait
+        __builtin_amdgcn_s_waitcnt(0);       // this wave's loads of slice ks have landed
+        __syncthreads();                 // everyone's have; Sn is no longer read
+        // product 1 (l x h) needs al, bh; products 2 and 3 ah, bl, bh: al is
+        // dead before ah and bl are read (20 fragments live at most)
+        const half8_t* A0 = &Sc[0][wm * 4 + lo];
+        const half8_t* A1 = &Sc[1][wm * 4 + lo];
+        const half8_t* B2 = &Sc[2][wn * 4 + lo];
+        const half8_t* B3 = &Sc[3][wn * 4 + lo];
+        half8_t ah[4], al[4], bh[8], bl[8];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) al[a] = A1[64 * a];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) bh[b] = B2[64 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) ah[a] = A0[64 * a];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) bl[b] = B3[64 * b];
+        if (ks + 1 < NKS) dma(ks + 1, Sn);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+    };
+    dma(0, S0);
+    for (int ks = 0; ks < NKS; ks += 2) {
+        step(ks, S0, S1);
+        step(ks + 1, S1, S0);
+    }
+    // epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of a block
+    const int64_t i0 = tm * S2_TILE, j0 = tn * S2_TILE;
+    const bool vec = tm != tn && i0 + S2_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int64_t gi = i0 + wm + 16 * a + 4 * fc;
+        float4 si = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (vec) si = *reinterpret_cast<const float4*>(scale + r_off + gi);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int64_t gj = j0 + wn + 16 * b + fr;
+            const float sj = scale[r_off + gj];
+            if (vec) {
+                float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
+                float4 c = *cp;
+                c.x -= si.x * sj * acc[a][b][0];
+                c.y -= si.y * sj * acc[a][b][1];
+                c.z -= si.z * sj * acc[a][b][2];
+                c.w -= si.w * sj * acc[a][b][3];
+                *cp = c;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = gi + r;
+                    if (i < m && gj < m && i >= gj) C[i + gj * ldc] -= scale[r_off + i] * sj * acc[a][b][r];
+                }
+            }
+        }
+    }
+}
+
 // The trailing matrix below the next block group.
 template <int KP>
 __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
@@ -1451,11 +1588,11 @@ static bool active_cols(const BlockFill& bf, int64_t r0, int64_t r1, int64_t c,
     return true;
 }
 
-// Tile rows (S_TILE, relative to row r) that the rank-S_KP update of the
+// Tile rows (of `tile` rows, relative to row r) that the rank-S_KP update of the
 // block group with columns [c0, c1) can touch: the segments whose rows may be
 // non-zero below those columns.  Returns false when that is every row.
-static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r, TileRemap& rm,
-                             int64_t& rows) {
+static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t c1, int64_t r, int64_t tile,
+                             TileRemap& rm, int64_t& rows) {
     const int ns = bf.nseg();
     std::vector<std::pair<int64_t, int64_t>> iv;   // tile intervals
     rows = 0;
@@ -1466,7 +1603,7 @@ static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t
         const int64_t lo = std::max(bf.start[s], r), hi = std::min(bf.start[s + 1], n);
         if (!act || hi <= lo) continue;
         rows += hi - lo;
-        iv.emplace_back((lo - r) / S_TILE, cdiv(hi - r, S_TILE));
+        iv.emplace_back((lo - r) / tile, cdiv(hi - r, tile));
     }
     std::sort(iv.begin(), iv.end());
     std::vector<std::pair<int64_t, int64_t>> mg;
@@ -1474,7 +1611,7 @@ static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t
         if (!mg.empty() && v.first <= mg.back().second) mg.back().second = std::max(mg.back().second, v.second);
         else mg.push_back(v);
     }
-    const int64_t ntm = cdiv(n - r, S_TILE);
+    const int64_t ntm = cdiv(n - r, tile);
     if (mg.empty() || mg.size() > (size_t)MAX_REMAP || (mg.size() == 1 && mg[0].first == 0 && mg[0].second >= ntm))
         return false;
     rm.n = (int)mg.size();
@@ -1486,6 +1623,17 @@ static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t
     }
     rm.c[rm.n] = c;
     return true;
+}
+
+// Tile of the main split update: 128 (default) or 256 (OMH_SYRK_TILE=256;
+// both give the same factor, each entry's products summed in the same
+// order).  At c2 both run at ~850 TFLOP/s: the update is bound by operand
+// bands fetched again from beyond L2 (the 256-row tiles halve the operand
+// bytes per output but, one workgroup per CU, share fewer bands per XCD: 11%
+// less fetch).
+static int syrk_tile() {
+    const char* e = std::getenv("OMH_SYRK_TILE");
+    return (e && std::atoi(e) == 256) ? 256 : 128;
 }
 
 static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st, const BlockFill* bf) {
@@ -1564,7 +1712,9 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         const int64_t w2 = std::min<int64_t>(S_KP, n - r0);
         // split operands of the rank-S_KP update of all rows from r0
         const int64_t ms = n - r0, r2 = r0 + w2, mb = n - r2;
-        const int64_t mp = cdiv(ms, S_TILE) * S_TILE;   // operand rows (zero padded)
+        // operand rows (zero padded): whole 128-row tiles of the strip and
+        // whole 256-row tiles of the main update
+        const int64_t mp = std::max(cdiv(ms, S_TILE) * S_TILE, cdiv(w2 + cdiv(mb, S2_TILE) * S2_TILE, S_TILE) * S_TILE);
         {
             SplitSrc src;
             for (int i = 0; i < S_PANELS; ++i) {
@@ -1597,19 +1747,33 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         OMH_HIP(hipEventRecord(la.done, la.side));
         // the rest of the trailing matrix
         if (mb > 0) {
-            // only the tile rows the group's panels can reach (block structure)
+            // only the tile rows the group's panels can reach (block
+            // structure); 256 x 256 tiles while they give >= 512 workgroups
+            const int64_t c0 = start_of(b0), c1 = std::min(n, start_of(b0 + S_PANELS));
             TileRemap rm;
             int64_t rows = mb;
-            int64_t ntm = cdiv(mb, S_TILE);
-            if (bf && active_tile_rows(*bf, n, start_of(b0), std::min(n, start_of(b0 + S_PANELS)), r2, rm, rows))
-                ntm = rm.c[rm.n];
+            int64_t ntm = cdiv(mb, S2_TILE);
+            if (bf && active_tile_rows(*bf, n, c0, c1, r2, S2_TILE, rm, rows)) ntm = rm.c[rm.n];
+            bool big = syrk_tile() == 256 && ntm * (ntm + 1) / 2 >= 512;
+            if (!big) {
+                rm = TileRemap{};
+                rows = mb;
+                ntm = cdiv(mb, S_TILE);
+                if (bf && active_tile_rows(*bf, n, c0, c1, r2, S_TILE, rm, rows)) ntm = rm.c[rm.n];
+            }
             const int64_t tiles = ntm * (ntm + 1) / 2;
             const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
                                               : TileMap{1, ntm, 0, 0, tiles, 0};
+            const unsigned grid = (unsigned)(map.kind == 4 ? 8 * map.per : tiles);
             ProfScope ps("factor_update_f32s", (double)rows * (double)(rows + 1) * (double)S_KP, st);
-            if (tiles > 0)
-                k_syrk_split<S_KP><<<(unsigned)(map.kind == 4 ? 8 * map.per : tiles), 256, 0, st>>>(
-                    mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, map, rm);
+            if (tiles > 0) {
+                if (big)
+                    k_syrk_split256<S_KP><<<grid, 512, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
+                                                                lda, map, rm);
+                else
+                    k_syrk_split<S_KP><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda,
+                                                             map, rm);
+            }
             OMH_LAUNCH_CHECK();
         }
         OMH_HIP(hipStreamWaitEvent(st, la.done, 0));

@@ -248,6 +248,33 @@ def test_block_structured_factor_and_solve(om, layers, precision):
     assert b"declared zero" in _lib.lib().omh_last_error()
 
 
+def test_split_factor_tiles_agree(om, monkeypatch):
+    """The split-FP16 factorisation's main update on 256 x 256 tiles (direct
+    global-to-LDS operand loads, 8 waves) and on 128 x 128 tiles
+    (the default) sums every entry's products in the same order: the
+    factors must be bitwise equal.  4 layers of 2562 vertices (n = 25608), so
+    that the early groups run on 256-row tiles, with and without the block
+    structure."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    H = _head_mat_device(g, far_field=True)
+    for structure in (None, g.factor_segments()):
+        monkeypatch.setenv("OMH_SYRK_TILE", "256")
+        F256 = factor(H, n, "mixed16", structure=structure)
+        monkeypatch.delenv("OMH_SYRK_TILE")
+        L256 = F256.L.view(n, n).t().tril().clone()
+        del F256
+        F128 = factor(H, n, "mixed16", structure=structure)
+        L128 = F128.L.view(n, n).t().tril()
+        print("n=%d structure %s: max |L256 - L128| %.3e" % (n, structure is not None,
+                                                             float((L256 - L128).abs().max())), flush=True)
+        assert torch.equal(L256, L128)
+        del F128, L128, L256
+
+
 def test_headmat_head1_matches_golden(om, head1):
     g, _, _ = head1
     hm = om.HeadMat(g)

@@ -26,7 +26,10 @@ def _cpu_gemm(ta, tb, m, n, k, A, lda, B, ldb, Cout, ldc, a_off=0, b_off=0, alph
     Am = A.as_strided((k, m) if ta else (m, k), (1, lda), a_off)
     Bm = B.as_strided((n, k) if tb else (k, n), (1, ldb), b_off)
     Cm = Cout.as_strided((m, n), (1, ldc))
-    Cm.copy_(alpha * ((Am.t() if ta else Am) @ (Bm.t() if tb else Bm)) + beta * Cm)
+    P = alpha * ((Am.t() if ta else Am) @ (Bm.t() if tb else Bm))
+    # beta = 0: C is not read (as the device GEMM; the buffers come from
+    # torch.empty and may hold NaN patterns)
+    Cm.copy_(P if beta == 0.0 else P + beta * Cm)
 
 
 def _gain(meshes, layer_sigma, dip, E):
