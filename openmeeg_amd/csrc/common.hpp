@@ -36,7 +36,8 @@ enum ScratchSlot {
     SCR_LINALG2 = 4,
     SCR_SMALL = 5,
     SCR_PAIRQ = 6,
-    SCR_NSLOTS = 7
+    SCR_SRCCOEF = 7,   // source-matrix coefficients (the source matrix may run beside the factorisation)
+    SCR_NSLOTS = 8
 };
 void* scratch(ScratchSlot slot, size_t bytes);
 void release_scratch();

@@ -542,7 +542,7 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
     std::vector<double> cD, cP;
     std::vector<char> used;
     dsm_coefficients(g, n_dip, domain, cD, cP, used);
-    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * std::max<size_t>(1, 2 * cD.size()));
+    double* d_c = (double*)scratch(SCR_SRCCOEF, sizeof(double) * std::max<size_t>(1, 2 * cD.size()));
     if (!cD.empty()) {
         OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
         OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
