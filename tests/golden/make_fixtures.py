@@ -15,6 +15,8 @@ What is extracted (all data files that the reference's own tests hold):
   plus the Head1 inputs (``data/Head1``: meshes, .geom, .cond, .dip, .patches).
 * ``head3/``: the Head3 inputs (642-vertex spheres, same geometry as the
   0.87/0.92/1.0 nested-sphere configuration), .dip and .patches.
+* ``headnna1/``: the HeadNNa1 inputs (non-nested: the north and south meshes
+  share the cut mesh's vertices), .dip and .patches.
 * ``analytic/``: ``tests/analytic/eeg_head{1,2,3}_analytic.txt`` (multilayer
   sphere series, n_electrodes x 6 dipoles).
 """
@@ -81,6 +83,12 @@ def main():
               "cortex.3.tri", "skull.3.tri", "scalp.3.tri"):
         shutil.copy(os.path.join(data3, f), h3)
 
+    hn = os.path.join(HERE, "headnna1")
+    os.makedirs(hn, exist_ok=True)
+    datan = os.path.join(REF, "data", "HeadNNa1")
+    for f in ("HeadNNa1.geom", "HeadNNa1.cond", "HeadNNa1.dip", "HeadNNa1.patches",
+              "cut.tri", "north.tri", "south.tri", "skull.tri", "scalp.tri"):
+        shutil.copy(os.path.join(datan, f), hn)
     an = os.path.join(HERE, "analytic")
     os.makedirs(an, exist_ok=True)
     for k in (1, 2, 3):

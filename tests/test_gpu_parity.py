@@ -38,6 +38,49 @@ def head1(om):
     return g, dip, sens
 
 
+@pytest.fixture(scope="module")
+def nna1(om):
+    """HeadNNa1 of the reference's regression suite: non-nested, the north and
+    south meshes share the cut mesh's vertices."""
+    from oracle import oracle as O
+    d = os.path.join(GOLDEN, "headnna1")
+    gf, cf = os.path.join(d, "HeadNNa1.geom"), os.path.join(d, "HeadNNa1.cond")
+    dip = O.read_dipoles(os.path.join(d, "HeadNNa1.dip"))
+    sens = om.Sensors()
+    sens.load(os.path.join(d, "HeadNNa1.patches"))
+    el = np.loadtxt(os.path.join(d, "HeadNNa1.patches"), usecols=(1, 2, 3))
+    return om.read_geometry(gf, cf), O.load_geometry(gf, cf), dip, sens, el
+
+
+def test_nonnested_nna1(om, nna1, monkeypatch):
+    """HeadNNa1 (shared vertices): the FP64 HeadMat against the oracle's
+    (1e-13); the far-field assembly (fused pass: shared vertices in its
+    vertex gathers and the N blocks' atomic accumulation) against the
+    per-operator kernels (1e-13 of the largest entry), symmetric; the FP64
+    adjoint gain against the oracle's (1e-10)."""
+    import torch
+    from oracle import oracle as O
+    from openmeeg_amd.api import _head_mat_device
+    g, og, dip, sens, el = nna1
+    n = g.size
+    H = O.unpack(O.head_mat(og), og.size)
+    hm = om.HeadMat(g)
+    e_h = rel_fro(hm.data(), O.pack(H))
+    Hf = _head_mat_device(g, far_field=True).view(n, n)
+    monkeypatch.setenv("OMH_FAR_SEPARATE", "1")
+    Hs = _head_mat_device(g, far_field=True).view(n, n)
+    monkeypatch.delenv("OMH_FAR_SEPARATE")
+    e_f = float((Hf - Hs).abs().max() / Hs.abs().max())
+    cols, vals = O.head2eeg(og, el)
+    Gref = O.gain_eeg(H, O.dip_source_mat(og, dip, ""), O.h2em_dense(og, cols, vals))
+    G = om.GainEEGadjoint(g, dip, hm, om.Head2EEGMat(g, sens)).array()
+    e_g = rel_fro(G, Gref)
+    print("NNa1 n=%d: HeadMat vs oracle %.3e, far fused vs separate %.3e, gain vs oracle %.3e" % (n, e_h, e_f, e_g),
+          flush=True)
+    assert e_h < 1e-13 and e_f < 1e-13 and e_g < 1e-10
+    assert torch.equal(Hf, Hf.t())
+
+
 def test_mfma_lane_layout(om):
     """D(i,j) of one 16x16x4 block sits in lane j + 16*(i % 4), element
     i // 4, for the FP64 instruction and in lane j + 16*(i // 4), element
