@@ -141,6 +141,22 @@ int omh_geometry_mesh_info(const omh_geometry* g, int32_t m, int64_t* info);
 int omh_head2eeg_mat(omh_geometry* g, const double* d_electrodes, int64_t n_el, int64_t* h_cols, double* h_vals,
                      void* stream);
 
+/* Head2MEGMat (assembleSensors.cpp:69-99) before the sensor weights: for
+ * npts integration points (d_pts, d_dirs: npts x 3 row-major) the npts x n
+ * matrix d_out (column-major, leading dimension ld) of MagFactor x the
+ * meshes' conductivity jumps x the Ferguson operator (assembleFerguson.cpp:
+ * 18-30, operators.h:40-59) along each point's direction; columns in the
+ * unknown order `ordering`, triangle columns zero.  The caller sums points
+ * into sensors with Sensors::getWeightsMatrix (sensors.h:191-196). */
+int omh_head2meg_mat(omh_geometry* g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
+                     int64_t ld, int32_t ordering, void* stream);
+
+/* DipSource2MEGMat (assembleSensors.cpp:133-156) before the sensor weights:
+ * d_out (npts x n_dip, column-major) = MagFactor (q x (x - r)) . d /
+ * (|x - r|^3 |d|) for dipoles d_dipoles (n_dip x 6 row-major). */
+int omh_dip_source2meg_mat(const double* d_dipoles, int64_t n_dip, const double* d_pts, const double* d_dirs,
+                           int64_t npts, double* d_out, int64_t ld, void* stream);
+
 /* ------------------------------------------------------- dense linear algebra */
 
 /* Workspace bytes for omh_sym_factor_* / omh_sym_solve_* (element size 8 or 4). */

@@ -10,12 +10,16 @@ graph-captured pass (no reference counterparts).  All computation runs in the HI
 (libopenmeeg_amd.so) through its C ABI (include/openmeeg_amd.h).
 """
 from .api import (  # noqa: F401
+    DipSource2MEGMat,
     DipSourceMat,
     Factor,
     GainEEG,
     GainEEGadjoint,
+    GainMEG,
+    GainMEGadjoint,
     Geometry,
     Head2EEGMat,
+    Head2MEGMat,
     HeadMat,
     Integrator,
     Matrix,

@@ -32,6 +32,8 @@ _SIGS = {
     "omh_locate_domains": ([vp, vp, i64, i64, p_i32, vp], C.c_int),
     "omh_dip_source_mat": ([vp, vp, i64, p_i32, C.c_int, C.c_int, f64, vp, i64, C.c_int, vp], C.c_int),
     "omh_head2eeg_mat": ([vp, vp, i64, p_i64, p_f64, vp], C.c_int),
+    "omh_head2meg_mat": ([vp, vp, vp, i64, vp, i64, i32, vp], C.c_int),
+    "omh_dip_source2meg_mat": ([vp, i64, vp, vp, i64, vp, i64, vp], C.c_int),
     "omh_dip_source_adjoint": ([vp, vp, i64, p_i32, C.c_int, vp, i64, C.c_int, vp, vp], C.c_int),
     "omh_geometry_domain_conductivity": ([vp, i32, p_f64], C.c_int),
     "omh_geometry_set_conductivity": ([vp, i32, f64], C.c_int),

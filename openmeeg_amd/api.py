@@ -709,15 +709,26 @@ def GainEEGadjoint(geom, dipoles, hm, h2em, precision="fp64"):
 
 
 class Sensors:
-    """EEG electrodes: Sensors::load (sensors.cpp:36-143) for labelled or
-    unlabelled position files, or Sensors(positions)."""
+    """Sensors (sensors.h:76-216): Sensors::load (sensors.cpp:36-143) for
+    labelled or unlabelled files -- positions (EEG electrodes), positions and
+    orientations (6 columns), or MEG integration points with a weight (7
+    columns: label x y z dx dy dz w; points with the same label are one
+    sensor) -- or Sensors(positions)."""
 
     def __init__(self, positions=None, geom=None):
         self.labels = []
         self.positions = np.zeros((0, 3))
+        self.orientations = None
+        self.weights = np.zeros(0)
+        self.point_sensor = np.zeros(0, dtype=np.int64)
         if positions is not None:
             a = positions.array() if isinstance(positions, Matrix) else np.asarray(positions, dtype=np.float64)
             self.positions = np.ascontiguousarray(a[:, :3], dtype=np.float64)
+            self._plain(len(self.positions))
+
+    def _plain(self, npts):
+        self.weights = np.ones(npts)
+        self.point_sensor = np.arange(npts, dtype=np.int64)
 
     def load(self, fname):
         rows = []
@@ -731,16 +742,145 @@ class Sensors:
         if any(len(r) != len(rows[0]) for r in rows):
             raise ValueError("Problem while reading Sensors file: each line should have the same number of elements")
         labeled = rows[0][0].count(".") != 1
+        names = [r[0] for r in rows] if labeled else []
         if labeled:
-            self.labels = [r[0] for r in rows]
             rows = [r[1:] for r in rows]
-        self.positions = np.ascontiguousarray(np.array(rows, dtype=np.float64)[:, :3])
+        mat = np.array(rows, dtype=np.float64)
+        ncol = mat.shape[1]
+        if ncol == 4:
+            raise ValueError("Sensors: spatially extended (EIT) sensors need a geometry")
+        self.positions = np.ascontiguousarray(mat[:, :3])
+        self.orientations = np.ascontiguousarray(mat[:, 3:6]) if ncol >= 6 else None
+        self.weights = mat[:, 6].copy() if ncol == 7 else np.ones(len(mat))
+        # sensor index per point: labels in order of first appearance (sensors.cpp:127-142)
+        self.labels = []
+        idx = {}
+        ps = np.zeros(len(mat), dtype=np.int64)
+        for i in range(len(mat)):
+            if labeled:
+                if names[i] not in idx:
+                    idx[names[i]] = len(self.labels)
+                    self.labels.append(names[i])
+                ps[i] = idx[names[i]]
+            else:
+                ps[i] = i
+        self.point_sensor = ps
 
     def getNumberOfSensors(self):
+        return int(self.point_sensor.max()) + 1 if len(self.point_sensor) else 0
+
+    def getNumberOfPositions(self):
         return len(self.positions)
 
     def getPositions(self):
         return Matrix(np.asfortranarray(self.positions))
+
+    def hasOrientations(self):
+        return self.orientations is not None
+
+    def getOrientations(self):
+        if self.orientations is None:
+            raise ValueError("these sensors have no orientations")
+        return Matrix(np.asfortranarray(self.orientations))
+
+    def getWeights(self):
+        return self.weights.copy()
+
+    def getWeightsMatrix(self):
+        """Sensors::getWeightsMatrix (sensors.h:191-196), dense: sensors x points."""
+        W = np.zeros((self.getNumberOfSensors(), self.getNumberOfPositions()))
+        W[self.point_sensor, np.arange(self.getNumberOfPositions())] = self.weights
+        return W
+
+
+def _meg_points(sensors):
+    if not isinstance(sensors, Sensors) or sensors.orientations is None:
+        raise ValueError("MEG needs Sensors with orientations (a .squids file)")
+    return _to_dev(np.ascontiguousarray(sensors.positions)), _to_dev(np.ascontiguousarray(sensors.orientations))
+
+
+def _sensor_sum(sensors, P, ncols):
+    """Weights matrix (sensors x points) times P (points x ncols, device,
+    column-major) on the device."""
+    npts, ns = sensors.getNumberOfPositions(), sensors.getNumberOfSensors()
+    W = _to_dev(np.asfortranarray(sensors.getWeightsMatrix()).ravel(order="F"))
+    out = _dev_f64(max(1, ns * ncols))
+    check(lib().omh_gemm_f64(0, 0, ns, ncols, npts, 1.0, _ptr(W), ns, _ptr(P), npts, 0.0, _ptr(out), ns,
+                             _stream()))
+    return out
+
+
+def _head2meg_device(geom, sensors, ordering=ORDER_TRIANGLES_FIRST):
+    """Head2MEGMat on the device (sensors x n, column-major, rows summed over
+    the sensors' integration points)."""
+    d_pts, d_dirs = _meg_points(sensors)
+    npts, n = sensors.getNumberOfPositions(), geom.size
+    P = _dev_f64(max(1, npts * n))
+    check(lib().omh_head2meg_mat(geom._h, _ptr(d_pts), _ptr(d_dirs), npts, _ptr(P), max(1, npts), ordering,
+                                 _stream()))
+    return _sensor_sum(sensors, P, n)
+
+
+def Head2MEGMat(geom, sensors):
+    """Head2MEGMat(geo, sensors) -- assemble.h:38, assembleSensors.cpp:69-99."""
+    M = _head2meg_device(geom, sensors, ORDER_REFERENCE)
+    ns = sensors.getNumberOfSensors()
+    return Matrix(M.cpu().numpy()[:ns * geom.size].reshape(geom.size, ns).T.copy(order="F"))
+
+
+def _ds2meg_device(dipoles, sensors):
+    d_pts, d_dirs = _meg_points(sensors)
+    dip = _dipoles_array(dipoles)
+    npts, nd = sensors.getNumberOfPositions(), len(dip)
+    P = _dev_f64(max(1, npts * nd))
+    check(lib().omh_dip_source2meg_mat(_ptr(_to_dev(dip)), nd, _ptr(d_pts), _ptr(d_dirs), npts, _ptr(P),
+                                       max(1, npts), _stream()))
+    return _sensor_sum(sensors, P, nd)
+
+
+def DipSource2MEGMat(dipoles, sensors):
+    """DipSource2MEGMat(dipoles, sensors) -- assemble.h:41, assembleSensors.cpp:133-156."""
+    ns, nd = sensors.getNumberOfSensors(), len(_dipoles_array(dipoles))
+    M = _ds2meg_device(dipoles, sensors)
+    return Matrix(M.cpu().numpy()[:ns * nd].reshape(nd, ns).T.copy(order="F"))
+
+
+def GainMEG(hminv, dsm, h2mm, ds2mm):
+    """GainMEG(HeadMatInv, SourceMat, Head2MEGMat, Source2MEGMat) =
+    Source2MEG + (Head2MEG H^-1) SourceMat -- gain.h:60-67."""
+    n = hminv.nlin()
+    nv = hminv.nv
+    if nv is None:
+        raise ValueError("GainMEG needs the HeadMat unknown layout (a SymMatrix from HeadMat)")
+    Hi = hminv.device()
+    M = h2mm.array()
+    ns = M.shape[0]
+    Md = to_device_rows(np.asfortranarray(M.T), n, nv)           # (n x ns) triangles-first rows
+    MH = _dev_f64(ns * n)
+    check(lib().omh_gemm_f64(1, 0, ns, n, n, 1.0, _ptr(Md), n, _ptr(Hi), n, 0.0, _ptr(MH), ns, _stream()))
+    D = dsm.array()
+    Dd = to_device_rows(D, n, nv)
+    nd = D.shape[1]
+    G = _to_dev(np.asfortranarray(ds2mm.array()).ravel(order="F"))
+    check(lib().omh_gemm_f64(0, 0, ns, nd, n, 1.0, _ptr(MH), ns, _ptr(Dd), n, 1.0, _ptr(G), ns, _stream()))
+    return Matrix(G.cpu().numpy().reshape(nd, ns).T.copy(order="F"))
+
+
+def GainMEGadjoint(geom, dipoles, hm, h2mm, ds2mm, precision="fp64"):
+    """GainMEGadjoint(geo, dipoles, HeadMat, Head2MEGMat, Source2MEGMat) --
+    gain.h:90-103: Y = H^-1 M^T with the stored factor, G = Y^T DSM +
+    Source2MEG."""
+    n, nv = geom.size, geom.n_vertex_unknowns
+    M = h2mm.array()
+    ns = M.shape[0]
+    F = factor(hm.device().clone(), n, precision)
+    Y = to_device_rows(np.asfortranarray(M.T), n, nv)
+    F.solve(Y, ns)
+    D = _dsm_device(geom, dipoles, "")
+    nd = len(_dipoles_array(dipoles))
+    G = _to_dev(np.asfortranarray(ds2mm.array()).ravel(order="F"))
+    check(lib().omh_gemm_f64(1, 0, ns, nd, n, 1.0, _ptr(Y), n, _ptr(D), n, 1.0, _ptr(G), ns, _stream()))
+    return Matrix(G.cpu().numpy().reshape(nd, ns).T.copy(order="F"))
 
 
 # ---------------------------------------------------------------------------

@@ -411,6 +411,37 @@ int omh_head2eeg_mat(omh_geometry* g, const double* d_electrodes, int64_t n_el, 
     });
 }
 
+int omh_head2meg_mat(omh_geometry* g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
+                     int64_t ld, int32_t ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_out, "d_out");
+        if (npts < 0 || ld < std::max<int64_t>(1, npts)) throw std::invalid_argument("bad sizes");
+        if (npts > 0) {
+            need(d_pts, "d_pts");
+            need(d_dirs, "d_dirs");
+        }
+        // the unknown order maps columns; rows are the points (ld >= npts)
+        omh::Layout L = layout_of(*g->g, g->g->size(), ordering);
+        L.ld = ld;
+        omh::head2meg(*g->g, d_pts, d_dirs, npts, d_out, ld, L, S(stream));
+    });
+}
+
+int omh_dip_source2meg_mat(const double* d_dipoles, int64_t n_dip, const double* d_pts, const double* d_dirs,
+                           int64_t npts, double* d_out, int64_t ld, void* stream) {
+    return guard([&] {
+        need(d_out, "d_out");
+        if (npts < 0 || n_dip < 0 || ld < std::max<int64_t>(1, npts)) throw std::invalid_argument("bad sizes");
+        if (npts > 0 && n_dip > 0) {
+            need(d_dipoles, "d_dipoles");
+            need(d_pts, "d_pts");
+            need(d_dirs, "d_dirs");
+        }
+        omh::dip2meg(d_dipoles, n_dip, d_pts, d_dirs, npts, d_out, ld, S(stream));
+    });
+}
+
 int64_t omh_factor_workspace_bytes(int64_t n, int32_t elem_size) {
     if (n <= 0 || (elem_size != 4 && elem_size != 8)) return -1;
     return (int64_t)omh::factor_workspace_bytes(n, (size_t)elem_size);

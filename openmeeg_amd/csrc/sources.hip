@@ -438,6 +438,60 @@ __global__ void __launch_bounds__(256) k_dsm_adjoint(const TriRec* __restrict__ 
         for (int c = 0; c < 6; ++c) atomicAdd(&grad[s * 6 + c], g[c]);
 }
 
+// MagFactor (constants.h:22): mu0 / (4 pi).
+constexpr double MAG_FACTOR = 1e-7;
+
+// Head2MEGMat before the sensor weights (assembleSensors.cpp:69-99) for one
+// mesh: assemble_ferguson (assembleFerguson.cpp:18-30) with
+// Details::operatorFerguson (operators.h:40-59),
+//   F(x, v) = sum over the mesh's triangles T around v of (B - A) S_T(x) / (2 |T|)
+// (A, B the edge of T opposite v, triangle.h:87-90,154; S_T = OperatorS),
+// out(i, v) += coeff F(x_i, v) . d_i / |d_i|.  Thread = vertex slot
+// (x), integration point (block y, looped).
+__global__ void __launch_bounds__(256) k_ferguson(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
+                                                  const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                  const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
+                                                  const double* __restrict__ pts, const double* __restrict__ dirs,
+                                                  int64_t npts, double coeff, double* __restrict__ out, int64_t ld,
+                                                  Layout L) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int64_t vi = vert_unk[slot_vid[s0 + s]];
+    if (vi < 0) return;
+    const int64_t col = dev_index(L, vi);
+    for (int64_t i = blockIdx.y; i < npts; i += gridDim.y) {
+        const d3 x = ld3(pts + 3 * i), d = ld3(dirs + 3 * i);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
+            const AdjRec e = adj[p];
+            const TriRec& T = tri[e.tri];
+            const double sv = ops_eval(load_ops(T), x) / (2 * T.area);
+            fx -= e.cb[0] * sv;   // cb = A - B
+            fy -= e.cb[1] * sv;
+            fz -= e.cb[2] * sv;
+        }
+        out[i + col * ld] += coeff * (fx * d.x + fy * d.y + fz * d.z) / norm(d);
+    }
+}
+
+// DipSource2MEGMat before the sensor weights (assembleSensors.cpp:133-156):
+// out(i, j) = MagFactor (q_j x (x_i - r_j)) . d_i / (|x_i - r_j|^3 |d_i|).
+// Thread = integration point (x), dipole (block y, looped).
+__global__ void __launch_bounds__(256) k_dip2meg(const double* __restrict__ dip, int64_t n_dip,
+                                                 const double* __restrict__ pts, const double* __restrict__ dirs,
+                                                 int64_t npts, double* __restrict__ out, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const d3 x = ld3(pts + 3 * i), d = ld3(dirs + 3 * i);
+    for (int64_t j = blockIdx.y; j < n_dip; j += gridDim.y) {
+        const d3 r = ld3(dip + 6 * j), q = ld3(dip + 6 * j + 3);
+        const d3 diff = sub(x, r);
+        const double nd = norm(diff);
+        const d3 f = scl(1.0 / (nd * nd * nd), cross(q, diff));
+        out[i + j * ld] = dot(f, d) * MAG_FACTOR / norm(d);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -630,6 +684,33 @@ void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, doub
             vals[3 * e + k] = alphas[k];
         }
     }
+}
+
+void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
+              const Layout& L, hipStream_t st) {
+    DeviceGeometry& dg = device_geometry(g);
+    fill_f64(d_out, npts, g.size(), ld_out, 0.0, st);
+    if (npts <= 0) return;
+    for (int m = 0; m < (int)g.meshes.size(); ++m) {
+        // Geometry::conductivity_jump (geometry.h:188-194)
+        double jump = 0.0;
+        for (int d : g.domains_of(m)) jump += g.domains[d].cond * g.domains[d].mesh_orientation(m);
+        const MeshDev& md = dg.meshes[m];
+        const int64_t ns = md.slot_end - md.slot_begin;
+        if (ns <= 0 || jump == 0.0) continue;
+        const dim3 grid((unsigned)cdiv(ns, 256), (unsigned)std::min<int64_t>(npts, 65535));
+        k_ferguson<<<grid, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, md.slot_begin, ns,
+                                         d_pts, d_dirs, npts, MAG_FACTOR * jump, d_out, ld_out, L);
+        OMH_LAUNCH_CHECK();
+    }
+}
+
+void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,
+             double* d_out, int64_t ld_out, hipStream_t st) {
+    if (npts <= 0 || n_dip <= 0) return;
+    const dim3 grid((unsigned)cdiv(npts, 256), (unsigned)std::min<int64_t>(n_dip, 65535));
+    k_dip2meg<<<grid, 256, 0, st>>>(d_dip, n_dip, d_pts, d_dirs, npts, d_out, ld_out);
+    OMH_LAUNCH_CHECK();
 }
 
 }  // namespace omh

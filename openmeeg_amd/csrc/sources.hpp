@@ -44,4 +44,18 @@ void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const s
 // closest triangle and their barycentric weights (host arrays n_el x 3).
 void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, double* vals, hipStream_t st);
 
+// Head2MEGMat before the sensor weights (assembleSensors.cpp:69-99): for
+// npts integration points (d_pts, d_dirs: npts x 3 row-major on the device)
+// the npts x n matrix d_out (column-major, leading dimension ld_out) mapping
+// the unknowns (order of L; triangle columns zero) to the field along each
+// point's direction, MagFactor x each mesh's conductivity jump x the
+// Ferguson operator.
+void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
+              const Layout& L, hipStream_t st);
+
+// DipSource2MEGMat before the sensor weights (assembleSensors.cpp:133-156):
+// d_out (npts x n_dip, column-major) for dipoles d_dip (n_dip x 6 row-major).
+void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,
+             double* d_out, int64_t ld_out, hipStream_t st);
+
 }  // namespace omh

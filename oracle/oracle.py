@@ -4,7 +4,8 @@ ctypes bindings to oracle/liboracle.so (the C restatement of the integration
 kernels, om_oracle.c) plus the drivers that restate, in the reference's
 order, HeadMat (assembleHeadMat.cpp:74-135), DipSourceMat
 (assembleSourceMat.cpp:66-104), Head2EEGMat (assembleSensors.cpp:25-39,
-danielsson.cpp:107-129) and the gains (gain.h:52-89).  Dense linear algebra
+danielsson.cpp:107-129), Head2MEGMat / DipSource2MEGMat
+(assembleSensors.cpp:69-156) and the gains (gain.h:52-89).  Dense linear algebra
 (the reference's LAPACK DSPTRF/DSPTRI/DSPTRS, symmatrix.h:259-297,
 symmatrix.cpp:133-152) is restated with numpy/scipy LAPACK (Bunch-Kaufman
 dsytrf/dsytrs, the unpacked storage form of the same factorisation).
@@ -320,6 +321,96 @@ def read_sensors(path):
     labeled = rows[0][0].count(".") != 1
     data = [r[1:] if labeled else r for r in rows]
     return np.array(data, dtype=np.float64)[:, :3]
+
+
+# ---------------------------------------------------------------------------
+# MEG: Head2MEGMat, DipSource2MEGMat (assembleSensors.cpp:69-156), GainMEG
+# (gain.h:60-67)
+# ---------------------------------------------------------------------------
+MAG_FACTOR = 1e-7   # constants.h:22
+_EDGE = ((1, 2), (2, 0), (0, 1))   # Triangle::indices, triangle.h:154
+
+
+def read_squids(path):
+    """Sensors::load (sensors.cpp:36-143) of MEG integration points
+    (label x y z dx dy dz w): positions, orientations, weights and the sensor
+    index of each point (labels in order of first appearance)."""
+    rows = []
+    with open(path) as f:
+        for ln in f:
+            tok = ln.split()
+            if tok and not tok[0].startswith("#"):
+                rows.append(tok)
+    labeled = rows[0][0].count(".") != 1
+    names = [r[0] for r in rows] if labeled else [str(i) for i in range(len(rows))]
+    mat = np.array([r[1:] if labeled else r for r in rows], dtype=np.float64)
+    order = {}
+    sensor = np.array([order.setdefault(nm, len(order)) for nm in names], dtype=np.int64)
+    w = mat[:, 6] if mat.shape[1] == 7 else np.ones(len(mat))
+    return mat[:, :3].copy(), mat[:, 3:6].copy(), w.copy(), sensor
+
+
+def _weights_matrix(weights, sensor):
+    W = np.zeros((int(sensor.max()) + 1, len(sensor)))
+    W[sensor, np.arange(len(sensor))] = weights
+    return W
+
+
+def conductivity_jump(g, m):
+    """Geometry::conductivity_jump (geometry.h:188-194)."""
+    return sum(d.cond * d.mesh_orientation(m) for d in g.domains)
+
+
+def head2meg(g, pts, dirs, weights, sensor):
+    """Head2MEGMat (assembleSensors.cpp:69-99): the Ferguson matrix of every
+    mesh (assemble_ferguson, assembleFerguson.cpp:18-30, coefficient
+    MagFactor x the mesh's conductivity jump; Details::operatorFerguson,
+    operators.h:40-59: sum over the triangles T around a vertex of
+    (B - A) S_T(x) / (2 |T|), A and B the edge opposite the vertex),
+    projected on each point's direction and summed into sensors."""
+    L = lib()
+    pts = np.asarray(pts, dtype=np.float64)
+    dirs = np.asarray(dirs, dtype=np.float64)
+    verts = np.array(g.vertices, dtype=np.float64)
+    F = np.zeros((len(pts), 3, len(verts)))
+    for mid, m in enumerate(g.meshes):
+        coeff = MAG_FACTOR * conductivity_jump(g, mid)
+        if coeff == 0.0:
+            continue
+        for t in m.tris:
+            T = verts[t]
+            area = 0.5 * np.linalg.norm(np.cross(T[1] - T[0], T[2] - T[0]))
+            t9 = np.ascontiguousarray(T.reshape(9))
+            for i, x in enumerate(pts):
+                sv = L.orc_operator_s(_p(t9), _p(np.ascontiguousarray(x))) / (2 * area) * coeff
+                for k in range(3):
+                    F[i, :, t[k]] += (T[_EDGE[k][1]] - T[_EDGE[k][0]]) * sv
+    mat = np.zeros((len(pts), g.size))
+    dn = np.linalg.norm(dirs, axis=1)
+    for v in range(len(verts)):
+        if g.vert_index[v] >= 0:
+            mat[:, g.vert_index[v]] = (F[:, :, v] * dirs).sum(axis=1) / dn
+    return _weights_matrix(weights, sensor) @ mat
+
+
+def dip_source2meg(dipoles, pts, dirs, weights, sensor):
+    """DipSource2MEGMat (assembleSensors.cpp:133-156): MagFactor (q x (x - r))
+    . d / (|x - r|^3 |d|), summed into sensors."""
+    dip = np.asarray(dipoles, dtype=np.float64)
+    pts = np.asarray(pts, dtype=np.float64)
+    dirs = np.asarray(dirs, dtype=np.float64)
+    diff = pts[:, None, :] - dip[None, :, :3]
+    nd = np.linalg.norm(diff, axis=2)
+    f = np.cross(np.broadcast_to(dip[None, :, 3:], diff.shape), diff) / (nd ** 3)[..., None]
+    mat = (f * dirs[:, None, :]).sum(axis=2) * MAG_FACTOR / np.linalg.norm(dirs, axis=1)[:, None]
+    return _weights_matrix(weights, sensor) @ mat
+
+
+def gain_meg(H, dsm, h2mm, ds2mm):
+    """GainMEG (gain.h:60-67): Source2MEG + (Head2MEG H^-1) SourceMat."""
+    import scipy.linalg as sla
+    Y = sla.solve(H, h2mm.T, assume_a="sym")
+    return ds2mm + Y.T @ dsm
 
 
 # ---------------------------------------------------------------------------

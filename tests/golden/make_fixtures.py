@@ -11,12 +11,15 @@ What is extracted (all data files that the reference's own tests hold):
   - ``hm.npy`` / ``hm_inv.npy``: packed upper storage of the SymMatrix
     (index ``i + j*(j+1)/2`` for ``i <= j``), length n(n+1)/2;
   - ``dsm.npy`` / ``dgem.npy``: dense matrices (rows x cols);
-  - ``h2em_{data,ir,jc}.npy``: compressed-sparse-column H2EM.
-  plus the Head1 inputs (``data/Head1``: meshes, .geom, .cond, .dip, .patches).
+  - ``h2em_{data,ir,jc}.npy``: compressed-sparse-column H2EM;
+  - ``h2mm.npy`` / ``ds2mm.npy`` / ``dgmm.npy``: the MEG goldens (Head2MEGMat,
+    DipSource2MEGMat, GainMEG with ``Head1.squids``), dense;
+  plus the Head1 inputs (``data/Head1``: meshes, .geom, .cond, .dip, .patches,
+  .squids).
 * ``head3/``: the Head3 inputs (642-vertex spheres, same geometry as the
   0.87/0.92/1.0 nested-sphere configuration), .dip and .patches.
 * ``headnna1/``: the HeadNNa1 inputs (non-nested: the north and south meshes
-  share the cut mesh's vertices), .dip and .patches.
+  share the cut mesh's vertices), .dip, .patches and .squids.
 * ``analytic/``: ``tests/analytic/eeg_head{1,2,3}_analytic.txt`` (multilayer
   sphere series, n_electrodes x 6 dipoles).
 """
@@ -54,7 +57,7 @@ def main():
         assert data.size == n * (n + 1) // 2, (name, n, data.size)
         np.save(os.path.join(h1, name + ".npy"), data)
 
-    for name in ("dsm", "dgem"):
+    for name in ("dsm", "dgem", "h2mm", "ds2mm", "dgmm"):
         path = os.path.join(initial, "Head1." + name)
         raw = h5_dataset(path, "/linop", "<f8")
         # The HDF5 dataset is (ncol, nrow) row-major, i.e. the column-major
@@ -72,7 +75,7 @@ def main():
                 h5_dataset(path, "/matrix/" + part, dt))
 
     data1 = os.path.join(REF, "data", "Head1")
-    for f in ("Head1.geom", "Head1.cond", "Head1.dip", "Head1.patches",
+    for f in ("Head1.geom", "Head1.cond", "Head1.dip", "Head1.patches", "Head1.squids",
               "cortex.1.tri", "skull.1.tri", "scalp.1.tri"):
         shutil.copy(os.path.join(data1, f), h1)
 
@@ -86,7 +89,7 @@ def main():
     hn = os.path.join(HERE, "headnna1")
     os.makedirs(hn, exist_ok=True)
     datan = os.path.join(REF, "data", "HeadNNa1")
-    for f in ("HeadNNa1.geom", "HeadNNa1.cond", "HeadNNa1.dip", "HeadNNa1.patches",
+    for f in ("HeadNNa1.geom", "HeadNNa1.cond", "HeadNNa1.dip", "HeadNNa1.patches", "HeadNNa1.squids",
               "cut.tri", "north.tri", "south.tri", "skull.tri", "scalp.tri"):
         shutil.copy(os.path.join(datan, f), hn)
     an = os.path.join(HERE, "analytic")

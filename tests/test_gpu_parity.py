@@ -81,6 +81,49 @@ def test_nonnested_nna1(om, nna1, monkeypatch):
     assert torch.equal(Hf, Hf.t())
 
 
+def test_meg_head1_matches_golden(om, head1):
+    """MEG forward on Head1 against the reference's goldens with
+    Head1.squids: Head2MEGMat (Ferguson operator on the device), 
+    DipSource2MEGMat, GainMEG (HeadMat inverse) and GainMEGadjoint (stored
+    factor)."""
+    g, dip, _ = head1
+    sq = om.Sensors()
+    sq.load(os.path.join(GOLDEN, "head1", "Head1.squids"))
+    h2mm = om.Head2MEGMat(g, sq)
+    ds2mm = om.DipSource2MEGMat(dip, sq)
+    e_h = rel_fro(h2mm.array(), np.load(os.path.join(GOLDEN, "head1", "h2mm.npy")))
+    e_s = rel_fro(ds2mm.array(), np.load(os.path.join(GOLDEN, "head1", "ds2mm.npy")))
+    hm = om.HeadMat(g)
+    dsm = om.DipSourceMat(g, dip, "Brain")
+    ref = np.load(os.path.join(GOLDEN, "head1", "dgmm.npy"))
+    e_g = rel_fro(om.GainMEG(hm.inverse(), dsm, h2mm, ds2mm).array(), ref)
+    e_a = rel_fro(om.GainMEGadjoint(g, dip, hm, h2mm, ds2mm).array(), ref)
+    print("Head1 MEG: h2mm %.3e, ds2mm %.3e, GainMEG %.3e, GainMEGadjoint %.3e" % (e_h, e_s, e_g, e_a), flush=True)
+    assert e_h < 1e-10 and e_s < 1e-14 and e_g < 1e-10 and e_a < 1e-10
+
+
+def test_meg_nonnested_nna1(om, nna1):
+    """Head2MEGMat and the MEG gain on the non-nested HeadNNa1 (meshes that
+    share vertices: every mesh's Ferguson term lands in the shared vertex
+    columns) against the oracle."""
+    from oracle import oracle as O
+    g, og, dip, _, _ = nna1
+    path = os.path.join(GOLDEN, "headnna1", "HeadNNa1.squids")
+    sq = om.Sensors()
+    sq.load(path)
+    pts, dirs, w, sens = O.read_squids(path)
+    Mref = O.head2meg(og, pts, dirs, w, sens)
+    Sref = O.dip_source2meg(dip, pts, dirs, w, sens)
+    h2mm = om.Head2MEGMat(g, sq)
+    ds2mm = om.DipSource2MEGMat(dip, sq)
+    e_h = rel_fro(h2mm.array(), Mref)
+    H = O.unpack(O.head_mat(og), og.size)
+    Gref = O.gain_meg(H, O.dip_source_mat(og, dip, ""), Mref, Sref)
+    e_g = rel_fro(om.GainMEGadjoint(g, dip, om.HeadMat(g), h2mm, ds2mm).array(), Gref)
+    print("NNa1 MEG: h2mm vs oracle %.3e, gain %.3e" % (e_h, e_g), flush=True)
+    assert e_h < 1e-12 and e_g < 1e-10
+
+
 def test_mfma_lane_layout(om):
     """D(i,j) of one 16x16x4 block sits in lane j + 16*(i % 4), element
     i // 4, for the FP64 instruction and in lane j + 16*(i // 4), element

@@ -250,3 +250,38 @@ def test_single_layer_deflation_nonsingular(golden):
     H = O.unpack(O.head_mat(g), g.size)
     s = np.linalg.svd(H, compute_uv=False)
     assert s.min() > 1e-8 * s.max()
+
+
+def test_oracle_meg_head1_matches_golden():
+    """The oracle's Head2MEGMat (Ferguson operator), DipSource2MEGMat and
+    GainMEG against the reference's Head1 goldens (Head1.{h2mm,ds2mm,dgmm},
+    Head1.squids)."""
+    g = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    pts, dirs, w, sens = O.read_squids(os.path.join(GOLDEN, "head1", "Head1.squids"))
+    h2mm = np.load(os.path.join(GOLDEN, "head1", "h2mm.npy"))
+    ds2mm = np.load(os.path.join(GOLDEN, "head1", "ds2mm.npy"))
+    M = O.head2meg(g, pts, dirs, w, sens)
+    assert M.shape == h2mm.shape
+    assert rel_fro(M, h2mm) < 1e-10
+    dip = O.read_dipoles(os.path.join(GOLDEN, "head1", "Head1.dip"))
+    S = O.dip_source2meg(dip, pts, dirs, w, sens)
+    assert rel_fro(S, ds2mm) < 1e-14
+    H = O.unpack(np.load(os.path.join(GOLDEN, "head1", "hm.npy")), g.size)
+    G = O.gain_meg(H, np.load(os.path.join(GOLDEN, "head1", "dsm.npy")), M, S)
+    assert rel_fro(G, np.load(os.path.join(GOLDEN, "head1", "dgmm.npy"))) < 1e-10
+
+
+def test_oracle_ferguson_sign_and_jump():
+    """Ferguson term checked from the mathematics: a sensor point far from a
+    closed mesh at potential 1 everywhere sees no field (the sum over all
+    vertices of (B - A) S_T(x) / (2|T|) telescopes to zero around every
+    triangle), and the coefficient follows the conductivity jump."""
+    g = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    pts = np.array([[0.3, -0.2, 2.5]])
+    dirs = np.array([[0.0, 0.6, 0.8]])
+    M = O.head2meg(g, pts, dirs, np.ones(1), np.zeros(1, dtype=np.int64))
+    nv = g.nv_unknowns
+    for mid, m in enumerate(g.meshes):
+        cols = g.vert_index[np.array(m.verts)]
+        assert abs(M[0, cols].sum()) <= 1e-12 * np.abs(M[0, :nv]).max()
+        assert O.conductivity_jump(g, mid) != 0.0
