@@ -120,6 +120,37 @@ def gradient_extra(om, S, torch):
     return out
 
 
+def meg_extra(om, S, torch, eeg_seconds=None):
+    """EEG + MEG forward on the benchmark head (c2): one factor and one solve
+    with the 256 electrode rows and 300 tilted magnetometer rows
+    (GainEEGMEGadjoint), mixed16, against the same pipeline in FP64."""
+    meshes, cond, dip, el = S.make_config("c2")
+    g = om.make_nested_geometry(meshes, cond)
+    p, o, w = S.sphere_squids(np.random.default_rng(5), 300)
+    sq = om.Sensors(p, orientations=o, weights=w)
+    ns, nd = 300, len(dip)
+    Ge, Gm, info = om.gain_eeg_meg_pipeline(g, dip, el, sq, precision="mixed16", domain_name="Brain")   # warm
+    del Ge, Gm
+    torch.cuda.synchronize()
+    t0 = time.time()
+    Ge, Gm, info = om.gain_eeg_meg_pipeline(g, dip, el, sq, precision="mixed16", domain_name="Brain")
+    torch.cuda.synchronize()
+    t = time.time() - t0
+    Gm = om.gain_to_numpy(Gm, ns, nd)
+    del Ge
+    torch.cuda.empty_cache()
+    _, G64, _ = om.gain_eeg_meg_pipeline(g, dip, el, sq, precision="fp64", domain_name="Brain")
+    G64 = om.gain_to_numpy(G64, ns, nd)
+    err = float(np.linalg.norm(Gm - G64) / np.linalg.norm(G64))
+    out = {"seconds": round(t, 4), "electrodes": len(el), "magnetometers": ns, "dipoles": nd, "n": g.size,
+           "stages_s": {k: round(v, 4) for k, v in info["times"].items()}, "refine_iterations":
+           info["refine_iterations"], "meg_rel_err_vs_fp64": err}
+    if eeg_seconds:
+        out["added_to_eeg_only_s"] = round(t - eeg_seconds, 4)
+    log("bench: c2 EEG+MEG %s" % out)
+    return out
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -219,6 +250,9 @@ def run(args):
         extras["gradient_c3"] = gradient_extra(om, S, torch)
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
+        extras["eeg_meg_c2"] = meg_extra(om, S, torch, dt)
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -270,10 +304,10 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration) and exit")
-    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient"],
-                    help="gain: the headline measurement (default); batched / gradient: only that extra")
+    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient", "meg"],
+                    help="gain: the headline measurement (default); batched / gradient / meg: only that extra")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched-heads and gradient measurements")
+                    help="skip the batched-heads, gradient and EEG+MEG measurements")
     ap.add_argument("--cpu-rows", type=int, default=1500)
     ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
@@ -296,7 +330,7 @@ def main():
         import openmeeg_amd as om
         from openmeeg_amd import spheres as S
         torch.cuda.set_device(0)
-        fn = batched_extra if args.workload == "batched" else gradient_extra
+        fn = {"batched": batched_extra, "gradient": gradient_extra, "meg": meg_extra}[args.workload]
         print(json.dumps({"workload": args.workload, "result": fn(om, S, torch)}), flush=True)
         return
     out = run(args)

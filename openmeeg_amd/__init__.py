@@ -27,6 +27,7 @@ from .api import (  # noqa: F401
     SparseMatrix,
     SymMatrix,
     factor,
+    gain_eeg_meg_pipeline,
     gain_eeg_pipeline,
     gain_to_numpy,
     make_geometry,

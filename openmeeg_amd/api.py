@@ -715,7 +715,7 @@ class Sensors:
     columns: label x y z dx dy dz w; points with the same label are one
     sensor) -- or Sensors(positions)."""
 
-    def __init__(self, positions=None, geom=None):
+    def __init__(self, positions=None, geom=None, orientations=None, weights=None):
         self.labels = []
         self.positions = np.zeros((0, 3))
         self.orientations = None
@@ -725,6 +725,16 @@ class Sensors:
             a = positions.array() if isinstance(positions, Matrix) else np.asarray(positions, dtype=np.float64)
             self.positions = np.ascontiguousarray(a[:, :3], dtype=np.float64)
             self._plain(len(self.positions))
+        if orientations is not None:
+            o = orientations.array() if isinstance(orientations, Matrix) else np.asarray(orientations, dtype=np.float64)
+            if o.shape != self.positions.shape:
+                raise ValueError("orientations must have one row of 3 per position")
+            self.orientations = np.ascontiguousarray(o, dtype=np.float64)
+        if weights is not None:
+            w = np.asarray(weights, dtype=np.float64).ravel()
+            if len(w) != len(self.positions):
+                raise ValueError("one weight per position")
+            self.weights = w.copy()
 
     def _plain(self, npts):
         self.weights = np.ones(npts)
@@ -925,6 +935,45 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
                 n_electrodes=h2em.nlin(), n_dipoles=nd)
     del F, H, D, Y
     return G, info
+
+
+def gain_eeg_meg_pipeline(geom, dipoles, electrodes, squids, precision="fp64", domain_name="", integrator=None,
+                          dsm_integrator=None, max_refine=10, refine_tol=1e-8, far_field=None):
+    """GainEEGMEGadjoint (gain.h:105-128) on the device: one factor of the
+    HeadMat, one solve with [E^T | M^T] (the Head2EEGMat and Head2MEGMat
+    rows), then G_EEG = Y_E^T DSM and G_MEG = Y_M^T DSM + DipSource2MEG.
+    Returns (G_EEG, G_MEG as device tensors, column-major, info)."""
+    if far_field is None:
+        far_field = precision != "fp64"
+    torch = _t()
+    n = geom.size
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    marks = [("start", ev())]
+    marks[-1][1].record()
+    H = _head_mat_device(geom, integrator, far_field)
+    marks.append(("headmat", ev())); marks[-1][1].record()
+    D = _dsm_device(geom, dipoles, domain_name, dsm_integrator)
+    marks.append(("dsm", ev())); marks[-1][1].record()
+    h2em = Head2EEGMat(geom, electrodes)
+    ne, ns = h2em.nlin(), squids.getNumberOfSensors()
+    M = _head2meg_device(geom, squids)                      # ns x n, triangles-first columns
+    nd = len(_dipoles_array(dipoles))
+    S2 = _ds2meg_device(dipoles, squids)                     # ns x nd
+    Y = torch.cat([_eeg_rhs(geom, h2em).view(ne, n), M[:ns * n].view(n, ns).t()], dim=0).contiguous().view(-1)
+    marks.append(("sensors", ev())); marks[-1][1].record()
+    F = factor(H, n, precision, structure=geom.factor_segments())
+    marks.append(("factor", ev())); marks[-1][1].record()
+    F.solve(Y, ne + ns, max_iter=max_refine, tol=refine_tol)
+    marks.append(("solve", ev())); marks[-1][1].record()
+    Ge = _gemm_tn(Y[:ne * n], D, n, ne, nd)
+    Gm = S2[:ns * nd].clone()
+    check(lib().omh_gemm_f64(1, 0, ns, nd, n, 1.0, _ptr(Y[ne * n:]), n, _ptr(D), n, 1.0, _ptr(Gm), ns, _stream()))
+    marks.append(("gain", ev())); marks[-1][1].record()
+    torch.cuda.synchronize()
+    times = {b: ea.elapsed_time(eb) / 1e3 for (a, ea), (b, eb) in zip(marks[:-1], marks[1:])}
+    info = dict(times=times, refine_iterations=F.iterations, n=n, n_electrodes=ne, n_squids=ns, n_dipoles=nd)
+    del F, H, D, Y
+    return Ge, Gm, info
 
 
 def gain_to_numpy(G, ne, nd):

@@ -2015,9 +2015,9 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     // product of depth SOLVE_GROUP x NB.
     constexpr int64_t GW = (int64_t)SOLVE_GROUP * FACT_NB;
     std::vector<std::pair<int64_t, int64_t>> iv;
-    // strips of 2 right-hand sides up to 512 of them (>= 128 workgroups
-    // for 256), else 8
-    const bool narrow = r <= 512;
+    // strips of 2 right-hand sides up to 1024 of them (128 workgroups for
+    // 256; 8-wide strips took twice as long per group at 256), else 8
+    const bool narrow = r <= 1024;
     auto group_solve = [&](bool back, int64_t k0, int64_t gend) {
         if (narrow) {
             const unsigned g = (unsigned)cdiv(r, 2);

@@ -161,6 +161,20 @@ def sphere_electrodes(rng, n, radius=1.0):
     return radius * e / np.linalg.norm(e, axis=1, keepdims=True)
 
 
+def sphere_squids(rng, n, radius=1.15, tilt=0.5):
+    """n MEG magnetometer points on a sphere outside the head: positions,
+    unit orientations (radial plus `tilt` times a random tangential
+    direction: a purely radial field would not see the volume currents of a
+    spherical head), weights 1."""
+    p = sphere_electrodes(rng, n, radius)
+    r = p / np.linalg.norm(p, axis=1, keepdims=True)
+    t = rng.standard_normal((n, 3))
+    t -= (t * r).sum(axis=1, keepdims=True) * r
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    o = r + tilt * t
+    return p, o / np.linalg.norm(o, axis=1, keepdims=True), np.ones(n)
+
+
 CONFIGS = {
     "c0": dict(freq=8, radii=(0.87, 0.92, 1.0), cond=(1.0, 0.0125, 1.0), n_dip=100,
                dip="uniform", rmax=0.8, n_elec=64, seed=0),

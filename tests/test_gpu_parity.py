@@ -124,6 +124,43 @@ def test_meg_nonnested_nna1(om, nna1):
     assert e_h < 1e-12 and e_g < 1e-10
 
 
+def test_eeg_meg_pipeline(om, c0_oracle):
+    """The combined device pipeline (GainEEGMEGadjoint, gain.h:105-128: one
+    factor, one solve with the EEG and MEG rows) on the 642-vertex head with
+    80 tilted magnetometers: its EEG gain equals gain_eeg_pipeline's (1e-12),
+    its MEG gain GainMEGadjoint's from the reference-API matrices (1e-10);
+    mixed16 within 1e-6 of FP64.  Independent of the reference: radial
+    magnetometers around a spherical head see no field from the volume
+    currents (Sarvas), so their MEG gain is the dipoles' own field up to the
+    discretisation (here 5%)."""
+    from openmeeg_amd import spheres as S
+    c = c0_oracle
+    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    rng = np.random.default_rng(11)
+    p, o, w = S.sphere_squids(rng, 80)
+    sq = om.Sensors(p, orientations=o, weights=w)
+    dip, el = c["dip"], c["el"]
+    ne, ns, nd = len(el), 80, len(dip)
+    Ge, Gm, info = om.gain_eeg_meg_pipeline(g, dip, el, sq, precision="fp64", domain_name="Brain")
+    Ge, Gm = om.gain_to_numpy(Ge, ne, nd), om.gain_to_numpy(Gm, ns, nd)
+    G0, _ = om.gain_eeg_pipeline(g, dip, el, precision="fp64", domain_name="Brain")
+    e_eeg = rel_fro(Ge, om.gain_to_numpy(G0, ne, nd))
+    h2mm, ds2mm = om.Head2MEGMat(g, sq), om.DipSource2MEGMat(dip, sq)
+    Gref = om.GainMEGadjoint(g, dip, om.HeadMat(g), h2mm, ds2mm).array()
+    e_meg = rel_fro(Gm, Gref)
+    _, Gm16, info16 = om.gain_eeg_meg_pipeline(g, dip, el, sq, precision="mixed16", domain_name="Brain")
+    e_mix = rel_fro(om.gain_to_numpy(Gm16, ns, nd), Gm)
+    pr, orad, _ = S.sphere_squids(np.random.default_rng(12), 40, tilt=0.0)
+    sr = om.Sensors(pr, orientations=orad)
+    _, Gr, _ = om.gain_eeg_meg_pipeline(g, dip, el, sr, precision="fp64", domain_name="Brain")
+    Gr = om.gain_to_numpy(Gr, 40, nd)
+    e_rad = rel_fro(Gr, om.DipSource2MEGMat(dip, sr).array())
+    print("EEG+MEG c0: EEG vs EEG pipeline %.3e, MEG vs GainMEGadjoint %.3e, mixed16 vs fp64 %.3e (%d "
+          "refinements), radial sensors vs dipole field %.3e" % (e_eeg, e_meg, e_mix, info16["refine_iterations"],
+                                                                  e_rad), flush=True)
+    assert e_eeg < 1e-12 and e_meg < 1e-10 and e_mix < 1e-6 and e_rad < 0.05
+
+
 def test_mfma_lane_layout(om):
     """D(i,j) of one 16x16x4 block sits in lane j + 16*(i % 4), element
     i // 4, for the FP64 instruction and in lane j + 16*(i // 4), element
