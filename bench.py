@@ -151,6 +151,75 @@ def meg_extra(om, S, torch, eeg_seconds=None):
     return out
 
 
+def surf_extra(om, S, torch, eeg_seconds=None, cpu=True):
+    """Surface sources on the benchmark head (c2): SurfSourceMat of a
+    10242-vertex source sphere (radius 0.7, inside the brain) and the
+    surface-source EEG gain pipeline (256 electrodes), mixed16 against the
+    same pipeline in FP64.  CPU: the oracle's SurfSourceMat on a 642-vertex
+    source sphere against the same head, scaled by the source triangle count
+    (the S, N and D work is proportional to it)."""
+    from openmeeg_amd import _lib
+    from openmeeg_amd.api import _ssm_device
+    L = _lib.lib()
+    meshes, cond, dip, el = S.make_config("c2")
+    g = om.make_nested_geometry(meshes, cond)
+    P, T = S.geodesic_sphere(32)
+    mesh = om.Mesh(0.7 * P, T)
+    ne, nsrc = len(el), len(P)
+    D = _ssm_device(g, mesh)                                        # warm
+    del D
+    torch.cuda.synchronize()
+    t0 = time.time()
+    D = _ssm_device(g, mesh)
+    torch.cuda.synchronize()
+    t_ssm = time.time() - t0
+    del D
+    # the same call again with the per-kernel event timers on
+    _lib.check(L.omh_profile_enable(1))
+    D = _ssm_device(g, mesh)
+    torch.cuda.synchronize()
+    del D
+    kern = {}
+    for k in ("s_block", "d_pairs"):
+        import ctypes as C
+        calls, sec, work = C.c_int64(), C.c_double(), C.c_double()
+        _lib.check(L.omh_profile_read(k.encode(), C.byref(calls), C.byref(sec), C.byref(work)))
+        if calls.value:
+            kern[k] = {"launches": calls.value, "seconds": round(sec.value, 4),
+                       "gauss_evaluations_per_s": work.value / sec.value}
+    _lib.check(L.omh_profile_enable(0))
+    Ge, _, info = om.gain_eeg_surf_pipeline(g, mesh, el, precision="mixed16")   # warm
+    del Ge
+    torch.cuda.synchronize()
+    t0 = time.time()
+    Ge, _, info = om.gain_eeg_surf_pipeline(g, mesh, el, precision="mixed16")
+    torch.cuda.synchronize()
+    t = time.time() - t0
+    Ge = om.gain_to_numpy(Ge, ne, nsrc)
+    torch.cuda.empty_cache()
+    G64, _, info64 = om.gain_eeg_surf_pipeline(g, mesh, el, precision="fp64")
+    G64 = om.gain_to_numpy(G64, ne, nsrc)
+    err = float(np.linalg.norm(Ge - G64) / np.linalg.norm(G64))
+    out = {"ssm_seconds": round(t_ssm, 4), "ssm_kernels": kern, "pipeline_seconds": round(t, 4),
+           "source_vertices": nsrc, "source_triangles": len(T), "electrodes": ne, "n": g.size,
+           "stages_s": {k: round(v, 4) for k, v in info["times"].items()},
+           "fp64_ssm_seconds": round(info64["times"]["ssm"], 4),
+           "refine_iterations": info["refine_iterations"], "rel_err_vs_fp64": err}
+    if cpu:
+        from oracle import oracle as O
+        og = O.nested_geometry(meshes, cond)
+        Ps, Ts = S.geodesic_sphere(8)
+        st = {}
+        O.surf_source_mat(og, 0.7 * Ps, Ts, stats=st)
+        wall = st["pair_seconds"]
+        out["cpu_ssm"] = {"value": round(wall * len(T) / len(Ts), 1), "unit": "s", "cores": 1, "kind": "port",
+                          "sample": "oracle SurfSourceMat (its C pair assembly, S N D, without the Python table "
+                                    "set-up) with a %d-vertex source sphere against the same head (%.2f s), scaled "
+                                    "by the source triangle count %d/%d" % (len(Ps), wall, len(T), len(Ts))}
+    log("bench: c2 surface sources %s" % out)
+    return out
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -253,6 +322,9 @@ def run(args):
         extras["eeg_meg_c2"] = meg_extra(om, S, torch, dt)
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
+        extras["surface_sources_c2"] = surf_extra(om, S, torch, cpu=not args.no_cpu_baseline)
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -304,10 +376,10 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration) and exit")
-    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient", "meg"],
-                    help="gain: the headline measurement (default); batched / gradient / meg: only that extra")
+    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient", "meg", "surf"],
+                    help="gain: the headline measurement (default); batched / gradient / meg / surf: only that extra")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched-heads, gradient and EEG+MEG measurements")
+                    help="skip the batched-heads, gradient, EEG+MEG and surface-source measurements")
     ap.add_argument("--cpu-rows", type=int, default=1500)
     ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
@@ -330,7 +402,8 @@ def main():
         import openmeeg_amd as om
         from openmeeg_amd import spheres as S
         torch.cuda.set_device(0)
-        fn = {"batched": batched_extra, "gradient": gradient_extra, "meg": meg_extra}[args.workload]
+        fn = {"batched": batched_extra, "gradient": gradient_extra, "meg": meg_extra,
+              "surf": surf_extra}[args.workload]
         print(json.dumps({"workload": args.workload, "result": fn(om, S, torch)}), flush=True)
         return
     out = run(args)

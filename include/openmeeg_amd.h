@@ -28,7 +28,12 @@ extern "C" {
 #define OMH_ORDER_REFERENCE 0
 #define OMH_ORDER_TRIANGLES_FIRST 1
 
+/* Status returned when a source mesh intersects the geometry
+ * (OverlappingSourceMesh, OMExceptions.H). */
+#define OMH_STATUS_OVERLAPPING_SOURCE_MESH 5
+
 typedef struct omh_geometry omh_geometry;
+typedef struct omh_mesh omh_mesh;
 
 /* Message of the last failed call on this thread (never NULL). */
 const char* omh_last_error(void);
@@ -150,6 +155,41 @@ int omh_head2eeg_mat(omh_geometry* g, const double* d_electrodes, int64_t n_el, 
  * into sensors with Sensors::getWeightsMatrix (sensors.h:191-196). */
 int omh_head2meg_mat(omh_geometry* g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
                      int64_t ld, int32_t ordering, void* stream);
+
+/* ------------------------------------------------------- surface sources */
+
+/* Mesh(filename) of a distributed source surface (mesh.h, Mesh::load
+ * mesh.cpp:255-265; vertices x y z, 0-based triangles, both row-major).
+ * Vertex i has index i (Mesh::generate_indices, mesh.cpp:266-272); vertices
+ * are not de-duplicated.  The device tables are built on the first device
+ * use and kept with the handle. */
+int omh_mesh_create(const double* vertices, int64_t n_vertices, const int64_t* triangles, int64_t n_triangles,
+                    omh_mesh** out);
+void omh_mesh_free(omh_mesh* m);
+/* out[0] = vertices, out[1] = triangles. */
+int omh_mesh_sizes(const omh_mesh* m, int64_t out[2]);
+
+/* Geometry::check(mesh) (geometry.cpp:329-343): *intersects = 1 when a
+ * triangle of the mesh meets a triangle of the geometry (Mesh::intersection,
+ * mesh.cpp:246-252), else 0.  Mesh::has_self_intersection is not part of
+ * it: its test never reports a pair there (mesh.cpp:227-237). */
+int omh_mesh_intersects(omh_geometry* g, omh_mesh* m, int32_t* intersects, void* stream);
+
+/* SurfSourceMat(geo, sources, Integrator(order,0,.)) (assembleSourceMat.cpp:
+ * 22-56): d_out (n x n_vertices of the mesh, column-major, leading dimension
+ * ld >= n) with rows in the unknown order `ordering`.  Returns
+ * OMH_STATUS_OVERLAPPING_SOURCE_MESH when the mesh meets the geometry.
+ * flags: OMH_HEADMAT_FAR_FIELD as for omh_head_mat_ex.  The D block of a
+ * current-barrier boundary mesh is left out (its triangles have no rows). */
+int omh_surf_source_mat(omh_geometry* g, omh_mesh* m, int order, int flags, double* d_out, int64_t ld,
+                        int32_t ordering, void* stream);
+
+/* SurfSource2MEGMat (assembleSensors.cpp:102-124) before the sensor weights:
+ * d_out (npts x n_vertices, column-major, leading dimension ld >= npts) =
+ * the mesh's Ferguson operator (coefficient 1) at each point along its
+ * direction (d_pts, d_dirs: npts x 3 row-major). */
+int omh_surf_source2meg_mat(omh_mesh* m, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
+                            int64_t ld, void* stream);
 
 /* DipSource2MEGMat (assembleSensors.cpp:133-156) before the sensor weights:
  * d_out (npts x n_dip, column-major) = MagFactor (q x (x - r)) . d /

@@ -23,12 +23,17 @@ from .api import (  # noqa: F401
     HeadMat,
     Integrator,
     Matrix,
+    Mesh,
     Sensors,
     SparseMatrix,
+    SurfSource2MEGMat,
+    SurfSourceMat,
     SymMatrix,
     factor,
     gain_eeg_meg_pipeline,
     gain_eeg_pipeline,
+    gain_eeg_surf_pipeline,
+    mesh_intersects,
     gain_to_numpy,
     make_geometry,
     make_nested_geometry,
@@ -36,5 +41,6 @@ from .api import (  # noqa: F401
 )
 
 __version__ = "0.1.0"
+from ._lib import OverlappingSourceMesh  # noqa: F401,E402
 from .differentiable import EEGForward  # noqa: F401,E402
 from .batch import BatchedGain  # noqa: F401,E402

@@ -34,6 +34,12 @@ _SIGS = {
     "omh_head2eeg_mat": ([vp, vp, i64, p_i64, p_f64, vp], C.c_int),
     "omh_head2meg_mat": ([vp, vp, vp, i64, vp, i64, i32, vp], C.c_int),
     "omh_dip_source2meg_mat": ([vp, i64, vp, vp, i64, vp, i64, vp], C.c_int),
+    "omh_mesh_create": ([p_f64, i64, p_i64, i64, C.POINTER(vp)], C.c_int),
+    "omh_mesh_free": ([vp], None),
+    "omh_mesh_sizes": ([vp, p_i64], C.c_int),
+    "omh_mesh_intersects": ([vp, vp, p_i32, vp], C.c_int),
+    "omh_surf_source_mat": ([vp, vp, C.c_int, C.c_int, vp, i64, i32, vp], C.c_int),
+    "omh_surf_source2meg_mat": ([vp, vp, vp, i64, vp, i64, vp], C.c_int),
     "omh_dip_source_adjoint": ([vp, vp, i64, p_i32, C.c_int, vp, i64, C.c_int, vp, vp], C.c_int),
     "omh_geometry_domain_conductivity": ([vp, i32, p_f64], C.c_int),
     "omh_geometry_set_conductivity": ([vp, i32, f64], C.c_int),
@@ -101,6 +107,13 @@ class OpenMEEGError(RuntimeError):
     pass
 
 
+class OverlappingSourceMesh(OpenMEEGError):
+    """OverlappingSourceMesh (OMExceptions.H): a source mesh meets the geometry."""
+
+
+STATUS_OVERLAPPING_SOURCE_MESH = 5
+
+
 def check(rc):
     if rc == 0:
         return
@@ -109,4 +122,6 @@ def check(rc):
         raise ValueError(msg)
     if rc == 4:
         raise MemoryError(msg)
+    if rc == STATUS_OVERLAPPING_SOURCE_MESH:
+        raise OverlappingSourceMesh(msg)
     raise OpenMEEGError(msg)

@@ -855,6 +855,114 @@ def DipSource2MEGMat(dipoles, sensors):
     return Matrix(M.cpu().numpy()[:ns * nd].reshape(nd, ns).T.copy(order="F"))
 
 
+class _VertexList(list):
+    def size(self):
+        return len(self)
+
+
+def _read_tri(path):
+    """tri.h:26-56: '- N' then N lines 'x y z nx ny nz', '- T T T' then T
+    lines of 0-based vertex indices."""
+    with open(path) as f:
+        tok = f.read().split()
+    if not tok or tok[0] != "-":
+        raise ValueError("%s: not a .tri mesh" % path)
+    nv = int(tok[1])
+    vals = np.array(tok[2:2 + 6 * nv], dtype=np.float64).reshape(nv, 6)
+    pos = 2 + 6 * nv
+    if tok[pos] != "-":
+        raise ValueError("%s: not a .tri mesh" % path)
+    nt = int(tok[pos + 1])
+    tris = np.array(tok[pos + 4:pos + 4 + 3 * nt], dtype=np.int64).reshape(nt, 3)
+    return vals[:, :3].copy(), tris
+
+
+class Mesh:
+    """Mesh (mesh.h) of a distributed source surface: Mesh(filename) for a
+    .tri file (Mesh::load, mesh.cpp:255-265), or Mesh(vertices, triangles)
+    with 0-based triangles.  Vertex i has index i (Mesh::generate_indices,
+    mesh.cpp:266-272)."""
+
+    def __init__(self, vertices=None, triangles=None):
+        if isinstance(vertices, (str, os.PathLike)):
+            if not str(vertices).endswith(".tri"):
+                raise ValueError("only .tri source meshes are read")
+            vertices, triangles = _read_tri(vertices)
+        self._v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 3))
+        self._t = np.ascontiguousarray(np.asarray(triangles, dtype=np.int64).reshape(-1, 3))
+        h = C.c_void_p()
+        check(lib().omh_mesh_create(self._v.ctypes.data_as(_lib.p_f64), len(self._v),
+                                    self._t.ctypes.data_as(_lib.p_i64), len(self._t), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().omh_mesh_free(h)
+            self._h = None
+
+    def vertices(self):
+        return _VertexList(map(tuple, self._v))
+
+    def triangles(self):
+        return self._t.copy()
+
+    def nb_vertices(self):
+        return len(self._v)
+
+    def nb_triangles(self):
+        return len(self._t)
+
+
+def mesh_intersects(geom, mesh):
+    """Geometry::check(mesh) (geometry.cpp:329-343): True when a triangle of
+    the source mesh meets a triangle of the geometry."""
+    out = C.c_int32()
+    check(lib().omh_mesh_intersects(geom._h, mesh._h, C.byref(out), _stream()))
+    return bool(out.value)
+
+
+def _ssm_device(geom, mesh, integrator=None, ordering=ORDER_TRIANGLES_FIRST, far_field=False):
+    integ = integrator or Integrator(3, 0, 0.005)
+    if integ.levels != 0:
+        raise ValueError("adaptive SurfSourceMat integration is not supported on this path (levels must be 0)")
+    n, ns = geom.size, mesh.nb_vertices()
+    D = _dev_f64(n * ns)
+    check(lib().omh_surf_source_mat(geom._h, mesh._h, integ.order, HEADMAT_FAR_FIELD if far_field else 0, _ptr(D),
+                                    n, ordering, _stream()))
+    return D
+
+
+def SurfSourceMat(geom, mesh, integrator=None):
+    """SurfSourceMat(geo, sources, Integrator(3,0,0.005)) --
+    assembleSourceMat.h:20, assembleSourceMat.cpp:22-56: n x (source
+    vertices), rows in the reference unknown order.  Raises
+    OverlappingSourceMesh when the mesh meets the geometry."""
+    if isinstance(mesh, (str, os.PathLike)):
+        mesh = Mesh(mesh)
+    D = _ssm_device(geom, mesh, integrator, ORDER_REFERENCE)
+    n, ns = geom.size, mesh.nb_vertices()
+    return Matrix(D.cpu().numpy().reshape(ns, n).T.copy(order="F"))
+
+
+def _ss2meg_device(mesh, sensors):
+    d_pts, d_dirs = _meg_points(sensors)
+    npts, ns = sensors.getNumberOfPositions(), mesh.nb_vertices()
+    P = _dev_f64(max(1, npts * ns))
+    check(lib().omh_surf_source2meg_mat(mesh._h, _ptr(d_pts), _ptr(d_dirs), npts, _ptr(P), max(1, npts), _stream()))
+    return _sensor_sum(sensors, P, ns)
+
+
+def SurfSource2MEGMat(mesh, sensors):
+    """SurfSource2MEGMat(sources, sensors) -- assemble.h:39,
+    assembleSensors.cpp:102-124."""
+    if isinstance(mesh, (str, os.PathLike)):
+        mesh = Mesh(mesh)
+    nsens, ns = sensors.getNumberOfSensors(), mesh.nb_vertices()
+    M = _ss2meg_device(mesh, sensors)
+    return Matrix(M.cpu().numpy()[:nsens * ns].reshape(ns, nsens).T.copy(order="F"))
+
+
 def GainMEG(hminv, dsm, h2mm, ds2mm):
     """GainMEG(HeadMatInv, SourceMat, Head2MEGMat, Source2MEGMat) =
     Source2MEG + (Head2MEG H^-1) SourceMat -- gain.h:60-67."""
@@ -972,6 +1080,57 @@ def gain_eeg_meg_pipeline(geom, dipoles, electrodes, squids, precision="fp64", d
     torch.cuda.synchronize()
     times = {b: ea.elapsed_time(eb) / 1e3 for (a, ea), (b, eb) in zip(marks[:-1], marks[1:])}
     info = dict(times=times, refine_iterations=F.iterations, n=n, n_electrodes=ne, n_squids=ns, n_dipoles=nd)
+    del F, H, D, Y
+    return Ge, Gm, info
+
+
+def gain_eeg_surf_pipeline(geom, mesh, electrodes, squids=None, precision="fp64", integrator=None, max_refine=10,
+                           refine_tol=1e-8, far_field=None):
+    """The surface-source gains on the device: SurfSourceMat of the source
+    mesh, the HeadMat factor, one solve with the Head2EEGMat rows (and the
+    Head2MEGMat rows when squids are given), then G_EEG = Y_E^T SSM
+    (= GainEEG(H^-1, SSM, H2EM), gain.h:69-76) and G_MEG = Y_M^T SSM +
+    SurfSource2MEGMat (= GainMEG, gain.h:60-67).  Returns (G_EEG, G_MEG or
+    None, info); gains are device tensors, column-major (sensors x source
+    vertices)."""
+    if far_field is None:
+        far_field = precision != "fp64"
+    if isinstance(mesh, (str, os.PathLike)):
+        mesh = Mesh(mesh)
+    torch = _t()
+    n, nsrc = geom.size, mesh.nb_vertices()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    marks = [("start", ev())]
+    marks[-1][1].record()
+    H = _head_mat_device(geom, integrator, far_field)
+    marks.append(("headmat", ev())); marks[-1][1].record()
+    D = _ssm_device(geom, mesh, integrator, ORDER_TRIANGLES_FIRST, far_field)
+    marks.append(("ssm", ev())); marks[-1][1].record()
+    h2em = Head2EEGMat(geom, electrodes)
+    ne = h2em.nlin()
+    Y = _eeg_rhs(geom, h2em)
+    ns = 0
+    if squids is not None:
+        ns = squids.getNumberOfSensors()
+        M = _head2meg_device(geom, squids)                   # ns x n, triangles-first columns
+        S2 = _ss2meg_device(mesh, squids)                    # ns x nsrc
+        Y = torch.cat([Y.view(ne, n), M[:ns * n].view(n, ns).t()], dim=0).contiguous().view(-1)
+    marks.append(("sensors", ev())); marks[-1][1].record()
+    F = factor(H, n, precision, structure=geom.factor_segments())
+    marks.append(("factor", ev())); marks[-1][1].record()
+    F.solve(Y, ne + ns, max_iter=max_refine, tol=refine_tol)
+    marks.append(("solve", ev())); marks[-1][1].record()
+    Ge = _gemm_tn(Y[:ne * n], D, n, ne, nsrc)
+    Gm = None
+    if ns:
+        Gm = S2[:ns * nsrc].clone()
+        check(lib().omh_gemm_f64(1, 0, ns, nsrc, n, 1.0, _ptr(Y[ne * n:]), n, _ptr(D), n, 1.0, _ptr(Gm), ns,
+                                 _stream()))
+    marks.append(("gain", ev())); marks[-1][1].record()
+    torch.cuda.synchronize()
+    times = {b: ea.elapsed_time(eb) / 1e3 for (a, ea), (b, eb) in zip(marks[:-1], marks[1:])}
+    info = dict(times=times, refine_iterations=F.iterations, n=n, n_electrodes=ne, n_squids=ns,
+                n_sources=nsrc)
     del F, H, D, Y
     return Ge, Gm, info
 

@@ -3,6 +3,8 @@
 // plus a per-thread message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -22,6 +24,10 @@ struct omh_geometry {
     std::unique_ptr<omh::Geometry> g;
 };
 
+struct omh_mesh {
+    std::unique_ptr<omh::Geometry> g;   // one-mesh geometry (make_source_mesh)
+};
+
 struct omh_batch {
     std::unique_ptr<omh::BatchPlan> plan;
 };
@@ -35,6 +41,9 @@ int guard(F&& f) {
     try {
         f();
         return 0;
+    } catch (const omh::OverlappingSourceMesh& e) {
+        g_err = e.what();
+        return OMH_STATUS_OVERLAPPING_SOURCE_MESH;
     } catch (const std::invalid_argument& e) {
         g_err = std::string("invalid argument: ") + e.what();
         return 2;
@@ -425,6 +434,69 @@ int omh_head2meg_mat(omh_geometry* g, const double* d_pts, const double* d_dirs,
         omh::Layout L = layout_of(*g->g, g->g->size(), ordering);
         L.ld = ld;
         omh::head2meg(*g->g, d_pts, d_dirs, npts, d_out, ld, L, S(stream));
+    });
+}
+
+int omh_mesh_create(const double* vertices, int64_t n_vertices, const int64_t* triangles, int64_t n_triangles,
+                    omh_mesh** out) {
+    return guard([&] {
+        need(out, "out");
+        if (n_vertices <= 0 || n_triangles <= 0) throw std::invalid_argument("empty mesh");
+        need(vertices, "vertices");
+        need(triangles, "triangles");
+        std::vector<std::array<double, 3>> pts(n_vertices);
+        for (int64_t i = 0; i < n_vertices; ++i) pts[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        std::vector<std::array<int64_t, 3>> tris(n_triangles);
+        for (int64_t i = 0; i < n_triangles; ++i) tris[i] = {triangles[3 * i], triangles[3 * i + 1], triangles[3 * i + 2]};
+        auto h = std::make_unique<omh_mesh>();
+        h->g = omh::make_source_mesh(pts, tris);
+        *out = h.release();
+    });
+}
+
+void omh_mesh_free(omh_mesh* m) { delete m; }
+
+int omh_mesh_sizes(const omh_mesh* m, int64_t out[2]) {
+    return guard([&] {
+        need(m, "mesh");
+        need(out, "out");
+        out[0] = (int64_t)m->g->vertices.size();
+        out[1] = (int64_t)m->g->meshes[0].tris.size();
+    });
+}
+
+int omh_mesh_intersects(omh_geometry* g, omh_mesh* m, int32_t* intersects, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(m, "mesh");
+        need(intersects, "intersects");
+        *intersects = omh::meshes_intersect(*g->g, *m->g, S(stream)) ? 1 : 0;
+    });
+}
+
+int omh_surf_source_mat(omh_geometry* g, omh_mesh* m, int order, int flags, double* d_out, int64_t ld,
+                        int32_t ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(m, "mesh");
+        need(d_out, "d_out");
+        if (flags & ~OMH_HEADMAT_FAR_FIELD) throw std::invalid_argument("unknown flags");
+        omh::surf_source_mat(*g->g, *m->g, order, d_out, layout_of(*g->g, ld, ordering), S(stream),
+                             (flags & OMH_HEADMAT_FAR_FIELD) != 0);
+    });
+}
+
+int omh_surf_source2meg_mat(omh_mesh* m, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
+                            int64_t ld, void* stream) {
+    return guard([&] {
+        need(m, "mesh");
+        need(d_out, "d_out");
+        if (npts < 0 || ld < std::max<int64_t>(1, npts)) throw std::invalid_argument("bad sizes");
+        if (npts > 0) {
+            need(d_pts, "d_pts");
+            need(d_dirs, "d_dirs");
+        }
+        omh::surf2meg(*m->g, d_pts, d_dirs, npts, d_out, ld, S(stream));
     });
 }
 

@@ -275,6 +275,7 @@ namespace {
 #endif
 
 constexpr int TILE = 16;
+constexpr double kK = 1.0 / (4.0 * 3.14159265358979323846);   // K, constants.h:23
 
 __device__ __forceinline__ int64_t cdiv_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -290,12 +291,15 @@ __device__ __forceinline__ void upper_tile(int64_t t, int64_t& lo, int64_t& hi) 
 // S block: value(t1,t2) = coeff * integral over t2 (Gauss) of OperatorS(t1).
 // TEMP: writes the dense temporary block tmp[i + j*n1] (and its mirror for
 // diagonal blocks) -- the SymBloc/Bloc of operators.h:159-173,337-352.
+// Row triangles come from (tri, gp), column triangles from (triB, gpB): the
+// same tables for a HeadMat pair, a separate source mesh's for SurfSourceMat.
 template <bool TEMP>
 __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
                                                  int64_t b0, int64_t n2, int diag, int64_t ntx, int order,
                                                  double coeff, double* __restrict__ H, Layout L,
                                                  int64_t ua0, int64_t ub0, double* __restrict__ tmp,
-                                                 const double* __restrict__ gp, int far) {
+                                                 const double* __restrict__ gp, int far,
+                                                 const TriRec* __restrict__ triB, const double* __restrict__ gpB) {
     int64_t ti, tj;
     if (diag) {
         upper_tile(blockIdx.x, ti, tj);
@@ -308,14 +312,14 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRe
     if (i >= n1 || j >= n2) return;
     if (diag && j < i) return;
     const TriRec& t1 = tri[a0 + i];
-    const TriRec& t2 = tri[b0 + j];
+    const TriRec& t2 = triB[b0 + j];
     int fc = 0;
     if (far && order == 3) {
-        fc = far_class(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
+        fc = far_class(gp + (a0 + i) * GP_STRIDE, gpB + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
     }
     double res = 0.0;
     if (fc) {   // Gauss on both triangles (outer: column triangle, inner: row triangle)
-        res = t1.area * s_far(gp + (b0 + j) * GP_STRIDE, gp + (a0 + i) * GP_STRIDE, fc);
+        res = t1.area * s_far(gpB + (b0 + j) * GP_STRIDE, gp + (a0 + i) * GP_STRIDE, fc);
     } else {
         const OpS o = load_ops(t1);
         const d3 P[3] = {ld3(t2.p[0]), ld3(t2.p[1]), ld3(t2.p[2])};
@@ -334,24 +338,26 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_s_block(const TriRe
 }
 
 // D pass 1: W[(k*n2 + j)*n1c + i] = integral over row triangle (a0+i)
-// (Gauss) of P1SLP(column triangle b0+j)_k  (operators.h:103-106).
+// (Gauss) of P1SLP(column triangle b0+j)_k  (operators.h:103-106).  Row
+// triangles from (tri, gp), column triangles from (triB, gpB).
 __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_d_pairs(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
                                                  int64_t b0, int64_t n2, int order, double* __restrict__ W,
-                                                 const double* __restrict__ gp, int far) {
+                                                 const double* __restrict__ gp, int far,
+                                                 const TriRec* __restrict__ triB, const double* __restrict__ gpB) {
     const int64_t ni = cdiv_dev(n1c, 64);
     const int64_t tj = blockIdx.x / ni, tb = blockIdx.x - tj * ni;
     const int64_t i = tb * 64 + threadIdx.x % 64;
     const int64_t j = tj * 4 + threadIdx.x / 64;
     if (i >= n1c || j >= n2) return;
     const TriRec& t1 = tri[a0 + i];
-    const TriRec& t2 = tri[b0 + j];
+    const TriRec& t2 = triB[b0 + j];
     int fc = 0;
     if (far && order == 3) {
-        fc = far_class(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
+        fc = far_class(gp + (a0 + i) * GP_STRIDE, gpB + (b0 + j) * GP_STRIDE, fmax(t1.diam, t2.diam));
     }
     double rx = 0.0, ry = 0.0, rz = 0.0;
     if (fc) {   // Gauss on both triangles (outer: row triangle, inner: column triangle)
-        const d3 v = d_far(gp + (a0 + i) * GP_STRIDE, gp + (b0 + j) * GP_STRIDE, ld3(t2.n), fc);
+        const d3 v = d_far(gp + (a0 + i) * GP_STRIDE, gpB + (b0 + j) * GP_STRIDE, ld3(t2.n), fc);
         rx = t2.area * v.x;
         ry = t2.area * v.y;
         rz = t2.area * v.z;
@@ -374,26 +380,30 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_d_pairs(const TriRe
 }
 
 // D pass 2: H(row t1, vertex v) += sum over the triangles t2 of v (mesh
-// order) of W_k(t1,t2) * coeff.
+// order) of W_k(t1,t2) * coeff.  rect: the column is the vertex's own index
+// (a source mesh's local vertex, not an unknown of L) and the mirror entry
+// is not written (SurfSourceMat's n x nsrc matrix).
 __global__ void __launch_bounds__(256) k_d_gather(const double* __restrict__ W, int64_t n1c, int64_t n2,
                                                   int64_t b0, const int64_t* __restrict__ adj_off,
                                                   const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
                                                   const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
-                                                  double coeff, double* __restrict__ H, Layout L, int64_t urow0) {
+                                                  double coeff, double* __restrict__ H, Layout L, int64_t urow0,
+                                                  int rect) {
     const int64_t ni = cdiv_dev(n1c, 64);
     const int64_t ts = blockIdx.x / ni, tb = blockIdx.x - ts * ni;
     const int64_t i = tb * 64 + threadIdx.x % 64;
     const int64_t s = ts * 4 + threadIdx.x / 64;
     if (i >= n1c || s >= ns) return;
     const int64_t r = dev_index(L, urow0 + i);
-    const int64_t c = dev_index(L, vert_unk[slot_vid[s0 + s]]);
+    const int64_t vu = vert_unk[slot_vid[s0 + s]];
+    const int64_t c = rect ? vu : dev_index(L, vu);
     double acc = H[r + c * L.ld];
     for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
         const AdjRec e = adj[p];
         acc += W[((int64_t)e.k * n2 + (e.tri - b0)) * n1c + i] * coeff;
     }
     H[r + c * L.ld] = acc;
-    H[c + r * L.ld] = acc;
+    if (!rect) H[c + r * L.ld] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -711,18 +721,23 @@ __global__ void __launch_bounds__(256) k_n_stage1(const TriRec* __restrict__ tri
 // columns b >= a.  Only the lower-triangle entry of each vertex pair is
 // accumulated (k_mirror_lower fills the upper one once the V x V block is
 // complete).
+// rect: mesh B is a separate source mesh (its slot and vertex tables
+// slot_vid_b / vert_unk_b): the column is B's local vertex index, no vertex
+// is shared, and each entry is written once, at (row, column).
 __global__ void __launch_bounds__(256) k_n_stage2(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
                                                   const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
                                                   const int64_t* __restrict__ vert_unk, int64_t sa0, int64_t na,
                                                   int64_t sb0, int64_t j0, int64_t nj, int same, int64_t ta0,
                                                   double coeff, const double* __restrict__ U, int64_t ldu,
-                                                  int64_t nt1, double* __restrict__ H, Layout L, int use_atomic) {
+                                                  int64_t nt1, double* __restrict__ H, Layout L, int use_atomic,
+                                                  const int32_t* __restrict__ slot_vid_b,
+                                                  const int64_t* __restrict__ vert_unk_b, int rect) {
     const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= nj) return;
     const int64_t b = j0 + jj;
     const int64_t cs = nt1 * ldu;
-    const int32_t v2 = slot_vid[sb0 + b];
-    const int64_t c = dev_index(L, vert_unk[v2]);
+    const int32_t v2 = slot_vid_b[sb0 + b];
+    const int64_t c = rect ? vert_unk_b[v2] : dev_index(L, vert_unk_b[v2]);
     for (int64_t a = blockIdx.y; a < na; a += gridDim.y) {
         if (same && b < a) continue;
         double acc = 0.0;
@@ -734,10 +749,10 @@ __global__ void __launch_bounds__(256) k_n_stage2(const TriRec* __restrict__ tri
             acc += w0 * u[0] + w1 * u[cs] + w2 * u[2 * cs];
         }
         const int32_t v1 = slot_vid[sa0 + a];
-        const double factor = same ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
+        const double factor = (same || rect) ? 0.25 : ((v1 == v2) ? 0.5 : 0.25);
         const double val = -factor * acc * coeff;
         const int64_t r = dev_index(L, vert_unk[v1]);
-        double* h = r <= c ? &H[c + r * L.ld] : &H[r + c * L.ld];
+        double* h = (rect || r > c) ? &H[r + c * L.ld] : &H[c + r * L.ld];
         if (use_atomic) atomicAdd(h, val);
         else *h += val;
     }
@@ -782,6 +797,94 @@ __global__ void __launch_bounds__(256) k_deflate(const int32_t* __restrict__ slo
     if (r != c) H[c + r * L.ld] += c0;
 }
 
+// ---------------------------------------------------------------------------
+// Triangle-triangle intersection (the overlap check of SurfSourceMat,
+// Geometry::check, geometry.cpp:329-343, with Mesh::intersection,
+// mesh.cpp:246-252, over every pair of a geometry triangle and a source
+// triangle).  Closed test from orientation determinants: two triangles
+// that are not coplanar meet iff an edge of one meets the other (the ends
+// of their common segment lie on the triangles' boundaries); exactly
+// coplanar triangles are tested in the plane (edge crossings, then one
+// vertex inside the other triangle).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double orient3(d3 a, d3 b, d3 c, d3 d) {   // det[b - a, c - a, d - a]
+    return dot(cross(sub(b, a), sub(c, a)), sub(d, a));
+}
+
+// Segment [a, b] against triangle (p, q, r), the segment not in its plane.
+__device__ __forceinline__ bool seg_meets_tri(d3 a, d3 b, d3 p, d3 q, d3 r) {
+    const double sa = orient3(p, q, r, a), sb = orient3(p, q, r, b);
+    if ((sa > 0.0 && sb > 0.0) || (sa < 0.0 && sb < 0.0) || (sa == 0.0 && sb == 0.0)) return false;
+    const double v0 = orient3(a, b, p, q), v1 = orient3(a, b, q, r), v2 = orient3(a, b, r, p);
+    return (v0 >= 0.0 && v1 >= 0.0 && v2 >= 0.0) || (v0 <= 0.0 && v1 <= 0.0 && v2 <= 0.0);
+}
+
+struct p2 { double u, v; };
+__device__ __forceinline__ double orient2(p2 a, p2 b, p2 c) { return (b.u - a.u) * (c.v - a.v) - (b.v - a.v) * (c.u - a.u); }
+__device__ __forceinline__ bool on_seg2(p2 a, p2 b, p2 c) {   // c collinear with [a, b]: inside its box
+    return fmin(a.u, b.u) <= c.u && c.u <= fmax(a.u, b.u) && fmin(a.v, b.v) <= c.v && c.v <= fmax(a.v, b.v);
+}
+__device__ __forceinline__ bool segs_meet2(p2 a, p2 b, p2 c, p2 d) {
+    const double d1 = orient2(c, d, a), d2 = orient2(c, d, b), d3v = orient2(a, b, c), d4 = orient2(a, b, d);
+    if (((d1 > 0 && d2 < 0) || (d1 < 0 && d2 > 0)) && ((d3v > 0 && d4 < 0) || (d3v < 0 && d4 > 0))) return true;
+    return (d1 == 0 && on_seg2(c, d, a)) || (d2 == 0 && on_seg2(c, d, b)) || (d3v == 0 && on_seg2(a, b, c)) ||
+           (d4 == 0 && on_seg2(a, b, d));
+}
+__device__ __forceinline__ bool inside_tri2(p2 x, const p2 T[3]) {
+    const double o0 = orient2(T[0], T[1], x), o1 = orient2(T[1], T[2], x), o2 = orient2(T[2], T[0], x);
+    return (o0 >= 0 && o1 >= 0 && o2 >= 0) || (o0 <= 0 && o1 <= 0 && o2 <= 0);
+}
+
+__device__ bool tris_meet(const d3 P[3], const d3 Q[3]) {
+    const double s0 = orient3(Q[0], Q[1], Q[2], P[0]), s1 = orient3(Q[0], Q[1], Q[2], P[1]),
+                 s2 = orient3(Q[0], Q[1], Q[2], P[2]);
+    if ((s0 > 0 && s1 > 0 && s2 > 0) || (s0 < 0 && s1 < 0 && s2 < 0)) return false;
+    if (s0 == 0.0 && s1 == 0.0 && s2 == 0.0) {   // coplanar: drop the normal's largest axis
+        const d3 nq = cross(sub(Q[1], Q[0]), sub(Q[2], Q[0]));
+        const double ax = fabs(nq.x), ay = fabs(nq.y), az = fabs(nq.z);
+        p2 A[3], B[3];
+        for (int k = 0; k < 3; ++k) {
+            if (ax >= ay && ax >= az) { A[k] = {P[k].y, P[k].z}; B[k] = {Q[k].y, Q[k].z}; }
+            else if (ay >= az) { A[k] = {P[k].x, P[k].z}; B[k] = {Q[k].x, Q[k].z}; }
+            else { A[k] = {P[k].x, P[k].y}; B[k] = {Q[k].x, Q[k].y}; }
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (segs_meet2(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3])) return true;
+        return inside_tri2(A[0], B) || inside_tri2(B[0], A);
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (seg_meets_tri(P[k], P[(k + 1) % 3], Q[0], Q[1], Q[2])) return true;
+        if (seg_meets_tri(Q[k], Q[(k + 1) % 3], P[0], P[1], P[2])) return true;
+    }
+    return false;
+}
+
+// hit[0] |= 1 when a triangle of A (na records from a0) meets a triangle
+// of B (nb records).  Thread = B triangle, block row = A triangles (looped).
+__global__ void __launch_bounds__(256) k_tri_overlap(const TriRec* __restrict__ triA, int64_t a0, int64_t na,
+                                                     const TriRec* __restrict__ triB, int64_t nb,
+                                                     unsigned* __restrict__ hit) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    const d3 Q[3] = {ld3(triB[j].p[0]), ld3(triB[j].p[1]), ld3(triB[j].p[2])};
+    const d3 qlo = mk(fmin(Q[0].x, fmin(Q[1].x, Q[2].x)), fmin(Q[0].y, fmin(Q[1].y, Q[2].y)),
+                      fmin(Q[0].z, fmin(Q[1].z, Q[2].z)));
+    const d3 qhi = mk(fmax(Q[0].x, fmax(Q[1].x, Q[2].x)), fmax(Q[0].y, fmax(Q[1].y, Q[2].y)),
+                      fmax(Q[0].z, fmax(Q[1].z, Q[2].z)));
+    for (int64_t i = blockIdx.y; i < na; i += gridDim.y) {
+        const TriRec& t = triA[a0 + i];
+        const d3 P[3] = {ld3(t.p[0]), ld3(t.p[1]), ld3(t.p[2])};
+        if (fmax(P[0].x, fmax(P[1].x, P[2].x)) < qlo.x || fmin(P[0].x, fmin(P[1].x, P[2].x)) > qhi.x) continue;
+        if (fmax(P[0].y, fmax(P[1].y, P[2].y)) < qlo.y || fmin(P[0].y, fmin(P[1].y, P[2].y)) > qhi.y) continue;
+        if (fmax(P[0].z, fmax(P[1].z, P[2].z)) < qlo.z || fmin(P[0].z, fmin(P[1].z, P[2].z)) > qhi.z) continue;
+        if (tris_meet(P, Q)) {
+            atomicOr(hit, 1u);
+            return;
+        }
+    }
+}
+
 int64_t n_upper_tiles(int64_t n) {
     const int64_t t = cdiv(n, TILE);
     return t * (t + 1) / 2;
@@ -806,10 +909,12 @@ static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff
     ProfScope ps("s_block", pairs * npts[order], st);
     if (tmp)
         k_s_block<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
-                                                        coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far);
+                                                        coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far, dg.tri,
+                                                        dg.gp);
     else
         k_s_block<false><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
-                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far);
+                                                         coeff, H, L, A.tri_unk0, B.tri_unk0, tmp, dg.gp, far, dg.tri,
+                                                         dg.gp);
     OMH_LAUNCH_CHECK();
 }
 
@@ -859,7 +964,8 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
         OMH_LAUNCH_CHECK();
         const dim3 g2((unsigned)cdiv(nj, 256), (unsigned)std::min<int64_t>(na, 65535));
         k_n_stage2<<<g2, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
-                                       B.slot_begin, j0, nj, same, A.tri_begin, coeff, U, ldu, nt1, H, L, use_atomic);
+                                       B.slot_begin, j0, nj, same, A.tri_begin, coeff, U, ldu, nt1, H, L, use_atomic,
+                                       dg.slot_vid, dg.vert_unk, 0);
         OMH_LAUNCH_CHECK();
     }
 }
@@ -881,12 +987,104 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
         static const int npts[4] = {3, 6, 7, 16};
         ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
         k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W, dg.gp,
-                                                  far);
+                                                  far, dg.tri, dg.gp);
         OMH_LAUNCH_CHECK();
         const int64_t blk2 = cdiv(nc, 64) * cdiv(ns, 4);
         k_d_gather<<<(unsigned)blk2, 256, 0, st>>>(W, nc, n2, Cm.tri_begin, dg.adj_off, dg.adj, dg.slot_vid,
-                                                   dg.vert_unk, Cm.slot_begin, ns, coeff, H, L, R.tri_unk0 + c0);
+                                                   dg.vert_unk, Cm.slot_begin, ns, coeff, H, L, R.tri_unk0 + c0, 0);
         OMH_LAUNCH_CHECK();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SurfSourceMat
+// ---------------------------------------------------------------------------
+bool meshes_intersect(Geometry& g, Geometry& src, hipStream_t st) {
+    DeviceGeometry& dg = device_geometry(g);
+    DeviceGeometry& ds = device_geometry(src);
+    if (dg.n_tris <= 0 || ds.n_tris <= 0) return false;
+    unsigned* hit = (unsigned*)scratch(SCR_SMALL, 64);
+    k_zero_counts<<<1, 64, 0, st>>>(hit);
+    OMH_LAUNCH_CHECK();
+    const dim3 grid((unsigned)cdiv(ds.n_tris, 256), (unsigned)std::min<int64_t>(dg.n_tris, 1024));
+    k_tri_overlap<<<grid, 256, 0, st>>>(dg.tri, 0, dg.n_tris, ds.tri, ds.n_tris, hit);
+    OMH_LAUNCH_CHECK();
+    unsigned h = 0;
+    OMH_HIP(hipMemcpyAsync(&h, hit, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    return h != 0;
+}
+
+void surf_source_mat(Geometry& g, Geometry& src, int order, double* out, const Layout& L, hipStream_t st,
+                     bool far_field) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    if (src.meshes.size() != 1) throw std::invalid_argument("a source mesh is one mesh");
+    const int64_t n = g.size(), nsrc = (int64_t)src.vertices.size();
+    if (meshes_intersect(g, src, st)) throw OverlappingSourceMesh();
+    const int dom = g.domain_of_point(src.vertices[src.meshes[0].verts.front()].data());
+    if (dom < 0) throw std::invalid_argument("the source mesh lies in no domain of the geometry");
+    const double sigma = g.domains[dom].cond;
+    if (sigma == 0.0) throw std::invalid_argument("the source mesh lies in a domain of zero conductivity");
+    DeviceGeometry& dg = device_geometry(g);
+    DeviceGeometry& ds = device_geometry(src);
+    fill_f64(out, n, nsrc, L.ld, 0.0, st);
+    const int far = (far_field && order == 3) ? 1 : 0;
+    const double Lc = -1.0 / sigma;
+    const MeshDev& B = ds.meshes[0];
+    const int64_t n2 = B.tri_end - B.tri_begin, nb = B.slot_end - B.slot_begin;
+    static const int npts[4] = {3, 6, 7, 16};
+    for (const auto& bd : g.domains[dom].boundaries) {
+        const double factorN = bd.second ? kK : -kK;
+        for (const auto& om : bd.first.omeshes) {
+            const double cN = factorN * om.second;
+            const MeshDev& A = dg.meshes[om.first];
+            const int64_t n1 = A.tri_end - A.tri_begin, na = A.slot_end - A.slot_begin;
+            if (n1 <= 0 || na <= 0) continue;
+            // N block from an S block of coefficient 1 in a temporary
+            // (operators.h:428-437)
+            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+            {
+                const int64_t ntx = cdiv(n2, TILE);
+                ProfScope ps("s_block", (double)n1 * (double)n2 * npts[order], st);
+                k_s_block<true><<<(unsigned)(cdiv(n1, TILE) * ntx), 256, 0, st>>>(
+                    dg.tri, A.tri_begin, n1, B.tri_begin, n2, 0, ntx, order, 1.0, nullptr, L, 0, 0, tmp, dg.gp, far,
+                    ds.tri, ds.gp);
+                OMH_LAUNCH_CHECK();
+            }
+            const size_t budget = size_t(1) << 30;
+            const int64_t chunk = std::min<int64_t>(
+                nb, std::max<int64_t>(32, (int64_t)(budget / (3 * sizeof(double) * (size_t)n1)) / 32 * 32));
+            const int64_t ldu = (chunk + 31) / 32 * 32;
+            double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)n1);
+            for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
+                const int64_t nj = std::min<int64_t>(chunk, nb - j0);
+                const dim3 g1((unsigned)cdiv(n1, N1_ROWS), (unsigned)cdiv(nj, N1_COLS));
+                k_n_stage1<<<g1, 256, 0, st>>>(ds.tri, ds.adj_off, ds.adj, B.slot_begin + j0, nj, tmp, n1,
+                                               B.tri_begin, U, ldu, n1);
+                OMH_LAUNCH_CHECK();
+                const dim3 g2((unsigned)cdiv(nj, 256), (unsigned)std::min<int64_t>(na, 65535));
+                k_n_stage2<<<g2, 256, 0, st>>>(dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, A.slot_begin, na,
+                                               B.slot_begin, j0, nj, 0, A.tri_begin, cN, U, ldu, n1, out, L, 0,
+                                               ds.slot_vid, ds.vert_unk, 1);
+                OMH_LAUNCH_CHECK();
+            }
+            // D block (operators.h:439-443), rows = the head mesh's triangle
+            // unknowns; a current barrier's triangles have no rows
+            if (A.barrier) continue;
+            const int64_t dchunk = std::min<int64_t>(n1, std::max<int64_t>(64, (int64_t)(budget / (3 * sizeof(double) * (size_t)n2))));
+            double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)dchunk);
+            for (int64_t c0 = 0; c0 < n1; c0 += dchunk) {
+                const int64_t nc = std::min(dchunk, n1 - c0);
+                ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
+                k_d_pairs<<<(unsigned)(cdiv(nc, 64) * cdiv(n2, 4)), 256, 0, st>>>(
+                    dg.tri, A.tri_begin + c0, nc, B.tri_begin, n2, order, W, dg.gp, far, ds.tri, ds.gp);
+                OMH_LAUNCH_CHECK();
+                k_d_gather<<<(unsigned)(cdiv(nc, 64) * cdiv(nb, 4)), 256, 0, st>>>(
+                    W, nc, n2, B.tri_begin, ds.adj_off, ds.adj, ds.slot_vid, ds.vert_unk, B.slot_begin, nb, cN * Lc,
+                    out, L, A.tri_unk0 + c0, 1);
+                OMH_LAUNCH_CHECK();
+            }
+        }
     }
 }
 

@@ -171,6 +171,21 @@ bool Geometry::interface_contains_sa(double sa) const {   // interface.cpp:18-35
     return std::abs(sa) > 2 * kPi;
 }
 
+int Geometry::domain_of_point(const double p[3]) const {
+    std::vector<double> sa(meshes.size());
+    for (size_t m = 0; m < meshes.size(); ++m) sa[m] = mesh_solid_angle((int)m, p);
+    for (size_t k = 0; k < domains.size(); ++k) {
+        bool inside = true;
+        for (const auto& b : domains[k].boundaries) {
+            double a = 0.0;
+            for (const auto& om : b.first.omeshes) a += om.second * sa[om.first];
+            inside = inside && (interface_contains_sa(a) == b.second);
+        }
+        if (inside) return (int)k;
+    }
+    return -1;
+}
+
 // Interface::is_mesh_orientations_coherent, interface.cpp:55-95, at the
 // bounding-box centre.
 void Geometry::fix_interface_orientation(Interface& itf) const {
@@ -469,6 +484,34 @@ std::unique_ptr<Geometry> load_geometry(const std::string& geom_file, const std:
         d.cond = it->second;
     }
     g->finalize();
+    return g;
+}
+
+std::unique_ptr<Geometry> make_source_mesh(const std::vector<std::array<double, 3>>& pts,
+                                           const std::vector<std::array<i64, 3>>& tris) {
+    if (pts.empty()) throw std::invalid_argument("source mesh without vertices");
+    if (tris.empty()) throw std::invalid_argument("source mesh without triangles");
+    const i64 nv = (i64)pts.size();
+    for (const auto& t : tris)
+        for (int k = 0; k < 3; ++k)
+            if (t[k] < 0 || t[k] >= nv) throw std::invalid_argument("source mesh triangle index out of range");
+    auto g = std::make_unique<Geometry>();
+    g->vertices = pts;
+    Mesh m;
+    m.name = "source";
+    m.verts.resize(nv);
+    for (i64 i = 0; i < nv; ++i) m.verts[i] = i;
+    m.tris = tris;
+    m.current_barrier = true;
+    m.outermost = true;
+    m.tri_first_index = nv;
+    g->meshes.push_back(std::move(m));
+    g->vert_index.resize(nv);
+    for (i64 i = 0; i < nv; ++i) g->vert_index[i] = i;
+    g->invalid.assign(nv, 0);
+    g->nb_parameters = nv + (i64)tris.size();
+    g->nb_current_barrier_triangles = (i64)tris.size();
+    g->n_vert_unknowns = nv;
     return g;
 }
 

@@ -99,6 +99,9 @@ struct Geometry {
     double interface_solid_angle(const Interface& itf, const double p[3]) const;
     bool interface_contains_sa(double solid_angle) const;
     void fix_interface_orientation(Interface& itf) const;
+    // Geometry::domain(point) (geometry.cpp:146-154) on the host: the first
+    // domain whose boundaries all hold the point as they should; -1 if none.
+    int domain_of_point(const double p[3]) const;
 
 private:
     void mark_current_barriers();
@@ -111,5 +114,14 @@ public:
 
 // Geometry(geom_file, cond_file), GeomFile.h:158-300 (format 1.1).
 std::unique_ptr<Geometry> load_geometry(const std::string& geom_file, const std::string& cond_file);
+
+// A standalone mesh (a distributed source surface) held in a one-mesh
+// Geometry so that the device tables can be built for it: vertices in the
+// given order (no de-duplication), vertex i has index i and the triangles
+// follow (Mesh::generate_indices, mesh.cpp:266-272); the mesh is marked
+// outermost and a current barrier, as SurfSourceMat does
+// (assembleSourceMat.cpp:33-36).  No domains, no pairs.
+std::unique_ptr<Geometry> make_source_mesh(const std::vector<std::array<double, 3>>& pts,
+                                           const std::vector<std::array<i64, 3>>& tris);
 
 }  // namespace omh

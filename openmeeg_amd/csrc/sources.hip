@@ -705,6 +705,28 @@ void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t np
     }
 }
 
+void surf2meg(Geometry& src, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
+              hipStream_t st) {
+    if (src.meshes.size() != 1) throw std::invalid_argument("a source mesh is one mesh");
+    DeviceGeometry& ds = device_geometry(src);
+    const int64_t nv = (int64_t)src.vertices.size();
+    fill_f64(d_out, npts, nv, ld_out, 0.0, st);
+    const MeshDev& md = ds.meshes[0];
+    const int64_t ns = md.slot_end - md.slot_begin;
+    if (npts <= 0 || ns <= 0) return;
+    // columns: the source mesh's local vertex indices (reference order of a
+    // one-mesh numbering, no triangle columns)
+    Layout L;
+    L.ld = ld_out;
+    L.nv = nv;
+    L.nt = 0;
+    L.tri_first = 0;
+    const dim3 grid((unsigned)cdiv(ns, 256), (unsigned)std::min<int64_t>(npts, 65535));
+    k_ferguson<<<grid, 256, 0, st>>>(ds.tri, ds.adj_off, ds.adj, ds.slot_vid, ds.vert_unk, md.slot_begin, ns, d_pts,
+                                     d_dirs, npts, 1.0, d_out, ld_out, L);
+    OMH_LAUNCH_CHECK();
+}
+
 void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,
              double* d_out, int64_t ld_out, hipStream_t st) {
     if (npts <= 0 || n_dip <= 0) return;

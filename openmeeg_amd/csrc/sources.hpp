@@ -53,6 +53,13 @@ void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, doub
 void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
               const Layout& L, hipStream_t st);
 
+// SurfSource2MEGMat before the sensor weights (assembleSensors.cpp:102-124):
+// d_out (npts x nsrc, column-major) = the Ferguson operator of the source
+// mesh src (a one-mesh Geometry) with coefficient 1 at each point along its
+// direction; columns are the source mesh's vertices.
+void surf2meg(Geometry& src, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
+              hipStream_t st);
+
 // DipSource2MEGMat before the sensor weights (assembleSensors.cpp:133-156):
 // d_out (npts x n_dip, column-major) for dipoles d_dip (n_dip x 6 row-major).
 void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,

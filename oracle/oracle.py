@@ -361,6 +361,21 @@ def conductivity_jump(g, m):
     return sum(d.cond * d.mesh_orientation(m) for d in g.domains)
 
 
+def _ferguson(verts, tris, pts, coeff, F):
+    """F[i, :, v] += coeff x the Ferguson operator of the triangles tris at
+    point i for vertex v (operators.h:40-59: sum over the triangles T around
+    v of (B - A) S_T(x) / (2 |T|), A and B the edge opposite v)."""
+    L = lib()
+    for t in tris:
+        T = verts[t]
+        area = 0.5 * np.linalg.norm(np.cross(T[1] - T[0], T[2] - T[0]))
+        t9 = np.ascontiguousarray(T.reshape(9))
+        for i, x in enumerate(pts):
+            sv = L.orc_operator_s(_p(t9), _p(np.ascontiguousarray(x))) / (2 * area) * coeff
+            for k in range(3):
+                F[i, :, t[k]] += (T[_EDGE[k][1]] - T[_EDGE[k][0]]) * sv
+
+
 def head2meg(g, pts, dirs, weights, sensor):
     """Head2MEGMat (assembleSensors.cpp:69-99): the Ferguson matrix of every
     mesh (assemble_ferguson, assembleFerguson.cpp:18-30, coefficient
@@ -368,23 +383,14 @@ def head2meg(g, pts, dirs, weights, sensor):
     operators.h:40-59: sum over the triangles T around a vertex of
     (B - A) S_T(x) / (2 |T|), A and B the edge opposite the vertex),
     projected on each point's direction and summed into sensors."""
-    L = lib()
     pts = np.asarray(pts, dtype=np.float64)
     dirs = np.asarray(dirs, dtype=np.float64)
     verts = np.array(g.vertices, dtype=np.float64)
     F = np.zeros((len(pts), 3, len(verts)))
     for mid, m in enumerate(g.meshes):
         coeff = MAG_FACTOR * conductivity_jump(g, mid)
-        if coeff == 0.0:
-            continue
-        for t in m.tris:
-            T = verts[t]
-            area = 0.5 * np.linalg.norm(np.cross(T[1] - T[0], T[2] - T[0]))
-            t9 = np.ascontiguousarray(T.reshape(9))
-            for i, x in enumerate(pts):
-                sv = L.orc_operator_s(_p(t9), _p(np.ascontiguousarray(x))) / (2 * area) * coeff
-                for k in range(3):
-                    F[i, :, t[k]] += (T[_EDGE[k][1]] - T[_EDGE[k][0]]) * sv
+        if coeff != 0.0:
+            _ferguson(verts, m.tris, pts, coeff, F)
     mat = np.zeros((len(pts), g.size))
     dn = np.linalg.norm(dirs, axis=1)
     for v in range(len(verts)):
@@ -411,6 +417,91 @@ def gain_meg(H, dsm, h2mm, ds2mm):
     import scipy.linalg as sla
     Y = sla.solve(H, h2mm.T, assume_a="sym")
     return ds2mm + Y.T @ dsm
+
+
+def surf_source2meg(src_pts, src_tris, pts, dirs, weights, sensor):
+    """SurfSource2MEGMat (assembleSensors.cpp:102-124): the Ferguson operator
+    of the source mesh with coefficient 1 (no MagFactor) at each point,
+    projected on the point's direction, one column per source-mesh vertex
+    (local index, Mesh::generate_indices mesh.cpp:266-272), summed into
+    sensors."""
+    verts = np.asarray(src_pts, dtype=np.float64)
+    pts = np.asarray(pts, dtype=np.float64)
+    dirs = np.asarray(dirs, dtype=np.float64)
+    F = np.zeros((len(pts), 3, len(verts)))
+    _ferguson(verts, np.asarray(src_tris, dtype=np.int64), pts, 1.0, F)
+    mat = (F * dirs[:, :, None]).sum(axis=1) / np.linalg.norm(dirs, axis=1)[:, None]
+    return _weights_matrix(weights, sensor) @ mat
+
+
+# ---------------------------------------------------------------------------
+# SurfSourceMat
+# ---------------------------------------------------------------------------
+class _SourceTables:
+    """The head's tables with the source mesh appended as one more mesh
+    (a current barrier, so that only its N and D blocks against a head mesh
+    are formed).  Its vertices take the unknown indices n .. n + nsrc - 1, so
+    that the rectangular block (head unknown, source vertex) lies in the
+    upper half of an (n + nsrc) packed matrix."""
+
+    def __init__(self, g, src_pts, src_tris):
+        from .geometry import Mesh
+        nvh, n = len(g.vertices), g.size
+        ns = len(src_pts)
+        sm = Mesh("source")
+        sm.verts = list(range(nvh, nvh + ns))
+        sm.tris = np.asarray(src_tris, dtype=np.int64) + nvh
+        sm.current_barrier = True
+        sm.tri_index = np.arange(len(sm.tris), dtype=np.int64) + n + ns   # never written
+        self.vertices = list(g.vertices) + [tuple(float(c) for c in p) for p in src_pts]
+        self.meshes = list(g.meshes) + [sm]
+        self.vert_index = np.r_[np.asarray(g.vert_index, dtype=np.int64), n + np.arange(ns, dtype=np.int64)]
+
+
+def surf_source_mat(g, src_pts, src_tris, order=3, stats=None):
+    """SurfSourceMat (assembleSourceMat.cpp:22-56) with the non-adaptive
+    Integrator(order, 0, .): the source mesh lies in the domain of its first
+    vertex (conductivity sigma); for each boundary (interface, inside) of that
+    domain and each oriented mesh of the interface, coeffN = (inside ? K :
+    -K) x orientation,
+      N block (head vertices x source vertices) with coefficient coeffN, from
+        an S block of coefficient 1 (operators.h:372, 428-437),
+      D block (head triangles x source vertices) with coefficient
+        -coeffN / sigma (operators.h:439-443).
+    The D block of a current-barrier head mesh is left out (its triangles
+    have no rows in the n x nsrc matrix; the reference would index past the
+    matrix there).  Returns the n x nsrc matrix (reference unknown order);
+    stats (a dict), when given, receives the seconds spent in the C pair
+    assembly ("pair_seconds"), without the Python table set-up."""
+    import time
+    src_pts = np.asarray(src_pts, dtype=np.float64)
+    src_tris = np.asarray(src_tris, dtype=np.int64)
+    n, ns = g.size, len(src_pts)
+    ext = _SourceTables(g, src_pts, src_tris)
+    t = _Tables(ext)
+    dom = g.domains[g.domain_of_point(src_pts[0], OracleLib())]
+    Lc = -1.0 / dom.cond
+    M = len(g.meshes)
+    N = n + ns
+    H = np.zeros(N * (N + 1) // 2)
+    L = lib()
+    t0 = time.perf_counter()
+    for iface, inside in dom.boundaries:
+        factorN = K if inside else -K
+        for mid, o in iface.omeshes:
+            coeffN = factorN * o
+            # the source mesh is a barrier: S goes to a temporary, N with
+            # coeffN, D(head mesh rows, source columns) unless the head mesh
+            # is a barrier, no D* (orc_mesh_pair's branches)
+            if L.orc_mesh_pair(C.byref(t.s), mid, M, order, 0.0, coeffN, coeffN * Lc, _p(H)) != 0:
+                raise MemoryError("oracle mesh pair")
+    if stats is not None:
+        stats["pair_seconds"] = time.perf_counter() - t0
+    out = np.zeros((n, ns))
+    for c in range(ns):
+        j = n + c
+        out[:, c] = H[np.arange(n) + j * (j + 1) // 2]
+    return out
 
 
 # ---------------------------------------------------------------------------

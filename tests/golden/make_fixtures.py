@@ -14,8 +14,12 @@ What is extracted (all data files that the reference's own tests hold):
   - ``h2em_{data,ir,jc}.npy``: compressed-sparse-column H2EM;
   - ``h2mm.npy`` / ``ds2mm.npy`` / ``dgmm.npy``: the MEG goldens (Head2MEGMat,
     DipSource2MEGMat, GainMEG with ``Head1.squids``), dense;
+  - ``ssm.npy`` / ``sgem.npy`` / ``ss2mm.npy`` / ``sgmm.npy`` / ``est_eeg.npy``:
+    the surface-source goldens (SurfSourceMat and SurfSource2MEGMat of the
+    source mesh ``Head1.tri``, the surface EEG and MEG gains, and the EEG
+    forward of the activation ``Head1.src`` with no noise), dense;
   plus the Head1 inputs (``data/Head1``: meshes, .geom, .cond, .dip, .patches,
-  .squids).
+  .squids, the source mesh .tri and its activation .src).
 * ``head3/``: the Head3 inputs (642-vertex spheres, same geometry as the
   0.87/0.92/1.0 nested-sphere configuration), .dip and .patches.
 * ``headnna1/``: the HeadNNa1 inputs (non-nested: the north and south meshes
@@ -57,7 +61,7 @@ def main():
         assert data.size == n * (n + 1) // 2, (name, n, data.size)
         np.save(os.path.join(h1, name + ".npy"), data)
 
-    for name in ("dsm", "dgem", "h2mm", "ds2mm", "dgmm"):
+    for name in ("dsm", "dgem", "h2mm", "ds2mm", "dgmm", "ssm", "sgem", "ss2mm", "sgmm", "est_eeg"):
         path = os.path.join(initial, "Head1." + name)
         raw = h5_dataset(path, "/linop", "<f8")
         # The HDF5 dataset is (ncol, nrow) row-major, i.e. the column-major
@@ -76,7 +80,7 @@ def main():
 
     data1 = os.path.join(REF, "data", "Head1")
     for f in ("Head1.geom", "Head1.cond", "Head1.dip", "Head1.patches", "Head1.squids",
-              "cortex.1.tri", "skull.1.tri", "scalp.1.tri"):
+              "Head1.tri", "Head1.src", "cortex.1.tri", "skull.1.tri", "scalp.1.tri"):
         shutil.copy(os.path.join(data1, f), h1)
 
     h3 = os.path.join(HERE, "head3")

@@ -95,3 +95,22 @@ def test_head1_block_pattern_matches_golden_headmat():
         for t in range(s):
             blk = Htf[starts[s]:ends[s], starts[t]:ends[t]]
             assert bool(np.any(blk != 0)) == bool(hfill[s, t]), (s, t)
+
+
+def test_source_mesh_handle_on_the_host():
+    """Mesh(.tri) and Mesh(vertices, triangles) build the handle on the host
+    (no device needed); out-of-range triangle indices and empty meshes are
+    refused with invalid-argument status."""
+    import ctypes as C
+    import openmeeg_amd as om
+    from openmeeg_amd import _lib
+    m = om.Mesh(os.path.join(GOLDEN, "head1", "Head1.tri"))
+    assert m.vertices().size() == 42 and m.nb_triangles() == 80
+    sz = (C.c_int64 * 2)()
+    _lib.check(_lib.lib().omh_mesh_sizes(m._h, sz))
+    assert list(sz) == [42, 80]
+    P = np.eye(3)
+    with pytest.raises(ValueError, match="out of range"):
+        om.Mesh(P, [[0, 1, 3]])
+    with pytest.raises(ValueError, match="empty"):
+        om.Mesh(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64))

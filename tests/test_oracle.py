@@ -285,3 +285,73 @@ def test_oracle_ferguson_sign_and_jump():
         cols = g.vert_index[np.array(m.verts)]
         assert abs(M[0, cols].sum()) <= 1e-12 * np.abs(M[0, :nv]).max()
         assert O.conductivity_jump(g, mid) != 0.0
+
+
+# ------------------------------------------------------- surface sources
+def _head1_surface():
+    g = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    P, T = O.read_tri(os.path.join(GOLDEN, "head1", "Head1.tri"))
+    return g, P, T
+
+
+def _h2em_golden(n):
+    """Head1.h2em (32 electrodes x n), dense."""
+    import scipy.sparse as sp
+    d = os.path.join(GOLDEN, "head1")
+    data, ir, jc = (np.load(os.path.join(d, "h2em_%s.npy" % k)) for k in ("data", "ir", "jc"))
+    return sp.csc_matrix((data, ir.astype(np.int64), jc.astype(np.int64)), shape=(32, n)).toarray()
+
+
+def test_oracle_surf_sources_head1_match_golden():
+    """The oracle's SurfSourceMat and SurfSource2MEGMat of the source mesh
+    Head1.tri against the reference's goldens (Head1.{ssm,ss2mm}), and the
+    surface gains they make with the golden HeadMat inverse (Head1.{sgem,
+    sgmm}) and the golden EEG forward of Head1.src (Head1.est_eeg)."""
+    g, P, T = _head1_surface()
+    d = os.path.join(GOLDEN, "head1")
+    S = O.surf_source_mat(g, P, T)
+    ref = np.load(os.path.join(d, "ssm.npy"))
+    assert S.shape == ref.shape == (g.size, len(P))
+    assert rel_fro(S, ref) < 1e-13
+    pts, dirs, w, sens = O.read_squids(os.path.join(d, "Head1.squids"))
+    M = O.surf_source2meg(P, T, pts, dirs, w, sens)
+    assert rel_fro(M, np.load(os.path.join(d, "ss2mm.npy"))) < 1e-10
+    Hi = O.unpack(np.load(os.path.join(d, "hm_inv.npy")), g.size)
+    E = _h2em_golden(g.size)
+    sgem = np.load(os.path.join(d, "sgem.npy"))
+    assert rel_fro(E @ Hi @ S, sgem) < 1e-12
+    H = O.unpack(np.load(os.path.join(d, "hm.npy")), g.size)
+    sgmm = O.gain_meg(H, S, np.load(os.path.join(d, "h2mm.npy")), M)
+    assert rel_fro(sgmm, np.load(os.path.join(d, "sgmm.npy"))) < 1e-10
+    src = np.loadtxt(os.path.join(d, "Head1.src"))
+    assert rel_fro(sgem @ src, np.load(os.path.join(d, "est_eeg.npy"))) < 1e-14
+
+
+def test_oracle_surf_source_closed_layer_and_dipole_limit():
+    """From the mathematics: (1) a closed source surface with density 1
+    everywhere is a uniform double layer and makes no field outside it --
+    every row of SurfSourceMat sums to zero (the N part telescopes around
+    each triangle, the D part is the closed surface's solid angle seen from
+    an outside point); (2) density rho on the source surface is the normal
+    current-dipole layer -rho n dA: the surface EEG gain equals the gain of
+    dipoles -n dA rho at quadrature points of the source triangles within the
+    quadrature error (here 2%, against 200% for a flipped sign)."""
+    g, P, T = _head1_surface()
+    S = O.surf_source_mat(g, P, T)
+    assert np.abs(S.sum(axis=1)).max() < 1e-14 * np.abs(S).max() * S.shape[1]
+    dips, W = [], []
+    for t in T:
+        A_, B_, C_ = P[t]
+        nrm = np.cross(B_ - A_, C_ - A_)
+        area = 0.5 * np.linalg.norm(nrm)
+        for k in range(3):
+            lam = np.full(3, 1.0 / 6.0)
+            lam[k] = 2.0 / 3.0
+            dips.append(np.r_[lam @ P[t], -nrm / np.linalg.norm(nrm)])
+            w = np.zeros(len(P))
+            w[t] = lam * area / 3.0
+            W.append(w)
+    H = O.unpack(np.load(os.path.join(GOLDEN, "head1", "hm.npy")), g.size)
+    E = _h2em_golden(g.size)
+    Gd = O.gain_eeg(H, O.dip_source_mat(g, np.array(dips), "", levels=0), E) @ np.array(W)
+    assert rel_fro(O.gain_eeg(H, S, E), Gd) < 0.02
