@@ -165,6 +165,8 @@ int omh_head2meg_mat(omh_geometry* g, const double* d_pts, const double* d_dirs,
  * use and kept with the handle. */
 int omh_mesh_create(const double* vertices, int64_t n_vertices, const int64_t* triangles, int64_t n_triangles,
                     omh_mesh** out);
+/* Mesh(filename) for a .tri file (MeshIOs/tri.h:26-56). */
+int omh_mesh_load(const char* path, omh_mesh** out);
 void omh_mesh_free(omh_mesh* m);
 /* out[0] = vertices, out[1] = triangles. */
 int omh_mesh_sizes(const omh_mesh* m, int64_t out[2]);

@@ -35,6 +35,7 @@ _SIGS = {
     "omh_head2meg_mat": ([vp, vp, vp, i64, vp, i64, i32, vp], C.c_int),
     "omh_dip_source2meg_mat": ([vp, i64, vp, vp, i64, vp, i64, vp], C.c_int),
     "omh_mesh_create": ([p_f64, i64, p_i64, i64, C.POINTER(vp)], C.c_int),
+    "omh_mesh_load": ([cp, C.POINTER(vp)], C.c_int),
     "omh_mesh_free": ([vp], None),
     "omh_mesh_sizes": ([vp, p_i64], C.c_int),
     "omh_mesh_intersects": ([vp, vp, p_i32, vp], C.c_int),

@@ -8,7 +8,6 @@ no CPU fallback -- without the library or a GPU the calls raise.
 """
 import ctypes as C
 import os
-import struct
 
 import numpy as np
 
@@ -373,6 +372,23 @@ class SparseMatrix:
         for k in range(3):
             np.add.at(E, (np.arange(self._n[0]), self.cols[:, k]), self.vals[:, k])
         return E
+
+    def save(self, fname):
+        """SparseMatrix::save: (row, col, value) entries, .mat / .bin / .txt."""
+        from . import io as omio
+        rows = np.repeat(np.arange(self._n[0], dtype=np.int64), 3)
+        omio.write_sparse(fname, rows, self.cols.ravel(), self.vals.ravel(), self._n)
+
+    @classmethod
+    def load(cls, fname):
+        """A Head2EEGMat file (three entries per row)."""
+        from . import io as omio
+        kind, (r, c, v, shape) = omio.read(fname)
+        if kind != "sparse":
+            raise ValueError("%s holds a %s matrix, not a sparse one" % (fname, kind))
+        if len(r) != 3 * shape[0] or np.any(r != np.repeat(np.arange(shape[0]), 3)):
+            raise ValueError("%s: a Head2EEGMat has three entries per row" % fname)
+        return cls(shape[0], shape[1], c.reshape(-1, 3), v.reshape(-1, 3))
 
 
 class SymMatrix:
@@ -1143,32 +1159,28 @@ def gain_to_numpy(G, ne, nd):
 # File IO: TrivialBinIO (.bin, TrivialBinIO.H:107-137) and ascii (.txt)
 # ---------------------------------------------------------------------------
 def save_matrix(fname, a):
-    a = np.asfortranarray(a, dtype=np.float64)
-    if fname.endswith(".bin"):
-        with open(fname, "wb") as f:
-            f.write(struct.pack("<II", a.shape[0], a.shape[1]))
-            f.write(a.tobytes(order="F"))
-    else:
-        np.savetxt(fname, a)
+    """Matrix::save: .mat (MAT 7.3), .bin or .txt by suffix (MathsIO.C:71-81)."""
+    from . import io as omio
+    omio.write_full(fname, np.asarray(a, dtype=np.float64))
 
 
 def load_matrix(fname):
-    if fname.endswith(".bin"):
-        with open(fname, "rb") as f:
-            nl, nc = struct.unpack("<II", f.read(8))
-            return np.frombuffer(f.read(8 * nl * nc), dtype=np.float64).reshape(nc, nl).T.copy(order="F")
-    return np.asfortranarray(np.loadtxt(fname, ndmin=2))
+    """Matrix(filename): the format identified from the content."""
+    from . import io as omio
+    kind, a = omio.read(fname)
+    if kind != "full":
+        raise ValueError("%s holds a %s matrix, not a full one" % (fname, kind))
+    return a if a.ndim == 2 else a.reshape(-1, 1).copy(order="F")
 
 
 def save_symmatrix(fname, packed):
-    packed = np.ascontiguousarray(packed, dtype=np.float64)
-    n = int((np.sqrt(8 * packed.size + 1) - 1) / 2)
-    with open(fname, "wb") as f:
-        f.write(struct.pack("<I", n))
-        f.write(packed.tobytes())
+    from . import io as omio
+    omio.write_symmetric(fname, packed)
 
 
 def load_symmatrix(fname):
-    with open(fname, "rb") as f:
-        (n,) = struct.unpack("<I", f.read(4))
-        return np.frombuffer(f.read(8 * n * (n + 1) // 2), dtype=np.float64).copy()
+    from . import io as omio
+    kind, p = omio.read(fname)
+    if kind != "symmetric":
+        raise ValueError("%s holds a %s matrix, not a symmetric one" % (fname, kind))
+    return p

@@ -1,0 +1,221 @@
+// om_assemble: the head matrices of the reference's command-line tool
+// (apps/assemble.cpp:30-330) for the EEG / MEG forward path, computed by
+// libopenmeeg_amd on the GPU and written in the format of the output
+// file's suffix (.mat, .bin, .txt).
+//
+//   -HeadMat, -HM, -hm                 geometry conductivity output
+//   -DipSourceMat, -DSM, -dsm          geometry conductivity dipoles output [domain]
+//   -DipSourceMatNoAdapt, -DSMNA, -dsmna   (the same without adaptive integration)
+//   -SurfSourceMat, -SSM, -ssm         geometry conductivity sourcemesh output
+//   -Head2EEGMat, -H2EM, -h2em         geometry conductivity electrodes output
+//   -Head2MEGMat, -H2MM, -h2mm         geometry conductivity squids output
+//   -DipSource2MEGMat, -DS2MM, -ds2mm  dipoles squids output
+//   -SurfSource2MEGMat, -SS2MM, -ss2mm sourcemesh squids output
+#include <algorithm>
+#include <cstring>
+#include <iostream>
+
+#include "cli.hpp"
+
+using namespace cli;
+
+namespace {
+
+bool is(const char* a, std::initializer_list<const char*> names) {
+    for (const char* n : names)
+        if (!std::strcmp(a, n)) return true;
+    return false;
+}
+
+void need_args(int argc, int k, int count, const char* opt) {
+    if (argc < k + 1 + count) throw std::runtime_error(std::string(opt) + " needs " + std::to_string(count) + " file arguments");
+}
+
+void head_mat(const std::string& geom, const std::string& cond, const std::string& out) {
+    Geom g(geom, cond);
+    const int64_t n = g.n();
+    Dev<double> H((size_t)(n * n));
+    check(omh_head_mat(g.g, 3, H.p, n, OMH_ORDER_REFERENCE, nullptr));
+    Dev<double> P((size_t)(n * (n + 1) / 2));
+    check(omh_sym_pack(H.p, n, n, n, OMH_ORDER_REFERENCE, P.p, nullptr));
+    write_symmetric(out, P.down(), n);
+}
+
+void dip_source_mat(const std::string& geom, const std::string& cond, const std::string& dipf, const std::string& out,
+                    const char* domain, int levels) {
+    Geom g(geom, cond);
+    int64_t nd = 0;
+    const std::vector<double> dip = read_dipoles(dipf, nd);
+    const int64_t n = g.n();
+    std::vector<int32_t> dom;
+    if (domain) {
+        int32_t k = -1;
+        check(omh_geometry_domain_index(g.g, domain, &k));
+        dom.assign((size_t)nd, k);
+    }
+    Dev<double> dd(dip.data(), dip.size());
+    Dev<double> D((size_t)(n * nd));
+    check(omh_dip_source_mat(g.g, dd.p, nd, domain ? dom.data() : nullptr, 3, levels, 0.001, D.p, n,
+                             OMH_ORDER_REFERENCE, nullptr));
+    Full m;
+    m.nlin = n;
+    m.ncol = nd;
+    m.a = D.down();
+    write_full(out, m);
+}
+
+void surf_source_mat(const std::string& geom, const std::string& cond, const std::string& meshf,
+                     const std::string& out) {
+    Geom g(geom, cond);
+    omh_mesh* mesh = nullptr;
+    check(omh_mesh_load(meshf.c_str(), &mesh));
+    int64_t ms[2];
+    check(omh_mesh_sizes(mesh, ms));
+    const int64_t n = g.n();
+    Dev<double> D((size_t)(n * ms[0]));
+    const int rc = omh_surf_source_mat(g.g, mesh, 3, 0, D.p, n, OMH_ORDER_REFERENCE, nullptr);
+    omh_mesh_free(mesh);
+    check(rc);
+    Full m;
+    m.nlin = n;
+    m.ncol = ms[0];
+    m.a = D.down();
+    write_full(out, m);
+}
+
+void head2eeg(const std::string& geom, const std::string& cond, const std::string& elf, const std::string& out) {
+    Geom g(geom, cond);
+    const Sensors s = read_sensors(elf);
+    const int64_t ne = s.npts();
+    Dev<double> de(s.pos.data(), s.pos.size());
+    std::vector<int64_t> cols((size_t)(3 * ne));
+    std::vector<double> vals((size_t)(3 * ne));
+    check(omh_head2eeg_mat(g.g, de.p, ne, cols.data(), vals.data(), nullptr));
+    // SparseMatrix(i, col) = val, a map: a repeated (i, col) keeps the last value
+    std::map<std::pair<int64_t, int64_t>, double> e;
+    for (int64_t i = 0; i < ne; ++i)
+        for (int k = 0; k < 3; ++k) e[{i, cols[3 * i + k]}] = vals[3 * i + k];
+    Sparse sp;
+    sp.nlin = ne;
+    sp.ncol = g.n();
+    for (const auto& x : e) {
+        sp.rows.push_back(x.first.first);
+        sp.cols.push_back(x.first.second);
+        sp.vals.push_back(x.second);
+    }
+    write_sparse(out, sp);
+}
+
+std::vector<double> dirs_of(const Sensors& s, const std::string& path) {
+    if (!s.oriented) throw std::runtime_error(path + ": MEG sensors need orientations (6 or 7 columns)");
+    return s.dir;
+}
+
+void head2meg(const std::string& geom, const std::string& cond, const std::string& sqf, const std::string& out) {
+    Geom g(geom, cond);
+    const Sensors s = read_sensors(sqf);
+    const int64_t np = s.npts(), n = g.n();
+    Dev<double> dp(s.pos.data(), s.pos.size());
+    const std::vector<double> dirs = dirs_of(s, sqf);
+    Dev<double> dd(dirs.data(), dirs.size());
+    Dev<double> P((size_t)(np * n));
+    check(omh_head2meg_mat(g.g, dp.p, dd.p, np, P.p, np, OMH_ORDER_REFERENCE, nullptr));
+    write_full(out, sensor_sum(s, P.down(), n));
+}
+
+void dip2meg(const std::string& dipf, const std::string& sqf, const std::string& out) {
+    int64_t nd = 0;
+    const std::vector<double> dip = read_dipoles(dipf, nd);
+    const Sensors s = read_sensors(sqf);
+    const int64_t np = s.npts();
+    Dev<double> dp(s.pos.data(), s.pos.size());
+    const std::vector<double> dirs = dirs_of(s, sqf);
+    Dev<double> dd(dirs.data(), dirs.size());
+    Dev<double> ddip(dip.data(), dip.size());
+    Dev<double> P((size_t)(np * nd));
+    check(omh_dip_source2meg_mat(ddip.p, nd, dp.p, dd.p, np, P.p, np, nullptr));
+    write_full(out, sensor_sum(s, P.down(), nd));
+}
+
+void surf2meg(const std::string& meshf, const std::string& sqf, const std::string& out) {
+    omh_mesh* mesh = nullptr;
+    check(omh_mesh_load(meshf.c_str(), &mesh));
+    int64_t ms[2];
+    check(omh_mesh_sizes(mesh, ms));
+    const Sensors s = read_sensors(sqf);
+    const int64_t np = s.npts();
+    Dev<double> dp(s.pos.data(), s.pos.size());
+    const std::vector<double> dirs = dirs_of(s, sqf);
+    Dev<double> dd(dirs.data(), dirs.size());
+    Dev<double> P((size_t)(np * ms[0]));
+    const int rc = omh_surf_source2meg_mat(mesh, dp.p, dd.p, np, P.p, np, nullptr);
+    omh_mesh_free(mesh);
+    check(rc);
+    write_full(out, sensor_sum(s, P.down(), ms[0]));
+}
+
+void help(const char* cmd) {
+    std::cout << cmd << " [-option] [filepaths...]\n\n"
+              << "   -HeadMat, -HM, -hm :                 geometry conductivity output\n"
+              << "   -DipSourceMat, -DSM, -dsm :          geometry conductivity dipoles output [domain]\n"
+              << "   -DipSourceMatNoAdapt, -DSMNA, -dsmna: geometry conductivity dipoles output [domain]\n"
+              << "   -SurfSourceMat, -SSM, -ssm :         geometry conductivity sourcemesh output\n"
+              << "   -Head2EEGMat, -H2EM, -h2em :         geometry conductivity electrodes output\n"
+              << "   -Head2MEGMat, -H2MM, -h2mm :         geometry conductivity squids output\n"
+              << "   -DipSource2MEGMat, -DS2MM, -ds2mm :  dipoles squids output\n"
+              << "   -SurfSource2MEGMat, -SS2MM, -ss2mm : sourcemesh squids output\n"
+              << "Output format from the suffix: .mat (MAT 7.3), .bin, .txt\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2 || is(argv[1], {"-h", "--help", "-help"})) {
+        help(argv[0]);
+        return 0;
+    }
+    try {
+        int done = 0;
+        for (int k = 1; k < argc; ++k) {
+            const char* o = argv[k];
+            if (o[0] != '-') continue;
+            if (is(o, {"-old-ordering"})) throw std::runtime_error("-old-ordering is not supported by this build");
+            if (is(o, {"-HeadMat", "-HM", "-hm"})) {
+                need_args(argc, k, 3, o);
+                head_mat(argv[k + 1], argv[k + 2], argv[k + 3]);
+            } else if (is(o, {"-DipSourceMat", "-DSM", "-dsm", "-DipSourceMatNoAdapt", "-DSMNA", "-dsmna"})) {
+                need_args(argc, k, 4, o);
+                const bool noadapt = is(o, {"-DipSourceMatNoAdapt", "-DSMNA", "-dsmna"});
+                const char* domain = (k + 5 < argc && argv[k + 5][0] != '-') ? argv[k + 5] : nullptr;
+                dip_source_mat(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4], domain, noadapt ? 0 : 10);
+            } else if (is(o, {"-SurfSourceMat", "-SSM", "-ssm"})) {
+                need_args(argc, k, 4, o);
+                surf_source_mat(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);
+            } else if (is(o, {"-Head2EEGMat", "-H2EM", "-h2em"})) {
+                need_args(argc, k, 4, o);
+                head2eeg(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);
+            } else if (is(o, {"-Head2MEGMat", "-H2MM", "-h2mm"})) {
+                need_args(argc, k, 4, o);
+                head2meg(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);
+            } else if (is(o, {"-DipSource2MEGMat", "-DS2MM", "-ds2mm"})) {
+                need_args(argc, k, 3, o);
+                dip2meg(argv[k + 1], argv[k + 2], argv[k + 3]);
+            } else if (is(o, {"-SurfSource2MEGMat", "-SS2MM", "-ss2mm"})) {
+                need_args(argc, k, 3, o);
+                surf2meg(argv[k + 1], argv[k + 2], argv[k + 3]);
+            } else {
+                throw std::runtime_error(std::string("option ") + o + " is not supported by this build");
+            }
+            ++done;
+            break;   // one matrix per call (assert_non_conflicting_options)
+        }
+        if (!done) {
+            help(argv[0]);
+            return 1;
+        }
+    } catch (const std::exception& e) {
+        std::cerr << argv[0] << ": " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

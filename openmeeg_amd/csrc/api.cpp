@@ -454,6 +454,16 @@ int omh_mesh_create(const double* vertices, int64_t n_vertices, const int64_t* t
     });
 }
 
+int omh_mesh_load(const char* path, omh_mesh** out) {
+    return guard([&] {
+        need(path, "path");
+        need(out, "out");
+        auto h = std::make_unique<omh_mesh>();
+        h->g = omh::load_source_mesh(path);
+        *out = h.release();
+    });
+}
+
 void omh_mesh_free(omh_mesh* m) { delete m; }
 
 int omh_mesh_sizes(const omh_mesh* m, int64_t out[2]) {

@@ -515,4 +515,13 @@ std::unique_ptr<Geometry> make_source_mesh(const std::vector<std::array<double, 
     return g;
 }
 
+std::unique_ptr<Geometry> load_source_mesh(const std::string& path) {
+    if (path.size() < 4 || path.compare(path.size() - 4, 4, ".tri") != 0)
+        throw std::invalid_argument("only .tri source meshes are read: " + path);
+    std::vector<std::array<double, 3>> pts;
+    std::vector<std::array<i64, 3>> tris;
+    read_tri(path, pts, tris);
+    return make_source_mesh(pts, tris);
+}
+
 }  // namespace omh

@@ -124,4 +124,7 @@ std::unique_ptr<Geometry> load_geometry(const std::string& geom_file, const std:
 std::unique_ptr<Geometry> make_source_mesh(const std::vector<std::array<double, 3>>& pts,
                                            const std::vector<std::array<i64, 3>>& tris);
 
+// Mesh(filename) of a .tri file (MeshIOs/tri.h:26-56) as a source mesh.
+std::unique_ptr<Geometry> load_source_mesh(const std::string& path);
+
 }  // namespace omh

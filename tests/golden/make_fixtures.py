@@ -19,7 +19,10 @@ What is extracted (all data files that the reference's own tests hold):
     source mesh ``Head1.tri``, the surface EEG and MEG gains, and the EEG
     forward of the activation ``Head1.src`` with no noise), dense;
   plus the Head1 inputs (``data/Head1``: meshes, .geom, .cond, .dip, .patches,
-  .squids, the source mesh .tri and its activation .src).
+  .squids, the source mesh .tri and its activation .src), and under
+  ``head1/mat/`` the MAT-file goldens Head1.{hm,dsm,h2em,ss2mm,est_eeg}
+  unconverted (a symmetric, full, sparse and full matrix as the reference
+  writes them) for the file-format tests.
 * ``head3/``: the Head3 inputs (642-vertex spheres, same geometry as the
   0.87/0.92/1.0 nested-sphere configuration), .dip and .patches.
 * ``headnna1/``: the HeadNNa1 inputs (non-nested: the north and south meshes
@@ -72,6 +75,12 @@ def main():
         ncol, nrow = (int(x) for x in dims.split(","))
         mat = raw.reshape(ncol, nrow).T.copy()
         np.save(os.path.join(h1, name + ".npy"), mat)
+
+    # the MAT-file goldens themselves, for the file-format tests
+    mat = os.path.join(h1, "mat")
+    os.makedirs(mat, exist_ok=True)
+    for name in ("hm", "dsm", "h2em", "ss2mm", "est_eeg"):
+        shutil.copy(os.path.join(initial, "Head1." + name), os.path.join(mat, "Head1." + name))
 
     path = os.path.join(initial, "Head1.h2em")
     for part, dt in (("data", "<f8"), ("ir", "<u8"), ("jc", "<u8")):
