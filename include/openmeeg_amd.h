@@ -193,6 +193,25 @@ int omh_surf_source_mat(omh_geometry* g, omh_mesh* m, int order, int flags, doub
 int omh_surf_source2meg_mat(omh_mesh* m, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out,
                             int64_t ld, void* stream);
 
+/* ------------------------------------------------------ internal potentials */
+
+/* Surf2VolMat(geo, points) (assembleHeadMat.cpp:325-363): the potential at
+ * points inside the head from the unknowns.  Points (d_points, npts x 3
+ * row-major) in a zero-conductivity domain are dropped; *n_kept rows are
+ * written to d_out (column-major, leading dimension ld >= npts, n columns in
+ * the unknown order `ordering`), and h_kept (npts entries, may be NULL)
+ * receives the input index of each row. */
+int omh_surf2vol_mat(omh_geometry* g, const double* d_points, int64_t npts, double* d_out, int64_t ld,
+                     int32_t ordering, int64_t* h_kept, int64_t* n_kept, void* stream);
+
+/* DipSource2InternalPotMat(geo, dipoles, points, domain) (assembleSourceMat.
+ * cpp:154-186): kept points x n_dip (as omh_surf2vol_mat), K / sigma x the
+ * dipole potential at the points of the dipole's domain (h_domains, or NULL
+ * to locate each dipole), zero elsewhere. */
+int omh_dip_source2internal_pot_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains,
+                                    const double* d_points, int64_t npts, double* d_out, int64_t ld, int64_t* h_kept,
+                                    int64_t* n_kept, void* stream);
+
 /* DipSource2MEGMat (assembleSensors.cpp:133-156) before the sensor weights:
  * d_out (npts x n_dip, column-major) = MagFactor (q x (x - r)) . d /
  * (|x - r|^3 |d|) for dipoles d_dipoles (n_dip x 6 row-major). */

@@ -10,11 +10,15 @@ graph-captured pass (no reference counterparts).  All computation runs in the HI
 (libopenmeeg_amd.so) through its C ABI (include/openmeeg_amd.h).
 """
 from .api import (  # noqa: F401
+    DipSource2InternalPotMat,
     DipSource2MEGMat,
     DipSourceMat,
     Factor,
+    Forward,
     GainEEG,
     GainEEGadjoint,
+    GainEITInternalPot,
+    GainInternalPot,
     GainMEG,
     GainMEGadjoint,
     Geometry,
@@ -26,6 +30,7 @@ from .api import (  # noqa: F401
     Mesh,
     Sensors,
     SparseMatrix,
+    Surf2VolMat,
     SurfSource2MEGMat,
     SurfSourceMat,
     SymMatrix,
@@ -33,6 +38,7 @@ from .api import (  # noqa: F401
     gain_eeg_meg_pipeline,
     gain_eeg_pipeline,
     gain_eeg_surf_pipeline,
+    gain_internal_pot_pipeline,
     mesh_intersects,
     gain_to_numpy,
     make_geometry,

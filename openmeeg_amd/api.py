@@ -859,7 +859,8 @@ def _ds2meg_device(dipoles, sensors):
     dip = _dipoles_array(dipoles)
     npts, nd = sensors.getNumberOfPositions(), len(dip)
     P = _dev_f64(max(1, npts * nd))
-    check(lib().omh_dip_source2meg_mat(_ptr(_to_dev(dip)), nd, _ptr(d_pts), _ptr(d_dirs), npts, _ptr(P),
+    dd = _to_dev(dip)          # held until the call has been enqueued
+    check(lib().omh_dip_source2meg_mat(_ptr(dd), nd, _ptr(d_pts), _ptr(d_dirs), npts, _ptr(P),
                                        max(1, npts), _stream()))
     return _sensor_sum(sensors, P, nd)
 
@@ -977,6 +978,104 @@ def SurfSource2MEGMat(mesh, sensors):
     nsens, ns = sensors.getNumberOfSensors(), mesh.nb_vertices()
     M = _ss2meg_device(mesh, sensors)
     return Matrix(M.cpu().numpy()[:nsens * ns].reshape(ns, nsens).T.copy(order="F"))
+
+
+def _points_array(points):
+    a = points.array() if isinstance(points, Matrix) else np.asarray(points, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] < 3:
+        raise ValueError("points must have 3 columns (x y z)")
+    return np.ascontiguousarray(a[:, :3])
+
+
+def _surf2vol_device(geom, points, ordering=ORDER_TRIANGLES_FIRST):
+    """Surf2VolMat on the device: (kept points x n, column-major, leading
+    dimension the number of points) and the kept point indices."""
+    pts = _points_array(points)
+    npts, n = len(pts), geom.size
+    M = _dev_f64(max(1, npts) * n)
+    kept = np.zeros(max(1, npts), dtype=np.int64)
+    nk = C.c_int64()
+    dp = _to_dev(pts)          # held until the call has been enqueued
+    check(lib().omh_surf2vol_mat(geom._h, _ptr(dp), npts, _ptr(M), max(1, npts), ordering,
+                                 kept.ctypes.data_as(_lib.p_i64), C.byref(nk), _stream()))
+    return M, kept[:nk.value]
+
+
+def Surf2VolMat(geom, points):
+    """Surf2VolMat(geo, points) -- assemble.h:33, assembleHeadMat.cpp:325-363:
+    the potential at the points inside the head from the unknowns (points in
+    a non-conductive domain are dropped)."""
+    pts = _points_array(points)
+    M, kept = _surf2vol_device(geom, pts, ORDER_REFERENCE)
+    ld = max(1, len(pts))
+    A = M.cpu().numpy().reshape(geom.size, ld).T[:len(kept)]
+    return Matrix(np.asfortranarray(A))
+
+
+def DipSource2InternalPotMat(geom, dipoles, points, domain_name=""):
+    """DipSource2InternalPotMat(geo, dipoles, points, domain) --
+    assembleSourceMat.h:32, assembleSourceMat.cpp:154-186."""
+    dip = _dipoles_array(dipoles)
+    pts = _points_array(points)
+    nd, npts = len(dip), len(pts)
+    ld = max(1, npts)
+    D = _dev_f64(ld * max(1, nd))
+    doms = None
+    if domain_name:
+        doms = (C.c_int32 * nd)(*([geom.domain_index(domain_name)] * nd))
+    kept = np.zeros(ld, dtype=np.int64)
+    nk = C.c_int64()
+    # both device copies are held until the call has been enqueued: a
+    # temporary freed inside the argument list can be handed to the next
+    # allocation (the points) and overwritten before the kernels run
+    dd, dp = _to_dev(dip), _to_dev(pts)
+    check(lib().omh_dip_source2internal_pot_mat(geom._h, _ptr(dd), nd, doms, _ptr(dp), npts,
+                                                _ptr(D), ld, kept.ctypes.data_as(_lib.p_i64), C.byref(nk),
+                                                _stream()))
+    A = D.cpu().numpy()[:ld * nd].reshape(nd, ld).T[:nk.value]
+    return Matrix(np.asfortranarray(A))
+
+
+def GainInternalPot(hminv, dsm, h2ip, ds2ip):
+    """GainInternalPot(HeadMatInv, SourceMat, Head2IPMat, Source2IPMat) =
+    Source2IP + (Head2IP H^-1) SourceMat -- gain.h:138-144."""
+    return _gain_with_inverse(hminv, dsm, h2ip, ds2ip)
+
+
+def GainEITInternalPot(hminv, sm, h2ip):
+    """GainEITInternalPot(HeadMatInv, SourceMat, Head2IPMat) = (Head2IP
+    H^-1) SourceMat -- gain.h:146-152."""
+    return _gain_with_inverse(hminv, sm, h2ip, None)
+
+
+def _gain_with_inverse(hminv, src, rows, add):
+    n, nv = hminv.nlin(), hminv.nv
+    if nv is None:
+        raise ValueError("the gain needs the HeadMat unknown layout (a SymMatrix from HeadMat)")
+    Hi = hminv.device()
+    R = rows.array()
+    nr = R.shape[0]
+    Rd = to_device_rows(np.asfortranarray(R.T), n, nv)             # (n x nr) triangles-first rows
+    RH = _dev_f64(nr * n)
+    check(lib().omh_gemm_f64(1, 0, nr, n, n, 1.0, _ptr(Rd), n, _ptr(Hi), n, 0.0, _ptr(RH), nr, _stream()))
+    D = src.array()
+    Dd = to_device_rows(D, n, nv)
+    nd = D.shape[1]
+    G = _to_dev(np.asfortranarray(add.array()).ravel(order="F")) if add is not None else _dev_f64(nr * nd)
+    check(lib().omh_gemm_f64(0, 0, nr, nd, n, 1.0, _ptr(RH), nr, _ptr(Dd), n, 1.0 if add is not None else 0.0,
+                             _ptr(G), nr, _stream()))
+    return Matrix(G.cpu().numpy().reshape(nd, nr).T.copy(order="F"))
+
+
+def Forward(gain, sources, noise_level=0.0, rng=None):
+    """Forward(GainMatrix, RealSourcesData, NoiseLevel) -- forward.h:14-36:
+    gain x sources plus Gaussian noise of standard deviation noise_level."""
+    G = gain.array() if isinstance(gain, Matrix) else np.asarray(gain, dtype=np.float64)
+    X = sources.array() if isinstance(sources, Matrix) else np.asarray(sources, dtype=np.float64)
+    Y = G @ X
+    if noise_level > 0:
+        Y = Y + (rng or np.random.default_rng()).normal(0.0, noise_level, Y.shape)
+    return Matrix(np.asfortranarray(Y))
 
 
 def GainMEG(hminv, dsm, h2mm, ds2mm):
@@ -1149,6 +1248,36 @@ def gain_eeg_surf_pipeline(geom, mesh, electrodes, squids=None, precision="fp64"
                 n_sources=nsrc)
     del F, H, D, Y
     return Ge, Gm, info
+
+
+def gain_internal_pot_pipeline(geom, dipoles, points, precision="fp64", domain_name="", integrator=None,
+                               dsm_integrator=None, max_refine=10, refine_tol=1e-8, far_field=None):
+    """GainInternalPot on the device in the adjoint form: one factor of the
+    HeadMat, one solve with the Surf2VolMat rows, then G = Y^T DSM +
+    DipSource2InternalPotMat.  Returns (G as a device tensor, column-major
+    kept points x dipoles, kept point indices, info)."""
+    if far_field is None:
+        far_field = precision != "fp64"
+    torch = _t()
+    n = geom.size
+    pts = _points_array(points)
+    H = _head_mat_device(geom, integrator, far_field)
+    D = _dsm_device(geom, dipoles, domain_name, dsm_integrator)
+    M, kept = _surf2vol_device(geom, pts)
+    ld, nk = max(1, len(pts)), len(kept)
+    nd = len(_dipoles_array(dipoles))
+    S2 = DipSource2InternalPotMat(geom, dipoles, pts, domain_name).array()
+    # M is (points x n, leading dimension ld): its first nk rows, transposed,
+    # are the right-hand sides (n x nk column-major, triangles-first rows)
+    Y = M[:ld * n].view(n, ld)[:, :nk].t().contiguous().view(-1)
+    F = factor(H, n, precision, structure=geom.factor_segments())
+    F.solve(Y, nk, max_iter=max_refine, tol=refine_tol)
+    G = _to_dev(np.asfortranarray(S2).ravel(order="F"))
+    check(lib().omh_gemm_f64(1, 0, nk, nd, n, 1.0, _ptr(Y), n, _ptr(D), n, 1.0, _ptr(G), nk, _stream()))
+    torch.cuda.synchronize()
+    info = dict(refine_iterations=F.iterations, n=n, n_points=nk, n_dipoles=nd)
+    del F, H, D, Y
+    return G, kept, info
 
 
 def gain_to_numpy(G, ne, nd):

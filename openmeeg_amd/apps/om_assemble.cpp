@@ -11,6 +11,8 @@
 //   -Head2MEGMat, -H2MM, -h2mm         geometry conductivity squids output
 //   -DipSource2MEGMat, -DS2MM, -ds2mm  dipoles squids output
 //   -SurfSource2MEGMat, -SS2MM, -ss2mm sourcemesh squids output
+//   -Head2InternalPotMat, -H2IPM, -h2ipm        geometry conductivity points output
+//   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm geometry conductivity dipoles points output [domain]
 #include <algorithm>
 #include <cstring>
 #include <iostream>
@@ -154,6 +156,64 @@ void surf2meg(const std::string& meshf, const std::string& sqf, const std::strin
     write_full(out, sensor_sum(s, P.down(), ms[0]));
 }
 
+// Points: a full n x 3 (or more) matrix file (Matrix(filename)), row-major.
+std::vector<double> read_points(const std::string& path, int64_t& np) {
+    const Full m = read_full(path);
+    if (m.ncol < 3) throw std::runtime_error(path + ": points need 3 columns");
+    np = m.nlin;
+    std::vector<double> p((size_t)(np * 3));
+    for (int64_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) p[i * 3 + c] = m(i, c);
+    return p;
+}
+
+void head2ip(const std::string& geom, const std::string& cond, const std::string& ptf, const std::string& out) {
+    Geom g(geom, cond);
+    int64_t np = 0;
+    const std::vector<double> pts = read_points(ptf, np);
+    const int64_t n = g.n(), ld = std::max<int64_t>(1, np);
+    Dev<double> dp(pts.data(), pts.size());
+    Dev<double> M((size_t)(ld * n));
+    int64_t nk = 0;
+    check(omh_surf2vol_mat(g.g, dp.p, np, M.p, ld, OMH_ORDER_REFERENCE, nullptr, &nk, nullptr));
+    const std::vector<double> h = M.down();
+    Full m;
+    m.nlin = nk;
+    m.ncol = n;
+    m.a.resize((size_t)(nk * n));
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < nk; ++i) m(i, j) = h[i + j * ld];
+    write_full(out, m);
+}
+
+void dip2ip(const std::string& geom, const std::string& cond, const std::string& dipf, const std::string& ptf,
+            const std::string& out, const char* domain) {
+    Geom g(geom, cond);
+    int64_t nd = 0, np = 0;
+    const std::vector<double> dip = read_dipoles(dipf, nd);
+    const std::vector<double> pts = read_points(ptf, np);
+    const int64_t ld = std::max<int64_t>(1, np);
+    std::vector<int32_t> dom;
+    if (domain) {
+        int32_t k = -1;
+        check(omh_geometry_domain_index(g.g, domain, &k));
+        dom.assign((size_t)nd, k);
+    }
+    Dev<double> dd(dip.data(), dip.size()), dp(pts.data(), pts.size());
+    Dev<double> M((size_t)(ld * std::max<int64_t>(1, nd)));
+    int64_t nk = 0;
+    check(omh_dip_source2internal_pot_mat(g.g, dd.p, nd, domain ? dom.data() : nullptr, dp.p, np, M.p, ld, nullptr,
+                                          &nk, nullptr));
+    const std::vector<double> h = M.down();
+    Full m;
+    m.nlin = nk;
+    m.ncol = nd;
+    m.a.resize((size_t)(nk * nd));
+    for (int64_t j = 0; j < nd; ++j)
+        for (int64_t i = 0; i < nk; ++i) m(i, j) = h[i + j * ld];
+    write_full(out, m);
+}
+
 void help(const char* cmd) {
     std::cout << cmd << " [-option] [filepaths...]\n\n"
               << "   -HeadMat, -HM, -hm :                 geometry conductivity output\n"
@@ -164,6 +224,8 @@ void help(const char* cmd) {
               << "   -Head2MEGMat, -H2MM, -h2mm :         geometry conductivity squids output\n"
               << "   -DipSource2MEGMat, -DS2MM, -ds2mm :  dipoles squids output\n"
               << "   -SurfSource2MEGMat, -SS2MM, -ss2mm : sourcemesh squids output\n"
+              << "   -Head2InternalPotMat, -H2IPM, -h2ipm : geometry conductivity points output\n"
+              << "   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm : geometry conductivity dipoles points output [domain]\n"
               << "Output format from the suffix: .mat (MAT 7.3), .bin, .txt\n";
 }
 
@@ -203,6 +265,13 @@ int main(int argc, char** argv) {
             } else if (is(o, {"-SurfSource2MEGMat", "-SS2MM", "-ss2mm"})) {
                 need_args(argc, k, 3, o);
                 surf2meg(argv[k + 1], argv[k + 2], argv[k + 3]);
+            } else if (is(o, {"-Head2InternalPotMat", "-H2IPM", "-h2ipm"})) {
+                need_args(argc, k, 4, o);
+                head2ip(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);
+            } else if (is(o, {"-DipSource2InternalPotMat", "-DS2IPM", "-ds2ipm"})) {
+                need_args(argc, k, 5, o);
+                const char* domain = (k + 6 < argc && argv[k + 6][0] != '-') ? argv[k + 6] : nullptr;
+                dip2ip(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4], argv[k + 5], domain);
             } else {
                 throw std::runtime_error(std::string("option ") + o + " is not supported by this build");
             }

@@ -7,6 +7,8 @@
 //   -MEGadjoint    geometry conductivity dipoles HeadMat Head2MEGMat Source2MEGMat output
 //   -EEGMEGadjoint geometry conductivity dipoles HeadMat Head2EEGMat Head2MEGMat Source2MEGMat
 //                  outputEEG outputMEG                                       (gain.h:105-128)
+//   -InternalPotential, -IP     HeadMatInv SourceMat Head2IPMat Source2IPMat output (gain.h:138-144)
+//   -EITInternalPotential, -EITIP HeadMatInv SourceMat Head2IPMat output         (gain.h:146-152)
 // The adjoint forms factor the HeadMat once (pivot-free signed Cholesky in
 // the triangles-first unknown order of the geometry) and solve with the
 // sensor rows; the dipole source matrix is assembled on the device.
@@ -148,6 +150,16 @@ int main(int argc, char** argv) {
             if (S.nlin != G.nlin || S.ncol != G.ncol) throw std::runtime_error("Source2MEGMat size mismatch");
             for (size_t q = 0; q < G.a.size(); ++q) G.a[q] = S.a[q] + G.a[q];
             write_full(argv[6], G);
+        } else if (is(o, "-InternalPotential") || is(o, "-IP") || is(o, "-ip")) {
+            need_args(argc, k, 5, o);   // HeadMatInv SourceMat Head2IPMat Source2IPMat output (gain.h:138-144)
+            Full G = gain_with_inverse(argv[2], read_full(argv[4]), read_full(argv[3]));
+            const Full S = read_full(argv[5]);
+            if (S.nlin != G.nlin || S.ncol != G.ncol) throw std::runtime_error("Source2IPMat size mismatch");
+            for (size_t q = 0; q < G.a.size(); ++q) G.a[q] = S.a[q] + G.a[q];
+            write_full(argv[6], G);
+        } else if (is(o, "-EITInternalPotential") || is(o, "-EITIP") || is(o, "-eitip")) {
+            need_args(argc, k, 4, o);   // HeadMatInv SourceMat Head2IPMat output (gain.h:146-152)
+            write_full(argv[5], gain_with_inverse(argv[2], read_full(argv[4]), read_full(argv[3])));
         } else if (is(o, "-EEGadjoint")) {
             need_args(argc, k, 6, o);
             const Full E = dense_of(read_sparse(argv[6]));

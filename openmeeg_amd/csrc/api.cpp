@@ -510,6 +510,51 @@ int omh_surf_source2meg_mat(omh_mesh* m, const double* d_pts, const double* d_di
     });
 }
 
+int omh_surf2vol_mat(omh_geometry* g, const double* d_points, int64_t npts, double* d_out, int64_t ld,
+                     int32_t ordering, int64_t* h_kept, int64_t* n_kept, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(n_kept, "n_kept");
+        if (npts < 0 || ld < std::max<int64_t>(1, npts)) throw std::invalid_argument("bad sizes");
+        if (npts > 0) {
+            need(d_points, "d_points");
+            need(d_out, "d_out");
+        }
+        omh::Layout L = layout_of(*g->g, g->g->size(), ordering);
+        L.ld = ld;
+        const std::vector<int64_t> kept = omh::surf2vol(*g->g, d_points, npts, d_out, L, S(stream));
+        *n_kept = (int64_t)kept.size();
+        if (h_kept)
+            for (size_t i = 0; i < kept.size(); ++i) h_kept[i] = kept[i];
+    });
+}
+
+int omh_dip_source2internal_pot_mat(omh_geometry* g, const double* d_dipoles, int64_t n_dip, const int32_t* h_domains,
+                                    const double* d_points, int64_t npts, double* d_out, int64_t ld, int64_t* h_kept,
+                                    int64_t* n_kept, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(n_kept, "n_kept");
+        if (npts < 0 || n_dip < 0 || ld < std::max<int64_t>(1, npts)) throw std::invalid_argument("bad sizes");
+        if (npts > 0) need(d_points, "d_points");
+        if (n_dip > 0) need(d_dipoles, "d_dipoles");
+        if (npts > 0 && n_dip > 0) need(d_out, "d_out");
+        std::vector<int> dom;
+        if (h_domains) {
+            dom.assign(h_domains, h_domains + n_dip);
+            for (int d : dom)
+                if (d < 0 || d >= (int)g->g->domains.size()) throw std::invalid_argument("domain index out of range");
+        } else if (n_dip > 0) {
+            dom = omh::locate_domains(*g->g, d_dipoles, 6, n_dip, S(stream));
+        }
+        const std::vector<int64_t> kept =
+            omh::dip2internal_pot(*g->g, d_dipoles, n_dip, dom, d_points, npts, d_out, ld, S(stream));
+        *n_kept = (int64_t)kept.size();
+        if (h_kept)
+            for (size_t i = 0; i < kept.size(); ++i) h_kept[i] = kept[i];
+    });
+}
+
 int omh_dip_source2meg_mat(const double* d_dipoles, int64_t n_dip, const double* d_pts, const double* d_dirs,
                            int64_t npts, double* d_out, int64_t ld, void* stream) {
     return guard([&] {

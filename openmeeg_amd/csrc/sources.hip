@@ -492,6 +492,78 @@ __global__ void __launch_bounds__(256) k_dip2meg(const double* __restrict__ dip,
     }
 }
 
+// Surf2VolMat (assembleHeadMat.cpp:325-363, PartialBlock operators.h:302-
+// 331) for one mesh.  Point r of the kept points sits at pts[3 kept[r]];
+// cD[r] (cS[r]) is its coefficient for this mesh's vertex (triangle)
+// columns, zero when the mesh does not bound the point's domain.
+// D part, thread = vertex slot of the mesh (x), kept point (block y,
+// looped): out(r, v) += cD sum over the triangles T of v (mesh order) of
+// P1SingleLayerPotential_T(x)[position of v in T].
+__global__ void __launch_bounds__(256) k_surf2vol_d(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
+                                                    const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+                                                    const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
+                                                    const double* __restrict__ pts, const int64_t* __restrict__ kept,
+                                                    int64_t nk, const double* __restrict__ cD, double* __restrict__ out,
+                                                    Layout L) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int64_t vi = vert_unk[slot_vid[s0 + s]];
+    if (vi < 0) return;
+    const int64_t col = dev_index(L, vi);
+    for (int64_t r = blockIdx.y; r < nk; r += gridDim.y) {
+        const double c = cD[r];
+        if (c == 0.0) continue;
+        const d3 x = ld3(pts + 3 * kept[r]);
+        double acc = 0.0;
+        for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
+            const AdjRec e = adj[p];
+            const d3 v = p1s_eval(load_p1s(tri[e.tri]), x);
+            acc += e.k == 0 ? v.x : (e.k == 1 ? v.y : v.z);
+        }
+        out[r + col * L.ld] += c * acc;
+    }
+}
+
+// S part, thread = triangle of the mesh (x), kept point (block y, looped):
+// out(r, t) = cS OperatorS_T(x) (the reference assigns these entries).
+__global__ void __launch_bounds__(256) k_surf2vol_s(const TriRec* __restrict__ tri, int64_t t0, int64_t nt,
+                                                    int64_t unk0, const double* __restrict__ pts,
+                                                    const int64_t* __restrict__ kept, int64_t nk,
+                                                    const double* __restrict__ cS, double* __restrict__ out, Layout L) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int64_t col = dev_index(L, unk0 + t);
+    const OpS o = load_ops(tri[t0 + t]);
+    for (int64_t r = blockIdx.y; r < nk; r += gridDim.y) {
+        const double c = cS[r];
+        if (c == 0.0) continue;
+        out[r + col * L.ld] = c * ops_eval(o, ld3(pts + 3 * kept[r]));
+    }
+}
+
+// DipSource2InternalPotMat (assembleSourceMat.cpp:154-186): out(r, j) =
+// coef[j] q_j . (x - r_j) / |x - r_j|^3 when kept point r lies in dipole
+// j's domain (pdom[r] == ddom[j]), else 0.  Thread = kept point (x),
+// dipole (block y, looped).
+__global__ void __launch_bounds__(256) k_dip2ip(const double* __restrict__ dip, int64_t n_dip,
+                                                const int32_t* __restrict__ ddom, const double* __restrict__ coef,
+                                                const double* __restrict__ pts, const int64_t* __restrict__ kept,
+                                                const int32_t* __restrict__ pdom, int64_t nk, double* __restrict__ out,
+                                                int64_t ld) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nk) return;
+    const d3 x = ld3(pts + 3 * kept[r]);
+    for (int64_t j = blockIdx.y; j < n_dip; j += gridDim.y) {
+        double v = 0.0;
+        if (pdom[r] == ddom[j]) {
+            const d3 diff = sub(x, ld3(dip + 6 * j));
+            const double nd = norm(diff);
+            v = coef[j] * dot(ld3(dip + 6 * j + 3), diff) / (nd * nd * nd);
+        }
+        out[r + j * ld] = v;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -733,6 +805,104 @@ void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const doub
     const dim3 grid((unsigned)cdiv(npts, 256), (unsigned)std::min<int64_t>(n_dip, 65535));
     k_dip2meg<<<grid, 256, 0, st>>>(d_dip, n_dip, d_pts, d_dirs, npts, d_out, ld_out);
     OMH_LAUNCH_CHECK();
+}
+
+// Kept points (those of a domain with non-zero conductivity, in input
+// order) with their domains; their indices go to the device in `slot`.
+static std::vector<int64_t> kept_points(Geometry& g, const double* d_pts, int64_t npts, hipStream_t st,
+                                        std::vector<int>& kdom) {
+    const std::vector<int> dom = locate_domains(g, d_pts, 3, npts, st);
+    std::vector<int64_t> kept;
+    kdom.clear();
+    for (int64_t i = 0; i < npts; ++i)
+        if (g.domains[dom[i]].cond != 0.0) {
+            kept.push_back(i);
+            kdom.push_back(dom[i]);
+        }
+    return kept;
+}
+
+std::vector<int64_t> surf2vol(Geometry& g, const double* d_pts, int64_t npts, double* d_out, const Layout& L,
+                              hipStream_t st) {
+    std::vector<int> kdom;
+    const std::vector<int64_t> kept = npts > 0 ? kept_points(g, d_pts, npts, st, kdom) : std::vector<int64_t>();
+    const int64_t nk = (int64_t)kept.size(), n = g.size();
+    if (nk == 0) return kept;
+    fill_f64(d_out, nk, n, L.ld, 0.0, st);
+    const int nm = (int)g.meshes.size();
+    // coefficients per (mesh, kept point): D = -K x the boundary orientation,
+    // S = K x orientation / sigma (not on a current barrier)
+    std::vector<double> cD((size_t)nm * nk, 0.0), cS((size_t)nm * nk, 0.0);
+    std::vector<char> used(nm, 0);
+    for (int64_t r = 0; r < nk; ++r) {
+        const Domain& d = g.domains[kdom[r]];
+        for (const auto& b : d.boundaries)
+            for (const auto& om : b.first.omeshes) {
+                const double coeff = (b.second ? om.second : -om.second) * kK;
+                cD[(size_t)om.first * nk + r] += -coeff;
+                if (!g.meshes[om.first].current_barrier) cS[(size_t)om.first * nk + r] = coeff / d.cond;
+                used[om.first] = 1;
+            }
+    }
+    DeviceGeometry& dg = device_geometry(g);
+    char* buf = (char*)scratch(SCR_SRCCOEF, sizeof(int64_t) * nk + 2 * sizeof(double) * cD.size());
+    int64_t* d_kept = (int64_t*)buf;
+    double* d_cD = (double*)(buf + sizeof(int64_t) * nk);
+    double* d_cS = d_cD + cD.size();
+    OMH_HIP(hipMemcpyAsync(d_kept, kept.data(), sizeof(int64_t) * nk, hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_cD, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_cS, cS.data(), sizeof(double) * cS.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));   // the host vectors die with this frame
+    const unsigned gy = (unsigned)std::min<int64_t>(nk, 65535);
+    for (int m = 0; m < nm; ++m) {
+        if (!used[m]) continue;
+        const MeshDev& md = dg.meshes[m];
+        const int64_t ns = md.slot_end - md.slot_begin, nt = md.tri_end - md.tri_begin;
+        if (ns > 0) {
+            k_surf2vol_d<<<dim3((unsigned)cdiv(ns, 256), gy), 256, 0, st>>>(
+                dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, md.slot_begin, ns, d_pts, d_kept, nk,
+                d_cD + (size_t)m * nk, d_out, L);
+            OMH_LAUNCH_CHECK();
+        }
+        if (nt > 0 && !md.barrier) {
+            k_surf2vol_s<<<dim3((unsigned)cdiv(nt, 256), gy), 256, 0, st>>>(
+                dg.tri, md.tri_begin, nt, md.tri_unk0, d_pts, d_kept, nk, d_cS + (size_t)m * nk, d_out, L);
+            OMH_LAUNCH_CHECK();
+        }
+    }
+    return kept;
+}
+
+std::vector<int64_t> dip2internal_pot(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain,
+                                      const double* d_pts, int64_t npts, double* d_out, int64_t ld_out,
+                                      hipStream_t st) {
+    std::vector<int> kdom;
+    const std::vector<int64_t> kept = npts > 0 ? kept_points(g, d_pts, npts, st, kdom) : std::vector<int64_t>();
+    const int64_t nk = (int64_t)kept.size();
+    if (nk == 0 || n_dip <= 0) return kept;
+    std::vector<int32_t> pdom(kdom.begin(), kdom.end()), ddom(n_dip);
+    std::vector<double> coef(n_dip);
+    for (int64_t j = 0; j < n_dip; ++j) {
+        ddom[j] = domain[j];
+        const double c = g.domains.at(domain[j]).cond;
+        // K / sigma (a dipole in a zero-conductivity domain has no point in it)
+        coef[j] = c != 0.0 ? kK / c : 0.0;
+    }
+    char* buf = (char*)scratch(SCR_SRCCOEF, sizeof(int64_t) * nk + sizeof(double) * n_dip +
+                                              sizeof(int32_t) * (nk + n_dip));
+    int64_t* d_kept = (int64_t*)buf;
+    double* d_coef = (double*)(buf + sizeof(int64_t) * nk);
+    int32_t* d_pdom = (int32_t*)(d_coef + n_dip);
+    int32_t* d_ddom = d_pdom + nk;
+    OMH_HIP(hipMemcpyAsync(d_kept, kept.data(), sizeof(int64_t) * nk, hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_coef, coef.data(), sizeof(double) * n_dip, hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_pdom, pdom.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, st));
+    OMH_HIP(hipMemcpyAsync(d_ddom, ddom.data(), sizeof(int32_t) * n_dip, hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    k_dip2ip<<<dim3((unsigned)cdiv(nk, 256), (unsigned)std::min<int64_t>(n_dip, 65535)), 256, 0, st>>>(
+        d_dip, n_dip, d_ddom, d_coef, d_pts, d_kept, d_pdom, nk, d_out, ld_out);
+    OMH_LAUNCH_CHECK();
+    return kept;
 }
 
 }  // namespace omh

@@ -60,6 +60,21 @@ void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t np
 void surf2meg(Geometry& src, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
               hipStream_t st);
 
+// Surf2VolMat (assembleHeadMat.cpp:325-363): for the npts points d_pts
+// (npts x 3 row-major) the rows of the points that lie in a domain of
+// non-zero conductivity (the others are dropped; their input indices are
+// returned, in order), n columns in the unknown order of L, written to
+// d_out (leading dimension L.ld >= number of kept points).
+std::vector<int64_t> surf2vol(Geometry& g, const double* d_pts, int64_t npts, double* d_out, const Layout& L,
+                              hipStream_t st);
+
+// DipSource2InternalPotMat (assembleSourceMat.cpp:154-186): kept points x
+// n_dip, K / sigma x the dipole potential at the points of the dipole's
+// domain (domain[j]).  Returns the kept points as surf2vol.
+std::vector<int64_t> dip2internal_pot(Geometry& g, const double* d_dip, int64_t n_dip, const std::vector<int>& domain,
+                                      const double* d_pts, int64_t npts, double* d_out, int64_t ld_out,
+                                      hipStream_t st);
+
 // DipSource2MEGMat before the sensor weights (assembleSensors.cpp:133-156):
 // d_out (npts x n_dip, column-major) for dipoles d_dip (n_dip x 6 row-major).
 void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,

@@ -505,6 +505,79 @@ def surf_source_mat(g, src_pts, src_tris, order=3, stats=None):
 
 
 # ---------------------------------------------------------------------------
+# Internal potentials
+# ---------------------------------------------------------------------------
+def _conducting_points(g, points):
+    """Points in a domain of non-zero conductivity, in input order, and their
+    domains (the others are dropped, assembleHeadMat.cpp:331-340)."""
+    ol = OracleLib()
+    kept, doms = [], []
+    for i, p in enumerate(np.asarray(points, dtype=np.float64)):
+        d = g.domain_of_point(p, ol)
+        if g.domains[d].cond != 0.0:
+            kept.append(i)
+            doms.append(d)
+    return np.asarray(kept, dtype=np.int64), doms
+
+
+def surf2vol(g, points):
+    """Surf2VolMat (assembleHeadMat.cpp:325-363) with PartialBlock
+    (operators.h:302-331): for a point x of domain Omega and each mesh of
+    Omega's boundaries, coeff = K x the boundary's orientation of the mesh,
+      row(x) at the mesh's vertices += -coeff x P1SingleLayerPotential_T(x)
+        (summed over the triangles T of the vertex),
+      row(x) at the mesh's triangles  = coeff / sigma_Omega x OperatorS_T(x)
+        (not for a current-barrier mesh).
+    Returns (matrix of the kept points x n, indices of the kept points)."""
+    L = lib()
+    pts = np.asarray(points, dtype=np.float64)
+    kept, doms = _conducting_points(g, pts)
+    verts = np.array(g.vertices, dtype=np.float64)
+    out = np.zeros((len(kept), g.size))
+    r3 = np.zeros(3)
+    for row, (i, d) in enumerate(zip(kept, doms)):
+        dom = g.domains[d]
+        x = np.ascontiguousarray(pts[i])
+        for iface, inside in dom.boundaries:
+            for mid, o in iface.omeshes:
+                m = g.meshes[mid]
+                coeff = (o if inside else -o) * K
+                for kt, t in enumerate(m.tris):
+                    t9 = np.ascontiguousarray(verts[t].reshape(9))
+                    L.orc_p1slp(_p(t9), _p(x), _p(r3))
+                    for k in range(3):
+                        out[row, g.vert_index[t[k]]] += -coeff * r3[k]
+                    if not m.current_barrier:
+                        out[row, m.tri_index[kt]] = coeff / dom.cond * L.orc_operator_s(_p(t9), _p(x))
+    return out, kept
+
+
+def dip_source2internal_pot(g, dipoles, points, domain_name=""):
+    """DipSource2InternalPotMat (assembleSourceMat.cpp:154-186): K / sigma x
+    the dipole's potential q . (x - r) / |x - r|^3 at the kept points of the
+    dipole's domain, zero elsewhere."""
+    ol = OracleLib()
+    pts = np.asarray(points, dtype=np.float64)
+    kept, doms = _conducting_points(g, pts)
+    dip = np.asarray(dipoles, dtype=np.float64)
+    out = np.zeros((len(kept), len(dip)))
+    for j, d6 in enumerate(dip):
+        dk = g.domain_index(domain_name) if domain_name else g.domain_of_point(d6[:3], ol)
+        coeff = K / g.domains[dk].cond
+        for row, (i, d) in enumerate(zip(kept, doms)):
+            if d == dk:
+                diff = pts[i] - d6[:3]
+                out[row, j] += coeff * np.dot(d6[3:], diff) / np.linalg.norm(diff) ** 3
+    return out
+
+
+def gain_internal_pot(H, dsm, h2ip, ds2ip):
+    """GainInternalPot (gain.h:138-144): Source2IP + Head2IP H^-1 SourceMat."""
+    import scipy.linalg as sla
+    return ds2ip + h2ip @ sla.solve(H, dsm, assume_a="sym")
+
+
+# ---------------------------------------------------------------------------
 # CPU baseline (bench.py's cpu_baseline leg)
 # ---------------------------------------------------------------------------
 def time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=6, n_dip_sample=16, n_factor=4000):

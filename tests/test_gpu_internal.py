@@ -1,0 +1,139 @@
+"""Internal potentials on the HIP path: Surf2VolMat, DipSource2InternalPotMat,
+GainInternalPot and the device pipeline, against the CPU oracle (Head1 with
+the reference's internal points, a 3-layer sphere head with points in every
+layer) and the exact reproduction of linear fields; the command-line
+options -H2IPM / -DS2IPM / om_gain -IP.  Needs a HIP GPU.
+
+Tolerances (relative Frobenius): matrices vs the oracle 1e-13, gains 1e-10,
+mixed precision vs FP64 1e-6."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+H1 = os.path.join(GOLDEN, "head1")
+POINTS = os.path.join(GOLDEN, "analytic", "eeg_internal_points.txt")
+
+
+@pytest.fixture(scope="module")
+def om():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import openmeeg_amd
+    return openmeeg_amd
+
+
+def test_internal_potential_head1_vs_oracle(om):
+    from oracle import oracle as O
+    gf, cf = os.path.join(H1, "Head1.geom"), os.path.join(H1, "Head1.cond")
+    g, og = om.read_geometry(gf, cf), O.load_geometry(gf, cf)
+    pts = np.loadtxt(POINTS, usecols=(0, 1, 2))
+    dip = O.read_dipoles(os.path.join(H1, "Head1.dip"))
+    print("Head1 internal potentials: %d points, %d dipoles" % (len(pts), len(dip)), flush=True)
+    Mref, kept = O.surf2vol(og, pts)
+    Sref = O.dip_source2internal_pot(og, dip, pts)
+    M = om.Surf2VolMat(g, pts).array()
+    S2 = om.DipSource2InternalPotMat(g, dip, pts).array()
+    e_m, e_s = rel_fro(M, Mref), rel_fro(S2, Sref)
+    H = O.unpack(np.load(os.path.join(H1, "hm.npy")), g.size)
+    Gref = O.gain_internal_pot(H, np.load(os.path.join(H1, "dsm.npy")), Mref, Sref)
+    hm = om.HeadMat(g)
+    dsm = om.DipSourceMat(g, dip, "Brain")
+    e_g = rel_fro(om.GainInternalPot(hm.inverse(), dsm, om.Matrix(M), om.Matrix(S2)).array(), Gref)
+    Gp, kp, _ = om.gain_internal_pot_pipeline(g, dip, pts, precision="fp64", domain_name="Brain")
+    e_p = rel_fro(om.gain_to_numpy(Gp, len(kp), len(dip)), Gref)
+    Gm, _, info = om.gain_internal_pot_pipeline(g, dip, pts, precision="mixed16", domain_name="Brain")
+    e_x = rel_fro(om.gain_to_numpy(Gm, len(kp), len(dip)), om.gain_to_numpy(Gp, len(kp), len(dip)))
+    print("  h2ip %.3e ds2ip %.3e GainInternalPot %.3e pipeline %.3e mixed16 vs fp64 %.3e"
+          % (e_m, e_s, e_g, e_p, e_x), flush=True)
+    assert list(kp) == list(kept)
+    assert e_m < 1e-13 and e_s < 1e-14 and e_g < 1e-10 and e_p < 1e-10 and e_x < 1e-6
+
+
+def test_internal_potential_layers_and_linear_fields(om):
+    """Points in every layer of a 3-layer sphere head (and outside it, where
+    they are dropped): Surf2VolMat in both unknown orders and
+    DipSource2InternalPotMat against the oracle; on the device, Surf2VolMat
+    reproduces a linear potential from its boundary values to rounding in
+    the inner domain and in a shell with sigma = 2."""
+    import torch
+    from oracle import oracle as O
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _surf2vol_device
+    meshes = S.nested_sphere_meshes(4, (0.5, 0.8, 1.0))
+    cond = (1.0, 2.0, 1.0)
+    g, og = om.make_nested_geometry(meshes, cond), O.nested_geometry(meshes, cond)
+    rng = np.random.default_rng(2)
+    d = rng.standard_normal((40, 3))
+    pts = d * (rng.random(40) * 1.2 / np.linalg.norm(d, axis=1))[:, None]
+    dip = S.random_dipoles(rng, 5, 0.4)
+    print("sphere head: %d points, %d dipoles" % (len(pts), len(dip)), flush=True)
+    Mref, kept = O.surf2vol(og, pts)
+    assert 0 < len(kept) < len(pts)
+    e_ref = rel_fro(om.Surf2VolMat(g, pts).array(), Mref)
+    n, nv = g.size, g.n_vertex_unknowns
+    perm = np.r_[np.arange(nv, n), np.arange(nv)]
+    Md, kd = _surf2vol_device(g, pts)
+    A = Md.cpu().numpy().reshape(n, len(pts)).T[:len(kd)]
+    e_tf = rel_fro(A, Mref[:, perm])
+    e_s = rel_fro(om.DipSource2InternalPotMat(g, dip, pts).array(), O.dip_source2internal_pot(og, dip, pts))
+    a = np.array([0.3, -0.5, 0.8])
+    verts = np.array(og.vertices)
+    errs = []
+    for rmin, rmax, ms, sigma in ((0.05, 0.45, (0,), 1.0), (0.55, 0.72, (0, 1), 2.0)):
+        q = rng.standard_normal((16, 3))
+        q *= ((rmin + (rmax - rmin) * rng.random(16)) / np.linalg.norm(q, axis=1))[:, None]
+        x = np.zeros(n)
+        for mi in ms:
+            m = og.meshes[mi]
+            for v in m.verts:
+                x[og.vert_index[v]] = verts[v] @ a
+            for kt, t in enumerate(m.tris):
+                P = verts[t]
+                nrm = np.cross(P[1] - P[0], P[2] - P[0])
+                nrm /= np.linalg.norm(nrm)
+                x[m.tri_index[kt]] = sigma * (nrm @ a) * (1 if nrm @ P.mean(axis=0) > 0 else -1)
+        u = om.Surf2VolMat(g, q).array() @ x
+        errs.append(np.abs(u - q @ a).max())
+    torch.cuda.synchronize()
+    print("  vs oracle %.3e, triangles-first %.3e, ds2ip %.3e, linear fields %s" % (e_ref, e_tf, e_s, errs),
+          flush=True)
+    assert e_ref < 1e-13 and e_tf < 1e-13 and e_s < 1e-14 and max(errs) < 1e-13
+
+
+def test_internal_potential_cli(om, tmp_path):
+    """om_assemble -H2IPM / -DS2IPM and om_gain -IP on Head1 give the
+    matrices of the Python surface."""
+    from openmeeg_amd import io as omio
+    BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "openmeeg_amd", "bin")
+    gf, cf, df = os.path.join(H1, "Head1.geom"), os.path.join(H1, "Head1.cond"), os.path.join(H1, "Head1.dip")
+    t = lambda f: str(tmp_path / f)
+
+    def run(tool, *args):
+        print("run", tool, args[0], flush=True)
+        r = subprocess.run([os.path.join(BIN, tool)] + list(args), capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+
+    run("om_assemble", "-H2IPM", gf, cf, POINTS, t("h2ip.mat"))
+    run("om_assemble", "-DS2IPM", gf, cf, df, POINTS, t("ds2ip.bin"))
+    run("om_assemble", "-HM", gf, cf, t("hm.bin"))
+    run("om_minverser", t("hm.bin"), t("hminv.bin"))
+    run("om_assemble", "-DSM", gf, cf, df, t("dsm.bin"))
+    run("om_gain", "-IP", t("hminv.bin"), t("dsm.bin"), t("h2ip.mat"), t("ds2ip.bin"), t("gip.mat"))
+    g = om.read_geometry(gf, cf)
+    pts = np.loadtxt(POINTS, usecols=(0, 1, 2))
+    from oracle import oracle as O
+    dip = O.read_dipoles(df)
+    M = om.Surf2VolMat(g, pts).array()
+    S2 = om.DipSource2InternalPotMat(g, dip, pts).array()
+    Gp, kp, _ = om.gain_internal_pot_pipeline(g, dip, pts, precision="fp64")
+    e = (rel_fro(omio.read(t("h2ip.mat"))[1], M), rel_fro(omio.read(t("ds2ip.bin"))[1], S2),
+         rel_fro(omio.read(t("gip.mat"))[1], om.gain_to_numpy(Gp, len(kp), len(dip))))
+    print("  CLI vs Python: h2ip %.3e ds2ip %.3e gain %.3e" % e, flush=True)
+    assert e[0] < 1e-15 and e[1] < 1e-15 and e[2] < 1e-10

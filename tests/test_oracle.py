@@ -355,3 +355,73 @@ def test_oracle_surf_source_closed_layer_and_dipole_limit():
     E = _h2em_golden(g.size)
     Gd = O.gain_eeg(H, O.dip_source_mat(g, np.array(dips), "", levels=0), E) @ np.array(W)
     assert rel_fro(O.gain_eeg(H, S, E), Gd) < 0.02
+
+
+# -------------------------------------------------------- internal potentials
+def _linear_field_unknowns(g, mesh_ids, a, sigma):
+    """Unknowns of the field u = a . x on the given meshes: V = u at the
+    vertices, p = sigma du/dn (outward normal) on the triangles."""
+    verts = np.array(g.vertices)
+    x = np.zeros(g.size)
+    for mi in mesh_ids:
+        m = g.meshes[mi]
+        for v in m.verts:
+            x[g.vert_index[v]] = verts[v] @ a
+        for kt, t in enumerate(m.tris):
+            P = verts[t]
+            nrm = np.cross(P[1] - P[0], P[2] - P[0])
+            nrm /= np.linalg.norm(nrm)
+            if nrm @ P.mean(axis=0) < 0:
+                nrm = -nrm
+            x[m.tri_index[kt]] = sigma * (nrm @ a)
+    return x
+
+
+@pytest.mark.parametrize("layer", ["inner", "shell"])
+def test_oracle_surf2vol_reproduces_linear_fields(layer):
+    """From the mathematics: Green's representation on a polyhedral domain
+    reproduces a linear potential u = a . x exactly (V is linear on each flat
+    triangle, du/dn constant on it), so Surf2VolMat applied to the boundary
+    values of u gives u at every inside point to rounding -- in a domain with
+    one inside boundary and in a shell with an outside and an inside
+    boundary and sigma != 1.  A flipped D or S sign, a dropped term or a
+    wrong 1/sigma breaks it by O(|u|)."""
+    from openmeeg_amd import spheres as S
+    a = np.array([0.3, -0.5, 0.8])
+    rng = np.random.default_rng(0)
+    pts = rng.standard_normal((20, 3))
+    if layer == "inner":
+        g = O.nested_geometry(S.nested_sphere_meshes(4, (0.6, 1.0)), (1.0, 1.0))
+        r, meshes, sigma = 0.1 + 0.35 * rng.random(20), (0,), 1.0
+    else:
+        g = O.nested_geometry(S.nested_sphere_meshes(4, (0.5, 0.8, 1.0)), (1.0, 2.0, 1.0))
+        r, meshes, sigma = 0.58 + 0.12 * rng.random(20), (0, 1), 2.0
+    pts *= (r / np.linalg.norm(pts, axis=1))[:, None]
+    M, kept = O.surf2vol(g, pts)
+    assert len(kept) == len(pts)
+    u = M @ _linear_field_unknowns(g, meshes, a, sigma)
+    assert np.abs(u - pts @ a).max() < 1e-13
+
+
+def test_oracle_internal_potential_head1_vs_analytic():
+    """The internal-potential gain of Head1 (Surf2VolMat and
+    DipSource2InternalPotMat of the oracle with the golden HeadMat and DSM)
+    for tests/analytic/eeg_internal_points.txt against the analytic sphere
+    solution eeg_internal_analytic.txt, with the comparison of the
+    reference's test (test_compare_matrix.cpp: entries divided by the
+    matrices' infinity norms) and its bound for Head1, 0.17
+    (ComparisonTest_caller.cmake:132), on dipoles 1-4.  The points outside
+    the coarse Head1 scalp are dropped (assembleHeadMat.cpp:331-340)."""
+    g = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    pts = np.loadtxt(os.path.join(GOLDEN, "analytic", "eeg_internal_points.txt"), usecols=(0, 1, 2))
+    dip = O.read_dipoles(os.path.join(GOLDEN, "head1", "Head1.dip"))
+    M, kept = O.surf2vol(g, pts)
+    assert list(kept) == [i for i in range(44) if i not in (40, 41, 43)]
+    S2 = O.dip_source2internal_pot(g, dip, pts)
+    H = O.unpack(np.load(os.path.join(GOLDEN, "head1", "hm.npy")), g.size)
+    est = O.gain_internal_pot(H, np.load(os.path.join(GOLDEN, "head1", "dsm.npy")), M, S2) @ \
+        np.loadtxt(os.path.join(GOLDEN, "head1", "Head1.srcdip"))
+    ana = np.loadtxt(os.path.join(GOLDEN, "analytic", "eeg_internal_analytic.txt"))[kept]
+    ninf = lambda X: np.abs(X).sum(axis=1).max()
+    err = [np.abs(est[:, c] / ninf(est) - ana[:, c] / ninf(ana)).max() for c in range(6)]
+    assert max(err[:4]) < 0.17, err
