@@ -220,6 +220,47 @@ def surf_extra(om, S, torch, eeg_seconds=None, cpu=True):
     return out
 
 
+def internal_extra(om, S, torch, eeg_seconds=None):
+    """Internal potentials on the benchmark head (c2): Surf2VolMat for 20000
+    points inside the brain (radius < 0.75) and the internal-potential gain
+    of the 5000 benchmark dipoles at 1000 of them through one factor and one
+    solve (gain_internal_pot_pipeline), mixed16 against FP64."""
+    from openmeeg_amd.api import _surf2vol_device
+    meshes, cond, dip, el = S.make_config("c2")
+    g = om.make_nested_geometry(meshes, cond)
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal((20000, 3))
+    pts = d * (0.75 * rng.random(20000) ** (1.0 / 3.0) / np.linalg.norm(d, axis=1))[:, None]
+    M, kept = _surf2vol_device(g, pts)                                   # warm
+    del M
+    torch.cuda.synchronize()
+    t0 = time.time()
+    M, kept = _surf2vol_device(g, pts)
+    torch.cuda.synchronize()
+    t_h2ip = time.time() - t0
+    del M
+    sub = pts[:1000]
+    G, kp, info = om.gain_internal_pot_pipeline(g, dip, sub, precision="mixed16", domain_name="Brain")   # warm
+    del G
+    torch.cuda.synchronize()
+    t0 = time.time()
+    G, kp, info = om.gain_internal_pot_pipeline(g, dip, sub, precision="mixed16", domain_name="Brain")
+    torch.cuda.synchronize()
+    t = time.time() - t0
+    G = om.gain_to_numpy(G, len(kp), len(dip))
+    torch.cuda.empty_cache()
+    G64, _, _ = om.gain_internal_pot_pipeline(g, dip, sub, precision="fp64", domain_name="Brain")
+    G64 = om.gain_to_numpy(G64, len(kp), len(dip))
+    err = float(np.linalg.norm(G - G64) / np.linalg.norm(G64))
+    nb = len(meshes[0][1])
+    out = {"surf2vol_seconds": round(t_h2ip, 4), "points": len(kept), "n": g.size,
+           "surf2vol_evaluations_per_s": len(kept) * 4.0 * nb / t_h2ip,
+           "pipeline_seconds": round(t, 4), "pipeline_points": len(kp), "dipoles": len(dip),
+           "refine_iterations": info["refine_iterations"], "rel_err_vs_fp64": err}
+    log("bench: c2 internal potentials %s" % out)
+    return out
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -325,6 +366,9 @@ def run(args):
         extras["surface_sources_c2"] = surf_extra(om, S, torch, cpu=not args.no_cpu_baseline)
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
+        extras["internal_potential_c2"] = internal_extra(om, S, torch)
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -376,10 +420,12 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true", help="do not run the plain FP64 comparison")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (counter calibration) and exit")
-    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient", "meg", "surf"],
-                    help="gain: the headline measurement (default); batched / gradient / meg / surf: only that extra")
+    ap.add_argument("--workload", default="gain", choices=["gain", "batched", "gradient", "meg", "surf", "internal"],
+                    help="gain: the headline measurement (default); batched / gradient / meg / surf / internal: "
+                         "only that extra")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched-heads, gradient, EEG+MEG and surface-source measurements")
+                    help="skip the batched-heads, gradient, EEG+MEG, surface-source and internal-potential "
+                         "measurements")
     ap.add_argument("--cpu-rows", type=int, default=1500)
     ap.add_argument("--cpu-factor-n", type=int, default=10000)
     args = ap.parse_args()
@@ -403,7 +449,7 @@ def main():
         from openmeeg_amd import spheres as S
         torch.cuda.set_device(0)
         fn = {"batched": batched_extra, "gradient": gradient_extra, "meg": meg_extra,
-              "surf": surf_extra}[args.workload]
+              "surf": surf_extra, "internal": internal_extra}[args.workload]
         print(json.dumps({"workload": args.workload, "result": fn(om, S, torch)}), flush=True)
         return
     out = run(args)
