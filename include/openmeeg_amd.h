@@ -212,6 +212,15 @@ int omh_dip_source2internal_pot_mat(omh_geometry* g, const double* d_dipoles, in
                                     const double* d_points, int64_t npts, double* d_out, int64_t ld, int64_t* h_kept,
                                     int64_t* n_kept, void* stream);
 
+/* EITSourceMat(geo, electrodes, Integrator(order,0,.)) (assembleSourceMat.
+ * cpp:109-152): d_out (n x n_el, column-major, leading dimension ld >= n,
+ * rows in the unknown order `ordering`) for EIT electrodes at d_positions
+ * (n_el x 3 row-major) with radii h_radii (host; NULL = all zero, the
+ * injected currents are intensities).  Injection triangles as
+ * Sensors::findInjectionTriangles (sensors.cpp:167-220). */
+int omh_eit_source_mat(omh_geometry* g, const double* d_positions, const double* h_radii, int64_t n_el, int order,
+                       double* d_out, int64_t ld, int32_t ordering, void* stream);
+
 /* DipSource2MEGMat (assembleSensors.cpp:133-156) before the sensor weights:
  * d_out (npts x n_dip, column-major) = MagFactor (q x (x - r)) . d /
  * (|x - r|^3 |d|) for dipoles d_dipoles (n_dip x 6 row-major). */

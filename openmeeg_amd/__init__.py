@@ -13,6 +13,7 @@ from .api import (  # noqa: F401
     DipSource2InternalPotMat,
     DipSource2MEGMat,
     DipSourceMat,
+    EITSourceMat,
     Factor,
     Forward,
     GainEEG,

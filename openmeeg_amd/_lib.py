@@ -37,6 +37,7 @@ _SIGS = {
     "omh_mesh_create": ([p_f64, i64, p_i64, i64, C.POINTER(vp)], C.c_int),
     "omh_mesh_load": ([cp, C.POINTER(vp)], C.c_int),
     "omh_surf2vol_mat": ([vp, vp, i64, vp, i64, i32, p_i64, p_i64, vp], C.c_int),
+    "omh_eit_source_mat": ([vp, vp, p_f64, i64, C.c_int, vp, i64, i32, vp], C.c_int),
     "omh_dip_source2internal_pot_mat": ([vp, vp, i64, p_i32, vp, i64, vp, i64, p_i64, p_i64, vp], C.c_int),
     "omh_mesh_free": ([vp], None),
     "omh_mesh_sizes": ([vp, p_i64], C.c_int),

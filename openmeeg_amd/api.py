@@ -734,12 +734,14 @@ class Sensors:
     def __init__(self, positions=None, geom=None, orientations=None, weights=None):
         self.labels = []
         self.positions = np.zeros((0, 3))
+        self.radii = np.zeros(0)
         self.orientations = None
         self.weights = np.zeros(0)
         self.point_sensor = np.zeros(0, dtype=np.int64)
         if positions is not None:
             a = positions.array() if isinstance(positions, Matrix) else np.asarray(positions, dtype=np.float64)
             self.positions = np.ascontiguousarray(a[:, :3], dtype=np.float64)
+            self.radii = np.zeros(len(self.positions))
             self._plain(len(self.positions))
         if orientations is not None:
             o = orientations.array() if isinstance(orientations, Matrix) else np.asarray(orientations, dtype=np.float64)
@@ -773,8 +775,8 @@ class Sensors:
             rows = [r[1:] for r in rows]
         mat = np.array(rows, dtype=np.float64)
         ncol = mat.shape[1]
-        if ncol == 4:
-            raise ValueError("Sensors: spatially extended (EIT) sensors need a geometry")
+        # 4 columns: EIT electrodes with a radius (sensors.cpp:104-111)
+        self.radii = np.ascontiguousarray(mat[:, 3]) if ncol == 4 else np.zeros(len(mat))
         self.positions = np.ascontiguousarray(mat[:, :3])
         self.orientations = np.ascontiguousarray(mat[:, 3:6]) if ncol >= 6 else None
         self.weights = mat[:, 6].copy() if ncol == 7 else np.ones(len(mat))
@@ -1034,6 +1036,27 @@ def DipSource2InternalPotMat(geom, dipoles, points, domain_name=""):
                                                 _stream()))
     A = D.cpu().numpy()[:ld * nd].reshape(nd, ld).T[:nk.value]
     return Matrix(np.asfortranarray(A))
+
+
+def EITSourceMat(geom, electrodes, integrator=None):
+    """EITSourceMat(geo, electrodes, Integrator(3,0,0.005)) --
+    assembleSourceMat.h:31, assembleSourceMat.cpp:109-152: n x electrodes;
+    electrodes with a radius (a 4-column sensor file) inject a current
+    density over their injection triangles, the others an intensity."""
+    integ = integrator or Integrator(3, 0, 0.005)
+    if isinstance(electrodes, Sensors):
+        pos, rad = electrodes.positions, electrodes.radii
+    else:
+        pos = _electrodes_array(electrodes)
+        rad = np.zeros(len(pos))
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    rad = np.ascontiguousarray(rad, dtype=np.float64)
+    n, ne = geom.size, len(pos)
+    D = _dev_f64(n * max(1, ne))
+    dp = _to_dev(pos)          # held until the call has been enqueued
+    check(lib().omh_eit_source_mat(geom._h, _ptr(dp), rad.ctypes.data_as(_lib.p_f64), ne, integ.order, _ptr(D), n,
+                                   ORDER_REFERENCE, _stream()))
+    return Matrix(D.cpu().numpy()[:n * ne].reshape(ne, n).T.copy(order="F"))
 
 
 def GainInternalPot(hminv, dsm, h2ip, ds2ip):

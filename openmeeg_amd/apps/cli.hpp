@@ -141,6 +141,7 @@ inline void write_sparse(const std::string& path, const Sparse& s) {
 // same label form one sensor, labels in order of first appearance.
 struct Sensors {
     std::vector<double> pos, dir, weight;   // npts x 3 row-major, npts x 3, npts
+    std::vector<double> radius;             // EIT electrodes (4 columns), else 0
     std::vector<int64_t> sensor;            // sensor index of each point
     int64_t nsensors = 0;
     bool oriented = false;
@@ -167,14 +168,16 @@ inline Sensors read_sensors(const std::string& path) {
     const bool labeled = std::count(t0.begin(), t0.end(), '.') != 1;
     const size_t off = labeled ? 1 : 0;
     const size_t ncol = rows[0].size() - off;
-    if (ncol != 3 && ncol != 6 && ncol != 7)
-        throw std::runtime_error(path + ": sensors need 3, 6 or 7 numeric columns");
+    if (ncol != 3 && ncol != 4 && ncol != 6 && ncol != 7)
+        throw std::runtime_error(path + ": sensors need 3, 4, 6 or 7 numeric columns");
     Sensors s;
     s.oriented = ncol >= 6;
     std::map<std::string, int64_t> index;
     for (size_t i = 0; i < rows.size(); ++i) {
         double v[7] = {0, 0, 0, 0, 0, 0, 1};
         for (size_t c = 0; c < ncol; ++c) v[c] = std::stod(rows[i][off + c]);
+        s.radius.push_back(ncol == 4 ? v[3] : 0.0);
+        if (ncol == 4) v[3] = 0.0;
         for (int c = 0; c < 3; ++c) {
             s.pos.push_back(v[c]);
             s.dir.push_back(v[3 + c]);

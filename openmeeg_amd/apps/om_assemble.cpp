@@ -11,6 +11,7 @@
 //   -Head2MEGMat, -H2MM, -h2mm         geometry conductivity squids output
 //   -DipSource2MEGMat, -DS2MM, -ds2mm  dipoles squids output
 //   -SurfSource2MEGMat, -SS2MM, -ss2mm sourcemesh squids output
+//   -EITSourceMat, -EITSM, -EITsm               geometry conductivity electrodes output
 //   -Head2InternalPotMat, -H2IPM, -h2ipm        geometry conductivity points output
 //   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm geometry conductivity dipoles points output [domain]
 #include <algorithm>
@@ -214,6 +215,20 @@ void dip2ip(const std::string& geom, const std::string& cond, const std::string&
     write_full(out, m);
 }
 
+void eit_source_mat(const std::string& geom, const std::string& cond, const std::string& elf, const std::string& out) {
+    Geom g(geom, cond);
+    const Sensors s = read_sensors(elf);
+    const int64_t ne = s.npts(), n = g.n();
+    Dev<double> dp(s.pos.data(), s.pos.size());
+    Dev<double> D((size_t)(n * ne));
+    check(omh_eit_source_mat(g.g, dp.p, s.radius.data(), ne, 3, D.p, n, OMH_ORDER_REFERENCE, nullptr));
+    Full m;
+    m.nlin = n;
+    m.ncol = ne;
+    m.a = D.down();
+    write_full(out, m);
+}
+
 void help(const char* cmd) {
     std::cout << cmd << " [-option] [filepaths...]\n\n"
               << "   -HeadMat, -HM, -hm :                 geometry conductivity output\n"
@@ -224,6 +239,7 @@ void help(const char* cmd) {
               << "   -Head2MEGMat, -H2MM, -h2mm :         geometry conductivity squids output\n"
               << "   -DipSource2MEGMat, -DS2MM, -ds2mm :  dipoles squids output\n"
               << "   -SurfSource2MEGMat, -SS2MM, -ss2mm : sourcemesh squids output\n"
+              << "   -EITSourceMat, -EITSM, -EITsm :      geometry conductivity electrodes output\n"
               << "   -Head2InternalPotMat, -H2IPM, -h2ipm : geometry conductivity points output\n"
               << "   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm : geometry conductivity dipoles points output [domain]\n"
               << "Output format from the suffix: .mat (MAT 7.3), .bin, .txt\n";
@@ -265,6 +281,9 @@ int main(int argc, char** argv) {
             } else if (is(o, {"-SurfSource2MEGMat", "-SS2MM", "-ss2mm"})) {
                 need_args(argc, k, 3, o);
                 surf2meg(argv[k + 1], argv[k + 2], argv[k + 3]);
+            } else if (is(o, {"-EITSourceMat", "-EITSM", "-EITsm"})) {
+                need_args(argc, k, 4, o);
+                eit_source_mat(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);
             } else if (is(o, {"-Head2InternalPotMat", "-H2IPM", "-h2ipm"})) {
                 need_args(argc, k, 4, o);
                 head2ip(argv[k + 1], argv[k + 2], argv[k + 3], argv[k + 4]);

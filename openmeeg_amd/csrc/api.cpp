@@ -555,6 +555,20 @@ int omh_dip_source2internal_pot_mat(omh_geometry* g, const double* d_dipoles, in
     });
 }
 
+int omh_eit_source_mat(omh_geometry* g, const double* d_positions, const double* h_radii, int64_t n_el, int order,
+                       double* d_out, int64_t ld, int32_t ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        if (n_el < 0) throw std::invalid_argument("negative number of electrodes");
+        if (n_el > 0) {
+            need(d_positions, "d_positions");
+            need(d_out, "d_out");
+        }
+        omh::eit_source_mat(*g->g, d_positions, h_radii, n_el, order, d_out, layout_of(*g->g, ld, ordering),
+                            S(stream));
+    });
+}
+
 int omh_dip_source2meg_mat(const double* d_dipoles, int64_t n_dip, const double* d_pts, const double* d_dirs,
                            int64_t npts, double* d_out, int64_t ld, void* stream) {
     return guard([&] {

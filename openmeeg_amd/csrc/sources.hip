@@ -15,6 +15,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <set>
+#include <stack>
 #include <cfloat>
 #include <cmath>
 #include <stdexcept>
@@ -564,6 +568,57 @@ __global__ void __launch_bounds__(256) k_dip2ip(const double* __restrict__ dip, 
     }
 }
 
+// EITSourceMat (assembleSourceMat.cpp:109-152) for one injection triangle t
+// of a current-barrier mesh and one mesh pair (t's mesh, m2): column `out`
+// (one electrode) += w x row t of the transfer matrix.
+// D and identity part, thread = vertex slot of m2: cD x D(t, v) (the P1
+// double layer of v's triangles integrated over t by Gauss, as the D
+// blocks) plus cI x |t| / 3 when v is a vertex of t (addIdentity,
+// operators.h:200-206,267-269).
+__global__ void __launch_bounds__(256) k_eit_d(const TriRec* __restrict__ tri, int64_t t, const int32_t* __restrict__ tvid,
+                                               const int64_t* __restrict__ adj_off, const AdjRec* __restrict__ adj,
+                                               const int32_t* __restrict__ slot_vid, const int64_t* __restrict__ vert_unk,
+                                               int64_t s0, int64_t ns, int order, double cD, double cI,
+                                               double* __restrict__ out, Layout L) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int32_t vid = slot_vid[s0 + s];
+    const int64_t vi = vert_unk[vid];
+    if (vi < 0) return;
+    const TriRec& T = tri[t];
+    const d3 P[3] = {ld3(T.p[0]), ld3(T.p[1]), ld3(T.p[2])};
+    const int np = c_nb_pts[order];
+    double acc = 0.0;
+    for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
+        const AdjRec e = adj[p];
+        const P1S o = load_p1s(tri[e.tri]);
+        double r = 0.0;
+        for (int k = 0; k < np; ++k) {
+            const d3 v = p1s_eval(o, gauss_point(order, k, P));
+            r += c_rules[order][k][3] * (e.k == 0 ? v.x : (e.k == 1 ? v.y : v.z));
+        }
+        acc += T.area * r;
+    }
+    double val = cD * acc;
+    if (cI != 0.0 && (vid == tvid[3 * t] || vid == tvid[3 * t + 1] || vid == tvid[3 * t + 2])) val += cI * T.area / 3.0;
+    out[dev_index(L, vi)] += val;
+}
+
+// S part, thread = triangle u of m2: cS x S(t, u) (OperatorS of t integrated
+// over u by Gauss, as the S blocks).
+__global__ void __launch_bounds__(256) k_eit_s(const TriRec* __restrict__ tri, int64_t t, int64_t u0, int64_t nu,
+                                               int64_t unk0, int order, double cS, double* __restrict__ out, Layout L) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const TriRec& U = tri[u0 + u];
+    const OpS o = load_ops(tri[t]);
+    const d3 P[3] = {ld3(U.p[0]), ld3(U.p[1]), ld3(U.p[2])};
+    const int np = c_nb_pts[order];
+    double r = 0.0;
+    for (int k = 0; k < np; ++k) r += c_rules[order][k][3] * ops_eval(o, gauss_point(order, k, P));
+    out[dev_index(L, unk0 + u)] += cS * (U.area * r);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -703,13 +758,23 @@ void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const s
     }
 }
 
-void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, double* vals, hipStream_t st) {
+// dist_point_geom (danielsson.cpp:107-129) for n device points: the
+// closest triangle (global id) on the interfaces of zero-conductivity
+// domains, that interface, and `alphas` as the reference leaves them (the
+// barycentric weights from the last interface scanned).
+struct Closest {
+    std::vector<int64_t> tri;
+    std::vector<const Interface*> itf;
+    std::vector<std::array<double, 3>> alphas;
+};
+
+static Closest closest_outer(Geometry& g, const double* d_pts, int64_t n, hipStream_t st) {
     DeviceGeometry& dg = device_geometry(g);
     const int nm = (int)g.meshes.size();
     std::vector<int64_t> mt(nm + 1);
     for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
     mt[nm] = dg.n_tris;
-    const size_t cnt = (size_t)n_el * nm;
+    const size_t cnt = (size_t)n * nm;
     char* buf = (char*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + cnt * (sizeof(double) * 4 + sizeof(int64_t)));
     int64_t* d_mt = (int64_t*)buf;
     double* d_dist = (double*)(d_mt + nm + 1);
@@ -717,8 +782,8 @@ void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, doub
     double* d_al = (double*)(d_tri + cnt);
     OMH_HIP(hipMemcpyAsync(d_mt, mt.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, st));
     OMH_HIP(hipStreamSynchronize(st));
-    if (n_el > 0) {
-        k_closest<<<(unsigned)(n_el * nm), 256, 0, st>>>(dg.tri, d_mt, nm, d_el, d_dist, d_tri, d_al);
+    if (n > 0) {
+        k_closest<<<(unsigned)(n * nm), 256, 0, st>>>(dg.tri, d_mt, nm, d_pts, d_dist, d_tri, d_al);
         OMH_LAUNCH_CHECK();
     }
     std::vector<double> dist(cnt), al(3 * cnt);
@@ -727,15 +792,12 @@ void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, doub
     OMH_HIP(hipMemcpyAsync(tri.data(), d_tri, sizeof(int64_t) * cnt, hipMemcpyDeviceToHost, st));
     OMH_HIP(hipMemcpyAsync(al.data(), d_al, sizeof(double) * 3 * cnt, hipMemcpyDeviceToHost, st));
     OMH_HIP(hipStreamSynchronize(st));
-    // global triangle id -> vertex ids
-    std::vector<std::array<i64, 3>> tv;
-    for (const auto& m : g.meshes) tv.insert(tv.end(), m.tris.begin(), m.tris.end());
-    // dist_point_geom (danielsson.cpp:107-129): interfaces of zero-conductivity
-    // domains; `alphas` holds the result of the last interface scanned.
-    for (int64_t e = 0; e < n_el; ++e) {
+    Closest c;
+    c.tri.assign(n, -1);
+    c.itf.assign(n, nullptr);
+    c.alphas.assign(n, {0, 0, 0});
+    for (int64_t e = 0; e < n; ++e) {
         double distmin = DBL_MAX;
-        int64_t best_tri = -1;
-        double alphas[3] = {0, 0, 0};
         for (const auto& d : g.domains) {
             if (d.cond != 0.0) continue;
             for (const auto& b : d.boundaries) {
@@ -746,16 +808,29 @@ void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, doub
                     const size_t k = (size_t)e * nm + om.first;
                     if (dist[k] < dm) { dm = dist[k]; bt = tri[k]; ba = &al[3 * k]; }
                 }
-                if (ba) { alphas[0] = ba[0]; alphas[1] = ba[1]; alphas[2] = ba[2]; }
-                if (dm < distmin) { distmin = dm; best_tri = bt; }
+                if (ba) c.alphas[e] = {ba[0], ba[1], ba[2]};
+                if (dm < distmin) {
+                    distmin = dm;
+                    c.tri[e] = bt;
+                    c.itf[e] = &b.first;
+                }
             }
         }
-        if (best_tri < 0) throw std::runtime_error("Head2EEGMat: the geometry has no zero-conductivity domain");
-        for (int k = 0; k < 3; ++k) {
-            cols[3 * e + k] = g.vert_index[tv[best_tri][k]];
-            vals[3 * e + k] = alphas[k];
-        }
+        if (c.tri[e] < 0) throw std::runtime_error("the geometry has no zero-conductivity domain");
     }
+    return c;
+}
+
+void head2eeg(Geometry& g, const double* d_el, int64_t n_el, int64_t* cols, double* vals, hipStream_t st) {
+    const Closest c = closest_outer(g, d_el, n_el, st);
+    // global triangle id -> vertex ids
+    std::vector<std::array<i64, 3>> tv;
+    for (const auto& m : g.meshes) tv.insert(tv.end(), m.tris.begin(), m.tris.end());
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int k = 0; k < 3; ++k) {
+            cols[3 * e + k] = g.vert_index[tv[c.tri[e]][k]];
+            vals[3 * e + k] = c.alphas[e][k];
+        }
 }
 
 void head2meg(Geometry& g, const double* d_pts, const double* d_dirs, int64_t npts, double* d_out, int64_t ld_out,
@@ -903,6 +978,127 @@ std::vector<int64_t> dip2internal_pot(Geometry& g, const double* d_dip, int64_t 
         d_dip, n_dip, d_ddom, d_coef, d_pts, d_kept, d_pdom, nk, d_out, ld_out);
     OMH_LAUNCH_CHECK();
     return kept;
+}
+
+// almost_equal(r, 0) of om_common.h:24-26.
+static bool almost_zero(double r) { return std::fabs(r) < DBL_MIN; }
+
+// Sensors::findInjectionTriangles (sensors.cpp:167-220): the closest
+// triangle of each electrode and, when its radius is not zero and the
+// triangle is smaller than 4 pi r^2, the triangles of the same interface
+// reached through shared edges whose centres lie within r; the weight
+// pi r^2 / (their area) (1 when r = 0).
+static void injection_triangles(Geometry& g, const double* d_pos, const double* radii, int64_t ne, hipStream_t st,
+                                std::vector<std::vector<int64_t>>& tris, std::vector<double>& weight) {
+    const Closest c = closest_outer(g, d_pos, ne, st);
+    DeviceGeometry& dg = device_geometry(g);
+    std::vector<std::array<i64, 3>> tv;
+    for (const auto& m : g.meshes) tv.insert(tv.end(), m.tris.begin(), m.tris.end());
+    auto area = [&](int64_t t) {
+        const auto& a = g.vertices[tv[t][0]];
+        const auto& b = g.vertices[tv[t][1]];
+        const auto& q = g.vertices[tv[t][2]];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+        const double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
+        return 0.5 * std::sqrt(x * x + y * y + z * z);
+    };
+    std::vector<double> pos(3 * ne);
+    OMH_HIP(hipMemcpy(pos.data(), d_pos, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost));
+    tris.assign(ne, {});
+    weight.assign(ne, 1.0);
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t t0 = c.tri[e];
+        tris[e].push_back(t0);
+        const double r = radii ? radii[e] : 0.0;
+        if (almost_zero(r)) continue;
+        if (area(t0) < 4 * kPi * r * r) {
+            // edge -> triangles of the interface (Mesh::adjacent_triangles over its meshes)
+            std::map<std::pair<i64, i64>, std::vector<int64_t>> edges;
+            for (const auto& om : c.itf[e]->omeshes)
+                for (int64_t t = dg.meshes[om.first].tri_begin; t < dg.meshes[om.first].tri_end; ++t)
+                    for (int k = 0; k < 3; ++k) {
+                        const i64 a = tv[t][k], b = tv[t][(k + 1) % 3];
+                        edges[{std::min(a, b), std::max(a, b)}].push_back(t);
+                    }
+            std::set<int64_t> seen{t0};
+            std::stack<int64_t> stk;
+            stk.push(t0);
+            const double px = pos[3 * e], py = pos[3 * e + 1], pz = pos[3 * e + 2];
+            while (!stk.empty()) {
+                const int64_t t = stk.top();
+                stk.pop();
+                double cx = 0, cy = 0, cz = 0;
+                for (int k = 0; k < 3; ++k) {
+                    cx += g.vertices[tv[t][k]][0] / 3;
+                    cy += g.vertices[tv[t][k]][1] / 3;
+                    cz += g.vertices[tv[t][k]][2] / 3;
+                }
+                if (std::sqrt((cx - px) * (cx - px) + (cy - py) * (cy - py) + (cz - pz) * (cz - pz)) < r) {
+                    if (t != t0) tris[e].push_back(t);
+                    for (int k = 0; k < 3; ++k) {
+                        const i64 a = tv[t][k], b = tv[t][(k + 1) % 3];
+                        for (int64_t u : edges[{std::min(a, b), std::max(a, b)}])
+                            if (u != t && seen.insert(u).second) stk.push(u);
+                    }
+                }
+            }
+        }
+        double sum = 0.0;
+        for (int64_t t : tris[e]) sum += area(t);
+        weight[e] = kPi * r * r / sum;
+    }
+}
+
+void eit_source_mat(Geometry& g, const double* d_pos, const double* radii, int64_t ne, int order, double* d_out,
+                    const Layout& L, hipStream_t st) {
+    if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    const int64_t n = g.size();
+    if (ne <= 0) return;
+    fill_f64(d_out, n, ne, L.ld, 0.0, st);
+    std::vector<std::vector<int64_t>> tris;
+    std::vector<double> weight;
+    injection_triangles(g, d_pos, radii, ne, st, tris, weight);
+    DeviceGeometry& dg = device_geometry(g);
+    std::vector<int> tmesh;
+    for (int m = 0; m < (int)g.meshes.size(); ++m) tmesh.insert(tmesh.end(), g.meshes[m].tris.size(), m);
+    for (int64_t e = 0; e < ne; ++e) {
+        double* col = d_out + e * L.ld;
+        const bool intensity = radii == nullptr || almost_zero(radii[e]);
+        for (int64_t t : tris[e]) {
+            const int m1 = tmesh[t];
+            if (!g.meshes[m1].current_barrier) continue;   // only barrier rows of the transfer matrix
+            double w = weight[e];
+            if (intensity) {   // 1 / |t|
+                const auto& a = g.vertices[g.meshes[m1].tris[t - dg.meshes[m1].tri_begin][0]];
+                const auto& b = g.vertices[g.meshes[m1].tris[t - dg.meshes[m1].tri_begin][1]];
+                const auto& q = g.vertices[g.meshes[m1].tris[t - dg.meshes[m1].tri_begin][2]];
+                const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                const double e2[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+                const double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2],
+                             z = e1[0] * e2[1] - e1[1] * e2[0];
+                w = 1.0 / (0.5 * std::sqrt(x * x + y * y + z * z));
+            }
+            for (const MeshPair& mp : g.pairs) {
+                if (mp.m1 != m1) continue;
+                const MeshDev& B = dg.meshes[mp.m2];
+                const double o = mp.orientation;
+                const int64_t ns = B.slot_end - B.slot_begin, nu = B.tri_end - B.tri_begin;
+                const double cI = (mp.m2 == m1) ? -0.5 * w : 0.0;
+                if (ns > 0) {
+                    k_eit_d<<<(unsigned)cdiv(ns, 256), 256, 0, st>>>(dg.tri, t, dg.tri_vid, dg.adj_off, dg.adj,
+                                                                     dg.slot_vid, dg.vert_unk, B.slot_begin, ns, order,
+                                                                     kK * o * w, cI, col, L);
+                    OMH_LAUNCH_CHECK();
+                }
+                if (mp.m2 != m1 && !B.barrier && nu > 0) {
+                    const double cS = -kK * o * g.sigma_inv(m1, mp.m2) * w;
+                    k_eit_s<<<(unsigned)cdiv(nu, 256), 256, 0, st>>>(dg.tri, t, B.tri_begin, nu, B.tri_unk0, order,
+                                                                     cS, col, L);
+                    OMH_LAUNCH_CHECK();
+                }
+            }
+        }
+    }
 }
 
 }  // namespace omh

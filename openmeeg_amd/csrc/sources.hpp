@@ -75,6 +75,16 @@ std::vector<int64_t> dip2internal_pot(Geometry& g, const double* d_dip, int64_t 
                                       const double* d_pts, int64_t npts, double* d_out, int64_t ld_out,
                                       hipStream_t st);
 
+// EITSourceMat (assembleSourceMat.cpp:109-152): d_out (n x ne, column-major,
+// leading dimension L.ld, rows in the unknown order of L) for EIT
+// electrodes at d_pos (ne x 3 row-major, device) with radii (host, NULL =
+// all zero: currents given as intensities): for each injection triangle t
+// of an electrode (Sensors::findInjectionTriangles, sensors.cpp:167-220),
+// weight x row t of the transfer matrix [D*_23 S_23 -I_33/2 + D*_33]
+// built from the mesh pairs whose first mesh is t's current-barrier mesh.
+void eit_source_mat(Geometry& g, const double* d_pos, const double* radii, int64_t ne, int order, double* d_out,
+                    const Layout& L, hipStream_t st);
+
 // DipSource2MEGMat before the sensor weights (assembleSensors.cpp:133-156):
 // d_out (npts x n_dip, column-major) for dipoles d_dip (n_dip x 6 row-major).
 void dip2meg(const double* d_dip, int64_t n_dip, const double* d_pts, const double* d_dirs, int64_t npts,

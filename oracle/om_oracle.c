@@ -563,6 +563,18 @@ double orc_solid_angle(const double* tri9, const double* x) {
     Tri t = make_tri(v3(tri9[0], tri9[1], tri9[2]), v3(tri9[3], tri9[4], tri9[5]), v3(tri9[6], tri9[7], tri9[8]));
     return tri_solid_angle(&t, v3(x[0], x[1], x[2]));
 }
+/* D pair of the D blocks (operators.h:103-106): the P1 double layer of
+ * tri_col integrated over tri_row (Gauss), one value per vertex of tri_col. */
+void orc_integrate_d_pair(const double* tri_row, const double* tri_col, int order, double* out3) {
+    Tri r = make_tri(v3(tri_row[0], tri_row[1], tri_row[2]), v3(tri_row[3], tri_row[4], tri_row[5]),
+                     v3(tri_row[6], tri_row[7], tri_row[8]));
+    Tri c = make_tri(v3(tri_col[0], tri_col[1], tri_col[2]), v3(tri_col[3], tri_col[4], tri_col[5]),
+                     v3(tri_col[6], tri_col[7], tri_col[8]));
+    P1S o; p1s_init(&o, &c);
+    V3 tot = integrate_D(&o, &r, order);
+    out3[0] = tot.x; out3[1] = tot.y; out3[2] = tot.z;
+}
+
 double orc_integrate_s_pair(const double* tri_a, const double* tri_b, int order) {
     Tri a = make_tri(v3(tri_a[0], tri_a[1], tri_a[2]), v3(tri_a[3], tri_a[4], tri_a[5]), v3(tri_a[6], tri_a[7], tri_a[8]));
     Tri b = make_tri(v3(tri_b[0], tri_b[1], tri_b[2]), v3(tri_b[3], tri_b[4], tri_b[5]), v3(tri_b[6], tri_b[7], tri_b[8]));

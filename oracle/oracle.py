@@ -65,6 +65,7 @@ def lib():
         L.orc_p1slp.argtypes = [dp, dp, dp]
         L.orc_solid_angle.argtypes = [dp, dp]
         L.orc_solid_angle.restype = C.c_double
+        L.orc_integrate_d_pair.argtypes = [dp, dp, C.c_int, dp]
         L.orc_integrate_s_pair.argtypes = [dp, dp, C.c_int]
         L.orc_integrate_s_pair.restype = C.c_double
         L.orc_sample_rows.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, i64, i64, C.c_int, dp]
@@ -568,6 +569,99 @@ def dip_source2internal_pot(g, dipoles, points, domain_name=""):
             if d == dk:
                 diff = pts[i] - d6[:3]
                 out[row, j] += coeff * np.dot(d6[3:], diff) / np.linalg.norm(diff) ** 3
+    return out
+
+
+def _closest_outer(g, p):
+    """dist_point_geom (danielsson.cpp:107-129): (global triangle, interface)
+    closest to p on the interfaces of zero-conductivity domains."""
+    t = tables(g)
+    L = lib()
+    pc = np.ascontiguousarray(p, dtype=np.float64)
+    distmin, best, best_if = np.finfo(np.float64).max, -1, None
+    for dom in g.domains:
+        if dom.cond != 0.0:
+            continue
+        for iface, _ in dom.boundaries:
+            dmin_i, tri_i = np.finfo(np.float64).max, -1
+            for mid, _ in iface.omeshes:
+                tri = C.c_int64(-1)
+                al = np.zeros(3)
+                dd = L.orc_dist_point_mesh(C.byref(t.s), mid, _p(pc), C.byref(tri), _p(al))
+                if dd < dmin_i:
+                    dmin_i, tri_i = dd, tri.value
+            if dmin_i < distmin:
+                distmin, best, best_if = dmin_i, tri_i, iface
+    return best, best_if
+
+
+def eit_source_mat(g, positions, radii=None, order=3):
+    """EITSourceMat (assembleSourceMat.cpp:109-152) with the injection
+    triangles of Sensors::findInjectionTriangles (sensors.cpp:167-220):
+    column e = sum over electrode e's injection triangles t of w x row t of
+    the transfer matrix, built from every mesh pair (m1, m2) whose first
+    mesh m1 (t's mesh) is a current barrier:
+      D block with coefficient K x orientation (rows m1's triangles, columns
+        m2's vertices),
+      m1 == m2: the P0-P1 identity with -1/2 (|t|/3 at t's vertices),
+      m1 != m2: S block with -K x orientation x sigma_inv(m1, m2).
+    w = 1/|t| for a zero radius (an intensity), else pi r^2 over the area of
+    the electrode's injection triangles."""
+    t = tables(g)
+    L = lib()
+    pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    rad = np.zeros(len(pos)) if radii is None else np.asarray(radii, dtype=np.float64)
+    verts = np.array(g.vertices, dtype=np.float64)
+    tmesh = np.concatenate([np.full(len(m.tris), i) for i, m in enumerate(g.meshes)])
+    tri9 = lambda k: np.ascontiguousarray(verts[t.tris[k]].reshape(9))
+    area = lambda k: 0.5 * np.linalg.norm(np.cross(*(verts[t.tris[k][1:]] - verts[t.tris[k][0]])))
+    n = g.size
+    out = np.zeros((n, len(pos)))
+    r3 = np.zeros(3)
+    for e, p in enumerate(pos):
+        t0, iface = _closest_outer(g, p)
+        tris = [t0]
+        if rad[e] != 0.0:
+            if area(t0) < 4 * math.pi * rad[e] ** 2:
+                edges = {}
+                for mid, _ in iface.omeshes:
+                    for k in range(t.tri_off[mid], t.tri_off[mid + 1]):
+                        for a in range(3):
+                            ed = tuple(sorted((t.tris[k][a], t.tris[k][(a + 1) % 3])))
+                            edges.setdefault(ed, []).append(k)
+                seen, stack = {t0}, [t0]
+                while stack:
+                    k = stack.pop()
+                    if np.linalg.norm(verts[t.tris[k]].mean(axis=0) - p) < rad[e]:
+                        if k != t0:
+                            tris.append(k)
+                        for a in range(3):
+                            for u in edges[tuple(sorted((t.tris[k][a], t.tris[k][(a + 1) % 3])))]:
+                                if u != k and u not in seen:
+                                    seen.add(u)
+                                    stack.append(u)
+            w_e = math.pi * rad[e] ** 2 / sum(area(k) for k in tris)
+        for k in tris:
+            m1 = int(tmesh[k])
+            if not g.meshes[m1].current_barrier:
+                continue
+            w = 1.0 / area(k) if rad[e] == 0.0 else w_e
+            row = np.zeros(g.nb_parameters)
+            for (pm1, m2, o, cS, cN, cD) in g.pairs:
+                if pm1 != m1:
+                    continue
+                for j in range(t.tri_off[m2], t.tri_off[m2 + 1]):
+                    L.orc_integrate_d_pair(_p(tri9(k)), _p(tri9(j)), order, _p(r3))
+                    for a in range(3):
+                        row[g.vert_index[t.tris[j][a]]] += K * o * r3[a]
+                if m2 == m1:
+                    for a in range(3):
+                        row[g.vert_index[t.tris[k][a]]] += -0.5 * area(k) / 3.0
+                else:
+                    for j in range(t.tri_off[m2], t.tri_off[m2 + 1]):
+                        row[t.tri_index[j]] = -K * o * g.sigma_inv(m1, m2) * \
+                            L.orc_integrate_s_pair(_p(tri9(k)), _p(tri9(j)), order)
+            out[:, e] += row[:n] * w
     return out
 
 

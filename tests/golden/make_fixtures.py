@@ -28,7 +28,10 @@ What is extracted (all data files that the reference's own tests hold):
 * ``headnna1/``: the HeadNNa1 inputs (non-nested: the north and south meshes
   share the cut mesh's vertices), .dip, .patches and .squids.
 * ``analytic/``: ``tests/analytic/eeg_head{1,2,3}_analytic.txt`` (multilayer
-  sphere series, n_electrodes x 6 dipoles).
+  sphere series, n_electrodes x 6 dipoles) and the internal-potential points
+  and analytic values ``eeg_internal_{points,analytic}.txt``; ``head1/`` also
+  holds ``Head1.srcdip`` (the dipole activations) and ``Head1-EIT.patches``
+  (EIT electrodes with radii).
 """
 import os
 import shutil
@@ -89,7 +92,7 @@ def main():
 
     data1 = os.path.join(REF, "data", "Head1")
     for f in ("Head1.geom", "Head1.cond", "Head1.dip", "Head1.patches", "Head1.squids",
-              "Head1.tri", "Head1.src", "cortex.1.tri", "skull.1.tri", "scalp.1.tri"):
+              "Head1.tri", "Head1.src", "Head1.srcdip", "Head1-EIT.patches", "cortex.1.tri", "skull.1.tri", "scalp.1.tri"):
         shutil.copy(os.path.join(data1, f), h1)
 
     h3 = os.path.join(HERE, "head3")
@@ -110,6 +113,8 @@ def main():
     for k in (1, 2, 3):
         shutil.copy(os.path.join(REF, "tests", "analytic",
                                  "eeg_head%d_analytic.txt" % k), an)
+    for f in ("eeg_internal_points.txt", "eeg_internal_analytic.txt"):
+        shutil.copy(os.path.join(REF, "tests", "analytic", f), an)
     print("fixtures written under", HERE)
 
 

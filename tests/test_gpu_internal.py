@@ -1,8 +1,10 @@
-"""Internal potentials on the HIP path: Surf2VolMat, DipSource2InternalPotMat,
-GainInternalPot and the device pipeline, against the CPU oracle (Head1 with
-the reference's internal points, a 3-layer sphere head with points in every
-layer) and the exact reproduction of linear fields; the command-line
-options -H2IPM / -DS2IPM / om_gain -IP.  Needs a HIP GPU.
+"""Internal potentials and EIT on the HIP path: Surf2VolMat,
+DipSource2InternalPotMat, GainInternalPot and the device pipeline, against
+the CPU oracle (Head1 with the reference's internal points, a 3-layer sphere
+head with points in every layer) and the exact reproduction of linear
+fields; EITSourceMat against the oracle and Helmholtz reciprocity; the
+command-line options -H2IPM / -DS2IPM / -EITSM and om_gain -IP / -EITIP.
+Needs a HIP GPU.
 
 Tolerances (relative Frobenius): matrices vs the oracle 1e-13, gains 1e-10,
 mixed precision vs FP64 1e-6."""
@@ -126,6 +128,8 @@ def test_internal_potential_cli(om, tmp_path):
     run("om_minverser", t("hm.bin"), t("hminv.bin"))
     run("om_assemble", "-DSM", gf, cf, df, t("dsm.bin"))
     run("om_gain", "-IP", t("hminv.bin"), t("dsm.bin"), t("h2ip.mat"), t("ds2ip.bin"), t("gip.mat"))
+    run("om_assemble", "-EITSM", gf, cf, os.path.join(H1, "Head1-EIT.patches"), t("eitsm.mat"))
+    run("om_gain", "-EITIP", t("hminv.bin"), t("eitsm.mat"), t("h2ip.mat"), t("geit.bin"))
     g = om.read_geometry(gf, cf)
     pts = np.loadtxt(POINTS, usecols=(0, 1, 2))
     from oracle import oracle as O
@@ -133,7 +137,61 @@ def test_internal_potential_cli(om, tmp_path):
     M = om.Surf2VolMat(g, pts).array()
     S2 = om.DipSource2InternalPotMat(g, dip, pts).array()
     Gp, kp, _ = om.gain_internal_pot_pipeline(g, dip, pts, precision="fp64")
+    sens = om.Sensors()
+    sens.load(os.path.join(H1, "Head1-EIT.patches"))
+    EIT = om.EITSourceMat(g, sens)
+    Geit = om.GainEITInternalPot(om.HeadMat(g).inverse(), EIT, om.Matrix(M)).array()
     e = (rel_fro(omio.read(t("h2ip.mat"))[1], M), rel_fro(omio.read(t("ds2ip.bin"))[1], S2),
-         rel_fro(omio.read(t("gip.mat"))[1], om.gain_to_numpy(Gp, len(kp), len(dip))))
-    print("  CLI vs Python: h2ip %.3e ds2ip %.3e gain %.3e" % e, flush=True)
-    assert e[0] < 1e-15 and e[1] < 1e-15 and e[2] < 1e-10
+         rel_fro(omio.read(t("gip.mat"))[1], om.gain_to_numpy(Gp, len(kp), len(dip))),
+         rel_fro(omio.read(t("eitsm.mat"))[1], EIT.array()), rel_fro(omio.read(t("geit.bin"))[1], Geit))
+    print("  CLI vs Python: h2ip %.3e ds2ip %.3e gain %.3e eitsm %.3e eit gain %.3e" % e, flush=True)
+    assert e[0] < 1e-15 and e[1] < 1e-15 and e[2] < 1e-10 and e[3] < 1e-15 and e[4] < 1e-10
+
+
+def test_eit_source_mat_vs_oracle_and_reciprocity(om):
+    """EITSourceMat on the device against the oracle: Head1 with the two
+    point electrodes of the reference's EIT validation (intensities), with
+    Head1-EIT.patches (radius 1e-3: one injection triangle, weight pi r^2 /
+    |t|), and a 642-vertex sphere head with 0.15-radius electrodes (several
+    injection triangles found through shared edges); then the reciprocity
+    of the reference's EIT validation on the device (dipole potential
+    differences vs q . grad of the EIT potential, 1% here, 13% in the
+    reference's test)."""
+    from oracle import oracle as O
+    from openmeeg_amd import spheres as S
+    gf, cf = os.path.join(H1, "Head1.geom"), os.path.join(H1, "Head1.cond")
+    g, og = om.read_geometry(gf, cf), O.load_geometry(gf, cf)
+    el = np.array([[0.886556, 0.278249, -0.166667], [0.547922, -0.269672, -0.719889]])
+    print("EIT: Head1 point electrodes", flush=True)
+    e1 = rel_fro(om.EITSourceMat(g, el).array(), O.eit_source_mat(og, el))
+    sens = om.Sensors()
+    sens.load(os.path.join(H1, "Head1-EIT.patches"))
+    e2 = rel_fro(om.EITSourceMat(g, sens).array(), O.eit_source_mat(og, sens.positions, sens.radii))
+    meshes = S.nested_sphere_meshes(8, (0.87, 0.92, 1.0))
+    cond = (1.0, 0.0125, 1.0)
+    gs, ogs = om.make_nested_geometry(meshes, cond), O.nested_geometry(meshes, cond)
+    pos = S.sphere_electrodes(np.random.default_rng(4), 6)
+    rad = np.full(6, 0.15)
+    es = om.Sensors(pos)
+    es.radii = rad
+    print("EIT: sphere head, 6 electrodes of radius 0.15", flush=True)
+    e3 = rel_fro(om.EITSourceMat(gs, es).array(), O.eit_source_mat(ogs, pos, rad))
+    # reciprocity on the device
+    dip = O.read_dipoles(os.path.join(H1, "Head1.dip"))
+    hm = om.HeadMat(g)
+    Hi = hm.inverse().array()
+    E = om.Head2EEGMat(g, el).array()
+    V = E @ Hi @ om.DipSourceMat(g, dip, "Brain").array()
+    diff = V[0] - V[1]
+    x = Hi @ (om.EITSourceMat(g, el).array() @ np.array([1.0, -1.0]))
+    P = dip[:, :3]
+    v0 = om.Surf2VolMat(g, P).array() @ x
+    grad = np.zeros((len(dip), 3))
+    for i in range(3):
+        Q = P.copy()
+        Q[:, i] += 1e-6
+        grad[:, i] = (om.Surf2VolMat(g, Q).array() @ x - v0) / 1e-6
+    rel = np.abs((dip[:, 3:] * grad).sum(axis=1) - diff) / np.abs(diff)
+    print("  vs oracle: point %.3e, radius 1e-3 %.3e, sphere r=0.15 %.3e; reciprocity max %.3e"
+          % (e1, e2, e3, rel.max()), flush=True)
+    assert e1 < 1e-13 and e2 < 1e-13 and e3 < 1e-13 and rel.max() < 0.01

@@ -425,3 +425,32 @@ def test_oracle_internal_potential_head1_vs_analytic():
     ninf = lambda X: np.abs(X).sum(axis=1).max()
     err = [np.abs(est[:, c] / ninf(est) - ana[:, c] / ninf(ana)).max() for c in range(6)]
     assert max(err[:4]) < 0.17, err
+
+
+def test_oracle_eit_reciprocity_head1():
+    """The reference's EIT validation (tests/test_validationEIT.cpp) restated
+    on the oracle with the Head1 goldens: by Helmholtz reciprocity, the
+    potential difference that a dipole q at r makes between two electrodes
+    equals q . grad V(r), V being the potential of a unit current injected
+    at the first electrode and extracted at the second (EITSourceMat, then
+    Surf2VolMat for the gradient by forward differences, step 1e-6).  The
+    reference's bound is 0.13 relative per dipole (ComparisonTest_caller.
+    cmake:52); the oracle is at 0.4% at most."""
+    g = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    dip = O.read_dipoles(os.path.join(GOLDEN, "head1", "Head1.dip"))
+    Hi = O.unpack(np.load(os.path.join(GOLDEN, "head1", "hm_inv.npy")), g.size)
+    el = np.array([[0.886556, 0.278249, -0.166667], [0.547922, -0.269672, -0.719889]])
+    cols, vals = O.head2eeg(g, el)
+    V = O.h2em_dense(g, cols, vals) @ Hi @ np.load(os.path.join(GOLDEN, "head1", "dsm.npy"))
+    diff = V[0] - V[1]
+    x = Hi @ (O.eit_source_mat(g, el) @ np.array([1.0, -1.0]))
+    P = dip[:, :3]
+    v0 = O.surf2vol(g, P)[0] @ x
+    grad = np.zeros((len(dip), 3))
+    for i in range(3):
+        Q = P.copy()
+        Q[:, i] += 1e-6
+        grad[:, i] = (O.surf2vol(g, Q)[0] @ x - v0) / 1e-6
+    qg = (dip[:, 3:] * grad).sum(axis=1)
+    rel = np.abs(qg - diff) / np.abs(diff)
+    assert rel.max() < 0.01, rel
