@@ -254,7 +254,7 @@ def internal_extra(om, S, torch, eeg_seconds=None):
     err = float(np.linalg.norm(G - G64) / np.linalg.norm(G64))
     nb = len(meshes[0][1])
     out = {"surf2vol_seconds": round(t_h2ip, 4), "points": len(kept), "n": g.size,
-           "surf2vol_evaluations_per_s": len(kept) * 4.0 * nb / t_h2ip,
+           "surf2vol_point_triangle_pairs_per_s": len(kept) * float(nb) / t_h2ip,
            "pipeline_seconds": round(t, 4), "pipeline_points": len(kp), "dipoles": len(dip),
            "refine_iterations": info["refine_iterations"], "rel_err_vs_fp64": err}
     log("bench: c2 internal potentials %s" % out)

@@ -497,52 +497,63 @@ __global__ void __launch_bounds__(256) k_dip2meg(const double* __restrict__ dip,
 }
 
 // Surf2VolMat (assembleHeadMat.cpp:325-363, PartialBlock operators.h:302-
-// 331) for one mesh.  Point r of the kept points sits at pts[3 kept[r]];
-// cD[r] (cS[r]) is its coefficient for this mesh's vertex (triangle)
-// columns, zero when the mesh does not bound the point's domain.
-// D part, thread = vertex slot of the mesh (x), kept point (block y,
-// looped): out(r, v) += cD sum over the triangles T of v (mesh order) of
-// P1SingleLayerPotential_T(x)[position of v in T].
-__global__ void __launch_bounds__(256) k_surf2vol_d(const TriRec* __restrict__ tri, const int64_t* __restrict__ adj_off,
-                                                    const AdjRec* __restrict__ adj, const int32_t* __restrict__ slot_vid,
+// 331) for one mesh, over a chunk of the kept points [p0, p0 + pc).  Kept
+// point r sits at pts[3 kept[r]]; cD[r] (cS[r]) is its coefficient for
+// this mesh's vertex (triangle) columns, zero when the mesh does not bound
+// the point's domain.  Threads run along the points, so that the writes to
+// out (points x unknowns, column-major) are coalesced.
+// Pass 1: W[(k nt + t) pc + p] = P1SingleLayerPotential_t(x_p)[k], one
+// evaluation per (triangle, point); block = 256 points of one triangle.
+__global__ void __launch_bounds__(256) k_surf2vol_p1(const TriRec* __restrict__ tri, int64_t t0, int64_t nt,
+                                                     const double* __restrict__ pts, const int64_t* __restrict__ kept,
+                                                     int64_t p0, int64_t pc, double* __restrict__ W) {
+    const int64_t nb = cdiv_dev(pc, 256);
+    const int64_t t = blockIdx.x / nb, p = (blockIdx.x - t * nb) * 256 + threadIdx.x;
+    if (p >= pc) return;
+    const d3 v = p1s_eval(load_p1s(tri[t0 + t]), ld3(pts + 3 * kept[p0 + p]));
+    W[(0 * nt + t) * pc + p] = v.x;
+    W[(1 * nt + t) * pc + p] = v.y;
+    W[(2 * nt + t) * pc + p] = v.z;
+}
+
+// Pass 2, thread = point (x), vertex slot of the mesh (block y, looped):
+// out(r, v) += cD[r] sum over the triangles T of v (mesh order) of W(T, r)
+// at v's position in T.
+__global__ void __launch_bounds__(256) k_surf2vol_d(const int64_t* __restrict__ adj_off, const AdjRec* __restrict__ adj,
+                                                    const int32_t* __restrict__ slot_vid,
                                                     const int64_t* __restrict__ vert_unk, int64_t s0, int64_t ns,
-                                                    const double* __restrict__ pts, const int64_t* __restrict__ kept,
-                                                    int64_t nk, const double* __restrict__ cD, double* __restrict__ out,
+                                                    int64_t t0, int64_t nt, const double* __restrict__ W, int64_t p0,
+                                                    int64_t pc, const double* __restrict__ cD, double* __restrict__ out,
                                                     Layout L) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
-    const int64_t vi = vert_unk[slot_vid[s0 + s]];
-    if (vi < 0) return;
-    const int64_t col = dev_index(L, vi);
-    for (int64_t r = blockIdx.y; r < nk; r += gridDim.y) {
-        const double c = cD[r];
-        if (c == 0.0) continue;
-        const d3 x = ld3(pts + 3 * kept[r]);
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pc) return;
+    const double c = cD[p0 + p];
+    if (c == 0.0) return;
+    for (int64_t s = blockIdx.y; s < ns; s += gridDim.y) {
+        const int64_t vi = vert_unk[slot_vid[s0 + s]];
+        if (vi < 0) continue;
         double acc = 0.0;
-        for (int64_t p = adj_off[s0 + s]; p < adj_off[s0 + s + 1]; ++p) {
-            const AdjRec e = adj[p];
-            const d3 v = p1s_eval(load_p1s(tri[e.tri]), x);
-            acc += e.k == 0 ? v.x : (e.k == 1 ? v.y : v.z);
+        for (int64_t q = adj_off[s0 + s]; q < adj_off[s0 + s + 1]; ++q) {
+            const AdjRec e = adj[q];
+            acc += W[((int64_t)e.k * nt + (e.tri - t0)) * pc + p];
         }
-        out[r + col * L.ld] += c * acc;
+        out[(p0 + p) + dev_index(L, vi) * L.ld] += c * acc;
     }
 }
 
-// S part, thread = triangle of the mesh (x), kept point (block y, looped):
-// out(r, t) = cS OperatorS_T(x) (the reference assigns these entries).
+// S part, thread = point (x), triangle of the mesh (block y, looped):
+// out(r, t) = cS[r] OperatorS_t(x_r) (the reference assigns these entries).
 __global__ void __launch_bounds__(256) k_surf2vol_s(const TriRec* __restrict__ tri, int64_t t0, int64_t nt,
                                                     int64_t unk0, const double* __restrict__ pts,
                                                     const int64_t* __restrict__ kept, int64_t nk,
                                                     const double* __restrict__ cS, double* __restrict__ out, Layout L) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const int64_t col = dev_index(L, unk0 + t);
-    const OpS o = load_ops(tri[t0 + t]);
-    for (int64_t r = blockIdx.y; r < nk; r += gridDim.y) {
-        const double c = cS[r];
-        if (c == 0.0) continue;
-        out[r + col * L.ld] = c * ops_eval(o, ld3(pts + 3 * kept[r]));
-    }
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nk) return;
+    const double c = cS[r];
+    if (c == 0.0) return;
+    const d3 x = ld3(pts + 3 * kept[r]);
+    for (int64_t t = blockIdx.y; t < nt; t += gridDim.y)
+        out[r + dev_index(L, unk0 + t) * L.ld] = c * ops_eval(load_ops(tri[t0 + t]), x);
 }
 
 // DipSource2InternalPotMat (assembleSourceMat.cpp:154-186): out(r, j) =
@@ -928,19 +939,29 @@ std::vector<int64_t> surf2vol(Geometry& g, const double* d_pts, int64_t npts, do
     OMH_HIP(hipMemcpyAsync(d_cD, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipMemcpyAsync(d_cS, cS.data(), sizeof(double) * cS.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipStreamSynchronize(st));   // the host vectors die with this frame
-    const unsigned gy = (unsigned)std::min<int64_t>(nk, 65535);
+    // pass-1 scratch: 3 x triangles x points of a chunk, at most 1 GiB
+    int64_t ntmax = 1;
+    for (int m = 0; m < nm; ++m) ntmax = std::max<int64_t>(ntmax, dg.meshes[m].tri_end - dg.meshes[m].tri_begin);
+    const int64_t pc_max = std::max<int64_t>(256, std::min<int64_t>(nk, ((int64_t(1) << 30) / (24 * ntmax)) / 256 * 256));
+    double* W = (double*)scratch(SCR_DTMP, sizeof(double) * 3 * (size_t)ntmax * (size_t)std::min<int64_t>(pc_max, nk));
     for (int m = 0; m < nm; ++m) {
         if (!used[m]) continue;
         const MeshDev& md = dg.meshes[m];
         const int64_t ns = md.slot_end - md.slot_begin, nt = md.tri_end - md.tri_begin;
-        if (ns > 0) {
-            k_surf2vol_d<<<dim3((unsigned)cdiv(ns, 256), gy), 256, 0, st>>>(
-                dg.tri, dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, md.slot_begin, ns, d_pts, d_kept, nk,
-                d_cD + (size_t)m * nk, d_out, L);
-            OMH_LAUNCH_CHECK();
+        if (ns > 0 && nt > 0) {
+            for (int64_t p0 = 0; p0 < nk; p0 += pc_max) {
+                const int64_t pc = std::min<int64_t>(pc_max, nk - p0);
+                k_surf2vol_p1<<<(unsigned)(nt * cdiv(pc, 256)), 256, 0, st>>>(dg.tri, md.tri_begin, nt, d_pts, d_kept,
+                                                                               p0, pc, W);
+                OMH_LAUNCH_CHECK();
+                k_surf2vol_d<<<dim3((unsigned)cdiv(pc, 256), (unsigned)std::min<int64_t>(ns, 65535)), 256, 0, st>>>(
+                    dg.adj_off, dg.adj, dg.slot_vid, dg.vert_unk, md.slot_begin, ns, md.tri_begin, nt, W, p0, pc,
+                    d_cD + (size_t)m * nk, d_out, L);
+                OMH_LAUNCH_CHECK();
+            }
         }
         if (nt > 0 && !md.barrier) {
-            k_surf2vol_s<<<dim3((unsigned)cdiv(nt, 256), gy), 256, 0, st>>>(
+            k_surf2vol_s<<<dim3((unsigned)cdiv(nk, 256), (unsigned)std::min<int64_t>(nt, 65535)), 256, 0, st>>>(
                 dg.tri, md.tri_begin, nt, md.tri_unk0, d_pts, d_kept, nk, d_cS + (size_t)m * nk, d_out, L);
             OMH_LAUNCH_CHECK();
         }
