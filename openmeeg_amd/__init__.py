@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     Forward,
     GainEEG,
     GainEEGadjoint,
+    GainEEGMEGadjoint,
     GainEITInternalPot,
     GainInternalPot,
     GainMEG,

@@ -1139,6 +1139,41 @@ def GainMEGadjoint(geom, dipoles, hm, h2mm, ds2mm, precision="fp64"):
     return Matrix(G.cpu().numpy().reshape(nd, ns).T.copy(order="F"))
 
 
+class GainEEGMEGadjoint:
+    """GainEEGMEGadjoint(geo, dipoles, HeadMat, Head2EEGMat, Head2MEGMat,
+    Source2MEGMat) -- gain.h:105-128: one factor of the HeadMat and one solve
+    with the EEG and MEG rows; EEGleadfield / MEGleadfield, saveEEG /
+    saveMEG, nlin() = EEG rows + MEG rows."""
+
+    def __init__(self, geom, dipoles, hm, h2em, h2mm, ds2mm, precision="fp64"):
+        n, nv = geom.size, geom.n_vertex_unknowns
+        ne = h2em.nlin()
+        M = h2mm.array()
+        ns = M.shape[0]
+        torch = _t()
+        F = factor(hm.device().clone(), n, precision)
+        Y = torch.cat([_eeg_rhs(geom, h2em).view(ne, n),
+                       to_device_rows(np.asfortranarray(M.T), n, nv).view(ns, n)], dim=0).contiguous().view(-1)
+        F.solve(Y, ne + ns)
+        D = _dsm_device(geom, dipoles, "")
+        nd = len(_dipoles_array(dipoles))
+        Ge = _gemm_tn(Y[:ne * n], D, n, ne, nd)
+        Gm = _to_dev(np.asfortranarray(ds2mm.array()).ravel(order="F"))
+        check(lib().omh_gemm_f64(1, 0, ns, nd, n, 1.0, _ptr(Y[ne * n:]), n, _ptr(D), n, 1.0, _ptr(Gm), ns,
+                                 _stream()))
+        self.EEGleadfield = Matrix(gain_to_numpy(Ge, ne, nd))
+        self.MEGleadfield = Matrix(gain_to_numpy(Gm, ns, nd))
+
+    def saveEEG(self, fname):
+        self.EEGleadfield.save(fname)
+
+    def saveMEG(self, fname):
+        self.MEGleadfield.save(fname)
+
+    def nlin(self):
+        return self.EEGleadfield.nlin() + self.MEGleadfield.nlin()
+
+
 # ---------------------------------------------------------------------------
 # Device pipeline (the path the benchmark times)
 # ---------------------------------------------------------------------------

@@ -98,8 +98,15 @@ def test_meg_head1_matches_golden(om, head1):
     ref = np.load(os.path.join(GOLDEN, "head1", "dgmm.npy"))
     e_g = rel_fro(om.GainMEG(hm.inverse(), dsm, h2mm, ds2mm).array(), ref)
     e_a = rel_fro(om.GainMEGadjoint(g, dip, hm, h2mm, ds2mm).array(), ref)
-    print("Head1 MEG: h2mm %.3e, ds2mm %.3e, GainMEG %.3e, GainMEGadjoint %.3e" % (e_h, e_s, e_g, e_a), flush=True)
-    assert e_h < 1e-10 and e_s < 1e-14 and e_g < 1e-10 and e_a < 1e-10
+    sens = om.Sensors()
+    sens.load(os.path.join(GOLDEN, "head1", "Head1.patches"))
+    both = om.GainEEGMEGadjoint(g, dip, hm, om.Head2EEGMat(g, sens), h2mm, ds2mm)
+    e_be = rel_fro(both.EEGleadfield.array(), np.load(os.path.join(GOLDEN, "head1", "dgem.npy")))
+    e_bm = rel_fro(both.MEGleadfield.array(), ref)
+    print("Head1 MEG: h2mm %.3e, ds2mm %.3e, GainMEG %.3e, GainMEGadjoint %.3e, GainEEGMEGadjoint %.3e / %.3e"
+          % (e_h, e_s, e_g, e_a, e_be, e_bm), flush=True)
+    assert e_h < 1e-10 and e_s < 1e-14 and e_g < 1e-10 and e_a < 1e-10 and e_be < 1e-10 and e_bm < 1e-10
+    assert both.nlin() == 32 + h2mm.nlin()
 
 
 def test_meg_nonnested_nna1(om, nna1):
