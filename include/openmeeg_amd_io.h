@@ -22,7 +22,9 @@
  *
  * Reading identifies the format from the content as the reference does
  * (MathsIO.C:82-108: MATLAB magic tag, else a leading number = ASCII, else
- * raw binary); writing picks it from the file suffix (MathsIO.C:71-81).
+ * raw binary); writing picks it from the file suffix (MathsIO.C:71-81),
+ * and a file without a known suffix (Head1.hm, Head1.dsm, ...) is written
+ * in the first format that stores the matrix, MAT (matrix.cpp:219-226).
  * Every entry point returns 0 on success and a non-zero status otherwise
  * (1 runtime error, 2 invalid argument, 6 file error); omh_io_last_error()
  * gives the message (per thread).
@@ -68,9 +70,9 @@ int omh_io_info(const char* path, omh_linop_info* info);
  * `info` must be what omh_io_info returned for the file. */
 int omh_io_read(const char* path, const omh_linop_info* info, double* data, int64_t* rows, int64_t* cols);
 
-/* Writes a matrix in the format of the suffix of `path` (.mat, .bin, .txt;
- * another suffix is an error, UnknownFileSuffix).  Layouts as omh_io_read;
- * sparse entries in any order (duplicates are an error). */
+/* Writes a matrix in the format of the suffix of `path` (.bin, .txt; .mat
+ * and any other suffix: MAT 7.3).  Layouts as omh_io_read; sparse entries
+ * in any order (duplicates are an error). */
 int omh_io_write(const char* path, const omh_linop_info* info, const double* data, const int64_t* rows,
                  const int64_t* cols);
 

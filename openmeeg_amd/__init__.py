@@ -43,9 +43,12 @@ from .api import (  # noqa: F401
     gain_internal_pot_pipeline,
     mesh_intersects,
     gain_to_numpy,
+    get_log_level,
     make_geometry,
     make_nested_geometry,
     read_geometry,
+    set_log_level,
+    use_log_level,
 )
 
 __version__ = "0.1.0"

@@ -330,6 +330,10 @@ class Matrix:
             self._a = np.zeros((int(a), int(ncol)), order="F")
         elif isinstance(a, Matrix):
             self._a = a._a
+        elif isinstance(a, (SymMatrix, SparseMatrix)):   # Matrix(const SymMatrix&), matrix.h:90-92
+            self._a = np.asfortranarray(a.array())
+        elif isinstance(a, (str, os.PathLike)):           # Matrix(filename)
+            self._a = load_matrix(str(a))
         else:
             arr = np.asarray(a)
             if arr.dtype != np.float64 or arr.ndim != 2:
@@ -732,6 +736,10 @@ class Sensors:
     sensor) -- or Sensors(positions)."""
 
     def __init__(self, positions=None, geom=None, orientations=None, weights=None):
+        if isinstance(positions, (str, os.PathLike)):      # Sensors(filename), sensors.h:82-88
+            self.__init__()
+            self.load(str(positions))
+            return
         self.labels = []
         self.positions = np.zeros((0, 3))
         self.radii = np.zeros(0)
@@ -1172,6 +1180,37 @@ class GainEEGMEGadjoint:
 
     def nlin(self):
         return self.EEGleadfield.nlin() + self.MEGleadfield.nlin()
+
+
+# ---------------------------------------------------------------------------
+# Logging (openmeeg/_utils.py): this library writes no log; the level is kept
+# so that code that sets and reads it runs unchanged.
+# ---------------------------------------------------------------------------
+_log_level = "warning"
+
+
+def set_log_level(level):
+    global _log_level
+    _log_level = str(level).lower()
+
+
+def get_log_level():
+    return _log_level
+
+
+class use_log_level:
+    """Context manager of the reference's log level (openmeeg._utils)."""
+
+    def __init__(self, level):
+        self.level, self.old = level, None
+
+    def __enter__(self):
+        self.old = get_log_level()
+        set_log_level(self.level)
+
+    def __exit__(self, *exc):
+        set_log_level(self.old)
+        return False
 
 
 # ---------------------------------------------------------------------------

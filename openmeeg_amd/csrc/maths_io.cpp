@@ -607,10 +607,12 @@ int omh_io_write(const char* path, const omh_linop_info* info, const double* dat
         }
         const std::string sfx = suffix_of(path);
         H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-        if (sfx == "mat") mat_write(path, *info, data, rows, cols);
-        else if (sfx == "bin") bin_write(path, *info, data, rows, cols);
+        // Matrix::save / SymMatrix::save (matrix.cpp:219-226, symmatrix.cpp:
+        // 208-216): the format of the suffix, else (no or unknown suffix)
+        // the first format in priority order that stores the matrix, MAT
+        if (sfx == "bin") bin_write(path, *info, data, rows, cols);
         else if (sfx == "txt") ascii_write(path, *info, data, rows, cols);
-        else throw std::invalid_argument("unknown file suffix '" + sfx + "' (UnknownFileSuffix): use .mat, .bin or .txt");
+        else mat_write(path, *info, data, rows, cols);
     });
 }
 

@@ -128,9 +128,15 @@ def test_mat_files_hold_the_reference_variables(tmp_path, name):
         assert np.array_equal(val, val2)
 
 
-def test_unknown_suffix_is_refused(tmp_path):
-    with pytest.raises(ValueError, match="suffix"):
-        omio.write_full(str(tmp_path / "a.xyz"), np.eye(2))
+def test_unknown_suffix_writes_mat(tmp_path):
+    """Matrix::save falls back to the first format that stores the matrix
+    when the suffix names none (matrix.cpp:219-226): MAT, as the reference's
+    Head1.hm, Head1.dsm, ... are."""
+    for name in ("a.hm", "a"):
+        f = str(tmp_path / name)
+        omio.write_full(f, np.eye(2))
+        assert open(f, "rb").read(10) == b"MATLAB 7.3"
+        assert omio.info(f).format == 0 and np.array_equal(omio.read(f)[1], np.eye(2))
 
 
 def test_tools_print_usage_and_refuse_unknown_options():
