@@ -220,7 +220,7 @@ def surf_extra(om, S, torch, eeg_seconds=None, cpu=True):
     return out
 
 
-def internal_extra(om, S, torch, eeg_seconds=None):
+def internal_extra(om, S, torch, eeg_seconds=None, cpu=True):
     """Internal potentials on the benchmark head (c2): Surf2VolMat for 20000
     points inside the brain (radius < 0.75) and the internal-potential gain
     of the 5000 benchmark dipoles at 1000 of them through one factor and one
@@ -257,6 +257,17 @@ def internal_extra(om, S, torch, eeg_seconds=None):
            "surf2vol_point_triangle_pairs_per_s": len(kept) * float(nb) / t_h2ip,
            "pipeline_seconds": round(t, 4), "pipeline_points": len(kp), "dipoles": len(dip),
            "refine_iterations": info["refine_iterations"], "rel_err_vs_fp64": err}
+    if cpu:
+        from oracle import oracle as O
+        og = O.nested_geometry(meshes, cond)
+        O.surf2vol(og, pts[:1])                         # tables and point location set up
+        t0 = time.time()
+        O.surf2vol(og, pts[:200])
+        wall = time.time() - t0
+        out["cpu_surf2vol"] = {"value": round(wall * len(kept) / 200.0, 1), "unit": "s", "cores": 1, "kind": "port",
+                               "sample": "oracle Surf2VolMat for 200 of the points (%.2f s: point location, then the "
+                                         "P1 double layer and OperatorS per triangle in C), scaled by the point "
+                                         "count %d/200" % (wall, len(kept))}
     log("bench: c2 internal potentials %s" % out)
     return out
 
@@ -366,7 +377,7 @@ def run(args):
         extras["surface_sources_c2"] = surf_extra(om, S, torch, cpu=not args.no_cpu_baseline)
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
-        extras["internal_potential_c2"] = internal_extra(om, S, torch)
+        extras["internal_potential_c2"] = internal_extra(om, S, torch, cpu=not args.no_cpu_baseline)
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
     cpu = None

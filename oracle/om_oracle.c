@@ -563,6 +563,25 @@ double orc_solid_angle(const double* tri9, const double* x) {
     Tri t = make_tri(v3(tri9[0], tri9[1], tri9[2]), v3(tri9[3], tri9[4], tri9[5]), v3(tri9[6], tri9[7], tri9[8]));
     return tri_solid_angle(&t, v3(x[0], x[1], x[2]));
 }
+/* Surf2VolMat's PartialBlock (operators.h:302-331) of mesh m at point x:
+ * row[vertex of T] += cD P1SingleLayerPotential_T(x) for the triangles T of
+ * m in mesh order, and row[T] = cS OperatorS_T(x) when with_s. */
+void orc_surf2vol_mesh(const Geom* g, i64 m, const double* x, double cD, double cS, int with_s, double* row) {
+    const V3 X = v3(x[0], x[1], x[2]);
+    for (i64 t = g->tri_off[m]; t < g->tri_off[m + 1]; ++t) {
+        Tri T = geom_tri(g, t);
+        P1S o; p1s_init(&o, &T);
+        const V3 r = p1s_eval(&o, X);
+        row[g->vert_index[g->tris[3 * t]]] += cD * r.x;
+        row[g->vert_index[g->tris[3 * t + 1]]] += cD * r.y;
+        row[g->vert_index[g->tris[3 * t + 2]]] += cD * r.z;
+        if (with_s) {
+            OpS s; ops_init(&s, &T);
+            row[g->tri_index[t]] = cS * ops_eval(&s, X);
+        }
+    }
+}
+
 /* D pair of the D blocks (operators.h:103-106): the P1 double layer of
  * tri_col integrated over tri_row (Gauss), one value per vertex of tri_col. */
 void orc_integrate_d_pair(const double* tri_row, const double* tri_col, int order, double* out3) {

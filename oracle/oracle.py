@@ -66,6 +66,7 @@ def lib():
         L.orc_solid_angle.argtypes = [dp, dp]
         L.orc_solid_angle.restype = C.c_double
         L.orc_integrate_d_pair.argtypes = [dp, dp, C.c_int, dp]
+        L.orc_surf2vol_mesh.argtypes = [C.POINTER(_Geom), i64, dp, C.c_double, C.c_double, C.c_int, dp]
         L.orc_integrate_s_pair.argtypes = [dp, dp, C.c_int]
         L.orc_integrate_s_pair.restype = C.c_double
         L.orc_sample_rows.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, i64, i64, C.c_int, dp]
@@ -531,25 +532,20 @@ def surf2vol(g, points):
         (not for a current-barrier mesh).
     Returns (matrix of the kept points x n, indices of the kept points)."""
     L = lib()
+    t = tables(g)
     pts = np.asarray(points, dtype=np.float64)
     kept, doms = _conducting_points(g, pts)
-    verts = np.array(g.vertices, dtype=np.float64)
     out = np.zeros((len(kept), g.size))
-    r3 = np.zeros(3)
     for row, (i, d) in enumerate(zip(kept, doms)):
         dom = g.domains[d]
         x = np.ascontiguousarray(pts[i])
+        orow = out[row]
         for iface, inside in dom.boundaries:
             for mid, o in iface.omeshes:
                 m = g.meshes[mid]
                 coeff = (o if inside else -o) * K
-                for kt, t in enumerate(m.tris):
-                    t9 = np.ascontiguousarray(verts[t].reshape(9))
-                    L.orc_p1slp(_p(t9), _p(x), _p(r3))
-                    for k in range(3):
-                        out[row, g.vert_index[t[k]]] += -coeff * r3[k]
-                    if not m.current_barrier:
-                        out[row, m.tri_index[kt]] = coeff / dom.cond * L.orc_operator_s(_p(t9), _p(x))
+                L.orc_surf2vol_mesh(C.byref(t.s), mid, _p(x), -coeff, coeff / dom.cond, int(not m.current_barrier),
+                                    _p(orow))
     return out, kept
 
 
