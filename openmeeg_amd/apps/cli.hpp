@@ -232,6 +232,5 @@ inline Full sensor_sum(const Sensors& s, const std::vector<double>& P, int64_t n
     return out;
 }
 
-inline bool has_arg(int argc, char** argv, int k) { return k < argc; }
 
 }  // namespace cli
