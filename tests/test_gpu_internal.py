@@ -195,3 +195,40 @@ def test_eit_source_mat_vs_oracle_and_reciprocity(om):
     print("  vs oracle: point %.3e, radius 1e-3 %.3e, sphere r=0.15 %.3e; reciprocity max %.3e"
           % (e1, e2, e3, rel.max()), flush=True)
     assert e1 < 1e-13 and e2 < 1e-13 and e3 < 1e-13 and rel.max() < 0.01
+
+
+def test_nonnested_nna1_surface_internal_eit(om):
+    """HeadNNa1 (non-nested: the NORTH and SOUTH domains share the cut
+    mesh's vertices): SurfSourceMat of a small source sphere in NORTH (its
+    boundary is north + cut, vertex columns shared between them), Surf2VolMat
+    for points in every conducting domain, and EITSourceMat for point and
+    0.15-radius electrodes, against the oracle."""
+    from oracle import oracle as O
+    from openmeeg_amd import spheres as S
+    d = os.path.join(GOLDEN, "headnna1")
+    gf, cf = os.path.join(d, "HeadNNa1.geom"), os.path.join(d, "HeadNNa1.cond")
+    g, og = om.read_geometry(gf, cf), O.load_geometry(gf, cf)
+    P, T = S.geodesic_sphere(2)
+    P = 0.15 * P + np.array([0.0, 0.0, 0.4])
+    print("HeadNNa1: surface source in NORTH, internal points, EIT", flush=True)
+    e_ssm = rel_fro(om.SurfSourceMat(g, om.Mesh(P, T)).array(), O.surf_source_mat(og, P, T))
+    rng = np.random.default_rng(6)
+    q = rng.standard_normal((60, 3))
+    q *= (rng.random(60) * 0.95 / np.linalg.norm(q, axis=1))[:, None]
+    q = q[np.abs(q[:, 2]) > 0.03]
+    Mref, kept = O.surf2vol(og, q)
+    e_ip = rel_fro(om.Surf2VolMat(g, q).array(), Mref)
+    el = S.sphere_electrodes(np.random.default_rng(8), 4, 0.97)
+    e_eit0 = rel_fro(om.EITSourceMat(g, el).array(), O.eit_source_mat(og, el))
+    es = om.Sensors(el)
+    es.radii = np.full(4, 0.15)
+    e_eit1 = rel_fro(om.EITSourceMat(g, es).array(), O.eit_source_mat(og, el, es.radii))
+    doms = set(og.domain_of_point(x, O.OracleLib()) for x in q[kept])
+    print("  ssm %.3e, surf2vol %.3e (%d points in %d domains), EIT %.3e / %.3e"
+          % (e_ssm, e_ip, len(kept), len(doms), e_eit0, e_eit1), flush=True)
+    assert len(doms) >= 3
+    # SurfSourceMat within 1e-12: its N block sums the edge-vector products
+    # in another association than the oracle (the two contractions), and
+    # here the source sits close enough for the cancellation in those sums
+    # to reach 1.3e-13
+    assert e_ssm < 1e-12 and e_ip < 1e-13 and e_eit0 < 1e-13 and e_eit1 < 1e-13
