@@ -32,7 +32,7 @@ def log(*a):
 
 
 def measured_traffic(config, dtype_f):
-    """HBM bytes per launch of the dominant kernel from the committed
+    """HBM bytes per launch of a trailing-update kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "factor_update_%s_%s_traffic.md" % (dtype_f, config))
     if not os.path.exists(path):
@@ -41,6 +41,34 @@ def measured_traffic(config, dtype_f):
         if ln.startswith("traffic_bytes_per_launch:"):
             return float(ln.split(":")[1])
     return None
+
+
+def roofline_of(precision, kern, config):
+    """Roofline entry of a path's dominant kernel, the trailing update of the
+    signed Cholesky, from the event timers: achieved = algorithmic flop of
+    the update (2 x rank per reached lower-triangle output element, the
+    FP32-equivalent flop for the split-FP16 form) / launch time.  The split
+    form issues 3 FP16 MFMA products per algorithmic product; its issued
+    rate is reported beside it (mfma_issued_*), not as `achieved`."""
+    if precision == "mixed16":
+        dom, peak, mult, tkey = "factor_update_f32s", PEAK_TFLOPS["f16"], 3.0, "f32s"
+        kname = "k_syrk_split (split-FP16 trailing update of the signed Cholesky)"
+    elif precision == "mixed":
+        dom, peak, mult, tkey = "factor_update_f32_t128", PEAK_TFLOPS["f32"], 1.0, "f32"
+        kname = "k_gemm<float,false,true,true,4,false> (interior tiles of the FP32 trailing update)"
+    else:
+        dom, peak, mult, tkey = "factor_update_f64_t64", PEAK_TFLOPS["f64"], 1.0, "f64"
+        kname = "k_gemm<double,false,true,true,2,false> (interior tiles of the FP64 trailing update)"
+    calls, sec, work = kern[dom]
+    achieved = work / sec / 1e12 if sec > 0 else 0.0
+    out = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+           "frac": round(achieved / peak, 4), "traffic": measured_traffic(config, tkey),
+           "kernel": kname, "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
+           "algorithmic_flop_per_launch": work / max(calls, 1)}
+    if mult != 1.0:
+        out["mfma_issued_tflops"] = round(mult * achieved, 2)
+        out["mfma_issued_frac"] = round(mult * achieved / peak, 4)
+    return out
 
 
 def batched_extra(om, S, torch, reps=3):
@@ -272,6 +300,60 @@ def internal_extra(om, S, torch, eeg_seconds=None, cpu=True):
     return out
 
 
+def c1_extra(om, S, torch, cpu=True):
+    """The 2562-vertex three-sphere head (n = 17926, 2000 dipoles, 128
+    electrodes) end to end: the device gain in FP64 and mixed16, the device
+    HeadMat inverse (SymMatrix::inverse, om_minverser's computation), and
+    the oracle's whole pipeline measured on this host's cores (HeadMat,
+    DipSourceMat, Head2EEGMat, LAPACK dsytrf + dsytrs, gain)."""
+    meshes, cond, dip, el = S.make_config("c1")
+    g = om.make_nested_geometry(meshes, cond)
+    ne, nd = len(el), len(dip)
+    out = {"n": g.size, "dipoles": nd, "electrodes": ne}
+    G64 = None
+    for prec in ("fp64", "mixed16"):
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t0 = time.time()
+            G, info = om.gain_eeg_pipeline(g, dip, el, precision=prec, domain_name="Brain")
+            torch.cuda.synchronize()
+            t = time.time() - t0
+        out["gpu_%s_s" % prec] = round(t, 4)
+        out["gpu_%s_stages_s" % prec] = {k: round(v, 4) for k, v in info["times"].items()}
+        Gh = om.gain_to_numpy(G, ne, nd)
+        del G
+        if prec == "fp64":
+            G64 = Gh
+        else:
+            out["gpu_mixed16_rel_err_vs_fp64"] = float(np.linalg.norm(Gh - G64) / np.linalg.norm(G64))
+    from openmeeg_amd.api import _head_mat_device, SymMatrix
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        hm = SymMatrix._from_device(_head_mat_device(g), g.size, g.n_vertex_unknowns)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        inv = hm.inverse()
+        torch.cuda.synchronize()
+        t2 = time.time()
+        del hm, inv
+        torch.cuda.empty_cache()
+    out["gpu_headmat_s"] = round(t1 - t0, 4)
+    out["gpu_inverse_s"] = round(t2 - t1, 4)
+    if cpu:
+        from oracle import oracle as O
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.time()
+        Gc, st, og = O.time_pipeline(meshes, cond, dip, el, "Brain")
+        wall = time.time() - t0
+        out["cpu_pipeline"] = {"value": round(sum(st.values()), 2), "unit": "s", "cores": cores, "kind": "port",
+                               "stages_s": {k: round(v, 3) for k, v in st.items()}, "wall_s": round(wall, 2),
+                               "rel_err_vs_gpu_fp64": float(np.linalg.norm(Gc - G64) / np.linalg.norm(Gc))}
+        out["_cpu_stages"] = (st, og, nd, ne)
+    log("bench: c1 %s" % {k: v for k, v in out.items() if not k.startswith("_")})
+    return out
+
+
 def run(args):
     import torch
     import openmeeg_amd as om
@@ -324,12 +406,16 @@ def run(args):
     err64 = None
     t64 = None
     stages64 = None
+    roofline64 = None
     if args.precision != "fp64" and not args.skip_fp64:
         torch.cuda.synchronize()
+        _lib.check(L.omh_profile_enable(1))
         t1 = time.time()
         G64, info64 = step("fp64")
         torch.cuda.synchronize()
         t64 = time.time() - t1
+        roofline64 = roofline_of("fp64", {"factor_update_f64_t64": prof("factor_update_f64_t64")}, args.config)
+        _lib.check(L.omh_profile_enable(0))
         stages64 = info64["times"]
         G64h = om.gain_to_numpy(G64, len(el), len(dip))
         del G64
@@ -339,25 +425,7 @@ def run(args):
             % (t64, {k: round(v, 4) for k, v in stages64.items()}, err64))
 
     stages = infos[-1]["times"]
-    if args.precision == "mixed16":
-        # split-FP16 trailing update: 3 FP16 MFMA products per FP32-equivalent
-        # product; the roofline counts the executed FP16 MFMA flops
-        dom, peak, mult, tkey = "factor_update_f32s", PEAK_TFLOPS["f16"], 3.0, "f32s"
-        kname = "k_syrk_split (split-FP16 trailing update of the signed Cholesky, 3 FP16 MFMA products)"
-    elif args.precision == "mixed":
-        dom, peak, mult, tkey = "factor_update_f32_t128", PEAK_TFLOPS["f32"], 1.0, "f32"
-        kname = "k_gemm<float,false,true,true,4,false> (interior tiles of the FP32 trailing update)"
-    else:
-        dom, peak, mult, tkey = "factor_update_f64_t64", PEAK_TFLOPS["f64"], 1.0, "f64"
-        kname = "k_gemm<double,false,true,true,2,false> (interior tiles of the FP64 trailing update)"
-    calls, sec, work = kern[dom]
-    achieved = mult * work / sec / 1e12 if sec > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": measured_traffic(args.config, tkey),
-                "kernel": kname, "launches": calls, "avg_launch_ms": round(sec / max(calls, 1) * 1e3, 4),
-                "flop_per_launch": mult * work / max(calls, 1),
-                "fp32_equivalent_tflops": round(work / sec / 1e12, 2) if sec > 0 else 0.0}
+    roofline = roofline_of(args.precision, kern, args.config)
     others = {}
     for k, (c, sc, wk) in kern.items():
         if c:
@@ -381,15 +449,27 @@ def run(args):
         _lib.check(L.omh_release_scratch())
         torch.cuda.empty_cache()
     cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        est, wall, desc, parts = O.time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=args.cpu_rows,
-                                                     n_factor=args.cpu_factor_n)
-        log("bench: cpu baseline %.1f s extrapolated from a %.1f s sample (%s)" % (est, wall, parts))
-        cpu = {"value": round(est, 1), "unit": "s", "cores": cores, "kind": "port",
-               "sample": desc + "; sample wall time %.1f s" % wall,
-               "stages_s": {k: round(v, 1) for k, v in parts.items()}}
+    if not args.no_c1:
+        _lib.check(L.omh_release_scratch())
+        torch.cuda.empty_cache()
+        c1 = c1_extra(om, S, torch, cpu=not args.no_cpu_baseline)
+        cpu_c1 = c1.pop("_cpu_stages", None)
+        extras["c1_end_to_end"] = c1
+        if cpu_c1 is not None and args.config != "c1":
+            from oracle import oracle as O
+            st, og1, nd1, ne1 = cpu_c1
+            og = O.nested_geometry(meshes, cond)
+            est = O.extrapolate_stages(st, og1, nd1, ne1, og, len(dip), len(el))
+            cpu = {"value": round(sum(est.values()), 1), "unit": "s", "cores": c1["cpu_pipeline"]["cores"],
+                   "kind": "port",
+                   "sample": "the oracle's whole pipeline measured on the n=%d three-sphere head (2000 dipoles, "
+                             "128 electrodes: %.1f s, stages in extras.c1_end_to_end.cpu_pipeline) and scaled to "
+                             "this workload stage by stage (HeadMat by S+D pair count, DipSourceMat by dipoles x "
+                             "source-mesh triangles, factor by n^3, solve by n^2 x electrodes); an extrapolation. "
+                             "The oracle factors with LAPACK's blocked dsytrf, the reference's om_minverser with "
+                             "the packed, unblocked DSPTRF + DSPTRI, which is many times slower"
+                             % (og1.size, c1["cpu_pipeline"]["value"]),
+                   "stages_s": {k: round(v, 1) for k, v in est.items()}}
     out = {
         "metric": "gain-matrix seconds, mixed vs FP64 on 1 GPU at <=1e-6 rel err vs FP64",
         "value": round(dt, 4),
@@ -412,6 +492,7 @@ def run(args):
         "fp64_stages_s": None if stages64 is None else {k: round(v, 4) for k, v in stages64.items()},
         "refine_iterations": infos[-1]["refine_iterations"],
         "roofline": roofline,
+        "roofline_fp64": roofline64,
         "kernels": others,
         "cpu_baseline": cpu,
         "extras": extras or None,
@@ -437,8 +518,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-heads, gradient, EEG+MEG, surface-source and internal-potential "
                          "measurements")
-    ap.add_argument("--cpu-rows", type=int, default=1500)
-    ap.add_argument("--cpu-factor-n", type=int, default=10000)
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 2562-vertex head end to end (device gains and inverse, oracle pipeline on the "
+                         "host), which also provides the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)

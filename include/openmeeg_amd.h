@@ -31,6 +31,10 @@ extern "C" {
 /* Status returned when a source mesh intersects the geometry
  * (OverlappingSourceMesh, OMExceptions.H). */
 #define OMH_STATUS_OVERLAPPING_SOURCE_MESH 5
+/* Status of a mixed-precision solve whose iterative refinement stopped
+ * without reaching its tolerance (see omh_refine_state); the outputs of
+ * the last correction are written all the same. */
+#define OMH_STATUS_NOT_CONVERGED 6
 
 typedef struct omh_geometry omh_geometry;
 typedef struct omh_mesh omh_mesh;
@@ -284,12 +288,22 @@ int omh_sym_solve_f64_blocks(const double* d_L, int64_t n, int64_t ld, const voi
 int omh_sym_solve_f32_blocks(const float* d_L, int64_t n, int64_t ld, const void* d_work, float* d_B, int64_t ldb,
                              int64_t nrhs, int32_t nseg, const int64_t* seg_start, const uint8_t* hfill, void* stream);
 
+/* Stopping rule of the refinement in omh_mixed_solve* after correction
+ * `it` (0-based) of relative size rel = ||dX||_F / ||X||_F, the one before
+ * being prev (no reference counterpart).  Returns 1 (converged) when
+ * rel <= tol, or when the corrections contract (rel < prev) and the
+ * estimated error left, rel x rel / prev, is <= tol, or when they no longer
+ * contract (rel > prev / 2, from it = 2 on) at the FP64 rounding floor
+ * rel <= min(1e3 tol, 1e-10); 2 (stalled: no longer contracting above that
+ * floor, or NaN); 0 (continue). */
+int32_t omh_refine_state(int32_t it, double rel, double prev, double tol);
+
 /* Mixed-precision solve X = H^-1 B: FP32 factor (d_L32, d_work32) with FP64
- * residuals on the FP64 matrix d_H and iterative refinement until the
- * relative correction r = ||dX||/||X|| <= tol, or the estimated error left
- * after it, r times its contraction ratio r / r_previous, <= tol, or
- * max_iter corrections.  No reference counterpart: it stands beside DSPTRS
- * for the mixed-precision path. */
+ * residuals on the FP64 matrix d_H and iterative refinement, stopped by
+ * omh_refine_state or after max_iter corrections.  Returns
+ * OMH_STATUS_NOT_CONVERGED (X, *iterations and *rel_update still written)
+ * unless omh_refine_state reached "converged".  No reference counterpart:
+ * it stands beside DSPTRS for the mixed-precision path. */
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
                     const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs,
                     int32_t max_iter, double tol, int32_t* iterations, double* rel_update, void* stream);
@@ -364,7 +378,9 @@ int omh_profile_read(const char* name, int64_t* calls, double* seconds, double* 
  * omh_batch_run: one pass after the work on `stream`.  The first pass (and
  * every pass with use_graph = 0) runs eagerly and fixes the number of
  * refinement steps; with use_graph = 1 later passes replay one captured HIP
- * graph of the whole pass (*used_graph tells which ran).
+ * graph of the whole pass (*used_graph tells which ran).  Returns
+ * OMH_STATUS_NOT_CONVERGED when the last eager pass's refinement did not
+ * converge (omh_refine_state) for every head.
  * omh_batch_tune: times the candidate tile sizes (64, 128) and grid caps of
  * the grouped trailing update on the batch's own shapes and keeps the
  * fastest (n_results x {tile, grid_cap, seconds} written to h_results if not

@@ -66,6 +66,7 @@ _SIGS = {
     "omh_sym_factor_f64_blocks": ([vp, i64, i64, vp, i32, vp, vp, p_i32, vp], C.c_int),
     "omh_sym_solve_f64": ([vp, i64, i64, vp, vp, i64, i64, vp], C.c_int),
     "omh_sym_solve_f32": ([vp, i64, i64, vp, vp, i64, i64, vp], C.c_int),
+    "omh_refine_state": ([i32, f64, f64, f64], i32),
     "omh_mixed_solve": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, p_i32, p_f64, vp], C.c_int),
     "omh_sym_solve_f64_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
     "omh_sym_solve_f32_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
@@ -115,7 +116,13 @@ class OverlappingSourceMesh(OpenMEEGError):
     """OverlappingSourceMesh (OMExceptions.H): a source mesh meets the geometry."""
 
 
+class NotConverged(OpenMEEGError):
+    """Iterative refinement of a mixed-precision solve stopped above its
+    tolerance (the results of the last correction are written)."""
+
+
 STATUS_OVERLAPPING_SOURCE_MESH = 5
+STATUS_NOT_CONVERGED = 6
 
 
 def check(rc):
@@ -128,4 +135,6 @@ def check(rc):
         raise MemoryError(msg)
     if rc == STATUS_OVERLAPPING_SOURCE_MESH:
         raise OverlappingSourceMesh(msg)
+    if rc == STATUS_NOT_CONVERGED:
+        raise NotConverged(msg)
     raise OpenMEEGError(msg)

@@ -519,18 +519,43 @@ def from_device_rows(D, n, nv, c):
 
 
 class Factor:
-    """A stored pivot-free signed Cholesky factor on the device."""
+    """A stored pivot-free signed Cholesky factor on the device.
 
-    def __init__(self, L, n, work, precision, H64=None, structure=None):
+    A mixed-precision factor whose refinement does not converge (the FP32
+    factor too coarse for the matrix) is replaced by the FP64 factor of the
+    FP64 matrix when fallback_fp64 is set (then `fell_back` is True);
+    otherwise solve() raises NotConverged."""
+
+    def __init__(self, L, n, work, precision, H64=None, structure=None, fallback_fp64=True):
         self.L, self.n, self.work, self.precision, self.H64 = L, n, work, precision, H64
         self.iterations = 0
         self.rel_update = 0.0
+        self.fell_back = False
+        self.fallback_fp64 = fallback_fp64
+        self._raw_structure = structure
         self._structure = _block_args(structure)
+
+    def _refactor_fp64(self):
+        """Factor the FP64 matrix in place (it is no longer needed for
+        residuals) and drop the FP32 factor."""
+        F = factor(self.H64, self.n, "fp64", structure=self._raw_structure)
+        self.L, self.work, self.precision, self.H64 = F.L, F.work, "fp64", None
+        self.fell_back = True
 
     def solve(self, B, nrhs, max_iter=10, tol=1e-12):
         """B <- H^-1 B (float64 device buffer, n x nrhs column-major).  With
         the block structure the factor was made with, the solves and the
         residuals skip its zero blocks."""
+        if self.precision != "fp64":
+            try:
+                return self._solve(B, nrhs, max_iter, tol)
+            except _lib.NotConverged:
+                if not self.fallback_fp64:
+                    raise
+                self._refactor_fp64()
+        return self._solve(B, nrhs, max_iter, tol)
+
+    def _solve(self, B, nrhs, max_iter, tol):
         blocks = self._structure[2:] if self._structure else ()
         if self.precision == "fp64":
             if blocks:
@@ -543,14 +568,15 @@ class Factor:
         X = _dev_f64(self.n * nrhs)
         it, rel = C.c_int32(), C.c_double()
         if blocks:
-            check(lib().omh_mixed_solve_blocks(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
-                                               _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, *blocks,
-                                               C.byref(it), C.byref(rel), _stream()))
+            rc = lib().omh_mixed_solve_blocks(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
+                                              _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, *blocks,
+                                              C.byref(it), C.byref(rel), _stream())
         else:
-            check(lib().omh_mixed_solve(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
-                                        _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, C.byref(it),
-                                        C.byref(rel), _stream()))
+            rc = lib().omh_mixed_solve(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
+                                       _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, C.byref(it),
+                                       C.byref(rel), _stream())
         self.iterations, self.rel_update = it.value, rel.value
+        check(rc)
         B.copy_(X)
         return B
 
@@ -777,7 +803,9 @@ class Sensors:
             raise ValueError("empty sensor file")
         if any(len(r) != len(rows[0]) for r in rows):
             raise ValueError("Problem while reading Sensors file: each line should have the same number of elements")
-        labeled = rows[0][0].count(".") != 1
+        # labelled unless some row's first token is a number with one '.'
+        # (sensors.cpp:67-69 scans every row)
+        labeled = not any(r[0].count(".") == 1 for r in rows)
         names = [r[0] for r in rows] if labeled else []
         if labeled:
             rows = [r[1:] for r in rows]
@@ -1251,7 +1279,8 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     torch.cuda.synchronize()
     for (a, ea), (b, eb) in zip(marks[:-1], marks[1:]):
         times[b] = ea.elapsed_time(eb) / 1e3
-    info = dict(times=times, refine_iterations=F.iterations, refine_rel_update=F.rel_update, n=n,
+    info = dict(times=times, refine_iterations=F.iterations, refine_rel_update=F.rel_update,
+                refine_fell_back_fp64=F.fell_back, n=n,
                 n_electrodes=h2em.nlin(), n_dipoles=nd)
     del F, H, D, Y
     return G, info
@@ -1291,7 +1320,8 @@ def gain_eeg_meg_pipeline(geom, dipoles, electrodes, squids, precision="fp64", d
     marks.append(("gain", ev())); marks[-1][1].record()
     torch.cuda.synchronize()
     times = {b: ea.elapsed_time(eb) / 1e3 for (a, ea), (b, eb) in zip(marks[:-1], marks[1:])}
-    info = dict(times=times, refine_iterations=F.iterations, n=n, n_electrodes=ne, n_squids=ns, n_dipoles=nd)
+    info = dict(times=times, refine_iterations=F.iterations, refine_fell_back_fp64=F.fell_back, n=n,
+                n_electrodes=ne, n_squids=ns, n_dipoles=nd)
     del F, H, D, Y
     return Ge, Gm, info
 

@@ -164,8 +164,11 @@ inline Sensors read_sensors(const std::string& path) {
     for (const auto& r : rows)
         if (r.size() != rows[0].size())
             throw std::runtime_error("Problem while reading Sensors file: each line should have the same number of elements");
-    const std::string& t0 = rows[0][0];
-    const bool labeled = std::count(t0.begin(), t0.end(), '.') != 1;
+    // labelled unless some row's first token is a number with one '.'
+    // (sensors.cpp:67-69 scans every row)
+    bool labeled = true;
+    for (const auto& r : rows)
+        if (std::count(r[0].begin(), r[0].end(), '.') == 1) labeled = false;
     const size_t off = labeled ? 1 : 0;
     const size_t ncol = rows[0].size() - off;
     if (ncol != 3 && ncol != 4 && ncol != 6 && ncol != 7)

@@ -28,7 +28,11 @@ Full dense_of(const Sparse& s) {
     m.nlin = s.nlin;
     m.ncol = s.ncol;
     m.a.assign((size_t)(s.nlin * s.ncol), 0.0);
-    for (size_t k = 0; k < s.vals.size(); ++k) m(s.rows[k], s.cols[k]) = s.vals[k];
+    for (size_t k = 0; k < s.vals.size(); ++k) {
+        if (s.rows[k] < 0 || s.rows[k] >= s.nlin || s.cols[k] < 0 || s.cols[k] >= s.ncol)
+            throw std::runtime_error("sparse matrix entry out of range");
+        m(s.rows[k], s.cols[k]) = s.vals[k];
+    }
     return m;
 }
 

@@ -1,7 +1,6 @@
 """Builds the HIP library openmeeg_amd/libopenmeeg_amd.so for gfx950, the
 host library of the matrix file formats openmeeg_amd/libopenmeeg_amd_io.so
-and the command-line tools openmeeg_amd/bin/{om_assemble,om_minverser,om_gain,
-om_forward}.
+and the command-line tools openmeeg_amd/bin/{om_assemble,om_minverser,om_gain}.
 
 hipcc compiles every csrc/*.hip (device + host) and csrc/*.cpp (host only,
 except maths_io.cpp) into openmeeg_amd/_build/, then links the shared
@@ -24,7 +23,7 @@ APPS = os.path.join(HERE, "apps")
 BIN = os.path.join(HERE, "bin")
 HDF5 = os.environ.get("OMH_HDF5_ROOT", "/opt/conda")
 IO_SRC = "maths_io.cpp"
-TOOLS = ("om_assemble", "om_minverser", "om_gain", "om_forward")
+TOOLS = ("om_assemble", "om_minverser", "om_gain")
 ARCH = os.environ.get("OMH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -65,7 +64,7 @@ def build_io(verbose=True):
 
 
 def build_tools(verbose=True):
-    """om_assemble, om_minverser, om_gain, om_forward (host code over both
+    """om_assemble, om_minverser, om_gain (host code over both
     libraries)."""
     os.makedirs(BIN, exist_ok=True)
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -44,6 +45,9 @@ int guard(F&& f) {
     } catch (const omh::OverlappingSourceMesh& e) {
         g_err = e.what();
         return OMH_STATUS_OVERLAPPING_SOURCE_MESH;
+    } catch (const omh::NotConverged& e) {
+        g_err = e.what();
+        return OMH_STATUS_NOT_CONVERGED;
     } catch (const std::invalid_argument& e) {
         g_err = std::string("invalid argument: ") + e.what();
         return 2;
@@ -74,6 +78,15 @@ omh::Layout layout_of(const omh::Geometry& g, int64_t ld, int ordering) {
     L.nt = g.n_tri_unknowns;
     L.tri_first = ordering == OMH_ORDER_TRIANGLES_FIRST ? 1 : 0;
     return L;
+}
+
+[[noreturn]] void not_converged(int it, double rel, double tol) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "iterative refinement did not converge: relative update %.3e after %d corrections (tol %.3e); "
+                  "the FP32 factor is too coarse for this matrix",
+                  rel, it, tol);
+    throw omh::NotConverged(buf);
 }
 
 void need(const void* p, const char* what) {
@@ -715,6 +728,10 @@ int omh_sym_solve_f32_blocks(const float* d_L, int64_t n, int64_t ld, const void
     });
 }
 
+int32_t omh_refine_state(int32_t it, double rel, double prev, double tol) {
+    return (int32_t)omh::refine_state(it, rel, prev, tol);
+}
+
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32, const void* d_work32,
                     const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs, int32_t max_iter,
                     double tol, int32_t* iterations, double* rel_update, void* stream) {
@@ -727,10 +744,12 @@ int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L3
         if (n <= 0 || ldh < n || ld32 < n || ldb < n || ldx < n || nrhs < 0 || max_iter < 1)
             throw std::invalid_argument("bad sizes");
         double rel = 0.0;
+        bool ok = false;
         const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
-                                         &rel, S(stream));
+                                         &rel, &ok, S(stream));
         if (iterations) *iterations = it;
         if (rel_update) *rel_update = rel;
+        if (!ok) not_converged(it, rel, tol);
     });
 }
 
@@ -748,10 +767,12 @@ int omh_mixed_solve_blocks(const double* d_H, int64_t n, int64_t ldh, const floa
             throw std::invalid_argument("bad sizes");
         const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
         double rel = 0.0;
+        bool ok = false;
         const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
-                                         &rel, S(stream), &bf);
+                                         &rel, &ok, S(stream), &bf);
         if (iterations) *iterations = it;
         if (rel_update) *rel_update = rel;
+        if (!ok) not_converged(it, rel, tol);
     });
 }
 

@@ -46,6 +46,8 @@ namespace {
 struct Arena {
     void* p[SCR_NSLOTS] = {};
     size_t sz[SCR_NSLOTS] = {};
+    hipStream_t last[SCR_NSLOTS] = {};
+    hipEvent_t ev[SCR_NSLOTS] = {};
 };
 std::mutex g_arena_mu;
 std::map<int, Arena> g_arenas;
@@ -53,45 +55,58 @@ std::map<int, Arena> g_arenas;
 
 thread_local ScratchArena* tl_arena = nullptr;   // ArenaScope override of this thread
 
-static void* grow(void*& p, size_t& sz, size_t bytes) {
-    OMH_HIP(hipDeviceSynchronize());
-    if (p) OMH_HIP(hipFree(p));
+// Hands out slot `slot` for work on stream st.  When the slot was last
+// handed out on another stream, st first waits for everything enqueued there
+// so far (which includes the previous user's kernels).  Growth is stream
+// ordered: the old buffer is freed and the new one allocated on st, so that
+// neither waits for the rest of the device.  A frozen arena (a batch plan's,
+// used on the plan's own stream, possibly under graph capture) adds no
+// cross-stream waits.
+static void* take(void*& p, size_t& sz, hipStream_t& last, hipEvent_t& ev, size_t bytes, hipStream_t st,
+                  bool frozen) {
+    if (p && last != st && !frozen) {
+        if (!ev) OMH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        OMH_HIP(hipEventRecord(ev, last));
+        OMH_HIP(hipStreamWaitEvent(st, ev, 0));
+    }
+    last = st;
+    if (sz >= bytes && p) return p;
+    if (frozen) throw std::runtime_error("frozen scratch arena is too small (plan sizes changed)");
+    if (p) OMH_HIP(hipFreeAsync(p, st));
     p = nullptr;
     sz = 0;
-    const size_t want = (bytes + 4095) & ~size_t(4095);
-    OMH_HIP(hipMalloc(&p, want));
+    const size_t want = (std::max<size_t>(bytes, 1) + 4095) & ~size_t(4095);
+    OMH_HIP(hipMallocAsync(&p, want, st));
     sz = want;
     return p;
 }
 
 void ScratchArena::release() {
-    for (int s = 0; s < SCR_NSLOTS; ++s)
+    for (int s = 0; s < SCR_NSLOTS; ++s) {
         if (p[s]) {
-            (void)hipDeviceSynchronize();
+            (void)hipStreamSynchronize(last[s]);
             (void)hipFree(p[s]);
             p[s] = nullptr;
             sz[s] = 0;
         }
+        if (ev[s]) (void)hipEventDestroy(ev[s]);
+        ev[s] = nullptr;
+    }
 }
 
 ArenaScope::ArenaScope(ScratchArena* a) : prev_(tl_arena) { tl_arena = a; }
 ArenaScope::~ArenaScope() { tl_arena = prev_; }
 
-void* scratch(ScratchSlot slot, size_t bytes) {
+void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st) {
     if (tl_arena) {
         ScratchArena& a = *tl_arena;
-        if (a.sz[slot] < bytes) {
-            if (a.frozen) throw std::runtime_error("frozen scratch arena is too small (plan sizes changed)");
-            grow(a.p[slot], a.sz[slot], bytes);
-        }
-        return a.p[slot];
+        return take(a.p[slot], a.sz[slot], a.last[slot], a.ev[slot], bytes, st, a.frozen);
     }
     int dev = 0;
     OMH_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena& a = g_arenas[dev];
-    if (a.sz[slot] < bytes) grow(a.p[slot], a.sz[slot], bytes);
-    return a.p[slot];
+    return take(a.p[slot], a.sz[slot], a.last[slot], a.ev[slot], bytes, st, false);
 }
 
 void release_scratch() {
@@ -100,9 +115,13 @@ void release_scratch() {
     (void)hipGetDevice(&cur);
     for (auto& kv : g_arenas) {
         (void)hipSetDevice(kv.first);
-        (void)hipDeviceSynchronize();
-        for (int s = 0; s < SCR_NSLOTS; ++s)
-            if (kv.second.p[s]) (void)hipFree(kv.second.p[s]);
+        for (int s = 0; s < SCR_NSLOTS; ++s) {
+            if (kv.second.p[s]) {
+                (void)hipStreamSynchronize(kv.second.last[s]);
+                (void)hipFree(kv.second.p[s]);
+            }
+            if (kv.second.ev[s]) (void)hipEventDestroy(kv.second.ev[s]);
+        }
     }
     g_arenas.clear();
     (void)hipSetDevice(cur);
@@ -133,8 +152,7 @@ DeviceGeometry::~DeviceGeometry() {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    (void)hipDeviceSynchronize();
-    (void)hipFree(tri);
+    (void)hipFree(tri);   // hipFree waits for the kernels that may still read the tables
     (void)hipFree(tri_vid);
     (void)hipFree(vert_unk);
     (void)hipFree(slot_vid);
@@ -257,7 +275,7 @@ DeviceGeometry& device_geometry(Geometry& g) {
     if (dg->n_tris > 0) {
         k_gauss_table<<<(unsigned)cdiv(dg->n_tris, 256), 256>>>(dg->tri, dg->n_tris, dg->gp);
         OMH_LAUNCH_CHECK();
-        OMH_HIP(hipDeviceSynchronize());
+        OMH_HIP(hipStreamSynchronize(nullptr));
     }
     dg->device = dev;
     g.dev = dg;
@@ -955,7 +973,7 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
         if (v > 0) chunk = std::min<int64_t>(chunk, (int64_t)v);
     }
     const int64_t ldu = (chunk + 31) / 32 * 32;
-    double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)nt1);
+    double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)nt1, st);
     for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
         const int64_t nj = std::min<int64_t>(chunk, nb - j0);
         const dim3 g1((unsigned)cdiv(nt1, N1_ROWS), (unsigned)cdiv(nj, N1_COLS));
@@ -980,7 +998,7 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
     const size_t budget = size_t(1) << 30;   // 1 GiB of pass-1 scratch
     int64_t chunk = std::max<int64_t>(64, (int64_t)(budget / (3 * sizeof(double) * (size_t)n2)));
     chunk = std::min(chunk, n1);
-    double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)chunk);
+    double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)chunk, st);
     for (int64_t c0 = 0; c0 < n1; c0 += chunk) {
         const int64_t nc = std::min(chunk, n1 - c0);
         const int64_t blk1 = cdiv(nc, 64) * cdiv(n2, 4);
@@ -1003,7 +1021,7 @@ bool meshes_intersect(Geometry& g, Geometry& src, hipStream_t st) {
     DeviceGeometry& dg = device_geometry(g);
     DeviceGeometry& ds = device_geometry(src);
     if (dg.n_tris <= 0 || ds.n_tris <= 0) return false;
-    unsigned* hit = (unsigned*)scratch(SCR_SMALL, 64);
+    unsigned* hit = (unsigned*)scratch(SCR_SMALL, 64, st);
     k_zero_counts<<<1, 64, 0, st>>>(hit);
     OMH_LAUNCH_CHECK();
     const dim3 grid((unsigned)cdiv(ds.n_tris, 256), (unsigned)std::min<int64_t>(dg.n_tris, 1024));
@@ -1042,7 +1060,7 @@ void surf_source_mat(Geometry& g, Geometry& src, int order, double* out, const L
             if (n1 <= 0 || na <= 0) continue;
             // N block from an S block of coefficient 1 in a temporary
             // (operators.h:428-437)
-            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2, st);
             {
                 const int64_t ntx = cdiv(n2, TILE);
                 ProfScope ps("s_block", (double)n1 * (double)n2 * npts[order], st);
@@ -1055,7 +1073,7 @@ void surf_source_mat(Geometry& g, Geometry& src, int order, double* out, const L
             const int64_t chunk = std::min<int64_t>(
                 nb, std::max<int64_t>(32, (int64_t)(budget / (3 * sizeof(double) * (size_t)n1)) / 32 * 32));
             const int64_t ldu = (chunk + 31) / 32 * 32;
-            double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)n1);
+            double* U = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)ldu * (size_t)n1, st);
             for (int64_t j0 = 0; j0 < nb; j0 += chunk) {
                 const int64_t nj = std::min<int64_t>(chunk, nb - j0);
                 const dim3 g1((unsigned)cdiv(n1, N1_ROWS), (unsigned)cdiv(nj, N1_COLS));
@@ -1072,7 +1090,7 @@ void surf_source_mat(Geometry& g, Geometry& src, int order, double* out, const L
             // unknowns; a current barrier's triangles have no rows
             if (A.barrier) continue;
             const int64_t dchunk = std::min<int64_t>(n1, std::max<int64_t>(64, (int64_t)(budget / (3 * sizeof(double) * (size_t)n2))));
-            double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)dchunk);
+            double* W = (double*)scratch(SCR_DTMP, 3 * sizeof(double) * (size_t)n2 * (size_t)dchunk, st);
             for (int64_t c0 = 0; c0 < n1; c0 += dchunk) {
                 const int64_t nc = std::min(dchunk, n1 - c0);
                 ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
@@ -1108,9 +1126,9 @@ static void launch_pairs_far_t(DeviceGeometry& dg, int m1, int m2, double coeffS
         if (v > 0) nc = std::min<int64_t>(nc, (int64_t)v);
     }
     const size_t wsz = 3 * (size_t)nc * (size_t)n2;
-    double* W = (DA || DB) ? (double*)scratch(SCR_DTMP, sizeof(double) * 2 * wsz) : nullptr;
+    double* W = (DA || DB) ? (double*)scratch(SCR_DTMP, sizeof(double) * 2 * wsz, st) : nullptr;
     const int64_t qcap = nc * n2;
-    char* qb = (char*)scratch(SCR_PAIRQ, sizeof(int64_t) * (size_t)qcap + 256);
+    char* qb = (char*)scratch(SCR_PAIRQ, sizeof(int64_t) * (size_t)qcap + 256, st);
     PairStripe ps{};
     ps.a0 = A.tri_begin;
     ps.n1 = n1;
@@ -1180,7 +1198,7 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
             double* tmp = nullptr;
             if (!inH) {
                 const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
-                tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+                tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2, st);
             }
             launch_pairs_far(dg, m1, m2, inH ? mp.cS : 1.0, tmp, !b1, same ? !b1 : !b2, mp.cD, H, L, st);
             launch_n(dg, g, m1, m2, inH ? mp.cN / mp.cS : mp.cN, tmp, H, L, st);
@@ -1195,7 +1213,7 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
             launch_n(dg, g, m1, m2, mp.cN / Scoeff, nullptr, H, L, st);
         } else {
             const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
-            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2);
+            double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2, st);
             launch_s(dg, m1, m2, order, 1.0, H, L, tmp, far, st);
             launch_n(dg, g, m1, m2, mp.cN, tmp, H, L, st);
         }
@@ -1211,7 +1229,7 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         }
     }
     // deflate all current barriers, one constant per isolated part
-    double* coef = (double*)scratch(SCR_SMALL, 64);
+    double* coef = (double*)scratch(SCR_SMALL, 64, st);
     for (const auto& part : g.parts) {
         int64_t nbv = 0, i_first = 0;
         for (int mid : part) {

@@ -379,6 +379,7 @@ void BatchPlan::run_eager(hipStream_t st) {
     if (precision_ == 1) {
         double prev = 2.0;
         int it = 0;
+        RefineState state = REFINE_CONTINUE;
         for (; it < max_refine_; ++it) {
             launch(ir_pre_, st);
             launch(solve_, st);
@@ -390,11 +391,13 @@ void BatchPlan::run_eager(hipStream_t st) {
                 H.rel_update = xn > 0 ? dn / xn : 0.0;
                 worst = std::max(worst, H.rel_update);
             }
-            if (refine_converged(it, worst, prev, refine_tol_)) { ++it; break; }
+            state = refine_state(it, worst, prev, refine_tol_);
+            if (state != REFINE_CONTINUE) { ++it; break; }
             prev = worst;
             launch(ir_resid_, st);
         }
         refine_steps_ = std::max(1, std::min(it, max_refine_));
+        refine_ok_ = state == REFINE_CONVERGED;
     } else {
         launch(solve_, st);
         refine_steps_ = 0;
@@ -435,6 +438,9 @@ bool BatchPlan::run(hipStream_t caller, bool use_graph) {
     }
     OMH_HIP(hipEventRecord(ev_out_, stream_));
     OMH_HIP(hipStreamWaitEvent(caller, ev_out_, 0));
+    if (!refine_ok_)
+        throw NotConverged("iterative refinement of the batch did not reach its tolerance: the FP32 factor of a "
+                           "head is too coarse for its matrix (use FP64)");
     return graph;
 }
 

@@ -119,7 +119,8 @@ private:
     bool calibrated_ = false;
     int refine_steps_ = 0;
     int max_refine_ = 10;
-    double refine_tol_ = 1e-10;   // estimated error left (refine_converged)
+    double refine_tol_ = 1e-10;   // estimated error left (refine_state)
+    bool refine_ok_ = true;       // the last eager pass reached refine_tol_
     int trail_wb_ = 4;
     int64_t trail_cap_ = 0;
 

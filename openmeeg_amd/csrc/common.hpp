@@ -14,6 +14,12 @@ struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// Iterative refinement stopped without reaching its tolerance (the results
+// of the last correction are still written).
+struct NotConverged : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 #define OMH_HIP(call)                                                                    \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \
@@ -25,9 +31,11 @@ struct HipError : std::runtime_error {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Grow-only device scratch buffer per slot and per device.  Kernels that use
-// it run on the caller's stream; growing synchronises the device first so
-// that no in-flight kernel still uses the old buffer.
+// Grow-only device scratch buffer per slot and per device, handed out for
+// work on stream st: st waits for the slot's previous stream when that
+// differs, and growth frees and allocates on st (stream ordered; no
+// device-wide synchronisation).  Calls that overlap on one device and use
+// the same slot from unrelated host threads race.
 enum ScratchSlot {
     SCR_ASSEMBLY = 0,
     SCR_DTMP = 1,
@@ -39,7 +47,7 @@ enum ScratchSlot {
     SCR_SRCCOEF = 7,   // source-matrix coefficients (the source matrix may run beside the factorisation)
     SCR_NSLOTS = 8
 };
-void* scratch(ScratchSlot slot, size_t bytes);
+void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st);
 void release_scratch();
 
 // A private set of scratch slots (e.g. owned by a batch plan whose launches
@@ -49,6 +57,8 @@ void release_scratch();
 struct ScratchArena {
     void* p[SCR_NSLOTS] = {};
     size_t sz[SCR_NSLOTS] = {};
+    hipStream_t last[SCR_NSLOTS] = {};
+    hipEvent_t ev[SCR_NSLOTS] = {};
     bool frozen = false;
     void release();
     ~ScratchArena() { release(); }

@@ -1659,7 +1659,7 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
             xp += (size_t)n * FACT_NB * sizeof(float);
         }
     }
-    int* info = (int*)scratch(SCR_SMALL, 64);
+    int* info = (int*)scratch(SCR_SMALL, 64, st);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
     const float tiny = 1e-30f;
@@ -1795,7 +1795,7 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
     T* sgn = Dinv + (size_t)nblk * FACT_NB * FACT_NB;
     T* Xbuf[2] = {(T*)work, sgn + sign_span(n)};
     if (split16 && sizeof(T) == 4) return sym_factor_split((float*)A, n, lda, work, st, bf);
-    int* info = (int*)scratch(SCR_SMALL, 64);
+    int* info = (int*)scratch(SCR_SMALL, 64, st);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
     const T tiny = sizeof(T) == 8 ? T(1e-290) : T(1e-30);
@@ -1871,7 +1871,7 @@ template FactorInfo sym_factor<float>(float*, int64_t, int64_t, void*, hipStream
 
 template <typename T>
 int block_fill_violated(const T* A, int64_t n, int64_t lda, const BlockFill& bf, hipStream_t st) {
-    int* flag = (int*)scratch(SCR_SMALL, 64);
+    int* flag = (int*)scratch(SCR_SMALL, 64, st);
     k_set_int<<<1, 1, 0, st>>>(flag, 0);
     OMH_LAUNCH_CHECK();
     const int ns = bf.nseg();
@@ -2082,36 +2082,38 @@ void copy_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t 
     OMH_LAUNCH_CHECK();
 }
 
-double frob_norm_f64(const double* a, int64_t lda, int64_t m, int64_t n, hipStream_t st) {
-    const unsigned g = std::min<unsigned>(grid_for(m * n), 1024);
-    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * 1024);
-    k_sumsq<double><<<g, 256, 0, st>>>(a, lda, m, n, part);
+// Frobenius norm through NORM_PARTS partial sums in `part` (device), added
+// on the host.
+template <typename T>
+static double frob_norm_with(const T* a, int64_t lda, int64_t m, int64_t n, double* part, hipStream_t st) {
+    const unsigned g = std::min<unsigned>(grid_for(m * n), (unsigned)NORM_PARTS);
+    k_sumsq<T><<<g, 256, 0, st>>>(a, lda, m, n, part);
     OMH_LAUNCH_CHECK();
-    std::vector<double> h(g);
-    OMH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
+    double h[NORM_PARTS];
+    OMH_HIP(hipMemcpyAsync(h, part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
     OMH_HIP(hipStreamSynchronize(st));
     double s = 0.0;
-    for (double v : h) s += v;
+    for (unsigned i = 0; i < g; ++i) s += h[i];
     return std::sqrt(s);
+}
+
+double frob_norm_f64(const double* a, int64_t lda, int64_t m, int64_t n, hipStream_t st) {
+    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
+    return frob_norm_with<double>(a, lda, m, n, part, st);
 }
 
 double frob_norm_f32(const float* a, int64_t lda, int64_t m, int64_t n, hipStream_t st) {
-    const unsigned g = std::min<unsigned>(grid_for(m * n), 1024);
-    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * 1024);
-    k_sumsq<float><<<g, 256, 0, st>>>(a, lda, m, n, part);
-    OMH_LAUNCH_CHECK();
-    std::vector<double> h(g);
-    OMH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
-    OMH_HIP(hipStreamSynchronize(st));
-    double s = 0.0;
-    for (double v : h) s += v;
-    return std::sqrt(s);
+    double* part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
+    return frob_norm_with<float>(a, lda, m, n, part, st);
 }
 
-bool refine_converged(int it, double rel, double prev, double tol) {
-    if (rel <= tol) return true;                          // the update itself
-    if (it >= 1 && rel * (rel / prev) <= tol) return true;   // update x its contraction ratio
-    return it >= 2 && rel > 0.5 * prev;                   // stagnation at the FP64 rounding floor
+RefineState refine_state(int it, double rel, double prev, double tol) {
+    if (!(rel == rel)) return REFINE_STALLED;                                     // NaN
+    if (rel <= tol) return REFINE_CONVERGED;                                      // the update itself
+    if (it >= 1 && rel < prev && rel * (rel / prev) <= tol) return REFINE_CONVERGED;   // update x contraction
+    if (it >= 2 && rel > 0.5 * prev)                                              // no longer contracting
+        return rel <= std::min(1e3 * tol, REFINE_FLOOR) ? REFINE_CONVERGED : REFINE_STALLED;
+    return REFINE_CONTINUE;
 }
 
 // Side streams (default priority) for independent products that are too
@@ -2175,28 +2177,43 @@ static void residual_update(const double* H, int64_t n, int64_t ldh, const doubl
 
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, hipStream_t st, const BlockFill* bf) {
-    float* R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r);
-    double* R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r);
+                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf) {
+    RefineBuffers buf;
+    buf.R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r, st);
+    buf.R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r, st);
+    buf.part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
+    return mixed_refine_in(H, n, ldh, L32, ld32, work32, B, ldb, X, ldx, r, max_iter, tol, final_rel_update, converged,
+                           buf, st, bf);
+}
+
+int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
+                    const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
+                    double* final_rel_update, bool* converged, const RefineBuffers& buf, hipStream_t st,
+                    const BlockFill* bf) {
+    float* R32 = buf.R32;
+    double* R = buf.R;
     fill_f64(X, n, r, ldx, 0.0, st);
     copy_f64(B, ldb, R, n, n, r, st);
     int it = 0;
     double rel = 1.0, prev = 2.0;
+    RefineState state = REFINE_CONTINUE;
     for (; it < max_iter; ++it) {
         convert_f64_to_f32(R, n, R32, n, n, r, false, st);
         sym_solve<float>(L32, n, ld32, work32, R32, n, r, st, bf);
         k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
         OMH_LAUNCH_CHECK();
-        const double dn = frob_norm_f32(R32, n, n, r, st);
-        const double xn = frob_norm_f64(X, ldx, n, r, st);
-        rel = xn > 0 ? dn / xn : 0.0;
-        if (refine_converged(it, rel, prev, tol)) { ++it; break; }
+        const double dn = frob_norm_with<float>(R32, n, n, r, buf.part, st);
+        const double xn = frob_norm_with<double>(X, ldx, n, r, buf.part, st);
+        rel = xn > 0 ? dn / xn : (dn > 0 ? 1.0 : 0.0);
+        state = refine_state(it, rel, prev, tol);
+        if (state != REFINE_CONTINUE) { ++it; break; }
         prev = rel;
         // R = B - H X
         copy_f64(B, ldb, R, n, n, r, st);
         residual_update(H, n, ldh, X, ldx, r, R, n, st, bf);
     }
     if (final_rel_update) *final_rel_update = rel;
+    if (converged) *converged = state == REFINE_CONVERGED;
     return it;
 }
 

@@ -169,17 +169,40 @@ double frob_norm_f32(const float* a, int64_t lda, int64_t m, int64_t n, hipStrea
 
 // Stopping rule of the refinement after correction `it` (0-based) with
 // relative size rel = ||dX||_F / ||X||_F (prev: that of the correction
-// before): rel <= tol; or the estimated error left, rel times the
-// contraction ratio rel / prev, <= tol; or stagnation (rel > prev / 2 from
-// the third correction on).
-bool refine_converged(int it, double rel, double prev, double tol);
+// before).  REFINE_CONVERGED: rel <= tol; or the corrections contract
+// (rel < prev) and the estimated error left, rel times the contraction
+// ratio rel / prev, is <= tol; or they stopped contracting (rel > prev / 2
+// from the third correction on) at the FP64 rounding floor, rel <=
+// min(1e3 tol, REFINE_FLOOR).  REFINE_STALLED: they stopped contracting
+// above that floor (a factor too coarse for the matrix, or divergence).
+// REFINE_CONTINUE otherwise.
+enum RefineState { REFINE_CONTINUE = 0, REFINE_CONVERGED = 1, REFINE_STALLED = 2 };
+constexpr double REFINE_FLOOR = 1e-10;
+RefineState refine_state(int it, double rel, double prev, double tol);
 
 // Iterative refinement: X = H^-1 B with the FP32 factor (L32, work32) and
-// FP64 residuals R = B - H X.  Stops by refine_converged or after max_iter
-// corrections; returns the number of corrections applied.
+// FP64 residuals R = B - H X.  Stops by refine_state or after max_iter
+// corrections; returns the number of corrections applied and sets
+// *converged (REFINE_CONVERGED reached).
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, hipStream_t st, const BlockFill* bf = nullptr);
+                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf = nullptr);
+
+// Device buffers of the refinement: the FP64 residual R (n x r), its FP32
+// copy R32 (n x r) and NORM_PARTS doubles of norm partial sums.
+constexpr int NORM_PARTS = 1024;
+struct RefineBuffers {
+    float* R32 = nullptr;
+    double* R = nullptr;
+    double* part = nullptr;
+};
+// mixed_refine with caller-given buffers (allocates nothing on the device;
+// with a block structure the residual's side streams are created on first
+// use).
+int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
+                    const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
+                    double* final_rel_update, bool* converged, const RefineBuffers& buf, hipStream_t st,
+                    const BlockFill* bf = nullptr);
 
 // out(e, j) = sum_k vals(e,k) Hm(cols(e,k), j)  (3 non-zeros per row).
 // out (n x ne, leading dimension ldo) = the dense transpose of a sparse

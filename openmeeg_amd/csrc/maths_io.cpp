@@ -385,11 +385,13 @@ omh_linop_info mat_info(const std::string& path) {
         li.storage = OMH_IO_SPARSE;
         li.dimension = 2;
         li.nlin = (int64_t)nlin;
+        if (dj.empty() || dj[0] < 1) throw FileError(path + ": sparse matrix without column starts");
         li.ncol = (int64_t)dj[0] - 1;
-        if (H5Lexists(g, "data", H5P_DEFAULT) > 0) {
-            H5Handle d(H5Dopen2(g, "data", H5P_DEFAULT), H5Dclose, "sparse data");
-            li.nnz = (int64_t)dims_of(d)[0];
-        }
+        // stored entries = jc[ncol] (MATLAB may pad ir/data to nzmax)
+        std::vector<uint64_t> jcv(dj[0]);
+        h5check(H5Dread(jc, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, jcv.data()), "read jc");
+        li.nnz = (int64_t)jcv.back();
+        if (li.nnz < 0) throw FileError(path + ": bad sparse column starts");
         return li;
     }
     throw FileError(path + ": variable is not a matrix, a symmatrix struct or a sparse matrix");
@@ -409,22 +411,38 @@ void mat_read(const std::string& path, const omh_linop_info& li, double* data, i
         h5check(H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "read symmatrix.data");
         return;
     }
-    // compressed columns -> (row, col) ordered triplets
-    std::vector<uint64_t> ir(li.nnz), jc(li.ncol + 1);
-    std::vector<double> v(li.nnz);
+    // compressed columns -> (row, col) ordered triplets; every dataset is
+    // read into a buffer of its own size, then checked against jc
+    std::vector<uint64_t> jc, ir;
+    std::vector<double> v;
     {
         H5Handle d(H5Dopen2(g, "jc", H5P_DEFAULT), H5Dclose, "sparse jc");
+        const auto dims = dims_of(d);
+        if (dims.empty() || (int64_t)dims[0] != li.ncol + 1) throw FileError(path + ": sparse jc size changed");
+        jc.resize(dims[0]);
         h5check(H5Dread(d, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, jc.data()), "read jc");
     }
-    if (li.nnz > 0) {
+    const uint64_t nnz = (uint64_t)li.nnz;
+    if (jc[0] != 0 || jc[li.ncol] != nnz) throw FileError(path + ": sparse jc does not match the entry count");
+    for (int64_t j = 0; j < li.ncol; ++j)
+        if (jc[j] > jc[j + 1]) throw FileError(path + ": sparse jc is not non-decreasing");
+    if (nnz > 0) {
         H5Handle d(H5Dopen2(g, "data", H5P_DEFAULT), H5Dclose, "sparse data");
+        const auto dd = dims_of(d);
+        if (dd.empty() || dd[0] < nnz) throw FileError(path + ": sparse data shorter than jc[ncol]");
+        v.resize(dd[0]);
         h5check(H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, v.data()), "read data");
         H5Handle r(H5Dopen2(g, "ir", H5P_DEFAULT), H5Dclose, "sparse ir");
+        const auto dr = dims_of(r);
+        if (dr.empty() || dr[0] < nnz) throw FileError(path + ": sparse ir shorter than jc[ncol]");
+        ir.resize(dr[0]);
         h5check(H5Dread(r, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, ir.data()), "read ir");
+        for (uint64_t k = 0; k < nnz; ++k)
+            if (ir[k] >= (uint64_t)li.nlin) throw FileError(path + ": sparse row index out of range");
     }
     std::vector<std::pair<std::pair<int64_t, int64_t>, double>> e;
     for (int64_t j = 0; j < li.ncol; ++j)
-        for (uint64_t k = jc[j]; k < jc[j + 1] && k < (uint64_t)li.nnz; ++k) e.push_back({{(int64_t)ir[k], j}, v[k]});
+        for (uint64_t k = jc[j]; k < jc[j + 1]; ++k) e.push_back({{(int64_t)ir[k], j}, v[k]});
     std::sort(e.begin(), e.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     for (size_t k = 0; k < e.size(); ++k) {
         rows[k] = e[k].first.first;

@@ -642,7 +642,7 @@ static std::vector<double> mesh_solid_angles(Geometry& g, const double* d_pts, i
     std::vector<int64_t> mt(nm + 1);
     for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
     mt[nm] = dg.n_tris;
-    int64_t* d_mt = (int64_t*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + sizeof(double) * npts * nm);
+    int64_t* d_mt = (int64_t*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + sizeof(double) * npts * nm, st);
     double* d_sa = (double*)(d_mt + nm + 1);
     OMH_HIP(hipMemcpyAsync(d_mt, mt.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, st));
     OMH_HIP(hipStreamSynchronize(st));
@@ -713,7 +713,7 @@ void dsm_launch(Geometry& g, const double* d_dip, int64_t n_dip, const double* d
         const size_t budget = size_t(1) << 29;
         int64_t chunk = std::max<int64_t>(4, (int64_t)(budget / (3 * sizeof(double) * (size_t)nt)));
         chunk = std::min(chunk, n_dip);
-        double* W = (double*)scratch(SCR_SOURCES, 3 * sizeof(double) * (size_t)nt * (size_t)chunk);
+        double* W = (double*)scratch(SCR_SOURCES, 3 * sizeof(double) * (size_t)nt * (size_t)chunk, st);
         for (int64_t s0 = 0; s0 < n_dip; s0 += chunk) {
             const int64_t nd = std::min(chunk, n_dip - s0);
             const int64_t b1 = cdiv(nt, 64) * cdiv(nd, 4);
@@ -734,7 +734,7 @@ void dip_source_mat(Geometry& g, const double* d_dip, int64_t n_dip, const std::
     std::vector<double> cD, cP;
     std::vector<char> used;
     dsm_coefficients(g, n_dip, domain, cD, cP, used);
-    double* d_c = (double*)scratch(SCR_SRCCOEF, sizeof(double) * std::max<size_t>(1, 2 * cD.size()));
+    double* d_c = (double*)scratch(SCR_SRCCOEF, sizeof(double) * std::max<size_t>(1, 2 * cD.size()), st);
     if (!cD.empty()) {
         OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
         OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
@@ -753,7 +753,7 @@ void dip_source_adjoint(Geometry& g, const double* d_dip, int64_t n_dip, const s
     std::vector<double> cD, cP;
     std::vector<char> used;
     dsm_coefficients(g, n_dip, domain, cD, cP, used);
-    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size());
+    double* d_c = (double*)scratch(SCR_SMALL, sizeof(double) * 2 * cD.size(), st);
     OMH_HIP(hipMemcpyAsync(d_c, cD.data(), sizeof(double) * cD.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipMemcpyAsync(d_c + cD.size(), cP.data(), sizeof(double) * cP.size(), hipMemcpyHostToDevice, st));
     OMH_HIP(hipStreamSynchronize(st));
@@ -786,7 +786,7 @@ static Closest closest_outer(Geometry& g, const double* d_pts, int64_t n, hipStr
     for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
     mt[nm] = dg.n_tris;
     const size_t cnt = (size_t)n * nm;
-    char* buf = (char*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + cnt * (sizeof(double) * 4 + sizeof(int64_t)));
+    char* buf = (char*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + cnt * (sizeof(double) * 4 + sizeof(int64_t)), st);
     int64_t* d_mt = (int64_t*)buf;
     double* d_dist = (double*)(d_mt + nm + 1);
     int64_t* d_tri = (int64_t*)(d_dist + cnt);
@@ -931,7 +931,7 @@ std::vector<int64_t> surf2vol(Geometry& g, const double* d_pts, int64_t npts, do
             }
     }
     DeviceGeometry& dg = device_geometry(g);
-    char* buf = (char*)scratch(SCR_SRCCOEF, sizeof(int64_t) * nk + 2 * sizeof(double) * cD.size());
+    char* buf = (char*)scratch(SCR_SRCCOEF, sizeof(int64_t) * nk + 2 * sizeof(double) * cD.size(), st);
     int64_t* d_kept = (int64_t*)buf;
     double* d_cD = (double*)(buf + sizeof(int64_t) * nk);
     double* d_cS = d_cD + cD.size();
@@ -943,7 +943,7 @@ std::vector<int64_t> surf2vol(Geometry& g, const double* d_pts, int64_t npts, do
     int64_t ntmax = 1;
     for (int m = 0; m < nm; ++m) ntmax = std::max<int64_t>(ntmax, dg.meshes[m].tri_end - dg.meshes[m].tri_begin);
     const int64_t pc_max = std::max<int64_t>(256, std::min<int64_t>(nk, ((int64_t(1) << 30) / (24 * ntmax)) / 256 * 256));
-    double* W = (double*)scratch(SCR_DTMP, sizeof(double) * 3 * (size_t)ntmax * (size_t)std::min<int64_t>(pc_max, nk));
+    double* W = (double*)scratch(SCR_DTMP, sizeof(double) * 3 * (size_t)ntmax * (size_t)std::min<int64_t>(pc_max, nk), st);
     for (int m = 0; m < nm; ++m) {
         if (!used[m]) continue;
         const MeshDev& md = dg.meshes[m];
@@ -985,7 +985,7 @@ std::vector<int64_t> dip2internal_pot(Geometry& g, const double* d_dip, int64_t 
         coef[j] = c != 0.0 ? kK / c : 0.0;
     }
     char* buf = (char*)scratch(SCR_SRCCOEF, sizeof(int64_t) * nk + sizeof(double) * n_dip +
-                                              sizeof(int32_t) * (nk + n_dip));
+                                              sizeof(int32_t) * (nk + n_dip), st);
     int64_t* d_kept = (int64_t*)buf;
     double* d_coef = (double*)(buf + sizeof(int64_t) * nk);
     int32_t* d_pdom = (int32_t*)(d_coef + n_dip);

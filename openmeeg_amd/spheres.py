@@ -218,8 +218,8 @@ def make_batched_heads(n_heads=32, seed=4, freqs=(8, 10, 12, 14, 16)):
         n_el = int(rng.integers(64, 257))
         # dipoles well inside the perturbed cortex (min radius >= 0.87*0.97)
         dip = random_dipoles(rng, n_dip, 0.78)
-        # electrodes projected radially on the perturbed scalp
-        P, T = meshes[-1]
+        # electrodes on the unit sphere; Head2EEGMat projects each onto the
+        # closest point of the perturbed scalp
         e = sphere_electrodes(rng, n_el, 1.0)
         heads.append((meshes, (1.0, 0.0125, 1.0), dip, e))
     return heads

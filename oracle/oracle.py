@@ -320,7 +320,7 @@ def read_sensors(path):
             if not tok or tok[0].startswith("#"):
                 continue
             rows.append(tok)
-    labeled = rows[0][0].count(".") != 1
+    labeled = not any(r[0].count(".") == 1 for r in rows)   # any numeric first token (sensors.cpp:67-69)
     data = [r[1:] if labeled else r for r in rows]
     return np.array(data, dtype=np.float64)[:, :3]
 
@@ -343,7 +343,7 @@ def read_squids(path):
             tok = ln.split()
             if tok and not tok[0].startswith("#"):
                 rows.append(tok)
-    labeled = rows[0][0].count(".") != 1
+    labeled = not any(r[0].count(".") == 1 for r in rows)   # any numeric first token (sensors.cpp:67-69)
     names = [r[0] for r in rows] if labeled else [str(i) for i in range(len(rows))]
     mat = np.array([r[1:] if labeled else r for r in rows], dtype=np.float64)
     order = {}
@@ -670,32 +670,12 @@ def gain_internal_pot(H, dsm, h2ip, ds2ip):
 # ---------------------------------------------------------------------------
 # CPU baseline (bench.py's cpu_baseline leg)
 # ---------------------------------------------------------------------------
-def time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=6, n_dip_sample=16, n_factor=4000):
-    """Times the oracle on a bounded sample of a gain workload and scales
-    each stage to the full problem:
-      * HeadMat: S+D evaluations of `rows_per_pair` rows of every
-        communicating mesh pair (per-evaluation time x the pair count of
-        every block, S on the upper triangle of diagonal blocks);
-      * DipSourceMat: `n_dip_sample` dipoles (x n_dip / sample);
-      * factor + solve: LAPACK dsytrf/dsytrs on an n_factor symmetric
-        matrix (x (n / n_factor)^3).
-    Returns (extrapolated seconds, sample wall seconds, description)."""
-    import time
-    import scipy.linalg.lapack as la
-    g = nested_geometry(meshes, cond)
-    t = tables(g)
-    L = lib()
-    sink = C.c_double()
-    wall0 = time.time()
-    # per-pair cost of S and D evaluations, sampled on the innermost mesh pair
-    costs = {}
-    for mode in (1, 2):
-        t0 = time.time()
-        pairs = L.orc_sample_rows(C.byref(t.s), 0, 0, 3, 0, rows_per_pair, mode, C.byref(sink))
-        costs[mode] = (time.time() - t0) / pairs
-    # pair counts of the whole HeadMat (assembleHeadMat.cpp:83-108 with the
-    # block rules of operators.h:180-197,363-384; N and deflation are O(V^2)
-    # gathers and not counted)
+
+def pair_counts(g):
+    """(S pairs, D pairs) of the HeadMat of geometry g: every communicating
+    mesh pair's S block (upper triangle of a diagonal block) and its D / D*
+    blocks (none for a current-barrier row mesh); assembleHeadMat.cpp:83-108
+    with the block rules of operators.h:180-197,363-384."""
     n_s = n_d = 0.0
     for (m1, m2, o, cS, cN, cD) in g.pairs:
         n1, n2 = len(g.meshes[m1].tris), len(g.meshes[m2].tris)
@@ -705,21 +685,60 @@ def time_cpu_baseline(meshes, cond, dip, el, rows_per_pair=6, n_dip_sample=16, n
             n_d += n1 * n2
         if m1 != m2 and not b2:
             n_d += n1 * n2
-    t_asm = n_s * costs[1] + n_d * costs[2]
-    ns = min(n_dip_sample, len(dip))
+    return n_s, n_d
+
+
+def time_pipeline(meshes, cond, dip, el, domain_name="Brain"):
+    """Runs the oracle's whole EEG gain pipeline and times each stage (wall
+    clock, all threads the process has): HeadMat (C, OpenMP), DipSourceMat
+    (adaptive Integrator(3,10,0.001)), Head2EEGMat, the factorisation of the
+    full symmetric matrix (LAPACK dsytrf, Bunch-Kaufman), the solve with the
+    electrode rows as right-hand sides (dsytrs; GainEEGadjoint's linsolve,
+    gain.h:78-89) and the gain product.  Returns (gain, stage seconds, g)."""
+    import time
+    import scipy.linalg.lapack as la
+    st = {}
     t0 = time.time()
-    dip_source_mat(g, dip[:ns], "Brain")
-    t_dsm = (time.time() - t0) * len(dip) / ns
-    rng = np.random.default_rng(0)
-    M = rng.standard_normal((n_factor, n_factor))
-    M = M + M.T + n_factor * np.eye(n_factor)
-    B = rng.standard_normal((n_factor, len(el)))
+    g = nested_geometry(meshes, cond)
+    Hp = head_mat(g)
+    st["headmat"] = time.time() - t0
+    H = unpack(Hp, g.size)
+    del Hp
     t0 = time.time()
-    lu, piv, info = la.dsytrf(M)
-    la.dsytrs(lu, piv, B)
-    t_fac = (time.time() - t0) * (g.size / n_factor) ** 3
-    wall = time.time() - wall0
-    desc = ("oracle on a sample of the same workload: S and D pairs of %d rows of the innermost mesh, %d of %d dipoles, "
-            "LAPACK dsytrf+dsytrs at n=%d; each stage scaled to n=%d (assembly by pair count, DSM by dipole "
-            "count, factor by n^3)" % (rows_per_pair, ns, len(dip), n_factor, g.size))
-    return t_asm + t_dsm + t_fac, wall, desc, dict(assembly=t_asm, dsm=t_dsm, factor_solve=t_fac)
+    D = dip_source_mat(g, dip, domain_name)
+    st["dsm"] = time.time() - t0
+    t0 = time.time()
+    cols, vals = head2eeg(g, el)
+    E = h2em_dense(g, cols, vals)
+    st["h2em"] = time.time() - t0
+    t0 = time.time()
+    lu, piv, info = la.dsytrf(H, lower=0, overwrite_a=1)
+    st["factor"] = time.time() - t0
+    if info != 0:
+        raise RuntimeError("dsytrf info %d" % info)
+    t0 = time.time()
+    Y, info = la.dsytrs(lu, piv, np.asfortranarray(E.T), lower=0)
+    st["solve"] = time.time() - t0
+    t0 = time.time()
+    G = Y.T @ D
+    st["gain"] = time.time() - t0
+    return G, st, g
+
+
+def extrapolate_stages(st, g_from, nd_from, ne_from, g_to, nd_to, ne_to):
+    """Scales measured oracle stage times to a larger workload: HeadMat by
+    the S + D pair count, DipSourceMat by dipoles x innermost triangles,
+    Head2EEGMat by electrodes x scalp triangles, the factorisation by n^3,
+    the solve by n^2 x electrodes, the gain by n_e x n x n_dip.  Returns the
+    scaled stage times."""
+    sf, df = pair_counts(g_from)
+    stt, dt_ = pair_counts(g_to)
+    # the measured HeadMat time splits over S and D like the pair counts
+    asm = st["headmat"] * (stt + dt_) / (sf + df)
+    tin_f, tin_t = len(g_from.meshes[0].tris), len(g_to.meshes[0].tris)
+    tout_f, tout_t = len(g_from.meshes[-1].tris), len(g_to.meshes[-1].tris)
+    n_f, n_t = float(g_from.size), float(g_to.size)
+    return dict(headmat=asm, dsm=st["dsm"] * (nd_to * tin_t) / (nd_from * tin_f),
+                h2em=st["h2em"] * (ne_to * tout_t) / (ne_from * tout_f),
+                factor=st["factor"] * (n_t / n_f) ** 3, solve=st["solve"] * (n_t / n_f) ** 2 * ne_to / ne_from,
+                gain=st["gain"] * (ne_to * n_t * nd_to) / (ne_from * n_f * nd_from))

@@ -35,17 +35,32 @@ def head1_oracle():
     return g
 
 
+HEAD3 = (os.path.join(GOLDEN, "head3", "Head3.geom"), os.path.join(GOLDEN, "head3", "Head3.cond"))
+
+
+def c0_inputs():
+    """The 642-vertex three-sphere head of the reference's Head3 test
+    (data/Head3: skull, cortex, scalp .tri files, radii 0.87/0.92/1.0,
+    conductivities 1/0.0125/1) with 100 seeded dipoles uniform in r < 0.8
+    (unit random moments) and 64 seeded electrodes on r = 1."""
+    from openmeeg_amd import spheres as S
+    rng = np.random.default_rng(0)
+    dip = S.random_dipoles(rng, 100, 0.8)
+    el = S.sphere_electrodes(rng, 64, 1.0)
+    return dip, el
+
+
 @pytest.fixture(scope="session")
 def c0_oracle():
-    """Oracle results on the 642-vertex nested-sphere head (n = 4486)."""
+    """Oracle results on the Head3 head (n = 4486)."""
     from oracle import oracle as O
-    from openmeeg_amd import spheres as S
-    meshes, cond, dip, el = S.make_config("c0")
-    g = O.nested_geometry(meshes, cond)
+    dip, el = c0_inputs()
+    g = O.load_geometry(*HEAD3)
     Hp = O.head_mat(g)
     H = O.unpack(Hp, g.size)
     D = O.dip_source_mat(g, dip, "Brain")
     cols, vals = O.head2eeg(g, el)
     E = O.h2em_dense(g, cols, vals)
     G = O.gain_eeg(H, D, E)
-    return dict(meshes=meshes, cond=cond, dip=dip, el=el, g=g, H=H, D=D, cols=cols, vals=vals, E=E, G=G)
+    return dict(geom_files=HEAD3, cond=(1.0, 0.0125, 1.0), radii=(0.87, 0.92, 1.0), dip=dip, el=el, g=g, H=H, D=D,
+                cols=cols, vals=vals, E=E, G=G)

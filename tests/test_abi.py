@@ -114,3 +114,35 @@ def test_source_mesh_handle_on_the_host():
         om.Mesh(P, [[0, 1, 3]])
     with pytest.raises(ValueError, match="empty"):
         om.Mesh(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64))
+
+
+def test_refinement_stopping_rule():
+    """omh_refine_state (the stop of omh_mixed_solve*): 1 converged, 2
+    stalled, 0 continue.  Divergence and stagnation above the FP64 floor are
+    failures, not convergence; stagnation at the floor is convergence."""
+    from openmeeg_amd import _lib
+    st = _lib.lib().omh_refine_state
+    tol = 1e-8
+
+    def run(seq, tol=tol):
+        prev = 2.0
+        for it, rel in enumerate(seq):
+            s = st(it, rel, prev, tol)
+            if s:
+                return s, it
+            prev = rel
+        return 0, len(seq)
+
+    assert run([3e-5, 6e-8, 2.7e-10]) == (1, 1)              # contracting: 6e-8 x 2e-3 <= 1e-8
+    assert run([3e-5, 6e-6, 1e-6, 2e-7]) == (0, 4)           # estimates 1.2e-6, 1.7e-7, 4e-8: continue
+    assert run([3e-5, 6e-6, 1e-6, 2e-7, 1e-9]) == (1, 4)
+    assert run([1e-9]) == (1, 0)                             # the update itself below tol
+    assert run([1e-2, 5e-2, 1e-1, 1.0]) == (2, 2)            # rising: divergence is a failure
+    assert run([1e-3, 9e-4, 8.5e-4]) == (2, 2)               # stalled far above tol
+    assert run([1e-3, 9e-4, 8.5e-4], tol=1e-16) == (2, 2)
+    assert run([1e-10, 2e-12, 1.5e-12], tol=1e-14) == (1, 2)  # stalled at the FP64 floor
+    assert run([1e-10, 2e-12, 1.5e-12], tol=1e-30) == (2, 2)  # ... but not within 1e3 tol of it
+    assert run([3e-2, 1e-2, 3.3e-3, 1.1e-3]) == (0, 4)        # slow contraction continues (until max_iter)
+    assert st(2, float("nan"), 1e-3, tol) == 2
+    assert st(1, 5e-9, 1e-3, tol) == 1
+    assert st(1, 2e-3, 1e-3, tol) == 0                        # a rise on the second correction is allowed once

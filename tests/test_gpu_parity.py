@@ -133,7 +133,7 @@ def test_meg_nonnested_nna1(om, nna1):
 
 def test_eeg_meg_pipeline(om, c0_oracle):
     """The combined device pipeline (GainEEGMEGadjoint, gain.h:105-128: one
-    factor, one solve with the EEG and MEG rows) on the 642-vertex head with
+    factor, one solve with the EEG and MEG rows) on the Head3 head with
     80 tilted magnetometers: its EEG gain equals gain_eeg_pipeline's (1e-12),
     its MEG gain GainMEGadjoint's from the reference-API matrices (1e-10);
     mixed16 within 1e-6 of FP64.  Independent of the reference: radial
@@ -142,7 +142,7 @@ def test_eeg_meg_pipeline(om, c0_oracle):
     discretisation (here 5%)."""
     from openmeeg_amd import spheres as S
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     rng = np.random.default_rng(11)
     p, o, w = S.sphere_squids(rng, 80)
     sq = om.Sensors(p, orientations=o, weights=w)
@@ -461,7 +461,7 @@ def test_solvelin_head1(om, head1):
 
 def test_headmat_c0_matches_oracle(om, c0_oracle):
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     hm = om.HeadMat(g)
     from oracle import oracle as O
     assert rel_fro(hm.data(), O.pack(c["H"])) < 1e-13
@@ -469,16 +469,33 @@ def test_headmat_c0_matches_oracle(om, c0_oracle):
 
 def test_dsm_h2em_c0_match_oracle(om, c0_oracle):
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     assert rel_fro(om.DipSourceMat(g, c["dip"], "Brain").array(), c["D"]) < 1e-13
     h2em = om.Head2EEGMat(g, c["el"])
     assert np.array_equal(h2em.cols, c["cols"])
     assert np.abs(h2em.vals - c["vals"]).max() < 1e-14
 
 
+def test_inverse_c0_matches_oracle(om, c0_oracle):
+    """SymMatrix::inverse / invert (symmatrix.h:259-297, om_minverser) on
+    Head3 against the inverse of the oracle's HeadMat (LAPACK): 1e-10
+    relative Frobenius, in the reference's packed storage and order."""
+    from oracle import oracle as O
+    c = c0_oracle
+    g = om.read_geometry(*c["geom_files"])
+    hm = om.HeadMat(g)
+    inv = hm.inverse()
+    ref = np.linalg.inv(c["H"])
+    e = rel_fro(inv.data(), O.pack(ref))
+    hm.invert()
+    e_inplace = rel_fro(hm.data(), O.pack(ref))
+    print("Head3 inverse vs oracle %.3e (in place %.3e)" % (e, e_inplace), flush=True)
+    assert e < 1e-10 and e_inplace < 1e-10
+
+
 def test_gain_c0_fp64_and_mixed(om, c0_oracle):
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     G64, info64 = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="fp64", domain_name="Brain")
     G64 = om.gain_to_numpy(G64, len(c["el"]), len(c["dip"]))
     assert rel_fro(G64, c["G"]) < 1e-10
@@ -494,12 +511,12 @@ def test_gain_c0_fp64_and_mixed(om, c0_oracle):
 def test_headmat_far_field_close_to_analytic(om, c0_oracle):
     """The mixed path's far-field assembly (Gauss quadrature on both
     triangles of pairs at least 4 diameters apart) against the reference's
-    analytic integrals on the 642-vertex head: entries agree to 1e-10 of the
+    analytic integrals on the Head3 head: entries agree to 1e-10 of the
     matrix scale."""
     import torch
     from openmeeg_amd.api import _head_mat_device
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     n = g.size
     Ha = _head_mat_device(g).view(n, n)
     Hf = _head_mat_device(g, far_field=True).view(n, n)
@@ -543,15 +560,15 @@ def test_far_field_fused_pass(om, monkeypatch, layers):
 
 
 def test_gain_c0_vs_analytic_sphere(om, c0_oracle):
-    """Independent of the reference: the BEM gain on the 642-vertex spheres
+    """Independent of the reference: the BEM gain on the Head3 spheres
     against Zhang's series (RDM/MAG, tests/analytic tolerances for Head3:
     0.03)."""
     from oracle import analytic as A
     c = c0_oracle
-    g = om.make_nested_geometry(c["meshes"], c["cond"])
+    g = om.read_geometry(*c["geom_files"])
     G, _ = om.gain_eeg_pipeline(g, c["dip"], c["el"], precision="fp64", domain_name="Brain")
     G = om.gain_to_numpy(G, len(c["el"]), len(c["dip"]))
-    ref = A.sphere_gain(c["el"], c["dip"], c["cond"], (0.87, 0.92, 1.0))
+    ref = A.sphere_gain(c["el"], c["dip"], c["cond"], c["radii"])
     r = np.linalg.norm(c["dip"][:, :3], axis=1)
     shallow = np.where(r < 0.7)[0]
     assert np.median([A.rdm(G[:, j], ref[:, j]) for j in shallow]) < 0.03

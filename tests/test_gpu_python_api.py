@@ -1,11 +1,16 @@
-"""The flow of the reference's own Python test (wrapping/python/openmeeg/
-tests/test_python.py:22-190) on Head1 against this module: geometry from the
-files or from numpy meshes, HeadMat / inverse / invert, DipSourceMat,
-Head2EEGMat, GainEEG / GainEEGadjoint, SurfSourceMat with a Mesh and the
-surface gains, the MEG matrices and gains (dipoles as an array or a Matrix,
-a transposed array refused with "6 columns"), GainEEGMEGadjoint, Forward,
-and saving / loading the matrices -- with the reference's shape checks, and
-values against the Head1 goldens.  Needs a HIP GPU."""
+"""The Python surface as a contract: the names the reference module exports
+(wrapping/python/openmeeg/__init__.py:15-39) exist here, and each product
+they build has the shape its arguments fix and, on Head1, the values of the
+reference's goldens.
+
+The shapes are written once as a table over five sizes of a problem: the
+HeadMat unknowns N (nb_parameters - nb_current_barrier_triangles,
+assembleHeadMat.cpp:77), the EEG electrodes E, the MEG sensors M, the
+dipoles D and the vertices V of a source mesh.  The products are built from
+two geometries (the .geom/.cond files and numpy meshes through
+make_nested_geometry) and must agree with each other, with the goldens and
+with the table.  The name check runs on the CPU; the rest needs a HIP GPU.
+"""
 import os
 
 import numpy as np
@@ -13,9 +18,50 @@ import pytest
 
 from conftest import GOLDEN, rel_fro
 
-pytestmark = pytest.mark.gpu
-
 H1 = os.path.join(GOLDEN, "head1")
+
+# The reference's exported names that belong to the paths built here.
+EXPORTED = ("HeadMat", "Sensors", "Integrator", "Head2EEGMat", "Head2MEGMat", "DipSourceMat", "DipSource2MEGMat",
+            "DipSource2InternalPotMat", "EITSourceMat", "GainEEG", "GainMEG", "GainEEGadjoint", "GainMEGadjoint",
+            "GainEEGMEGadjoint", "GainInternalPot", "GainEITInternalPot", "Forward", "Surf2VolMat", "SurfSourceMat",
+            "SurfSource2MEGMat", "Matrix", "SymMatrix", "make_geometry", "make_nested_geometry", "read_geometry",
+            "get_log_level", "set_log_level", "use_log_level")
+# Exported by the reference but outside these paths: monopole sources,
+# ECoG projection and the sensor-array generator of _make_sensors.py.
+NOT_BUILT = ("MonopoleSourceMat", "Head2ECoGMat", "make_sensors")
+
+# product -> (rows, columns) in the sizes N, E, M, D, V
+SHAPES = {
+    "HeadMat": ("N", "N"),
+    "HeadMat.inverse": ("N", "N"),
+    "DipSourceMat": ("N", "D"),
+    "SurfSourceMat": ("N", "V"),
+    "Head2EEGMat": ("E", "N"),
+    "Head2MEGMat": ("M", "N"),
+    "DipSource2MEGMat": ("M", "D"),
+    "SurfSource2MEGMat": ("M", "V"),
+    "GainEEG(dipoles)": ("E", "D"),
+    "GainEEG(surface)": ("E", "V"),
+    "GainEEGadjoint": ("E", "D"),
+    "GainMEG(dipoles)": ("M", "D"),
+    "GainMEG(surface)": ("M", "V"),
+    "GainMEGadjoint": ("M", "D"),
+}
+
+# product -> golden file of the reference's Head1 regression data
+GOLDENS = {
+    "DipSourceMat": "dsm", "SurfSourceMat": "ssm", "Head2MEGMat": "h2mm", "DipSource2MEGMat": "ds2mm",
+    "SurfSource2MEGMat": "ss2mm", "GainEEG(dipoles)": "dgem", "GainEEG(surface)": "sgem",
+    "GainEEGadjoint": "dgem", "GainMEG(dipoles)": "dgmm", "GainMEG(surface)": "sgmm", "GainMEGadjoint": "dgmm",
+}
+
+
+def test_reference_names_are_exported():
+    import openmeeg_amd as om
+    missing = [n for n in EXPORTED if not hasattr(om, n)]
+    assert not missing, missing
+    assert not any(hasattr(om, n) for n in NOT_BUILT)
+    assert isinstance(om.get_log_level(), str)
 
 
 @pytest.fixture(scope="module")
@@ -27,105 +73,116 @@ def om():
     return openmeeg_amd
 
 
-@pytest.mark.parametrize("load_from_numpy", [True, False])
-def test_reference_python_flow_head1(om, load_from_numpy, tmp_path):
+def _geometry(om, source):
+    if source == "files":
+        return om.read_geometry(os.path.join(H1, "Head1.geom"), os.path.join(H1, "Head1.cond"))
     from oracle.geometry import read_tri
-    geom_file, cond_file = os.path.join(H1, "Head1.geom"), os.path.join(H1, "Head1.cond")
-    dipole_file, patches_file = os.path.join(H1, "Head1.dip"), os.path.join(H1, "Head1.patches")
-    print("reference Python flow, load_from_numpy=%s" % load_from_numpy, flush=True)
-    if load_from_numpy:
-        meshes = [read_tri(os.path.join(H1, f)) for f in ("cortex.1.tri", "skull.1.tri", "scalp.1.tri")]
-        geom = om.make_nested_geometry(meshes, conductivity=(1, 0.0125, 1))
-        dipoles = om.Matrix(np.asfortranarray(np.loadtxt(dipole_file)))
-        patches = om.Sensors(om.Matrix(np.asfortranarray(np.loadtxt(patches_file, usecols=range(1, 4)))), geom)
-    else:
-        geom = om.read_geometry(geom_file, cond_file)
-        dipoles = om.Matrix()
-        dipoles.load(dipole_file)
-        patches = om.Sensors()
-        patches.load(patches_file)
-    assert geom.is_nested()
-    n_dipoles = dipoles.nlin()
-    sensors = om.Sensors()
-    sensors.load(os.path.join(H1, "Head1.squids"))
-    n_meg_sensors = sensors.getNumberOfSensors()
-    n_eeg_sensors = patches.getNumberOfSensors()
+    layers = [read_tri(os.path.join(H1, name)) for name in ("cortex.1.tri", "skull.1.tri", "scalp.1.tri")]
+    return om.make_nested_geometry(layers, conductivity=(1, 0.0125, 1))
 
-    integrator = om.Integrator(3, 0, 0.005)
-    with om.use_log_level("debug"):
-        hm = om.HeadMat(geom, integrator)
-    hminv = hm.inverse()
-    with om.use_log_level("debug"):
-        hminv_inplace = om.HeadMat(geom, integrator)
-    hminv_inplace.invert()
-    np.testing.assert_allclose(om.Matrix(hminv).array(), om.Matrix(hminv_inplace).array())
 
-    dsm = om.DipSourceMat(geom, dipoles, "Brain")
-    h2em = om.Head2EEGMat(geom, patches)
-    gain_eeg_dip = om.GainEEG(hminv, dsm, h2em)
-    gain_adjoint_eeg_dip = om.GainEEGadjoint(geom, dipoles, hm, h2em)
-    n_hm_unknowns = geom.nb_parameters() - geom.nb_current_barrier_triangles()
-    assert hm.nlin() == hm.ncol() == n_hm_unknowns
-    assert hminv.nlin() == hminv.ncol() == n_hm_unknowns
-    assert (dsm.nlin(), dsm.ncol()) == (n_hm_unknowns, n_dipoles)
-    assert (gain_eeg_dip.nlin(), gain_eeg_dip.ncol()) == (n_eeg_sensors, n_dipoles)
-    assert (gain_adjoint_eeg_dip.nlin(), gain_adjoint_eeg_dip.ncol()) == (n_eeg_sensors, n_dipoles)
+def _products(om, geom):
+    """Every product of the table on Head1, built through the exported names."""
+    integ = om.Integrator(3, 0, 0.005)
+    dip = np.loadtxt(os.path.join(H1, "Head1.dip"))
+    eeg = om.Sensors()
+    eeg.load(os.path.join(H1, "Head1.patches"))
+    meg = om.Sensors()
+    meg.load(os.path.join(H1, "Head1.squids"))
+    src = om.Mesh(os.path.join(H1, "Head1.tri"))
+    hm = om.HeadMat(geom, integ)
+    inv = hm.inverse()
+    out = {"HeadMat": hm, "HeadMat.inverse": inv}
+    out["DipSourceMat"] = om.DipSourceMat(geom, om.Matrix(np.asfortranarray(dip)), "Brain")
+    out["SurfSourceMat"] = om.SurfSourceMat(geom, src, integ)
+    out["Head2EEGMat"] = om.Head2EEGMat(geom, eeg)
+    out["Head2MEGMat"] = om.Head2MEGMat(geom, meg)
+    out["DipSource2MEGMat"] = om.DipSource2MEGMat(dip, meg)
+    out["SurfSource2MEGMat"] = om.SurfSource2MEGMat(src, meg)
+    out["GainEEG(dipoles)"] = om.GainEEG(inv, out["DipSourceMat"], out["Head2EEGMat"])
+    out["GainEEG(surface)"] = om.GainEEG(inv, out["SurfSourceMat"], out["Head2EEGMat"])
+    out["GainEEGadjoint"] = om.GainEEGadjoint(geom, dip, hm, out["Head2EEGMat"])
+    out["GainMEG(dipoles)"] = om.GainMEG(inv, out["DipSourceMat"], out["Head2MEGMat"], out["DipSource2MEGMat"])
+    out["GainMEG(surface)"] = om.GainMEG(inv, out["SurfSourceMat"], out["Head2MEGMat"], out["SurfSource2MEGMat"])
+    out["GainMEGadjoint"] = om.GainMEGadjoint(geom, dip, hm, out["Head2MEGMat"], out["DipSource2MEGMat"])
+    sizes = dict(N=geom.nb_parameters() - geom.nb_current_barrier_triangles(), E=eeg.getNumberOfSensors(),
+                 M=meg.getNumberOfSensors(), D=len(dip), V=src.nb_vertices())
+    return out, sizes, dict(dip=dip, eeg=eeg, meg=meg)
 
-    mesh = om.Mesh(os.path.join(H1, "Head1.tri"))
-    n_dipoles_surf = mesh.vertices().size()
-    ssm = om.SurfSourceMat(geom, mesh, integrator)
-    gain_eeg_surf = om.GainEEG(hminv, ssm, h2em)
-    assert (ssm.nlin(), ssm.ncol()) == (n_hm_unknowns, n_dipoles_surf)
-    assert (gain_eeg_surf.nlin(), gain_eeg_surf.ncol()) == (n_eeg_sensors, n_dipoles_surf)
 
-    ss2mm = om.SurfSource2MEGMat(mesh, sensors)
-    ds2mm = om.DipSource2MEGMat(dipoles, sensors)
-    dip_arr = np.zeros((4, 6))
-    dip_arr[:, 2] = [0.02, 0.03, 0.04, 0.05]
-    dip_arr[:, 3] = 1.0
-    dip_arr = np.asfortranarray(dip_arr)
-    np.testing.assert_allclose(om.DipSource2MEGMat(dip_arr, sensors).array(),
-                               om.DipSource2MEGMat(om.Matrix(dip_arr), sensors).array())
-    with pytest.raises(Exception, match="6 columns"):
-        om.DipSource2MEGMat(np.asfortranarray(dip_arr.T), sensors)
-    h2mm = om.Head2MEGMat(geom, sensors)
-    gain_meg_surf = om.GainMEG(hminv, ssm, h2mm, ss2mm)
-    gain_meg_dip = om.GainMEG(hminv, dsm, h2mm, ds2mm)
-    gain_adjoint_meg_dip = om.GainMEGadjoint(geom, dipoles, hm, h2mm, ds2mm)
-    gain_adjoint_eeg_meg_dip = om.GainEEGMEGadjoint(geom, dipoles, hm, h2em, h2mm, ds2mm)
-    assert gain_adjoint_eeg_meg_dip.nlin() == n_meg_sensors + n_eeg_sensors
-    assert (ss2mm.nlin(), ss2mm.ncol()) == (n_meg_sensors, n_dipoles_surf)
-    assert (ds2mm.nlin(), ds2mm.ncol()) == (n_meg_sensors, n_dipoles)
-    assert (h2mm.nlin(), h2mm.ncol()) == (n_meg_sensors, n_hm_unknowns)
-    assert (h2em.nlin(), h2em.ncol()) == (n_eeg_sensors, n_hm_unknowns)
-    assert (gain_meg_dip.nlin(), gain_meg_dip.ncol()) == (n_meg_sensors, n_dipoles)
-    assert (gain_meg_surf.nlin(), gain_meg_surf.ncol()) == (n_meg_sensors, n_dipoles_surf)
-    assert (gain_adjoint_meg_dip.nlin(), gain_adjoint_meg_dip.ncol()) == (n_meg_sensors, n_dipoles)
-    gain_meg_surf_one_line = om.GainMEG(om.HeadMat(geom, integrator).inverse(), om.SurfSourceMat(geom, mesh, integrator),
-                                        om.Head2MEGMat(geom, sensors), om.SurfSource2MEGMat(mesh, sensors))
-    np.testing.assert_allclose(gain_meg_surf_one_line.array(), gain_meg_surf.array())
+@pytest.fixture(scope="module")
+def built(om):
+    print("Head1 products from the .geom files and from numpy meshes", flush=True)
+    return {s: _products(om, _geometry(om, s)) for s in ("files", "arrays")}
 
-    sources = om.Matrix(os.path.join(H1, "Head1.srcdip"))
-    n_times = sources.ncol()
-    for G, ns in ((gain_meg_dip, n_meg_sensors), (gain_adjoint_meg_dip, n_meg_sensors),
-                  (gain_eeg_dip, n_eeg_sensors), (gain_adjoint_eeg_dip, n_eeg_sensors)):
-        est = om.Forward(G, sources, 0.0)
-        assert (est.nlin(), est.ncol()) == (ns, n_times)
 
-    hm_fname, ssm_fname = str(tmp_path / "Head1.hm"), str(tmp_path / "Head1.ssm")
-    hm.save(hm_fname)
-    ssm.save(ssm_fname)
-    m1 = om.SymMatrix()
-    m1.load(hm_fname)
-    m2 = om.Matrix()
-    m2.load(ssm_fname)
-    assert np.array_equal(m1.data(), hm.data()) and np.array_equal(m2.array(), ssm.array())
+@pytest.mark.gpu
+def test_shapes_follow_the_table(built):
+    for source, (out, sizes, _) in built.items():
+        assert set(out) == set(SHAPES)
+        for name, (r, c) in SHAPES.items():
+            m = out[name]
+            assert (m.nlin(), m.ncol()) == (sizes[r], sizes[c]), (source, name)
+    n = built["files"][1]
+    assert n["N"] == 286 and n["E"] == 32 and n["D"] == 5
 
-    gold = lambda n: np.load(os.path.join(H1, n + ".npy"))
-    errs = dict(hm=rel_fro(hm.data(), gold("hm")), dgem=rel_fro(gain_eeg_dip.array(), gold("dgem")),
-                dgem_adj=rel_fro(gain_adjoint_eeg_dip.array(), gold("dgem")),
-                dgmm=rel_fro(gain_meg_dip.array(), gold("dgmm")), sgem=rel_fro(gain_eeg_surf.array(), gold("sgem")),
-                sgmm=rel_fro(gain_meg_surf.array(), gold("sgmm")),
-                eegmeg=rel_fro(gain_adjoint_eeg_meg_dip.MEGleadfield.array(), gold("dgmm")))
-    print("  vs goldens:", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
-    assert errs["hm"] < 1e-13 and max(v for k, v in errs.items() if k != "hm") < 1e-10
+
+@pytest.mark.gpu
+def test_values_match_goldens_and_both_geometries_agree(built):
+    files, arrays = built["files"][0], built["arrays"][0]
+    errs = {}
+    for name, gold in GOLDENS.items():
+        ref = np.load(os.path.join(H1, gold + ".npy"))
+        errs[name] = rel_fro(files[name].array(), ref)
+        assert rel_fro(arrays[name].array(), files[name].array()) < 1e-12, name
+    errs["HeadMat"] = rel_fro(files["HeadMat"].data(), np.load(os.path.join(H1, "hm.npy")))
+    errs["HeadMat.inverse"] = rel_fro(files["HeadMat.inverse"].data(), np.load(os.path.join(H1, "hm_inv.npy")))
+    print("Head1 vs goldens:", {k: "%.1e" % v for k, v in errs.items()}, flush=True)
+    for name, e in errs.items():
+        assert e < (1e-13 if name in ("HeadMat", "DipSourceMat", "SurfSourceMat", "DipSource2MEGMat") else 1e-10), name
+
+
+@pytest.mark.gpu
+def test_combined_lead_fields_and_forward(om, built):
+    """GainEEGMEGadjoint stacks the EEG and MEG lead fields of one solve
+    (gain.h:105-128); Forward maps (sensors x sources) by (sources x times)
+    to (sensors x times)."""
+    out, sizes, inp = built["files"]
+    geom = _geometry(om, "files")
+    both = om.GainEEGMEGadjoint(geom, inp["dip"], out["HeadMat"], out["Head2EEGMat"], out["Head2MEGMat"],
+                                out["DipSource2MEGMat"])
+    assert both.nlin() == sizes["E"] + sizes["M"]
+    assert rel_fro(both.EEGleadfield.array(), out["GainEEGadjoint"].array()) < 1e-12
+    assert rel_fro(both.MEGleadfield.array(), out["GainMEGadjoint"].array()) < 1e-12
+    act = om.Matrix(os.path.join(H1, "Head1.srcdip"))
+    for name in ("GainEEG(dipoles)", "GainMEGadjoint"):
+        y = om.Forward(out[name], act, 0.0)
+        assert (y.nlin(), y.ncol()) == (out[name].nlin(), act.ncol())
+        np.testing.assert_allclose(y.array(), out[name].array() @ act.array(), rtol=1e-12, atol=0)
+
+
+@pytest.mark.gpu
+def test_argument_contract(om, built, tmp_path):
+    """Dipoles are an n x 6 array or Matrix (both give the same source
+    matrix); another column count is a ValueError; HeadMat.invert() equals
+    inverse(); the symmetric and full matrices survive save/load."""
+    out, _, inp = built["files"]
+    geom = _geometry(om, "files")
+    as_array = om.DipSourceMat(geom, inp["dip"], "Brain").array()
+    assert np.array_equal(as_array, out["DipSourceMat"].array())
+    with pytest.raises(ValueError, match="6 columns"):
+        om.DipSourceMat(geom, inp["dip"][:, :5], "Brain")
+    with pytest.raises(ValueError, match="6 columns"):
+        om.DipSource2MEGMat(np.asfortranarray(inp["dip"][:4].T), inp["meg"])
+    inv2 = om.HeadMat(geom)
+    inv2.invert()
+    assert rel_fro(inv2.data(), out["HeadMat.inverse"].data()) < 1e-14
+    for suffix in (".mat", ".bin"):
+        hm_path, ssm_path = str(tmp_path / ("hm" + suffix)), str(tmp_path / ("ssm" + suffix))
+        out["HeadMat"].save(hm_path)
+        out["SurfSourceMat"].save(ssm_path)
+        back_h, back_s = om.SymMatrix(), om.Matrix()
+        back_h.load(hm_path)
+        back_s.load(ssm_path)
+        assert np.array_equal(back_h.data(), out["HeadMat"].data())
+        assert np.array_equal(back_s.array(), out["SurfSourceMat"].array())
