@@ -272,6 +272,30 @@ int omh_sym_factor_f32_blocks(float* d_A, int64_t n, int64_t ld, void* d_work, i
 int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg,
                               const int64_t* seg_start, const uint8_t* hfill, int32_t* info, void* stream);
 
+/* Pivot signs of a finished omh_sym_factor_* factorisation (workspace
+ * d_work, element size 4 or 8) against the pattern +1 for the first
+ * n_positive unknowns and -1 for the rest (the triangle, then the vertex
+ * unknowns of a HeadMat in OMH_ORDER_TRIANGLES_FIRST): *first = the first
+ * unknown whose sign differs, or -1.  In that pattern the matrix is
+ * quasi-definite and the pivot-free factor is stable; a mismatch means the
+ * order needs pivoting (omh_lu_factor_f64).  No reference counterpart. */
+int omh_factor_sign_mismatch(const void* d_work, int64_t n, int32_t elem_size, int64_t n_positive, int64_t* first,
+                             void* stream);
+
+/* Pivoted fallback of SymMatrix::invert/inverse/solveLin (DSPTRF/DSPTRS,
+ * symmatrix.cpp:133-152, symmatrix.h:259-297): FP64 LU with partial row
+ * pivoting, P A = L U in place (any unknown order; for matrices on which the
+ * pivot-free factorisation breaks down or shows a sign mismatch).  d_ipiv:
+ * n device int32 (row j exchanged with row d_ipiv[j] at step j, 0-based);
+ * *info = 0, or 1 + the first column with an exactly zero pivot.
+ * omh_lu_solve_f64: d_B <- A^-1 d_B for nrhs columns.  Workspace bytes from
+ * omh_lu_workspace_bytes (-1 for n <= 0). */
+int64_t omh_lu_workspace_bytes(int64_t n);
+int omh_lu_factor_f64(double* d_A, int64_t n, int64_t ld, int32_t* d_ipiv, void* d_work, int32_t* info,
+                      void* stream);
+int omh_lu_solve_f64(const double* d_LU, int64_t n, int64_t ld, const int32_t* d_ipiv, const void* d_work,
+                     double* d_B, int64_t ldb, int64_t nrhs, void* stream);
+
 /* d_B <- H^-1 d_B for nrhs columns with a stored factor (DSPTRS,
  * symmatrix.cpp:133-152; linsolve gain.h:52-57). */
 int omh_sym_solve_f64(const double* d_L, int64_t n, int64_t ld, const void* d_work, double* d_B, int64_t ldb,

@@ -467,7 +467,9 @@ class SymMatrix:
     def _factor(self, precision="fp64"):
         if self.nv is None:
             raise ValueError("SymMatrix.invert/solveLin need the unknown layout of a HeadMat (nv)")
-        return factor(self.device().clone(), self._n, precision)
+        n = self._n
+        return factor(self.device().clone(), n, precision, n_positive=n - self.nv,
+                      rebuild=lambda: self.device().clone())
 
     def inverse(self):
         """SymMatrix::inverse (symmatrix.h:259-278)."""
@@ -519,34 +521,46 @@ def from_device_rows(D, n, nv, c):
 
 
 class Factor:
-    """A stored pivot-free signed Cholesky factor on the device.
+    """A stored factor of a device HeadMat: the pivot-free signed Cholesky
+    (precision 'fp64', or 'mixed' for an FP32 factor refined against the FP64
+    matrix) or, after a breakdown of it, the FP64 LU with partial pivoting
+    ('lu'; `pivoted` is True).
 
     A mixed-precision factor whose refinement does not converge (the FP32
     factor too coarse for the matrix) is replaced by the FP64 factor of the
     FP64 matrix when fallback_fp64 is set (then `fell_back` is True);
     otherwise solve() raises NotConverged."""
 
-    def __init__(self, L, n, work, precision, H64=None, structure=None, fallback_fp64=True):
+    def __init__(self, L, n, work, precision, H64=None, structure=None, fallback_fp64=True, ipiv=None,
+                 n_positive=None):
         self.L, self.n, self.work, self.precision, self.H64 = L, n, work, precision, H64
+        self.ipiv = ipiv
         self.iterations = 0
         self.rel_update = 0.0
         self.fell_back = False
+        self.pivoted = precision == "lu"
         self.fallback_fp64 = fallback_fp64
+        self.n_positive = n_positive
         self._raw_structure = structure
         self._structure = _block_args(structure)
 
     def _refactor_fp64(self):
         """Factor the FP64 matrix in place (it is no longer needed for
-        residuals) and drop the FP32 factor."""
-        F = factor(self.H64, self.n, "fp64", structure=self._raw_structure)
-        self.L, self.work, self.precision, self.H64 = F.L, F.work, "fp64", None
+        residuals) and drop the FP32 factor; the LU with partial pivoting
+        when the pivot-free factorisation breaks down on it."""
+        H = self.H64
+        backup = H.clone() if self.n_positive is not None else None
+        F = factor(H, self.n, "fp64", structure=self._raw_structure, n_positive=self.n_positive,
+                   rebuild=(lambda: backup) if backup is not None else None)
+        self.L, self.work, self.precision, self.H64, self.ipiv = F.L, F.work, F.precision, None, F.ipiv
+        self.pivoted = F.pivoted
         self.fell_back = True
 
     def solve(self, B, nrhs, max_iter=10, tol=1e-12):
         """B <- H^-1 B (float64 device buffer, n x nrhs column-major).  With
         the block structure the factor was made with, the solves and the
         residuals skip its zero blocks."""
-        if self.precision != "fp64":
+        if self.precision == "mixed":
             try:
                 return self._solve(B, nrhs, max_iter, tol)
             except _lib.NotConverged:
@@ -557,6 +571,10 @@ class Factor:
 
     def _solve(self, B, nrhs, max_iter, tol):
         blocks = self._structure[2:] if self._structure else ()
+        if self.precision == "lu":
+            check(lib().omh_lu_solve_f64(_ptr(self.L), self.n, self.n, _ptr(self.ipiv), _ptr(self.work), _ptr(B),
+                                         self.n, nrhs, _stream()))
+            return B
         if self.precision == "fp64":
             if blocks:
                 check(lib().omh_sym_solve_f64_blocks(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n,
@@ -593,14 +611,38 @@ def _block_args(structure):
     return (starts, hfill, len(starts), starts.ctypes.data_as(C.c_void_p), hfill.ctypes.data_as(C.c_void_p))
 
 
-def factor(Hdev, n, precision="fp64", structure=None):
+def _lu_factor(Hdev, n):
+    """FP64 LU with partial pivoting of Hdev in place (the pivoted fallback)."""
+    torch = _t()
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    work = torch.empty(int(lib().omh_lu_workspace_bytes(n)), dtype=torch.uint8, device="cuda")
+    info = C.c_int32()
+    check(lib().omh_lu_factor_f64(_ptr(Hdev), n, n, _ptr(ipiv), _ptr(work), C.byref(info), _stream()))
+    if info.value != 0:
+        raise _lib.OpenMEEGError("the matrix is singular: zero pivot in column %d of the pivoted LU" % (info.value - 1))
+    return Factor(Hdev, n, work, "lu", ipiv=ipiv)
+
+
+def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=None):
     """Factor a device HeadMat (triangles-first order).  precision 'fp64'
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
     factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
-    trailing updates.  Given structure = Geometry.factor_segments(), the
-    trailing updates, and later the solves, skip the parts that only
-    multiply structural zeros (the factor is the same)."""
+    trailing updates; 'fp64_lu' goes straight to the pivoted LU.  Given
+    structure = Geometry.factor_segments(), the trailing updates, and later
+    the solves, skip the parts that only multiply structural zeros (the
+    factor is the same).
+
+    The pivot-free factorisation is used when it is stable: it reports a
+    vanishing pivot, and given n_positive (the triangle unknowns, which come
+    first) any pivot whose sign differs from + for the first n_positive
+    unknowns and - for the rest also counts as a breakdown.  Then the FP64
+    matrix is factored by LU with partial pivoting (the reference's DSPTRF
+    pivots too, symmatrix.cpp:133-152): for the mixed paths Hdev itself,
+    which is intact; for 'fp64', whose failed factorisation overwrote Hdev,
+    the matrix rebuild() returns (else OpenMEEGError)."""
     torch = _t()
+    if precision == "fp64_lu":
+        return _lu_factor(Hdev, n)
     info = C.c_int32()
     sargs = _block_args(structure)
     blocks = sargs[2:] if sargs else ()
@@ -610,7 +652,8 @@ def factor(Hdev, n, precision="fp64", structure=None):
             check(lib().omh_sym_factor_f64_blocks(_ptr(Hdev), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
         else:
             check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(Hdev, n, work, "fp64", structure=structure)
+        F = Factor(Hdev, n, work, "fp64", structure=structure, n_positive=n_positive)
+        elem = 8
     elif precision in ("mixed", "mixed16"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
         check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
@@ -621,11 +664,29 @@ def factor(Hdev, n, precision="fp64", structure=None):
         else:
             fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
             check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(L32, n, work, "mixed", H64=Hdev, structure=structure)
+        F = Factor(L32, n, work, "mixed", H64=Hdev, structure=structure, n_positive=n_positive)
+        elem = 4
     else:
-        raise ValueError("precision must be 'fp64', 'mixed' or 'mixed16'")
+        raise ValueError("precision must be 'fp64', 'mixed', 'mixed16' or 'fp64_lu'")
+    reason = None
     if info.value != 0:
-        raise _lib.OpenMEEGError("pivot-free factorisation broke down at pivot %d" % (info.value - 1))
+        reason = "vanishing pivot %d" % (info.value - 1)
+    elif n_positive is not None:
+        first = C.c_int64()
+        check(lib().omh_factor_sign_mismatch(_ptr(F.work), n, elem, int(n_positive), C.byref(first), _stream()))
+        if first.value >= 0:
+            reason = "pivot sign at unknown %d" % first.value
+    if reason is None:
+        return F
+    if precision == "fp64":
+        del F
+        if rebuild is None:
+            raise _lib.OpenMEEGError("pivot-free factorisation broke down (%s)" % reason)
+        Hdev = rebuild()
+    else:
+        del F
+    F = _lu_factor(Hdev, n)
+    F.breakdown = reason
     return F
 
 
@@ -633,6 +694,15 @@ def factor(Hdev, n, precision="fp64", structure=None):
 # Assembly (assemble.h:32-35, assembleSourceMat.h:24-29)
 # ---------------------------------------------------------------------------
 HEADMAT_FAR_FIELD = 1
+
+
+def _head_factor(geom, H, precision, integrator=None, far_field=False):
+    """factor() of a pipeline's HeadMat H with the geometry's block
+    structure and pivot-sign pattern (its triangle unknowns first); an FP64
+    breakdown re-assembles H for the pivoted LU."""
+    n = geom.size
+    return factor(H, n, precision, structure=geom.factor_segments(), n_positive=n - geom.n_vertex_unknowns,
+                  rebuild=lambda: _head_mat_device(geom, integrator, far_field))
 
 
 def _head_mat_device(geom, integrator=None, far_field=False):
@@ -745,7 +815,7 @@ def GainEEGadjoint(geom, dipoles, hm, h2em, precision="fp64"):
     """GainEEGadjoint(geo, dipoles, HeadMat, Head2EEGMat) -- gain.h:78-89:
     Y = H^-1 E^T with the stored factor, then G = Y^T DSM."""
     n, ne = geom.size, h2em.nlin()
-    F = factor(hm.device().clone(), n, precision)
+    F = hm._factor(precision)
     Y = _eeg_rhs(geom, h2em)
     F.solve(Y, ne)
     D = _dsm_device(geom, dipoles, "")
@@ -1165,7 +1235,7 @@ def GainMEGadjoint(geom, dipoles, hm, h2mm, ds2mm, precision="fp64"):
     n, nv = geom.size, geom.n_vertex_unknowns
     M = h2mm.array()
     ns = M.shape[0]
-    F = factor(hm.device().clone(), n, precision)
+    F = hm._factor(precision)
     Y = to_device_rows(np.asfortranarray(M.T), n, nv)
     F.solve(Y, ns)
     D = _dsm_device(geom, dipoles, "")
@@ -1187,7 +1257,7 @@ class GainEEGMEGadjoint:
         M = h2mm.array()
         ns = M.shape[0]
         torch = _t()
-        F = factor(hm.device().clone(), n, precision)
+        F = hm._factor(precision)
         Y = torch.cat([_eeg_rhs(geom, h2em).view(ne, n),
                        to_device_rows(np.asfortranarray(M.T), n, nv).view(ns, n)], dim=0).contiguous().view(-1)
         F.solve(Y, ne + ns)
@@ -1255,7 +1325,7 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     well-separated pairs by Gauss quadrature (far_field defaults to True for
     the mixed paths, False for 'fp64')."""
     if far_field is None:
-        far_field = precision != "fp64"
+        far_field = precision not in ("fp64", "fp64_lu")
     torch = _t()
     n = geom.size
     times = {}
@@ -1269,7 +1339,7 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     h2em = Head2EEGMat(geom, electrodes)
     Y = _eeg_rhs(geom, h2em)
     marks.append(("h2em", ev())); marks[-1][1].record()
-    F = factor(H, n, precision, structure=geom.factor_segments())
+    F = _head_factor(geom, H, precision, integrator, far_field)
     marks.append(("factor", ev())); marks[-1][1].record()
     F.solve(Y, h2em.nlin(), max_iter=max_refine, tol=refine_tol)
     marks.append(("solve", ev())); marks[-1][1].record()
@@ -1280,7 +1350,7 @@ def gain_eeg_pipeline(geom, dipoles, electrodes, precision="fp64", domain_name="
     for (a, ea), (b, eb) in zip(marks[:-1], marks[1:]):
         times[b] = ea.elapsed_time(eb) / 1e3
     info = dict(times=times, refine_iterations=F.iterations, refine_rel_update=F.rel_update,
-                refine_fell_back_fp64=F.fell_back, n=n,
+                refine_fell_back_fp64=F.fell_back, pivoted_lu=F.pivoted, n=n,
                 n_electrodes=h2em.nlin(), n_dipoles=nd)
     del F, H, D, Y
     return G, info
@@ -1293,7 +1363,7 @@ def gain_eeg_meg_pipeline(geom, dipoles, electrodes, squids, precision="fp64", d
     rows), then G_EEG = Y_E^T DSM and G_MEG = Y_M^T DSM + DipSource2MEG.
     Returns (G_EEG, G_MEG as device tensors, column-major, info)."""
     if far_field is None:
-        far_field = precision != "fp64"
+        far_field = precision not in ("fp64", "fp64_lu")
     torch = _t()
     n = geom.size
     ev = lambda: torch.cuda.Event(enable_timing=True)
@@ -1310,7 +1380,7 @@ def gain_eeg_meg_pipeline(geom, dipoles, electrodes, squids, precision="fp64", d
     S2 = _ds2meg_device(dipoles, squids)                     # ns x nd
     Y = torch.cat([_eeg_rhs(geom, h2em).view(ne, n), M[:ns * n].view(n, ns).t()], dim=0).contiguous().view(-1)
     marks.append(("sensors", ev())); marks[-1][1].record()
-    F = factor(H, n, precision, structure=geom.factor_segments())
+    F = _head_factor(geom, H, precision, integrator, far_field)
     marks.append(("factor", ev())); marks[-1][1].record()
     F.solve(Y, ne + ns, max_iter=max_refine, tol=refine_tol)
     marks.append(("solve", ev())); marks[-1][1].record()
@@ -1336,7 +1406,7 @@ def gain_eeg_surf_pipeline(geom, mesh, electrodes, squids=None, precision="fp64"
     None, info); gains are device tensors, column-major (sensors x source
     vertices)."""
     if far_field is None:
-        far_field = precision != "fp64"
+        far_field = precision not in ("fp64", "fp64_lu")
     if isinstance(mesh, (str, os.PathLike)):
         mesh = Mesh(mesh)
     torch = _t()
@@ -1358,7 +1428,7 @@ def gain_eeg_surf_pipeline(geom, mesh, electrodes, squids=None, precision="fp64"
         S2 = _ss2meg_device(mesh, squids)                    # ns x nsrc
         Y = torch.cat([Y.view(ne, n), M[:ns * n].view(n, ns).t()], dim=0).contiguous().view(-1)
     marks.append(("sensors", ev())); marks[-1][1].record()
-    F = factor(H, n, precision, structure=geom.factor_segments())
+    F = _head_factor(geom, H, precision, integrator, far_field)
     marks.append(("factor", ev())); marks[-1][1].record()
     F.solve(Y, ne + ns, max_iter=max_refine, tol=refine_tol)
     marks.append(("solve", ev())); marks[-1][1].record()
@@ -1384,7 +1454,7 @@ def gain_internal_pot_pipeline(geom, dipoles, points, precision="fp64", domain_n
     DipSource2InternalPotMat.  Returns (G as a device tensor, column-major
     kept points x dipoles, kept point indices, info)."""
     if far_field is None:
-        far_field = precision != "fp64"
+        far_field = precision not in ("fp64", "fp64_lu")
     torch = _t()
     n = geom.size
     pts = _points_array(points)
@@ -1397,7 +1467,7 @@ def gain_internal_pot_pipeline(geom, dipoles, points, precision="fp64", domain_n
     # M is (points x n, leading dimension ld): its first nk rows, transposed,
     # are the right-hand sides (n x nk column-major, triangles-first rows)
     Y = M[:ld * n].view(n, ld)[:, :nk].t().contiguous().view(-1)
-    F = factor(H, n, precision, structure=geom.factor_segments())
+    F = _head_factor(geom, H, precision, integrator, far_field)
     F.solve(Y, nk, max_iter=max_refine, tol=refine_tol)
     G = _to_dev(np.asfortranarray(S2).ravel(order="F"))
     check(lib().omh_gemm_f64(1, 0, nk, nd, n, 1.0, _ptr(Y), n, _ptr(D), n, 1.0, _ptr(G), nk, _stream()))

@@ -220,7 +220,45 @@ struct Geom {
     ~Geom() { omh_geometry_free(g); }
     int64_t n() const { return sizes[0]; }
     int64_t nv() const { return sizes[3]; }
+    // The reference's old unknown numbering (Geometry::generate_indices with
+    // OLD_ORDERING, geometry.cpp:256-295): mesh by mesh in geometry order,
+    // its vertices then its triangles (none for a current barrier or an
+    // isolated mesh); nested geometries only (geometry.cpp:270).  Returns
+    // perm with perm[u] = the old index of reference-order unknown u < n().
+    std::vector<int64_t> old_order() const {
+        if (!sizes[7]) throw std::runtime_error("-old-ordering needs a nested geometry");
+        const int64_t n = this->n();
+        std::vector<int64_t> perm((size_t)n, -1);
+        int64_t idx = 0;
+        for (int32_t m = 0; m < (int32_t)sizes[5]; ++m) {
+            int64_t info[8];
+            check(omh_geometry_mesh_info(g, m, info));
+            if (info[2] < 0) throw std::runtime_error("-old-ordering needs meshes that share no vertices");
+            for (int64_t k = 0; k < info[4]; ++k) perm[(size_t)(info[2] + k)] = idx++;
+            if (!(info[5] & 3))
+                for (int64_t t = 0; t < info[1]; ++t)
+                    if (info[0] + t < n) perm[(size_t)(info[0] + t)] = idx++;
+        }
+        for (int64_t u = 0; u < n; ++u)
+            if (perm[(size_t)u] < 0 || perm[(size_t)u] >= n) throw std::runtime_error("old ordering: inconsistent numbering");
+        return perm;
+    }
 };
+
+// Rows (or columns) of a full matrix moved to the old numbering: row u goes
+// to row perm[u].
+inline Full permute_rows(const Full& a, const std::vector<int64_t>& perm) {
+    Full b = a;
+    for (int64_t j = 0; j < a.ncol; ++j)
+        for (int64_t i = 0; i < a.nlin; ++i) b(perm[(size_t)i], j) = a(i, j);
+    return b;
+}
+inline Full permute_cols(const Full& a, const std::vector<int64_t>& perm) {
+    Full b = a;
+    for (int64_t j = 0; j < a.ncol; ++j)
+        for (int64_t i = 0; i < a.nlin; ++i) b(i, perm[(size_t)j]) = a(i, j);
+    return b;
+}
 
 // Sums integration points into sensors: out (nsensors x ncols) = W P with W
 // the weights matrix (Sensors::getWeightsMatrix, sensors.h:191-196).

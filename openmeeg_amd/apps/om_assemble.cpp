@@ -14,6 +14,8 @@
 //   -EITSourceMat, -EITSM, -EITsm               geometry conductivity electrodes output
 //   -Head2InternalPotMat, -H2IPM, -h2ipm        geometry conductivity points output
 //   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm geometry conductivity dipoles points output [domain]
+//   -old-ordering (with any of the above): unknowns numbered mesh by mesh,
+//   (V1, p1, V2, p2, V3) instead of (V1, V2, V3, p1, p2) (apps/assemble.cpp:41)
 #include <algorithm>
 #include <cstring>
 #include <iostream>
@@ -23,6 +25,8 @@
 using namespace cli;
 
 namespace {
+
+bool g_old_ordering = false;   // -old-ordering
 
 bool is(const char* a, std::initializer_list<const char*> names) {
     for (const char* n : names)
@@ -39,10 +43,25 @@ void head_mat(const std::string& geom, const std::string& cond, const std::strin
     const int64_t n = g.n();
     Dev<double> H((size_t)(n * n));
     check(omh_head_mat(g.g, 3, H.p, n, OMH_ORDER_REFERENCE, nullptr));
+    if (g_old_ordering) {   // packed upper storage (symmatrix.h:52-62) in the old numbering
+        const std::vector<int64_t> perm = g.old_order();
+        std::vector<int64_t> inv((size_t)n);
+        for (int64_t u = 0; u < n; ++u) inv[(size_t)perm[(size_t)u]] = u;
+        const std::vector<double> h = H.down();
+        std::vector<double> P((size_t)(n * (n + 1) / 2));
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i <= j; ++i) P[(size_t)(i + j * (j + 1) / 2)] = h[(size_t)(inv[i] + inv[j] * n)];
+        write_symmetric(out, P, n);
+        return;
+    }
     Dev<double> P((size_t)(n * (n + 1) / 2));
     check(omh_sym_pack(H.p, n, n, n, OMH_ORDER_REFERENCE, P.p, nullptr));
     write_symmetric(out, P.down(), n);
 }
+
+// The unknown rows / columns of a matrix in the numbering the options ask for.
+Full rows_out(const Geom& g, const Full& m) { return g_old_ordering ? permute_rows(m, g.old_order()) : m; }
+Full cols_out(const Geom& g, const Full& m) { return g_old_ordering ? permute_cols(m, g.old_order()) : m; }
 
 void dip_source_mat(const std::string& geom, const std::string& cond, const std::string& dipf, const std::string& out,
                     const char* domain, int levels) {
@@ -64,7 +83,7 @@ void dip_source_mat(const std::string& geom, const std::string& cond, const std:
     m.nlin = n;
     m.ncol = nd;
     m.a = D.down();
-    write_full(out, m);
+    write_full(out, rows_out(g, m));
 }
 
 void surf_source_mat(const std::string& geom, const std::string& cond, const std::string& meshf,
@@ -83,7 +102,7 @@ void surf_source_mat(const std::string& geom, const std::string& cond, const std
     m.nlin = n;
     m.ncol = ms[0];
     m.a = D.down();
-    write_full(out, m);
+    write_full(out, rows_out(g, m));
 }
 
 void head2eeg(const std::string& geom, const std::string& cond, const std::string& elf, const std::string& out) {
@@ -95,6 +114,10 @@ void head2eeg(const std::string& geom, const std::string& cond, const std::strin
     std::vector<double> vals((size_t)(3 * ne));
     check(omh_head2eeg_mat(g.g, de.p, ne, cols.data(), vals.data(), nullptr));
     // SparseMatrix(i, col) = val, a map: a repeated (i, col) keeps the last value
+    if (g_old_ordering) {
+        const std::vector<int64_t> perm = g.old_order();
+        for (auto& c : cols) c = perm[(size_t)c];
+    }
     std::map<std::pair<int64_t, int64_t>, double> e;
     for (int64_t i = 0; i < ne; ++i)
         for (int k = 0; k < 3; ++k) e[{i, cols[3 * i + k]}] = vals[3 * i + k];
@@ -123,7 +146,7 @@ void head2meg(const std::string& geom, const std::string& cond, const std::strin
     Dev<double> dd(dirs.data(), dirs.size());
     Dev<double> P((size_t)(np * n));
     check(omh_head2meg_mat(g.g, dp.p, dd.p, np, P.p, np, OMH_ORDER_REFERENCE, nullptr));
-    write_full(out, sensor_sum(s, P.down(), n));
+    write_full(out, cols_out(g, sensor_sum(s, P.down(), n)));
 }
 
 void dip2meg(const std::string& dipf, const std::string& sqf, const std::string& out) {
@@ -184,7 +207,7 @@ void head2ip(const std::string& geom, const std::string& cond, const std::string
     m.a.resize((size_t)(nk * n));
     for (int64_t j = 0; j < n; ++j)
         for (int64_t i = 0; i < nk; ++i) m(i, j) = h[i + j * ld];
-    write_full(out, m);
+    write_full(out, cols_out(g, m));
 }
 
 void dip2ip(const std::string& geom, const std::string& cond, const std::string& dipf, const std::string& ptf,
@@ -226,7 +249,7 @@ void eit_source_mat(const std::string& geom, const std::string& cond, const std:
     m.nlin = n;
     m.ncol = ne;
     m.a = D.down();
-    write_full(out, m);
+    write_full(out, rows_out(g, m));
 }
 
 void help(const char* cmd) {
@@ -242,6 +265,7 @@ void help(const char* cmd) {
               << "   -EITSourceMat, -EITSM, -EITsm :      geometry conductivity electrodes output\n"
               << "   -Head2InternalPotMat, -H2IPM, -h2ipm : geometry conductivity points output\n"
               << "   -DipSource2InternalPotMat, -DS2IPM, -ds2ipm : geometry conductivity dipoles points output [domain]\n"
+              << "   -old-ordering : with any of the above, unknowns numbered (V1, p1, V2, p2, V3)\n"
               << "Output format from the suffix: .mat (MAT 7.3), .bin, .txt\n";
 }
 
@@ -253,11 +277,13 @@ int main(int argc, char** argv) {
         return 0;
     }
     try {
+        for (int k = 1; k < argc; ++k)
+            if (is(argv[k], {"-old-ordering"})) g_old_ordering = true;
         int done = 0;
         for (int k = 1; k < argc; ++k) {
             const char* o = argv[k];
             if (o[0] != '-') continue;
-            if (is(o, {"-old-ordering"})) throw std::runtime_error("-old-ordering is not supported by this build");
+            if (is(o, {"-old-ordering"})) continue;
             if (is(o, {"-HeadMat", "-HM", "-hm"})) {
                 need_args(argc, k, 3, o);
                 head_mat(argv[k + 1], argv[k + 2], argv[k + 3]);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -19,6 +20,7 @@
 #include "geometry.hpp"
 #include "layout.hpp"
 #include "linalg.hpp"
+#include "lu.hpp"
 #include "sources.hpp"
 
 struct omh_geometry {
@@ -725,6 +727,44 @@ int omh_sym_solve_f32_blocks(const float* d_L, int64_t n, int64_t ld, const void
         if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
         const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
         omh::sym_solve<float>(d_L, n, ld, d_work, d_B, ldb, nrhs, S(stream), &bf);
+    });
+}
+
+int omh_factor_sign_mismatch(const void* d_work, int64_t n, int32_t elem_size, int64_t n_positive, int64_t* first,
+                             void* stream) {
+    return guard([&] {
+        need(d_work, "d_work");
+        need(first, "first");
+        if (n <= 0 || (elem_size != 4 && elem_size != 8) || n_positive < 0 || n_positive > n)
+            throw std::invalid_argument("bad sizes");
+        *first = omh::factor_sign_mismatch(d_work, n, (size_t)elem_size, n_positive, S(stream));
+    });
+}
+
+int64_t omh_lu_workspace_bytes(int64_t n) { return n > 0 ? (int64_t)omh::lu_workspace_bytes(n) : -1; }
+
+int omh_lu_factor_f64(double* d_A, int64_t n, int64_t ld, int32_t* d_ipiv, void* d_work, int32_t* info,
+                      void* stream) {
+    return guard([&] {
+        need(d_A, "d_A");
+        need(d_ipiv, "d_ipiv");
+        need(d_work, "d_work");
+        if (n <= 0 || ld < n) throw std::invalid_argument("bad sizes");
+        if (n > INT32_MAX) throw std::invalid_argument("n exceeds the 32-bit pivot indices");
+        const int i = omh::lu_factor(d_A, n, ld, d_ipiv, d_work, S(stream));
+        if (info) *info = i;
+    });
+}
+
+int omh_lu_solve_f64(const double* d_LU, int64_t n, int64_t ld, const int32_t* d_ipiv, const void* d_work,
+                     double* d_B, int64_t ldb, int64_t nrhs, void* stream) {
+    return guard([&] {
+        need(d_LU, "d_LU");
+        need(d_ipiv, "d_ipiv");
+        need(d_work, "d_work");
+        need(d_B, "d_B");
+        if (n <= 0 || ld < n || ldb < n || nrhs < 0) throw std::invalid_argument("bad sizes");
+        omh::lu_solve(d_LU, n, ld, d_ipiv, d_work, d_B, ldb, nrhs, S(stream));
     });
 }
 

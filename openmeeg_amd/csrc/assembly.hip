@@ -47,7 +47,6 @@ struct Arena {
     void* p[SCR_NSLOTS] = {};
     size_t sz[SCR_NSLOTS] = {};
     hipStream_t last[SCR_NSLOTS] = {};
-    hipEvent_t ev[SCR_NSLOTS] = {};
 };
 std::mutex g_arena_mu;
 std::map<int, Arena> g_arenas;
@@ -55,20 +54,15 @@ std::map<int, Arena> g_arenas;
 
 thread_local ScratchArena* tl_arena = nullptr;   // ArenaScope override of this thread
 
-// Hands out slot `slot` for work on stream st.  When the slot was last
-// handed out on another stream, st first waits for everything enqueued there
-// so far (which includes the previous user's kernels).  Growth is stream
-// ordered: the old buffer is freed and the new one allocated on st, so that
-// neither waits for the rest of the device.  A frozen arena (a batch plan's,
-// used on the plan's own stream, possibly under graph capture) adds no
-// cross-stream waits.
-static void* take(void*& p, size_t& sz, hipStream_t& last, hipEvent_t& ev, size_t bytes, hipStream_t st,
-                  bool frozen) {
-    if (p && last != st && !frozen) {
-        if (!ev) OMH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        OMH_HIP(hipEventRecord(ev, last));
-        OMH_HIP(hipStreamWaitEvent(st, ev, 0));
-    }
+// Hands out slot `slot` for work on stream st.  Growth is stream ordered:
+// the old buffer is freed and the new one allocated on st, so that the
+// caller's other streams do not wait.  Only when the slot was last handed
+// out on another stream (whose kernels may still use it, and which may no
+// longer exist) does the device finish its work first.  A frozen arena (a
+// batch plan's, used on the plan's own stream, possibly under graph
+// capture) neither grows nor waits.
+static void* take(void*& p, size_t& sz, hipStream_t& last, size_t bytes, hipStream_t st, bool frozen) {
+    if (p && last != st && !frozen) OMH_HIP(hipDeviceSynchronize());
     last = st;
     if (sz >= bytes && p) return p;
     if (frozen) throw std::runtime_error("frozen scratch arena is too small (plan sizes changed)");
@@ -81,18 +75,22 @@ static void* take(void*& p, size_t& sz, hipStream_t& last, hipEvent_t& ev, size_
     return p;
 }
 
-void ScratchArena::release() {
-    for (int s = 0; s < SCR_NSLOTS; ++s) {
+// Frees stream-ordered allocations once the device has finished with them.
+static void free_slots(void** p, size_t* sz) {
+    bool any = false;
+    for (int s = 0; s < SCR_NSLOTS; ++s) any = any || p[s];
+    if (!any) return;
+    (void)hipDeviceSynchronize();
+    for (int s = 0; s < SCR_NSLOTS; ++s)
         if (p[s]) {
-            (void)hipStreamSynchronize(last[s]);
-            (void)hipFree(p[s]);
+            (void)hipFreeAsync(p[s], nullptr);
             p[s] = nullptr;
             sz[s] = 0;
         }
-        if (ev[s]) (void)hipEventDestroy(ev[s]);
-        ev[s] = nullptr;
-    }
+    (void)hipStreamSynchronize(nullptr);
 }
+
+void ScratchArena::release() { free_slots(p, sz); }
 
 ArenaScope::ArenaScope(ScratchArena* a) : prev_(tl_arena) { tl_arena = a; }
 ArenaScope::~ArenaScope() { tl_arena = prev_; }
@@ -100,13 +98,13 @@ ArenaScope::~ArenaScope() { tl_arena = prev_; }
 void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st) {
     if (tl_arena) {
         ScratchArena& a = *tl_arena;
-        return take(a.p[slot], a.sz[slot], a.last[slot], a.ev[slot], bytes, st, a.frozen);
+        return take(a.p[slot], a.sz[slot], a.last[slot], bytes, st, a.frozen);
     }
     int dev = 0;
     OMH_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena& a = g_arenas[dev];
-    return take(a.p[slot], a.sz[slot], a.last[slot], a.ev[slot], bytes, st, false);
+    return take(a.p[slot], a.sz[slot], a.last[slot], bytes, st, false);
 }
 
 void release_scratch() {
@@ -115,13 +113,7 @@ void release_scratch() {
     (void)hipGetDevice(&cur);
     for (auto& kv : g_arenas) {
         (void)hipSetDevice(kv.first);
-        for (int s = 0; s < SCR_NSLOTS; ++s) {
-            if (kv.second.p[s]) {
-                (void)hipStreamSynchronize(kv.second.last[s]);
-                (void)hipFree(kv.second.p[s]);
-            }
-            if (kv.second.ev[s]) (void)hipEventDestroy(kv.second.ev[s]);
-        }
+        free_slots(kv.second.p, kv.second.sz);
     }
     g_arenas.clear();
     (void)hipSetDevice(cur);

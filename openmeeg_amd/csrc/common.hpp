@@ -32,10 +32,10 @@ struct NotConverged : std::runtime_error {
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Grow-only device scratch buffer per slot and per device, handed out for
-// work on stream st: st waits for the slot's previous stream when that
-// differs, and growth frees and allocates on st (stream ordered; no
-// device-wide synchronisation).  Calls that overlap on one device and use
-// the same slot from unrelated host threads race.
+// work on stream st.  Growth frees and allocates on st (stream ordered, the
+// caller's other streams do not wait); a slot last used on another stream is
+// handed out after the device has finished its work.  Calls that overlap on
+// one device and use the same slot from unrelated host threads race.
 enum ScratchSlot {
     SCR_ASSEMBLY = 0,
     SCR_DTMP = 1,
@@ -58,7 +58,6 @@ struct ScratchArena {
     void* p[SCR_NSLOTS] = {};
     size_t sz[SCR_NSLOTS] = {};
     hipStream_t last[SCR_NSLOTS] = {};
-    hipEvent_t ev[SCR_NSLOTS] = {};
     bool frozen = false;
     void release();
     ~ScratchArena() { release(); }

@@ -945,6 +945,22 @@ __global__ void __launch_bounds__(64 * 4 * S_PANELS) k_split_group(SplitSrc src,
 // lanes over 64 banks or 8 lanes over 32 banks.
 __device__ __forceinline__ int swz(int r, int c) { return r * 4 + (c ^ (((r >> 1) ^ (r >> 2)) & 3)); }
 
+// Slice clock of the split update (k-synchronised schedule): tiles that run
+// at the same time on an XCD share operand bands, but each walks its k
+// slices from its own start, so with the k extent of 1024 the bands of the
+// 64 tiles in flight (16 bands of 512 KB) overflow the XCD's 4 MB L2.  With
+// the clock on, a tile starts at the slice (realtime / period) mod NKS and
+// wraps around, so that every tile in flight works on about the same slice
+// and a band's slice is fetched once for all of them.  period[0] holds the
+// period in ticks of the 100 MHz realtime counter (0: every tile starts at
+// slice 0); workgroup 0 writes the period it measured on its tile to
+// period_out (the next launch's period[0]).  The products summed per entry
+// are the same; their order follows the clock.
+struct SliceClock {
+    const unsigned* period = nullptr;
+    unsigned* period_out = nullptr;
+};
+
 // C (m x m, leading dimension ldc) -= split product of operand rows
 // r_off + [0, m), over k extent KP, on the tiles of map: the lower triangle,
 // or (kind 6) the lower part of a strip of whole tile columns; entries with
@@ -957,8 +973,9 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
                                                 const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
                                                 const _Float16* __restrict__ Bl, const float* __restrict__ scale,
                                                 float* __restrict__ C, int64_t ldc, TileMap map,
-                                                const TileRemap& rm) {
+                                                const TileRemap& rm, SliceClock clk = SliceClock{}) {
     __shared__ half8_t S[2][4][S_TILE * 4];
+    __shared__ int rot_sh;
     int64_t t = blockIdx.x;
     if (map.kind >= 4) {
         t = xcd_tile(t, map.per);
@@ -974,10 +991,25 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const _Float16* src[4] = {Ah, Al, Bh, Bl};
+    constexpr int NKS = KP / SK;
+    static_assert((NKS & (NKS - 1)) == 0, "the slice clock wraps k with a mask");
+    // first slice of this tile (the slice clock, read once for the workgroup)
+    uint64_t t_start = 0;
+    int rot = 0;
+    if (clk.period) {
+        if (tid == 0) {
+            t_start = __builtin_amdgcn_s_memrealtime();
+            const unsigned per = clk.period[0];
+            rot_sh = per ? (int)((t_start / per) & (uint64_t)(NKS - 1)) : 0;
+        }
+        __syncthreads();
+        rot = rot_sh;
+    }
     // operand slices travel global -> registers (two slots, loaded two
     // slices ahead) -> LDS (two buffers)
     half8_t stg[2][4][2];
     auto gload = [&](int ks, int slot) {
+        const int64_t kk = (int64_t)((ks + rot) & (NKS - 1));
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -985,7 +1017,7 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
                 const int q = tid + 256 * p, r = q >> 2, c = q & 3;
                 const int64_t base = a < 2 ? row0 : col0;
                 stg[slot][a][p] =
-                    *reinterpret_cast<const half8_t*>(src[a] + (((int64_t)ks * mp + base + r) * SK + 8 * c));
+                    *reinterpret_cast<const half8_t*>(src[a] + ((kk * mp + base + r) * SK + 8 * c));
             }
     };
     auto lstore = [&](int slot, int buf) {
@@ -1002,7 +1034,6 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    constexpr int NKS = KP / SK;
     const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
     const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
     float4 c_pre[4][4], si_pre[4];
@@ -1086,6 +1117,10 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
                 }
             }
         }
+    if (clk.period_out && blockIdx.x == 0 && tid == 0) {   // this tile's time per slice, for the next launch
+        const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t_start;
+        clk.period_out[0] = (unsigned)max((uint64_t)1, min(dt / NKS, (uint64_t)1000000));
+    }
 }
 
 // The same update on 256 x 256 tiles: 512 threads, 8 waves of 64 x 128
@@ -1230,8 +1265,8 @@ __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64
                                                     const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                     const float* __restrict__ scale, float* __restrict__ C,
-                                                    int64_t ldc, TileMap map, TileRemap rm) {
-    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, rm);
+                                                    int64_t ldc, TileMap map, TileRemap rm, SliceClock clk) {
+    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, rm, clk);
 }
 
 // The next block group's columns (a strip of whole tile columns, map kind 6).
@@ -1631,6 +1666,13 @@ static bool active_tile_rows(const BlockFill& bf, int64_t n, int64_t c0, int64_t
 // bands fetched again from beyond L2 (the 256-row tiles halve the operand
 // bytes per output but, one workgroup per CU, share fewer bands per XCD: 11%
 // less fetch).
+// The split update's slice clock (k-synchronised tile schedule); off with
+// OMH_SYRK_CLOCK=0 (every tile then sums its k slices in order 0, 1, ...).
+static bool slice_clock_on() {
+    const char* e = std::getenv("OMH_SYRK_CLOCK");
+    return !(e && e[0] == '0');
+}
+
 static int syrk_tile() {
     const char* e = std::getenv("OMH_SYRK_TILE");
     return (e && std::atoi(e) == 256) ? 256 : 128;
@@ -1659,9 +1701,20 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
             xp += (size_t)n * FACT_NB * sizeof(float);
         }
     }
-    int* info = (int*)scratch(SCR_SMALL, 64, st);
+    int* info = (int*)scratch(SCR_SMALL, 128, st);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
     OMH_LAUNCH_CHECK();
+    // slice clock of the main update (two period slots, alternating between
+    // launches: one is read while the other is written); the first launch
+    // assumes 1.8 us per k-slice (180 ticks of 10 ns)
+    unsigned* clk_slots = nullptr;
+    if (slice_clock_on()) {
+        clk_slots = (unsigned*)(info + 16);
+        k_set_int<<<1, 1, 0, st>>>((int*)clk_slots, 180);
+        k_set_int<<<1, 1, 0, st>>>((int*)clk_slots + 1, 180);
+        OMH_LAUNCH_CHECK();
+    }
+    int clk_launch = 0;
     const float tiny = 1e-30f;
     LookAhead& la = lookahead();
     auto start_of = [&](int64_t b) { return b * FACT_NB; };
@@ -1770,9 +1823,16 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
                 if (big)
                     k_syrk_split256<S_KP><<<grid, 512, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
                                                                 lda, map, rm);
-                else
+                else {
+                    SliceClock clk;
+                    if (clk_slots && map.kind == 4) {
+                        clk.period = clk_slots + (clk_launch & 1);
+                        clk.period_out = clk_slots + ((clk_launch + 1) & 1);
+                        ++clk_launch;
+                    }
                     k_syrk_split<S_KP><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda,
-                                                             map, rm);
+                                                             map, rm, clk);
+                }
             }
             OMH_LAUNCH_CHECK();
         }
@@ -1864,6 +1924,24 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
     fi.dinv = Dinv;
     fi.signs = sgn;
     return fi;
+}
+
+int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, int64_t n_pos, hipStream_t st) {
+    const int64_t nblk = cdiv(n, FACT_NB);
+    const size_t off = elem * ((size_t)n * FACT_NB + (size_t)nblk * FACT_NB * FACT_NB);
+    std::vector<double> s((size_t)n);
+    if (elem == 8) {
+        OMH_HIP(hipMemcpyAsync(s.data(), (const char*)work + off, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        OMH_HIP(hipStreamSynchronize(st));
+    } else {
+        std::vector<float> f((size_t)n);
+        OMH_HIP(hipMemcpyAsync(f.data(), (const char*)work + off, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+        OMH_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < n; ++i) s[i] = f[i];
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if ((i < n_pos) != (s[i] > 0)) return i;
+    return -1;
 }
 
 template FactorInfo sym_factor<double>(double*, int64_t, int64_t, void*, hipStream_t, bool, const BlockFill*);

@@ -150,6 +150,14 @@ template <typename T>
 FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, bool split16 = false,
                       const BlockFill* bf = nullptr);
 
+// First unknown whose pivot sign in a finished factorisation (workspace
+// `work` of sym_factor, element size elem) differs from +1 for the first
+// n_pos unknowns and -1 for the rest, or -1.  In that pattern the matrix is
+// quasi-definite (its leading n_pos block positive definite, the Schur
+// complement negative definite) and the pivot-free factor is stable; a
+// mismatch means that this order needs pivoting.
+int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, int64_t n_pos, hipStream_t st);
+
 // 1 if any entry of the blocks a block structure declares zero (below the
 // diagonal) is non-zero.
 template <typename T>

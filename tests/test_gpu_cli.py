@@ -100,3 +100,41 @@ def test_cli_head1_eeg_meg_surface(gpu, tmp_path):
     for k in ("hminv", "gain", "gain_meg", "eegmeg_eeg", "eegmeg_meg", "sgem"):
         assert errs[k] < 1e-10, k
     assert errs["gain_adjoint_txt"] < 1e-5          # ASCII: 6 significant digits
+
+
+def test_cli_old_ordering(gpu, tmp_path):
+    """om_assemble -old-ordering (apps/assemble.cpp:41, geometry.cpp:256-295):
+    unknowns numbered mesh by mesh, (V1, p1, V2, p2, V3).  Head1 (cortex,
+    skull, scalp; 42 vertices and 80 triangles each; scalp triangles are
+    current-barrier unknowns outside the matrix): the HeadMat diagonal must
+    read 42 negative (vertex) then 80 positive (triangle) entries per inner
+    mesh, then the scalp's 42 vertices; the matrices equal the default ones
+    under that renumbering (HeadMat both ways, DipSourceMat rows,
+    Head2EEGMat columns), bit for bit."""
+    t = lambda f: str(tmp_path / f)
+    g, c = h1("Head1.geom"), h1("Head1.cond")
+    for flag, tag in (([], "new"), (["-old-ordering"], "old")):
+        run("om_assemble", *flag, "-HM", g, c, t("hm_%s.bin" % tag))
+        run("om_assemble", *flag, "-DSM", g, c, h1("Head1.dip"), t("dsm_%s.bin" % tag), "Brain")
+        run("om_assemble", *flag, "-H2EM", g, c, h1("Head1.patches"), t("h2em_%s.bin" % tag))
+    from oracle import oracle as O
+    H_new = O.unpack(load(t("hm_new.bin"))[1], 286)
+    H_old = O.unpack(load(t("hm_old.bin"))[1], 286)
+    # reference numbering: V_cortex V_skull V_scalp (3 x 42), T_cortex T_skull (2 x 80)
+    perm = np.empty(286, dtype=np.int64)
+    nv, nt = 42, 80
+    for m in range(3):
+        base = m * (nv + nt)
+        perm[m * nv:(m + 1) * nv] = base + np.arange(nv)
+        if m < 2:
+            perm[3 * nv + m * nt:3 * nv + (m + 1) * nt] = base + nv + np.arange(nt)
+    d = np.sign(np.diag(H_old))
+    expect = np.concatenate([-np.ones(nv), np.ones(nt), -np.ones(nv), np.ones(nt), -np.ones(nv)])
+    assert np.array_equal(d, expect)
+    P = np.zeros((286, 286))
+    P[perm, np.arange(286)] = 1.0
+    assert np.array_equal(H_old, P @ H_new @ P.T)
+    D_new, D_old = load(t("dsm_new.bin"))[1], load(t("dsm_old.bin"))[1]
+    assert np.array_equal(D_old[perm], D_new)
+    E_new, E_old = dense(load(t("h2em_new.bin"))[1]), dense(load(t("h2em_old.bin"))[1])
+    assert np.array_equal(E_old[:, perm], E_new)

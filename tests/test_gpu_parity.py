@@ -322,7 +322,7 @@ def test_factor_solve_residual_large(om, precision):
 
 
 @pytest.mark.parametrize("layers,precision", [(3, "mixed16"), (4, "mixed16"), (4, "mixed"), (4, "fp64")])
-def test_block_structured_factor_and_solve(om, layers, precision):
+def test_block_structured_factor_and_solve(om, monkeypatch, layers, precision):
     """The factorisations told the HeadMat's block structure
     (Geometry.factor_segments) skip trailing-update work that only multiplies
     structural zeros: the factor must be bitwise the dense one (the skipped
@@ -335,6 +335,7 @@ def test_block_structured_factor_and_solve(om, layers, precision):
     import torch
     from openmeeg_amd import spheres as S, _lib
     from openmeeg_amd.api import _head_mat_device, factor
+    monkeypatch.setenv("OMH_SYRK_CLOCK", "0")   # k slices in order: bitwise comparable
     if layers == 3:
         g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
     else:
@@ -391,6 +392,7 @@ def test_split_factor_tiles_agree(om, monkeypatch):
     g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
     n = g.size
     H = _head_mat_device(g, far_field=True)
+    monkeypatch.setenv("OMH_SYRK_CLOCK", "0")   # k slices in order: bitwise comparable
     for structure in (None, g.factor_segments()):
         monkeypatch.setenv("OMH_SYRK_TILE", "256")
         F256 = factor(H, n, "mixed16", structure=structure)
@@ -403,6 +405,34 @@ def test_split_factor_tiles_agree(om, monkeypatch):
                                                              float((L256 - L128).abs().max())), flush=True)
         assert torch.equal(L256, L128)
         del F128, L128, L256
+
+
+def test_split_factor_slice_clock(om, monkeypatch):
+    """The split update's slice clock changes only the order in which each
+    tile sums its k slices (the tiles in flight start at the clock's slice
+    and wrap around): the factor with the clock equals the factor without it
+    to FP32 rounding (1e-5 of the largest entry of L, well below the factor's
+    own split-FP16 error), and the refined solutions of both agree to 1e-12.
+    4 layers of 2562 vertices (n = 25608) with the block structure."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    H = _head_mat_device(g, far_field=True)
+    B = torch.from_numpy(np.random.default_rng(8).standard_normal(n * 4)).cuda()
+    out = {}
+    for clock in ("1", "0"):
+        monkeypatch.setenv("OMH_SYRK_CLOCK", clock)
+        F = factor(H, n, "mixed16", structure=g.factor_segments())
+        X = B.clone()
+        F.solve(X, 4, tol=1e-12)
+        out[clock] = (F.L.view(n, n).t().tril().clone(), X)
+        del F
+    dL = float((out["1"][0] - out["0"][0]).abs().max() / out["0"][0].abs().max())
+    dX = float(torch.linalg.norm(out["1"][1] - out["0"][1]) / torch.linalg.norm(out["0"][1]))
+    print("n=%d slice clock vs in-order k slices: max |dL| %.3e, solutions %.3e" % (n, dL, dX), flush=True)
+    assert dL < 1e-5 and dX < 1e-12
 
 
 def test_headmat_head1_matches_golden(om, head1):

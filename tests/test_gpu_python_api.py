@@ -124,7 +124,7 @@ def test_shapes_follow_the_table(built):
             m = out[name]
             assert (m.nlin(), m.ncol()) == (sizes[r], sizes[c]), (source, name)
     n = built["files"][1]
-    assert n["N"] == 286 and n["E"] == 32 and n["D"] == 5
+    assert n["N"] == 286 and n["E"] == 32 and n["D"] == 6
 
 
 @pytest.mark.gpu
