@@ -103,6 +103,14 @@ int omh_head_mat(omh_geometry* g, int order, double* d_H, int64_t ld, int orderi
 #define OMH_HEADMAT_FAR_FIELD 1
 int omh_head_mat_ex(omh_geometry* g, int order, int32_t flags, double* d_H, int64_t ld, int ordering, void* stream);
 
+/* HeadMat(geo, Integrator(order, levels, tol)) with adaptive integration
+ * (assemble.h:32, integrator.h:44-93): every S and D pair integral is
+ * refined by 4-way subdivision of its Gauss triangle while the refined sum
+ * differs from the coarse one by more than tol (relative), to at most
+ * `levels` (0..11) subdivisions; levels = 0 is omh_head_mat. */
+int omh_head_mat_integrator(omh_geometry* g, int order, int levels, double tol, double* d_H, int64_t ld, int ordering,
+                            void* stream);
+
 /* Geometry::domain(point) for n device points (x y z every `stride`
  * doubles): geometry.cpp:146-154.  Writes domain indices to h_domains. */
 int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,

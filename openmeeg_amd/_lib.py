@@ -29,6 +29,7 @@ _SIGS = {
     "omh_geometry_domain_name": ([vp, i32, C.c_char_p, i64], C.c_int),
     "omh_head_mat": ([vp, C.c_int, vp, i64, C.c_int, vp], C.c_int),
     "omh_head_mat_ex": ([vp, C.c_int, i32, vp, i64, C.c_int, vp], C.c_int),
+    "omh_head_mat_integrator": ([vp, C.c_int, C.c_int, f64, vp, i64, C.c_int, vp], C.c_int),
     "omh_locate_domains": ([vp, vp, i64, i64, p_i32, vp], C.c_int),
     "omh_dip_source_mat": ([vp, vp, i64, p_i32, C.c_int, C.c_int, f64, vp, i64, C.c_int, vp], C.c_int),
     "omh_head2eeg_mat": ([vp, vp, i64, p_i64, p_f64, vp], C.c_int),

@@ -709,10 +709,12 @@ def _head_mat_device(geom, integrator=None, far_field=False):
     """Device HeadMat (triangles-first order).  far_field: Gauss quadrature on
     both triangles of well-separated pairs (the mixed-precision path)."""
     integ = integrator or Integrator(3, 0, 0.005)
-    if integ.levels != 0:
-        raise ValueError("adaptive HeadMat integration is not supported on this path (levels must be 0)")
     n = geom.size
     H = _dev_f64(n * n)
+    if integ.levels > 0:   # adaptive integration (the analytic pair integrals, refined)
+        check(lib().omh_head_mat_integrator(geom._h, integ.order, integ.levels, integ.tolerance, _ptr(H), n,
+                                            ORDER_TRIANGLES_FIRST, _stream()))
+        return H
     check(lib().omh_head_mat_ex(geom._h, integ.order, HEADMAT_FAR_FIELD if far_field else 0, _ptr(H), n,
                                 ORDER_TRIANGLES_FIRST, _stream()))
     return H

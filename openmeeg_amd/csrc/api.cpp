@@ -370,6 +370,15 @@ int omh_head_mat_ex(omh_geometry* g, int order, int32_t flags, double* d_H, int6
     });
 }
 
+int omh_head_mat_integrator(omh_geometry* g, int order, int levels, double tol, double* d_H, int64_t ld,
+                            int ordering, void* stream) {
+    return guard([&] {
+        need(g, "geometry");
+        need(d_H, "d_H");
+        omh::assemble_headmat(*g->g, order, d_H, layout_of(*g->g, ld, ordering), S(stream), false, levels, tol);
+    });
+}
+
 int omh_locate_domains(omh_geometry* g, const double* d_points, int64_t stride, int64_t n, int32_t* h_domains,
                        void* stream) {
     return guard([&] {

@@ -32,6 +32,7 @@
 #include <mutex>
 #include <vector>
 
+#include "adaptive.hpp"
 #include "assembly.hpp"
 #include "bem_device.hpp"
 #include "common.hpp"
@@ -387,6 +388,77 @@ __global__ void __launch_bounds__(256, OMH_PAIR_MIN_WAVES) k_d_pairs(const TriRe
     W[(0 * n2 + j) * n1c + i] = A * rx;
     W[(1 * n2 + j) * n1c + i] = A * ry;
     W[(2 * n2 + j) * n1c + i] = A * rz;
+}
+
+// Adaptive integration of the pair integrals (Integrator(order, levels, tol)
+// with levels > 0; integrator.h:69-93): the S entry integrates OperatorS of
+// the row triangle over the column triangle, the D pass the P1 double layer
+// of the column triangle over the row triangle, each through the stack of
+// adaptive.hpp.  Same pair indexing and outputs as k_s_block / k_d_pairs.
+struct OpsIntegrand {
+    OpS o;
+    __device__ __forceinline__ void operator()(d3 x, double* out) const { out[0] = ops_eval(o, x); }
+};
+struct P1Integrand {
+    P1S o;
+    __device__ __forceinline__ void operator()(d3 x, double* out) const {
+        const d3 v = p1s_eval(o, x);
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    }
+};
+
+template <bool TEMP>
+__global__ void __launch_bounds__(256) k_s_block_adapt(const TriRec* __restrict__ tri, int64_t a0, int64_t n1,
+                                                       int64_t b0, int64_t n2, int diag, int64_t ntx, int order,
+                                                       int levels, double tol, double coeff, double* __restrict__ H,
+                                                       Layout L, int64_t ua0, int64_t ub0, double* __restrict__ tmp) {
+    int64_t ti, tj;
+    if (diag) {
+        upper_tile(blockIdx.x, ti, tj);
+    } else {
+        ti = blockIdx.x / ntx;
+        tj = blockIdx.x - ti * ntx;
+    }
+    const int64_t i = ti * TILE + threadIdx.x / TILE;
+    const int64_t j = tj * TILE + threadIdx.x % TILE;
+    if (i >= n1 || j >= n2) return;
+    if (diag && j < i) return;
+    const TriRec& t1 = tri[a0 + i];
+    const TriRec& t2 = tri[b0 + j];
+    OpsIntegrand f{load_ops(t1)};
+    const d3 P[3] = {ld3(t2.p[0]), ld3(t2.p[1]), ld3(t2.p[2])};
+    double res = 0.0;
+    integrate<1>(f, order, levels, tol, P, t2.area, &res);
+    const double v = res * coeff;
+    if (TEMP) {
+        tmp[i + j * n1] = v;
+        if (diag) tmp[j + i * n1] = v;
+    } else {
+        const int64_t r = dev_index(L, ua0 + i), c = dev_index(L, ub0 + j);
+        H[r + c * L.ld] = v;
+        H[c + r * L.ld] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_d_pairs_adapt(const TriRec* __restrict__ tri, int64_t a0, int64_t n1c,
+                                                       int64_t b0, int64_t n2, int order, int levels, double tol,
+                                                       double* __restrict__ W) {
+    const int64_t ni = cdiv_dev(n1c, 64);
+    const int64_t tj = blockIdx.x / ni, tb = blockIdx.x - tj * ni;
+    const int64_t i = tb * 64 + threadIdx.x % 64;
+    const int64_t j = tj * 4 + threadIdx.x / 64;
+    if (i >= n1c || j >= n2) return;
+    const TriRec& t1 = tri[a0 + i];
+    const TriRec& t2 = tri[b0 + j];
+    P1Integrand f{load_p1s(t2)};
+    const d3 P[3] = {ld3(t1.p[0]), ld3(t1.p[1]), ld3(t1.p[2])};
+    double r[3];
+    integrate<3>(f, order, levels, tol, P, t1.area, r);
+    W[(0 * n2 + j) * n1c + i] = r[0];
+    W[(1 * n2 + j) * n1c + i] = r[1];
+    W[(2 * n2 + j) * n1c + i] = r[2];
 }
 
 // D pass 2: H(row t1, vertex v) += sum over the triangles t2 of v (mesh
@@ -906,7 +978,7 @@ int64_t n_upper_tiles(int64_t n) {
 // host drivers
 // ---------------------------------------------------------------------------
 static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff, double* H, const Layout& L,
-                     double* tmp, int far, hipStream_t st) {
+                     double* tmp, int far, hipStream_t st, int levels = 0, double tol = 0.0) {
     const MeshDev& A = dg.meshes[m1];
     const MeshDev& B = dg.meshes[m2];
     const int64_t n1 = A.tri_end - A.tri_begin, n2 = B.tri_end - B.tri_begin;
@@ -916,6 +988,18 @@ static void launch_s(DeviceGeometry& dg, int m1, int m2, int order, double coeff
     // algorithmic work: OperatorS evaluations (pairs x Gauss points)
     static const int npts[4] = {3, 6, 7, 16};
     const double pairs = diag ? 0.5 * (double)n1 * (double)(n1 + 1) : (double)n1 * (double)n2;
+    if (levels > 0) {
+        if (tmp)
+            k_s_block_adapt<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx,
+                                                                 order, levels, tol, coeff, H, L, A.tri_unk0,
+                                                                 B.tri_unk0, tmp);
+        else
+            k_s_block_adapt<false><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx,
+                                                                  order, levels, tol, coeff, H, L, A.tri_unk0,
+                                                                  B.tri_unk0, tmp);
+        OMH_LAUNCH_CHECK();
+        return;
+    }
     ProfScope ps("s_block", pairs * npts[order], st);
     if (tmp)
         k_s_block<true><<<(unsigned)nblk, 256, 0, st>>>(dg.tri, A.tri_begin, n1, B.tri_begin, n2, diag, ntx, order,
@@ -982,7 +1066,7 @@ static void launch_n(DeviceGeometry& dg, const Geometry& g, int m1, int m2, doub
 
 // D(rows = triangles of mesh mr, columns = vertices of mesh mc), operators.h:84-115.
 static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff, double* H, const Layout& L,
-                     int far, hipStream_t st) {
+                     int far, hipStream_t st, int levels = 0, double tol = 0.0) {
     const MeshDev& R = dg.meshes[mr];
     const MeshDev& Cm = dg.meshes[mc];
     const int64_t n1 = R.tri_end - R.tri_begin, n2 = Cm.tri_end - Cm.tri_begin;
@@ -995,9 +1079,14 @@ static void launch_d(DeviceGeometry& dg, int mr, int mc, int order, double coeff
         const int64_t nc = std::min(chunk, n1 - c0);
         const int64_t blk1 = cdiv(nc, 64) * cdiv(n2, 4);
         static const int npts[4] = {3, 6, 7, 16};
-        ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
-        k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W, dg.gp,
-                                                  far, dg.tri, dg.gp);
+        if (levels > 0) {
+            k_d_pairs_adapt<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order,
+                                                            levels, tol, W);
+        } else {
+            ProfScope ps("d_pairs", (double)nc * (double)n2 * npts[order], st);
+            k_d_pairs<<<(unsigned)blk1, 256, 0, st>>>(dg.tri, R.tri_begin + c0, nc, Cm.tri_begin, n2, order, W, dg.gp,
+                                                      far, dg.tri, dg.gp);
+        }
         OMH_LAUNCH_CHECK();
         const int64_t blk2 = cdiv(nc, 64) * cdiv(ns, 4);
         k_d_gather<<<(unsigned)blk2, 256, 0, st>>>(W, nc, n2, Cm.tri_begin, dg.adj_off, dg.adj, dg.slot_vid,
@@ -1171,9 +1260,11 @@ static void launch_pairs_far(DeviceGeometry& dg, int m1, int m2, double coeffS, 
     else launch_pairs_far_t<false, false>(dg, m1, m2, coeffS, tmp, cD, H, L, st);
 }
 
-void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field) {
-    const int far = far_field ? 1 : 0;
+void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field, int levels,
+                      double tol) {
     if (order < 0 || order > 3) throw std::invalid_argument("Gauss order must be 0..3");
+    if (levels < 0 || levels >= ADAPT_MAX_FRAMES) throw std::invalid_argument("adaptive levels must be 0..11");
+    const int far = (far_field && levels == 0) ? 1 : 0;   // adaptive integration: analytic pairs only
     DeviceGeometry& dg = device_geometry(g);
     const int64_t n = g.size();
     fill_f64(H, n, n, L.ld, 0.0, st);
@@ -1198,7 +1289,7 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         }
         double Scoeff = 0.0;
         if (same ? !b1 : (!b1 && !b2)) {   // set_S_block
-            launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, far, st);
+            launch_s(dg, m1, m2, order, mp.cS, H, L, nullptr, far, st, levels, tol);
             Scoeff = mp.cS;
         }
         if (Scoeff != 0.0) {               // set_N_block
@@ -1206,11 +1297,11 @@ void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStr
         } else {
             const int64_t n1 = (int64_t)g.meshes[m1].tris.size(), n2 = (int64_t)g.meshes[m2].tris.size();
             double* tmp = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n1 * (size_t)n2, st);
-            launch_s(dg, m1, m2, order, 1.0, H, L, tmp, far, st);
+            launch_s(dg, m1, m2, order, 1.0, H, L, tmp, far, st, levels, tol);
             launch_n(dg, g, m1, m2, mp.cN, tmp, H, L, st);
         }
-        if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, far, st);              // set_D_block
-        if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, far, st);     // set_Dstar_block
+        if (!b1) launch_d(dg, m1, m2, order, mp.cD, H, L, far, st, levels, tol);              // set_D_block
+        if (!same && !b2) launch_d(dg, m2, m1, order, mp.cD, H, L, far, st, levels, tol);     // set_Dstar_block
     }
     if (fused) {   // the D blocks' other half
         const int64_t nt = L.nt, nv = L.nv;

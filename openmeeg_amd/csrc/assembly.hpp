@@ -13,12 +13,15 @@ namespace omh {
 // Assembles the symmetric HeadMat (Details::HeadMatrix + deflate,
 // assembleHeadMat.cpp:28-115) in FP64 into the device matrix H (n x n,
 // column-major, both halves written) in the unknown order of L, with the
-// non-adaptive Gauss rule of the given order (Integrator(order,0,.),
-// integrator.h:33-48).
+// Gauss rule of the given order (Integrator(order,levels,tol),
+// integrator.h:33-93; levels > 0: adaptive subdivision of the S and D pair
+// integrals, 0..11).
 // far_field: pairs at least FAR_NEAR diameters apart use Gauss quadrature
 // on both triangles instead of the analytic inner integral (the mixed
-// path; the plain FP64 path keeps the reference's integrals everywhere).
-void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field = false);
+// path; the plain FP64 path keeps the reference's integrals everywhere;
+// ignored with levels > 0).
+void assemble_headmat(Geometry& g, int order, double* H, const Layout& L, hipStream_t st, bool far_field = false,
+                      int levels = 0, double tol = 0.0);
 
 // OverlappingSourceMesh (OMExceptions.H): a source mesh meets the geometry.
 struct OverlappingSourceMesh : std::runtime_error {

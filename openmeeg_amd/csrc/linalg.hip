@@ -862,6 +862,11 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 // ---------------------------------------------------------------------------
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 constexpr int SK = 32;        // k-slice of the FP16 MFMA
+// 1: the split update reads and writes its output tiles with nontemporal
+// accesses (they are streamed once and need not stay in L2)
+#ifndef OMH_SYRK_NT_C
+#define OMH_SYRK_NT_C 0
+#endif
 constexpr int S_TILE = 128;   // output tile of the split SYRK
 // Panels per split update (its rank / FACT_NB).  n = 102400: 2 panels
 // 1.48 s, 3 panels 1.51 s, 4 panels 1.34 s for the whole factorisation
@@ -1087,9 +1092,16 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
             for (int a = 0; a < 4; ++a) {
                 si_pre[a] = *reinterpret_cast<const float4*>(scale + r_off + i0 + wm + 16 * a + 4 * fc);
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    c_pre[a][b] = *reinterpret_cast<const float4*>(
-                        C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc);
+                for (int b = 0; b < 4; ++b) {
+                    const float* cp = C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc;
+#if OMH_SYRK_NT_C
+                    // streamed once: keep it from evicting the operand bands in L2
+                    const float4_t v = __builtin_nontemporal_load(reinterpret_cast<const float4_t*>(cp));
+                    c_pre[a][b] = make_float4(v[0], v[1], v[2], v[3]);
+#else
+                    c_pre[a][b] = *reinterpret_cast<const float4*>(cp);
+#endif
+                }
             }
         }
     }
@@ -1108,7 +1120,11 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
                 c.y -= si.y * sj * acc[a][b][1];
                 c.z -= si.z * sj * acc[a][b][2];
                 c.w -= si.w * sj * acc[a][b][3];
+#if OMH_SYRK_NT_C
+                __builtin_nontemporal_store(float4_t{c.x, c.y, c.z, c.w}, reinterpret_cast<float4_t*>(cp));
+#else
                 *cp = c;
+#endif
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -1142,9 +1158,10 @@ __global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, in
                                                        const _Float16* __restrict__ Bh,
                                                        const _Float16* __restrict__ Bl,
                                                        const float* __restrict__ scale, float* __restrict__ C,
-                                                       int64_t ldc, TileMap map, TileRemap rm) {
+                                                       int64_t ldc, TileMap map, TileRemap rm, SliceClock clk) {
     __shared__ half8_t S0[4][S2_TILE * 4];
     __shared__ half8_t S1[4][S2_TILE * 4];
+    __shared__ int rot_sh;
     int64_t t = blockIdx.x;
     if (map.kind >= 4) {
         t = xcd_tile(t, map.per);
@@ -1164,8 +1181,21 @@ __global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, in
     // depends on the row's low 4 bits only: one 32-bit offset serves both p)
     const int rl = lane >> 2;
     const uint32_t goff = (uint32_t)(((32 * wave + rl) * SK + 8 * ((lane & 3) ^ (((rl >> 1) ^ (rl >> 2)) & 3))) * 2);
+    constexpr int NKS = KP / SK;
+    static_assert((NKS & (NKS - 1)) == 0, "the slice clock wraps k with a mask");
+    uint64_t t_start = 0;
+    int rot = 0;
+    if (clk.period) {   // first slice of this tile (the slice clock)
+        if (tid == 0) {
+            t_start = __builtin_amdgcn_s_memrealtime();
+            const unsigned per = clk.period[0];
+            rot_sh = per ? (int)((t_start / per) & (uint64_t)(NKS - 1)) : 0;
+        }
+        __syncthreads();
+        rot = rot_sh;
+    }
     auto dma = [&](int ks, half8_t (*S)[S2_TILE * 4]) {
-        const int64_t so = (int64_t)ks * mp * SK;
+        const int64_t so = (int64_t)((ks + rot) & (NKS - 1)) * mp * SK;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const char* base = reinterpret_cast<const char*>((a == 0 ? Ah : a == 1 ? Al : a == 2 ? Bh : Bl) + so +
@@ -1185,7 +1215,6 @@ __global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, in
     // fragment (row r0 + fr, chunk fc) of a 16-row block at r0: slot
     // r0 * 4 + swz(fr, fc), r0 a multiple of 16
     const int lo = swz(fr, fc);
-    constexpr int NKS = KP / SK;
     auto step = [&](int ks, half8_t (*Sc)[S2_TILE * 4], half8_t (*Sn)[S2_TILE * 4]) {
         __builtin_amdgcn_sched_barrier(0);   // the previous slice's products stay above the wait
         __builtin_amdgcn_s_waitcnt(0);       // this wave's loads of slice ks have landed
@@ -1256,6 +1285,10 @@ __global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, in
                 }
             }
         }
+    }
+    if (clk.period_out && blockIdx.x == 0 && tid == 0) {
+        const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t_start;
+        clk.period_out[0] = (unsigned)max((uint64_t)1, min(dt / NKS, (uint64_t)1000000));
     }
 }
 
@@ -1820,19 +1853,18 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
             const unsigned grid = (unsigned)(map.kind == 4 ? 8 * map.per : tiles);
             ProfScope ps("factor_update_f32s", (double)rows * (double)(rows + 1) * (double)S_KP, st);
             if (tiles > 0) {
+                SliceClock clk;
+                if (clk_slots && map.kind == 4) {
+                    clk.period = clk_slots + (clk_launch & 1);
+                    clk.period_out = clk_slots + ((clk_launch + 1) & 1);
+                    ++clk_launch;
+                }
                 if (big)
                     k_syrk_split256<S_KP><<<grid, 512, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
-                                                                lda, map, rm);
-                else {
-                    SliceClock clk;
-                    if (clk_slots && map.kind == 4) {
-                        clk.period = clk_slots + (clk_launch & 1);
-                        clk.period_out = clk_slots + ((clk_launch + 1) & 1);
-                        ++clk_launch;
-                    }
+                                                                lda, map, rm, clk);
+                else
                     k_syrk_split<S_KP><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda,
                                                              map, rm, clk);
-                }
             }
             OMH_LAUNCH_CHECK();
         }
@@ -1891,7 +1923,11 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         if (m > nb2)
             gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, ldx, A21, lda, T(1), A22 + nb2, lda, st);
         // factor the next block and form its panel on the side stream (it
-        // touches only the next block's columns, which (b) does not)
+        // touches only the next block's columns, which (b) does not).  Beside
+        // the FP64 update the single-workgroup diagonal kernels wait for CU
+        // slots (~10 ms instead of 0.42 ms alone), yet the 4-layer 10242-vertex
+        // FP64 factorisation still takes 4.70 s this way against 4.90 s with
+        // the diagonal block factored on the main stream before (b).
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
         diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
@@ -2033,10 +2069,22 @@ __device__ __forceinline__ void group_product(const T* __restrict__ A, int64_t l
 // forward, block by block, W_b <- Linv_b W_b, then W_b' -= L_b'b W_b for
 // the group's later blocks; backward (BACK) W_b <- Linv_b^T W_b, then
 // W_b' -= L_bb'^T W_b for its earlier blocks.
+// Split-FP16 copy of a solved group's rows of W (the B operand of the split
+// solve products): column c scaled by tw[c], a power of two at or above the
+// column's largest magnitude over the group's rows, then W / tw[c] = h + l
+// with h and l in FP16, stored slice-major like the split factor's operands:
+// (row i, column c) at ((i / 32) rp + c) 32 + i % 32.
+struct WSplit {
+    _Float16* h = nullptr;
+    _Float16* l = nullptr;
+    float* tw = nullptr;
+    int64_t rp = 0;   // columns per slice (the right-hand sides rounded up to 128)
+};
+
 template <typename T, bool BACK, int SV>
 __global__ void __launch_bounds__(256) k_solve_group(const T* __restrict__ L, int64_t lda, const T* __restrict__ Dinv,
                                                      int64_t k0, int64_t gend, T* __restrict__ W, int64_t ldw,
-                                                     int64_t r) {
+                                                     int64_t r, WSplit ws) {
     __shared__ T Wg[SOLVE_GROUP * FACT_NB][SV];
     const int tid = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * SV;
@@ -2077,12 +2125,315 @@ __global__ void __launch_bounds__(256) k_solve_group(const T* __restrict__ L, in
         const int i = e % g, c = e / g;
         if (c < nc) W[k0 + i + (c0 + c) * ldw] = Wg[i][c];
     }
+    if (sizeof(T) == 4 && ws.h) {   // FP32 solves only: the split copy for the products that follow
+        __shared__ float red[SV][256];
+#pragma unroll
+        for (int c = 0; c < SV; ++c) {
+            float mx = 0.f;
+            for (int i = tid; i < g; i += 256) mx = fmaxf(mx, fabsf((float)Wg[i][c]));
+            red[c][tid] = mx;
+        }
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (tid < h)
+#pragma unroll
+                for (int c = 0; c < SV; ++c) red[c][tid] = fmaxf(red[c][tid], red[c][tid + h]);
+            __syncthreads();
+        }
+        for (int e = tid; e < g * SV; e += 256) {
+            const int i = e % g, c = e / g;
+            int ex = 0;
+            if (red[c][0] > 0.f) (void)frexpf(red[c][0], &ex);   // max = f 2^ex, f in [0.5, 1)
+            const float x = (float)Wg[i][c] * ldexpf(1.f, -ex);
+            const _Float16 hx = (_Float16)x;
+            const int64_t o = ((int64_t)(i / 32) * ws.rp + c0 + c) * 32 + i % 32;
+            ws.h[o] = hx;
+            ws.l[o] = (_Float16)(x - (float)hx);
+            if (i == 0) ws.tw[c0 + c] = ldexpf(1.f, ex);
+        }
+    }
+}
+
+// Row intervals [start, start + len) of a split solve product, cut into
+// tiles of `tile` rows: tile x lies in the interval k with tile0[k] <= x <
+// tile0[k + 1].
+constexpr int MAX_SPANS = 8;
+struct Spans {
+    int n = 0;
+    int64_t start[MAX_SPANS] = {};
+    int64_t len[MAX_SPANS] = {};
+    int64_t tile0[MAX_SPANS + 1] = {};
+};
+
+__device__ __forceinline__ bool span_tile(const Spans& sp, int64_t x, int tile, int64_t& row0, int64_t& rend) {
+    if (x >= sp.tile0[sp.n]) return false;
+    int k = 0;
+    while (k + 1 < sp.n && x >= sp.tile0[k + 1]) ++k;
+    row0 = sp.start[k] + (x - sp.tile0[k]) * tile;
+    rend = sp.start[k] + sp.len[k];
+    return true;
+}
+
+// Power of two at or above a non-negative value's magnitude (1 for 0), so
+// that value / scale lies in [0, 1).
+__device__ __forceinline__ float pow2_above(float mx) {
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);
+    return ldexpf(1.f, e);
+}
+
+// Forward scales: s[i] = pow2_above(max_j |L(i, k0 + j)|, j < SOLVE_GROUP NB)
+// for the rows i of the spans (64-row tiles; 256 threads = 64 rows x 4
+// quarters of the group's columns).
+__global__ void __launch_bounds__(256) k_span_row_scale(const float* __restrict__ L, int64_t lda, int64_t k0, Spans sp,
+                                                        float* __restrict__ s) {
+    __shared__ float red[4][64];
+    constexpr int GW = SOLVE_GROUP * FACT_NB;
+    int64_t row0, rend;
+    if (!span_tile(sp, blockIdx.x, 64, row0, rend)) return;
+    const int ri = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t i = row0 + ri;
+    float mx = 0.f;
+    if (i < rend) {
+        const float* p = L + i + (k0 + (int64_t)q * (GW / 4)) * lda;
+#pragma unroll 8
+        for (int j = 0; j < GW / 4; ++j) mx = fmaxf(mx, fabsf(p[(int64_t)j * lda]));
+    }
+    red[q][ri] = mx;
+    __syncthreads();
+    if (q == 0 && i < rend) s[i] = pow2_above(fmaxf(fmaxf(red[0][ri], red[1][ri]), fmaxf(red[2][ri], red[3][ri])));
+}
+
+// Backward scales: s[j] = pow2_above(max_i |L(k0 + i, j)|, i < SOLVE_GROUP NB)
+// for the columns j of the spans (4-column tiles, one wave per column; L's
+// leading dimension and k0 are multiples of 4 and L is 16-byte aligned).
+__global__ void __launch_bounds__(256) k_span_col_scale(const float* __restrict__ L, int64_t lda, int64_t k0, Spans sp,
+                                                        float* __restrict__ s) {
+    constexpr int GW = SOLVE_GROUP * FACT_NB;
+    int64_t col0, cend;
+    if (!span_tile(sp, blockIdx.x, 4, col0, cend)) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = col0 + (threadIdx.x >> 6);
+    if (j >= cend) return;
+    const float4_t* p = reinterpret_cast<const float4_t*>(L + k0 + j * lda);
+    float mx = 0.f;
+#pragma unroll
+    for (int u = 0; u < GW / 256; ++u) {
+        const float4_t v = p[lane + 64 * u];
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) s[j] = pow2_above(mx);
+}
+
+// Split-FP16 product of the triangular solves:
+//   W(row, c) -= s[row] tw[c] sum_k A(row, k) / s[row] . (h + l)(k, c)
+// over k < SOLVE_GROUP NB, for the rows of the spans (128-row tiles) and the
+// columns c < r (128-column tiles), with
+//   forward (TRANS = false): A(row, k) = L(row, k0 + k), the group's columns
+//     of the rows below it, s the row scales;
+//   backward (TRANS = true): A(row, k) = L(k0 + k, row), the group's rows of
+//     the earlier columns, s the column scales;
+// and (h + l) = W(k0 + k, c) / tw[c] the group's split rows of W.  A / s is
+// split into FP16 h + l as it is staged into LDS; the three products
+// Al Bh + Ah Bl + Ah Bh are summed in FP32 by MFMA.  256 threads, 2 x 2 waves
+// of 64 x 64, k-slices of 32 double-buffered in LDS (the XOR swizzle of the
+// split factor's update), operands staged through registers one slice
+// ahead.  Workgroups go to the XCDs in runs of consecutive tiles (the tiles
+// of one row tile share its A rows).
+template <bool TRANS>
+__global__ void __launch_bounds__(256) k_solve_split(const float* __restrict__ L, int64_t lda, int64_t k0, Spans sp,
+                                                     int64_t nct, int64_t per, const float* __restrict__ s,
+                                                     WSplit ws, float* __restrict__ W, int64_t ldw, int64_t r) {
+    __shared__ half8_t S[2][4][S_TILE * 4];
+    constexpr int NKS = SOLVE_GROUP * FACT_NB / SK;
+    const int64_t t = xcd_tile(blockIdx.x, per);
+    int64_t row0, rend;
+    if (!span_tile(sp, t / nct, S_TILE, row0, rend)) return;
+    const int64_t c0 = (t % nct) * S_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // A staging: forward, row tid % 128 and k in [16 (tid / 128), + 16);
+    // backward, row tid / 2 and k in [16 (tid % 2), + 16)
+    const int ar = TRANS ? tid >> 1 : tid & 127, akh = TRANS ? tid & 1 : tid >> 7;
+    const int64_t arow = row0 + ar;
+    const bool a_in = arow < rend;
+    const float a_inv = a_in ? 1.f / s[arow] : 0.f;   // exact: s is a power of two
+    float av[16];
+    half8_t bv[2][2];
+    auto gload = [&](int ks) {
+        const int64_t kb = k0 + (int64_t)ks * SK + 16 * akh;
+        if (TRANS) {
+            const float4_t* p = reinterpret_cast<const float4_t*>(L + kb + arow * lda);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float4_t v = a_in ? p[u] : float4_t{0.f, 0.f, 0.f, 0.f};
+                av[4 * u] = v[0];
+                av[4 * u + 1] = v[1];
+                av[4 * u + 2] = v[2];
+                av[4 * u + 3] = v[3];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) av[u] = a_in ? L[arow + (kb + u) * lda] : 0.f;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int q = tid + 256 * p, cr = q >> 2, ch = q & 3;
+                const _Float16* src = a == 0 ? ws.h : ws.l;
+                bv[a][p] = *reinterpret_cast<const half8_t*>(src + (((int64_t)ks * ws.rp + c0 + cr) * SK + 8 * ch));
+            }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            half8_t h, l;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float x = av[8 * c + e] * a_inv;
+                const _Float16 hx = (_Float16)x;
+                h[e] = hx;
+                l[e] = (_Float16)(x - (float)hx);
+            }
+            S[buf][0][swz(ar, 2 * akh + c)] = h;
+            S[buf][1][swz(ar, 2 * akh + c)] = l;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int q = tid + 256 * p, cr = q >> 2, ch = q & 3;
+                S[buf][2 + a][swz(cr, ch)] = bv[a][p];
+            }
+    };
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int fr = lane & 15, fc = lane >> 4;
+    for (int ks = 0; ks < NKS; ++ks) {
+        const int buf = ks & 1;
+        if (ks + 1 < NKS) gload(ks + 1);
+        half8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            ah[a] = S[buf][0][swz(wm + 16 * a + fr, fc)];
+            al[a] = S[buf][1][swz(wm + 16 * a + fr, fc)];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bh[b] = S[buf][2][swz(wn + 16 * b + fr, fc)];
+            bl[b] = S[buf][3][swz(wn + 16 * b + fr, fc)];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        if (ks + 1 < NKS) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    // epilogue: lane holds rows 4 (lane / 16) + e of column lane % 16 of a block
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int64_t c = c0 + wn + 16 * b + fr;
+        if (c >= r) continue;
+        const float tc = ws.tw[c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = row0 + wm + 16 * a + 4 * fc + e;
+                if (i < rend) W[i + c * ldw] -= s[i] * tc * acc[a][b][e];
+            }
+    }
 }
 }  // namespace
 
+// Buffers of the split-FP16 solve products (float solves): the split rows of
+// the current group of W and one scale per row or column of L.
+struct SplitSolveBuf {
+    WSplit ws;
+    float* s = nullptr;
+};
+
+size_t split_solve_bytes(int64_t n, int64_t r) {
+    const size_t rp = (size_t)cdiv(r, S_TILE) * S_TILE;
+    return 2 * (size_t)SOLVE_GROUP * FACT_NB * rp * sizeof(_Float16) + rp * sizeof(float) + (size_t)n * sizeof(float) +
+           3 * 256;
+}
+
+static SplitSolveBuf split_solve_buffers(void* p, int64_t n, int64_t r) {
+    SplitSolveBuf b;
+    const size_t rp = (size_t)cdiv(r, S_TILE) * S_TILE;
+    char* q = (char*)p;
+    b.ws.rp = (int64_t)rp;
+    b.ws.h = (_Float16*)q;
+    q += (size_t)SOLVE_GROUP * FACT_NB * rp * sizeof(_Float16);
+    b.ws.l = (_Float16*)q;
+    q += (size_t)SOLVE_GROUP * FACT_NB * rp * sizeof(_Float16);
+    b.ws.tw = (float*)q;
+    q += (rp * sizeof(float) + 255) / 256 * 256;
+    b.s = (float*)q;
+    return b;
+}
+
+static Spans make_spans(const std::vector<std::pair<int64_t, int64_t>>& iv, int tile) {
+    Spans sp;
+    sp.n = (int)iv.size();
+    for (int k = 0; k < sp.n; ++k) {
+        sp.start[k] = iv[k].first;
+        sp.len[k] = iv[k].second - iv[k].first;
+        sp.tile0[k + 1] = sp.tile0[k] + cdiv(sp.len[k], tile);
+    }
+    return sp;
+}
+
+// W -= (split product of the group [k0, k0 + SOLVE_GROUP NB)) on the rows
+// (forward: rows below the group, TRANS = false) or columns (backward:
+// columns before it, TRANS = true) listed in iv.
+static void split_solve_update(bool trans, const float* L, int64_t lda, int64_t k0,
+                               const std::vector<std::pair<int64_t, int64_t>>& iv, const SplitSolveBuf& sb, float* W,
+                               int64_t ldw, int64_t r, hipStream_t st) {
+    for (size_t p = 0; p < iv.size(); p += MAX_SPANS) {
+        const std::vector<std::pair<int64_t, int64_t>> part(iv.begin() + p,
+                                                             iv.begin() + std::min(iv.size(), p + MAX_SPANS));
+        int64_t rows = 0;
+        for (const auto& v : part) rows += v.second - v.first;
+        if (rows <= 0) continue;
+        const Spans sc = make_spans(part, trans ? 4 : 64);
+        if (trans) k_span_col_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, sb.s);
+        else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, sb.s);
+        OMH_LAUNCH_CHECK();
+        const Spans sp = make_spans(part, S_TILE);
+        const int64_t nct = cdiv(r, S_TILE), total = sp.tile0[sp.n] * nct, per = cdiv(total, 8);
+        ProfScope ps("solve_update_f32s", 2.0 * (double)rows * (double)r * (double)(SOLVE_GROUP * FACT_NB), st);
+        if (trans) k_solve_split<true><<<(unsigned)(8 * per), 256, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W, ldw, r);
+        else k_solve_split<false><<<(unsigned)(8 * per), 256, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W, ldw, r);
+        OMH_LAUNCH_CHECK();
+    }
+}
+
 template <typename T>
-void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st,
-               const BlockFill* bf) {
+static void sym_solve_impl(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r,
+                           hipStream_t st, const BlockFill* bf, const SplitSolveBuf* sb) {
     if (r <= 0) return;
     const T* Dinv = (const T*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
@@ -2093,31 +2444,38 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     // product of depth SOLVE_GROUP x NB.
     constexpr int64_t GW = (int64_t)SOLVE_GROUP * FACT_NB;
     std::vector<std::pair<int64_t, int64_t>> iv;
+    // The split products need 16-byte aligned columns of L (backward) and
+    // serve the full groups; the others take the FP32 products.
+    const bool split_ok = sb && sizeof(T) == 4 && lda % 4 == 0 && reinterpret_cast<uintptr_t>(L) % 16 == 0;
     // strips of 2 right-hand sides up to 1024 of them (128 workgroups for
     // 256; 8-wide strips took twice as long per group at 256), else 8
     const bool narrow = r <= 1024;
-    auto group_solve = [&](bool back, int64_t k0, int64_t gend) {
+    auto group_solve = [&](bool back, int64_t k0, int64_t gend, bool split) {
+        WSplit ws;
+        if (split) ws = sb->ws;
         if (narrow) {
             const unsigned g = (unsigned)cdiv(r, 2);
-            if (back) k_solve_group<T, true, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
-            else k_solve_group<T, false, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+            if (back) k_solve_group<T, true, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r, ws);
+            else k_solve_group<T, false, 2><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r, ws);
         } else {
             const unsigned g = (unsigned)cdiv(r, 8);
-            if (back) k_solve_group<T, true, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
-            else k_solve_group<T, false, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r);
+            if (back) k_solve_group<T, true, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r, ws);
+            else k_solve_group<T, false, 8><<<g, 256, 0, st>>>(L, lda, Dinv, k0, gend, W, ldw, r, ws);
         }
         OMH_LAUNCH_CHECK();
     };
     // forward: W <- L^-1 W
     for (int64_t k0 = 0; k0 < n; k0 += GW) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
-        group_solve(false, k0, gend);
+        const bool split = split_ok && gend - k0 == GW && gend < n;
+        group_solve(false, k0, gend, split);
         // rows below the group: with a block structure only those its
         // columns reach in L
         if (gend < n) {
-            if (!bf || !active_rows(*bf, n, k0, gend, gend, iv))
-                gemm<T>(false, false, false, n - gend, r, gend - k0, T(-1), L + gend + k0 * lda, lda, W + k0, ldw,
-                        T(1), W + gend, ldw, st);
+            const bool some = bf && active_rows(*bf, n, k0, gend, gend, iv);
+            if (!some) iv.assign(1, {gend, n});
+            if (split)
+                split_solve_update(false, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st);
             else
                 for (const auto& v : iv)
                     gemm<T>(false, false, false, v.second - v.first, r, gend - k0, T(-1), L + v.first + k0 * lda,
@@ -2131,17 +2489,26 @@ void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64
     // groups' contributions)
     for (int64_t k0 = ((n - 1) / GW) * GW; k0 >= 0; k0 -= GW) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
-        group_solve(true, k0, gend);
+        const bool split = split_ok && gend - k0 == GW && k0 > 0;
+        group_solve(true, k0, gend, split);
         // rows above the group: only the columns its rows reach in L
         if (k0 > 0) {
-            if (!bf || !active_cols(*bf, k0, gend, k0, iv))
-                gemm<T>(true, false, false, k0, r, gend - k0, T(-1), L + k0, lda, W + k0, ldw, T(1), W, ldw, st);
+            const bool some = bf && active_cols(*bf, k0, gend, k0, iv);
+            if (!some) iv.assign(1, {0, k0});
+            if (split)
+                split_solve_update(true, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st);
             else
                 for (const auto& v : iv)
                     gemm<T>(true, false, false, v.second - v.first, r, gend - k0, T(-1), L + k0 + v.first * lda, lda,
                             W + k0, ldw, T(1), W + v.first, ldw, st);
         }
     }
+}
+
+template <typename T>
+void sym_solve(const T* L, int64_t n, int64_t lda, const void* work, T* W, int64_t ldw, int64_t r, hipStream_t st,
+               const BlockFill* bf) {
+    sym_solve_impl<T>(L, n, lda, work, W, ldw, r, st, bf, nullptr);
 }
 
 template void sym_solve<double>(const double*, int64_t, int64_t, const void*, double*, int64_t, int64_t, hipStream_t,
@@ -2260,6 +2627,7 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
     buf.R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r, st);
     buf.R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r, st);
     buf.part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
+    buf.split = scratch(SCR_LINALG2, split_solve_bytes(n, r), st);
     return mixed_refine_in(H, n, ldh, L32, ld32, work32, B, ldb, X, ldx, r, max_iter, tol, final_rel_update, converged,
                            buf, st, bf);
 }
@@ -2270,6 +2638,8 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
                     const BlockFill* bf) {
     float* R32 = buf.R32;
     double* R = buf.R;
+    SplitSolveBuf sb;
+    if (buf.split) sb = split_solve_buffers(buf.split, n, r);
     fill_f64(X, n, r, ldx, 0.0, st);
     copy_f64(B, ldb, R, n, n, r, st);
     int it = 0;
@@ -2277,7 +2647,7 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
     RefineState state = REFINE_CONTINUE;
     for (; it < max_iter; ++it) {
         convert_f64_to_f32(R, n, R32, n, n, r, false, st);
-        sym_solve<float>(L32, n, ld32, work32, R32, n, r, st, bf);
+        sym_solve_impl<float>(L32, n, ld32, work32, R32, n, r, st, bf, buf.split ? &sb : nullptr);
         k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
         OMH_LAUNCH_CHECK();
         const double dn = frob_norm_with<float>(R32, n, n, r, buf.part, st);

@@ -191,7 +191,11 @@ RefineState refine_state(int it, double rel, double prev, double tol);
 // Iterative refinement: X = H^-1 B with the FP32 factor (L32, work32) and
 // FP64 residuals R = B - H X.  Stops by refine_state or after max_iter
 // corrections; returns the number of corrections applied and sets
-// *converged (REFINE_CONVERGED reached).
+// *converged (REFINE_CONVERGED reached).  The triangular solves run the
+// products of their full 1024-column groups as split-FP16 MFMA: the FP32
+// factor's tiles over a power-of-two scale per row (forward) or column
+// (backward) and the group's rows of W over one per column, each split into
+// FP16 h + l, three products summed in FP32.
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
                  double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf = nullptr);
@@ -203,7 +207,12 @@ struct RefineBuffers {
     float* R32 = nullptr;
     double* R = nullptr;
     double* part = nullptr;
+    // optional: buffers of the split-FP16 products of the FP32 triangular
+    // solves; without them the solves use FP32 products
+    void* split = nullptr;
 };
+// Bytes of RefineBuffers::split for n unknowns and r right-hand sides.
+size_t split_solve_bytes(int64_t n, int64_t r);
 // mixed_refine with caller-given buffers (allocates nothing on the device;
 // with a block structure the residual's side streams are created on first
 // use).

@@ -199,6 +199,87 @@ static V3 integrate_D(const P1S* o, const Tri* t1, int order) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Adaptive integration, integrator.h:44-93                             */
+/* ------------------------------------------------------------------ */
+typedef struct { V3 r0, q; } Dip;     /* dipole.h:18-52 */
+
+static double dip_potential(const Dip* d, V3 r) {     /* dipole.h:29-35 */
+    V3 x = sub(r, d->r0);
+    double n2 = nrm2(x);
+    return dot(d->q, x) / (n2 * sqrt(n2));
+}
+static double dip_gradient(const Dip* d, V3 r, V3 n) { /* dipole.h:38-46 */
+    V3 x = sub(r, d->r0);
+    double inv = 1.0 / nrm2(x);
+    V3 gr = sub(d->q, scl(3 * dot(d->q, x) * inv, x));
+    return dot(gr, n) * (inv * sqrt(inv));
+}
+
+/* BarycentricCoordinates, BarycentricCoordinates.h:30-70 */
+typedef struct { V3 p0, e10, e20; double d11, d12, d22; } Bary;
+static void bary_init(Bary* b, const Tri* t) {
+    b->p0 = t->p[0];
+    b->e10 = sub(t->p[1], t->p[0]);
+    b->e20 = sub(t->p[2], t->p[0]);
+    double c11 = dot(b->e10, b->e10), c12 = dot(b->e10, b->e20), c22 = dot(b->e20, b->e20);
+    double inv = 1.0 / (c11 * c22 - c12 * c12);
+    b->d11 = c22 * inv; b->d12 = -c12 * inv; b->d22 = c11 * inv;
+}
+static V3 bary_eval(const Bary* b, V3 r) {
+    V3 v = sub(r, b->p0);
+    double d1 = dot(v, b->e10), d2 = dot(v, b->e20);
+    double l1 = b->d11 * d1 + b->d12 * d2;
+    double l2 = b->d12 * d1 + b->d22 * d2;
+    return v3(1.0 - l1 - l2, l1, l2);
+}
+
+/* The integrand: kind 0 = Dipole::potential (scalar in .x),
+ * kind 1 = PotentialDerivative (analytics.h:132-158), kind 2 = OperatorS of
+ * a triangle (scalar, the S pair integral), kind 3 = P1SingleLayerPotential
+ * of a triangle (the D pair integral). */
+typedef struct { int kind; Dip dip; Bary bary; V3 n; const OpS* ops; const P1S* p1s; } Fn;
+static V3 fn_eval(const Fn* f, V3 r) {
+    if (f->kind == 0) return v3(dip_potential(&f->dip, r), 0, 0);
+    if (f->kind == 2) return v3(ops_eval(f->ops, r), 0, 0);
+    if (f->kind == 3) return p1s_eval(f->p1s, r);
+    V3 P1 = bary_eval(&f->bary, r);
+    double g = dip_gradient(&f->dip, r, f->n);
+    return scl(g, P1);
+}
+static double fn_norm(int kind, V3 a) { return (kind == 0 || kind == 2) ? fabs(a.x) : nrm(a); }
+
+static V3 tri_integration(const Fn* f, int order, const V3 P[3], double area) {
+    V3 res = v3(0, 0, 0);
+    for (int i = 0; i < nb_pts[order]; ++i)
+        res = add(res, scl(rules[order][i][3], fn_eval(f, gauss_point(order, i, P))));
+    return scl(area, res);
+}
+
+static V3 adaptive(const Fn* f, int order, double tol, const V3 P[3], double area,
+                   V3 coarse, unsigned level) {
+    V3 mid[3] = {scl(0.5, add(P[1], P[2])), scl(0.5, add(P[2], P[0])), scl(0.5, add(P[0], P[1]))};
+    V3 nt[4][3] = {{P[0], mid[1], mid[2]}, {mid[0], P[1], mid[2]},
+                   {mid[0], mid[1], P[2]}, {mid[0], mid[1], mid[2]}};
+    double sa = 0.25 * area;
+    V3 refined = v3(0, 0, 0), integ[4];
+    for (int i = 0; i < 4; ++i) {
+        integ[i] = tri_integration(f, order, nt[i], sa);
+        refined = add(refined, integ[i]);
+    }
+    if (fn_norm(f->kind, sub(coarse, refined)) <= tol * fn_norm(f->kind, coarse) || level == 0)
+        return refined;
+    V3 sum = v3(0, 0, 0);
+    for (int i = 0; i < 4; ++i)
+        sum = add(sum, adaptive(f, order, tol, nt[i], sa, integ[i], level - 1));
+    return sum;
+}
+
+static V3 integrate_fn(const Fn* f, int order, unsigned levels, double tol, const Tri* t) {
+    V3 coarse = tri_integration(f, order, t->p, t->area);
+    return levels == 0 ? coarse : adaptive(f, order, tol, t->p, t->area, coarse, levels);
+}
+
+/* ------------------------------------------------------------------ */
 /* Geometry tables handed over from oracle/geometry.py                  */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -235,7 +316,7 @@ static inline i64 pidx(i64 i, i64 j) { /* symmatrix.h:52-62 */
  * the HeadMat entry (tri_index(t1), tri_index(t2)); otherwise into the dense
  * temporary block `tmp` (row t1-local, column t2-local, column-major with
  * leading dimension n1) -- the SymBloc/Bloc temporaries of operators.h:159-173,337-352. */
-static void s_block(const Geom* g, i64 m1, i64 m2, int order, double coeff,
+static void s_block(const Geom* g, i64 m1, i64 m2, int order, unsigned levels, double tol, double coeff,
                     double* dst_packed, double* tmp) {
     const i64 a0 = g->tri_off[m1], a1 = g->tri_off[m1 + 1];
     const i64 b0 = g->tri_off[m2], b1 = g->tri_off[m2 + 1];
@@ -247,7 +328,15 @@ static void s_block(const Geom* g, i64 m1, i64 m2, int order, double coeff,
         OpS o; ops_init(&o, &t1);
         for (i64 j = diag ? i : b0; j < b1; ++j) {
             Tri t2 = geom_tri(g, j);
-            double v = integrate_S(&o, &t2, order) * coeff;
+            double v;
+            if (levels == 0) {
+                v = integrate_S(&o, &t2, order) * coeff;
+            } else {   /* Integrator::integrate with subdivision, integrator.h:44-93 */
+                Fn f = {0};
+                f.kind = 2;
+                f.ops = &o;
+                v = integrate_fn(&f, order, levels, tol, &t2).x * coeff;
+            }
             if (dst_packed) dst_packed[pidx(g->tri_index[i], g->tri_index[j])] = v;
             else {
                 tmp[(i - a0) + (j - b0) * n1] = v;
@@ -259,7 +348,8 @@ static void s_block(const Geom* g, i64 m1, i64 m2, int order, double coeff,
 
 /* ------------------- D blocks, operators.h:84-115 ---------------- */
 /* H(tri_index(t1), vert_index(vertex(t2,k))) += coeff * int_{t1} P1SLP(t2)_k */
-static void d_block(const Geom* g, i64 mrow, i64 mcol, int order, double coeff, double* H) {
+static void d_block(const Geom* g, i64 mrow, i64 mcol, int order, unsigned levels, double tol, double coeff,
+                    double* H) {
     const i64 a0 = g->tri_off[mrow], a1 = g->tri_off[mrow + 1];
     const i64 b0 = g->tri_off[mcol], b1 = g->tri_off[mcol + 1];
     /* Each row t1 is owned by one thread; within a row the t2 loop runs in
@@ -271,7 +361,15 @@ static void d_block(const Geom* g, i64 mrow, i64 mcol, int order, double coeff, 
         for (i64 j = b0; j < b1; ++j) {
             Tri t2 = geom_tri(g, j);
             P1S o; p1s_init(&o, &t2);
-            V3 tot = integrate_D(&o, &t1, order);
+            V3 tot;
+            if (levels == 0) {
+                tot = integrate_D(&o, &t1, order);
+            } else {
+                Fn f = {0};
+                f.kind = 3;
+                f.p1s = &o;
+                tot = integrate_fn(&f, order, levels, tol, &t1);
+            }
             double c[3] = {tot.x, tot.y, tot.z};
             for (int k = 0; k < 3; ++k)
                 H[pidx(row, g->vert_index[g->tris[3 * j + k]])] += c[k] * coeff;
@@ -340,14 +438,14 @@ static void n_block(const Geom* g, i64 m1, i64 m2, double coeff, const double* H
 /* One communicating mesh pair: HeadMatrixBlocks::set_blocks,
  * operators.h:517-525, with the block rules of DiagonalBlock /
  * NonDiagonalBlock (operators.h:180-197, 363-384). */
-int orc_mesh_pair(const Geom* g, i64 m1, i64 m2, int order,
+int orc_mesh_pair(const Geom* g, i64 m1, i64 m2, int order, unsigned levels, double tol,
                   double cS, double cN, double cD, double* H) {
     const int b1 = (int)g->barrier[m1], b2 = (int)g->barrier[m2];
     const int same = (m1 == m2);
     double Scoeff = 0.0;
     /* set_S_block */
     if (same ? !b1 : (!b1 && !b2)) {
-        s_block(g, m1, m2, order, cS, H, NULL);
+        s_block(g, m1, m2, order, levels, tol, cS, H, NULL);
         Scoeff = cS;
     }
     /* set_N_block */
@@ -358,14 +456,14 @@ int orc_mesh_pair(const Geom* g, i64 m1, i64 m2, int order,
         const i64 n2 = g->tri_off[m2 + 1] - g->tri_off[m2];
         double* tmp = (double*)calloc((size_t)(n1 * n2), sizeof(double));
         if (!tmp) return -1;
-        s_block(g, m1, m2, order, 1.0, NULL, tmp);
+        s_block(g, m1, m2, order, levels, tol, 1.0, NULL, tmp);
         n_block(g, m1, m2, cN, NULL, 1.0, tmp, H);
         free(tmp);
     }
     /* set_D_block */
-    if (!b1) d_block(g, m1, m2, order, cD, H);
+    if (!b1) d_block(g, m1, m2, order, levels, tol, cD, H);
     /* set_Dstar_block */
-    if (!same && !b2) d_block(g, m2, m1, order, cD, H);
+    if (!same && !b2) d_block(g, m2, m1, order, levels, tol, cD, H);
     return 0;
 }
 
@@ -379,83 +477,6 @@ void orc_deflate_mesh(const Geom* g, i64 m, double coef, double* H) {
 }
 
 double orc_packed_get(const double* H, i64 i, i64 j) { return H[pidx(i, j)]; }
-
-/* ------------------------------------------------------------------ */
-/* Adaptive integration, integrator.h:44-93                             */
-/* ------------------------------------------------------------------ */
-typedef struct { V3 r0, q; } Dip;     /* dipole.h:18-52 */
-
-static double dip_potential(const Dip* d, V3 r) {     /* dipole.h:29-35 */
-    V3 x = sub(r, d->r0);
-    double n2 = nrm2(x);
-    return dot(d->q, x) / (n2 * sqrt(n2));
-}
-static double dip_gradient(const Dip* d, V3 r, V3 n) { /* dipole.h:38-46 */
-    V3 x = sub(r, d->r0);
-    double inv = 1.0 / nrm2(x);
-    V3 gr = sub(d->q, scl(3 * dot(d->q, x) * inv, x));
-    return dot(gr, n) * (inv * sqrt(inv));
-}
-
-/* BarycentricCoordinates, BarycentricCoordinates.h:30-70 */
-typedef struct { V3 p0, e10, e20; double d11, d12, d22; } Bary;
-static void bary_init(Bary* b, const Tri* t) {
-    b->p0 = t->p[0];
-    b->e10 = sub(t->p[1], t->p[0]);
-    b->e20 = sub(t->p[2], t->p[0]);
-    double c11 = dot(b->e10, b->e10), c12 = dot(b->e10, b->e20), c22 = dot(b->e20, b->e20);
-    double inv = 1.0 / (c11 * c22 - c12 * c12);
-    b->d11 = c22 * inv; b->d12 = -c12 * inv; b->d22 = c11 * inv;
-}
-static V3 bary_eval(const Bary* b, V3 r) {
-    V3 v = sub(r, b->p0);
-    double d1 = dot(v, b->e10), d2 = dot(v, b->e20);
-    double l1 = b->d11 * d1 + b->d12 * d2;
-    double l2 = b->d12 * d1 + b->d22 * d2;
-    return v3(1.0 - l1 - l2, l1, l2);
-}
-
-/* The integrand: kind 0 = Dipole::potential (scalar in .x),
- * kind 1 = PotentialDerivative (analytics.h:132-158). */
-typedef struct { int kind; Dip dip; Bary bary; V3 n; } Fn;
-static V3 fn_eval(const Fn* f, V3 r) {
-    if (f->kind == 0) return v3(dip_potential(&f->dip, r), 0, 0);
-    V3 P1 = bary_eval(&f->bary, r);
-    double g = dip_gradient(&f->dip, r, f->n);
-    return scl(g, P1);
-}
-static double fn_norm(int kind, V3 a) { return kind == 0 ? fabs(a.x) : nrm(a); }
-
-static V3 tri_integration(const Fn* f, int order, const V3 P[3], double area) {
-    V3 res = v3(0, 0, 0);
-    for (int i = 0; i < nb_pts[order]; ++i)
-        res = add(res, scl(rules[order][i][3], fn_eval(f, gauss_point(order, i, P))));
-    return scl(area, res);
-}
-
-static V3 adaptive(const Fn* f, int order, double tol, const V3 P[3], double area,
-                   V3 coarse, unsigned level) {
-    V3 mid[3] = {scl(0.5, add(P[1], P[2])), scl(0.5, add(P[2], P[0])), scl(0.5, add(P[0], P[1]))};
-    V3 nt[4][3] = {{P[0], mid[1], mid[2]}, {mid[0], P[1], mid[2]},
-                   {mid[0], mid[1], P[2]}, {mid[0], mid[1], mid[2]}};
-    double sa = 0.25 * area;
-    V3 refined = v3(0, 0, 0), integ[4];
-    for (int i = 0; i < 4; ++i) {
-        integ[i] = tri_integration(f, order, nt[i], sa);
-        refined = add(refined, integ[i]);
-    }
-    if (fn_norm(f->kind, sub(coarse, refined)) <= tol * fn_norm(f->kind, coarse) || level == 0)
-        return refined;
-    V3 sum = v3(0, 0, 0);
-    for (int i = 0; i < 4; ++i)
-        sum = add(sum, adaptive(f, order, tol, nt[i], sa, integ[i], level - 1));
-    return sum;
-}
-
-static V3 integrate_fn(const Fn* f, int order, unsigned levels, double tol, const Tri* t) {
-    V3 coarse = tri_integration(f, order, t->p, t->area);
-    return levels == 0 ? coarse : adaptive(f, order, tol, t->p, t->area, coarse, levels);
-}
 
 /* One DSM column contribution of one mesh (assembleSourceMat.cpp:84-99):
  * operatorPotentialDerivative (operators.cpp:63-88) into the vertex rows and,
@@ -599,6 +620,29 @@ double orc_integrate_s_pair(const double* tri_a, const double* tri_b, int order)
     Tri b = make_tri(v3(tri_b[0], tri_b[1], tri_b[2]), v3(tri_b[3], tri_b[4], tri_b[5]), v3(tri_b[6], tri_b[7], tri_b[8]));
     OpS o; ops_init(&o, &a);
     return integrate_S(&o, &b, order);
+}
+
+/* The adaptive pair integrals of a HeadMat with Integrator(order, levels,
+ * tol): kind 2 = the S integral of OperatorS(a) over b (out3[0]); kind 3 =
+ * the D integral over a of P1SLP(b) (out3[0..2]). */
+void orc_integrate_pair_adaptive(const double* tri_a, const double* tri_b, int kind, int order, unsigned levels,
+                                 double tol, double* out3) {
+    Tri a = make_tri(v3(tri_a[0], tri_a[1], tri_a[2]), v3(tri_a[3], tri_a[4], tri_a[5]), v3(tri_a[6], tri_a[7], tri_a[8]));
+    Tri b = make_tri(v3(tri_b[0], tri_b[1], tri_b[2]), v3(tri_b[3], tri_b[4], tri_b[5]), v3(tri_b[6], tri_b[7], tri_b[8]));
+    OpS o; P1S p;
+    Fn f = {0};
+    f.kind = kind;
+    V3 r;
+    if (kind == 2) {
+        ops_init(&o, &a);
+        f.ops = &o;
+        r = integrate_fn(&f, order, levels, tol, &b);
+    } else {
+        p1s_init(&p, &b);
+        f.p1s = &p;
+        r = integrate_fn(&f, order, levels, tol, &a);
+    }
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
 }
 
 /* Sample of the HeadMat work for the CPU-baseline timing: rows [r0, r1) of

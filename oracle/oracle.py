@@ -48,7 +48,7 @@ def lib():
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
-        L.orc_mesh_pair.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, C.c_double,
+        L.orc_mesh_pair.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, C.c_uint, C.c_double, C.c_double,
                                     C.c_double, C.c_double, dp]
         L.orc_mesh_pair.restype = C.c_int
         L.orc_deflate_mesh.argtypes = [C.POINTER(_Geom), i64, C.c_double, dp]
@@ -67,6 +67,7 @@ def lib():
         L.orc_solid_angle.restype = C.c_double
         L.orc_integrate_d_pair.argtypes = [dp, dp, C.c_int, dp]
         L.orc_surf2vol_mesh.argtypes = [C.POINTER(_Geom), i64, dp, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_integrate_pair_adaptive.argtypes = [dp, dp, C.c_int, C.c_int, C.c_uint, C.c_double, dp]
         L.orc_integrate_s_pair.argtypes = [dp, dp, C.c_int]
         L.orc_integrate_s_pair.restype = C.c_double
         L.orc_sample_rows.argtypes = [C.POINTER(_Geom), i64, i64, C.c_int, i64, i64, C.c_int, dp]
@@ -169,15 +170,17 @@ def nested_geometry(meshes, conductivity):
 # ---------------------------------------------------------------------------
 # HeadMat
 # ---------------------------------------------------------------------------
-def head_mat(g, order=3):
-    """Details::HeadMatrix + deflate (assembleHeadMat.cpp:28-60,74-115).
-    Returns the packed upper SymMatrix data (length n(n+1)/2)."""
+def head_mat(g, order=3, levels=0, tol=0.0):
+    """Details::HeadMatrix + deflate (assembleHeadMat.cpp:28-60,74-115) with
+    Integrator(order, levels, tol) (levels > 0: adaptive pair integrals,
+    integrator.h:44-93).  Returns the packed upper SymMatrix data (length
+    n(n+1)/2)."""
     t = tables(g)
     n = g.size
     H = np.zeros(n * (n + 1) // 2, dtype=np.float64)
     L = lib()
     for (m1, m2, o, cS, cN, cD) in g.pairs:
-        rc = L.orc_mesh_pair(C.byref(t.s), m1, m2, order, cS, cN, cD, _p(H))
+        rc = L.orc_mesh_pair(C.byref(t.s), m1, m2, order, levels, tol, cS, cN, cD, _p(H))
         if rc != 0:
             raise MemoryError("oracle mesh pair")
     for part in g.parts:
@@ -495,7 +498,7 @@ def surf_source_mat(g, src_pts, src_tris, order=3, stats=None):
             # the source mesh is a barrier: S goes to a temporary, N with
             # coeffN, D(head mesh rows, source columns) unless the head mesh
             # is a barrier, no D* (orc_mesh_pair's branches)
-            if L.orc_mesh_pair(C.byref(t.s), mid, M, order, 0.0, coeffN, coeffN * Lc, _p(H)) != 0:
+            if L.orc_mesh_pair(C.byref(t.s), mid, M, order, 0, 0.0, 0.0, coeffN, coeffN * Lc, _p(H)) != 0:
                 raise MemoryError("oracle mesh pair")
     if stats is not None:
         stats["pair_seconds"] = time.perf_counter() - t0

@@ -435,6 +435,56 @@ def test_split_factor_slice_clock(om, monkeypatch):
     assert dL < 1e-5 and dX < 1e-12
 
 
+def test_split_solve_products(om):
+    """The mixed refinement's triangular solves run the products of their
+    full 1024-column groups as split-FP16 MFMA (k_solve_split: the FP32
+    factor over a power-of-two scale per row or column, W's group rows over
+    one per column, three FP16 products, FP32 sums).  One solve of the
+    refinement (max_iter 1: no correction yet) against the FP32-product solve
+    of the same factor (omh_sym_solve_f32), both against the refined
+    solution: the split solve's error is at most 4x the FP32 solve's (plus
+    1e-7), and the refined solution's residual is < 1e-12.  3 layers of 2562
+    vertices (n = 17926: 17 full groups and a partial one, whose products
+    stay FP32) and 200 right-hand sides (a full and a partial 128-column
+    tile)."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import spheres as S, _lib
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    n, r = g.size, 200
+    H = _head_mat_device(g, far_field=True)
+    F = factor(H, n, "mixed16")
+    B = torch.from_numpy(np.random.default_rng(21).standard_normal(n * r)).cuda()
+
+    def mixed(max_iter, tol):
+        X = torch.full((n * r,), np.nan, dtype=torch.float64, device="cuda")
+        it, rel = C.c_int32(), C.c_double()
+        rc = _lib.lib().omh_mixed_solve(C.c_void_p(H.data_ptr()), n, n, C.c_void_p(F.L.data_ptr()), n,
+                                        C.c_void_p(F.work.data_ptr()), C.c_void_p(B.data_ptr()), n,
+                                        C.c_void_p(X.data_ptr()), n, r, max_iter, tol, C.byref(it), C.byref(rel),
+                                        C.c_void_p(0))
+        assert rc in (0, _lib.STATUS_NOT_CONVERGED), _lib.lib().omh_last_error()
+        return X, rc, it.value
+
+    X1, rc1, it1 = mixed(1, 1e-12)          # one split solve, then stop
+    Xr, rcr, itr = mixed(10, 1e-12)          # refined
+    assert rc1 == _lib.STATUS_NOT_CONVERGED and rcr == 0
+    W = B.to(torch.float32).clone()
+    _lib.check(_lib.lib().omh_sym_solve_f32(C.c_void_p(F.L.data_ptr()), n, n, C.c_void_p(F.work.data_ptr()),
+                                            C.c_void_p(W.data_ptr()), n, r, C.c_void_p(0)))
+    torch.cuda.synchronize()
+    nr = torch.linalg.norm(Xr)
+    e_split = float(torch.linalg.norm(X1 - Xr) / nr)
+    e_fp32 = float(torch.linalg.norm(W.double() - Xr) / nr)
+    Hm = H.view(n, n)
+    res = float(torch.linalg.norm(B.view(r, n) - Xr.view(r, n) @ Hm) / torch.linalg.norm(B))
+    print("n=%d r=%d: one solve, error vs refined: split products %.3e, FP32 products %.3e; refined in %d "
+          "corrections, residual %.3e" % (n, r, e_split, e_fp32, itr, res), flush=True)
+    assert e_split <= 4 * e_fp32 + 1e-7
+    assert res < 1e-12
+
+
 def test_headmat_head1_matches_golden(om, head1):
     g, _, _ = head1
     hm = om.HeadMat(g)
@@ -617,3 +667,20 @@ def test_factor_breakdown_is_reported(om):
     _lib.check(_lib.lib().omh_sym_factor_f64(C.c_void_p(A.data_ptr()), n, n, C.c_void_p(work.data_ptr()),
                                              C.byref(info), C.c_void_p(0)))
     assert info.value == 1
+
+
+@pytest.mark.parametrize("levels,tol", [(2, 1e-3), (1, 0.0)])
+def test_adaptive_headmat_matches_oracle(om, head1, levels, tol):
+    """HeadMat(geo, Integrator(3, levels, tol)) (assemble.h:32 with the
+    adaptive recursion of integrator.h:69-93 on every S and D pair
+    integral) on Head1 against the oracle's: 1e-13 relative Frobenius; it
+    differs from the non-adaptive HeadMat."""
+    from oracle import oracle as O
+    g, _, _ = head1
+    og = O.load_geometry(os.path.join(GOLDEN, "head1", "Head1.geom"), os.path.join(GOLDEN, "head1", "Head1.cond"))
+    ref = O.head_mat(og, 3, levels, tol)
+    hm = om.HeadMat(g, om.Integrator(3, levels, tol))
+    e = rel_fro(hm.data(), ref)
+    d0 = rel_fro(hm.data(), np.load(os.path.join(GOLDEN, "head1", "hm.npy")))
+    print("adaptive HeadMat (levels %d, tol %g) vs oracle %.3e; vs non-adaptive %.3e" % (levels, tol, e, d0), flush=True)
+    assert e < 1e-13 and d0 > 1e-9

@@ -454,3 +454,85 @@ def test_oracle_eit_reciprocity_head1():
     qg = (dip[:, 3:] * grad).sum(axis=1)
     rel = np.abs(qg - diff) / np.abs(diff)
     assert rel.max() < 0.01, rel
+
+
+def _composite(P, f, levels):
+    """The 16-point rule on the 4^levels sub-triangles of P (the midpoint
+    split of integrator.h:71-75) for a point function f(x) -> array."""
+    w = np.array([r[3] for r in _RULE16])
+    b = np.array([r[:3] for r in _RULE16])
+    tris = [np.asarray(P, dtype=np.float64)]
+    for _ in range(levels):
+        nxt = []
+        for T in tris:
+            m0, m1, m2 = (T[1] + T[2]) / 2, (T[2] + T[0]) / 2, (T[0] + T[1]) / 2
+            nxt += [np.array([T[0], m1, m2]), np.array([m0, T[1], m2]), np.array([m0, m1, T[2]]),
+                    np.array([m0, m1, m2])]
+        tris = nxt
+    tot = 0.0
+    for T in tris:
+        area = 0.5 * np.linalg.norm(np.cross(T[1] - T[0], T[2] - T[0]))
+        tot = tot + area * sum(wk * np.asarray(f(x)) for wk, x in zip(w, b @ T))
+    return tot
+
+
+def test_adaptive_pair_integrals_follow_the_subdivision(head1_oracle):
+    """HeadMat pair integrals with Integrator(3, levels, tol)
+    (integrator.h:69-93): with tol = 0 every level subdivides, so the result
+    is the 16-point rule on 4^(levels + 1) sub-triangles; with a tolerance
+    no difference can exceed, the first refinement (4 sub-triangles) is
+    returned.  Checked for the S integral (OperatorS of a over b) and the D
+    integral (P1 double layer of b over a) of two triangles that share an
+    edge, against the same composite rule summed here from point values."""
+    L = O.lib()
+    g = head1_oracle
+    tris = g.meshes[0].tris
+    V = np.asarray(g.vertices, dtype=np.float64)
+    a = tris[0]
+    b = next(t for t in tris[1:] if len(set(t) & set(a)) == 2)
+    Pa, Pb = V[a], V[b]
+    ta, tb = np.ascontiguousarray(Pa.ravel()), np.ascontiguousarray(Pb.ravel())
+
+    def opS(x):
+        return L.orc_operator_s(O._p(ta), O._p(np.ascontiguousarray(x)))
+
+    def p1(x):
+        out = np.zeros(3)
+        L.orc_p1slp(O._p(tb), O._p(np.ascontiguousarray(x)), O._p(out))
+        return out
+
+    def pair(kind, levels, tol):
+        out = np.zeros(3)
+        L.orc_integrate_pair_adaptive(O._p(ta), O._p(tb), kind, 3, levels, tol, O._p(out))
+        return out
+
+    for levels in (1, 2):
+        s = pair(2, levels, 0.0)[0]
+        assert abs(s - _composite(Pb, opS, levels + 1)) <= 1e-13 * abs(s)
+        d = pair(3, levels, 0.0)
+        assert np.abs(d - _composite(Pa, p1, levels + 1)).max() <= 1e-13 * np.abs(d).max()
+    assert abs(pair(2, 4, 1e30)[0] - _composite(Pb, opS, 1)) <= 1e-13 * abs(_composite(Pb, opS, 1))
+    # the refined values move toward the converged integral
+    fine = _composite(Pb, opS, 5)
+    errs = [abs(pair(2, lv, 0.0)[0] - fine) for lv in (0, 1, 2)]
+    assert errs[0] > errs[1] > errs[2]
+
+
+def test_adaptive_headmat_oracle(head1_oracle):
+    """The oracle's HeadMat with Integrator(3, 2, 1e-3): entries differ from
+    the non-adaptive ones only by quadrature refinement (the matrix keeps
+    its inertia and changes by < 1e-3 relative), and with a tolerance no
+    difference can exceed it equals the matrix whose pair integrals are
+    each refined once."""
+    g = head1_oracle
+    n = g.size
+    H0 = O.unpack(O.head_mat(g), n)
+    H2 = O.unpack(O.head_mat(g, 3, 2, 1e-3), n)
+    d = rel_fro(H2, H0)
+    assert 1e-9 < d < 1e-3
+    ev = np.linalg.eigvalsh(H2)
+    nv = sum(len(m.verts) for m in g.meshes)
+    assert (ev < 0).sum() == nv and (ev > 0).sum() == n - nv
+    H_once = O.unpack(O.head_mat(g, 3, 5, 1e30), n)
+    H_once2 = O.unpack(O.head_mat(g, 3, 1, 1e30), n)
+    assert np.array_equal(H_once, H_once2)
