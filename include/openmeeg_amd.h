@@ -339,6 +339,19 @@ int32_t omh_refine_state(int32_t it, double rel, double prev, double tol);
 int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
                     const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx, int64_t nrhs,
                     int32_t max_iter, double tol, int32_t* iterations, double* rel_update, void* stream);
+/* omh_mixed_solve without any device allocation (no reference counterpart;
+ * for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The refinement buffers
+ * live in a fixed device array of the library (66 MiB + 8 KiB per GPU, held
+ * until the process ends) instead of its growable scratch: n x nrhs must
+ * not exceed 4194304 (2^22), else invalid argument.  Same arguments,
+ * results and status as omh_mixed_solve.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU race. */
+int omh_mixed_solve_noalloc(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                            const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                            int64_t nrhs, int32_t max_iter, double tol, int32_t* iterations, double* rel_update,
+                            void* stream);
+
 /* The same with the block structure of d_H (as for the *_blocks
  * factorisations, and the one the factor d_L32 was made with): the FP64
  * residual skips d_H's zero blocks, the FP32 solves L's.  Not checked

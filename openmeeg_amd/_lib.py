@@ -73,6 +73,8 @@ _SIGS = {
     "omh_lu_factor_f64": ([vp, i64, i64, vp, vp, p_i32, vp], C.c_int),
     "omh_lu_solve_f64": ([vp, i64, i64, vp, vp, vp, i64, i64, vp], C.c_int),
     "omh_mixed_solve": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, p_i32, p_f64, vp], C.c_int),
+    "omh_mixed_solve_noalloc": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, p_i32, p_f64, vp],
+                                C.c_int),
     "omh_sym_solve_f64_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
     "omh_sym_solve_f32_blocks": ([vp, i64, i64, vp, vp, i64, i64, i32, vp, vp, vp], C.c_int),
     "omh_mixed_solve_blocks": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, i32, vp, vp, p_i32, p_f64,

@@ -21,6 +21,7 @@
 #include "layout.hpp"
 #include "linalg.hpp"
 #include "lu.hpp"
+#include "noalloc.hpp"
 #include "sources.hpp"
 
 struct omh_geometry {
@@ -796,6 +797,28 @@ int omh_mixed_solve(const double* d_H, int64_t n, int64_t ldh, const float* d_L3
         bool ok = false;
         const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
                                          &rel, &ok, S(stream));
+        if (iterations) *iterations = it;
+        if (rel_update) *rel_update = rel;
+        if (!ok) not_converged(it, rel, tol);
+    });
+}
+
+int omh_mixed_solve_noalloc(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                            const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                            int64_t nrhs, int32_t max_iter, double tol, int32_t* iterations, double* rel_update,
+                            void* stream) {
+    return guard([&] {
+        need(d_H, "d_H");
+        need(d_L32, "d_L32");
+        need(d_work32, "d_work32");
+        need(d_B, "d_B");
+        need(d_X, "d_X");
+        if (n <= 0 || ldh < n || ld32 < n || ldb < n || ldx < n || nrhs < 0 || max_iter < 1)
+            throw std::invalid_argument("bad sizes");
+        double rel = 0.0;
+        bool ok = false;
+        const int it = omh::mixed_refine_noalloc(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs,
+                                                 max_iter, tol, &rel, &ok, S(stream));
         if (iterations) *iterations = it;
         if (rel_update) *rel_update = rel;
         if (!ok) not_converged(it, rel, tol);
