@@ -73,7 +73,7 @@ def roofline_of(precision, kern, config):
 
 def batched_extra(om, S, torch, reps=3):
     """32 uneven perturbed heads (642..2562 vertices per surface, 200..2000
-    dipoles, 64..256 electrodes): the single-head mixed pipeline head after
+    dipoles, 64..256 electrodes): the single-head mixed16 pipeline head after
     head vs one batch (tuned, eager, then graph replays)."""
     hs = S.make_batched_heads(n_heads=32, seed=4)
     heads = [(om.make_nested_geometry(m, c), d, e) for m, c, d, e in hs]
@@ -83,7 +83,7 @@ def batched_extra(om, S, torch, reps=3):
         torch.cuda.synchronize()
         t0 = time.time()
         for g, d, e in heads:
-            G, _ = om.gain_eeg_pipeline(g, d, e, precision="mixed")
+            G, _ = om.gain_eeg_pipeline(g, d, e, precision="mixed16")
             if rep == 1:
                 ref.append(om.gain_to_numpy(G, len(e), len(d)))
             del G
@@ -115,7 +115,7 @@ def batched_extra(om, S, torch, reps=3):
             "sequential_pipeline_s": round(t_seq, 4), "batch_eager_s": round(t_eager, 4),
             "batch_graph_replay_s": round(t_graph, 4), "speedup_vs_sequential": round(t_seq / t_graph, 2),
             "max_rel_diff_vs_pipeline": err, "config": cfg,
-            "tuning": [{"tile": t, "grid_cap": c, "s": round(x, 6)} for t, c, x in tune]}
+            "tuning": [{"tile": t, "grid_cap": c, "split16": sp, "s": round(x, 6)} for t, c, sp, x in tune]}
 
 
 def gradient_extra(om, S, torch):

@@ -426,16 +426,20 @@ int omh_profile_read(const char* name, int64_t* calls, double* seconds, double* 
  * graph of the whole pass (*used_graph tells which ran).  Returns
  * OMH_STATUS_NOT_CONVERGED when the last eager pass's refinement did not
  * converge (omh_refine_state) for every head.
- * omh_batch_tune: times the candidate tile sizes (64, 128) and grid caps of
- * the grouped trailing update on the batch's own shapes and keeps the
- * fastest (n_results x {tile, grid_cap, seconds} written to h_results if not
- * NULL, at most max_results); the next pass runs eagerly again.
+ * omh_batch_tune: times the candidates of the grouped trailing update on the
+ * batch's own shapes and keeps the fastest: FP32 products on 64- or 128-wide
+ * tiles with grid caps, and (precision 1) the split-FP16 update (the panel
+ * split into FP16 parts, three products per 128 x 128 tile, FP32 sums).
+ * n_results x {tile, grid_cap, split16 (0/1), seconds} are written to
+ * h_results if not NULL (at most max_results); the next pass runs eagerly
+ * again.
  * omh_batch_gain: device pointer to head h's gain (n_el x n_dip,
  * column-major), valid until omh_batch_free.
  * omh_batch_stats: factor status (0 or 1 + first vanishing pivot) and the
  * last relative refinement update of head h.
  * omh_batch_config: refinement steps, trailing tile and grid cap, grouped
- * launches per pass and nodes of the captured graph (0 before capture). */
+ * launches per pass, nodes of the captured graph (0 before capture) and
+ * whether the trailing update is the split-FP16 one. */
 typedef struct omh_batch omh_batch;
 int omh_batch_create(int32_t n_heads, omh_geometry* const* geoms, const double* const* h_dipoles,
                      const int64_t* n_dipoles, const double* const* h_electrodes, const int64_t* n_electrodes,
@@ -445,7 +449,7 @@ int omh_batch_tune(omh_batch* b, double* h_results, int32_t max_results, int32_t
 int omh_batch_gain(const omh_batch* b, int32_t head, const double** d_gain, int64_t* n_el, int64_t* n_dip);
 int omh_batch_stats(omh_batch* b, int32_t head, int32_t* factor_info, double* rel_update, void* stream);
 int omh_batch_config(const omh_batch* b, int32_t* refine_steps, int32_t* tile, int64_t* grid_cap,
-                     int64_t* launches, int64_t* graph_nodes);
+                     int64_t* launches, int64_t* graph_nodes, int32_t* split16);
 int omh_batch_free(omh_batch* b);
 
 /* Frees the library's cached device scratch buffers. */

@@ -56,7 +56,7 @@ _SIGS = {
     "omh_batch_tune": ([vp, p_f64, i32, p_i32, vp], C.c_int),
     "omh_batch_gain": ([vp, i32, vp, p_i64, p_i64], C.c_int),
     "omh_batch_stats": ([vp, i32, p_i32, p_f64, vp], C.c_int),
-    "omh_batch_config": ([vp, p_i32, p_i32, p_i64, p_i64, p_i64], C.c_int),
+    "omh_batch_config": ([vp, p_i32, p_i32, p_i64, p_i64, p_i64, p_i32], C.c_int),
     "omh_batch_free": ([vp], C.c_int),
     "omh_factor_workspace_bytes": ([i64, i32], i64),
     "omh_sym_factor_f64": ([vp, i64, i64, vp, p_i32, vp], C.c_int),

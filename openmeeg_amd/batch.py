@@ -2,11 +2,13 @@
 
 ``BatchedGain([(geometry, dipoles, electrodes), ...], precision)`` wraps the
 library's batch plan (omh_batch_*): every head's HeadMat and source matrix
-are assembled, and the dense algebra of all heads (factorisation, adjoint
-solve with refinement, gain product) runs as grouped launches in lock step.
-After a first eager pass, ``run(graph=True)`` replays the whole pass as one
-captured HIP graph; ``tune()`` picks the trailing-update tile size and grid
-by timing the candidates on the batch's own shapes.  The reference computes
+are assembled (the heads spread over four streams), and the dense algebra of
+all heads (factorisation, adjoint solve with refinement, gain product) runs
+as grouped launches in lock step.  After a first eager pass,
+``run(graph=True)`` replays the whole pass as one captured HIP graph;
+``tune()`` picks the trailing update by timing the candidates on the
+batch's own shapes: FP32 products on 64- or 128-wide tiles and grid caps,
+or ("mixed") the split-FP16 update, the default before tuning.  The reference computes
 one head per call (gain.h:78-89); there is no batched counterpart.
 """
 import ctypes as C
@@ -65,19 +67,20 @@ class BatchedGain:
 
     def tune(self):
         """Time the trailing-update candidates; returns [(tile, grid_cap,
-        seconds)] and keeps the fastest."""
-        buf = (C.c_double * 96)()
+        split16, seconds)] and keeps the fastest."""
+        buf = (C.c_double * 128)()
         n = C.c_int32()
         check(lib().omh_batch_tune(self._h, buf, 32, C.byref(n), _stream()))
-        return [(int(buf[3 * i]), int(buf[3 * i + 1]), buf[3 * i + 2]) for i in range(min(n.value, 32))]
+        return [(int(buf[4 * i]), int(buf[4 * i + 1]), bool(buf[4 * i + 2]), buf[4 * i + 3])
+                for i in range(min(n.value, 32))]
 
     def config(self):
-        rs, tile = C.c_int32(), C.c_int32()
+        rs, tile, split = C.c_int32(), C.c_int32(), C.c_int32()
         cap, launches, nodes = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().omh_batch_config(self._h, C.byref(rs), C.byref(tile), C.byref(cap), C.byref(launches),
-                                     C.byref(nodes)))
-        return dict(refine_steps=rs.value, tile=tile.value, grid_cap=cap.value, grouped_launches=launches.value,
-                    graph_nodes=nodes.value)
+                                     C.byref(nodes), C.byref(split)))
+        return dict(refine_steps=rs.value, tile=tile.value, grid_cap=cap.value, split16=bool(split.value),
+                    grouped_launches=launches.value, graph_nodes=nodes.value)
 
     def gain_device(self, h):
         """Head h's gain as a torch tensor (n_el x n_dip) viewing the batch's

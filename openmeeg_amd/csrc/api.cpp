@@ -979,9 +979,10 @@ int omh_batch_tune(omh_batch* b, double* h_results, int32_t max_results, int32_t
         if (n_results) *n_results = (int32_t)res.size();
         if (h_results)
             for (int i = 0; i < (int)res.size() && i < max_results; ++i) {
-                h_results[3 * i] = res[i].tile;
-                h_results[3 * i + 1] = (double)res[i].grid_cap;
-                h_results[3 * i + 2] = res[i].seconds;
+                h_results[4 * i] = res[i].tile;
+                h_results[4 * i + 1] = (double)res[i].grid_cap;
+                h_results[4 * i + 2] = res[i].split16 ? 1.0 : 0.0;
+                h_results[4 * i + 3] = res[i].seconds;
             }
     });
 }
@@ -1011,9 +1012,10 @@ int omh_batch_stats(omh_batch* b, int32_t head, int32_t* factor_info, double* re
 }
 
 int omh_batch_config(const omh_batch* b, int32_t* refine_steps, int32_t* tile, int64_t* grid_cap, int64_t* launches,
-                     int64_t* graph_nodes) {
+                     int64_t* graph_nodes, int32_t* split16) {
     return guard([&] {
         need(b, "batch");
+        if (split16) *split16 = b->plan->trailing_split16() ? 1 : 0;
         if (refine_steps) *refine_steps = b->plan->refine_steps();
         if (tile) *tile = b->plan->trailing_tile();
         if (grid_cap) *grid_cap = b->plan->trailing_grid_cap();

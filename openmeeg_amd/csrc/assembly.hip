@@ -120,20 +120,27 @@ void release_scratch() {
     (void)hipSetDevice(cur);
 }
 
+// Columns over grid.y (strided), two consecutive rows per thread over
+// grid.x; 16-byte stores where the pair is aligned.
 __global__ void k_fill(double* p, int64_t rows, int64_t cols, int64_t ld, double v) {
-    const int64_t total = rows * cols;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = t / rows, r = t - c * rows;
-        p[r + c * ld] = v;
+    for (int64_t c = blockIdx.y; c < cols; c += gridDim.y) {
+        double* col = p + c * ld;
+        for (int64_t r = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x); r < rows;
+             r += 2 * (int64_t)gridDim.x * blockDim.x) {
+            if (r + 1 < rows && (reinterpret_cast<uintptr_t>(col + r) & 15) == 0) {
+                *reinterpret_cast<double2*>(col + r) = make_double2(v, v);
+            } else {
+                col[r] = v;
+                if (r + 1 < rows) col[r + 1] = v;
+            }
+        }
     }
 }
 
 void fill_f64(double* p, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s) {
     if (rows <= 0 || cols <= 0) return;
-    const int64_t total = rows * cols;
-    const int64_t blocks = std::min<int64_t>(cdiv(total, 256), 65536);
-    k_fill<<<(unsigned)blocks, 256, 0, s>>>(p, rows, cols, ld, v);
+    const dim3 grid((unsigned)std::min<int64_t>(cdiv(rows, 512), 64), (unsigned)std::min<int64_t>(cols, 8192));
+    k_fill<<<grid, 256, 0, s>>>(p, rows, cols, ld, v);
     OMH_LAUNCH_CHECK();
 }
 

@@ -109,12 +109,30 @@ BatchPlan::BatchPlan(const std::vector<BatchHeadInput>& in, int precision) : pre
         H.W = f32 ? (void*)dalloc<float>((size_t)H.ld * H.ne, owned_) : (void*)H.Y;
         H.R = f32 ? dalloc<double>((size_t)H.ld * H.ne, owned_) : nullptr;
         H.tmp = dalloc<char>(te * (size_t)FACT_NB * H.ne, owned_);
+        if (f32 && H.n > FACT_NB) H.sp = dalloc<char>(split_operand_bytes(H.n), owned_);
         H.G = dalloc<double>((size_t)H.ne * H.nd, owned_);
         H.info = info_ + h;
         max_nblk_ = std::max(max_nblk_, H.nblk);
         heads_.push_back(std::move(H));
     }
     OMH_HIP(hipStreamSynchronize(stream_));
+    // assembly streams: the largest heads first, each to the stream with the
+    // least assembly so far (pairs of unknowns as the measure)
+    OMH_HIP(hipEventCreateWithFlags(&asm_fork_, hipEventDisableTiming));
+    for (int s = 0; s < ASM_STREAMS; ++s) {
+        OMH_HIP(hipStreamCreateWithFlags(&asm_stream_[s], hipStreamNonBlocking));
+        OMH_HIP(hipEventCreateWithFlags(&asm_join_[s], hipEventDisableTiming));
+    }
+    std::vector<int> order(heads_.size());
+    for (size_t h = 0; h < order.size(); ++h) order[h] = (int)h;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return heads_[a].n > heads_[b].n; });
+    asm_of_.assign(heads_.size(), 0);
+    double load[ASM_STREAMS] = {};
+    for (int h : order) {
+        const int s = (int)(std::min_element(load, load + ASM_STREAMS) - load);
+        asm_of_[h] = s;
+        load[s] += (double)heads_[h].n * (double)heads_[h].n;
+    }
     build_schedule();
 }
 
@@ -126,10 +144,18 @@ BatchPlan::~BatchPlan() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     arena_.release();
+    for (int s = 0; s < ASM_STREAMS; ++s) {
+        if (asm_stream_[s]) (void)hipStreamSynchronize(asm_stream_[s]);
+        asm_arena_[s].release();
+        if (asm_join_[s]) (void)hipEventDestroy(asm_join_[s]);
+        if (asm_stream_[s]) (void)hipStreamDestroy(asm_stream_[s]);
+    }
+    if (asm_fork_) (void)hipEventDestroy(asm_fork_);
     for (void* p : owned_) (void)hipFree(p);
     if (d_gemm_) (void)hipFree(d_gemm_);
     if (d_diag_) (void)hipFree(d_diag_);
     if (d_elem_) (void)hipFree(d_elem_);
+    if (d_split_) (void)hipFree(d_split_);
     if (ev_in_) (void)hipEventDestroy(ev_in_);
     if (ev_out_) (void)hipEventDestroy(ev_out_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -164,6 +190,30 @@ void BatchPlan::add_elem(Segment& seg, const std::vector<ElemDesc>& ds) {
     seg.push_back(op);
 }
 
+// A split of every listed matrix, then their split updates (descriptors
+// get their block and tile offsets here).
+void BatchPlan::add_split(Segment& seg, std::vector<SplitDesc>& ds) {
+    if (ds.empty()) return;
+    int64_t blocks = 0, tiles = 0;
+    for (SplitDesc& d : ds) {
+        d.blk0 = blocks;
+        d.tile0 = tiles;
+        blocks += d.mp / 64;
+        const int64_t t = cdiv(d.m, 128);
+        tiles += t * (t + 1) / 2;
+    }
+    Op op;
+    op.off = split_descs_.size();
+    op.ng = (int)ds.size();
+    split_descs_.insert(split_descs_.end(), ds.begin(), ds.end());
+    op.type = Op::SPLIT;
+    op.blocks = blocks;
+    seg.push_back(op);
+    op.type = Op::SPLIT_UPDATE;
+    op.blocks = tiles;
+    seg.push_back(op);
+}
+
 void BatchPlan::add_diag(Segment& seg, const std::vector<DiagDesc>& ds, bool f32) {
     if (ds.empty()) return;
     Op op;
@@ -189,6 +239,7 @@ void BatchPlan::factor_ops(Segment& seg) {
         std::vector<DiagDesc> diag;
         std::vector<GemmJob> panel, trail;
         std::vector<ElemDesc> scale;
+        std::vector<SplitDesc> split;
         for (const Head& H : heads_) {
             if (k >= H.n) continue;
             const int nb = (int)std::min<int64_t>(FACT_NB, H.n - k);
@@ -201,12 +252,17 @@ void BatchPlan::factor_ops(Segment& seg) {
             panel.push_back(GemmJob{m, nb, nb, 1.0, A21, H.ld, Li, FACT_NB, 0.0, H.X, ldx});
             scale.push_back(ElemDesc{f32 ? ELEM_SCALE_COLS_F32 : ELEM_SCALE_COLS_F64, H.X, ldx, A21, H.ld, m, nb,
                                      (char*)H.sgn + te * k});
-            trail.push_back(GemmJob{m, m, nb, -1.0, H.X, ldx, A21, H.ld, 1.0, at(H, H.A, e, e), H.ld});
+            if (f32 && trail_split_)
+                split.push_back(SplitDesc{(const float*)H.X, ldx, (const float*)H.sgn + k, m, round_up(m, 128), nb,
+                                          H.sp, (float*)at(H, H.A, e, e), H.ld, 0, 0});
+            else
+                trail.push_back(GemmJob{m, m, nb, -1.0, H.X, ldx, A21, H.ld, 1.0, at(H, H.A, e, e), H.ld});
         }
         add_diag(seg, diag, f32);
         add_gemm(seg, panel, f32, false, true, false, false);
         add_elem(seg, scale);
         add_gemm(seg, trail, f32, false, true, true, true);
+        add_split(seg, split);
     }
 }
 
@@ -266,6 +322,7 @@ void BatchPlan::build_schedule() {
     gemm_descs_.clear();
     diag_descs_.clear();
     elem_descs_.clear();
+    split_descs_.clear();
     prep_.clear();
     factor_.clear();
     solve_.clear();
@@ -327,6 +384,7 @@ void BatchPlan::upload_tables() {
     upload_table(gemm_descs_, d_gemm_, cap_gemm_);
     upload_table(diag_descs_, d_diag_, cap_diag_);
     upload_table(elem_descs_, d_elem_, cap_elem_);
+    upload_table(split_descs_, d_split_, cap_split_);
 }
 
 void BatchPlan::launch(const Segment& seg, hipStream_t st) {
@@ -335,6 +393,8 @@ void BatchPlan::launch(const Segment& seg, hipStream_t st) {
             case Op::GEMM: launch_grouped_gemm(op.gemm, d_gemm_, op.trailing ? trail_cap_ : 0, st); break;
             case Op::DIAG: launch_grouped_diag(op.f32, d_diag_ + op.off, op.ng, st); break;
             case Op::ELEM: launch_grouped_elem(d_elem_ + op.off, op.ng, op.max_elems, st); break;
+            case Op::SPLIT: launch_grouped_split(d_split_ + op.off, op.ng, op.blocks, st); break;
+            case Op::SPLIT_UPDATE: launch_grouped_split_update(d_split_ + op.off, op.ng, op.blocks, st); break;
         }
     }
 }
@@ -342,12 +402,23 @@ void BatchPlan::launch(const Segment& seg, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // execution
 // ---------------------------------------------------------------------------
+// Each head's HeadMat and source matrix on its assembly stream (forked from
+// st and joined back: under capture, graph dependencies), so that the small
+// heads' launches run beside each other instead of one after another.
 void BatchPlan::enqueue_assembly(hipStream_t st) {
-    ArenaScope scope(&arena_);
-    for (Head& H : heads_) {
+    OMH_HIP(hipEventRecord(asm_fork_, st));
+    for (int s = 0; s < ASM_STREAMS; ++s) OMH_HIP(hipStreamWaitEvent(asm_stream_[s], asm_fork_, 0));
+    for (size_t h = 0; h < heads_.size(); ++h) {
+        Head& H = heads_[h];
+        const int s = asm_of_[h];
+        ArenaScope scope(&asm_arena_[s]);
         const Layout L{H.ld, H.nv, H.nt, 1};
-        assemble_headmat(*H.g, 3, H.H, L, st, precision_ == 1);   // far-field Gauss for the mixed path
-        dsm_launch(*H.g, H.dip, H.nd, H.cD, H.cP, H.used, 3, 10, 0.001, H.D, H.ld, L, st);
+        assemble_headmat(*H.g, 3, H.H, L, asm_stream_[s], precision_ == 1);   // far-field Gauss for the mixed path
+        dsm_launch(*H.g, H.dip, H.nd, H.cD, H.cP, H.used, 3, 10, 0.001, H.D, H.ld, L, asm_stream_[s]);
+    }
+    for (int s = 0; s < ASM_STREAMS; ++s) {
+        OMH_HIP(hipEventRecord(asm_join_[s], asm_stream_[s]));
+        OMH_HIP(hipStreamWaitEvent(st, asm_join_[s], 0));
     }
 }
 
@@ -409,6 +480,7 @@ void BatchPlan::run_eager(hipStream_t st) {
 void BatchPlan::capture() {
     if (!calibrated_) throw std::runtime_error("run the batch once without the graph before capturing it");
     arena_.frozen = true;
+    for (int s = 0; s < ASM_STREAMS; ++s) asm_arena_[s].frozen = true;
     OMH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
         enqueue(stream_, refine_steps_);
@@ -511,11 +583,65 @@ std::vector<BatchTuneResult> BatchPlan::tune(hipStream_t caller) {
         OMH_HIP(hipEventElapsedTime(&ms, a, b));
         OMH_HIP(hipFree(dd));
         const double t = ms * 1e-3 / reps;
-        res.push_back(BatchTuneResult{c.wb * 32, c.cap, t});
+        res.push_back(BatchTuneResult{c.wb * 32, c.cap, false, t});
         if (t < best) {
             best = t;
             trail_wb_ = c.wb;
             trail_cap_ = c.cap;
+            trail_split_ = false;
+        }
+    }
+    if (f32) {   // the split-FP16 update: split and update of the same steps
+        std::vector<std::vector<SplitDesc>> per_step;
+        for (int64_t s : steps) {
+            const int64_t k = s * FACT_NB;
+            std::vector<SplitDesc> sd;
+            int64_t blocks = 0, tiles = 0;
+            for (const Head& H : heads_) {
+                const int64_t e = k + FACT_NB, m = H.n - e;
+                if (m <= 0) continue;
+                SplitDesc d{(const float*)H.X, round_up(m, 4), (const float*)H.sgn + k, m, round_up(m, 128),
+                            FACT_NB, H.sp, (float*)((char*)H.A + te * (size_t)(e + e * H.ld)), H.ld, blocks, tiles};
+                blocks += d.mp / 64;
+                const int64_t t = cdiv(m, 128);
+                tiles += t * (t + 1) / 2;
+                sd.push_back(d);
+            }
+            if (!sd.empty()) per_step.push_back(sd);
+        }
+        if (!per_step.empty()) {
+            std::vector<SplitDesc*> dd(per_step.size());
+            std::vector<int64_t> nb(per_step.size()), nt(per_step.size());
+            for (size_t i = 0; i < per_step.size(); ++i) {
+                const auto& sd = per_step[i];
+                OMH_HIP(hipMalloc((void**)&dd[i], sd.size() * sizeof(SplitDesc)));
+                OMH_HIP(hipMemcpy(dd[i], sd.data(), sd.size() * sizeof(SplitDesc), hipMemcpyHostToDevice));
+                const SplitDesc& last = sd.back();
+                const int64_t t = cdiv(last.m, 128);
+                nb[i] = last.blk0 + last.mp / 64;
+                nt[i] = last.tile0 + t * (t + 1) / 2;
+            }
+            auto pass = [&]() {
+                for (size_t i = 0; i < per_step.size(); ++i) {
+                    launch_grouped_split(dd[i], (int)per_step[i].size(), nb[i], stream_);
+                    launch_grouped_split_update(dd[i], (int)per_step[i].size(), nt[i], stream_);
+                }
+            };
+            pass();   // warm-up
+            OMH_HIP(hipEventRecord(a, stream_));
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r) pass();
+            OMH_HIP(hipEventRecord(b, stream_));
+            OMH_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            OMH_HIP(hipEventElapsedTime(&ms, a, b));
+            for (SplitDesc* p : dd) OMH_HIP(hipFree(p));
+            const double t = ms * 1e-3 / reps;
+            res.push_back(BatchTuneResult{128, 0, true, t});
+            if (t < best) {
+                best = t;
+                trail_split_ = true;
+            }
         }
     }
     OMH_HIP(hipEventDestroy(a));

@@ -21,6 +21,7 @@ struct BatchHeadInput {
 struct BatchTuneResult {
     int tile;                       // 64 or 128
     int64_t grid_cap;               // 0: one workgroup per tile
+    bool split16;                   // split-FP16 update (128 x 128 tiles, one workgroup per tile)
     double seconds;                 // sampled trailing updates, per repetition
 };
 
@@ -51,7 +52,8 @@ public:
     }
     void stats(int h, int& factor_info, double& rel_update, hipStream_t caller);
     int refine_steps() const { return refine_steps_; }
-    int trailing_tile() const { return trail_wb_ * 32; }
+    int trailing_tile() const { return trail_split_ ? 128 : trail_wb_ * 32; }
+    bool trailing_split16() const { return trail_split_; }
     int64_t trailing_grid_cap() const { return trail_cap_; }
     int64_t graph_nodes() const { return graph_nodes_; }
     int64_t launches() const;   // grouped launches of the linear algebra per pass
@@ -75,18 +77,20 @@ private:
         void* W = nullptr;          // right-hand sides in factor precision
         double* R = nullptr;        // FP64 residual (mixed)
         void* tmp = nullptr;        // FACT_NB x ne
+        void* sp = nullptr;         // split-FP16 operands of the trailing update (mixed)
         double* G = nullptr;        // gain, ne x nd
         int* info = nullptr;
         double rel_update = 0.0;
     };
     struct Op {
-        enum Type { GEMM, DIAG, ELEM } type = ELEM;
+        enum Type { GEMM, DIAG, ELEM, SPLIT, SPLIT_UPDATE } type = ELEM;
         GemmLaunch gemm{};
         bool trailing = false;
         bool f32 = false;
         size_t off = 0;
         int ng = 0;
         int64_t max_elems = 0;
+        int64_t blocks = 0;         // SPLIT: 64-row blocks; SPLIT_UPDATE: tiles
     };
     typedef std::vector<Op> Segment;
 
@@ -97,6 +101,7 @@ private:
                   bool trailing);
     void add_elem(Segment& seg, const std::vector<ElemDesc>& ds);
     void add_diag(Segment& seg, const std::vector<DiagDesc>& ds, bool f32);
+    void add_split(Segment& seg, std::vector<SplitDesc>& ds);
     void upload_tables();
     void launch(const Segment& seg, hipStream_t st);
     void enqueue_assembly(hipStream_t st);
@@ -111,6 +116,13 @@ private:
     int* info_ = nullptr;
     std::vector<void*> owned_;
     ScratchArena arena_;
+    // assembly: heads spread over ASM_STREAMS streams forked from the pass's
+    // stream (each with its own scratch arena), joined before the algebra
+    static constexpr int ASM_STREAMS = 4;
+    ScratchArena asm_arena_[ASM_STREAMS];
+    hipStream_t asm_stream_[ASM_STREAMS] = {};
+    hipEvent_t asm_fork_ = nullptr, asm_join_[ASM_STREAMS] = {};
+    std::vector<int> asm_of_;     // stream of each head
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
     hipGraph_t graph_ = nullptr;
@@ -119,18 +131,21 @@ private:
     bool calibrated_ = false;
     int refine_steps_ = 0;
     int max_refine_ = 10;
-    double refine_tol_ = 1e-10;   // estimated error left (refine_state)
+    double refine_tol_ = 1e-8;    // estimated error left (refine_state), as the single-head gain pipelines
     bool refine_ok_ = true;       // the last eager pass reached refine_tol_
     int trail_wb_ = 4;
     int64_t trail_cap_ = 0;
+    bool trail_split_ = true;     // mixed: split-FP16 trailing update (else FP32 grouped GEMM)
 
     std::vector<GemmDesc> gemm_descs_;
     std::vector<DiagDesc> diag_descs_;
     std::vector<ElemDesc> elem_descs_;
+    std::vector<SplitDesc> split_descs_;
     GemmDesc* d_gemm_ = nullptr;
     DiagDesc* d_diag_ = nullptr;
     ElemDesc* d_elem_ = nullptr;
-    size_t cap_gemm_ = 0, cap_diag_ = 0, cap_elem_ = 0;
+    SplitDesc* d_split_ = nullptr;
+    size_t cap_gemm_ = 0, cap_diag_ = 0, cap_elem_ = 0, cap_split_ = 0;
     Segment prep_, factor_, solve_, ir_init_, ir_pre_, ir_post_, ir_resid_, gain_;
 };
 

@@ -782,14 +782,18 @@ __global__ void k_scale_rows(T* __restrict__ W, int64_t ldw, int64_t m, int64_t 
     }
 }
 
+// dst = src (m x n), or only its lower triangle (rows i >= j of column j):
+// columns over grid.y (strided), rows over grid.x and the threads.
 template <typename TS, typename TD>
 __global__ void k_convert(const TS* __restrict__ src, int64_t lds, TD* __restrict__ dst, int64_t ldd, int64_t m,
                           int64_t n, int lower_only) {
-    const int64_t total = m * n;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = t / m, i = t - j * m;
-        if (lower_only && i < j) continue;
-        dst[i + j * ldd] = (TD)src[i + j * lds];
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+        const TS* s = src + j * lds;
+        TD* d = dst + j * ldd;
+        const int64_t i0 = lower_only ? j : 0;
+        for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+             i += (int64_t)gridDim.x * blockDim.x)
+            d[i] = (TD)s[i];
     }
 }
 
@@ -862,11 +866,6 @@ __global__ void k_sparse_rows(const int64_t* __restrict__ cols, const double* __
 // ---------------------------------------------------------------------------
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 constexpr int SK = 32;        // k-slice of the FP16 MFMA
-// 1: the split update reads and writes its output tiles with nontemporal
-// accesses (they are streamed once and need not stay in L2)
-#ifndef OMH_SYRK_NT_C
-#define OMH_SYRK_NT_C 0
-#endif
 constexpr int S_TILE = 128;   // output tile of the split SYRK
 // Panels per split update (its rank / FACT_NB).  n = 102400: 2 panels
 // 1.48 s, 3 panels 1.51 s, 4 panels 1.34 s for the whole factorisation
@@ -967,31 +966,19 @@ struct SliceClock {
 };
 
 // C (m x m, leading dimension ldc) -= split product of operand rows
-// r_off + [0, m), over k extent KP, on the tiles of map: the lower triangle,
-// or (kind 6) the lower part of a strip of whole tile columns; entries with
-// row < col or an index >= m are not written.  256 threads, a 128 x 128
-// tile, 2 x 2 waves of 64 x 64 (4 x 4 MFMA blocks, 3 products each),
-// k-slices double-buffered in LDS.  Operand rows up to
+// r_off + [0, m), over k extent KP, on tile (tm, tn) of C (tm >= tn);
+// entries with row < col or an index >= m are not written.  256 threads, a
+// 128 x 128 tile, 2 x 2 waves of 64 x 64 (4 x 4 MFMA blocks, 3 products
+// each), k-slices double-buffered in LDS.  Operand rows up to
 // r_off + 128 * ceil(m / 128) exist (zero padding), so loads need no guards.
 template <int KP>
-__device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
+__device__ __forceinline__ void syrk_split_tile(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
                                                 const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
                                                 const _Float16* __restrict__ Bl, const float* __restrict__ scale,
-                                                float* __restrict__ C, int64_t ldc, TileMap map,
-                                                const TileRemap& rm, SliceClock clk = SliceClock{}) {
+                                                float* __restrict__ C, int64_t ldc, int64_t tm, int64_t tn,
+                                                SliceClock clk = SliceClock{}) {
     __shared__ half8_t S[2][4][S_TILE * 4];
     __shared__ int rot_sh;
-    int64_t t = blockIdx.x;
-    if (map.kind >= 4) {
-        t = xcd_tile(t, map.per);
-        if (t >= map.ntiles) return;
-    }
-    int64_t tm, tn;
-    map_tile(map, t, tm, tn);
-    if (rm.n) {   // compressed order over the structurally non-zero tile rows
-        tm = remap_tile(rm, tm);
-        tn = remap_tile(rm, tn);
-    }
     const int64_t row0 = r_off + tm * S_TILE, col0 = r_off + tn * S_TILE;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -1042,12 +1029,12 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
     const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
     const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
     float4 c_pre[4][4], si_pre[4];
+    const int fr = lane & 15, fc = lane >> 4;
     gload(0, 0);
     lstore(0, 0);
     gload(1, 1);
     gload(2, 0);
     __syncthreads();
-    const int fr = lane & 15, fc = lane >> 4;
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {   // unrolled: register slots stay static
         const int buf = ks & 1;
@@ -1092,16 +1079,9 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
             for (int a = 0; a < 4; ++a) {
                 si_pre[a] = *reinterpret_cast<const float4*>(scale + r_off + i0 + wm + 16 * a + 4 * fc);
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const float* cp = C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc;
-#if OMH_SYRK_NT_C
-                    // streamed once: keep it from evicting the operand bands in L2
-                    const float4_t v = __builtin_nontemporal_load(reinterpret_cast<const float4_t*>(cp));
-                    c_pre[a][b] = make_float4(v[0], v[1], v[2], v[3]);
-#else
-                    c_pre[a][b] = *reinterpret_cast<const float4*>(cp);
-#endif
-                }
+                for (int b = 0; b < 4; ++b)
+                    c_pre[a][b] = *reinterpret_cast<const float4*>(
+                        C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc);
             }
         }
     }
@@ -1120,11 +1100,7 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
                 c.y -= si.y * sj * acc[a][b][1];
                 c.z -= si.z * sj * acc[a][b][2];
                 c.w -= si.w * sj * acc[a][b][3];
-#if OMH_SYRK_NT_C
-                __builtin_nontemporal_store(float4_t{c.x, c.y, c.z, c.w}, reinterpret_cast<float4_t*>(cp));
-#else
                 *cp = c;
-#endif
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -1137,6 +1113,28 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
         const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t_start;
         clk.period_out[0] = (unsigned)max((uint64_t)1, min(dt / NKS, (uint64_t)1000000));
     }
+}
+
+// The same on the tile of map that this workgroup takes: the lower triangle,
+// or (kind 6) the lower part of a strip of whole tile columns.
+template <int KP>
+__device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
+                                                const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
+                                                const _Float16* __restrict__ Bl, const float* __restrict__ scale,
+                                                float* __restrict__ C, int64_t ldc, TileMap map,
+                                                const TileRemap& rm, SliceClock clk = SliceClock{}) {
+    int64_t t = blockIdx.x;
+    if (map.kind >= 4) {
+        t = xcd_tile(t, map.per);
+        if (t >= map.ntiles) return;
+    }
+    int64_t tm, tn;
+    map_tile(map, t, tm, tn);
+    if (rm.n) {   // compressed order over the structurally non-zero tile rows
+        tm = remap_tile(rm, tm);
+        tn = remap_tile(rm, tn);
+    }
+    syrk_split_tile<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, tm, tn, clk);
 }
 
 // The same update on 256 x 256 tiles: 512 threads, 8 waves of 64 x 128
@@ -1312,6 +1310,79 @@ __global__ void __launch_bounds__(256) k_syrk_strip(int64_t m, int64_t mp, int64
     syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, TileRemap{});
 }
 
+// Grouped rank-nb splits: workgroup b takes 64-row block b - blk0 of the
+// matrix whose blocks it falls in (blk0 ascending); thread (row t % 64,
+// 64-wide k quarter t / 64).  Columns k >= nb and rows >= m are zero.
+__device__ __forceinline__ int desc_of(const SplitDesc* __restrict__ d, int ng, int64_t b, bool tiles) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if ((tiles ? d[mid].tile0 : d[mid].blk0) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_split_grouped(const SplitDesc* __restrict__ ds, int ng) {
+    __shared__ float red[4][64];
+    const SplitDesc& d = ds[desc_of(ds, ng, blockIdx.x, false)];
+    const int ri = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int64_t row = ((int64_t)blockIdx.x - d.blk0) * 64 + ri;
+    const int kb = kq * 64;
+    float v[64];
+    float mx = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < 64; ++kk) {
+        v[kk] = (row < d.m && kb + kk < d.nb) ? d.X[row + (int64_t)(kb + kk) * d.ldx] : 0.f;
+        mx = fmaxf(mx, fabsf(v[kk]));
+    }
+    red[kq][ri] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0][ri], red[1][ri]), fmaxf(red[2][ri], red[3][ri]));
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
+    const float inv = ldexpf(1.f, -e);
+    const size_t arr = (size_t)d.mp * FACT_NB;
+    _Float16* Ah = (_Float16*)d.Ah;
+    _Float16* Al = Ah + arr;
+    _Float16* Bh = Al + arr;
+    _Float16* Bl = Bh + arr;
+    if (kq == 0) reinterpret_cast<float*>(Bl + arr)[row] = ldexpf(1.f, e);
+#pragma unroll
+    for (int kk = 0; kk < 64; kk += 8) {
+        half8_t h, l, bh, bl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float x = v[kk + q] * inv;
+            const _Float16 hx = (_Float16)x;
+            const _Float16 lx = (_Float16)(x - (float)hx);
+            const bool neg = kb + kk + q < d.nb && d.sgn[kb + kk + q] < 0.f;
+            h[q] = hx;
+            l[q] = lx;
+            bh[q] = neg ? -hx : hx;
+            bl[q] = neg ? -lx : lx;
+        }
+        const int k0 = kb + kk;
+        const int64_t off = ((int64_t)(k0 / SK) * d.mp + row) * SK + (k0 % SK);
+        *reinterpret_cast<half8_t*>(Ah + off) = h;
+        *reinterpret_cast<half8_t*>(Al + off) = l;
+        *reinterpret_cast<half8_t*>(Bh + off) = bh;
+        *reinterpret_cast<half8_t*>(Bl + off) = bl;
+    }
+}
+
+// Grouped split updates: workgroup b takes tile b - tile0 (lower triangle,
+// row by row) of the matrix it falls in.
+__global__ void __launch_bounds__(256) k_syrk_split_grouped(const SplitDesc* __restrict__ ds, int ng) {
+    const SplitDesc& d = ds[desc_of(ds, ng, blockIdx.x, true)];
+    int64_t tm, tn;
+    upper_tile((int64_t)blockIdx.x - d.tile0, tn, tm);   // tn <= tm
+    const size_t arr = (size_t)d.mp * FACT_NB;
+    const _Float16* Ah = (const _Float16*)d.Ah;
+    syrk_split_tile<FACT_NB>(d.m, d.mp, 0, Ah, Ah + arr, Ah + 2 * arr, Ah + 3 * arr,
+                             reinterpret_cast<const float*>(Ah + 4 * arr), d.C, d.ldc, tm, tn);
+}
+
 // flag = 1 if any entry of the m x n block is non-zero.
 template <typename T>
 __global__ void __launch_bounds__(256) k_any_nonzero(const T* __restrict__ A, int64_t lda, int64_t m, int64_t n,
@@ -1465,6 +1536,23 @@ void build_grouped_gemm(const std::vector<GemmJob>& jobs, bool f32, bool ta, boo
         out.push_back(GemmLaunch{base | 32, descs.size(), (int)ed.size(), te});
         descs.insert(descs.end(), ed.begin(), ed.end());
     }
+}
+
+size_t split_operand_bytes(int64_t m) {
+    const size_t mp = (size_t)cdiv(m, S_TILE) * S_TILE;
+    return 4 * mp * FACT_NB * sizeof(_Float16) + mp * sizeof(float);
+}
+
+void launch_grouped_split(const SplitDesc* d_descs, int ng, int64_t blocks, hipStream_t st) {
+    if (ng <= 0 || blocks <= 0) return;
+    k_split_grouped<<<(unsigned)blocks, 256, 0, st>>>(d_descs, ng);
+    OMH_LAUNCH_CHECK();
+}
+
+void launch_grouped_split_update(const SplitDesc* d_descs, int ng, int64_t tiles, hipStream_t st) {
+    if (ng <= 0 || tiles <= 0) return;
+    k_syrk_split_grouped<<<(unsigned)tiles, 256, 0, st>>>(d_descs, ng);
+    OMH_LAUNCH_CHECK();
 }
 
 void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t grid_cap, hipStream_t st) {
@@ -2518,7 +2606,9 @@ template void sym_solve<float>(const float*, int64_t, int64_t, const void*, floa
 
 void convert_f64_to_f32(const double* src, int64_t lds, float* dst, int64_t ldd, int64_t m, int64_t n, bool lower,
                         hipStream_t st) {
-    k_convert<double, float><<<grid_for(m * n), 256, 0, st>>>(src, lds, dst, ldd, m, n, lower ? 1 : 0);
+    if (m <= 0 || n <= 0) return;
+    const dim3 grid((unsigned)std::min<int64_t>(cdiv(m, 256), 32), (unsigned)std::min<int64_t>(n, 16384));
+    k_convert<double, float><<<grid, 256, 0, st>>>(src, lds, dst, ldd, m, n, lower ? 1 : 0);
     OMH_LAUNCH_CHECK();
 }
 

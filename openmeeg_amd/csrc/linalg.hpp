@@ -91,6 +91,28 @@ void build_grouped_gemm(const std::vector<GemmJob>& jobs, bool f32, bool ta, boo
 // Launches l with descriptors at d_descs (device); grid_cap > 0 caps the grid
 // (workgroups then loop over tiles).
 void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t grid_cap, hipStream_t st);
+
+// One matrix's rank-nb trailing update C -= X S X^T (lower triangle) as a
+// split-FP16 product in a grouped launch: X (m x nb <= FACT_NB, FP32) is
+// split into its FP16 parts (one power-of-two scale per row, slice-major,
+// mp = m rounded up to 128 rows of zero padding), then every 128 x 128 tile of
+// C's lower triangle gets the three products.
+struct SplitDesc {
+    const float* X;
+    int64_t ldx;
+    const float* sgn;               // the nb signs S
+    int64_t m, mp;
+    int nb;
+    void* Ah;                       // 4 arrays of mp x FACT_NB FP16 values, then mp row scales
+    float* C;
+    int64_t ldc;
+    int64_t blk0;                   // first 64-row block of this matrix in the split launch
+    int64_t tile0;                  // first tile of this matrix in the update launch
+};
+// Bytes of SplitDesc::Ah for m rows.
+size_t split_operand_bytes(int64_t m);
+void launch_grouped_split(const SplitDesc* d_descs, int ng, int64_t blocks, hipStream_t st);
+void launch_grouped_split_update(const SplitDesc* d_descs, int ng, int64_t tiles, hipStream_t st);
 void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st);
 void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hipStream_t st);
 

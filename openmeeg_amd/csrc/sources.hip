@@ -521,7 +521,9 @@ static std::vector<double> mesh_solid_angles(Geometry& g, const double* d_pts, i
     std::vector<int64_t> mt(nm + 1);
     for (int m = 0; m < nm; ++m) mt[m] = dg.meshes[m].tri_begin;
     mt[nm] = dg.n_tris;
-    int64_t* d_mt = (int64_t*)scratch(SCR_SMALL, sizeof(int64_t) * (nm + 1) + sizeof(double) * npts * nm, st);
+    // the source-matrix slot (not SCR_SMALL): a source matrix on a side stream
+    // then shares no slot with a factorisation or solve on the caller's
+    int64_t* d_mt = (int64_t*)scratch(SCR_SRCCOEF, sizeof(int64_t) * (nm + 1) + sizeof(double) * npts * nm, st);
     double* d_sa = (double*)(d_mt + nm + 1);
     OMH_HIP(hipMemcpyAsync(d_mt, mt.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, st));
     OMH_HIP(hipStreamSynchronize(st));

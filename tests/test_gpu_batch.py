@@ -5,7 +5,8 @@ head's gain must equal the single-head pipeline's (FP64: 1e-12, the same
 arithmetic in another launch grouping), the oracle's (1e-10), and in mixed
 precision the FP64 gain (1e-6).  Graph replays must reproduce the eager pass
 exactly, with other GPU work between the replays, and tuning must not change
-the results.
+the results (beyond the refinement tolerance when it switches between the
+FP32 and the split-FP16 trailing update).
 """
 import numpy as np
 import pytest
@@ -84,11 +85,14 @@ def test_batch_mixed_graph_replays(om, heads, fp64_batch):
     print("mixed batch config before capture", cfg, flush=True)
     assert 1 <= cfg["refine_steps"] <= 10
     for i in range(len(heads)):
-        # refinement stops at an estimated error of 1e-10 (last update times
-        # its contraction ratio): the gain is far inside the 1e-6 bar
-        assert rel_fro(eager[i], G64[i]) < 1e-8
+        # refinement stops at an estimated error of 1e-8 (last update times
+        # its contraction ratio, as the single-head gain pipelines): the gain
+        # is well inside the 1e-6 bar
+        assert rel_fro(eager[i], G64[i]) < 1e-7
         st = b.stats(i)
-        assert st["factor_info"] == 0 and st["rel_update"] <= 1e-6
+        # the last correction's relative size: the refinement stopped on its
+        # contraction estimate, (rel / prev) rel <= 1e-8
+        assert st["factor_info"] == 0 and st["rel_update"] <= 1e-4
     for rep in range(3):
         x = torch.randn(512, 512, device="cuda", dtype=torch.float64)
         y = (x @ x).sum().item()                 # unrelated work between replays
@@ -106,14 +110,18 @@ def test_batch_tune_keeps_results(om, heads, fp64_batch):
     b = om.BatchedGain([h[:3] for h in heads], precision="mixed")
     b.run(graph=False)
     before = [b.gain(i) for i in range(len(heads))]
+    split_before = b.config()["split16"]
     res = b.tune()
     print("tune", res, b.config(), flush=True)
-    assert res and all(t > 0 for _, _, t in res)
-    assert {r[0] for r in res} == {64, 128}
+    assert res and all(t > 0 for _, _, _, t in res)
+    assert {r[0] for r in res if not r[2]} == {64, 128}
+    assert any(r[2] for r in res)                # the split-FP16 candidate
     assert not b.run(graph=True)                 # eager again after tuning
     assert b.run(graph=True)
+    same = b.config()["split16"] == split_before
     for i in range(len(heads)):
-        assert rel_fro(b.gain(i), before[i]) < 1e-12
+        # the same update kernel: the same sums; the other one: both refined
+        assert rel_fro(b.gain(i), before[i]) < (1e-12 if same else 1e-7)
         assert rel_fro(b.gain(i), G64[i]) < 1e-6
 
 
