@@ -7,9 +7,11 @@
 //   MAT 7.3 variables         OpenMEEGMaths/include/MatlabIO.H:28-120, 268-394
 //   raw binary                OpenMEEGMaths/include/TrivialBinIO.H:34-195
 //   ASCII                     OpenMEEGMaths/include/AsciiIO.H:34-265
-// The MAT files are written uncompressed and contiguous (the reference's
-// matio writes zlib-compressed chunks); readers of either see the same
-// variables, classes, shapes and values.
+// The MAT files store their variables as the reference's matio writes them:
+// dense datasets (those with a MATLAB class) in DEFLATE-9 chunks whose sides
+// are powers of two of at most 4096 elements in all (the chunk shapes of the
+// reference's own files), sparse components contiguous.  Byte identity is
+// not possible: the header records the creating library and time.
 #include <hdf5.h>
 
 #include <algorithm>
@@ -451,12 +453,42 @@ void mat_read(const std::string& path, const omh_linop_info& li, double* data, i
     }
 }
 
+// Chunk of a compressed dataset: per dimension the largest power of two not
+// above it, then the largest side halved while the chunk holds more than 4096
+// elements.
+std::vector<hsize_t> chunk_of(const std::vector<hsize_t>& dims) {
+    std::vector<hsize_t> c(dims.size());
+    hsize_t total = 1;
+    for (size_t i = 0; i < dims.size(); ++i) {
+        hsize_t p = 1;
+        while (2 * p <= dims[i]) p *= 2;
+        c[i] = p;
+        total *= p;
+    }
+    while (total > 4096) {
+        size_t big = c.size() - 1;
+        for (size_t i = c.size(); i-- > 0;)
+            if (c[i] > c[big]) big = i;
+        c[big] /= 2;
+        total /= 2;
+    }
+    return c;
+}
+
+// Dense datasets (with a MATLAB class) are written in DEFLATE-9 chunks, the
+// components of sparse matrices (no class) contiguous.
 void write_dataset(hid_t loc, const char* name, hid_t file_type, hid_t mem_type, const std::vector<hsize_t>& dims,
                    const void* buf, const char* cls) {
     H5Handle sp(H5Screate_simple((int)dims.size(), dims.data(), nullptr), H5Sclose, "dataspace");
-    H5Handle d(H5Dcreate2(loc, name, file_type, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose, name);
     hsize_t count = 1;
     for (hsize_t x : dims) count *= x;
+    H5Handle dcpl(H5Pcreate(H5P_DATASET_CREATE), H5Pclose, "dataset creation properties");
+    if (cls && count > 0) {
+        const std::vector<hsize_t> chunk = chunk_of(dims);
+        h5check(H5Pset_chunk(dcpl, (int)chunk.size(), chunk.data()), "chunk");
+        h5check(H5Pset_deflate(dcpl, 9), "deflate");
+    }
+    H5Handle d(H5Dcreate2(loc, name, file_type, sp, H5P_DEFAULT, dcpl, H5P_DEFAULT), H5Dclose, name);
     if (count > 0) h5check(H5Dwrite(d, mem_type, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf), name);
     if (cls) write_class(d, cls);
 }

@@ -102,12 +102,22 @@ def _structure(path):
     return [ln.strip() for ln in lines]
 
 
+def _layout(path):
+    """Per dataset its storage layout (CHUNKED with the chunk shape, or
+    CONTIGUOUS) and filters, from h5dump -p -H (compressed sizes left out)."""
+    out = subprocess.run([H5DUMP, "-p", "-H", path], capture_output=True, text=True, check=True).stdout
+    keep = ("DATASET", "CHUNKED", "CONTIGUOUS", "COMPACT", "COMPRESSION DEFLATE")
+    return [ln.strip() for ln in out.splitlines() if ln.strip().startswith(keep)]
+
+
 @pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump not available")
 @pytest.mark.parametrize("name", ["hm", "dsm", "h2em", "ss2mm"])
 def test_mat_files_hold_the_reference_variables(tmp_path, name):
     """A MAT file written here holds the same variables as the reference's
     (names, MATLAB classes and fields, HDF5 types and shapes) -- what matio
-    reads -- and starts with a MATLAB 7.3 header in a 512-byte user block."""
+    reads -- stored the same way (chunk shapes, DEFLATE level, contiguous
+    sparse components), and starts with a MATLAB 7.3 header in a 512-byte
+    user block."""
     ref = os.path.join(MAT, "Head1." + name)
     kind, val = omio.read(ref)
     f = str(tmp_path / (name + ".mat"))
@@ -118,6 +128,9 @@ def test_mat_files_hold_the_reference_variables(tmp_path, name):
     else:
         omio.write_full(f, val)
     assert _structure(f) == _structure(ref)
+    # the reference's storage: dense datasets in DEFLATE-9 chunks of the same
+    # shape, sparse components contiguous
+    assert _layout(f) == _layout(ref), (_layout(f), _layout(ref))
     head = open(f, "rb").read(128)
     assert head.startswith(b"MATLAB 7.3 MAT-file") and head[124:128] == b"\x00\x02IM"
     kind2, val2 = omio.read(f)
