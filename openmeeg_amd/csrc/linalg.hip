@@ -2166,7 +2166,7 @@ struct WSplit {
     _Float16* h = nullptr;
     _Float16* l = nullptr;
     float* tw = nullptr;
-    int64_t rp = 0;   // columns per slice (the right-hand sides rounded up to 128)
+    int64_t rp = 0;   // columns per slice (the right-hand sides rounded up to 256)
 };
 
 template <typename T, bool BACK, int SV>
@@ -2318,83 +2318,86 @@ __global__ void __launch_bounds__(256) k_span_col_scale(const float* __restrict_
 // Split-FP16 product of the triangular solves:
 //   W(row, c) -= s[row] tw[c] sum_k A(row, k) / s[row] . (h + l)(k, c)
 // over k < SOLVE_GROUP NB, for the rows of the spans (128-row tiles) and the
-// columns c < r (128-column tiles), with
+// columns c < r (256-column tiles), with
 //   forward (TRANS = false): A(row, k) = L(row, k0 + k), the group's columns
 //     of the rows below it, s the row scales;
 //   backward (TRANS = true): A(row, k) = L(k0 + k, row), the group's rows of
 //     the earlier columns, s the column scales;
 // and (h + l) = W(k0 + k, c) / tw[c] the group's split rows of W.  A / s is
 // split into FP16 h + l as it is staged into LDS; the three products
-// Al Bh + Ah Bl + Ah Bh are summed in FP32 by MFMA.  256 threads, 2 x 2 waves
-// of 64 x 64, k-slices of 32 double-buffered in LDS (the XOR swizzle of the
-// split factor's update), operands staged through registers one slice
-// ahead.  Workgroups go to the XCDs in runs of consecutive tiles (the tiles
-// of one row tile share its A rows).
+// Al Bh + Ah Bl + Ah Bh are summed in FP32 by MFMA.  512 threads, 2 x 4 waves
+// of 64 x 64, so that a tile of L is read once for 256 right-hand sides;
+// k-slices of 32 double-buffered in LDS (the XOR swizzle of the split
+// factor's update), operands staged through two register slots (slot s % 2
+// holds slice s, loaded two slices ahead).  Workgroups go to the XCDs in runs
+// of consecutive tiles (the tiles of one row tile share its A rows).
+constexpr int SW_COLS = 256;
 template <bool TRANS>
-__global__ void __launch_bounds__(256) k_solve_split(const float* __restrict__ L, int64_t lda, int64_t k0, Spans sp,
-                                                     int64_t nct, int64_t per, const float* __restrict__ s,
-                                                     WSplit ws, float* __restrict__ W, int64_t ldw, int64_t r) {
-    __shared__ half8_t S[2][4][S_TILE * 4];
+__global__ void __launch_bounds__(512) k_solve_split_wide(const float* __restrict__ L, int64_t lda, int64_t k0,
+                                                          Spans sp, int64_t nct, int64_t per,
+                                                          const float* __restrict__ s, WSplit ws,
+                                                          float* __restrict__ W, int64_t ldw, int64_t r) {
+    __shared__ half8_t SA[2][2][S_TILE * 4];
+    __shared__ half8_t SB[2][2][SW_COLS * 4];
     constexpr int NKS = SOLVE_GROUP * FACT_NB / SK;
     const int64_t t = xcd_tile(blockIdx.x, per);
     int64_t row0, rend;
     if (!span_tile(sp, t / nct, S_TILE, row0, rend)) return;
-    const int64_t c0 = (t % nct) * S_TILE;
+    const int64_t c0 = (t % nct) * SW_COLS;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    // A staging: forward, row tid % 128 and k in [16 (tid / 128), + 16);
-    // backward, row tid / 2 and k in [16 (tid % 2), + 16)
-    const int ar = TRANS ? tid >> 1 : tid & 127, akh = TRANS ? tid & 1 : tid >> 7;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    // A staging: forward, row tid % 128 and k in [8 (tid / 128), + 8);
+    // backward, row tid / 4 and k in [8 (tid % 4), + 8)
+    const int ar = TRANS ? tid >> 2 : tid & 127, akq = TRANS ? tid & 3 : tid >> 7;
     const int64_t arow = row0 + ar;
     const bool a_in = arow < rend;
     const float a_inv = a_in ? 1.f / s[arow] : 0.f;   // exact: s is a power of two
-    float av[16];
-    half8_t bv[2][2];
-    auto gload = [&](int ks) {
-        const int64_t kb = k0 + (int64_t)ks * SK + 16 * akh;
+    const float* arow_p = TRANS ? L + arow * lda : L + arow;
+    float av[2][8];
+    half8_t bv[2][2][2];
+    auto gload = [&](int ks, int slot) {
+        const int64_t kb = k0 + (int64_t)ks * SK + 8 * akq;
         if (TRANS) {
-            const float4_t* p = reinterpret_cast<const float4_t*>(L + kb + arow * lda);
+            const float4_t* p = reinterpret_cast<const float4_t*>(arow_p + kb);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 2; ++u) {
                 const float4_t v = a_in ? p[u] : float4_t{0.f, 0.f, 0.f, 0.f};
-                av[4 * u] = v[0];
-                av[4 * u + 1] = v[1];
-                av[4 * u + 2] = v[2];
-                av[4 * u + 3] = v[3];
+                av[slot][4 * u] = v[0];
+                av[slot][4 * u + 1] = v[1];
+                av[slot][4 * u + 2] = v[2];
+                av[slot][4 * u + 3] = v[3];
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) av[u] = a_in ? L[arow + (kb + u) * lda] : 0.f;
+            for (int u = 0; u < 8; ++u) av[slot][u] = a_in ? arow_p[(kb + u) * lda] : 0.f;
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const int q = tid + 256 * p, cr = q >> 2, ch = q & 3;
+                const int q = tid + 512 * p, cr = q >> 2, ch = q & 3;
                 const _Float16* src = a == 0 ? ws.h : ws.l;
-                bv[a][p] = *reinterpret_cast<const half8_t*>(src + (((int64_t)ks * ws.rp + c0 + cr) * SK + 8 * ch));
+                bv[slot][a][p] =
+                    *reinterpret_cast<const half8_t*>(src + (((int64_t)ks * ws.rp + c0 + cr) * SK + 8 * ch));
             }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int slot, int buf) {
+        half8_t h, l;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            half8_t h, l;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float x = av[8 * c + e] * a_inv;
-                const _Float16 hx = (_Float16)x;
-                h[e] = hx;
-                l[e] = (_Float16)(x - (float)hx);
-            }
-            S[buf][0][swz(ar, 2 * akh + c)] = h;
-            S[buf][1][swz(ar, 2 * akh + c)] = l;
+        for (int e = 0; e < 8; ++e) {
+            const float x = av[slot][e] * a_inv;
+            const _Float16 hx = (_Float16)x;
+            h[e] = hx;
+            l[e] = (_Float16)(x - (float)hx);
         }
+        SA[buf][0][swz(ar, akq)] = h;
+        SA[buf][1][swz(ar, akq)] = l;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const int q = tid + 256 * p, cr = q >> 2, ch = q & 3;
-                S[buf][2 + a][swz(cr, ch)] = bv[a][p];
+                const int q = tid + 512 * p, cr = q >> 2, ch = q & 3;
+                SB[buf][a][swz(cr, ch)] = bv[slot][a][p];
             }
     };
     float4_t acc[4][4];
@@ -2402,23 +2405,25 @@ __global__ void __launch_bounds__(256) k_solve_split(const float* __restrict__ L
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-    gload(0);
-    lstore(0);
+    gload(0, 0);
+    lstore(0, 0);
+    gload(1, 1);
+    gload(2, 0);
     __syncthreads();
     const int fr = lane & 15, fc = lane >> 4;
-    for (int ks = 0; ks < NKS; ++ks) {
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {   // unrolled: register slots stay static
         const int buf = ks & 1;
-        if (ks + 1 < NKS) gload(ks + 1);
         half8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            ah[a] = S[buf][0][swz(wm + 16 * a + fr, fc)];
-            al[a] = S[buf][1][swz(wm + 16 * a + fr, fc)];
+            ah[a] = SA[buf][0][swz(wm + 16 * a + fr, fc)];
+            al[a] = SA[buf][1][swz(wm + 16 * a + fr, fc)];
         }
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            bh[b] = S[buf][2][swz(wn + 16 * b + fr, fc)];
-            bl[b] = S[buf][3][swz(wn + 16 * b + fr, fc)];
+            bh[b] = SB[buf][0][swz(wn + 16 * b + fr, fc)];
+            bl[b] = SB[buf][1][swz(wn + 16 * b + fr, fc)];
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -2435,8 +2440,11 @@ __global__ void __launch_bounds__(256) k_solve_split(const float* __restrict__ L
 #pragma unroll
             for (int b = 0; b < 4; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        if (ks + 1 < NKS) lstore(buf ^ 1);
+        // slice ks + 1 (register slot (ks + 1) % 2) into the other buffer,
+        // then that slot gets slice ks + 3
+        if (ks + 1 < NKS) lstore((ks + 1) & 1, buf ^ 1);
         __syncthreads();
+        if (ks + 3 < NKS) gload(ks + 3, (ks + 1) & 1);
     }
     // epilogue: lane holds rows 4 (lane / 16) + e of column lane % 16 of a block
 #pragma unroll
@@ -2463,14 +2471,14 @@ struct SplitSolveBuf {
 };
 
 size_t split_solve_bytes(int64_t n, int64_t r) {
-    const size_t rp = (size_t)cdiv(r, S_TILE) * S_TILE;
+    const size_t rp = (size_t)cdiv(r, SW_COLS) * SW_COLS;   // whole column tiles of the product kernel
     return 2 * (size_t)SOLVE_GROUP * FACT_NB * rp * sizeof(_Float16) + rp * sizeof(float) + (size_t)n * sizeof(float) +
            3 * 256;
 }
 
 static SplitSolveBuf split_solve_buffers(void* p, int64_t n, int64_t r) {
     SplitSolveBuf b;
-    const size_t rp = (size_t)cdiv(r, S_TILE) * S_TILE;
+    const size_t rp = (size_t)cdiv(r, SW_COLS) * SW_COLS;
     char* q = (char*)p;
     b.ws.rp = (int64_t)rp;
     b.ws.h = (_Float16*)q;
@@ -2511,10 +2519,14 @@ static void split_solve_update(bool trans, const float* L, int64_t lda, int64_t 
         else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, sb.s);
         OMH_LAUNCH_CHECK();
         const Spans sp = make_spans(part, S_TILE);
-        const int64_t nct = cdiv(r, S_TILE), total = sp.tile0[sp.n] * nct, per = cdiv(total, 8);
         ProfScope ps("solve_update_f32s", 2.0 * (double)rows * (double)r * (double)(SOLVE_GROUP * FACT_NB), st);
-        if (trans) k_solve_split<true><<<(unsigned)(8 * per), 256, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W, ldw, r);
-        else k_solve_split<false><<<(unsigned)(8 * per), 256, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W, ldw, r);
+        const int64_t nct = cdiv(r, SW_COLS), total = sp.tile0[sp.n] * nct, per = cdiv(total, 8);
+        if (trans)
+            k_solve_split_wide<true><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W,
+                                                                          ldw, r);
+        else
+            k_solve_split_wide<false><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W,
+                                                                           ldw, r);
         OMH_LAUNCH_CHECK();
     }
 }
