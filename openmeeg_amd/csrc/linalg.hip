@@ -1115,9 +1115,179 @@ __device__ __forceinline__ void syrk_split_tile(int64_t m, int64_t mp, int64_t r
     }
 }
 
+// The same tile with a software pipeline: operand slices go global -> LDS
+// directly (global_load_lds, 16 bytes per lane, two LDS buffers) and each
+// wave's fragments are double-buffered in registers, so that a wave reads
+// slice ks + 1 from LDS while its MFMAs work on slice ks, and one barrier per
+// slice separates the buffers' reuse.  Slice ks + 2 is fetched into the
+// buffer that slice ks left.  Every entry sums the same products in the same
+// order as syrk_split_tile: the results are bitwise equal.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int KP>
+__device__ __forceinline__ void syrk_split_tile_pp(int64_t m, int64_t mp, int64_t r_off,
+                                                   const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
+                                                   const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
+                                                   const float* __restrict__ scale, float* __restrict__ C,
+                                                   int64_t ldc, int64_t tm, int64_t tn, SliceClock clk = SliceClock{}) {
+    // two separate arrays, so that the compiler can tell a DMA into one
+    // from fragment reads of the other
+    __shared__ half8_t S0[4][S_TILE * 4];
+    __shared__ half8_t S1[4][S_TILE * 4];
+    __shared__ int rot_sh;
+    const int64_t row0 = r_off + tm * S_TILE, col0 = r_off + tn * S_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    constexpr int NKS = KP / SK;
+    static_assert(NKS >= 2 && (NKS & (NKS - 1)) == 0, "the slice clock wraps k with a mask");
+    uint64_t t_start = 0;
+    int rot = 0;
+    if (clk.period) {
+        if (tid == 0) {
+            t_start = __builtin_amdgcn_s_memrealtime();
+            const unsigned per = clk.period[0];
+            rot_sh = per ? (int)((t_start / per) & (uint64_t)(NKS - 1)) : 0;
+        }
+        __syncthreads();
+        rot = __builtin_amdgcn_readfirstlane(rot_sh);
+    }
+    // DMA: wave w moves rows 32 w + 16 p + lane / 4 (p = 0, 1) of each of
+    // the four arrays; lane L lands in LDS slot 64 (8 w + 4 p) + L, so it
+    // fetches the 16-byte chunk that the swizzle puts there (the swizzle
+    // depends on the row's low 4 bits only)
+    const int rl = lane >> 2;
+    const uint32_t goff = (uint32_t)(((32 * wave + rl) * SK + 8 * ((lane & 3) ^ (((rl >> 1) ^ (rl >> 2)) & 3))) * 2);
+    auto dma = [&](int ks, half8_t (*Sb)[S_TILE * 4]) {
+        const int64_t so = (int64_t)((ks + rot) & (NKS - 1)) * mp * SK;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const char* base = reinterpret_cast<const char*>((a == 0 ? Ah : a == 1 ? Al : a == 2 ? Bh : Bl) + so +
+                                                             (a < 2 ? row0 : col0) * SK);
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+                __builtin_amdgcn_global_load_lds((const void*)(base + goff + p * 16 * SK * 2),
+                                                 (lds_void*)&Sb[a][(32 * wave + 16 * p) * 4], 16, 0, 0);
+        }
+    };
+    const int fr = lane & 15, fc = lane >> 4;
+    const int lo = swz(fr, fc);   // slot of (row r0 + fr, chunk fc) is r0 * 4 + lo for r0 % 16 == 0
+    // fragments: [0, 4) ah, [4, 8) al, [8, 12) bh, [12, 16) bl
+    half8_t F[2][16];
+    auto frags = [&](const half8_t (*Sb)[S_TILE * 4], half8_t (&f)[16]) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            f[a] = Sb[0][(wm + 16 * a) * 4 + lo];
+            f[4 + a] = Sb[1][(wm + 16 * a) * 4 + lo];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            f[8 + b] = Sb[2][(wn + 16 * b) * 4 + lo];
+            f[12 + b] = Sb[3][(wn + 16 * b) * 4 + lo];
+        }
+    };
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const int64_t i0 = tm * S_TILE, j0 = tn * S_TILE;   // tile origin in C
+    const bool vec = tm != tn && i0 + S_TILE <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+    float4 c_pre[4][4], si_pre[4];
+    // prologue: slices 0 and 1 in flight, slice 0's fragments in registers
+    dma(0, S0);
+    dma(1, S1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    frags(S0, F[0]);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();   // buffer 0 is free for slice 2
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+        half8_t (&f)[16] = F[ks & 1];
+        half8_t (&g)[16] = F[(ks + 1) & 1];
+        // MFMAs touch no memory: keep each slice's products inside its own
+        // region, after the barrier that ends the previous one
+        __builtin_amdgcn_sched_barrier(0);
+        if (ks + 2 < NKS) dma(ks + 2, (ks & 1) ? S1 : S0);
+        if (ks + 1 < NKS) {
+            frags((ks & 1) ? S0 : S1, g);
+        } else if (vec) {
+            // last slice: the C tile and the row scales take the registers of
+            // the next slice's fragments
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                si_pre[a] = *reinterpret_cast<const float4*>(scale + r_off + i0 + wm + 16 * a + 4 * fc);
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    c_pre[a][b] = *reinterpret_cast<const float4*>(
+                        C + (i0 + wm + 16 * a + 4 * fc) + (j0 + wn + 16 * b + fr) * ldc);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[4 + a], f[8 + b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[a], f[12 + b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[a], f[8 + b], acc[a][b], 0, 0, 0);
+        // fragment reads one per three products, the slice fetch first
+        if (ks + 1 < NKS) {
+            if (ks + 2 < NKS) __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);   // VMEM read (DMA)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);   // MFMA
+            }
+        }
+        if (ks + 2 < NKS) {
+            // slice ks + 1's fragments are read everywhere and slice ks + 2
+            // has landed everywhere before either buffer changes role
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    }
+    // epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of a block
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t gi = i0 + wm + 16 * a + 4 * fc, gj = j0 + wn + 16 * b + fr;
+            const float sj = scale[r_off + gj];
+            if (vec) {
+                const float4 si = si_pre[a];
+                float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
+                float4 c = c_pre[a][b];
+                c.x -= si.x * sj * acc[a][b][0];
+                c.y -= si.y * sj * acc[a][b][1];
+                c.z -= si.z * sj * acc[a][b][2];
+                c.w -= si.w * sj * acc[a][b][3];
+                *cp = c;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = gi + r;
+                    if (i < m && gj < m && i >= gj) C[i + gj * ldc] -= scale[r_off + i] * sj * acc[a][b][r];
+                }
+            }
+        }
+    if (clk.period_out && blockIdx.x == 0 && tid == 0) {   // this tile's time per slice, for the next launch
+        const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t_start;
+        clk.period_out[0] = (unsigned)max((uint64_t)1, min(dt / NKS, (uint64_t)1000000));
+    }
+}
+
 // The same on the tile of map that this workgroup takes: the lower triangle,
 // or (kind 6) the lower part of a strip of whole tile columns.
-template <int KP>
+template <int KP, bool PP>
 __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r_off, const _Float16* __restrict__ Ah,
                                                 const _Float16* __restrict__ Al, const _Float16* __restrict__ Bh,
                                                 const _Float16* __restrict__ Bl, const float* __restrict__ scale,
@@ -1134,7 +1304,10 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
         tm = remap_tile(rm, tm);
         tn = remap_tile(rm, tn);
     }
-    syrk_split_tile<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, tm, tn, clk);
+    if (PP)
+        syrk_split_tile_pp<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, tm, tn, clk);
+    else
+        syrk_split_tile<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, tm, tn, clk);
 }
 
 // The same update on 256 x 256 tiles: 512 threads, 8 waves of 64 x 128
@@ -1146,7 +1319,6 @@ __device__ __forceinline__ void syrk_split_body(int64_t m, int64_t mp, int64_t r
 // separate LDS arrays; a slice's fragments are all read before the next
 // slice's loads are issued, and the loads run under the slice's MFMAs.
 // Operand rows up to r_off + 256 * ceil(m / 256) exist (zero padding).
-typedef __attribute__((address_space(3))) void lds_void;
 constexpr int S2_TILE = 256;
 
 template <int KP>
@@ -1290,24 +1462,24 @@ __global__ void __launch_bounds__(512) k_syrk_split256(int64_t m, int64_t mp, in
     }
 }
 
-// The trailing matrix below the next block group.
-template <int KP>
+// The trailing matrix below the next block group (PP: the pipelined tile).
+template <int KP, bool PP>
 __global__ void __launch_bounds__(256) k_syrk_split(int64_t m, int64_t mp, int64_t r_off,
                                                     const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                     const float* __restrict__ scale, float* __restrict__ C,
                                                     int64_t ldc, TileMap map, TileRemap rm, SliceClock clk) {
-    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, rm, clk);
+    syrk_split_body<KP, PP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, rm, clk);
 }
 
 // The next block group's columns (a strip of whole tile columns, map kind 6).
-template <int KP>
+template <int KP, bool PP>
 __global__ void __launch_bounds__(256) k_syrk_strip(int64_t m, int64_t mp, int64_t r_off,
                                                     const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
                                                     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl,
                                                     const float* __restrict__ scale, float* __restrict__ C,
                                                     int64_t ldc, TileMap map) {
-    syrk_split_body<KP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, TileRemap{});
+    syrk_split_body<KP, PP>(m, mp, r_off, Ah, Al, Bh, Bl, scale, C, ldc, map, TileRemap{});
 }
 
 // Grouped rank-nb splits: workgroup b takes 64-row block b - blk0 of the
@@ -1379,8 +1551,8 @@ __global__ void __launch_bounds__(256) k_syrk_split_grouped(const SplitDesc* __r
     upper_tile((int64_t)blockIdx.x - d.tile0, tn, tm);   // tn <= tm
     const size_t arr = (size_t)d.mp * FACT_NB;
     const _Float16* Ah = (const _Float16*)d.Ah;
-    syrk_split_tile<FACT_NB>(d.m, d.mp, 0, Ah, Ah + arr, Ah + 2 * arr, Ah + 3 * arr,
-                             reinterpret_cast<const float*>(Ah + 4 * arr), d.C, d.ldc, tm, tn);
+    syrk_split_tile_pp<FACT_NB>(d.m, d.mp, 0, Ah, Ah + arr, Ah + 2 * arr, Ah + 3 * arr,
+                                reinterpret_cast<const float*>(Ah + 4 * arr), d.C, d.ldc, tm, tn);
 }
 
 // flag = 1 if any entry of the m x n block is non-zero.
@@ -1794,6 +1966,12 @@ static bool slice_clock_on() {
     return !(e && e[0] == '0');
 }
 
+// the software-pipelined 128 x 128 tile (OMH_SYRK_PP=0: the register-staged one)
+static bool syrk_pipelined() {
+    const char* e = std::getenv("OMH_SYRK_PP");
+    return !(e && e[0] == '0');
+}
+
 static int syrk_tile() {
     const char* e = std::getenv("OMH_SYRK_TILE");
     return (e && std::atoi(e) == 256) ? 256 : 128;
@@ -1836,6 +2014,7 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         OMH_LAUNCH_CHECK();
     }
     int clk_launch = 0;
+    const bool pp = syrk_pipelined();
     const float tiny = 1e-30f;
     LookAhead& la = lookahead();
     auto start_of = [&](int64_t b) { return b * FACT_NB; };
@@ -1908,8 +2087,12 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
             const int64_t tiles = nc * (nc + 1) / 2 + (ntm - nc) * nc;
             const TileMap map{6, ntm, 0, nc, tiles, cdiv(tiles, 8)};
             ProfScope ps("factor_strip_f32s", (double)(2 * ms - w2 + 1) * (double)w2 * (double)S_KP, st);
-            k_syrk_strip<S_KP><<<(unsigned)(8 * map.per), 256, 0, st>>>(ms, mp, 0, Ah, Al, Bh, Bl, sc,
-                                                                     A + r0 + r0 * lda, lda, map);
+            if (pp)
+                k_syrk_strip<S_KP, true><<<(unsigned)(8 * map.per), 256, 0, st>>>(ms, mp, 0, Ah, Al, Bh, Bl, sc,
+                                                                                A + r0 + r0 * lda, lda, map);
+            else
+                k_syrk_strip<S_KP, false><<<(unsigned)(8 * map.per), 256, 0, st>>>(ms, mp, 0, Ah, Al, Bh, Bl, sc,
+                                                                                 A + r0 + r0 * lda, lda, map);
             OMH_LAUNCH_CHECK();
         }
         // the next group's prologue on the side stream (its columns are
@@ -1950,9 +2133,12 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
                 if (big)
                     k_syrk_split256<S_KP><<<grid, 512, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
                                                                 lda, map, rm, clk);
+                else if (pp)
+                    k_syrk_split<S_KP, true><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
+                                                                   lda, map, rm, clk);
                 else
-                    k_syrk_split<S_KP><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda,
-                                                             map, rm, clk);
+                    k_syrk_split<S_KP, false><<<grid, 256, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
+                                                                    lda, map, rm, clk);
             }
             OMH_LAUNCH_CHECK();
         }

@@ -407,6 +407,33 @@ def test_split_factor_tiles_agree(om, monkeypatch):
         del F128, L128, L256
 
 
+def test_split_factor_pipelined_tile_bitwise(om, monkeypatch):
+    """The software-pipelined 128 x 128 split-update tile (operand slices by
+    direct global-to-LDS loads, fragments double-buffered in registers) and
+    the register-staged one (OMH_SYRK_PP=0) sum every entry's products in the
+    same order: bitwise equal factors, with and without the block structure,
+    on 4 layers of 2562 vertices (n = 25608)."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    H = _head_mat_device(g, far_field=True)
+    monkeypatch.setenv("OMH_SYRK_CLOCK", "0")   # k slices in order: bitwise comparable
+    for structure in (None, g.factor_segments()):
+        monkeypatch.setenv("OMH_SYRK_PP", "0")
+        F0 = factor(H, n, "mixed16", structure=structure)
+        monkeypatch.delenv("OMH_SYRK_PP")
+        L0 = F0.L.view(n, n).t().tril().clone()
+        del F0
+        F1 = factor(H, n, "mixed16", structure=structure)
+        L1 = F1.L.view(n, n).t().tril()
+        print("n=%d structure %s: max |L_pp - L_staged| %.3e" % (n, structure is not None,
+                                                                 float((L1 - L0).abs().max())), flush=True)
+        assert torch.equal(L0, L1)
+        del F1, L1, L0
+
+
 def test_split_factor_slice_clock(om, monkeypatch):
     """The split update's slice clock changes only the order in which each
     tile sums its k slices (the tiles in flight start at the clock's slice
