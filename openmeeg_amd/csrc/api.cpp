@@ -852,8 +852,8 @@ int omh_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t
                  int64_t lda, const double* d_B, int64_t ldb, double beta, double* d_C, int64_t ldc, void* stream) {
     return guard([&] {
         if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("negative size");
-        omh::gemm<double>(trans_a != 0, trans_b != 0, false, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
-                          S(stream));
+        omh::gemm_entry<double>(trans_a != 0, trans_b != 0, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
+                                S(stream));
     });
 }
 
@@ -861,8 +861,8 @@ int omh_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t
                  int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C, int64_t ldc, void* stream) {
     return guard([&] {
         if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("negative size");
-        omh::gemm<float>(trans_a != 0, trans_b != 0, false, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
-                         S(stream));
+        omh::gemm_entry<float>(trans_a != 0, trans_b != 0, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc,
+                               S(stream));
     });
 }
 

@@ -45,7 +45,8 @@ enum ScratchSlot {
     SCR_SMALL = 5,
     SCR_PAIRQ = 6,
     SCR_SRCCOEF = 7,   // source-matrix coefficients (the source matrix may run beside the factorisation)
-    SCR_NSLOTS = 8
+    SCR_GEMM = 8,      // split-K partial products of the C-ABI GEMM entry points
+    SCR_NSLOTS = 9
 };
 void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st);
 void release_scratch();

@@ -369,6 +369,37 @@ __global__ void __launch_bounds__(256) k_gemm(int64_t m, int64_t n, int64_t k, T
     gemm_tile<T, TA, TB, LOWER, WB, EDGE>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 
+// Split-K product for outputs of few tiles with a long inner dimension:
+// workgroup (x, y) computes tile x of W_y = op(A) op(B) restricted to the
+// y-th chunk of kc inner indices (the last one shorter), W_y at W + y m n
+// with leading dimension m.  k_splitk_sum then forms C = alpha sum_y W_y +
+// beta C, summing the chunks in order (deterministic).
+template <typename T, bool TA, bool TB, int WB, bool EDGE>
+__global__ void __launch_bounds__(256) k_gemm_splitk(int64_t m, int64_t n, int64_t k, int64_t kc,
+                                                     const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+                                                     int64_t ldb, T* __restrict__ W, TileMap map) {
+    int64_t tm, tn;
+    map_tile(map, blockIdx.x, tm, tn);
+    const int64_t k0 = (int64_t)blockIdx.y * kc;
+    const int64_t ks = k - k0 < kc ? k - k0 : kc;
+    gemm_tile<T, TA, TB, false, WB, EDGE>(m, n, ks, T(1), A + (TA ? k0 : k0 * lda), lda, B + (TB ? k0 * ldb : k0),
+                                          ldb, T(0), W + (int64_t)blockIdx.y * m * n, m, tm, tn);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_splitk_sum(const T* __restrict__ W, int nsplit, int64_t m, int64_t n, T alpha,
+                                                    T beta, T* __restrict__ C, int64_t ldc) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        T acc = W[t];
+        for (int y = 1; y < nsplit; ++y) acc += W[t + y * total];
+        const int64_t i = t % m, j = t / m;
+        T v = alpha * acc;
+        if (beta != T(0)) v += beta * C[i + j * ldc];
+        C[i + j * ldc] = v;
+    }
+}
+
 // Grouped GEMM: the tiles of ng independent products (descriptor d[g], its
 // tiles numbered from d[g].tile0) in one launch; each workgroup walks the
 // tile list with stride gridDim.x (persistent when the grid is capped).
@@ -1791,6 +1822,61 @@ void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha
     else
         gemm_launch<T, 2>(ta, tb, lower, vec, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st, prof);
 }
+
+// Split-K launch: interior tiles unguarded (16-byte aligned operands and
+// chunk starts), the partial last tile row and column guarded.
+template <typename T, bool TA, bool TB>
+static void splitk_tiles(bool vec, int64_t m, int64_t n, int64_t k, int64_t kc, int nsplit, const T* A, int64_t lda,
+                         const T* B, int64_t ldb, T* W, hipStream_t st) {
+    constexpr int WB = 2, TILE = 64;
+    const int64_t ntm = cdiv(m, TILE), ntn = cdiv(n, TILE);
+    if (!vec) {
+        k_gemm_splitk<T, TA, TB, WB, true><<<dim3((unsigned)(ntm * ntn), nsplit), 256, 0, st>>>(
+            m, n, k, kc, A, lda, B, ldb, W, TileMap{0, ntm, 0, 0});
+        OMH_LAUNCH_CHECK();
+        return;
+    }
+    const int64_t fm = m / TILE, fn = n / TILE;
+    if (fm * fn > 0)
+        k_gemm_splitk<T, TA, TB, WB, false><<<dim3((unsigned)(fm * fn), nsplit), 256, 0, st>>>(
+            m, n, k, kc, A, lda, B, ldb, W, TileMap{0, fm, 0, 0});
+    if (fm < ntm)   // partial last tile row (all tile columns)
+        k_gemm_splitk<T, TA, TB, WB, true><<<dim3((unsigned)ntn, nsplit), 256, 0, st>>>(
+            m, n, k, kc, A, lda, B, ldb, W, TileMap{2, ntm, fm, 0});
+    if (fn < ntn && fm > 0)   // partial last tile column (full tile rows only)
+        k_gemm_splitk<T, TA, TB, WB, true><<<dim3((unsigned)fm, nsplit), 256, 0, st>>>(
+            m, n, k, kc, A, lda, B, ldb, W, TileMap{3, ntm, 0, fn});
+    OMH_LAUNCH_CHECK();
+}
+
+template <typename T>
+void gemm_entry(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st) {
+    if (m <= 0 || n <= 0) return;
+    const int64_t tiles = cdiv(m, 64) * cdiv(n, 64);
+    if (tiles >= 512 || k < 8 * std::max(m, n) || k < 4096) {
+        gemm<T>(ta, tb, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+        return;
+    }
+    // about 2048 workgroups, chunks of whole 64-index slices
+    const int nsplit = (int)std::min<int64_t>(16, std::max<int64_t>(2, 2048 / tiles));
+    const int64_t kc = cdiv(cdiv(k, nsplit), 64) * 64;
+    const int ns = (int)cdiv(k, kc);
+    T* W = (T*)scratch(SCR_GEMM, sizeof(T) * (size_t)ns * (size_t)m * (size_t)n, st);
+    const bool vec = aligned_for_vec(A, lda) && aligned_for_vec(B, ldb);
+    if (ta && tb) splitk_tiles<T, true, true>(vec, m, n, k, kc, ns, A, lda, B, ldb, W, st);
+    else if (ta) splitk_tiles<T, true, false>(vec, m, n, k, kc, ns, A, lda, B, ldb, W, st);
+    else if (tb) splitk_tiles<T, false, true>(vec, m, n, k, kc, ns, A, lda, B, ldb, W, st);
+    else splitk_tiles<T, false, false>(vec, m, n, k, kc, ns, A, lda, B, ldb, W, st);
+    k_splitk_sum<T><<<(unsigned)std::min<int64_t>(cdiv(m * n, 256), 4096), 256, 0, st>>>(W, ns, m, n, alpha, beta, C,
+                                                                                         ldc);
+    OMH_LAUNCH_CHECK();
+}
+
+template void gemm_entry<double>(bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t,
+                                 const double*, int64_t, double, double*, int64_t, hipStream_t);
+template void gemm_entry<float>(bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*,
+                                int64_t, float, float*, int64_t, hipStream_t);
 
 template void gemm<double>(bool, bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
                            int64_t, double, double*, int64_t, hipStream_t, const char*);

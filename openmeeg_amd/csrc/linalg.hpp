@@ -124,6 +124,14 @@ template <typename T>
 void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
           const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st, const char* prof = nullptr);
 
+// The same product (not lower) for callers outside the library: outputs of
+// fewer than 512 64 x 64 tiles with an inner dimension of at least 4096 and
+// 8 x the larger output side run split-K (chunks of k concurrently into the
+// SCR_GEMM scratch slot, then summed in order).
+template <typename T>
+void gemm_entry(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st);
+
 struct FactorInfo {
     int info = 0;              // 0, or 1 + index of the first (near-)zero pivot
     const void* dinv = nullptr;

@@ -254,6 +254,39 @@ def test_gemm_large_tile_order(om, dtype, ta):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("shape", [(128, 320, 40000), (70, 133, 9001)])
+def test_gemm_split_k_matches_numpy(om, dtype, ta, tb, shape):
+    """Outputs of few tiles with a long inner dimension (the gain product
+    Y^T D has 256 x 5000 outputs over 102408 unknowns) run split-K: chunks
+    of the inner dimension in one launch, summed in order.  Full tiles and
+    partial edge tiles, aligned and odd leading dimensions, beta != 0."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    m, n, k = shape
+    rng = np.random.default_rng(7 + 4 * ta + 2 * tb + m)
+    npdt = np.float64 if dtype == "f64" else np.float32
+    A = rng.standard_normal((k, m) if ta else (m, k)).astype(npdt)
+    B = rng.standard_normal((n, k) if tb else (k, n)).astype(npdt)
+    Cm = rng.standard_normal((m, n)).astype(npdt)
+    dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    dB = torch.from_numpy(np.asfortranarray(B).ravel(order="F").copy()).cuda()
+    dC = torch.from_numpy(np.asfortranarray(Cm).ravel(order="F").copy()).cuda()
+    fn = _lib.lib().omh_gemm_f64 if dtype == "f64" else _lib.lib().omh_gemm_f32
+    for rep in range(2):   # the second call reuses the grown workspace
+        dC.copy_(torch.from_numpy(np.asfortranarray(Cm).ravel(order="F").copy()))
+        _lib.check(fn(ta, tb, m, n, k, 1.5, C.c_void_p(dA.data_ptr()), A.shape[0], C.c_void_p(dB.data_ptr()),
+                      B.shape[0], -0.5, C.c_void_p(dC.data_ptr()), m, C.c_void_p(0)))
+        torch.cuda.synchronize()
+        ref = 1.5 * (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64) - 0.5 * Cm
+        got = dC.cpu().numpy().reshape(n, m).T
+        err = rel_fro(got, ref)
+        print("%s ta=%d tb=%d %s: %.2e" % (dtype, ta, tb, shape, err), flush=True)
+        assert err < (1e-14 if dtype == "f64" else 2e-6)
+
+
 def test_n_blocks_chunked(om, monkeypatch):
     """The N blocks are assembled in column chunks (1 GiB of per-triangle
     sums each, several chunks from about 2200 vertices per surface on); with
