@@ -457,9 +457,13 @@ void BatchPlan::run_eager(hipStream_t st) {
             launch(ir_post_, st);
             double worst = 0.0;
             for (Head& H : heads_) {
-                const double dn = frob_norm_f32((const float*)H.W, H.ld, H.n, H.ne, st);
-                const double xn = frob_norm_f64(H.Y, H.ld, H.n, H.ne, st);
-                H.rel_update = xn > 0 ? dn / xn : 0.0;
+                if (it == 0) {   // Y was zero: the first update is Y itself
+                    H.rel_update = 1.0;
+                } else {
+                    const double dn = frob_norm_f32((const float*)H.W, H.ld, H.n, H.ne, st);
+                    const double xn = frob_norm_f64(H.Y, H.ld, H.n, H.ne, st);
+                    H.rel_update = xn > 0 ? dn / xn : 0.0;
+                }
                 worst = std::max(worst, H.rel_update);
             }
             state = refine_state(it, worst, prev, refine_tol_);

@@ -1316,6 +1316,231 @@ __device__ __forceinline__ void syrk_split_tile_pp(int64_t m, int64_t mp, int64_
     }
 }
 
+// Wide tiles of the split update: 256 rows x 128 columns of C, 512 threads
+// as 4 x 2 waves of 64 x 64 (the same wave tile, products and summation
+// order as syrk_split_tile, so the factor is bitwise the same), one
+// workgroup per CU.  Tile coordinates are in units of 128 rows of the
+// (possibly remapped) trailing matrix: column unit tb, and the row pair
+// (2 ta, 2 ta + 1), each unit remapped on its own (rows are contiguous
+// within a unit).  A slice (32 k) of the tile's operands is 48 KB of LDS,
+// rows stacked as [Ah 256 | Al 256 | Bh 128 | Bl 128], chunks swizzled as
+// in swz(); three slices live in a ring of buffers, so a slice is fetched by
+// direct global-to-LDS loads two slices before it is read, and each wave's
+// fragments are double-buffered in registers.  The barrier that closes a
+// slice waits only for the loads of the next slice (the s_barrier is issued
+// on its own: a workgroup fence would also wait for the loads in flight).
+constexpr int WROWS = 256, WCOLS = 128, WSTACK = 2 * WROWS + 2 * WCOLS;
+
+// Tiles (ta, tb) with tb <= min(2 ta + 1, ntm - 1) in bands of 4 row pairs
+// (8 units): the band's columns left of its diagonal block, column by
+// column, then its diagonal block column by column.  Full bands before band
+// b hold 16 b^2 + 4 b tiles.
+__device__ __forceinline__ void wide_band_tile(int64_t t, int64_t ntm, int64_t& ta, int64_t& tb) {
+    const int64_t nta = (ntm + 1) / 2, nfull = nta / 4;
+    int64_t b = (int64_t)((sqrt(16.0 + 64.0 * (double)t) - 4.0) / 32.0);
+    while (b > 0 && 16 * b * b + 4 * b > t) --b;
+    while (16 * (b + 1) * (b + 1) + 4 * (b + 1) <= t) ++b;
+    if (b > nfull) b = nfull;
+    const int64_t h = min((int64_t)4, nta - 4 * b);
+    int64_t o = t - (16 * b * b + 4 * b);
+    if (o < 8 * b * h) {
+        tb = o / h;
+        ta = 4 * b + o % h;
+        return;
+    }
+    o -= 8 * b * h;
+    for (int j = 0; j < 8; ++j) {   // column 8 b + j: rows a >= ceil((j - 1) / 2) of the band
+        const int64_t c = 8 * b + j;
+        if (c >= ntm) break;
+        const int a0 = j <= 1 ? 0 : j / 2;
+        const int64_t cnt = h - a0 > 0 ? h - a0 : 0;
+        if (o < cnt) {
+            tb = c;
+            ta = 4 * b + a0 + o;
+            return;
+        }
+        o -= cnt;
+    }
+    ta = tb = -1;   // past the last tile
+}
+
+template <int KP>
+__global__ void __launch_bounds__(512) k_syrk_split_wide(int64_t m, int64_t mp, int64_t r_off,
+                                                         const _Float16* __restrict__ Ah,
+                                                         const _Float16* __restrict__ Al,
+                                                         const _Float16* __restrict__ Bh,
+                                                         const _Float16* __restrict__ Bl,
+                                                         const float* __restrict__ scale, float* __restrict__ C,
+                                                         int64_t ldc, int64_t ntm, int64_t ntiles, int64_t per,
+                                                         TileRemap rm, SliceClock clk) {
+    __shared__ half8_t S0[WSTACK * 4];
+    __shared__ half8_t S1[WSTACK * 4];
+    __shared__ half8_t S2[WSTACK * 4];
+    __shared__ int rot_sh;
+    const int64_t t = xcd_tile(blockIdx.x, per);
+    if (t >= ntiles) return;
+    int64_t ta, tb;
+    wide_band_tile(t, ntm, ta, tb);
+    if (ta < 0) return;
+    // units (remapped): the row pair and the column; a second row unit past
+    // the last one is not read (its rows alias the first unit's) nor written
+    const bool h1 = 2 * ta + 1 < ntm;
+    const int64_t u0 = rm.n ? remap_tile(rm, 2 * ta) : 2 * ta;
+    const int64_t u1 = h1 ? (rm.n ? remap_tile(rm, 2 * ta + 1) : 2 * ta + 1) : u0;
+    const int64_t uc = rm.n ? remap_tile(rm, tb) : tb;
+    // the wave index is uniform: the DMA rows and their arrays stay scalar
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    constexpr int NKS = KP / SK;
+    static_assert(NKS >= 3 && (NKS & (NKS - 1)) == 0, "the slice clock wraps k with a mask");
+    uint64_t t_start = 0;
+    int rot = 0;
+    if (clk.period) {
+        if (tid == 0) {
+            t_start = __builtin_amdgcn_s_memrealtime();
+            const unsigned pd = clk.period[0];
+            rot_sh = pd ? (int)((t_start / pd) & (uint64_t)(NKS - 1)) : 0;
+        }
+        __syncthreads();
+        rot = __builtin_amdgcn_readfirstlane(rot_sh);
+    }
+    // DMA: wave w moves 16-row groups of each array, rows 32 w + 16 q
+    // (q < 2) of Ah and Al and rows 16 w of Bh and Bl; lane L lands in slot
+    // 4 R + L (R the group's first stacked row), so it fetches the 16-byte
+    // chunk that the swizzle puts there (the swizzle depends on the row's
+    // low 4 bits only)
+    const int rl = lane >> 2;
+    const uint32_t goff = (uint32_t)((rl * SK + 8 * ((lane & 3) ^ (((rl >> 1) ^ (rl >> 2)) & 3))) * 2);
+    const int64_t arow = (wave < 4 ? u0 : u1) * 128 + 32 * (wave & 3), brow = uc * 128 + 16 * wave;
+    auto dma = [&](int ks, half8_t* Sb) {
+        const int64_t so = (int64_t)((ks + rot) & (NKS - 1)) * mp * SK;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            const _Float16* src = p < 2 ? Ah : p < 4 ? Al : p == 4 ? Bh : Bl;
+            const int64_t row = p < 4 ? arow + 16 * (p & 1) : brow;
+            const int R = p < 2 ? 32 * wave + 16 * p : p < 4 ? WROWS + 32 * wave + 16 * (p - 2)
+                                                     : p == 4 ? 2 * WROWS + 16 * wave : 2 * WROWS + WCOLS + 16 * wave;
+            const char* base = reinterpret_cast<const char*>(src + so + (r_off + row) * SK);
+            __builtin_amdgcn_global_load_lds((const void*)(base + goff), (lds_void*)&Sb[R * 4], 16, 0, 0);
+        }
+    };
+    const int fr = lane & 15, fc = lane >> 4;
+    const int lo = swz(fr, fc);
+    half8_t F[2][16];
+    auto frags = [&](const half8_t* Sb, half8_t (&f)[16]) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            f[a] = Sb[(wm + 16 * a) * 4 + lo];
+            f[4 + a] = Sb[(WROWS + wm + 16 * a) * 4 + lo];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            f[8 + b] = Sb[(2 * WROWS + wn + 16 * b) * 4 + lo];
+            f[12 + b] = Sb[(2 * WROWS + WCOLS + wn + 16 * b) * 4 + lo];
+        }
+    };
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    // this wave's 64 rows lie in one unit; C origin of the wave's block
+    const bool wvalid = wm < 128 || h1;
+    const int64_t gi0 = (wm < 128 ? u0 : u1) * 128 + (wm & 127), gj0 = uc * 128 + wn;
+    const bool vec = gi0 >= gj0 + 64 && gi0 + 64 <= m && (ldc % 4 == 0) && (reinterpret_cast<uintptr_t>(C) % 16 == 0);
+    float4 c_pre[4][4], si_pre[4];
+    auto ring = [&](int i) -> half8_t* { return i == 0 ? S0 : i == 1 ? S1 : S2; };
+    dma(0, S0);
+    dma(1, S1);
+    dma(2, S2);
+    __builtin_amdgcn_s_waitcnt(0x7C);   // vmcnt(12) lgkmcnt(0): slice 0 has landed
+    asm volatile("s_barrier" ::: "memory");
+    frags(S0, F[0]);
+    __builtin_amdgcn_s_waitcnt(0x76);   // vmcnt(6): slice 1 too; slice 0's fragments are read
+    asm volatile("s_barrier" ::: "memory");
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+        half8_t (&f)[16] = F[ks & 1];
+        half8_t (&g)[16] = F[(ks + 1) & 1];
+        __builtin_amdgcn_sched_barrier(0);
+        if (ks + 3 < NKS) dma(ks + 3, ring(ks % 3));
+        if (ks + 1 < NKS) {
+            frags(ring((ks + 1) % 3), g);
+        } else if (vec && wvalid) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                si_pre[a] = *reinterpret_cast<const float4*>(scale + r_off + gi0 + 16 * a + 4 * fc);
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    c_pre[a][b] = *reinterpret_cast<const float4*>(C + (gi0 + 16 * a + 4 * fc) +
+                                                                   (gj0 + 16 * b + fr) * ldc);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[4 + a], f[8 + b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[a], f[12 + b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(f[a], f[8 + b], acc[a][b], 0, 0, 0);
+        if (ks + 1 < NKS) {
+            if (ks + 3 < NKS) __builtin_amdgcn_sched_group_barrier(0x020, 6, 0);   // VMEM read (DMA)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);   // MFMA
+            }
+        }
+        if (ks + 2 < NKS) {
+            // slice ks + 2 has landed everywhere (only this slice's fetch of
+            // ks + 3 may still be in flight) and slice ks + 1's fragments are
+            // read everywhere before its buffer takes slice ks + 4
+            __builtin_amdgcn_sched_barrier(0);
+            if (ks + 3 < NKS) __builtin_amdgcn_s_waitcnt(0x76);   // vmcnt(6) lgkmcnt(0)
+            else __builtin_amdgcn_s_waitcnt(0x70);                // vmcnt(0) lgkmcnt(0)
+            asm volatile("s_barrier" ::: "memory");
+        }
+    }
+    // epilogue: lane holds rows 4 (lane / 16) + r of column lane % 16 of a block
+    if (wvalid) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int64_t gi = gi0 + 16 * a + 4 * fc, gj = gj0 + 16 * b + fr;
+                const float sj = scale[r_off + gj];
+                if (vec) {
+                    const float4 si = si_pre[a];
+                    float4* cp = reinterpret_cast<float4*>(C + gi + gj * ldc);
+                    float4 c = c_pre[a][b];
+                    c.x -= si.x * sj * acc[a][b][0];
+                    c.y -= si.y * sj * acc[a][b][1];
+                    c.z -= si.z * sj * acc[a][b][2];
+                    c.w -= si.w * sj * acc[a][b][3];
+                    *cp = c;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t i = gi + r;
+                        if (i < m && gj < m && i >= gj) C[i + gj * ldc] -= scale[r_off + i] * sj * acc[a][b][r];
+                    }
+                }
+            }
+    }
+    if (clk.period_out && blockIdx.x == 0 && tid == 0) {
+        const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t_start;
+        clk.period_out[0] = (unsigned)max((uint64_t)1, min(dt / NKS, (uint64_t)1000000));
+    }
+}
+
 // The same on the tile of map that this workgroup takes: the lower triangle,
 // or (kind 6) the lower part of a strip of whole tile columns.
 template <int KP, bool PP>
@@ -2052,6 +2277,21 @@ static bool slice_clock_on() {
     return !(e && e[0] == '0');
 }
 
+// 256 x 128 tiles for the main split update when it has enough of them
+// (OMH_SYRK_WIDE=0: 128 x 128 tiles throughout)
+static bool syrk_wide() {
+    const char* e = std::getenv("OMH_SYRK_WIDE");
+    return !(e && e[0] == '0');
+}
+
+// Tiles of k_syrk_split_wide over ntm units of 128 rows: row pair ta holds
+// min(2 ta + 2, ntm) of them.
+static int64_t wide_tiles(int64_t ntm) {
+    int64_t t = 0;
+    for (int64_t ta = 0; 2 * ta < ntm; ++ta) t += std::min<int64_t>(2 * ta + 2, ntm);
+    return t;
+}
+
 // the software-pipelined 128 x 128 tile (OMH_SYRK_PP=0: the register-staged one)
 static bool syrk_pipelined() {
     const char* e = std::getenv("OMH_SYRK_PP");
@@ -2101,6 +2341,7 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     }
     int clk_launch = 0;
     const bool pp = syrk_pipelined();
+    const bool wide_ok = pp && syrk_wide();
     const float tiny = 1e-30f;
     LookAhead& la = lookahead();
     auto start_of = [&](int64_t b) { return b * FACT_NB; };
@@ -2208,15 +2449,23 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
             const TileMap map = tiles >= 1024 ? TileMap{4, ntm, 0, 0, tiles, cdiv(tiles, 8)}
                                               : TileMap{1, ntm, 0, 0, tiles, 0};
             const unsigned grid = (unsigned)(map.kind == 4 ? 8 * map.per : tiles);
+            // 256 x 128 tiles (units of 128 rows, the same remap) while they
+            // give at least 4 per CU
+            const int64_t wtiles = (!big && wide_ok) ? wide_tiles(ntm) : 0;
+            const bool wide = wtiles >= 1024;
             ProfScope ps("factor_update_f32s", (double)rows * (double)(rows + 1) * (double)S_KP, st);
             if (tiles > 0) {
                 SliceClock clk;
-                if (clk_slots && map.kind == 4) {
+                if (clk_slots && (map.kind == 4 || wide)) {
                     clk.period = clk_slots + (clk_launch & 1);
                     clk.period_out = clk_slots + ((clk_launch + 1) & 1);
                     ++clk_launch;
                 }
-                if (big)
+                if (wide) {
+                    const int64_t per = cdiv(wtiles, 8);
+                    k_syrk_split_wide<S_KP><<<(unsigned)(8 * per), 512, 0, st>>>(
+                        mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda, lda, ntm, wtiles, per, rm, clk);
+                } else if (big)
                     k_syrk_split256<S_KP><<<grid, 512, 0, st>>>(mb, mp, w2, Ah, Al, Bh, Bl, sc, A + r2 + r2 * lda,
                                                                 lda, map, rm, clk);
                 else if (pp)
@@ -3024,9 +3273,16 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
         sym_solve_impl<float>(L32, n, ld32, work32, R32, n, r, st, bf, buf.split ? &sb : nullptr);
         k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
         OMH_LAUNCH_CHECK();
-        const double dn = frob_norm_with<float>(R32, n, n, r, buf.part, st);
-        const double xn = frob_norm_with<double>(X, ldx, n, r, buf.part, st);
-        rel = xn > 0 ? dn / xn : (dn > 0 ? 1.0 : 0.0);
+        if (it == 0) {
+            // X was zero, so the first update is X itself (relative size 1):
+            // no norms, no wait for the device before the residual (an
+            // all-zero right-hand side converges on the next pass instead)
+            rel = 1.0;
+        } else {
+            const double dn = frob_norm_with<float>(R32, n, n, r, buf.part, st);
+            const double xn = frob_norm_with<double>(X, ldx, n, r, buf.part, st);
+            rel = xn > 0 ? dn / xn : (dn > 0 ? 1.0 : 0.0);
+        }
         state = refine_state(it, rel, prev, tol);
         if (state != REFINE_CONTINUE) { ++it; break; }
         prev = rel;

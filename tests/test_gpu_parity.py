@@ -467,6 +467,35 @@ def test_split_factor_pipelined_tile_bitwise(om, monkeypatch):
         del F1, L1, L0
 
 
+def test_split_factor_wide_tiles_bitwise(om, monkeypatch):
+    """The main split update on 256 x 128 tiles (8 waves, one workgroup per
+    CU, a ring of three operand slices in LDS; the row pair of a tile may
+    come from two intervals of the block structure and the last pair may
+    have one unit) sums every entry's products in the same order as the
+    128 x 128 tiles (OMH_SYRK_WIDE=0): bitwise equal factors, with and
+    without the block structure, on 4 layers of 2562 vertices (n = 25608,
+    an odd number of 128-row units below the first groups)."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    H = _head_mat_device(g, far_field=True)
+    monkeypatch.setenv("OMH_SYRK_CLOCK", "0")   # k slices in order: bitwise comparable
+    for structure in (g.factor_segments(), None):
+        monkeypatch.setenv("OMH_SYRK_WIDE", "0")
+        F0 = factor(H, n, "mixed16", structure=structure)
+        monkeypatch.delenv("OMH_SYRK_WIDE")
+        L0 = F0.L.view(n, n).t().tril().clone()
+        del F0
+        F1 = factor(H, n, "mixed16", structure=structure)
+        L1 = F1.L.view(n, n).t().tril()
+        print("n=%d structure %s: max |L_wide - L_128| %.3e" % (n, structure is not None,
+                                                               float((L1 - L0).abs().max())), flush=True)
+        assert torch.equal(L0, L1)
+        del F1, L1, L0
+
+
 def test_split_factor_slice_clock(om, monkeypatch):
     """The split update's slice clock changes only the order in which each
     tile sums its k slices (the tiles in flight start at the clock's slice
