@@ -101,7 +101,7 @@ BatchPlan::BatchPlan(const std::vector<BatchHeadInput>& in, int precision) : pre
         // matrices and workspaces
         H.H = dalloc<double>((size_t)H.ld * H.n, owned_);
         H.A = f32 ? (void*)dalloc<float>((size_t)H.ld * H.n, owned_) : (void*)H.H;
-        H.X = dalloc<char>(te * (size_t)H.ld * FACT_NB, owned_);
+        H.X = dalloc<char>(te * (size_t)H.ld * FACT_NB * (f32 ? S4_PANELS : 1), owned_);
         H.Dinv = dalloc<char>(te * (size_t)H.nblk * FACT_NB * FACT_NB, owned_);
         H.sgn = dalloc<char>(te * (size_t)H.ld, owned_);
         H.D = dalloc<double>((size_t)H.ld * H.nd, owned_);
@@ -109,7 +109,7 @@ BatchPlan::BatchPlan(const std::vector<BatchHeadInput>& in, int precision) : pre
         H.W = f32 ? (void*)dalloc<float>((size_t)H.ld * H.ne, owned_) : (void*)H.Y;
         H.R = f32 ? dalloc<double>((size_t)H.ld * H.ne, owned_) : nullptr;
         H.tmp = dalloc<char>(te * (size_t)FACT_NB * H.ne, owned_);
-        if (f32 && H.n > FACT_NB) H.sp = dalloc<char>(split_operand_bytes(H.n), owned_);
+        if (f32 && H.n > FACT_NB) H.sp = dalloc<char>(std::max(split_operand_bytes(H.n), split4_operand_bytes(H.n)), owned_);
         H.G = dalloc<double>((size_t)H.ne * H.nd, owned_);
         H.info = info_ + h;
         max_nblk_ = std::max(max_nblk_, H.nblk);
@@ -156,6 +156,7 @@ BatchPlan::~BatchPlan() {
     if (d_diag_) (void)hipFree(d_diag_);
     if (d_elem_) (void)hipFree(d_elem_);
     if (d_split_) (void)hipFree(d_split_);
+    if (d_split4_) (void)hipFree(d_split4_);
     if (ev_in_) (void)hipEventDestroy(ev_in_);
     if (ev_out_) (void)hipEventDestroy(ev_out_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -214,6 +215,55 @@ void BatchPlan::add_split(Segment& seg, std::vector<SplitDesc>& ds) {
     seg.push_back(op);
 }
 
+// The rank-1024 split of every listed matrix, then their updates.
+void BatchPlan::add_split4(Segment& seg, std::vector<Split4Desc>& ds) {
+    if (ds.empty()) return;
+    int64_t blocks = 0, tiles = 0;
+    for (Split4Desc& d : ds) {
+        d.blk0 = blocks;
+        d.tile0 = tiles;
+        blocks += d.mp / 64;
+        const int64_t t = cdiv(d.m, 128);
+        tiles += t * (t + 1) / 2;
+    }
+    Op op;
+    op.off = split4_descs_.size();
+    op.ng = (int)ds.size();
+    split4_descs_.insert(split4_descs_.end(), ds.begin(), ds.end());
+    op.type = Op::SPLIT4;
+    op.blocks = blocks;
+    seg.push_back(op);
+    op.type = Op::SPLIT4_UPDATE;
+    op.blocks = tiles;
+    seg.push_back(op);
+}
+
+// Descriptors of block group q's rank-1024 update: the heads whose group
+// is whole and has rows below it (the four panels' rows from the group's
+// end).
+std::vector<Split4Desc> BatchPlan::split4_group(int64_t q) const {
+    std::vector<Split4Desc> sd;
+    for (const Head& H : heads_) {
+        const int64_t k0 = q * S4_PANELS * FACT_NB, gend = k0 + S4_PANELS * FACT_NB;
+        if (gend >= H.n) continue;
+        Split4Desc d{};
+        for (int p = 0; p < S4_PANELS; ++p) {
+            const int64_t e = k0 + (p + 1) * FACT_NB;   // panel p holds the rows from e
+            d.X[p] = (const float*)((const char*)H.X + sizeof(float) * (size_t)H.ld * FACT_NB * p);
+            d.ldx[p] = round_up(H.n - e, 4);
+            d.off[p] = gend - e;
+        }
+        d.sgn = (const float*)H.sgn + k0;
+        d.m = H.n - gend;
+        d.mp = round_up(d.m, 128);
+        d.Ah = H.sp;
+        d.C = (float*)H.A + (size_t)gend + (size_t)gend * H.ld;
+        d.ldc = H.ld;
+        sd.push_back(d);
+    }
+    return sd;
+}
+
 void BatchPlan::add_diag(Segment& seg, const std::vector<DiagDesc>& ds, bool f32) {
     if (ds.empty()) return;
     Op op;
@@ -234,6 +284,51 @@ void BatchPlan::factor_ops(Segment& seg) {
     std::vector<ElemDesc> zero;
     for (const Head& H : heads_) zero.push_back(ElemDesc{ELEM_ZERO_INT, nullptr, 1, H.info, 1, 1, 1, nullptr});
     add_elem(seg, zero);
+    if (f32 && trail_split_) {
+        // Super-steps of S4_PANELS blocks (the single-head factor's grouping):
+        // inside a group each block's diagonal factor, panel (its own slot of
+        // X) and the update of the group's later columns; then one rank-1024
+        // split update of everything below the group, a quarter of the
+        // trailing matrix's reads and writes of rank-256 steps.
+        for (int64_t q = 0; q * S4_PANELS < max_nblk_; ++q) {
+            for (int b = 0; b < S4_PANELS; ++b) {
+                const int64_t s = q * S4_PANELS + b, k = s * FACT_NB;
+                std::vector<DiagDesc> diag;
+                std::vector<GemmJob> panel, ulow, urect;
+                std::vector<ElemDesc> scale;
+                for (const Head& H : heads_) {
+                    if (k >= H.n) continue;
+                    const int nb = (int)std::min<int64_t>(FACT_NB, H.n - k);
+                    void* Li = (char*)H.Dinv + te * (size_t)s * FACT_NB * FACT_NB;
+                    diag.push_back(DiagDesc{at(H, H.A, k, k), H.ld, nb, Li, H.sgn, H.info, k});
+                    const int64_t e = k + nb, m = H.n - e;
+                    if (m <= 0) continue;
+                    const int64_t ldx = round_up(m, 4);
+                    void* Xb = (char*)H.X + te * (size_t)H.ld * FACT_NB * b;
+                    void* A21 = at(H, H.A, e, k);
+                    panel.push_back(GemmJob{m, nb, nb, 1.0, A21, H.ld, Li, FACT_NB, 0.0, Xb, ldx});
+                    scale.push_back(ElemDesc{ELEM_SCALE_COLS_F32, Xb, ldx, A21, H.ld, m, nb, (char*)H.sgn + te * k});
+                    // the group's later columns [e, e + w): their lower
+                    // square and the rows below it
+                    const int64_t w = std::min<int64_t>((q + 1) * S4_PANELS * FACT_NB, H.n) - e;
+                    if (w > 0) {
+                        ulow.push_back(GemmJob{w, w, nb, -1.0, Xb, ldx, A21, H.ld, 1.0, at(H, H.A, e, e), H.ld});
+                        if (m > w)
+                            urect.push_back(GemmJob{m - w, w, nb, -1.0, (char*)Xb + te * w, ldx, A21, H.ld, 1.0,
+                                                    at(H, H.A, e + w, e), H.ld});
+                    }
+                }
+                add_diag(seg, diag, f32);
+                add_gemm(seg, panel, f32, false, true, false, false);
+                add_elem(seg, scale);
+                add_gemm(seg, ulow, f32, false, true, true, false);
+                add_gemm(seg, urect, f32, false, true, false, false);
+            }
+            std::vector<Split4Desc> sd = split4_group(q);
+            add_split4(seg, sd);
+        }
+        return;
+    }
     for (int64_t s = 0; s < max_nblk_; ++s) {
         const int64_t k = s * FACT_NB;
         std::vector<DiagDesc> diag;
@@ -323,6 +418,7 @@ void BatchPlan::build_schedule() {
     diag_descs_.clear();
     elem_descs_.clear();
     split_descs_.clear();
+    split4_descs_.clear();
     prep_.clear();
     factor_.clear();
     solve_.clear();
@@ -385,6 +481,7 @@ void BatchPlan::upload_tables() {
     upload_table(diag_descs_, d_diag_, cap_diag_);
     upload_table(elem_descs_, d_elem_, cap_elem_);
     upload_table(split_descs_, d_split_, cap_split_);
+    upload_table(split4_descs_, d_split4_, cap_split4_);
 }
 
 void BatchPlan::launch(const Segment& seg, hipStream_t st) {
@@ -395,6 +492,8 @@ void BatchPlan::launch(const Segment& seg, hipStream_t st) {
             case Op::ELEM: launch_grouped_elem(d_elem_ + op.off, op.ng, op.max_elems, st); break;
             case Op::SPLIT: launch_grouped_split(d_split_ + op.off, op.ng, op.blocks, st); break;
             case Op::SPLIT_UPDATE: launch_grouped_split_update(d_split_ + op.off, op.ng, op.blocks, st); break;
+            case Op::SPLIT4: launch_grouped_split4(d_split4_ + op.off, op.ng, op.blocks, st); break;
+            case Op::SPLIT4_UPDATE: launch_grouped_split4_update(d_split4_ + op.off, op.ng, op.blocks, st); break;
         }
     }
 }
@@ -595,40 +694,42 @@ std::vector<BatchTuneResult> BatchPlan::tune(hipStream_t caller) {
             trail_split_ = false;
         }
     }
-    if (f32) {   // the split-FP16 update: split and update of the same steps
-        std::vector<std::vector<SplitDesc>> per_step;
-        for (int64_t s : steps) {
-            const int64_t k = s * FACT_NB;
-            std::vector<SplitDesc> sd;
+    if (f32) {
+        // the split-FP16 update in rank-1024 super-steps: the split and
+        // update of the block groups that start at the sampled steps, per
+        // FACT_NB columns
+        std::vector<std::vector<Split4Desc>> per_group;
+        std::vector<int64_t> groups;
+        for (int64_t s : steps) groups.push_back(s / S4_PANELS);
+        groups.erase(std::unique(groups.begin(), groups.end()), groups.end());
+        for (int64_t q : groups) {
+            std::vector<Split4Desc> sd = split4_group(q);
             int64_t blocks = 0, tiles = 0;
-            for (const Head& H : heads_) {
-                const int64_t e = k + FACT_NB, m = H.n - e;
-                if (m <= 0) continue;
-                SplitDesc d{(const float*)H.X, round_up(m, 4), (const float*)H.sgn + k, m, round_up(m, 128),
-                            FACT_NB, H.sp, (float*)((char*)H.A + te * (size_t)(e + e * H.ld)), H.ld, blocks, tiles};
+            for (Split4Desc& d : sd) {
+                d.blk0 = blocks;
+                d.tile0 = tiles;
                 blocks += d.mp / 64;
-                const int64_t t = cdiv(m, 128);
+                const int64_t t = cdiv(d.m, 128);
                 tiles += t * (t + 1) / 2;
-                sd.push_back(d);
             }
-            if (!sd.empty()) per_step.push_back(sd);
+            if (!sd.empty()) per_group.push_back(sd);
         }
-        if (!per_step.empty()) {
-            std::vector<SplitDesc*> dd(per_step.size());
-            std::vector<int64_t> nb(per_step.size()), nt(per_step.size());
-            for (size_t i = 0; i < per_step.size(); ++i) {
-                const auto& sd = per_step[i];
-                OMH_HIP(hipMalloc((void**)&dd[i], sd.size() * sizeof(SplitDesc)));
-                OMH_HIP(hipMemcpy(dd[i], sd.data(), sd.size() * sizeof(SplitDesc), hipMemcpyHostToDevice));
-                const SplitDesc& last = sd.back();
+        if (!per_group.empty()) {
+            std::vector<Split4Desc*> dd(per_group.size());
+            std::vector<int64_t> nb(per_group.size()), nt(per_group.size());
+            for (size_t i = 0; i < per_group.size(); ++i) {
+                const auto& sd = per_group[i];
+                OMH_HIP(hipMalloc((void**)&dd[i], sd.size() * sizeof(Split4Desc)));
+                OMH_HIP(hipMemcpy(dd[i], sd.data(), sd.size() * sizeof(Split4Desc), hipMemcpyHostToDevice));
+                const Split4Desc& last = sd.back();
                 const int64_t t = cdiv(last.m, 128);
                 nb[i] = last.blk0 + last.mp / 64;
                 nt[i] = last.tile0 + t * (t + 1) / 2;
             }
             auto pass = [&]() {
-                for (size_t i = 0; i < per_step.size(); ++i) {
-                    launch_grouped_split(dd[i], (int)per_step[i].size(), nb[i], stream_);
-                    launch_grouped_split_update(dd[i], (int)per_step[i].size(), nt[i], stream_);
+                for (size_t i = 0; i < per_group.size(); ++i) {
+                    launch_grouped_split4(dd[i], (int)per_group[i].size(), nb[i], stream_);
+                    launch_grouped_split4_update(dd[i], (int)per_group[i].size(), nt[i], stream_);
                 }
             };
             pass();   // warm-up
@@ -639,8 +740,9 @@ std::vector<BatchTuneResult> BatchPlan::tune(hipStream_t caller) {
             OMH_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             OMH_HIP(hipEventElapsedTime(&ms, a, b));
-            for (SplitDesc* p : dd) OMH_HIP(hipFree(p));
-            const double t = ms * 1e-3 / reps;
+            for (Split4Desc* p : dd) OMH_HIP(hipFree(p));
+            // per FACT_NB columns, comparable with the rank-FACT_NB candidates
+            const double t = ms * 1e-3 / reps / S4_PANELS * (double)steps.size() / (double)per_group.size();
             res.push_back(BatchTuneResult{128, 0, true, t});
             if (t < best) {
                 best = t;

@@ -21,7 +21,8 @@ struct BatchHeadInput {
 struct BatchTuneResult {
     int tile;                       // 64 or 128
     int64_t grid_cap;               // 0: one workgroup per tile
-    bool split16;                   // split-FP16 update (128 x 128 tiles, one workgroup per tile)
+    bool split16;                   // split-FP16 update (rank-1024 super-steps, 128 x 128 tiles); seconds per
+                                    // 256 columns
     double seconds;                 // sampled trailing updates, per repetition
 };
 
@@ -64,7 +65,7 @@ private:
         int64_t n = 0, nv = 0, nt = 0, ld = 0, nd = 0, ne = 0, nblk = 0;
         double* H = nullptr;        // FP64 HeadMat, triangles-first, ld x n
         void* A = nullptr;          // matrix factored (FP32 copy, or H itself)
-        void* X = nullptr;          // panel
+        void* X = nullptr;          // panels (mixed: the four of a block group, ld x FACT_NB apart)
         void* Dinv = nullptr;       // inverses of the diagonal blocks
         void* sgn = nullptr;        // signs
         double* dip = nullptr;
@@ -77,20 +78,20 @@ private:
         void* W = nullptr;          // right-hand sides in factor precision
         double* R = nullptr;        // FP64 residual (mixed)
         void* tmp = nullptr;        // FACT_NB x ne
-        void* sp = nullptr;         // split-FP16 operands of the trailing update (mixed)
+        void* sp = nullptr;         // split-FP16 operands of the rank-1024 trailing update (mixed)
         double* G = nullptr;        // gain, ne x nd
         int* info = nullptr;
         double rel_update = 0.0;
     };
     struct Op {
-        enum Type { GEMM, DIAG, ELEM, SPLIT, SPLIT_UPDATE } type = ELEM;
+        enum Type { GEMM, DIAG, ELEM, SPLIT, SPLIT_UPDATE, SPLIT4, SPLIT4_UPDATE } type = ELEM;
         GemmLaunch gemm{};
         bool trailing = false;
         bool f32 = false;
         size_t off = 0;
         int ng = 0;
         int64_t max_elems = 0;
-        int64_t blocks = 0;         // SPLIT: 64-row blocks; SPLIT_UPDATE: tiles
+        int64_t blocks = 0;         // SPLIT(4): 64-row blocks; SPLIT(4)_UPDATE: tiles
     };
     typedef std::vector<Op> Segment;
 
@@ -102,6 +103,8 @@ private:
     void add_elem(Segment& seg, const std::vector<ElemDesc>& ds);
     void add_diag(Segment& seg, const std::vector<DiagDesc>& ds, bool f32);
     void add_split(Segment& seg, std::vector<SplitDesc>& ds);
+    void add_split4(Segment& seg, std::vector<Split4Desc>& ds);
+    std::vector<Split4Desc> split4_group(int64_t q) const;
     void upload_tables();
     void launch(const Segment& seg, hipStream_t st);
     void enqueue_assembly(hipStream_t st);
@@ -141,11 +144,13 @@ private:
     std::vector<DiagDesc> diag_descs_;
     std::vector<ElemDesc> elem_descs_;
     std::vector<SplitDesc> split_descs_;
+    std::vector<Split4Desc> split4_descs_;
     GemmDesc* d_gemm_ = nullptr;
     DiagDesc* d_diag_ = nullptr;
     ElemDesc* d_elem_ = nullptr;
     SplitDesc* d_split_ = nullptr;
-    size_t cap_gemm_ = 0, cap_diag_ = 0, cap_elem_ = 0, cap_split_ = 0;
+    Split4Desc* d_split4_ = nullptr;
+    size_t cap_gemm_ = 0, cap_diag_ = 0, cap_elem_ = 0, cap_split_ = 0, cap_split4_ = 0;
     Segment prep_, factor_, solve_, ir_init_, ir_pre_, ir_post_, ir_resid_, gain_;
 };
 

@@ -1811,6 +1811,87 @@ __global__ void __launch_bounds__(256) k_syrk_split_grouped(const SplitDesc* __r
                                 reinterpret_cast<const float*>(Ah + 4 * arr), d.C, d.ldc, tm, tn);
 }
 
+// Rank-1024 grouped splits (the batch's super-steps): workgroup b takes
+// 64-row block b - blk0 of the matrix it falls in; thread (row t % 64,
+// 64-wide k group t / 64), k in [256 p, 256 p + 256) from panel p, one
+// power-of-two scale per row over all 1024 columns.  Rows >= m are zero.
+__device__ __forceinline__ int desc4_of(const Split4Desc* __restrict__ d, int ng, int64_t b, bool tiles) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if ((tiles ? d[mid].tile0 : d[mid].blk0) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(64 * 4 * S_PANELS) k_split4_grouped(const Split4Desc* __restrict__ ds, int ng) {
+    static_assert(S_PANELS == S4_PANELS, "super-steps of the factorisation's block groups");
+    __shared__ float red[4 * S_PANELS][64];
+    const Split4Desc& d = ds[desc4_of(ds, ng, blockIdx.x, false)];
+    const int ri = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int64_t row = ((int64_t)blockIdx.x - d.blk0) * 64 + ri;
+    const int pn = kq >> 2;
+    const float* X = d.X[pn] + d.off[pn];
+    const int64_t ldx = d.ldx[pn];
+    const int kb = (kq & 3) * 64;   // first column inside its panel
+    float v[64];
+    float mx = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < 64; ++kk) {
+        v[kk] = row < d.m ? X[row + (int64_t)(kb + kk) * ldx] : 0.f;
+        mx = fmaxf(mx, fabsf(v[kk]));
+    }
+    red[kq][ri] = mx;
+    __syncthreads();
+    mx = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4 * S_PANELS; ++g) mx = fmaxf(mx, red[g][ri]);
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
+    const float inv = ldexpf(1.f, -e);
+    const size_t arr = (size_t)d.mp * S_KP;
+    _Float16* Ah = (_Float16*)d.Ah;
+    _Float16* Al = Ah + arr;
+    _Float16* Bh = Al + arr;
+    _Float16* Bl = Bh + arr;
+    if (kq == 0) reinterpret_cast<float*>(Bl + arr)[row] = ldexpf(1.f, e);
+    const float* sg = d.sgn + pn * FACT_NB;
+#pragma unroll
+    for (int kk = 0; kk < 64; kk += 8) {
+        half8_t h, l, bh, bl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float x = v[kk + q] * inv;
+            const _Float16 hx = (_Float16)x;
+            const _Float16 lx = (_Float16)(x - (float)hx);
+            const bool neg = sg[kb + kk + q] < 0.f;
+            h[q] = hx;
+            l[q] = lx;
+            bh[q] = neg ? -hx : hx;
+            bl[q] = neg ? -lx : lx;
+        }
+        const int k0 = kq * 64 + kk;
+        const int64_t off = ((int64_t)(k0 / SK) * d.mp + row) * SK + (k0 % SK);
+        *reinterpret_cast<half8_t*>(Ah + off) = h;
+        *reinterpret_cast<half8_t*>(Al + off) = l;
+        *reinterpret_cast<half8_t*>(Bh + off) = bh;
+        *reinterpret_cast<half8_t*>(Bl + off) = bl;
+    }
+}
+
+// Rank-1024 grouped split updates: workgroup b takes tile b - tile0 (lower
+// triangle, row by row) of the matrix it falls in.
+__global__ void __launch_bounds__(256) k_syrk_split4_grouped(const Split4Desc* __restrict__ ds, int ng) {
+    const Split4Desc& d = ds[desc4_of(ds, ng, blockIdx.x, true)];
+    int64_t tm, tn;
+    upper_tile((int64_t)blockIdx.x - d.tile0, tn, tm);   // tn <= tm
+    const size_t arr = (size_t)d.mp * S_KP;
+    const _Float16* Ah = (const _Float16*)d.Ah;
+    syrk_split_tile_pp<S_KP>(d.m, d.mp, 0, Ah, Ah + arr, Ah + 2 * arr, Ah + 3 * arr,
+                             reinterpret_cast<const float*>(Ah + 4 * arr), d.C, d.ldc, tm, tn);
+}
+
 // flag = 1 if any entry of the m x n block is non-zero.
 template <typename T>
 __global__ void __launch_bounds__(256) k_any_nonzero(const T* __restrict__ A, int64_t lda, int64_t m, int64_t n,
@@ -1974,6 +2055,23 @@ size_t split_operand_bytes(int64_t m) {
 void launch_grouped_split(const SplitDesc* d_descs, int ng, int64_t blocks, hipStream_t st) {
     if (ng <= 0 || blocks <= 0) return;
     k_split_grouped<<<(unsigned)blocks, 256, 0, st>>>(d_descs, ng);
+    OMH_LAUNCH_CHECK();
+}
+
+size_t split4_operand_bytes(int64_t m) {
+    const size_t mp = (size_t)cdiv(m, S_TILE) * S_TILE;
+    return 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float);
+}
+
+void launch_grouped_split4(const Split4Desc* d_descs, int ng, int64_t blocks, hipStream_t st) {
+    if (ng <= 0 || blocks <= 0) return;
+    k_split4_grouped<<<(unsigned)blocks, 64 * 4 * S_PANELS, 0, st>>>(d_descs, ng);
+    OMH_LAUNCH_CHECK();
+}
+
+void launch_grouped_split4_update(const Split4Desc* d_descs, int ng, int64_t tiles, hipStream_t st) {
+    if (ng <= 0 || tiles <= 0) return;
+    k_syrk_split4_grouped<<<(unsigned)tiles, 256, 0, st>>>(d_descs, ng);
     OMH_LAUNCH_CHECK();
 }
 

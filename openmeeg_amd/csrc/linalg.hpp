@@ -111,6 +111,28 @@ struct SplitDesc {
 };
 // Bytes of SplitDesc::Ah for m rows.
 size_t split_operand_bytes(int64_t m);
+
+// Rank-1024 split update of one matrix of a batch (a group of S4_PANELS = 4
+// panels of FACT_NB columns):
+// row r of the update is row off[p] + r of panel p (FP32, leading dimension
+// ldx[p]); sgn holds the 1024 signs of the group's columns.  Ah: 4 arrays of
+// mp x 1024 FP16 values, then mp row scales (split4_operand_bytes).
+constexpr int S4_PANELS = 4;
+struct Split4Desc {
+    const float* X[4];
+    int64_t ldx[4];
+    int64_t off[4];
+    const float* sgn;
+    int64_t m, mp;
+    void* Ah;
+    float* C;
+    int64_t ldc;
+    int64_t blk0;                   // first 64-row block of this matrix in the split launch
+    int64_t tile0;                  // first tile of this matrix in the update launch
+};
+size_t split4_operand_bytes(int64_t m);
+void launch_grouped_split4(const Split4Desc* d_descs, int ng, int64_t blocks, hipStream_t st);
+void launch_grouped_split4_update(const Split4Desc* d_descs, int ng, int64_t tiles, hipStream_t st);
 void launch_grouped_split(const SplitDesc* d_descs, int ng, int64_t blocks, hipStream_t st);
 void launch_grouped_split_update(const SplitDesc* d_descs, int ng, int64_t tiles, hipStream_t st);
 void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st);
