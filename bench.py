@@ -57,8 +57,8 @@ def roofline_of(precision, kern, config):
         dom, peak, mult, tkey = "factor_update_f32_t128", PEAK_TFLOPS["f32"], 1.0, "f32"
         kname = "k_gemm<float,false,true,true,4,false> (interior tiles of the FP32 trailing update)"
     else:
-        dom, peak, mult, tkey = "factor_update_f64_t64", PEAK_TFLOPS["f64"], 1.0, "f64"
-        kname = "k_gemm<double,false,true,true,2,false> (interior tiles of the FP64 trailing update)"
+        dom, peak, mult, tkey = "factor_update_f64_t128", PEAK_TFLOPS["f64"], 1.0, "f64"
+        kname = "k_gemm_f64_dma<true> (interior 128 x 128 tiles of the FP64 trailing update)"
     calls, sec, work = kern[dom]
     achieved = work / sec / 1e12 if sec > 0 else 0.0
     out = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
@@ -395,7 +395,7 @@ def run(args):
     torch.cuda.synchronize()
     dt = (time.time() - t0) / max(1, args.steps)
     kern = {k: prof(k) for k in ("factor_update_f32s", "factor_update_f32_t128", "factor_update_f32_t64",
-                                  "factor_update_f64_t64",
+                                  "factor_update_f64_t128",
                                   "diag_factor_f32", "diag_factor_f64", "s_block", "d_pairs", "pairs_far")}
     _lib.check(L.omh_profile_enable(0))
     Gm_h = om.gain_to_numpy(Gm, len(el), len(dip))
@@ -414,7 +414,7 @@ def run(args):
         G64, info64 = step("fp64")
         torch.cuda.synchronize()
         t64 = time.time() - t1
-        roofline64 = roofline_of("fp64", {"factor_update_f64_t64": prof("factor_update_f64_t64")}, args.config)
+        roofline64 = roofline_of("fp64", {"factor_update_f64_t128": prof("factor_update_f64_t128")}, args.config)
         _lib.check(L.omh_profile_enable(0))
         stages64 = info64["times"]
         G64h = om.gain_to_numpy(G64, len(el), len(dip))

@@ -46,7 +46,8 @@ enum ScratchSlot {
     SCR_PAIRQ = 6,
     SCR_SRCCOEF = 7,   // source-matrix coefficients (the source matrix may run beside the factorisation)
     SCR_GEMM = 8,      // split-K partial products of the C-ABI GEMM entry points
-    SCR_NSLOTS = 9
+    SCR_RESID_XT = 9,  // the refinement's transposed solution (structured residual products)
+    SCR_NSLOTS = 10
 };
 void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st);
 void release_scratch();

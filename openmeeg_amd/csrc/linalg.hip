@@ -41,6 +41,7 @@ namespace omh {
 namespace {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;   // LDS destination of direct global-to-LDS loads
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Acc;
@@ -398,6 +399,137 @@ __global__ void __launch_bounds__(256) k_splitk_sum(const T* __restrict__ W, int
         if (beta != T(0)) v += beta * C[i + j * ldc];
         C[i + j * ldc] = v;
     }
+}
+
+// FP64 C = alpha A B^T + beta C on 128 x 128 tiles (A m x k and B n x k,
+// column-major: op(A)(i, kk) = A[i + kk lda], op(B)(kk, j) = B[j + kk ldb]),
+// every tile inside m x n (interior launches; a partial last k-slice is
+// zero-padded).  512
+// threads as 2 x 4 waves of 64 x 32 (4 x 2 MFMA blocks of 16 x 16 x 4);
+// k-slices of 16 in two LDS buffers ([kk][row], a k-row of 128 values is
+// 1 KB), each k-row fetched by one direct global-to-LDS load one slice
+// ahead; each wave's fragments are double-buffered in registers over the
+// 4-wide k steps.  Every entry sums its products in the same order as
+// gemm_tile (MFMAs over k in steps of 4): the results are bitwise those of
+// the 64 x 64 tiles.  LOWER: entries with row < col are not written.
+constexpr int DT = 128, DBK = 16, DPAD = 16;
+
+template <bool LOWER>
+__global__ void __launch_bounds__(512) k_gemm_f64_dma(int64_t k, double alpha, const double* __restrict__ A,
+                                                      int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                      double beta, double* __restrict__ C, int64_t ldc, TileMap map) {
+    __shared__ double As0[DBK * (DT + DPAD)];
+    __shared__ double As1[DBK * (DT + DPAD)];
+    __shared__ double Bs0[DBK * (DT + DPAD)];
+    __shared__ double Bs1[DBK * (DT + DPAD)];
+    int64_t t = blockIdx.x;
+    if (map.kind >= 4) {
+        t = xcd_tile(t, map.per);
+        if (t >= map.ntiles) return;
+    }
+    int64_t tm, tn;
+    map_tile(map, t, tm, tn);
+    const int64_t row0 = tm * DT, col0 = tn * DT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+    // DMA: wave w moves k-rows 2 w and 2 w + 1 of the A and B slices; past
+    // k (a partial last slice) the rows are zero, as gemm_tile's guarded
+    // loads make them
+    auto dma = [&](int64_t k0, double* As, double* Bs) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int kk = 2 * wave + q;
+            if (k0 + kk < k) {
+                __builtin_amdgcn_global_load_lds((const void*)(A + row0 + (k0 + kk) * lda + 2 * lane),
+                                                 (lds_void*)&As[kk * (DT + DPAD)], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(B + col0 + (k0 + kk) * ldb + 2 * lane),
+                                                 (lds_void*)&Bs[kk * (DT + DPAD)], 16, 0, 0);
+            } else {
+                As[kk * (DT + DPAD) + 2 * lane] = 0.0;
+                As[kk * (DT + DPAD) + 2 * lane + 1] = 0.0;
+                Bs[kk * (DT + DPAD) + 2 * lane] = 0.0;
+                Bs[kk * (DT + DPAD) + 2 * lane + 1] = 0.0;
+            }
+        }
+    };
+    const int fi = lane & 15, fk = lane >> 4;
+    // fragments of k step j (4 k values) of a slice: A(i, k) in lane i + 16 k
+    double F[2][6];
+    auto frags = [&](const double* As, const double* Bs, int j, double (&f)[6]) {
+        const int kr = 4 * j + fk;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) f[a] = As[kr * (DT + DPAD) + wr + 16 * a + fi];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) f[4 + b] = Bs[kr * (DT + DPAD) + wc + 16 * b + fi];
+    };
+    double4_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const int64_t ns = (k + DBK - 1) / DBK;
+    dma(0, As0, Bs0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    frags(As0, Bs0, 0, F[0]);
+    for (int64_t s = 0; s < ns; s += 2) {
+        // two slices per trip, so that buffers and fragment slots stay static
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (s + h >= ns) break;
+            double* Ac = h ? As1 : As0;
+            double* Bc = h ? Bs1 : Bs0;
+            double* An = h ? As0 : As1;
+            double* Bn = h ? Bs0 : Bs1;
+            const bool more = s + h + 1 < ns;
+            __builtin_amdgcn_sched_barrier(0);
+            if (more) dma((s + h + 1) * DBK, An, Bn);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                double (&f)[6] = F[j & 1];
+                double (&g)[6] = F[(j + 1) & 1];
+                if (j == 3) {
+                    // the next slice has landed everywhere, and this one is
+                    // read everywhere before its buffer is refilled
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                    if (more) frags(An, Bn, 0, g);
+                } else {
+                    frags(Ac, Bc, j + 1, g);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[a], f[4 + b], acc[a][b], 0, 0, 0);
+                // the next step's fragment reads between this step's MFMAs,
+                // after the first two (whose wait covers this step's reads only)
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);   // MFMA
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    // epilogue: FP64 accumulator element r of lane is row (lane / 16) + 4 r
+    // of column lane % 16 of its block
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gi = row0 + wr + 16 * a + fk + 4 * r, gj = col0 + wc + 16 * b + fi;
+                if (!LOWER || gi >= gj) {
+                    double v = alpha * acc[a][b][r];
+                    if (beta != 0.0) v += beta * C[gi + gj * ldc];
+                    C[gi + gj * ldc] = v;
+                }
+            }
 }
 
 // Grouped GEMM: the tiles of ng independent products (descriptor d[g], its
@@ -1153,8 +1285,6 @@ __device__ __forceinline__ void syrk_split_tile(int64_t m, int64_t mp, int64_t r
 // slice separates the buffers' reuse.  Slice ks + 2 is fetched into the
 // buffer that slice ks left.  Every entry sums the same products in the same
 // order as syrk_split_tile: the results are bitwise equal.
-typedef __attribute__((address_space(3))) void lds_void;
-
 template <int KP>
 __device__ __forceinline__ void syrk_split_tile_pp(int64_t m, int64_t mp, int64_t r_off,
                                                    const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
@@ -2130,11 +2260,61 @@ void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hip
     OMH_LAUNCH_CHECK();
 }
 
+// FP64 A B^T on the 128 x 128 DMA tiles over the interior [0, 128 fm) x
+// [0, 128 fn) (lower: its lower triangle of tiles), the rest by gemm_launch
+// on the strips around it.  Returns false (nothing launched) when the
+// product does not fit the kernel: operands 16-byte aligned with even
+// leading dimensions, enough whole tiles.
+static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t st,
+                         const char* prof) {
+    const char* e = std::getenv("OMH_GEMM_F64_DMA");
+    if (e && e[0] == '0') return false;
+    if (k <= 0 || lda % 2 || ldb % 2 || reinterpret_cast<uintptr_t>(A) % 16 ||
+        reinterpret_cast<uintptr_t>(B) % 16)
+        return false;
+    const int64_t fm = m / DT, fn = lower ? fm : n / DT;
+    const int64_t tiles = lower ? fm * (fm + 1) / 2 : fm * fn;
+    if (tiles < 256) return false;
+    const TileMap map = l2_map(lower, fm, fn);
+    const unsigned grid = (unsigned)(map.kind >= 4 ? 8 * map.per : map.ntiles);
+    {
+        std::string name;
+        if (prof) name = std::string(prof) + "_t" + std::to_string(DT);
+        const double side = (double)(fm * DT);
+        ProfScope ps(prof ? name.c_str() : "", prof ? side * (side + 1) * (double)k : 0.0, prof ? st : nullptr);
+        if (!prof) ps.cancel();
+        if (lower)
+            k_gemm_f64_dma<true><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map);
+        else
+            k_gemm_f64_dma<false><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map);
+        OMH_LAUNCH_CHECK();
+    }
+    const int64_t r1 = fm * DT, c1 = fn * DT;
+    if (lower) {
+        if (r1 < m) {   // the rows below the interior square: a rectangle and the last lower square
+            gemm<double>(false, true, false, m - r1, r1, k, alpha, A + r1, lda, B, ldb, beta, C + r1, ldc, st);
+            gemm<double>(false, true, true, m - r1, m - r1, k, alpha, A + r1, lda, B + r1, ldb, beta,
+                         C + r1 + r1 * ldc, ldc, st);
+        }
+        return true;
+    }
+    if (r1 < m)   // rows below the interior, all columns
+        gemm<double>(false, true, false, m - r1, n, k, alpha, A + r1, lda, B, ldb, beta, C + r1, ldc, st);
+    if (c1 < n)   // columns right of the interior, its rows
+        gemm<double>(false, true, false, r1, n - c1, k, alpha, A, lda, B + c1, ldb, beta, C + c1 * ldc, ldc, st);
+    return true;
+}
+
 template <typename T>
 void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
           const T* B, int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t st, const char* prof) {
     if (m <= 0 || n <= 0) return;
     if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
+    if (sizeof(T) == 8 && !ta && tb &&
+        gemm_f64_dma(lower, m, n, k, (double)alpha, (const double*)A, lda, (const double*)B, ldb, (double)beta,
+                     (double*)C, ldc, st, prof))
+        return;
     const bool vec = aligned_for_vec(A, lda) && aligned_for_vec(B, ldb);
     // FP64: 64 x 64 tiles (128 x 128 accumulators would limit FP64 to one wave
     // per SIMD).  FP32: 128 x 128 tiles when they give at least ~2 waves of
@@ -3312,12 +3492,36 @@ static SidePool& side_pool() {
 // segments write disjoint rows of R and run on up to N_SIDE + 1 streams, so
 // that the last, partly filled round of tiles of one product overlaps the
 // others.
+// XT (r x n, leading dimension r) = X^T for X n x r (leading dimension ldx):
+// 32 x 32 tiles through LDS, reads and writes along contiguous memory.
+__global__ void __launch_bounds__(256) k_transpose_f64(const double* __restrict__ X, int64_t ldx, int64_t n, int64_t r,
+                                                       double* __restrict__ XT) {
+    __shared__ double t[32][33];
+    const int64_t i0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int q = ty; q < 32; q += 8) {
+        const int64_t i = i0 + tx, j = j0 + q;
+        t[q][tx] = (i < n && j < r) ? X[i + j * ldx] : 0.0;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const int64_t j = j0 + tx, i = i0 + q;
+        if (i < n && j < r) XT[j + i * r] = t[tx][q];
+    }
+}
+
 static void residual_update(const double* H, int64_t n, int64_t ldh, const double* X, int64_t ldx, int64_t r,
                             double* R, int64_t ldr, hipStream_t st, const BlockFill* bf) {
     if (!bf) {
         gemm<double>(false, false, false, n, r, n, -1.0, H, ldh, X, ldx, 1.0, R, ldr, st);
         return;
     }
+    // X^T, so that the segment products are A B^T with contiguous rows of
+    // both operands (the 128 x 128 FP64 tiles); the same sums in the same
+    // order as the products with X
+    double* XT = (double*)scratch(SCR_RESID_XT, sizeof(double) * (size_t)n * (size_t)r, st);
+    k_transpose_f64<<<dim3((unsigned)cdiv(n, 32), (unsigned)cdiv(r, 32)), 256, 0, st>>>(X, ldx, n, r, XT);
+    OMH_LAUNCH_CHECK();
     SidePool& sp = side_pool();
     OMH_HIP(hipEventRecord(sp.fork, st));
     for (int i = 0; i < N_SIDE; ++i) OMH_HIP(hipStreamWaitEvent(sp.s[i], sp.fork, 0));
@@ -3330,7 +3534,7 @@ static void residual_update(const double* H, int64_t n, int64_t ldh, const doubl
             int u = t + 1;
             while (u < ns && bf->hnz(s, u)) ++u;
             const int64_t c0 = bf->start[t], c1 = bf->start[u];
-            gemm<double>(false, false, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, X + c0, ldx, 1.0,
+            gemm<double>(false, true, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r, 1.0,
                          R + r0, ldr, ss);
             t = u;
         }

@@ -254,6 +254,86 @@ def test_gemm_large_tile_order(om, dtype, ta):
     assert rel_fro(got, ref) < (1e-14 if dtype == "f64" else 2e-6)
 
 
+@pytest.mark.parametrize("shape", [(2200, 2100, 100), (2304, 2048, 256), (4100, 256, 10242)])
+def test_gemm_f64_dma_tiles(om, monkeypatch, shape):
+    """FP64 A B^T on 128 x 128 tiles fed by direct global-to-LDS loads (the
+    kernel of the FP64 trailing update and of the structured residual) over
+    the interior, the 64 x 64 tiles on the strips around it: against numpy
+    (1e-14) and bitwise against the 64 x 64 tiles throughout
+    (OMH_GEMM_F64_DMA=0: the same MFMAs over k in the same order).  k a
+    multiple of the 16-wide k-slice and not (a zero-padded last slice),
+    partial edge tiles, a residual-like 256-column output."""
+    import ctypes as C
+    import torch
+    from openmeeg_amd import _lib
+    m, n, k = shape
+    rng = np.random.default_rng(m + k)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    Cm = rng.standard_normal((m, n))
+    dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    dB = torch.from_numpy(np.asfortranarray(B).ravel(order="F").copy()).cuda()
+    out = {}
+    for mode in ("dma", "tiles64"):
+        if mode == "tiles64":
+            monkeypatch.setenv("OMH_GEMM_F64_DMA", "0")
+        dC = torch.from_numpy(np.asfortranarray(Cm).ravel(order="F").copy()).cuda()
+        _lib.check(_lib.lib().omh_gemm_f64(0, 1, m, n, k, -1.0, C.c_void_p(dA.data_ptr()), m,
+                                           C.c_void_p(dB.data_ptr()), n, 1.0, C.c_void_p(dC.data_ptr()), m,
+                                           C.c_void_p(0)))
+        torch.cuda.synchronize()
+        out[mode] = dC.cpu().numpy().reshape(n, m).T
+        monkeypatch.delenv("OMH_GEMM_F64_DMA", raising=False)
+    err = rel_fro(out["dma"], Cm - A @ B.T)
+    print("%s: vs numpy %.2e, bitwise the 64 x 64 tiles: %s" % (shape, err, np.array_equal(out["dma"],
+                                                                                             out["tiles64"])),
+          flush=True)
+    assert err < 1e-14
+    assert np.array_equal(out["dma"], out["tiles64"])
+
+
+def test_fp64_factor_dma_tiles_bitwise(om, monkeypatch):
+    """The FP64 factorisation's trailing updates (lower products) on the
+    128 x 128 DMA tiles give the factor bitwise, with and without the block
+    structure, and the refined mixed solution through the structured
+    residual (its products with a transposed solution) is bitwise the same
+    as with the 64 x 64 tiles: 3 layers of 2562 vertices (n = 17926)."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    n = g.size
+    H = _head_mat_device(g)
+    for structure in (g.factor_segments(), None):
+        L = {}
+        for mode in ("dma", "tiles64"):
+            if mode == "tiles64":
+                monkeypatch.setenv("OMH_GEMM_F64_DMA", "0")
+            F = factor(H.clone(), n, "fp64", structure=structure)
+            L[mode] = F.L.view(n, n).t().tril().clone()
+            del F
+            monkeypatch.delenv("OMH_GEMM_F64_DMA", raising=False)
+        print("n=%d structure %s: FP64 factor max |diff| %.3e" % (n, structure is not None,
+                                                                  float((L["dma"] - L["tiles64"]).abs().max())),
+              flush=True)
+        assert torch.equal(L["dma"], L["tiles64"])
+        del L
+    Hf = _head_mat_device(g, far_field=True)
+    B = torch.from_numpy(np.random.default_rng(3).standard_normal(n * 64)).cuda()
+    F = factor(Hf, n, "mixed16", structure=g.factor_segments())
+    X = {}
+    for mode in ("dma", "tiles64"):
+        if mode == "tiles64":
+            monkeypatch.setenv("OMH_GEMM_F64_DMA", "0")
+        X[mode] = B.clone()
+        F.solve(X[mode], 64, max_iter=10, tol=1e-10)
+        torch.cuda.synchronize()
+        monkeypatch.delenv("OMH_GEMM_F64_DMA", raising=False)
+    print("refined solution max |diff| %.3e after %d iterations" % (float((X["dma"] - X["tiles64"]).abs().max()),
+                                                                    F.iterations), flush=True)
+    assert torch.equal(X["dma"], X["tiles64"])
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("shape", [(128, 320, 40000), (70, 133, 9001)])
