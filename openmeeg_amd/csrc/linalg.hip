@@ -2279,10 +2279,13 @@ static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alp
     const TileMap map = l2_map(lower, fm, fn);
     const unsigned grid = (unsigned)(map.kind >= 4 ? 8 * map.per : map.ntiles);
     {
+        // optional timing; algorithmic work: the interior's lower triangle
+        // (lower) or rectangle, 2 k flop per entry
         std::string name;
         if (prof) name = std::string(prof) + "_t" + std::to_string(DT);
         const double side = (double)(fm * DT);
-        ProfScope ps(prof ? name.c_str() : "", prof ? side * (side + 1) * (double)k : 0.0, prof ? st : nullptr);
+        const double work = lower ? side * (side + 1) * (double)k : 2.0 * side * (double)(fn * DT) * (double)k;
+        ProfScope ps(prof ? name.c_str() : "", prof ? work : 0.0, prof ? st : nullptr);
         if (!prof) ps.cancel();
         if (lower)
             k_gemm_f64_dma<true><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map);
