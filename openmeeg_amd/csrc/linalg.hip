@@ -2814,10 +2814,11 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
             gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, ldx, A21, lda, T(1), A22 + nb2, lda, st);
         // factor the next block and form its panel on the side stream (it
         // touches only the next block's columns, which (b) does not).  Beside
-        // the FP64 update the single-workgroup diagonal kernels wait for CU
-        // slots (~10 ms instead of 0.42 ms alone), yet the 4-layer 10242-vertex
-        // FP64 factorisation still takes 4.70 s this way against 4.90 s with
-        // the diagonal block factored on the main stream before (b).
+        // the FP64 update the diagonal kernels wait for CU slots (~8 ms
+        // instead of 0.42 ms alone: the inverse's 142 KB of LDS fit only on a
+        // CU without update workgroups), yet the 4-layer 10242-vertex FP64
+        // factorisation still takes 4.02 s this way against 4.20 s with the
+        // diagonal block factored on the main stream before (b).
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
         diag_factor<T>(A, lda, e, nb2, Dinv, sgn, info, tiny, la.side);
