@@ -289,6 +289,35 @@ int omh_sym_factor_f64_blocks(double* d_A, int64_t n, int64_t ld, void* d_work, 
  * order needs pivoting (omh_lu_factor_f64).  No reference counterpart. */
 int omh_factor_sign_mismatch(const void* d_work, int64_t n, int32_t elem_size, int64_t n_positive, int64_t* first,
                              void* stream);
+/* The same against one expected sign per segment (seg_sign[s] = +1 or -1
+ * for the unknowns of segment s; segments as for the *_blocks entry points,
+ * in the factor's own order).  No reference counterpart. */
+int omh_factor_sign_mismatch_segments(const void* d_work, int64_t n, int32_t elem_size, int32_t nseg,
+                                      const int64_t* seg_start, const int8_t* seg_sign, int64_t* first, void* stream);
+
+/* Factor order.  The factorisations may run on a symmetric reordering of the
+ * matrix that moves whole segments (seg_start as for the *_blocks entry
+ * points; the reordered matrix takes the segments in the order
+ * seg_order[0..nseg-1], a permutation of 0..nseg-1).  A HeadMat taken mesh
+ * by mesh (each mesh's triangle unknowns, then its vertex unknowns, the
+ * innermost mesh first) is block tridiagonal over the meshes, and its
+ * pivot-free factor fills no block outside that band: a quarter fewer
+ * trailing-update flops than the triangles-first order at 4 layers.  None of
+ * these has a reference counterpart (the reference factors the packed
+ * matrix in its own order, symmatrix.cpp:133-152).
+ *
+ * omh_sym_reorder: the lower triangle of the reordered matrix, d_dst(i, j) =
+ * d_src(p(i), p(j)) for i >= j, from the full symmetric FP64 d_src (both
+ * triangles stored), as FP64 (dst_elem_size 8) or as the FP32 copy of the
+ * mixed factorisations (4).  d_dst must not overlap d_src. */
+int omh_sym_reorder(const double* d_src, int64_t lds, void* d_dst, int64_t ldd, int32_t dst_elem_size, int64_t n,
+                    int32_t nseg, const int64_t* seg_start, const int32_t* seg_order, void* stream);
+/* Rows of an n x ncol FP64 matrix into the factor order (inverse 0:
+ * d_dst(i, :) = d_src(p(i), :)) or back (inverse 1: d_dst(p(i), :) =
+ * d_src(i, :)), for the solves of a factor of the reordered matrix. */
+int omh_reorder_rows_f64(const double* d_src, int64_t lds, double* d_dst, int64_t ldd, int64_t n, int64_t ncol,
+                         int32_t nseg, const int64_t* seg_start, const int32_t* seg_order, int32_t inverse,
+                         void* stream);
 
 /* Pivoted fallback of SymMatrix::invert/inverse/solveLin (DSPTRF/DSPTRS,
  * symmatrix.cpp:133-152, symmatrix.h:259-297): FP64 LU with partial row
@@ -360,6 +389,17 @@ int omh_mixed_solve_blocks(const double* d_H, int64_t n, int64_t ldh, const floa
                            const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
                            int64_t nrhs, int32_t max_iter, double tol, int32_t nseg, const int64_t* seg_start,
                            const uint8_t* hfill, int32_t* iterations, double* rel_update, void* stream);
+/* omh_mixed_solve_blocks with a factor d_L32 of the reordered matrix (the
+ * FP32 copy from omh_sym_reorder, factored with the reordered structure):
+ * seg_start / hfill describe d_H in its own order, seg_order the factor's
+ * segment order.  Residuals in d_H's order, each correction solved in the
+ * factor's (the residual's rows gathered into it, the correction's rows
+ * scattered back); d_B and d_X in d_H's order. */
+int omh_mixed_solve_reordered(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                              const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                              int64_t nrhs, int32_t max_iter, double tol, int32_t nseg, const int64_t* seg_start,
+                              const uint8_t* hfill, const int32_t* seg_order, int32_t* iterations,
+                              double* rel_update, void* stream);
 
 /* C = alpha op(A) op(B) + beta C: the DGEMM products of
  * OpenMEEGMaths/include/matrix.h:333-350 and gain.h:69-89. */

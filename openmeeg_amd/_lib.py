@@ -69,6 +69,11 @@ _SIGS = {
     "omh_sym_solve_f32": ([vp, i64, i64, vp, vp, i64, i64, vp], C.c_int),
     "omh_refine_state": ([i32, f64, f64, f64], i32),
     "omh_factor_sign_mismatch": ([vp, i64, i32, i64, p_i64, vp], C.c_int),
+    "omh_factor_sign_mismatch_segments": ([vp, i64, i32, i32, vp, vp, p_i64, vp], C.c_int),
+    "omh_sym_reorder": ([vp, i64, vp, i64, i32, i64, i32, vp, vp, vp], C.c_int),
+    "omh_reorder_rows_f64": ([vp, i64, vp, i64, i64, i64, i32, vp, vp, i32, vp], C.c_int),
+    "omh_mixed_solve_reordered": ([vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, i64, i32, f64, i32, vp, vp, vp, p_i32,
+                                   p_f64, vp], C.c_int),
     "omh_lu_workspace_bytes": ([i64], i64),
     "omh_lu_factor_f64": ([vp, i64, i64, vp, vp, p_i32, vp], C.c_int),
     "omh_lu_solve_f64": ([vp, i64, i64, vp, vp, vp, i64, i64, vp], C.c_int),

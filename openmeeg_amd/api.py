@@ -239,6 +239,39 @@ class Geometry:
         hfill = np.tril(adj, -1).astype(np.uint8)
         return np.array([a for a, _, _, _ in segs], dtype=np.int64), hfill
 
+    def factor_order(self):
+        """Segment order of the factorisation: each mesh's triangle segment,
+        then its vertex segment, mesh by mesh in the geometry's order, as a
+        permutation of factor_segments()' segments; or None when the meshes
+        do not form a chain (every communicating pair neighbours in mesh
+        order, as in a nested head) or no reordering is possible.  Over a
+        chain of meshes this order makes the HeadMat block tridiagonal, and
+        its pivot-free factor fills no block outside the band (the
+        triangles-first order fills the blocks between every vertex segment
+        and the later triangle segments): at 4 layers a quarter fewer
+        trailing-update flops."""
+        seg = self.factor_segments()
+        if seg is None:
+            return None
+        for p in self.mesh_pairs():
+            m1, m2 = p["meshes"]
+            if abs(m1 - m2) > 1:
+                return None
+        nt, nv = self._nt, self._nv
+        where = {}
+        for m in range(self._nm):
+            inf = self.mesh_info(m)
+            if inf["ntris"] > 0 and not inf["current_barrier"]:
+                where[(m, 0)] = inf["tri_first"] - nv
+            if inf["nverts"] > 0:
+                where[(m, 1)] = inf["vert_first"] + nt
+        starts = seg[0].tolist()
+        index = {a: i for i, a in enumerate(starts)}
+        order = [index[where[key]] for key in sorted(where)]
+        if sorted(order) != list(range(len(starts))) or order == list(range(len(starts))):
+            return None
+        return np.array(order, dtype=np.int32)
+
     def tri_first_index(self, ref_rows):
         """Map reference-order unknown indices to triangles-first order."""
         r = np.asarray(ref_rows, dtype=np.int64)
@@ -545,7 +578,7 @@ class Factor:
     otherwise solve() raises NotConverged."""
 
     def __init__(self, L, n, work, precision, H64=None, structure=None, fallback_fp64=True, ipiv=None,
-                 n_positive=None):
+                 n_positive=None, order=None):
         self.L, self.n, self.work, self.precision, self.H64 = L, n, work, precision, H64
         self.ipiv = ipiv
         self.iterations = 0
@@ -556,6 +589,10 @@ class Factor:
         self.n_positive = n_positive
         self._raw_structure = structure
         self._structure = _block_args(structure)
+        # segment order of the factor (None: the matrix's own order); the
+        # factor's block structure in that order
+        self.order = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        self._fstructure = _block_args(_reorder_structure(structure, self.order, n)) if self.order is not None else None
 
     def _refactor_fp64(self):
         """Factor the FP64 matrix in place (it is no longer needed for
@@ -566,6 +603,7 @@ class Factor:
         F = factor(H, self.n, "fp64", structure=self._raw_structure, n_positive=self.n_positive,
                    rebuild=(lambda: backup) if backup is not None else None)
         self.L, self.work, self.precision, self.H64, self.ipiv = F.L, F.work, F.precision, None, F.ipiv
+        self.order, self._fstructure = F.order, F._fstructure
         self.pivoted = F.pivoted
         self.fell_back = True
 
@@ -589,7 +627,18 @@ class Factor:
                                          self.n, nrhs, _stream()))
             return B
         if self.precision == "fp64":
-            if blocks:
+            if self.order is not None:
+                # rows into the factor's order, solve there, rows back
+                starts, optr = self._structure[3], self.order.ctypes.data_as(C.c_void_p)
+                nseg = self._structure[2]
+                T = _dev_f64(self.n * nrhs)
+                check(lib().omh_reorder_rows_f64(_ptr(B), self.n, _ptr(T), self.n, self.n, nrhs, nseg, starts, optr,
+                                                 0, _stream()))
+                check(lib().omh_sym_solve_f64_blocks(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(T), self.n,
+                                                     nrhs, *self._fstructure[2:], _stream()))
+                check(lib().omh_reorder_rows_f64(_ptr(T), self.n, _ptr(B), self.n, self.n, nrhs, nseg, starts, optr,
+                                                 1, _stream()))
+            elif blocks:
                 check(lib().omh_sym_solve_f64_blocks(_ptr(self.L), self.n, self.n, _ptr(self.work), _ptr(B), self.n,
                                                      nrhs, *blocks, _stream()))
             else:
@@ -598,7 +647,12 @@ class Factor:
             return B
         X = _dev_f64(self.n * nrhs)
         it, rel = C.c_int32(), C.c_double()
-        if blocks:
+        if self.order is not None:
+            rc = lib().omh_mixed_solve_reordered(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n,
+                                                 _ptr(self.work), _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter,
+                                                 tol, *blocks, self.order.ctypes.data_as(C.c_void_p), C.byref(it),
+                                                 C.byref(rel), _stream())
+        elif blocks:
             rc = lib().omh_mixed_solve_blocks(_ptr(self.H64), self.n, self.n, _ptr(self.L), self.n, _ptr(self.work),
                                               _ptr(B), self.n, _ptr(X), self.n, nrhs, max_iter, tol, *blocks,
                                               C.byref(it), C.byref(rel), _stream())
@@ -624,6 +678,38 @@ def _block_args(structure):
     return (starts, hfill, len(starts), starts.ctypes.data_as(C.c_void_p), hfill.ctypes.data_as(C.c_void_p))
 
 
+def _reorder_structure(structure, order, n):
+    """The block structure (starts, hfill) of the matrix with its segments
+    taken in `order`: segment q of the reordered matrix is segment order[q];
+    the pattern is read from hfill's lower triangle, as the *_blocks entry
+    points do (omh_mixed_solve_reordered derives the same one)."""
+    starts = np.asarray(structure[0], dtype=np.int64)
+    hfill = np.tril(np.asarray(structure[1], dtype=np.uint8) != 0, -1)
+    o = np.asarray(order, dtype=np.int64)
+    lens = np.append(starts[1:], n) - starts
+    fstarts = np.concatenate([[0], np.cumsum(lens[o])[:-1]]).astype(np.int64)
+    adj = hfill | hfill.T
+    return fstarts, np.tril(adj[np.ix_(o, o)], -1).astype(np.uint8)
+
+
+def _segment_signs(structure, order, n_positive):
+    """Expected pivot sign per segment of the reordered matrix: + for the
+    unknowns below n_positive in the matrix's order (a HeadMat's triangle
+    unknowns), - for the rest."""
+    starts = np.asarray(structure[0], dtype=np.int64)
+    sg = np.where(starts < n_positive, 1, -1).astype(np.int8)
+    return np.ascontiguousarray(sg[np.asarray(order, dtype=np.int64)])
+
+
+def _room_for(nbytes):
+    """True when the device can hold nbytes more (free memory plus torch's
+    cached blocks) with 4 GiB to spare."""
+    torch = _t()
+    free, _ = torch.cuda.mem_get_info()
+    cached = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+    return free + cached >= nbytes + (4 << 30)
+
+
 def _lu_factor(Hdev, n):
     """FP64 LU with partial pivoting of Hdev in place (the pivoted fallback)."""
     torch = _t()
@@ -636,7 +722,7 @@ def _lu_factor(Hdev, n):
     return Factor(Hdev, n, work, "lu", ipiv=ipiv)
 
 
-def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=None):
+def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=None, order=None):
     """Factor a device HeadMat (triangles-first order).  precision 'fp64'
     factors in place; 'mixed' keeps Hdev as the FP64 residual operator and
     factors an FP32 copy; 'mixed16' does the same with split-FP16 MFMA
@@ -644,6 +730,13 @@ def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=N
     structure = Geometry.factor_segments(), the trailing updates, and later
     the solves, skip the parts that only multiply structural zeros (the
     factor is the same).
+
+    order (with structure; Geometry.factor_order()): factor the matrix with
+    its segments in that order instead (the solves take right-hand sides and
+    return solutions in Hdev's order).  The mixed paths build their FP32 copy
+    reordered; 'fp64' factors a reordered FP64 copy when the device has room
+    for one (Hdev is then left intact), else Hdev in place in its own order.
+    A reordered factorisation that breaks down is redone in Hdev's order.
 
     The pivot-free factorisation is used when it is stable: it reports a
     vanishing pivot, and given n_positive (the triangle unknowns, which come
@@ -656,28 +749,47 @@ def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=N
     torch = _t()
     if precision == "fp64_lu":
         return _lu_factor(Hdev, n)
+    if order is not None and structure is None:
+        raise ValueError("a factor order needs the block structure")
+    if order is not None and precision == "fp64" and not _room_for(8 * n * n):
+        order = None
     info = C.c_int32()
     sargs = _block_args(structure)
     blocks = sargs[2:] if sargs else ()
+    if order is not None:
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        fargs = _block_args(_reorder_structure(structure, order, n))
+        fblocks = fargs[2:]
+        optr = order.ctypes.data_as(C.c_void_p)
     if precision == "fp64":
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 8)), dtype=torch.uint8, device="cuda")
-        if blocks:
+        if order is not None:
+            L = torch.empty(n * n, dtype=torch.float64, device="cuda")
+            check(lib().omh_sym_reorder(_ptr(Hdev), n, _ptr(L), n, 8, n, blocks[0], blocks[1], optr, _stream()))
+            check(lib().omh_sym_factor_f64_blocks(_ptr(L), n, n, _ptr(work), *fblocks, C.byref(info), _stream()))
+        elif blocks:
+            L = Hdev
             check(lib().omh_sym_factor_f64_blocks(_ptr(Hdev), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
         else:
+            L = Hdev
             check(lib().omh_sym_factor_f64(_ptr(Hdev), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(Hdev, n, work, "fp64", structure=structure, n_positive=n_positive)
+        F = Factor(L, n, work, "fp64", structure=structure, n_positive=n_positive, order=order)
         elem = 8
     elif precision in ("mixed", "mixed16"):
         L32 = torch.empty(n * n, dtype=torch.float32, device="cuda")
-        check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
+        if order is not None:
+            check(lib().omh_sym_reorder(_ptr(Hdev), n, _ptr(L32), n, 4, n, blocks[0], blocks[1], optr, _stream()))
+        else:
+            check(lib().omh_convert_f64_f32(_ptr(Hdev), n, _ptr(L32), n, n, n, 1, _stream()))
         work = torch.empty(int(lib().omh_factor_workspace_bytes(n, 4)), dtype=torch.uint8, device="cuda")
         if blocks:
             fn = lib().omh_sym_factor_f32_blocks if precision == "mixed" else lib().omh_sym_factor_f32_split_blocks
-            check(fn(_ptr(L32), n, n, _ptr(work), *blocks, C.byref(info), _stream()))
+            check(fn(_ptr(L32), n, n, _ptr(work), *(fblocks if order is not None else blocks), C.byref(info),
+                     _stream()))
         else:
             fn = lib().omh_sym_factor_f32 if precision == "mixed" else lib().omh_sym_factor_f32_split
             check(fn(_ptr(L32), n, n, _ptr(work), C.byref(info), _stream()))
-        F = Factor(L32, n, work, "mixed", H64=Hdev, structure=structure, n_positive=n_positive)
+        F = Factor(L32, n, work, "mixed", H64=Hdev, structure=structure, n_positive=n_positive, order=order)
         elem = 4
     else:
         raise ValueError("precision must be 'fp64', 'mixed', 'mixed16' or 'fp64_lu'")
@@ -686,18 +798,26 @@ def factor(Hdev, n, precision="fp64", structure=None, n_positive=None, rebuild=N
         reason = "vanishing pivot %d" % (info.value - 1)
     elif n_positive is not None:
         first = C.c_int64()
-        check(lib().omh_factor_sign_mismatch(_ptr(F.work), n, elem, int(n_positive), C.byref(first), _stream()))
+        if order is not None:
+            sg = _segment_signs(structure, order, n_positive)
+            check(lib().omh_factor_sign_mismatch_segments(_ptr(F.work), n, elem, fblocks[0], fblocks[1],
+                                                          sg.ctypes.data_as(C.c_void_p), C.byref(first), _stream()))
+        else:
+            check(lib().omh_factor_sign_mismatch(_ptr(F.work), n, elem, int(n_positive), C.byref(first), _stream()))
         if first.value >= 0:
             reason = "pivot sign at unknown %d" % first.value
     if reason is None:
         return F
+    del F
+    if order is not None:
+        # Hdev is intact: the factorisation in its own order
+        F = factor(Hdev, n, precision, structure, n_positive, rebuild)
+        F.reorder_breakdown = reason
+        return F
     if precision == "fp64":
-        del F
         if rebuild is None:
             raise _lib.OpenMEEGError("pivot-free factorisation broke down (%s)" % reason)
         Hdev = rebuild()
-    else:
-        del F
     F = _lu_factor(Hdev, n)
     F.breakdown = reason
     return F
@@ -711,11 +831,17 @@ HEADMAT_FAR_FIELD = 1
 
 def _head_factor(geom, H, precision, integrator=None, far_field=False):
     """factor() of a pipeline's HeadMat H with the geometry's block
-    structure and pivot-sign pattern (its triangle unknowns first); an FP64
-    breakdown re-assembles H for the pivoted LU."""
+    structure, its mesh-by-mesh factor order (Geometry.factor_order; the
+    environment variable OMH_FACTOR_ORDER=0 keeps the triangles-first order)
+    and pivot-sign pattern (triangle unknowns +); an FP64 breakdown
+    re-assembles H for the pivoted LU."""
     n = geom.size
-    return factor(H, n, precision, structure=geom.factor_segments(), n_positive=n - geom.n_vertex_unknowns,
-                  rebuild=lambda: _head_mat_device(geom, integrator, far_field))
+    structure = geom.factor_segments()
+    order = None
+    if structure is not None and os.environ.get("OMH_FACTOR_ORDER", "1") != "0":
+        order = geom.factor_order()
+    return factor(H, n, precision, structure=structure, n_positive=n - geom.n_vertex_unknowns,
+                  rebuild=lambda: _head_mat_device(geom, integrator, far_field), order=order)
 
 
 def _head_mat_device(geom, integrator=None, far_field=False):

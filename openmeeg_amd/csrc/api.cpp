@@ -662,6 +662,57 @@ static omh::BlockFill block_pattern(int64_t n, int32_t nseg, const int64_t* seg_
     return bf;
 }
 
+// Segment starts with n appended (checked as in block_pattern) and a checked
+// segment permutation.
+static std::vector<int64_t> segment_starts(int64_t n, int32_t nseg, const int64_t* seg_start) {
+    need(seg_start, "seg_start");
+    if (nseg < 1 || nseg > 4096) throw std::invalid_argument("bad segment count");
+    std::vector<int64_t> start(seg_start, seg_start + nseg);
+    start.push_back(n);
+    if (start[0] != 0) throw std::invalid_argument("the first segment must start at 0");
+    for (int s = 0; s < nseg; ++s)
+        if (start[s + 1] <= start[s]) throw std::invalid_argument("segment starts must increase below n");
+    return start;
+}
+
+static std::vector<int32_t> segment_order(int32_t nseg, const int32_t* seg_order) {
+    need(seg_order, "seg_order");
+    std::vector<int32_t> order(seg_order, seg_order + nseg);
+    std::vector<char> seen((size_t)nseg, 0);
+    for (const int32_t s : order) {
+        if (s < 0 || s >= nseg || seen[(size_t)s]) throw std::invalid_argument("seg_order is not a permutation of the segments");
+        seen[(size_t)s] = 1;
+    }
+    return order;
+}
+
+// The row map on the device (its own scratch slot; the upload has finished
+// when this returns).
+static const int32_t* device_rowmap(const std::vector<int32_t>& map, hipStream_t st) {
+    int32_t* d = (int32_t*)omh::scratch(omh::SCR_ROWMAP, sizeof(int32_t) * map.size(), st);
+    OMH_HIP(hipMemcpyAsync(d, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice, st));
+    OMH_HIP(hipStreamSynchronize(st));
+    return d;
+}
+
+// The matrix's block structure (segments start, lower pattern hfill) in the
+// segment order `order`: segment q of the reordered matrix is segment
+// order[q] of the matrix.
+static omh::BlockFill reordered_pattern(const std::vector<int64_t>& start, const uint8_t* hfill,
+                                        const std::vector<int32_t>& order) {
+    need(hfill, "hfill");
+    const int ns = (int)order.size();
+    std::vector<int64_t> fstart(1, 0);
+    for (int q = 0; q + 1 < ns; ++q) fstart.push_back(fstart.back() + start[order[q] + 1] - start[order[q]]);
+    std::vector<uint8_t> fh((size_t)ns * ns, 0);
+    for (int q = 0; q < ns; ++q)
+        for (int p = 0; p < q; ++p) {
+            const int a = std::max(order[q], order[p]), b = std::min(order[q], order[p]);
+            fh[(size_t)q * ns + p] = hfill[(size_t)a * ns + b] ? 1 : 0;
+        }
+    return block_pattern(start.back(), ns, fstart.data(), fh.data());
+}
+
 template <typename T>
 static int factor_blocks(T* d_A, int64_t n, int64_t ld, void* d_work, int32_t nseg, const int64_t* seg_start,
                          const uint8_t* hfill, int32_t* info, void* stream, bool split16) {
@@ -748,6 +799,57 @@ int omh_factor_sign_mismatch(const void* d_work, int64_t n, int32_t elem_size, i
         if (n <= 0 || (elem_size != 4 && elem_size != 8) || n_positive < 0 || n_positive > n)
             throw std::invalid_argument("bad sizes");
         *first = omh::factor_sign_mismatch(d_work, n, (size_t)elem_size, n_positive, S(stream));
+    });
+}
+
+int omh_factor_sign_mismatch_segments(const void* d_work, int64_t n, int32_t elem_size, int32_t nseg,
+                                      const int64_t* seg_start, const int8_t* seg_sign, int64_t* first, void* stream) {
+    return guard([&] {
+        need(d_work, "d_work");
+        need(first, "first");
+        need(seg_sign, "seg_sign");
+        if (n <= 0 || (elem_size != 4 && elem_size != 8)) throw std::invalid_argument("bad sizes");
+        const std::vector<int64_t> start = segment_starts(n, nseg, seg_start);
+        std::vector<int8_t> expect((size_t)n);
+        for (int s = 0; s < nseg; ++s) {
+            if (seg_sign[s] != 1 && seg_sign[s] != -1) throw std::invalid_argument("segment signs must be +1 or -1");
+            for (int64_t i = start[s]; i < start[s + 1]; ++i) expect[(size_t)i] = seg_sign[s];
+        }
+        *first = omh::factor_sign_mismatch(d_work, n, (size_t)elem_size, expect.data(), S(stream));
+    });
+}
+
+int omh_sym_reorder(const double* d_src, int64_t lds, void* d_dst, int64_t ldd, int32_t dst_elem_size, int64_t n,
+                    int32_t nseg, const int64_t* seg_start, const int32_t* seg_order, void* stream) {
+    return guard([&] {
+        need(d_src, "d_src");
+        need(d_dst, "d_dst");
+        if (n <= 0 || lds < n || ldd < n || (dst_elem_size != 4 && dst_elem_size != 8))
+            throw std::invalid_argument("bad sizes");
+        if ((const void*)d_src == d_dst) throw std::invalid_argument("d_dst must not be d_src");
+        if (n > INT32_MAX) throw std::invalid_argument("n too large for a row map");
+        const std::vector<int64_t> start = segment_starts(n, nseg, seg_start);
+        const std::vector<int32_t> order = segment_order(nseg, seg_order);
+        hipStream_t st = S(stream);
+        const int32_t* rowmap = device_rowmap(omh::segment_rowmap(start, order), st);
+        omh::sym_reorder(d_src, lds, d_dst, ldd, dst_elem_size, n, rowmap, st);
+    });
+}
+
+int omh_reorder_rows_f64(const double* d_src, int64_t lds, double* d_dst, int64_t ldd, int64_t n, int64_t ncol,
+                         int32_t nseg, const int64_t* seg_start, const int32_t* seg_order, int32_t inverse,
+                         void* stream) {
+    return guard([&] {
+        need(d_src, "d_src");
+        need(d_dst, "d_dst");
+        if (n <= 0 || ncol < 0 || lds < n || ldd < n) throw std::invalid_argument("bad sizes");
+        if (d_src == d_dst) throw std::invalid_argument("d_dst must not be d_src");
+        if (n > INT32_MAX) throw std::invalid_argument("n too large for a row map");
+        const std::vector<int64_t> start = segment_starts(n, nseg, seg_start);
+        const std::vector<int32_t> order = segment_order(nseg, seg_order);
+        hipStream_t st = S(stream);
+        const int32_t* rowmap = device_rowmap(omh::segment_rowmap(start, order), st);
+        omh::rows_reorder_f64(d_src, lds, d_dst, ldd, n, ncol, rowmap, inverse != 0, st);
     });
 }
 
@@ -842,6 +944,36 @@ int omh_mixed_solve_blocks(const double* d_H, int64_t n, int64_t ldh, const floa
         bool ok = false;
         const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
                                          &rel, &ok, S(stream), &bf);
+        if (iterations) *iterations = it;
+        if (rel_update) *rel_update = rel;
+        if (!ok) not_converged(it, rel, tol);
+    });
+}
+
+int omh_mixed_solve_reordered(const double* d_H, int64_t n, int64_t ldh, const float* d_L32, int64_t ld32,
+                              const void* d_work32, const double* d_B, int64_t ldb, double* d_X, int64_t ldx,
+                              int64_t nrhs, int32_t max_iter, double tol, int32_t nseg, const int64_t* seg_start,
+                              const uint8_t* hfill, const int32_t* seg_order, int32_t* iterations,
+                              double* rel_update, void* stream) {
+    return guard([&] {
+        need(d_H, "d_H");
+        need(d_L32, "d_L32");
+        need(d_work32, "d_work32");
+        need(d_B, "d_B");
+        need(d_X, "d_X");
+        if (n <= 0 || ldh < n || ld32 < n || ldb < n || ldx < n || nrhs < 0 || max_iter < 1)
+            throw std::invalid_argument("bad sizes");
+        if (n > INT32_MAX) throw std::invalid_argument("n too large for a row map");
+        const omh::BlockFill bf = block_pattern(n, nseg, seg_start, hfill);
+        const std::vector<int32_t> order = segment_order(nseg, seg_order);
+        hipStream_t st = S(stream);
+        omh::FactorOrder fo;
+        fo.fill = reordered_pattern(bf.start, hfill, order);
+        fo.rowmap = device_rowmap(omh::segment_rowmap(bf.start, order), st);
+        double rel = 0.0;
+        bool ok = false;
+        const int it = omh::mixed_refine(d_H, n, ldh, d_L32, ld32, d_work32, d_B, ldb, d_X, ldx, nrhs, max_iter, tol,
+                                         &rel, &ok, st, &bf, &fo);
         if (iterations) *iterations = it;
         if (rel_update) *rel_update = rel;
         if (!ok) not_converged(it, rel, tol);

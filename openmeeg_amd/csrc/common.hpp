@@ -47,7 +47,8 @@ enum ScratchSlot {
     SCR_SRCCOEF = 7,   // source-matrix coefficients (the source matrix may run beside the factorisation)
     SCR_GEMM = 8,      // split-K partial products of the C-ABI GEMM entry points
     SCR_RESID_XT = 9,  // the refinement's transposed solution (structured residual products)
-    SCR_NSLOTS = 10
+    SCR_ROWMAP = 10,   // row map of a reordered factor (segment order)
+    SCR_NSLOTS = 11
 };
 void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st);
 void release_scratch();

@@ -960,12 +960,44 @@ __global__ void k_convert(const TS* __restrict__ src, int64_t lds, TD* __restric
     }
 }
 
+// x += d; with a row map, row i of d updates row rowmap[i] of x
 __global__ void k_axpy_f32_to_f64(const float* __restrict__ d, int64_t ldd, double* __restrict__ x, int64_t ldx,
-                                  int64_t m, int64_t n) {
+                                  int64_t m, int64_t n, const int32_t* __restrict__ rowmap) {
     const int64_t total = m * n;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = t / m, i = t - j * m;
-        x[i + j * ldx] += (double)d[i + j * ldd];
+        const int64_t xi = rowmap ? (int64_t)rowmap[i] : i;
+        x[xi + j * ldx] += (double)d[i + j * ldd];
+    }
+}
+
+// Segment reorder.  rowmap[i] = the row of the original matrix that is row
+// i of the reordered one (whole segments move, so consecutive i read
+// consecutive rows inside a segment).
+// dst's lower triangle (i >= j) = src(rowmap[i], rowmap[j]) of the full
+// symmetric src: columns over grid.y, rows over grid.x and the threads
+template <typename TD>
+__global__ void k_sym_gather(const double* __restrict__ src, int64_t lds, TD* __restrict__ dst, int64_t ldd, int64_t n,
+                             const int32_t* __restrict__ rowmap) {
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+        const double* s = src + (int64_t)rowmap[j] * lds;
+        TD* d = dst + j * ldd;
+        for (int64_t i = j + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x)
+            d[i] = (TD)s[rowmap[i]];
+    }
+}
+
+// m x n rows: gather dst(i, :) = src(rowmap[i], :), or scatter
+// dst(rowmap[i], :) = src(i, :)
+template <typename TS, typename TD>
+__global__ void k_rows_map(const TS* __restrict__ src, int64_t lds, TD* __restrict__ dst, int64_t ldd, int64_t m,
+                           int64_t n, const int32_t* __restrict__ rowmap, int scatter) {
+    const int64_t total = m * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / m, i = t - j * m, r = rowmap[i];
+        if (scatter) dst[r + j * ldd] = (TD)src[i + j * lds];
+        else dst[i + j * ldd] = (TD)src[r + j * lds];
     }
 }
 
@@ -2854,6 +2886,12 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
 }
 
 int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, int64_t n_pos, hipStream_t st) {
+    std::vector<int8_t> expect((size_t)n, (int8_t)-1);
+    for (int64_t i = 0; i < std::min(n, n_pos); ++i) expect[i] = 1;
+    return factor_sign_mismatch(work, n, elem, expect.data(), st);
+}
+
+int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, const int8_t* expect, hipStream_t st) {
     const int64_t nblk = cdiv(n, FACT_NB);
     const size_t off = elem * ((size_t)n * FACT_NB + (size_t)nblk * FACT_NB * FACT_NB);
     std::vector<double> s((size_t)n);
@@ -2867,7 +2905,7 @@ int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, int64_t n
         for (int64_t i = 0; i < n; ++i) s[i] = f[i];
     }
     for (int64_t i = 0; i < n; ++i)
-        if ((i < n_pos) != (s[i] > 0)) return i;
+        if ((expect[i] > 0) != (s[i] > 0)) return i;
     return -1;
 }
 
@@ -3427,6 +3465,30 @@ void convert_f64_to_f32(const double* src, int64_t lds, float* dst, int64_t ldd,
     OMH_LAUNCH_CHECK();
 }
 
+std::vector<int32_t> segment_rowmap(const std::vector<int64_t>& start, const std::vector<int32_t>& order) {
+    std::vector<int32_t> map;
+    map.reserve((size_t)start.back());
+    for (const int32_t s : order)
+        for (int64_t i = start[s]; i < start[s + 1]; ++i) map.push_back((int32_t)i);
+    return map;
+}
+
+void sym_reorder(const double* src, int64_t lds, void* dst, int64_t ldd, int elem, int64_t n, const int32_t* rowmap,
+                 hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 grid((unsigned)std::min<int64_t>(cdiv(n, 256), 32), (unsigned)std::min<int64_t>(n, 16384));
+    if (elem == 8) k_sym_gather<double><<<grid, 256, 0, st>>>(src, lds, (double*)dst, ldd, n, rowmap);
+    else k_sym_gather<float><<<grid, 256, 0, st>>>(src, lds, (float*)dst, ldd, n, rowmap);
+    OMH_LAUNCH_CHECK();
+}
+
+void rows_reorder_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t ncol,
+                      const int32_t* rowmap, bool scatter, hipStream_t st) {
+    if (m <= 0 || ncol <= 0) return;
+    k_rows_map<double, double><<<grid_for(m * ncol), 256, 0, st>>>(src, lds, dst, ldd, m, ncol, rowmap, scatter ? 1 : 0);
+    OMH_LAUNCH_CHECK();
+}
+
 void copy_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t n, hipStream_t st) {
     k_copy<double><<<grid_for(m * n), 256, 0, st>>>(src, lds, dst, ldd, m, n);
     OMH_LAUNCH_CHECK();
@@ -3551,20 +3613,25 @@ static void residual_update(const double* H, int64_t n, int64_t ldh, const doubl
 
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf) {
+                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf,
+                 const FactorOrder* fo) {
     RefineBuffers buf;
     buf.R32 = (float*)scratch(SCR_LINALG, sizeof(float) * (size_t)n * (size_t)r, st);
     buf.R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r, st);
     buf.part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
     buf.split = scratch(SCR_LINALG2, split_solve_bytes(n, r), st);
     return mixed_refine_in(H, n, ldh, L32, ld32, work32, B, ldb, X, ldx, r, max_iter, tol, final_rel_update, converged,
-                           buf, st, bf);
+                           buf, st, bf, fo);
 }
 
 int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                     const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
                     double* final_rel_update, bool* converged, const RefineBuffers& buf, hipStream_t st,
-                    const BlockFill* bf) {
+                    const BlockFill* bf, const FactorOrder* fo) {
+    // the factor's unknown order: the residual R is in H's, the correction
+    // solve runs in the factor's (R32 gathered, added back scattered)
+    const BlockFill* bfl = fo ? &fo->fill : bf;
+    const int32_t* rowmap = fo ? fo->rowmap : nullptr;
     float* R32 = buf.R32;
     double* R = buf.R;
     SplitSolveBuf sb;
@@ -3575,9 +3642,14 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
     double rel = 1.0, prev = 2.0;
     RefineState state = REFINE_CONTINUE;
     for (; it < max_iter; ++it) {
-        convert_f64_to_f32(R, n, R32, n, n, r, false, st);
-        sym_solve_impl<float>(L32, n, ld32, work32, R32, n, r, st, bf, buf.split ? &sb : nullptr);
-        k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r);
+        if (rowmap) {
+            k_rows_map<double, float><<<grid_for(n * r), 256, 0, st>>>(R, n, R32, n, n, r, rowmap, 0);
+            OMH_LAUNCH_CHECK();
+        } else {
+            convert_f64_to_f32(R, n, R32, n, n, r, false, st);
+        }
+        sym_solve_impl<float>(L32, n, ld32, work32, R32, n, r, st, bfl, buf.split ? &sb : nullptr);
+        k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r, rowmap);
         OMH_LAUNCH_CHECK();
         if (it == 0) {
             // X was zero, so the first update is X itself (relative size 1):

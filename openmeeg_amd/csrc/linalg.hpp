@@ -209,6 +209,28 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
 // complement negative definite) and the pivot-free factor is stable; a
 // mismatch means that this order needs pivoting.
 int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, int64_t n_pos, hipStream_t st);
+// The same against an expected sign per unknown (host array of n, +1 / -1).
+int64_t factor_sign_mismatch(const void* work, int64_t n, size_t elem, const int8_t* expect, hipStream_t st);
+
+// The unknown order a factor was computed in, when it differs from the
+// matrix's: whole segments of the matrix reordered.  rowmap (device, n
+// entries) gives, for each row of the factor's order, the matrix row it
+// holds; fill is the block structure in the factor's order.
+struct FactorOrder {
+    const int32_t* rowmap = nullptr;
+    BlockFill fill;
+};
+// rowmap of the segments start[0..nseg] (start[nseg] = n) taken in the
+// order order[0..nseg-1] (a permutation of the segments).
+std::vector<int32_t> segment_rowmap(const std::vector<int64_t>& start, const std::vector<int32_t>& order);
+// dst's lower triangle (element size elem, 8 or 4) = the reordered matrix
+// src(rowmap[i], rowmap[j]) of the full symmetric FP64 src.
+void sym_reorder(const double* src, int64_t lds, void* dst, int64_t ldd, int elem, int64_t n, const int32_t* rowmap,
+                 hipStream_t st);
+// m x ncol rows: dst(i, :) = src(rowmap[i], :), or with scatter
+// dst(rowmap[i], :) = src(i, :).
+void rows_reorder_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t ncol,
+                      const int32_t* rowmap, bool scatter, hipStream_t st);
 
 // 1 if any entry of the blocks a block structure declares zero (below the
 // diagonal) is non-zero.
@@ -250,7 +272,8 @@ RefineState refine_state(int it, double rel, double prev, double tol);
 // FP16 h + l, three products summed in FP32.
 int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                  const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
-                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf = nullptr);
+                 double* final_rel_update, bool* converged, hipStream_t st, const BlockFill* bf = nullptr,
+                 const FactorOrder* fo = nullptr);
 
 // Device buffers of the refinement: the FP64 residual R (n x r), its FP32
 // copy R32 (n x r) and NORM_PARTS doubles of norm partial sums.
@@ -271,7 +294,7 @@ size_t split_solve_bytes(int64_t n, int64_t r);
 int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, int64_t ld32, const void* work32,
                     const double* B, int64_t ldb, double* X, int64_t ldx, int64_t r, int max_iter, double tol,
                     double* final_rel_update, bool* converged, const RefineBuffers& buf, hipStream_t st,
-                    const BlockFill* bf = nullptr);
+                    const BlockFill* bf = nullptr, const FactorOrder* fo = nullptr);
 
 // out(e, j) = sum_k vals(e,k) Hm(cols(e,k), j)  (3 non-zeros per row).
 // out (n x ne, leading dimension ldo) = the dense transpose of a sparse
