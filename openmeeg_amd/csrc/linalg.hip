@@ -858,6 +858,72 @@ __device__ __forceinline__ void diag_inverse_body(const T* __restrict__ A, int64
     }
 }
 
+// FP64 variant of diag_inverse_body with a third of the LDS (47.6 KB
+// against 142 KB), so that its workgroups start on a CU beside one running
+// FP64 trailing-update workgroup (73.7 KB) instead of waiting for an empty
+// CU: each workgroup builds 16 columns of Linv (the columns of a forward
+// substitution are independent), and the block row L(r0.., c0..r0) is staged
+// 32 columns at a time through a buffer that then holds Mi of the block row.
+// Every entry sums its products in the same order as diag_inverse_body, so
+// the two give the same bits.  Workgroup cb: columns 16 cb .. 16 cb + 15.
+constexpr int INV64_CW = 16;
+__device__ __forceinline__ void diag_inverse_body_f64(const double* __restrict__ A, int64_t lda, int nb,
+                                                      double* __restrict__ Linv, int64_t ldl, int cb) {
+    __shared__ double Xs[FACT_NB][INV64_CW + 1];   // rows c0.. of the columns j0 .. j0 + 15
+    __shared__ double Ls[IB][IB + 1];              // a 32-column chunk of the block row, then Mi of the row
+    __shared__ double Tt[IB][INV64_CW + 1];
+    const int tid = threadIdx.x;
+    const int j0 = cb * INV64_CW;
+    if (j0 >= nb) return;
+    const int c0 = j0 / IB * IB;                   // the IB-block the columns lie in
+    const int jo = j0 - c0;                        // their offset inside it
+    const int wc = min(INV64_CW, nb - j0);
+    for (int t = tid; t < c0 * wc; t += DIAG_THREADS) {
+        const int i = t % c0, j = t / c0;
+        Linv[i + (int64_t)(j0 + j) * ldl] = 0.0;
+    }
+    const int dw = min(IB, nb - c0);               // rows of the diagonal IB-block
+    for (int t = tid; t < IB * INV64_CW; t += DIAG_THREADS) {
+        const int i = t % IB, j = t / IB;
+        Xs[i][j] = (i < dw && j < wc) ? Linv[(c0 + i) + (int64_t)(j0 + j) * ldl] : 0.0;
+    }
+    const int ti = tid / INV64_CW, tj = tid % INV64_CW;   // 32 rows x 16 columns
+    for (int r0 = c0 + IB; r0 < nb; r0 += IB) {
+        const int rw = min(IB, nb - r0);
+        const int K = r0 - c0;
+        double a = 0.0;
+        for (int q0 = 0; q0 < K; q0 += IB) {
+            __syncthreads();                       // Xs rows written, the last chunk read
+            for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+                const int i = t % IB, q = t / IB;
+                Ls[i][q] = (i < rw) ? A[(r0 + i) + (int64_t)(c0 + q0 + q) * lda] : 0.0;
+            }
+            __syncthreads();
+            for (int q = 0; q < IB; ++q) a += Ls[ti][q] * Xs[q0 + q][tj];
+        }
+        Tt[ti][tj] = a;
+        __syncthreads();                           // Tt complete, Ls free
+        for (int t = tid; t < IB * IB; t += DIAG_THREADS) {
+            const int i = t % IB, j = t / IB;
+            Ls[i][j] = (i < rw && j < rw) ? Linv[(r0 + i) + (int64_t)(r0 + j) * ldl] : 0.0;
+        }
+        __syncthreads();
+        double b = 0.0;
+        for (int l = 0; l <= ti; ++l) b += Ls[ti][l] * Tt[l][tj];
+        Xs[K + ti][tj] = -b;
+    }
+    __syncthreads();
+    const int rows = nb - c0 - IB;
+    for (int t = tid; t < rows * wc; t += DIAG_THREADS) {
+        const int i = t % rows, j = t / rows;
+        Linv[(c0 + IB + i) + (int64_t)(j0 + j) * ldl] = Xs[IB + i][j];
+    }
+}
+
+// workgroups of the inverse of one diagonal block
+template <typename T>
+constexpr int inv_blocks() { return sizeof(T) == 8 ? FACT_NB / INV64_CW : FACT_NB / IB; }
+
 template <typename T>
 __global__ void __launch_bounds__(DF_THREADS) k_diag_factor(T* __restrict__ A, int64_t lda, int nb,
                                                               T* __restrict__ sgn, int* __restrict__ info, int64_t k0,
@@ -868,7 +934,17 @@ __global__ void __launch_bounds__(DF_THREADS) k_diag_factor(T* __restrict__ A, i
 template <typename T>
 __global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse(const T* __restrict__ A, int64_t lda, int nb,
                                                                T* __restrict__ Linv, int64_t ldl) {
-    diag_inverse_body<T>(A, lda, nb, Linv, ldl, blockIdx.x);
+    if constexpr (sizeof(T) == 8)
+        diag_inverse_body_f64((const double*)A, lda, nb, (double*)Linv, ldl, blockIdx.x);
+    else
+        diag_inverse_body<T>(A, lda, nb, Linv, ldl, blockIdx.x);
+}
+
+// The 32-column FP64 inverse (142 KB of LDS), for comparison runs
+// (OMH_DIAG_INV64_WIDE=1).
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse_wide64(const double* __restrict__ A, int64_t lda,
+                                                                      int nb, double* __restrict__ Linv, int64_t ldl) {
+    diag_inverse_body<double>(A, lda, nb, Linv, ldl, blockIdx.x);
 }
 
 // One workgroup per diagonal block of several factorisations.
@@ -882,7 +958,10 @@ __global__ void __launch_bounds__(DF_THREADS) k_diag_grouped(const DiagDesc* __r
 template <typename T>
 __global__ void __launch_bounds__(DIAG_THREADS) k_diag_inverse_grouped(const DiagDesc* __restrict__ d) {
     const DiagDesc& g = d[blockIdx.y];
-    diag_inverse_body<T>((const T*)g.A11, g.lda, g.nb, (T*)g.Linv, FACT_NB, blockIdx.x);
+    if constexpr (sizeof(T) == 8)
+        diag_inverse_body_f64((const double*)g.A11, g.lda, g.nb, (double*)g.Linv, FACT_NB, blockIdx.x);
+    else
+        diag_inverse_body<T>((const T*)g.A11, g.lda, g.nb, (T*)g.Linv, FACT_NB, blockIdx.x);
 }
 
 // Element-wise operations of several matrices (group = blockIdx.y).
@@ -2272,15 +2351,14 @@ void launch_grouped_gemm(const GemmLaunch& l, const GemmDesc* d_descs, int64_t g
 
 void launch_grouped_diag(bool f32, const DiagDesc* d_descs, int ng, hipStream_t st) {
     if (ng <= 0) return;
-    const dim3 inv(FACT_NB / IB, (unsigned)ng);
     if (f32) {
         k_diag_grouped<float><<<ng, DF_THREADS, 0, st>>>(d_descs, 1e-30f);
         OMH_LAUNCH_CHECK();
-        k_diag_inverse_grouped<float><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
+        k_diag_inverse_grouped<float><<<dim3(inv_blocks<float>(), (unsigned)ng), DIAG_THREADS, 0, st>>>(d_descs);
     } else {
         k_diag_grouped<double><<<ng, DF_THREADS, 0, st>>>(d_descs, 1e-290);
         OMH_LAUNCH_CHECK();
-        k_diag_inverse_grouped<double><<<inv, DIAG_THREADS, 0, st>>>(d_descs);
+        k_diag_inverse_grouped<double><<<dim3(inv_blocks<double>(), (unsigned)ng), DIAG_THREADS, 0, st>>>(d_descs);
     }
     OMH_LAUNCH_CHECK();
 }
@@ -2297,9 +2375,12 @@ void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hip
 // on the strips around it.  Returns false (nothing launched) when the
 // product does not fit the kernel: operands 16-byte aligned with even
 // leading dimensions, enough whole tiles.
+// min_tiles: fewer interior tiles than this leave the product to gemm's
+// smaller tiles (a launch of its own should fill the CUs; products that run
+// concurrently on several streams can take fewer each).
 static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
                          const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t st,
-                         const char* prof) {
+                         const char* prof, int64_t min_tiles = 256) {
     const char* e = std::getenv("OMH_GEMM_F64_DMA");
     if (e && e[0] == '0') return false;
     if (k <= 0 || lda % 2 || ldb % 2 || reinterpret_cast<uintptr_t>(A) % 16 ||
@@ -2307,7 +2388,7 @@ static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alp
         return false;
     const int64_t fm = m / DT, fn = lower ? fm : n / DT;
     const int64_t tiles = lower ? fm * (fm + 1) / 2 : fm * fn;
-    if (tiles < 256) return false;
+    if (tiles < std::max<int64_t>(min_tiles, 1)) return false;
     const TileMap map = l2_map(lower, fm, fn);
     const unsigned grid = (unsigned)(map.kind >= 4 ? 8 * map.per : map.ntiles);
     {
@@ -2477,7 +2558,13 @@ static void diag_factor(T* A, int64_t lda, int64_t k, int nb, T* Dinv, T* sgn, i
     ProfScope ps(sizeof(T) == 8 ? "diag_factor_f64" : "diag_factor_f32", (double)nb * nb * nb / 2.0, st);
     k_diag_factor<T><<<1, DF_THREADS, 0, st>>>(A11, lda, nb, sgn, info, k, tiny, Li, FACT_NB);
     OMH_LAUNCH_CHECK();
-    k_diag_inverse<T><<<cdiv(nb, IB), DIAG_THREADS, 0, st>>>(A11, lda, nb, Li, FACT_NB);
+    const char* wide64 = sizeof(T) == 8 ? std::getenv("OMH_DIAG_INV64_WIDE") : nullptr;
+    if (wide64 && wide64[0] == '1')
+        k_diag_inverse_wide64<<<cdiv(nb, IB), DIAG_THREADS, 0, st>>>((const double*)A11, lda, nb, (double*)Li,
+                                                                     FACT_NB);
+    else
+        k_diag_inverse<T><<<cdiv(nb, sizeof(T) == 8 ? INV64_CW : IB), DIAG_THREADS, 0, st>>>(A11, lda, nb, Li,
+                                                                                          FACT_NB);
     OMH_LAUNCH_CHECK();
 }
 
@@ -3600,8 +3687,12 @@ static void residual_update(const double* H, int64_t n, int64_t ldh, const doubl
             int u = t + 1;
             while (u < ns && bf->hnz(s, u)) ++u;
             const int64_t c0 = bf->start[t], c1 = bf->start[u];
-            gemm<double>(false, true, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r, 1.0,
-                         R + r0, ldr, ss);
+            // the row segments run concurrently: 128 x 128 tiles down to 64
+            // of them per product (a 10242-row segment has 160)
+            if (!gemm_f64_dma(false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r, 1.0, R + r0,
+                              ldr, ss, nullptr, 64))
+                gemm<double>(false, true, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r,
+                             1.0, R + r0, ldr, ss);
             t = u;
         }
     }

@@ -334,6 +334,35 @@ def test_fp64_factor_dma_tiles_bitwise(om, monkeypatch):
     assert torch.equal(X["dma"], X["tiles64"])
 
 
+def test_fp64_diag_inverse_narrow_bitwise(om, monkeypatch):
+    """The FP64 diagonal-block inverse on 16-column workgroups (a third of
+    the LDS of the 32-column ones, so that it runs beside the trailing
+    update) sums every entry in the same order: factor and workspace
+    (diagonal inverses, signs) bitwise those of the 32-column kernel.  3
+    layers of 2562 vertices (n = 17926: a last diagonal block of 6
+    columns), with the block structure."""
+    import torch
+    from openmeeg_amd import spheres as S
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = om.make_nested_geometry(S.nested_sphere_meshes(16, (0.87, 0.92, 1.0)), (1.0, 0.0125, 1.0))
+    n = g.size
+    H = _head_mat_device(g)
+    out = {}
+    for mode in ("narrow", "wide"):
+        if mode == "wide":
+            monkeypatch.setenv("OMH_DIAG_INV64_WIDE", "1")
+        F = factor(H.clone(), n, "fp64", structure=g.factor_segments())
+        torch.cuda.synchronize()
+        out[mode] = (F.L.view(n, n).t().tril().clone(), F.work.clone())
+        del F
+        monkeypatch.delenv("OMH_DIAG_INV64_WIDE", raising=False)
+    print("n=%d: factor max |diff| %.3e, workspace equal %s" % (
+        n, float((out["narrow"][0] - out["wide"][0]).abs().max()), bool(torch.equal(out["narrow"][1], out["wide"][1]))),
+        flush=True)
+    assert torch.equal(out["narrow"][0], out["wide"][0])
+    assert torch.equal(out["narrow"][1], out["wide"][1])
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("shape", [(128, 320, 40000), (70, 133, 9001)])
