@@ -2747,11 +2747,28 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     auto start_of = [&](int64_t b) { return b * FACT_NB; };
     auto width_of = [&](int64_t b) { return (int)std::min<int64_t>(FACT_NB, n - b * FACT_NB); };
     auto ldx_of = [&](int64_t e) { return (n - e + 3) / 4 * 4; };   // panel whose rows start at e
-    // panel of block b into Xbuf[b % S_PANELS]: X = A21 Linv^T, L21 = X S1
-    auto panel = [&](int64_t b, hipStream_t s) {
+    // The end (exclusive) of the rows that L's blocks reach from group g's
+    // columns: below it the group's panels, the in-group updates, the split
+    // operands and the strip run; beyond it every panel of the group is zero
+    // (n without a block structure).
+    auto reach_of = [&](int64_t g) -> int64_t {
+        const int64_t c0 = start_of(g * S_PANELS), c1 = std::min<int64_t>(start_of((g + 1) * S_PANELS), n);
+        if (!bf) return n;
+        int64_t e = c1;
+        const int ns = bf->nseg();
+        for (int t = 0; t < ns; ++t) {
+            if (bf->start[t] >= c1 || bf->start[t + 1] <= c0) continue;
+            for (int s2 = t; s2 < ns; ++s2)
+                if (bf->nz(s2, t)) e = std::max(e, bf->start[s2 + 1]);
+        }
+        return std::min(e, n);
+    };
+    // panel of block b into Xbuf[b % S_PANELS]: X = A21 Linv^T, L21 = X S1,
+    // rows below the block up to re (its group's reach)
+    auto panel = [&](int64_t b, int64_t re, hipStream_t s) {
         const int64_t k = start_of(b);
         const int nb = width_of(b);
-        const int64_t e = k + nb, m = n - e;
+        const int64_t e = k + nb, m = re - e;
         if (m <= 0) return;
         float* X = Xbuf[b % S_PANELS];
         const float* Li = Dinv + (size_t)b * FACT_NB * FACT_NB;
@@ -2760,27 +2777,29 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         k_scale_cols<float><<<grid_for(m * nb), 256, 0, s>>>(X, ldx_of(e), A21, lda, m, nb, sgn + k);
         OMH_LAUNCH_CHECK();
     };
-    // C(rows >= r, cols [r, r + w)) -= X (rows from r) L21^T with the panel
-    // of (full) block b: lower w x w block plus the rows below
-    auto update_cols = [&](int64_t b, int64_t r, int64_t w, hipStream_t s) {
+    // C(rows [r, re), cols [r, r + w)) -= X (rows from r) L21^T with the
+    // panel of (full) block b: lower w x w block plus the rows below (re >= r
+    // + w: the group's reach covers its own columns)
+    auto update_cols = [&](int64_t b, int64_t r, int64_t w, int64_t re, hipStream_t s) {
         const int64_t k = start_of(b), e = k + FACT_NB;
         const int64_t ldx = ldx_of(e);
         const float* X = Xbuf[b % S_PANELS] + (r - e);
         const float* L21 = A + r + k * lda;
         float* C = A + r + r * lda;
         gemm<float>(false, true, true, w, w, FACT_NB, -1.f, X, ldx, L21, lda, 1.f, C, lda, s);
-        if (n - r > w) gemm<float>(false, true, false, n - r - w, w, FACT_NB, -1.f, X + w, ldx, L21, lda, 1.f, C + w,
-                                   lda, s);
+        if (re - r > w) gemm<float>(false, true, false, re - r - w, w, FACT_NB, -1.f, X + w, ldx, L21, lda, 1.f,
+                                    C + w, lda, s);
     };
     // everything of group g that touches only its own columns
     auto group_prologue = [&](int64_t g, hipStream_t s) {
         const int64_t b0 = g * S_PANELS, b1 = std::min<int64_t>(b0 + S_PANELS, nblk);
         const int64_t gend = std::min<int64_t>(start_of(b1), n);
+        const int64_t re = reach_of(g);
         for (int64_t b = b0; b < b1; ++b) {
             diag_factor<float>(A, lda, start_of(b), width_of(b), Dinv, sgn, info, tiny, s);
-            panel(b, s);
+            panel(b, re, s);
             const int64_t r = start_of(b) + width_of(b);
-            if (b + 1 < b1 && r < gend) update_cols(b, r, gend - r, s);
+            if (b + 1 < b1 && r < gend) update_cols(b, r, gend - r, re, s);
         }
     };
     const int64_t ngroups = cdiv(nblk, S_PANELS);
@@ -2790,11 +2809,15 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         const int64_t r0 = start_of(b0 + S_PANELS);   // first column of the next group
         if (r0 >= n) break;
         const int64_t w2 = std::min<int64_t>(S_KP, n - r0);
-        // split operands of the rank-S_KP update of all rows from r0
-        const int64_t ms = n - r0, r2 = r0 + w2, mb = n - r2;
+        // split operands of the rank-S_KP update of the rows from r0 that the
+        // group's panels reach (rows [r0, re); the operands are zero beyond);
+        // the strip covers at least the next group's own columns
+        const int64_t re = reach_of(g);
+        const int64_t r2 = r0 + w2, mb = n - r2;
+        const int64_t mx = std::max<int64_t>(re - r0, 0), ms = std::max(mx, w2), mbr = std::max<int64_t>(re - r2, 0);
         // operand rows (zero padded): whole 128-row tiles of the strip and
-        // whole 256-row tiles of the main update
-        const int64_t mp = std::max(cdiv(ms, S_TILE) * S_TILE, cdiv(w2 + cdiv(mb, S2_TILE) * S2_TILE, S_TILE) * S_TILE);
+        // whole 256-row tiles of the main update's reached rows
+        const int64_t mp = std::max(cdiv(ms, S_TILE) * S_TILE, cdiv(w2 + cdiv(mbr, S2_TILE) * S2_TILE, S_TILE) * S_TILE);
         {
             SplitSrc src;
             for (int i = 0; i < S_PANELS; ++i) {
@@ -2804,7 +2827,7 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
                 src.off[i] = r0 - e;
                 src.sgn[i] = sgn + start_of(b);
             }
-            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, ms, mp, Ah, Al, Bh, Bl, sc);
+            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, mx, mp, Ah, Al, Bh, Bl, sc);
             OMH_LAUNCH_CHECK();
         }
         // the next group's columns: a strip of whole tile columns (w2 = S_KP
