@@ -2703,6 +2703,20 @@ static int syrk_tile() {
     return (e && std::atoi(e) == 256) ? 256 : 128;
 }
 
+// The end (exclusive) of the rows that L's blocks reach from columns [c0,
+// c1): the columns' panels are zero below it (n without a block structure).
+static int64_t reach_end(const BlockFill* bf, int64_t n, int64_t c0, int64_t c1) {
+    if (!bf) return n;
+    int64_t e = c1;
+    const int ns = bf->nseg();
+    for (int t = 0; t < ns; ++t) {
+        if (bf->start[t] >= c1 || bf->start[t + 1] <= c0) continue;
+        for (int s2 = t; s2 < ns; ++s2)
+            if (bf->nz(s2, t)) e = std::max(e, bf->start[s2 + 1]);
+    }
+    return std::min(e, n);
+}
+
 static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work, hipStream_t st, const BlockFill* bf) {
     float* Dinv = (float*)work + (size_t)n * FACT_NB;
     const int64_t nblk = cdiv(n, FACT_NB);
@@ -2752,16 +2766,7 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     // operands and the strip run; beyond it every panel of the group is zero
     // (n without a block structure).
     auto reach_of = [&](int64_t g) -> int64_t {
-        const int64_t c0 = start_of(g * S_PANELS), c1 = std::min<int64_t>(start_of((g + 1) * S_PANELS), n);
-        if (!bf) return n;
-        int64_t e = c1;
-        const int ns = bf->nseg();
-        for (int t = 0; t < ns; ++t) {
-            if (bf->start[t] >= c1 || bf->start[t + 1] <= c0) continue;
-            for (int s2 = t; s2 < ns; ++s2)
-                if (bf->nz(s2, t)) e = std::max(e, bf->start[s2 + 1]);
-        }
-        return std::min(e, n);
+        return reach_end(bf, n, start_of(g * S_PANELS), std::min<int64_t>(start_of((g + 1) * S_PANELS), n));
     };
     // panel of block b into Xbuf[b % S_PANELS]: X = A21 Linv^T, L21 = X S1,
     // rows below the block up to re (its group's reach)
@@ -2928,10 +2933,10 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
     // panel of block k: X = A21 Linv^T (leading dimension m rounded up to 4
     // for aligned vector loads), L21 = X S1 written over A21
     auto panel = [&](int64_t k, int nb, int b, hipStream_t s) {
-        const int64_t e = k + nb, m = n - e;
+        const int64_t e = k + nb, m = reach_end(bf, n, k, e) - e;   // rows the block's columns reach
         if (m <= 0) return;
         const T* Li = Dinv + (size_t)(k / FACT_NB) * FACT_NB * FACT_NB;
-        const int64_t ldx = (m + 3) / 4 * 4;
+        const int64_t ldx = (n - e + 3) / 4 * 4;   // the layout the update reads (all rows below the block)
         T* A21 = A + e + k * lda;
         T* X = Xbuf[b];
         gemm<T>(false, true, false, m, nb, nb, T(1), A21, lda, Li, FACT_NB, T(0), X, ldx, s);
@@ -2950,10 +2955,13 @@ FactorInfo sym_factor(T* A, int64_t n, int64_t lda, void* work, hipStream_t st, 
         // trailing update A22 -= X L21^T (lower)
         const int nb2 = (int)std::min<int64_t>(FACT_NB, m);   // next block
         T* A22 = A + e + e * lda;
-        // (a) columns of the next block
-        gemm<T>(false, true, true, nb2, nb2, nb, T(-1), X, ldx, A21, lda, T(1), A22, lda, st);
-        if (m > nb2)
-            gemm<T>(false, true, false, m - nb2, nb2, nb, T(-1), X + nb2, ldx, A21, lda, T(1), A22 + nb2, lda, st);
+        // (a) columns of the next block, over the rows this block's panel
+        // reaches (its rows beyond are zero and were not formed)
+        const int64_t mr = reach_end(bf, n, k, e) - e;
+        const int wa = (int)std::min<int64_t>(nb2, mr);
+        if (wa > 0) gemm<T>(false, true, true, wa, wa, nb, T(-1), X, ldx, A21, lda, T(1), A22, lda, st);
+        if (mr > nb2)
+            gemm<T>(false, true, false, mr - nb2, nb2, nb, T(-1), X + nb2, ldx, A21, lda, T(1), A22 + nb2, lda, st);
         // factor the next block and form its panel on the side stream (it
         // touches only the next block's columns, which (b) does not).  Beside
         // the FP64 update the diagonal kernels wait for CU slots (~8 ms
