@@ -2512,7 +2512,10 @@ static size_t sign_span(int64_t n) { return (size_t)((n + 63) / 64 * 64); }
 static size_t split_set_bytes(int64_t n) {
     const size_t mp = (size_t)cdiv(n, S_TILE) * S_TILE;
     return 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float) +
-           (size_t)(S_PANELS > 2 ? S_PANELS - 2 : 0) * ((size_t)n * FACT_NB * sizeof(float) + 256);
+           (size_t)(S_PANELS > 2 ? S_PANELS - 2 : 0) * ((size_t)n * FACT_NB * sizeof(float) + 256) +
+           // the second set of operands (the look-ahead splits the next
+           // group's panels while the main update reads the current set)
+           256 + 4 * mp * S_KP * sizeof(_Float16) + mp * sizeof(float);
 }
 
 static size_t dense_workspace_bytes(int64_t n, size_t elem) {
@@ -2724,21 +2727,33 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
     const int64_t mp_max = cdiv(n, S_TILE) * S_TILE;
     char* sp = (char*)work + (dense_workspace_bytes(n, 4) + 64 + 255) / 256 * 256;
     const size_t arr = (size_t)mp_max * S_KP * sizeof(_Float16);
-    _Float16* Ah = (_Float16*)sp;
-    _Float16* Al = (_Float16*)(sp + arr);
-    _Float16* Bh = (_Float16*)(sp + 2 * arr);
-    _Float16* Bl = (_Float16*)(sp + 3 * arr);
-    float* sc = (float*)(sp + 4 * arr);
+    // two sets of split operands: set g % 2 holds group g's (row stride mp of
+    // that group)
+    struct SplitSet {
+        _Float16 *Ah, *Al, *Bh, *Bl;
+        float* sc;
+        int64_t mp;
+    } sets[2];
+    auto place = [&](char* p, SplitSet& q) {
+        q.Ah = (_Float16*)p;
+        q.Al = (_Float16*)(p + arr);
+        q.Bh = (_Float16*)(p + 2 * arr);
+        q.Bl = (_Float16*)(p + 3 * arr);
+        q.sc = (float*)(p + 4 * arr);
+        q.mp = 0;
+    };
+    place(sp, sets[0]);
     float* Xbuf[S_PANELS];
     Xbuf[0] = (float*)work;
     Xbuf[1] = sgn + sign_span(n);
     {
-        char* xp = (char*)(sc + mp_max);
+        char* xp = (char*)(sets[0].sc + mp_max);
         for (int b = 2; b < S_PANELS; ++b) {
             xp = (char*)(((uintptr_t)xp + 255) / 256 * 256);
             Xbuf[b] = (float*)xp;
             xp += (size_t)n * FACT_NB * sizeof(float);
         }
+        place((char*)(((uintptr_t)xp + 255) / 256 * 256), sets[1]);
     }
     int* info = (int*)scratch(SCR_SMALL, 128, st);
     k_set_int<<<1, 1, 0, st>>>(info, 0);
@@ -2808,33 +2823,58 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
         }
     };
     const int64_t ngroups = cdiv(nblk, S_PANELS);
-    group_prologue(0, st);
-    for (int64_t g = 0; g + 1 < ngroups; ++g) {
-        const int64_t b0 = g * S_PANELS;
-        const int64_t r0 = start_of(b0 + S_PANELS);   // first column of the next group
-        if (r0 >= n) break;
-        const int64_t w2 = std::min<int64_t>(S_KP, n - r0);
-        // split operands of the rank-S_KP update of the rows from r0 that the
-        // group's panels reach (rows [r0, re); the operands are zero beyond);
-        // the strip covers at least the next group's own columns
-        const int64_t re = reach_of(g);
-        const int64_t r2 = r0 + w2, mb = n - r2;
-        const int64_t mx = std::max<int64_t>(re - r0, 0), ms = std::max(mx, w2), mbr = std::max<int64_t>(re - r2, 0);
+    // rows of group g's update: the next group's columns start at r0 (a
+    // strip of w2 columns), the rest of the trailing matrix at r2; the
+    // group's panels reach the rows below re (the operands are zero beyond);
+    // the strip covers at least the next group's own columns
+    struct GroupRows {
+        int64_t r0, w2, re, r2, mb, mx, ms, mbr, mp;
+    };
+    auto rows_of = [&](int64_t g) {
+        GroupRows q;
+        q.r0 = start_of((g + 1) * S_PANELS);
+        q.w2 = std::min<int64_t>(S_KP, n - q.r0);
+        q.re = reach_of(g);
+        q.r2 = q.r0 + q.w2;
+        q.mb = n - q.r2;
+        q.mx = std::max<int64_t>(q.re - q.r0, 0);
+        q.ms = std::max(q.mx, q.w2);
+        q.mbr = std::max<int64_t>(q.re - q.r2, 0);
         // operand rows (zero padded): whole 128-row tiles of the strip and
         // whole 256-row tiles of the main update's reached rows
-        const int64_t mp = std::max(cdiv(ms, S_TILE) * S_TILE, cdiv(w2 + cdiv(mbr, S2_TILE) * S2_TILE, S_TILE) * S_TILE);
-        {
-            SplitSrc src;
-            for (int i = 0; i < S_PANELS; ++i) {
-                const int64_t b = b0 + i, e = start_of(b) + FACT_NB;
-                src.X[i] = Xbuf[b % S_PANELS];
-                src.ldx[i] = ldx_of(e);
-                src.off[i] = r0 - e;
-                src.sgn[i] = sgn + start_of(b);
-            }
-            k_split_group<<<(unsigned)(mp / 64), 64 * 4 * S_PANELS, 0, st>>>(src, mx, mp, Ah, Al, Bh, Bl, sc);
-            OMH_LAUNCH_CHECK();
+        q.mp = std::max(cdiv(q.ms, S_TILE) * S_TILE, cdiv(q.w2 + cdiv(q.mbr, S2_TILE) * S2_TILE, S_TILE) * S_TILE);
+        return q;
+    };
+    // split operands of group g (its S_PANELS panels) into set g % 2
+    auto split_of = [&](int64_t g, hipStream_t s) {
+        const GroupRows q = rows_of(g);
+        SplitSet& o = sets[g & 1];
+        o.mp = q.mp;
+        SplitSrc src;
+        for (int i = 0; i < S_PANELS; ++i) {
+            const int64_t b = g * S_PANELS + i, e = start_of(b) + FACT_NB;
+            src.X[i] = Xbuf[b % S_PANELS];
+            src.ldx[i] = ldx_of(e);
+            src.off[i] = q.r0 - e;
+            src.sgn[i] = sgn + start_of(b);
         }
+        k_split_group<<<(unsigned)(q.mp / 64), 64 * 4 * S_PANELS, 0, s>>>(src, q.mx, q.mp, o.Ah, o.Al, o.Bh, o.Bl,
+                                                                          o.sc);
+        OMH_LAUNCH_CHECK();
+    };
+    // group g has an update (its columns are not the last ones)
+    auto has_update = [&](int64_t g) { return g + 1 < ngroups && start_of((g + 1) * S_PANELS) < n; };
+    group_prologue(0, st);
+    if (has_update(0)) split_of(0, st);
+    for (int64_t g = 0; g + 1 < ngroups; ++g) {
+        const int64_t b0 = g * S_PANELS;
+        if (!has_update(g)) break;
+        const GroupRows q = rows_of(g);
+        const int64_t r0 = q.r0, w2 = q.w2, r2 = q.r2, mb = q.mb, ms = q.ms;
+        const SplitSet& o = sets[g & 1];
+        const int64_t mp = o.mp;
+        _Float16 *Ah = o.Ah, *Al = o.Al, *Bh = o.Bh, *Bl = o.Bl;
+        float* sc = o.sc;
         // the next group's columns: a strip of whole tile columns (w2 = S_KP
         // whenever rows remain below it, else the strip is the remainder)
         {
@@ -2850,12 +2890,15 @@ static FactorInfo sym_factor_split(float* A, int64_t n, int64_t lda, void* work,
                                                                                  A + r0 + r0 * lda, lda, map);
             OMH_LAUNCH_CHECK();
         }
-        // the next group's prologue on the side stream (its columns are
-        // outside the SYRK's region; the panel buffers are free once the
-        // split operands exist)
+        // the next group's prologue and its split operands (into the other
+        // set) on the side stream: its columns are outside the SYRK's region,
+        // and the panel buffers and that set are free once this group's
+        // split and the previous group's update are done (both precede this
+        // point on the main stream)
         OMH_HIP(hipEventRecord(la.ready, st));
         OMH_HIP(hipStreamWaitEvent(la.side, la.ready, 0));
         group_prologue(g + 1, la.side);
+        if (has_update(g + 1)) split_of(g + 1, la.side);
         OMH_HIP(hipEventRecord(la.done, la.side));
         // the rest of the trailing matrix
         if (mb > 0) {
