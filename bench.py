@@ -58,7 +58,7 @@ def roofline_of(precision, kern, config):
         kname = "k_gemm<float,false,true,true,4,false> (interior tiles of the FP32 trailing update)"
     else:
         dom, peak, mult, tkey = "factor_update_f64_t128", PEAK_TFLOPS["f64"], 1.0, "f64"
-        kname = "k_gemm_f64_dma<true> (interior 128 x 128 tiles of the FP64 trailing update)"
+        kname = "k_gemm_f64_dma<true, false> (interior 128 x 128 tiles of the FP64 trailing update)"
     calls, sec, work = kern[dom]
     achieved = work / sec / 1e12 if sec > 0 else 0.0
     out = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",

@@ -414,10 +414,14 @@ __global__ void __launch_bounds__(256) k_splitk_sum(const T* __restrict__ W, int
 // the 64 x 64 tiles.  LOWER: entries with row < col are not written.
 constexpr int DT = 128, DBK = 16, DPAD = 16;
 
-template <bool LOWER>
+// mrows: rows of C (and A) that exist; a last tile row past them reads its
+// A rows clamped to arow_max (the last even row pair, so that the 16-byte
+// loads stay inside A) and writes only rows below mrows.
+template <bool LOWER, bool PARTIAL>
 __global__ void __launch_bounds__(512) k_gemm_f64_dma(int64_t k, double alpha, const double* __restrict__ A,
                                                       int64_t lda, const double* __restrict__ B, int64_t ldb,
-                                                      double beta, double* __restrict__ C, int64_t ldc, TileMap map) {
+                                                      double beta, double* __restrict__ C, int64_t ldc, TileMap map,
+                                                      int64_t mrows, int64_t arow_max) {
     __shared__ double As0[DBK * (DT + DPAD)];
     __shared__ double As1[DBK * (DT + DPAD)];
     __shared__ double Bs0[DBK * (DT + DPAD)];
@@ -440,7 +444,8 @@ __global__ void __launch_bounds__(512) k_gemm_f64_dma(int64_t k, double alpha, c
         for (int q = 0; q < 2; ++q) {
             const int kk = 2 * wave + q;
             if (k0 + kk < k) {
-                __builtin_amdgcn_global_load_lds((const void*)(A + row0 + (k0 + kk) * lda + 2 * lane),
+                const int64_t ar = PARTIAL ? min(row0 + 2 * lane, arow_max) : row0 + 2 * lane;
+                __builtin_amdgcn_global_load_lds((const void*)(A + ar + (k0 + kk) * lda),
                                                  (lds_void*)&As[kk * (DT + DPAD)], 16, 0, 0);
                 __builtin_amdgcn_global_load_lds((const void*)(B + col0 + (k0 + kk) * ldb + 2 * lane),
                                                  (lds_void*)&Bs[kk * (DT + DPAD)], 16, 0, 0);
@@ -524,7 +529,7 @@ __global__ void __launch_bounds__(512) k_gemm_f64_dma(int64_t k, double alpha, c
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t gi = row0 + wr + 16 * a + fk + 4 * r, gj = col0 + wc + 16 * b + fi;
-                if (!LOWER || gi >= gj) {
+                if ((!PARTIAL || gi < mrows) && (!LOWER || gi >= gj)) {
                     double v = alpha * acc[a][b][r];
                     if (beta != 0.0) v += beta * C[gi + gj * ldc];
                     C[gi + gj * ldc] = v;
@@ -2378,17 +2383,23 @@ void launch_grouped_elem(const ElemDesc* d_descs, int ng, int64_t max_elems, hip
 // min_tiles: fewer interior tiles than this leave the product to gemm's
 // smaller tiles (a launch of its own should fill the CUs; products that run
 // concurrently on several streams can take fewer each).
+// partial_rows (rectangular products with an even m): the rows past the last
+// whole tile row run as a partial tile row of the same kernel instead of a
+// 64 x 64 edge strip (whose few rows would walk all of k on a handful of
+// workgroups).
 static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
                          const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t st,
-                         const char* prof, int64_t min_tiles = 256) {
+                         const char* prof, int64_t min_tiles = 256, bool partial_rows = false) {
     const char* e = std::getenv("OMH_GEMM_F64_DMA");
     if (e && e[0] == '0') return false;
     if (k <= 0 || lda % 2 || ldb % 2 || reinterpret_cast<uintptr_t>(A) % 16 ||
         reinterpret_cast<uintptr_t>(B) % 16)
         return false;
-    const int64_t fm = m / DT, fn = lower ? fm : n / DT;
+    const bool part = partial_rows && !lower && m % 2 == 0 && m % DT != 0;
+    const int64_t fm = part ? cdiv(m, DT) : m / DT, fn = lower ? fm : n / DT;
     const int64_t tiles = lower ? fm * (fm + 1) / 2 : fm * fn;
     if (tiles < std::max<int64_t>(min_tiles, 1)) return false;
+    const int64_t mrows = part ? m : fm * DT, arow_max = part ? m - 2 : fm * DT;
     const TileMap map = l2_map(lower, fm, fn);
     const unsigned grid = (unsigned)(map.kind >= 4 ? 8 * map.per : map.ntiles);
     {
@@ -2396,17 +2407,22 @@ static bool gemm_f64_dma(bool lower, int64_t m, int64_t n, int64_t k, double alp
         // (lower) or rectangle, 2 k flop per entry
         std::string name;
         if (prof) name = std::string(prof) + "_t" + std::to_string(DT);
-        const double side = (double)(fm * DT);
+        const double side = (double)std::min(fm * DT, m);
         const double work = lower ? side * (side + 1) * (double)k : 2.0 * side * (double)(fn * DT) * (double)k;
         ProfScope ps(prof ? name.c_str() : "", prof ? work : 0.0, prof ? st : nullptr);
         if (!prof) ps.cancel();
         if (lower)
-            k_gemm_f64_dma<true><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map);
+            k_gemm_f64_dma<true, false><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map, mrows,
+                                                              arow_max);
+        else if (part)
+            k_gemm_f64_dma<false, true><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map, mrows,
+                                                              arow_max);
         else
-            k_gemm_f64_dma<false><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map);
+            k_gemm_f64_dma<false, false><<<grid, 512, 0, st>>>(k, alpha, A, lda, B, ldb, beta, C, ldc, map, mrows,
+                                                               arow_max);
         OMH_LAUNCH_CHECK();
     }
-    const int64_t r1 = fm * DT, c1 = fn * DT;
+    const int64_t r1 = std::min(fm * DT, m), c1 = fn * DT;
     if (lower) {
         if (r1 < m) {   // the rows below the interior square: a rectangle and the last lower square
             gemm<double>(false, true, false, m - r1, r1, k, alpha, A + r1, lda, B, ldb, beta, C + r1, ldc, st);
@@ -2429,7 +2445,7 @@ void gemm(bool ta, bool tb, bool lower, int64_t m, int64_t n, int64_t k, T alpha
     if (lower && m != n) throw std::invalid_argument("lower gemm needs a square output");
     if (sizeof(T) == 8 && !ta && tb &&
         gemm_f64_dma(lower, m, n, k, (double)alpha, (const double*)A, lda, (const double*)B, ldb, (double)beta,
-                     (double*)C, ldc, st, prof))
+                     (double*)C, ldc, st, prof, 256, true))
         return;
     const bool vec = aligned_for_vec(A, lda) && aligned_for_vec(B, ldb);
     // FP64: 64 x 64 tiles (128 x 128 accumulators would limit FP64 to one wave
@@ -3753,18 +3769,37 @@ static void residual_update(const double* H, int64_t n, int64_t ldh, const doubl
     OMH_HIP(hipEventRecord(sp.fork, st));
     for (int i = 0; i < N_SIDE; ++i) OMH_HIP(hipStreamWaitEvent(sp.s[i], sp.fork, 0));
     const int ns = bf->nseg();
+    // row segments to streams, the largest products first, each onto the
+    // stream with the least work so far
+    std::vector<std::pair<double, int>> work;
     for (int s = 0; s < ns; ++s) {
+        double w = 0.0;
+        for (int t = 0; t < ns; ++t)
+            if (bf->hnz(s, t)) w += (double)(bf->start[t + 1] - bf->start[t]);
+        work.emplace_back(w * (double)(bf->start[s + 1] - bf->start[s]), s);
+    }
+    std::sort(work.begin(), work.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+        return a.first > b.first || (a.first == b.first && a.second < b.second);
+    });
+    double load[N_SIDE + 1] = {};
+    for (const auto& ws : work) {
+        const int s = ws.second;
+        int q = 0;
+        for (int i = 1; i <= N_SIDE; ++i)
+            if (load[i] < load[q]) q = i;
+        load[q] += ws.first;
         const int64_t r0 = bf->start[s], r1 = bf->start[s + 1];
-        hipStream_t ss = (s % (N_SIDE + 1)) == 0 ? st : sp.s[s % (N_SIDE + 1) - 1];
+        hipStream_t ss = q == 0 ? st : sp.s[q - 1];
         for (int t = 0; t < ns;) {
             if (!bf->hnz(s, t)) { ++t; continue; }
             int u = t + 1;
             while (u < ns && bf->hnz(s, u)) ++u;
             const int64_t c0 = bf->start[t], c1 = bf->start[u];
             // the row segments run concurrently: 128 x 128 tiles down to 64
-            // of them per product (a 10242-row segment has 160)
+            // of them per product (a 10242-row segment has 160), the rows
+            // past the last whole tile row as a partial tile row
             if (!gemm_f64_dma(false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r, 1.0, R + r0,
-                              ldr, ss, nullptr, 64))
+                              ldr, ss, nullptr, 64, true))
                 gemm<double>(false, true, false, r1 - r0, r, c1 - c0, -1.0, H + r0 + c0 * ldh, ldh, XT + c0 * r, r,
                              1.0, R + r0, ldr, ss);
             t = u;
