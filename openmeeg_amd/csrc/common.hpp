@@ -48,7 +48,8 @@ enum ScratchSlot {
     SCR_GEMM = 8,      // split-K partial products of the C-ABI GEMM entry points
     SCR_RESID_XT = 9,  // the refinement's transposed solution (structured residual products)
     SCR_ROWMAP = 10,   // row map of a reordered factor (segment order)
-    SCR_NSLOTS = 11
+    SCR_SOLVE_SCALES = 11,   // the refinement's cache of the split solve products' scales
+    SCR_NSLOTS = 12
 };
 void* scratch(ScratchSlot slot, size_t bytes, hipStream_t st);
 void release_scratch();

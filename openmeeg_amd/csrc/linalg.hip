@@ -3486,7 +3486,18 @@ __global__ void __launch_bounds__(512) k_solve_split_wide(const float* __restric
 struct SplitSolveBuf {
     WSplit ws;
     float* s = nullptr;
+    // optional: the products' row (forward) and column (backward) scales of
+    // every group, n each (they depend on L only); `cached` once a solve has
+    // filled them, so that the next solve with the same factor and block
+    // structure reads them instead
+    float* cache = nullptr;
+    int64_t cache_n = 0, cache_groups = 0;
+    bool cached = false;
 };
+
+size_t solve_scale_cache_bytes(int64_t n) {
+    return 2 * (size_t)cdiv(n, (int64_t)SOLVE_GROUP * FACT_NB) * (size_t)n * sizeof(float) + 256;
+}
 
 size_t split_solve_bytes(int64_t n, int64_t r) {
     const size_t rp = (size_t)cdiv(r, SW_COLS) * SW_COLS;   // whole column tiles of the product kernel
@@ -3532,18 +3543,24 @@ static void split_solve_update(bool trans, const float* L, int64_t lda, int64_t 
         int64_t rows = 0;
         for (const auto& v : part) rows += v.second - v.first;
         if (rows <= 0) continue;
-        const Spans sc = make_spans(part, trans ? 4 : 64);
-        if (trans) k_span_col_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, sb.s);
-        else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, sb.s);
-        OMH_LAUNCH_CHECK();
+        float* scl = sb.s;
+        if (sb.cache)
+            scl = sb.cache + (size_t)((trans ? sb.cache_groups : 0) + k0 / ((int64_t)SOLVE_GROUP * FACT_NB)) *
+                                 (size_t)sb.cache_n;
+        if (!sb.cache || !sb.cached) {
+            const Spans sc = make_spans(part, trans ? 4 : 64);
+            if (trans) k_span_col_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
+            else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
+            OMH_LAUNCH_CHECK();
+        }
         const Spans sp = make_spans(part, S_TILE);
         ProfScope ps("solve_update_f32s", 2.0 * (double)rows * (double)r * (double)(SOLVE_GROUP * FACT_NB), st);
         const int64_t nct = cdiv(r, SW_COLS), total = sp.tile0[sp.n] * nct, per = cdiv(total, 8);
         if (trans)
-            k_solve_split_wide<true><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W,
+            k_solve_split_wide<true><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, scl, sb.ws, W,
                                                                           ldw, r);
         else
-            k_solve_split_wide<false><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, sb.s, sb.ws, W,
+            k_solve_split_wide<false><<<(unsigned)(8 * per), 512, 0, st>>>(L, lda, k0, sp, nct, per, scl, sb.ws, W,
                                                                            ldw, r);
         OMH_LAUNCH_CHECK();
     }
@@ -3820,6 +3837,9 @@ int mixed_refine(const double* H, int64_t n, int64_t ldh, const float* L32, int6
     buf.R = (double*)scratch(SCR_ASSEMBLY, sizeof(double) * (size_t)n * (size_t)r, st);
     buf.part = (double*)scratch(SCR_SMALL, sizeof(double) * NORM_PARTS, st);
     buf.split = scratch(SCR_LINALG2, split_solve_bytes(n, r), st);
+    const char* sce = std::getenv("OMH_SOLVE_SCALE_CACHE");   // 0: every solve forms its scales
+    if (!(sce && sce[0] == '0'))
+        buf.scale_cache = (float*)scratch(SCR_SOLVE_SCALES, solve_scale_cache_bytes(n), st);
     return mixed_refine_in(H, n, ldh, L32, ld32, work32, B, ldb, X, ldx, r, max_iter, tol, final_rel_update, converged,
                            buf, st, bf, fo);
 }
@@ -3835,7 +3855,14 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
     float* R32 = buf.R32;
     double* R = buf.R;
     SplitSolveBuf sb;
-    if (buf.split) sb = split_solve_buffers(buf.split, n, r);
+    if (buf.split) {
+        sb = split_solve_buffers(buf.split, n, r);
+        if (buf.scale_cache) {
+            sb.cache = buf.scale_cache;
+            sb.cache_n = n;
+            sb.cache_groups = cdiv(n, (int64_t)SOLVE_GROUP * FACT_NB);
+        }
+    }
     fill_f64(X, n, r, ldx, 0.0, st);
     copy_f64(B, ldb, R, n, n, r, st);
     int it = 0;
@@ -3849,6 +3876,7 @@ int mixed_refine_in(const double* H, int64_t n, int64_t ldh, const float* L32, i
             convert_f64_to_f32(R, n, R32, n, n, r, false, st);
         }
         sym_solve_impl<float>(L32, n, ld32, work32, R32, n, r, st, bfl, buf.split ? &sb : nullptr);
+        sb.cached = sb.cache != nullptr;   // the same factor and structure on the next pass
         k_axpy_f32_to_f64<<<grid_for(n * r), 256, 0, st>>>(R32, n, X, ldx, n, r, rowmap);
         OMH_LAUNCH_CHECK();
         if (it == 0) {

@@ -285,7 +285,12 @@ struct RefineBuffers {
     // optional: buffers of the split-FP16 products of the FP32 triangular
     // solves; without them the solves use FP32 products
     void* split = nullptr;
+    // optional: the split products' scales of every group (their row and
+    // column scales depend on the factor only; solve_scale_cache_bytes(n)),
+    // filled by the first solve of a call and read by the later ones
+    float* scale_cache = nullptr;
 };
+size_t solve_scale_cache_bytes(int64_t n);
 // Bytes of RefineBuffers::split for n unknowns and r right-hand sides.
 size_t split_solve_bytes(int64_t n, int64_t r);
 // mixed_refine with caller-given buffers (allocates nothing on the device;

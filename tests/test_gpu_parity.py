@@ -882,3 +882,32 @@ def test_adaptive_headmat_matches_oracle(om, head1, levels, tol):
     d0 = rel_fro(hm.data(), np.load(os.path.join(GOLDEN, "head1", "hm.npy")))
     print("adaptive HeadMat (levels %d, tol %g) vs oracle %.3e; vs non-adaptive %.3e" % (levels, tol, e, d0), flush=True)
     assert e < 1e-13 and d0 > 1e-9
+
+
+def test_refinement_scale_cache_bitwise(om, monkeypatch):
+    """The refinement's later solves read the split products' row and column
+    scales the first one formed (they depend on the factor only): the
+    refined solution is bitwise the one that forms them in every solve.  4
+    layers of 1442 vertices (n = 14408), mesh-by-mesh factor order, block
+    structure."""
+    import torch
+    from openmeeg_amd.api import _head_mat_device, factor
+    from openmeeg_amd import spheres as S
+    g = om.make_nested_geometry(S.nested_sphere_meshes(12, (0.8, 0.84, 0.9, 1.0)), (0.33, 1.79, 0.0042, 0.33))
+    n = g.size
+    H = _head_mat_device(g, far_field=True)
+    F = factor(H, n, "mixed16", structure=g.factor_segments(), n_positive=n - g.n_vertex_unknowns,
+               order=g.factor_order())
+    B = torch.from_numpy(np.random.default_rng(4).standard_normal(n * 48)).cuda()
+    X = {}
+    for mode in ("cache", "none"):
+        if mode == "none":
+            monkeypatch.setenv("OMH_SOLVE_SCALE_CACHE", "0")
+        X[mode] = B.clone()
+        F.solve(X[mode], 48, max_iter=10, tol=1e-12)
+        torch.cuda.synchronize()
+        monkeypatch.delenv("OMH_SOLVE_SCALE_CACHE", raising=False)
+    print("refined with cached scales vs formed each time: max |diff| %.3e after %d passes" % (
+        float((X["cache"] - X["none"]).abs().max()), F.iterations), flush=True)
+    assert F.iterations >= 2
+    assert torch.equal(X["cache"], X["none"])
