@@ -831,6 +831,7 @@ int omh_sym_reorder(const double* d_src, int64_t lds, void* d_dst, int64_t ldd, 
         const std::vector<int64_t> start = segment_starts(n, nseg, seg_start);
         const std::vector<int32_t> order = segment_order(nseg, seg_order);
         hipStream_t st = S(stream);
+        if (omh::sym_reorder_blocks(d_src, lds, d_dst, ldd, dst_elem_size, start, order, st)) return;
         const int32_t* rowmap = device_rowmap(omh::segment_rowmap(start, order), st);
         omh::sym_reorder(d_src, lds, d_dst, ldd, dst_elem_size, n, rowmap, st);
     });

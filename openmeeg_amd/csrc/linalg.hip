@@ -1072,6 +1072,32 @@ __global__ void k_sym_gather(const double* __restrict__ src, int64_t lds, TD* __
     }
 }
 
+// The same reorder as block copies: block b of the reordered lower triangle
+// (rows [dr, dr + h), columns [dc, dc + w) of dst; the diagonal blocks only
+// their lower triangle) is the block of src at rows sr, columns sc: rows are
+// contiguous on both sides, no per-element row map.  Blocks over grid.z,
+// their columns over grid.y (strided), rows over grid.x and the threads.
+constexpr int SG_MAX_BLOCKS = 64;
+struct SymGatherBlocks {
+    int64_t dr[SG_MAX_BLOCKS], dc[SG_MAX_BLOCKS], sr[SG_MAX_BLOCKS], sc[SG_MAX_BLOCKS];
+    int32_t h[SG_MAX_BLOCKS], w[SG_MAX_BLOCKS];
+    int32_t diag[SG_MAX_BLOCKS];
+};
+template <typename TD>
+__global__ void k_sym_gather_blocks(const double* __restrict__ src, int64_t lds, TD* __restrict__ dst, int64_t ldd,
+                                    SymGatherBlocks tb) {
+    const int b = blockIdx.z;
+    const int64_t h = tb.h[b], w = tb.w[b];
+    for (int64_t j = blockIdx.y; j < w; j += gridDim.y) {
+        const double* sp = src + tb.sr[b] + (tb.sc[b] + j) * lds;
+        TD* dp = dst + tb.dr[b] + (tb.dc[b] + j) * ldd;
+        const int64_t i0 = tb.diag[b] ? j : 0;
+        for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < h;
+             i += (int64_t)gridDim.x * blockDim.x)
+            dp[i] = (TD)sp[i];
+    }
+}
+
 // m x n rows: gather dst(i, :) = src(rowmap[i], :), or scatter
 // dst(rowmap[i], :) = src(i, :)
 template <typename TS, typename TD>
@@ -3665,6 +3691,36 @@ std::vector<int32_t> segment_rowmap(const std::vector<int64_t>& start, const std
     for (const int32_t s : order)
         for (int64_t i = start[s]; i < start[s + 1]; ++i) map.push_back((int32_t)i);
     return map;
+}
+
+bool sym_reorder_blocks(const double* src, int64_t lds, void* dst, int64_t ldd, int elem,
+                        const std::vector<int64_t>& start, const std::vector<int32_t>& order, hipStream_t st) {
+    const int ns = (int)order.size();
+    if (ns * (ns + 1) / 2 > SG_MAX_BLOCKS) return false;
+    std::vector<int64_t> fs(ns + 1, 0);
+    for (int q = 0; q < ns; ++q) fs[q + 1] = fs[q] + start[order[q] + 1] - start[order[q]];
+    SymGatherBlocks tb{};
+    int nb = 0;
+    int64_t hmax = 0, wmax = 0;
+    for (int p = 0; p < ns; ++p)
+        for (int q = p; q < ns; ++q) {
+            tb.dr[nb] = fs[q];
+            tb.dc[nb] = fs[p];
+            tb.sr[nb] = start[order[q]];
+            tb.sc[nb] = start[order[p]];
+            tb.h[nb] = (int32_t)(fs[q + 1] - fs[q]);
+            tb.w[nb] = (int32_t)(fs[p + 1] - fs[p]);
+            tb.diag[nb] = q == p;
+            hmax = std::max<int64_t>(hmax, tb.h[nb]);
+            wmax = std::max<int64_t>(wmax, tb.w[nb]);
+            ++nb;
+        }
+    const dim3 grid((unsigned)std::min<int64_t>(cdiv(hmax, 256), 32), (unsigned)std::min<int64_t>(wmax, 4096),
+                    (unsigned)nb);
+    if (elem == 8) k_sym_gather_blocks<double><<<grid, 256, 0, st>>>(src, lds, (double*)dst, ldd, tb);
+    else k_sym_gather_blocks<float><<<grid, 256, 0, st>>>(src, lds, (float*)dst, ldd, tb);
+    OMH_LAUNCH_CHECK();
+    return true;
 }
 
 void sym_reorder(const double* src, int64_t lds, void* dst, int64_t ldd, int elem, int64_t n, const int32_t* rowmap,

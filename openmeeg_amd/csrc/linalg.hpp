@@ -227,6 +227,10 @@ std::vector<int32_t> segment_rowmap(const std::vector<int64_t>& start, const std
 // src(rowmap[i], rowmap[j]) of the full symmetric FP64 src.
 void sym_reorder(const double* src, int64_t lds, void* dst, int64_t ldd, int elem, int64_t n, const int32_t* rowmap,
                  hipStream_t st);
+// The same as block copies between segment pairs (no row map), for up to 10
+// segments; false (nothing done) beyond.
+bool sym_reorder_blocks(const double* src, int64_t lds, void* dst, int64_t ldd, int elem,
+                        const std::vector<int64_t>& start, const std::vector<int32_t>& order, hipStream_t st);
 // m x ncol rows: dst(i, :) = src(rowmap[i], :), or with scatter
 // dst(rowmap[i], :) = src(i, :).
 void rows_reorder_f64(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m, int64_t ncol,

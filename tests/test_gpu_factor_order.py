@@ -29,16 +29,23 @@ def _p(t):
     return C.c_void_p(t.data_ptr())
 
 
-def test_sym_reorder_and_rows_match_numpy(om):
+@pytest.mark.parametrize("nseg", [5, 12])
+def test_sym_reorder_and_rows_match_numpy(om, nseg):
     """omh_sym_reorder's lower triangle (FP64 and FP32) is the symmetric
-    segment permutation of the full matrix, its upper triangle untouched;
-    omh_reorder_rows_f64 gathers and scatters rows as numpy indexing does."""
+    segment permutation of the full matrix, its upper triangle untouched
+    (5 segments: block copies between segment pairs; 12: more pairs than the
+    block table holds, the per-row map); omh_reorder_rows_f64 gathers and
+    scatters rows as numpy indexing does."""
     import torch
     from openmeeg_amd import _lib
     L = _lib.lib()
     n = 1000
-    starts = np.array([0, 300, 520, 700, 910], dtype=np.int64)
-    order = np.array([1, 3, 0, 4, 2], dtype=np.int32)
+    if nseg == 5:
+        starts = np.array([0, 300, 520, 700, 910], dtype=np.int64)
+        order = np.array([1, 3, 0, 4, 2], dtype=np.int32)
+    else:
+        starts = np.array([0, 40, 130, 200, 330, 400, 520, 610, 700, 777, 910, 950], dtype=np.int64)
+        order = np.random.default_rng(2).permutation(nseg).astype(np.int32)
     ends = np.append(starts[1:], n)
     idx = np.concatenate([np.arange(starts[s], ends[s]) for s in order])
     rng = np.random.default_rng(11)
