@@ -2576,9 +2576,11 @@ size_t factor_workspace_bytes(int64_t n, size_t elem) {
 
 // Side stream and events of the look-ahead (one set per device, default
 // priority).
+constexpr int SCALE_EVENTS = 32;   // chunks of groups whose solve scales are formed ahead (half per direction)
 struct LookAhead {
     hipStream_t side = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
+    hipEvent_t scale_ev[SCALE_EVENTS] = {};
 };
 
 static LookAhead& lookahead() {
@@ -2592,6 +2594,7 @@ static LookAhead& lookahead() {
         OMH_HIP(hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking));
         OMH_HIP(hipEventCreateWithFlags(&la.ready, hipEventDisableTiming));
         OMH_HIP(hipEventCreateWithFlags(&la.done, hipEventDisableTiming));
+        for (int i = 0; i < SCALE_EVENTS; ++i) OMH_HIP(hipEventCreateWithFlags(&la.scale_ev[i], hipEventDisableTiming));
     }
     return la;
 }
@@ -3560,25 +3563,45 @@ static Spans make_spans(const std::vector<std::pair<int64_t, int64_t>>& iv, int 
 // W -= (split product of the group [k0, k0 + SOLVE_GROUP NB)) on the rows
 // (forward: rows below the group, TRANS = false) or columns (backward:
 // columns before it, TRANS = true) listed in iv.
-static void split_solve_update(bool trans, const float* L, int64_t lda, int64_t k0,
-                               const std::vector<std::pair<int64_t, int64_t>>& iv, const SplitSolveBuf& sb, float* W,
-                               int64_t ldw, int64_t r, hipStream_t st) {
+// The array group k0's scales go to: its cache slot, or the one scratch
+// array without a cache.
+static float* scale_slot(bool trans, int64_t k0, const SplitSolveBuf& sb) {
+    if (!sb.cache) return sb.s;
+    return sb.cache +
+           (size_t)((trans ? sb.cache_groups : 0) + k0 / ((int64_t)SOLVE_GROUP * FACT_NB)) * (size_t)sb.cache_n;
+}
+
+// Row (forward) or column (backward) scales of the group at k0 over the
+// spans iv, into scl.
+static void form_scales(bool trans, const float* L, int64_t lda, int64_t k0,
+                        const std::vector<std::pair<int64_t, int64_t>>& iv, float* scl, hipStream_t st) {
     for (size_t p = 0; p < iv.size(); p += MAX_SPANS) {
         const std::vector<std::pair<int64_t, int64_t>> part(iv.begin() + p,
                                                              iv.begin() + std::min(iv.size(), p + MAX_SPANS));
         int64_t rows = 0;
         for (const auto& v : part) rows += v.second - v.first;
         if (rows <= 0) continue;
-        float* scl = sb.s;
-        if (sb.cache)
-            scl = sb.cache + (size_t)((trans ? sb.cache_groups : 0) + k0 / ((int64_t)SOLVE_GROUP * FACT_NB)) *
-                                 (size_t)sb.cache_n;
-        if (!sb.cache || !sb.cached) {
-            const Spans sc = make_spans(part, trans ? 4 : 64);
-            if (trans) k_span_col_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
-            else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
-            OMH_LAUNCH_CHECK();
-        }
+        const Spans sc = make_spans(part, trans ? 4 : 64);
+        if (trans) k_span_col_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
+        else k_span_row_scale<<<(unsigned)sc.tile0[sc.n], 256, 0, st>>>(L, lda, k0, sc, scl);
+        OMH_LAUNCH_CHECK();
+    }
+}
+
+// ready: the scales were formed ahead on another stream; wait for them
+// instead of forming them here.
+static void split_solve_update(bool trans, const float* L, int64_t lda, int64_t k0,
+                               const std::vector<std::pair<int64_t, int64_t>>& iv, const SplitSolveBuf& sb, float* W,
+                               int64_t ldw, int64_t r, hipStream_t st, hipEvent_t ready = nullptr) {
+    float* scl = scale_slot(trans, k0, sb);
+    if (ready) OMH_HIP(hipStreamWaitEvent(st, ready, 0));
+    else if (!sb.cache || !sb.cached) form_scales(trans, L, lda, k0, iv, scl, st);
+    for (size_t p = 0; p < iv.size(); p += MAX_SPANS) {
+        const std::vector<std::pair<int64_t, int64_t>> part(iv.begin() + p,
+                                                             iv.begin() + std::min(iv.size(), p + MAX_SPANS));
+        int64_t rows = 0;
+        for (const auto& v : part) rows += v.second - v.first;
+        if (rows <= 0) continue;
         const Spans sp = make_spans(part, S_TILE);
         ProfScope ps("solve_update_f32s", 2.0 * (double)rows * (double)r * (double)(SOLVE_GROUP * FACT_NB), st);
         const int64_t nct = cdiv(r, SW_COLS), total = sp.tile0[sp.n] * nct, per = cdiv(total, 8);
@@ -3625,18 +3648,55 @@ static void sym_solve_impl(const T* L, int64_t n, int64_t lda, const void* work,
         }
         OMH_LAUNCH_CHECK();
     };
+    // rows below the group at k0 (forward) / columns before it (backward):
+    // with a block structure only those L's blocks reach
+    auto rows_below = [&](int64_t k0, int64_t gend) {
+        const bool some = bf && active_rows(*bf, n, k0, gend, gend, iv);
+        if (!some) iv.assign(1, {gend, n});
+    };
+    auto cols_before = [&](int64_t k0, int64_t gend) {
+        const bool some = bf && active_cols(*bf, k0, gend, k0, iv);
+        if (!some) iv.assign(1, {0, k0});
+    };
+    // The products' scales depend on L only: when a scale cache is to be
+    // filled, every group's are formed on the look-ahead stream ahead of the
+    // products (beside the latency-bound group solves), with one event per
+    // chunk of groups that the products wait for.
+    const bool ahead = split_ok && sb->cache && !sb->cached;
+    LookAhead* la = ahead ? &lookahead() : nullptr;
+    const int64_t ngroups = cdiv(n, GW);
+    const int64_t chunk = std::max<int64_t>(1, cdiv(ngroups, SCALE_EVENTS / 2));
+    if (ahead) {
+        OMH_HIP(hipEventRecord(la->ready, st));
+        OMH_HIP(hipStreamWaitEvent(la->side, la->ready, 0));
+        for (int64_t k0 = 0, gi = 0; k0 < n; k0 += GW, ++gi) {
+            const int64_t gend = std::min<int64_t>(k0 + GW, n);
+            if (gend - k0 == GW && gend < n) {
+                rows_below(k0, gend);
+                form_scales(false, (const float*)L, lda, k0, iv, scale_slot(false, k0, *sb), la->side);
+            }
+            if ((gi + 1) % chunk == 0 || gend >= n) OMH_HIP(hipEventRecord(la->scale_ev[gi / chunk], la->side));
+        }
+        for (int64_t k0 = ((n - 1) / GW) * GW, bi = 0; k0 >= 0; k0 -= GW, ++bi) {
+            const int64_t gend = std::min<int64_t>(k0 + GW, n);
+            if (gend - k0 == GW && k0 > 0) {
+                cols_before(k0, gend);
+                form_scales(true, (const float*)L, lda, k0, iv, scale_slot(true, k0, *sb), la->side);
+            }
+            if ((bi + 1) % chunk == 0 || k0 == 0)
+                OMH_HIP(hipEventRecord(la->scale_ev[SCALE_EVENTS / 2 + bi / chunk], la->side));
+        }
+    }
     // forward: W <- L^-1 W
-    for (int64_t k0 = 0; k0 < n; k0 += GW) {
+    for (int64_t k0 = 0, gi = 0; k0 < n; k0 += GW, ++gi) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
         const bool split = split_ok && gend - k0 == GW && gend < n;
         group_solve(false, k0, gend, split);
-        // rows below the group: with a block structure only those its
-        // columns reach in L
         if (gend < n) {
-            const bool some = bf && active_rows(*bf, n, k0, gend, gend, iv);
-            if (!some) iv.assign(1, {gend, n});
+            rows_below(k0, gend);
             if (split)
-                split_solve_update(false, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st);
+                split_solve_update(false, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st,
+                                   ahead ? la->scale_ev[gi / chunk] : nullptr);
             else
                 for (const auto& v : iv)
                     gemm<T>(false, false, false, v.second - v.first, r, gend - k0, T(-1), L + v.first + k0 * lda,
@@ -3648,16 +3708,16 @@ static void sym_solve_impl(const T* L, int64_t n, int64_t lda, const void* work,
     OMH_LAUNCH_CHECK();
     // backward: W <- L^-T W (rows of a group already carry the later
     // groups' contributions)
-    for (int64_t k0 = ((n - 1) / GW) * GW; k0 >= 0; k0 -= GW) {
+    for (int64_t k0 = ((n - 1) / GW) * GW, bi = 0; k0 >= 0; k0 -= GW, ++bi) {
         const int64_t gend = std::min<int64_t>(k0 + GW, n);
         const bool split = split_ok && gend - k0 == GW && k0 > 0;
         group_solve(true, k0, gend, split);
         // rows above the group: only the columns its rows reach in L
         if (k0 > 0) {
-            const bool some = bf && active_cols(*bf, k0, gend, k0, iv);
-            if (!some) iv.assign(1, {0, k0});
+            cols_before(k0, gend);
             if (split)
-                split_solve_update(true, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st);
+                split_solve_update(true, (const float*)L, lda, k0, iv, *sb, (float*)W, ldw, r, st,
+                                   ahead ? la->scale_ev[SCALE_EVENTS / 2 + bi / chunk] : nullptr);
             else
                 for (const auto& v : iv)
                     gemm<T>(true, false, false, v.second - v.first, r, gend - k0, T(-1), L + k0 + v.first * lda, lda,
