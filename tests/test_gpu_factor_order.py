@@ -150,3 +150,31 @@ def test_factor_order_breakdown_falls_back(om):
     X = F.solve(B.clone(), 4)
     res = float(torch.linalg.norm(B.view(4, n).t() - H0.view(n, n) @ X.view(4, n).t()) / torch.linalg.norm(B))
     assert res < 1e-10
+
+
+def test_fp64_factor_order_without_room_factors_in_place(om, monkeypatch):
+    """Without room for a reordered FP64 copy (forced from the host), the
+    FP64 factorisation keeps the matrix's own order and factors it in place;
+    the solution is the reordered factor's (1e-10)."""
+    import torch
+    from openmeeg_amd import api
+    from openmeeg_amd.api import _head_mat_device, factor
+    g = _four_layers(om, 8)
+    n = g.size
+    structure, order = g.factor_segments(), g.factor_order()
+    npos = n - g.n_vertex_unknowns
+    H = _head_mat_device(g)
+    H0 = H.clone()
+    torch.manual_seed(3)
+    B = torch.randn(n * 8, dtype=torch.float64, device="cuda")
+    F1 = factor(H.clone(), n, "fp64", structure=structure, n_positive=npos, order=order)
+    assert F1.order is not None
+    X1 = F1.solve(B.clone(), 8)
+    del F1
+    monkeypatch.setattr(api, "_room_for", lambda nbytes: False)
+    F0 = factor(H, n, "fp64", structure=structure, n_positive=npos, order=order)
+    assert F0.order is None and not torch.equal(H, H0)      # factored in place
+    X0 = F0.solve(B.clone(), 8)
+    diff = float(torch.linalg.norm(X1 - X0) / torch.linalg.norm(X0))
+    print("fp64 without room for the reordered copy: in place, solutions agree to %.3e" % diff, flush=True)
+    assert diff < 1e-10
